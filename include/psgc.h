@@ -1,0 +1,293 @@
+/*
+ * psgc.h — C ABI of the MI355X (gfx950) reconstruction path for binocular
+ * phase-shift structured light: wrapped-phase decoding, three-frequency
+ * heterodyne or Gray-code unwrapping, order-constrained row matching with a
+ * sub-pixel root, Q-matrix triangulation and the organized point-cloud filters
+ * (Gaussian, radius outlier, statistical outlier, MLS).
+ *
+ * Reference seams this header replaces (paths relative to
+ * PhaseShiftGrayCode3D/src/ of the reference):
+ *   JSLAB_DecodePhase_GPU / _Warmup / _Release
+ *       JSLAB_PhaseShiftDecoder_V2_GPU.h:50-60
+ *   JSLAB_ReconstructPipeline_GPU
+ *       JSLAB_PhaseEpipolarReconstructor_V2_GPU.h:131-137
+ *   applyMLSFilterToPointCloud_GPU / applyMLSFilterToPointCloud
+ *       JSLAB_MLS_Filter.h:42-57
+ *   the CPU-side building blocks that the GPU seams do not cover
+ *   (JSLAB_PhaseEpipolarReconstructor_V2.cpp:79-195, 397-480, 933-1006)
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes; no exceptions cross the ABI.
+ *   - Every entry point returns an int status: PSGC_OK (0) or a negative
+ *     PSGC_E_* code; psgc_last_error() returns a thread-local message.
+ *   - Image-shaped buffers are dense row-major (row stride = cols elements),
+ *     except the input frames, which take an explicit row stride in bytes.
+ *   - `mem` selects where the caller's buffers live: PSGC_MEM_DEVICE (HBM,
+ *     the fast path; the call is asynchronous on the context stream) or
+ *     PSGC_MEM_HOST (the library stages through context-owned device buffers
+ *     and the call returns after the results are back in host memory, as the
+ *     reference launchers do).
+ *   - One context per thread; a context owns its stream, staging buffers and
+ *     captured graphs (the reference keeps process-global static buffers,
+ *     JSLAB_PhaseShiftDecoder_V2_GPU.cu:153-158, so it is not re-entrant).
+ */
+#ifndef PSGC_H
+#define PSGC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define PSGC_API __attribute__((visibility("default")))
+#else
+#define PSGC_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSGC_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    PSGC_OK = 0,
+    PSGC_E_INVALID = -1,   /* null pointer, bad size, bad configuration       */
+    PSGC_E_NODEVICE = -2,  /* no HIP device, or device index out of range      */
+    PSGC_E_ALLOC = -3,     /* device or host allocation failed                 */
+    PSGC_E_HIP = -4,       /* a HIP runtime call failed                        */
+    PSGC_E_UNSUPPORTED = -5
+};
+
+enum { PSGC_MEM_DEVICE = 0, PSGC_MEM_HOST = 1 };
+
+/* Sample type of the input frames. The reference CPU decoder accepts CV_8U
+ * and CV_64F (JSLAB_PhaseShiftDecoder_V2.cpp:260-261); 12-bit cameras deliver
+ * uint16, which is read natively here (each sample converts exactly to f64). */
+enum { PSGC_SAMPLE_U8 = 0, PSGC_SAMPLE_U16 = 1, PSGC_SAMPLE_F64 = 2 };
+
+/* Unwrapping schedule. */
+enum {
+    PSGC_UNWRAP_HETERODYNE3 = 0, /* three-frequency heterodyne (reference V2)  */
+    PSGC_UNWRAP_GRAYCODE = 1     /* one phase-shift frequency + Gray code with
+                                    complement frames (textbook; not in the
+                                    reference)                                 */
+};
+
+/* Which of the reference's two behaviours to follow where its CPU and GPU
+ * paths disagree (JSLAB_Dual_GPU移植说明.md:68-78):
+ *   CPU: point-cloud Gaussian keeps the mask and divides by multiplying with
+ *        the reciprocal (JSLAB_PhaseEpipolarReconstructor_V2.cpp:79-124); the
+ *        radius outlier filter reads the pre-Gaussian mask (:466); MLS falls
+ *        back straight to order 0 (JSLAB_MLS_Filter.cpp:277-279).
+ *   GPU: Gaussian clears the mask of rejected points and divides
+ *        (JSLAB_PhaseEpipolarReconstructor_V2_GPU.cu:220-274); the outlier
+ *        filter reads that mask (:476-485); MLS falls back 2 -> 1 -> 0
+ *        (JSLAB_MLS_Filter.cu:283-286). */
+enum { PSGC_SEM_CPU = 0, PSGC_SEM_GPU = 1 };
+
+/* Outlier stage after the Gaussian. */
+enum { PSGC_OUTLIER_NONE = 0, PSGC_OUTLIER_ROR = 1, PSGC_OUTLIER_SOR = 2 };
+
+/* ---- decoder configuration ----------------------------------------------
+ * Fields 1..9 mirror JSLAB_PhaseShiftDecoder::Config
+ * (JSLAB_PhaseShiftDecoder.h:22-54) with the same meaning and defaults. */
+typedef struct psgc_decode_config {
+    double freq[3];               /* fringe counts of the three levels        */
+    int32_t shift_num[3];         /* phase steps per level (3..12)            */
+    double modulation_threshold;  /* default 1.0                              */
+    double preblur_sigma;         /* default 3.0; <=0: fixed [1/4,1/2,1/4]    */
+    int32_t avg_three_phase;      /* default 1                                */
+    int32_t post_smooth;          /* default 1                                */
+    int32_t enable_hdr;           /* default 0                                */
+    double hdr_saturation_gray;   /* default 255                              */
+    double hdr_min_amplitude;     /* default 2                                */
+    /* additions (no reference counterpart) */
+    int32_t unwrap_mode;          /* PSGC_UNWRAP_*; default heterodyne         */
+    int32_t gray_bits;            /* Gray-code bits (graycode mode), e.g. 6    */
+    double gray_period_px;        /* projector pixels per phase period          */
+    double gray_min_contrast;     /* |I - I_complement| below => invalid        */
+} psgc_decode_config;
+
+PSGC_API void psgc_decode_config_default(psgc_decode_config* cfg);
+
+/* ---- reconstructor configuration -----------------------------------------
+ * Fields up to mls_iterations mirror JSLAB_PhaseEpipolarReconstructor_V2::Config
+ * (JSLAB_PhaseEpipolarReconstructor_V2.h:18-55); the reference's GPU class
+ * has the same fields with other defaults
+ * (JSLAB_PhaseEpipolarReconstructor_V2_GPU.h:27-61). */
+typedef struct psgc_recon_config {
+    double min_depth;             /* 0                                        */
+    double max_depth;             /* 1000                                     */
+    double min_phase_diff;        /* 0.03 (unused by the reference's path)     */
+    double max_phase_diff;        /* 0.4  (unused by the reference's path)     */
+    double grad_threshold;        /* 1.0: max |phiR - phiL| of a match         */
+    int32_t use_rectified_coords; /* 1                                        */
+    int32_t use_disp_median_filter; /* 1                                      */
+    int32_t use_full_left_rectification; /* 1 (host-side setup only)          */
+    float radius_search;          /* CPU 0.7 / GPU 0.05                        */
+    int32_t radius_min_neighbors; /* CPU 10  / GPU 5                           */
+    int32_t radius_window_size;   /* CPU 10  / GPU 5 (half = size / 2)         */
+    int32_t use_mls;              /* CPU 0   / GPU 1                           */
+    int32_t mls_window_size;      /* half window: CPU 30 / GPU 25              */
+    float mls_spatial_sigma;      /* CPU 0.6 / GPU 0.5                         */
+    int32_t use_gpu_mls;          /* 1                                        */
+    int32_t mls_polynomial_order; /* 1                                        */
+    float mls_max_radius;         /* 0 = 3 sigma                               */
+    int32_t mls_iterations;       /* CPU 3   / GPU 2                           */
+    /* additions */
+    int32_t semantics;            /* PSGC_SEM_CPU / PSGC_SEM_GPU               */
+    int32_t gauss_window;         /* 3 in the reference                        */
+    float gauss_sigma;            /* 0.7 in the reference                      */
+    int32_t outlier_mode;         /* PSGC_OUTLIER_*; reference = ROR           */
+    int32_t sor_k;                /* neighbours averaged by SOR                */
+    float sor_std_mul;            /* threshold = mean + std_mul * stddev       */
+    int32_t sor_window;           /* half window of the organized kNN search   */
+} psgc_recon_config;
+
+/* Defaults of the reference CPU class (semantics = PSGC_SEM_CPU). */
+PSGC_API void psgc_recon_config_default(psgc_recon_config* cfg);
+/* Defaults of the reference GPU class (semantics = PSGC_SEM_GPU). */
+PSGC_API void psgc_recon_config_default_gpu(psgc_recon_config* cfg);
+
+/* ---- context ------------------------------------------------------------- */
+typedef struct psgc_ctx psgc_ctx;
+
+PSGC_API const char* psgc_last_error(void);
+PSGC_API int psgc_abi_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+PSGC_API int psgc_device_count(void);
+
+/* Creates a context on `device` with its own stream. */
+PSGC_API int psgc_create(psgc_ctx** out, int device);
+PSGC_API int psgc_destroy(psgc_ctx* ctx);
+/* Use the caller's hipStream_t (e.g. torch's current stream) instead of the
+ * context's own; NULL restores the context's stream. */
+PSGC_API int psgc_set_stream(psgc_ctx* ctx, void* hip_stream);
+PSGC_API int psgc_synchronize(psgc_ctx* ctx);
+
+/* Replaces JSLAB_DecodePhase_GPU_Warmup (JSLAB_PhaseShiftDecoder_V2_GPU.h:56):
+ * initialises the HIP runtime on the current device. */
+PSGC_API int psgc_warmup(void);
+/* Replaces JSLAB_DecodePhase_GPU_Release (JSLAB_PhaseShiftDecoder_V2_GPU.h:60):
+ * frees the context's cached staging buffers and graphs. */
+PSGC_API int psgc_release(psgc_ctx* ctx);
+
+/* ---- decode --------------------------------------------------------------
+ * Replaces JSLAB_DecodePhase_GPU (JSLAB_PhaseShiftDecoder_V2_GPU.h:50-52) and
+ * JSLAB_PhaseShiftDecoderV2::setImages + startCalculate + getOrderMap +
+ * getModulationMap (JSLAB_PhaseShiftDecoder_V2.cpp:256-310).
+ *   frames[n_frames]: one plane per frame, rows x cols samples, row stride
+ *     `row_stride_bytes` (0 = dense). Heterodyne: frames ordered level 0
+ *     steps, level 1 steps, level 2 steps (sum of shift_num). Gray code:
+ *     shift_num[0] phase frames, then gray_bits pattern frames, then
+ *     gray_bits complement frames (MSB first).
+ *   phase (f64, post-smoothed if post_smooth), k (i32 fringe order),
+ *   mod (f64 normalized modulation of level 0; may be NULL),
+ *   valid (u8, 255 valid / 0 invalid; may be NULL).
+ * Unlike the reference GPU launcher (which saturates everything to 8 bit,
+ * JSLAB_PhaseShiftDecoder_V2_GPU.cu:252-258) every sample type is decoded
+ * exactly as the reference CPU decoder would after an exact f64 conversion. */
+PSGC_API int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg,
+                const void* const* frames, int n_frames, int sample_type,
+                int rows, int cols, int64_t row_stride_bytes,
+                double* phase, int32_t* k, double* mod, uint8_t* valid,
+                int mem);
+
+/* ---- reconstruct ---------------------------------------------------------
+ * Replaces JSLAB_ReconstructPipeline_GPU
+ * (JSLAB_PhaseEpipolarReconstructor_V2_GPU.h:131-137) and
+ * JSLAB_PhaseEpipolarReconstructor_V2::startCalculate
+ * (JSLAB_PhaseEpipolarReconstructor_V2.cpp:397-480): order-constrained match
+ * -> optional 5x5 median -> Q reprojection -> Gaussian -> outlier filter
+ * -> optional MLS. Inputs are rectified maps (remap is the caller's, as in
+ * the reference, where it stays on the host).
+ *   pha_f = number of fringe orders (the reference passes freq[0]).
+ *   Q: 4x4 row-major, R1: 3x3 row-major (f64).
+ *   xyz: rows*cols*3 f32 (camera or rectified coordinates), P: u8 0/1,
+ *   disp (may be NULL): f32 disparity after match (+median), 0 = none,
+ *   match_col (may be NULL): i32 right column of the integer match, -1 none. */
+PSGC_API int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg,
+                     const double* l_phase, const double* r_phase,
+                     const int32_t* l_k, const int32_t* r_k,
+                     int rows, int cols, int pha_f,
+                     const double Q[16], const double R1[9],
+                     float* xyz, uint8_t* P, float* disp, int32_t* match_col,
+                     int mem);
+
+/* ---- individual stages (stage-isolated parity, as the reference's compare
+ * harness does, JSLAB_Demo_Dual.cpp:448-513) ------------------------------ */
+
+/* stereoMatch (JSLAB_PhaseEpipolarReconstructor_V2.cpp:933-1006): disp is
+ * f64 (0 = no match); match_col may be NULL. */
+PSGC_API int psgc_stereo_match(psgc_ctx* ctx, const double* l_phase,
+                      const double* r_phase, const int32_t* l_k,
+                      const int32_t* r_k, int rows, int cols, int pha_f,
+                      double grad_threshold, double* disp, int32_t* match_col,
+                      int mem);
+
+/* 5x5 median with replicated border on f32 (cv::medianBlur(…, 5),
+ * JSLAB_PhaseEpipolarReconstructor_V2.cpp:405-412). */
+PSGC_API int psgc_median5(psgc_ctx* ctx, const float* src, float* dst, int rows,
+                 int cols, int mem);
+
+/* Q reprojection (JSLAB_PhaseEpipolarReconstructor_V2.cpp:414-461); disp is
+ * f32 as the reference stores it after the median. */
+PSGC_API int psgc_reproject(psgc_ctx* ctx, const float* disp, int rows, int cols,
+                   const double Q[16], const double R1[9], double min_depth,
+                   double max_depth, int use_rectified_coords, float* xyz,
+                   uint8_t* P, int mem);
+
+/* Point-cloud Gaussian (JSLAB_PhaseEpipolarReconstructor_V2.cpp:79-124):
+ * window `win` (odd), sigma; with PSGC_SEM_GPU dst_P receives the updated
+ * mask, with PSGC_SEM_CPU dst_P receives a copy of P. */
+PSGC_API int psgc_filter_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
+                         float* dst_xyz, uint8_t* dst_P, int rows, int cols,
+                         int win, float sigma, int semantics, int mem);
+
+/* Radius outlier removal (JSLAB_PhaseEpipolarReconstructor_V2.cpp:126-195). */
+PSGC_API int psgc_filter_ror(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
+                    float* dst_xyz, uint8_t* dst_P, int rows, int cols,
+                    float radius, int min_neighbors, int window_size, int mem);
+
+/* Statistical outlier removal (textbook/PCL definition, no reference code):
+ * for every valid point, mean Euclidean distance to its k nearest valid
+ * neighbours inside a (2*half_window+1)^2 pixel window; points with fewer
+ * than k candidates are removed and excluded from the statistics; keep
+ * if mean_dist <= mean + std_mul * stddev (sample stddev over kept-eligible
+ * points). mean_dist (may be NULL) receives the per-point distance (f64,
+ * 0 where not computed); stats (may be NULL) receives {count, mean, stddev,
+ * threshold}. */
+PSGC_API int psgc_filter_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
+                    float* dst_xyz, uint8_t* dst_P, int rows, int cols, int k,
+                    float std_mul, int half_window, double* mean_dist,
+                    double stats[4], int mem);
+
+/* MLS projection (applyMLSFilterToPointCloud_GPU, JSLAB_MLS_Filter.h:42-47;
+ * semantics selects the order-fallback rule). */
+PSGC_API int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t* P,
+                    float* dst_xyz, int rows, int cols, int window_size,
+                    float spatial_sigma, int polynomial_order,
+                    float max_radius, int iterations, int semantics, int mem);
+
+/* Closed-form rectification for an ideal, already rectified pair (zero
+ * distortion, R = I, translation along x): fills Q (4x4) and R1 (3x3) as
+ * cv::stereoRectify would with flags 0, plus the full-left canvas shift of
+ * computeFullLeftRectification (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+ * 248-289) when full_left != 0 (a zero shift for this case). */
+PSGC_API int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1,
+                       double cx2, double baseline, double Q[16],
+                       double R1[9]);
+
+/* ---- instrumentation ------------------------------------------------------
+ * Per-kernel-family timing with hipEvents recorded on the launch stream.
+ * psgc_profile_enable(1) clears and starts recording; psgc_profile_read
+ * synchronises the device and returns, for family `name`, the number of
+ * launches and their summed duration in milliseconds (0 launches if unknown). */
+PSGC_API int psgc_profile_enable(int on);
+PSGC_API int psgc_profile_read(const char* name, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSGC_H */

@@ -1,0 +1,993 @@
+// psgc_oracle.cpp — TEST INFRASTRUCTURE, not product code.
+//
+// A plain C++17 restatement of the reference's CPU reconstruction path,
+// written to be read against the reference line by line. It is the checker
+// for the HIP library and the "port" CPU baseline timed beside it; only the
+// tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+//
+// Reference files (PhaseShiftGrayCode3D/src/):
+//   decode          JSLAB_PhaseShiftDecoder_V2.cpp:6-310
+//   HDR fit         JSLAB_HDRPhaseFit.h:10-61
+//   match           JSLAB_PhaseEpipolarReconstructor_V2.cpp:933-1006
+//   median/Q        JSLAB_PhaseEpipolarReconstructor_V2.cpp:397-461
+//   Gaussian        JSLAB_PhaseEpipolarReconstructor_V2.cpp:79-124
+//   ROR             JSLAB_PhaseEpipolarReconstructor_V2.cpp:126-195
+//   MLS             JSLAB_MLS_Filter.cpp:11-441 (GPU fallback rule:
+//                   JSLAB_MLS_Filter.cu:283-286)
+// Not in the reference (textbook definitions): SOR, Gray-code unwrapping.
+//
+// Where the reference relies on OpenCV arithmetic the exact operation order
+// is not observable without OpenCV (absent here), so it is fixed as follows
+// and called "parity unpinned" at that boundary:
+//   - cv::GaussianBlur 3x3 on f64: separable, row pass then column pass,
+//     each out = c*x + s*(x_left + x_right), BORDER_REPLICATE;
+//   - cv::getGaussianKernel: e_i = exp(-(i-h)^2/(2 sigma^2)) in f64, scaled by
+//     the reciprocal of their sum, rounded to the kernel type; for ksize 3 and
+//     sigma <= 0 the fixed [1/4, 1/2, 1/4];
+//   - cv::medianBlur 5 on f32: exact 13th order statistic, BORDER_REPLICATE.
+// atan2 is psgc::atan2 (shared with the device, within 2 ulp of libm).
+//
+// Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "psgc.h"
+#include "psgc_math.h"
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+// ---- OpenCV kernel helpers ------------------------------------------------
+
+// getGaussianKernel(n, sigma) in f64.
+std::vector<double> gauss_kernel_d(int n, double sigma) {
+  std::vector<double> k(n);
+  if (n == 3 && sigma <= 0) {
+    k = {0.25, 0.5, 0.25};
+    return k;
+  }
+  if (sigma <= 0) sigma = ((n - 1) * 0.5 - 1) * 0.3 + 0.8;
+  const double scale2 = -0.5 / (sigma * sigma);
+  double sum = 0;
+  for (int i = 0; i < n; ++i) {
+    const double x = i - (n - 1) * 0.5;
+    k[i] = std::exp(scale2 * x * x);
+    sum += k[i];
+  }
+  sum = 1.0 / sum;
+  for (int i = 0; i < n; ++i) k[i] *= sum;
+  return k;
+}
+
+// 3x3 separable blur, BORDER_REPLICATE: row pass, then column pass.
+void blur3_replicate(const double* src, double* dst, int rows, int cols,
+                     double c, double s) {
+  std::vector<double> tmp((size_t)rows * cols);
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r) {
+    const double* in = src + (size_t)r * cols;
+    double* out = tmp.data() + (size_t)r * cols;
+    for (int x = 0; x < cols; ++x) {
+      const double xl = in[x > 0 ? x - 1 : 0];
+      const double xr = in[x < cols - 1 ? x + 1 : cols - 1];
+      out[x] = c * in[x] + s * (xl + xr);
+    }
+  }
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r) {
+    const double* up = tmp.data() + (size_t)(r > 0 ? r - 1 : 0) * cols;
+    const double* mid = tmp.data() + (size_t)r * cols;
+    const double* dn = tmp.data() + (size_t)(r < rows - 1 ? r + 1 : rows - 1) * cols;
+    double* out = dst + (size_t)r * cols;
+    for (int x = 0; x < cols; ++x) out[x] = c * mid[x] + s * (up[x] + dn[x]);
+  }
+}
+
+double sample_at(const void* plane, int type, int64_t stride, int r, int x) {
+  const char* row = (const char*)plane + (size_t)r * stride;
+  switch (type) {
+    case PSGC_SAMPLE_U8: return (double)((const uint8_t*)row)[x];
+    case PSGC_SAMPLE_U16: return (double)((const uint16_t*)row)[x];
+    default: return ((const double*)row)[x];
+  }
+}
+
+int sample_size(int type) {
+  return type == PSGC_SAMPLE_U8 ? 1 : (type == PSGC_SAMPLE_U16 ? 2 : 8);
+}
+
+// Frame -> f64 plane (convertTo CV_64F), JSLAB_PhaseShiftDecoder_V2.cpp:22.
+std::vector<double> to_f64(const void* plane, int type, int64_t stride,
+                           int rows, int cols) {
+  std::vector<double> out((size_t)rows * cols);
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < cols; ++x)
+      out[(size_t)r * cols + x] = sample_at(plane, type, stride, r, x);
+  return out;
+}
+
+// ---- HDR fit (JSLAB_HDRPhaseFit.h:10-61) -----------------------------------
+// Design A = [1, cos d, -sin d] over the usable subset; pinv(A) = (A^T A)^-1 A^T
+// for the full-rank subsets that the reference keeps (its SVD rank test,
+// :29-31). The last bits of OpenCV's SVD-based invert are parity unpinned.
+struct HdrFit {
+  int steps = 0;
+  std::vector<char> valid;
+  std::vector<double> w;  // [mask][k][3]
+  explicit HdrFit(int n) : steps(n) {
+    const size_t nm = size_t(1) << n;
+    valid.assign(nm, 0);
+    w.assign(nm * n * 3, 0.0);
+    const double tau = 2.0 * PI;
+    for (size_t mask = 0; mask < nm; ++mask) {
+      std::vector<int> idx;
+      for (int k = 0; k < n; ++k)
+        if (mask & (size_t(1) << k)) idx.push_back(k);
+      if (idx.size() < 3) continue;
+      double AtA[3][3] = {};
+      std::vector<double> rows(idx.size() * 3);
+      for (size_t j = 0; j < idx.size(); ++j) {
+        const double d = tau * idx[j] / n;
+        rows[j * 3 + 0] = 1.0;
+        rows[j * 3 + 1] = std::cos(d);
+        rows[j * 3 + 2] = -std::sin(d);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) AtA[a][b] += rows[j * 3 + a] * rows[j * 3 + b];
+      }
+      // eigenvalue-free rank test: smallest singular value of A relative to
+      // the largest via the 3x3 Gram determinant and trace.
+      const double det =
+          AtA[0][0] * (AtA[1][1] * AtA[2][2] - AtA[1][2] * AtA[2][1]) -
+          AtA[0][1] * (AtA[1][0] * AtA[2][2] - AtA[1][2] * AtA[2][0]) +
+          AtA[0][2] * (AtA[1][0] * AtA[2][1] - AtA[1][1] * AtA[2][0]);
+      const double tr = AtA[0][0] + AtA[1][1] + AtA[2][2];
+      if (!(det > 1e-20 * tr * tr * tr)) continue;
+      double inv[3][3];
+      inv[0][0] = (AtA[1][1] * AtA[2][2] - AtA[1][2] * AtA[2][1]) / det;
+      inv[0][1] = (AtA[0][2] * AtA[2][1] - AtA[0][1] * AtA[2][2]) / det;
+      inv[0][2] = (AtA[0][1] * AtA[1][2] - AtA[0][2] * AtA[1][1]) / det;
+      inv[1][0] = (AtA[1][2] * AtA[2][0] - AtA[1][0] * AtA[2][2]) / det;
+      inv[1][1] = (AtA[0][0] * AtA[2][2] - AtA[0][2] * AtA[2][0]) / det;
+      inv[1][2] = (AtA[0][2] * AtA[1][0] - AtA[0][0] * AtA[1][2]) / det;
+      inv[2][0] = (AtA[1][0] * AtA[2][1] - AtA[1][1] * AtA[2][0]) / det;
+      inv[2][1] = (AtA[0][1] * AtA[2][0] - AtA[0][0] * AtA[2][1]) / det;
+      inv[2][2] = (AtA[0][0] * AtA[1][1] - AtA[0][1] * AtA[1][0]) / det;
+      for (size_t j = 0; j < idx.size(); ++j)
+        for (int a = 0; a < 3; ++a) {
+          double v = 0;
+          for (int b = 0; b < 3; ++b) v += inv[a][b] * rows[j * 3 + b];
+          w[(mask * n + idx[j]) * 3 + a] = v;
+        }
+      valid[mask] = 1;
+    }
+  }
+  bool solve(const double* I, double sat, double beta[3]) const {
+    size_t mask = 0;
+    for (int k = 0; k < steps; ++k)
+      if (I[k] >= 0.0 && I[k] < sat) mask |= size_t(1) << k;
+    if (!valid[mask]) return false;
+    beta[0] = beta[1] = beta[2] = 0;
+    for (int k = 0; k < steps; ++k)
+      if (mask & (size_t(1) << k))
+        for (int a = 0; a < 3; ++a) beta[a] += w[(mask * steps + k) * 3 + a] * I[k];
+    return std::isfinite(beta[0]) && std::isfinite(beta[1]) && std::isfinite(beta[2]);
+  }
+};
+
+// Wrapped phase + modulation of one level at one pixel
+// (JSLAB_PhaseShiftDecoder_V2.cpp:69-117). Returns false when the level
+// rejects the pixel; *mod receives the normalized modulation.
+bool level_phase(const psgc_decode_config& cfg, int level, const double* I,
+                 const std::vector<double>& st, const std::vector<double>& ct,
+                 const HdrFit* hdr, double* wrap, double* mod) {
+  const int inum = cfg.shift_num[level];
+  double sin_sum = 0, cos_sum = 0, sum_gray = 0;
+  if (cfg.enable_hdr) {
+    double beta[3];
+    if (!hdr->solve(I, cfg.hdr_saturation_gray, beta) ||
+        std::hypot(beta[1], beta[2]) <= cfg.hdr_min_amplitude || beta[0] <= 1e-3) {
+      *mod = 0.0;
+      return false;
+    }
+    cos_sum = beta[1] * inum / 2.0;
+    sin_sum = beta[2] * inum / 2.0;
+    sum_gray = beta[0] * inum;
+  } else {
+    for (int k = 0; k < inum; ++k) {
+      sin_sum -= st[k] * I[k];
+      cos_sum += ct[k] * I[k];
+      sum_gray += I[k];
+    }
+  }
+  *wrap = psgc::atan2(sin_sum, cos_sum);
+  double b = sin_sum * sin_sum + cos_sum * cos_sum;
+  const double mean_val = sum_gray / inum;
+  b = 2.0 * std::sqrt(b) / inum;
+  double norm_mod = 0.0;
+  if (mean_val > 1e-3) norm_mod = b / mean_val;
+  *mod = norm_mod;
+  return !(norm_mod < cfg.modulation_threshold);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Full decode, JSLAB_PhaseShiftDecoderV2::setImages + startCalculate
+// (JSLAB_PhaseShiftDecoder_V2.cpp:256-295) and the per-pixel body (:54-182).
+int orc_decode(const psgc_decode_config* cfg_, const void* const* frames,
+               int n_frames, int type, int rows, int cols, int64_t stride,
+               double* phase, int32_t* kmap, double* modmap, uint8_t* validmap) {
+  const psgc_decode_config cfg = *cfg_;
+  if (stride == 0) stride = (int64_t)cols * sample_size(type);
+  const size_t npix = (size_t)rows * cols;
+  const bool gray = cfg.unwrap_mode == PSGC_UNWRAP_GRAYCODE;
+  const int nlev = gray ? 1 : 3;
+  int nphase = 0;
+  for (int i = 0; i < nlev; ++i) nphase += cfg.shift_num[i];
+  const int need = gray ? nphase + 2 * cfg.gray_bits : nphase;
+  if (n_frames != need) return PSGC_E_INVALID;
+
+  // sin/cos tables (JSLAB_PhaseShiftDecoder_V2.cpp:203-211).
+  std::vector<double> st[3], ct[3];
+  for (int i = 0; i < nlev; ++i)
+    for (int k = 0; k < cfg.shift_num[i]; ++k) {
+      st[i].push_back(std::sin(2.0 * PI * k / cfg.shift_num[i]));
+      ct[i].push_back(std::cos(2.0 * PI * k / cfg.shift_num[i]));
+    }
+  std::vector<HdrFit> hdr;
+  if (cfg.enable_hdr)
+    for (int i = 0; i < nlev; ++i) hdr.emplace_back(cfg.shift_num[i]);
+
+  // Phase frames: f64 + pre-blur (JSLAB_ImageTypeConvertBodyV2, :18-28);
+  // with HDR the raw samples are kept (:262-265).
+  std::vector<std::vector<double>> img(nphase);
+  const std::vector<double> gk = gauss_kernel_d(3, cfg.preblur_sigma);
+  for (int f = 0; f < nphase; ++f) {
+    img[f] = to_f64(frames[f], type, stride, rows, cols);
+    if (!cfg.enable_hdr)
+      blur3_replicate(img[f].data(), img[f].data(), rows, cols, gk[1], gk[0]);
+  }
+
+  std::vector<double> ph(npix, 0.0), md(npix, 0.0);
+  std::vector<int32_t> kk(npix, cfg.enable_hdr ? -1 : 0);
+  std::vector<uint8_t> vd(npix, 0);
+  const double f0 = cfg.freq[0], f1 = cfg.freq[1], f2 = cfg.freq[2];
+
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r) {
+    for (int x = 0; x < cols; ++x) {
+      const size_t p = (size_t)r * cols + x;
+      double wrap[3];
+      bool ok = true;
+      int base = 0;
+      for (int i = 0; i < nlev; ++i) {
+        double I[32];
+        for (int k = 0; k < cfg.shift_num[i]; ++k) I[k] = img[base + k][p];
+        double m;
+        ok = level_phase(cfg, i, I, st[i], ct[i], cfg.enable_hdr ? &hdr[i] : nullptr,
+                         &wrap[i], &m);
+        if (i == 0) md[p] = m;
+        base += cfg.shift_num[i];
+        if (!ok) break;
+      }
+      if (!ok) continue;
+
+      if (gray) {
+        // Gray-code order with complement frames: bit b is set when the
+        // pattern frame is brighter than its complement.
+        unsigned g = 0;
+        for (int b = 0; b < cfg.gray_bits && ok; ++b) {
+          const double a = sample_at(frames[nphase + b], type, stride, r, x);
+          const double c = sample_at(frames[nphase + cfg.gray_bits + b], type, stride, r, x);
+          const double d = a - c;
+          if (std::fabs(d) < cfg.gray_min_contrast) ok = false;
+          g = (g << 1) | (d > 0 ? 1u : 0u);
+        }
+        if (!ok) continue;
+        unsigned n = g;
+        for (unsigned s = 1; s < 32; s <<= 1) n ^= n >> s;  // gray -> binary
+        const double w = wrap[0] < 0 ? wrap[0] + 2 * PI : wrap[0];
+        ph[p] = w + 2 * PI * (double)n;
+        kk[p] = (int32_t)n;
+        vd[p] = 255;
+        continue;
+      }
+
+      // Heterodyne unwrapping (JSLAB_PhaseShiftDecoder_V2.cpp:124-179).
+      double phase12, phase23, phase123;
+      if (wrap[0] > wrap[1]) phase12 = wrap[0] - wrap[1];
+      else phase12 = wrap[0] - wrap[1] + 2 * PI;
+      if (f1 > f2) {
+        if (wrap[1] > wrap[2]) phase23 = wrap[1] - wrap[2];
+        else phase23 = wrap[1] - wrap[2] + 2 * PI;
+      } else {
+        if (wrap[2] > wrap[1]) phase23 = wrap[2] - wrap[1];
+        else phase23 = wrap[2] - wrap[1] + 2 * PI;
+      }
+      if (phase12 > phase23) phase123 = phase12 - phase23;
+      else phase123 = phase12 - phase23 + 2 * PI;
+      const int f12 = std::abs(static_cast<int>(f0 - f1));
+      const int f23 = std::abs(static_cast<int>(f1 - f2));
+      const double uphase12 =
+          phase12 + std::round((phase123 * f12 - phase12) / (2 * PI)) * 2 * PI;
+      const double uphase1 =
+          wrap[0] + std::round((uphase12 * f0 / f12 - wrap[0]) / (2 * PI)) * 2 * PI;
+      const double k_val = std::round((uphase12 * f0 / f12 - wrap[0]) / (2 * PI));
+      const double uphase23 =
+          phase23 + std::round((phase123 * f23 - phase23) / (2 * PI)) * 2 * PI;
+      double uphase2 =
+          wrap[1] + std::round((uphase23 * f1 / f23 - wrap[1]) / (2 * PI)) * 2 * PI;
+      uphase2 = (uphase2 * f0) / f1;
+      double uphase3 =
+          wrap[2] + std::round((uphase23 * f2 / f23 - wrap[2]) / (2 * PI)) * 2 * PI;
+      uphase3 = (uphase3 * f0) / f2;
+      ph[p] = cfg.avg_three_phase ? (uphase1 + uphase2 + uphase3) / 3.0 : uphase1;
+      kk[p] = (int32_t)k_val;
+      vd[p] = 255;
+    }
+  }
+
+  if (gray) {
+    // Period-jump correction: an order that disagrees with the median of
+    // the valid absolute phases of its row neighbours (x-2..x+2) by about a
+    // full period is moved by that many periods.
+    std::vector<double> ph0 = ph;
+    std::vector<int32_t> k0 = kk;
+#pragma omp parallel for
+    for (int r = 0; r < rows; ++r)
+      for (int x = 0; x < cols; ++x) {
+        const size_t p = (size_t)r * cols + x;
+        if (!vd[p]) continue;
+        double v[5];
+        int n = 0;
+        for (int d = -2; d <= 2; ++d) {
+          const int xx = x + d;
+          if (xx < 0 || xx >= cols || !vd[(size_t)r * cols + xx]) continue;
+          v[n++] = ph0[(size_t)r * cols + xx];
+        }
+        std::sort(v, v + n);
+        const double med = v[n / 2];
+        const double jump = std::round((med - ph0[p]) / (2 * PI));
+        if (jump != 0.0) {
+          ph[p] = ph0[p] + jump * 2 * PI;
+          kk[p] = k0[p] + (int32_t)jump;
+        }
+      }
+  }
+
+  // Optional post-smooth (JSLAB_PhaseShiftDecoder_V2.cpp:280-294).
+  if (cfg.post_smooth) {
+    if (cfg.enable_hdr) {
+      std::vector<double> wgt(npix), ph2(npix), w2(npix);
+      for (size_t p = 0; p < npix; ++p) wgt[p] = vd[p] ? 1.0 : 0.0;  // 255 * (1/255)
+      blur3_replicate(ph.data(), ph2.data(), rows, cols, 0.5, 0.25);
+      blur3_replicate(wgt.data(), w2.data(), rows, cols, 0.5, 0.25);
+      for (size_t p = 0; p < npix; ++p) {
+        const double w = std::max(w2[p], 1e-12);
+        ph[p] = vd[p] ? ph2[p] / w : 0.0;
+      }
+    } else {
+      blur3_replicate(ph.data(), ph.data(), rows, cols, 0.5, 0.25);
+    }
+  }
+
+  if (phase) std::memcpy(phase, ph.data(), npix * sizeof(double));
+  if (kmap) std::memcpy(kmap, kk.data(), npix * sizeof(int32_t));
+  if (modmap) std::memcpy(modmap, md.data(), npix * sizeof(double));
+  if (validmap) std::memcpy(validmap, vd.data(), npix);
+  return PSGC_OK;
+}
+
+// stereoMatch (JSLAB_PhaseEpipolarReconstructor_V2.cpp:933-1006).
+int orc_stereo_match(const double* l_phase, const double* r_phase,
+                     const int32_t* l_k, const int32_t* r_k, int rows, int cols,
+                     int pha_f, double grad_threshold, double* disp,
+                     int32_t* match_col) {
+  const int max_rank_limit = pha_f;
+  const int knum = pha_f;
+#pragma omp parallel for
+  for (int i = 0; i < rows; ++i) {
+    const double* lp = l_phase + (size_t)i * cols;
+    const double* rp = r_phase + (size_t)i * cols;
+    const int32_t* lk = l_k + (size_t)i * cols;
+    const int32_t* rk = r_k + (size_t)i * cols;
+    double* dp = disp + (size_t)i * cols;
+    int32_t* mc = match_col ? match_col + (size_t)i * cols : nullptr;
+    for (int j = 0; j < cols; ++j) {
+      dp[j] = 0.0;
+      if (mc) mc[j] = -1;
+    }
+    std::vector<std::vector<int>> table(knum + 1);
+    for (int j = 0; j < cols; ++j) {
+      const int kv = rk[j];
+      if (kv >= 0 && kv <= knum) table[kv].push_back(j);
+    }
+    for (int j = 0; j < cols; ++j) {
+      const int k = lk[j];
+      if (k > max_rank_limit || k <= 0) continue;
+      if (k > knum) continue;
+      if (table[k].empty()) continue;
+      double min_cost = 1e9;
+      int best = -1;
+      const double lv = lp[j];
+      for (int c : table[k]) {
+        const double diff = std::fabs(rp[c] - lv);
+        if (diff < min_cost) {
+          min_cost = diff;
+          best = c;
+        }
+      }
+      if (min_cost >= grad_threshold) continue;
+      if (best < 2 || best >= cols - 1) continue;
+      const double c0 = rp[best - 1] - lv;
+      const double c1 = rp[best] - lv;
+      const double c2 = rp[best + 1] - lv;
+      const double num = c1 * (c0 - c2);
+      const double den = 2.0 * (c0 - c1) * (c1 - c2);
+      double delta = 0.0;
+      if (std::fabs(den) > 1e-9) delta = num / den;
+      dp[j] = (double)j - (double)best - delta;
+      if (mc) mc[j] = best;
+    }
+  }
+  return PSGC_OK;
+}
+
+// cv::medianBlur(src, dst, 5) on f32 with BORDER_REPLICATE.
+int orc_median5(const float* src, float* dst, int rows, int cols) {
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < cols; ++x) {
+      float v[25];
+      int n = 0;
+      for (int dy = -2; dy <= 2; ++dy) {
+        const int yy = std::min(std::max(r + dy, 0), rows - 1);
+        for (int dx = -2; dx <= 2; ++dx) {
+          const int xx = std::min(std::max(x + dx, 0), cols - 1);
+          v[n++] = src[(size_t)yy * cols + xx];
+        }
+      }
+      std::nth_element(v, v + 12, v + 25);
+      dst[(size_t)r * cols + x] = v[12];
+    }
+  return PSGC_OK;
+}
+
+// Q reprojection (JSLAB_PhaseEpipolarReconstructor_V2.cpp:414-461).
+int orc_reproject(const double* disp, int rows, int cols, const double* Q,
+                  const double* R1, double min_depth, double max_depth,
+                  int use_rect, float* xyz, uint8_t* P) {
+#pragma omp parallel for
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) {
+      const size_t p = (size_t)i * cols + j;
+      xyz[p * 3 + 0] = xyz[p * 3 + 1] = xyz[p * 3 + 2] = 0.f;
+      P[p] = 0;
+      const double d = disp[p];
+      if (std::fabs(d) < 1e-5) continue;
+      const double w = Q[14] * d + Q[15];
+      double x = j + Q[3];
+      double y = i + Q[7];
+      double z = Q[11];
+      const double inv_w = 1.0 / w;
+      x *= inv_w;
+      y *= inv_w;
+      z *= inv_w;
+      const double rx = R1[0] * x + R1[3] * y + R1[6] * z;
+      const double ry = R1[1] * x + R1[4] * y + R1[7] * z;
+      const double rz = R1[2] * x + R1[5] * y + R1[8] * z;
+      if (use_rect) {
+        if (z > min_depth && z < max_depth) {
+          xyz[p * 3 + 0] = (float)x;
+          xyz[p * 3 + 1] = (float)y;
+          xyz[p * 3 + 2] = (float)z;
+          P[p] = 1;
+        }
+      } else if (rz > min_depth && rz < max_depth) {
+        xyz[p * 3 + 0] = (float)rx;
+        xyz[p * 3 + 1] = (float)ry;
+        xyz[p * 3 + 2] = (float)rz;
+        P[p] = 1;
+      }
+    }
+  return PSGC_OK;
+}
+
+// applyGaussianFilterToPointCloudV2 (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+// 79-124); semantics GPU = the reference kernel's mask update and division
+// (JSLAB_PhaseEpipolarReconstructor_V2_GPU.cu:220-274).
+int orc_gaussian(const float* src, const uint8_t* P, float* dst, uint8_t* dstP,
+                 int rows, int cols, int win, float sigma, int semantics) {
+  const float count_threshold = 0.8f;
+  const int half = (win - 1) / 2;
+  const std::vector<double> k1 = gauss_kernel_d(win, sigma);
+  std::vector<float> kf(win), k2((size_t)win * win);
+  for (int i = 0; i < win; ++i) kf[i] = (float)k1[i];
+  for (int i = 0; i < win; ++i)
+    for (int j = 0; j < win; ++j) k2[(size_t)i * win + j] = kf[i] * kf[j];
+  const size_t npix = (size_t)rows * cols;
+  std::fill(dst, dst + npix * 3, 0.f);
+  if (semantics == PSGC_SEM_CPU) std::memcpy(dstP, P, npix);
+  else std::fill(dstP, dstP + npix, (uint8_t)0);
+#pragma omp parallel for
+  for (int r = half; r < rows - half; ++r)
+    for (int x = half; x < cols - half; ++x) {
+      const size_t p = (size_t)r * cols + x;
+      if (P[p] == 0) continue;
+      const float cx = src[p * 3];
+      float s0 = 0, s1 = 0, s2 = 0, count = 0;
+      for (int ik = -half; ik <= half; ++ik)
+        for (int jk = -half; jk <= half; ++jk) {
+          const size_t q = (size_t)(r + ik) * cols + (x + jk);
+          if (P[q] == 0) continue;
+          if (std::fabs(src[q * 3] - cx) < 3.0f) {
+            const float w = k2[(size_t)(ik + half) * win + (jk + half)];
+            s0 += src[q * 3 + 0] * w;
+            s1 += src[q * 3 + 1] * w;
+            s2 += src[q * 3 + 2] * w;
+            count += w;
+          }
+        }
+      if (count > count_threshold) {
+        if (semantics == PSGC_SEM_CPU) {
+          const float inv = 1.f / count;
+          dst[p * 3 + 0] = s0 * inv;
+          dst[p * 3 + 1] = s1 * inv;
+          dst[p * 3 + 2] = s2 * inv;
+        } else {
+          dst[p * 3 + 0] = s0 / count;
+          dst[p * 3 + 1] = s1 / count;
+          dst[p * 3 + 2] = s2 / count;
+          dstP[p] = 1;
+        }
+      }
+    }
+  return PSGC_OK;
+}
+
+// applyRadiusOutlierFilterToPointCloud
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:126-195).
+int orc_ror(const float* src, const uint8_t* P, float* dst, uint8_t* dstP,
+            int rows, int cols, float radius, int min_neighbors, int window) {
+  const size_t npix = (size_t)rows * cols;
+  std::fill(dst, dst + npix * 3, 0.f);
+  std::fill(dstP, dstP + npix, (uint8_t)0);
+  const float r2 = radius * radius;
+  const int half = window / 2;
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < cols; ++x) {
+      const size_t p = (size_t)r * cols + x;
+      if (P[p] == 0) continue;
+      const float px = src[p * 3], py = src[p * 3 + 1], pz = src[p * 3 + 2];
+      int cnt = 0;
+      const int r0 = std::max(0, r - half), r1 = std::min(rows, r + half + 1);
+      const int c0 = std::max(0, x - half), c1 = std::min(cols, x + half + 1);
+      for (int i = r0; i < r1 && cnt < min_neighbors; ++i)
+        for (int j = c0; j < c1; ++j) {
+          const size_t q = (size_t)i * cols + j;
+          if (P[q] == 0 || q == p) continue;
+          const float dx = px - src[q * 3];
+          const float dy = py - src[q * 3 + 1];
+          const float dz = pz - src[q * 3 + 2];
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 <= r2 && ++cnt >= min_neighbors) break;
+        }
+      if (cnt >= min_neighbors) {
+        dst[p * 3] = px;
+        dst[p * 3 + 1] = py;
+        dst[p * 3 + 2] = pz;
+        dstP[p] = 1;
+      }
+    }
+  return PSGC_OK;
+}
+
+// Fixed pairwise reduction: slots [0, n) with n a power of two, summed as
+// s[i] += s[i + h] for h = n/2, n/4, ..., 1. The HIP kernels reduce in the
+// same order so that the statistics are bit-identical.
+static double tree_sum(std::vector<double> s) {
+  for (size_t h = s.size() / 2; h >= 1; h /= 2)
+    for (size_t i = 0; i < h; ++i) s[i] += s[i + h];
+  return s.empty() ? 0.0 : s[0];
+}
+
+// Statistical outlier removal (PCL StatisticalOutlierRemoval definition on
+// an organized window); see psgc.h for the exact rule. Reduction order:
+// chunks of 1024 consecutive pixels are tree-summed, the chunk partials are
+// tree-summed over the next power of two.
+int orc_sor(const float* src, const uint8_t* P, float* dst, uint8_t* dstP,
+            int rows, int cols, int k, float std_mul, int half,
+            double* mean_dist, double* stats) {
+  const size_t npix = (size_t)rows * cols;
+  std::vector<double> md(npix, 0.0);
+  std::vector<uint8_t> ok(npix, 0);
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r) {
+    std::vector<float> d2s;
+    for (int x = 0; x < cols; ++x) {
+      const size_t p = (size_t)r * cols + x;
+      if (P[p] == 0) continue;
+      d2s.clear();
+      const float px = src[p * 3], py = src[p * 3 + 1], pz = src[p * 3 + 2];
+      for (int i = std::max(0, r - half); i < std::min(rows, r + half + 1); ++i)
+        for (int j = std::max(0, x - half); j < std::min(cols, x + half + 1); ++j) {
+          const size_t q = (size_t)i * cols + j;
+          if (P[q] == 0 || q == p) continue;
+          const float dx = px - src[q * 3];
+          const float dy = py - src[q * 3 + 1];
+          const float dz = pz - src[q * 3 + 2];
+          d2s.push_back(dx * dx + dy * dy + dz * dz);
+        }
+      if ((int)d2s.size() < k) continue;
+      std::partial_sort(d2s.begin(), d2s.begin() + k, d2s.end());
+      double s = 0;
+      for (int m = 0; m < k; ++m) s += std::sqrt((double)d2s[m]);
+      md[p] = s / k;
+      ok[p] = 1;
+    }
+  }
+  const size_t chunk = 1024;
+  const size_t nchunks = (npix + chunk - 1) / chunk;
+  size_t np2 = 1;
+  while (np2 < nchunks) np2 *= 2;
+  std::vector<double> cs(np2, 0.0), cq(np2, 0.0);
+  std::vector<long long> cn(nchunks, 0);
+  for (size_t c = 0; c < nchunks; ++c) {
+    std::vector<double> s(chunk, 0.0), q(chunk, 0.0);
+    for (size_t t = 0; t < chunk; ++t) {
+      const size_t p = c * chunk + t;
+      if (p < npix && ok[p]) {
+        s[t] = md[p];
+        q[t] = md[p] * md[p];
+        cn[c]++;
+      }
+    }
+    cs[c] = tree_sum(s);
+    cq[c] = tree_sum(q);
+  }
+  long long n = 0;
+  for (auto v : cn) n += v;
+  const double sum = tree_sum(cs), sq = tree_sum(cq);
+  double mean = 0, sd = 0;
+  if (n > 0) mean = sum / (double)n;
+  if (n > 1) {
+    const double var = (sq - sum * sum / (double)n) / ((double)n - 1.0);
+    sd = std::sqrt(std::max(var, 0.0));
+  }
+  const double thr = mean + (double)std_mul * sd;
+  for (size_t p = 0; p < npix; ++p) {
+    const bool keep = ok[p] && md[p] <= thr;
+    dstP[p] = keep ? 1 : 0;
+    for (int c = 0; c < 3; ++c) dst[p * 3 + c] = keep ? src[p * 3 + c] : 0.f;
+  }
+  if (mean_dist) std::memcpy(mean_dist, md.data(), npix * sizeof(double));
+  if (stats) {
+    stats[0] = (double)n;
+    stats[1] = mean;
+    stats[2] = sd;
+    stats[3] = thr;
+  }
+  return PSGC_OK;
+}
+
+}  // extern "C"
+
+// ---- MLS (JSLAB_MLS_Filter.cpp:11-415) ------------------------------------
+namespace {
+
+void eigen3(const float C[3][3], float ev[3], float V[3][3]) {
+  const float p1 = C[0][1] * C[0][1] + C[0][2] * C[0][2] + C[1][2] * C[1][2];
+  if (p1 < 1e-20f) {
+    ev[0] = C[0][0];
+    ev[1] = C[1][1];
+    ev[2] = C[2][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.f : 0.f;
+    for (int i = 0; i < 2; ++i)
+      for (int j = i + 1; j < 3; ++j)
+        if (ev[i] > ev[j]) {
+          std::swap(ev[i], ev[j]);
+          for (int c = 0; c < 3; ++c) std::swap(V[i][c], V[j][c]);
+        }
+    return;
+  }
+  const float trace = C[0][0] + C[1][1] + C[2][2];
+  const float q = trace / 3.0f;
+  float B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B[i][j] = C[i][j];
+  B[0][0] -= q;
+  B[1][1] -= q;
+  B[2][2] -= q;
+  const float p2 = B[0][0] * B[0][0] + B[1][1] * B[1][1] + B[2][2] * B[2][2] + 2.0f * p1;
+  const float p = std::sqrt(p2 / 6.0f);
+  const float inv_p = (p > 1e-15f) ? (1.0f / p) : 0.0f;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B[i][j] *= inv_p;
+  const float detB = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                     B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                     B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
+  float r = detB * 0.5f;
+  r = std::max(-1.0f, std::min(1.0f, r));
+  const float phi = std::acos(r) / 3.0f;
+  const float PIf = 3.14159265358979323846f;
+  ev[0] = q + 2.0f * p * std::cos(phi + 2.0f * PIf / 3.0f);
+  ev[2] = q + 2.0f * p * std::cos(phi);
+  ev[1] = trace - ev[0] - ev[2];
+  for (int idx = 0; idx < 3; ++idx) {
+    float A[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[i][j] = C[i][j];
+    A[0][0] -= ev[idx];
+    A[1][1] -= ev[idx];
+    A[2][2] -= ev[idx];
+    float c01[3] = {A[0][1] * A[1][2] - A[0][2] * A[1][1],
+                    A[0][2] * A[1][0] - A[0][0] * A[1][2],
+                    A[0][0] * A[1][1] - A[0][1] * A[1][0]};
+    float c02[3] = {A[0][1] * A[2][2] - A[0][2] * A[2][1],
+                    A[0][2] * A[2][0] - A[0][0] * A[2][2],
+                    A[0][0] * A[2][1] - A[0][1] * A[2][0]};
+    float c12[3] = {A[1][1] * A[2][2] - A[1][2] * A[2][1],
+                    A[1][2] * A[2][0] - A[1][0] * A[2][2],
+                    A[1][0] * A[2][1] - A[1][1] * A[2][0]};
+    const float l01 = c01[0] * c01[0] + c01[1] * c01[1] + c01[2] * c01[2];
+    const float l02 = c02[0] * c02[0] + c02[1] * c02[1] + c02[2] * c02[2];
+    const float l12 = c12[0] * c12[0] + c12[1] * c12[1] + c12[2] * c12[2];
+    float* best = c12;
+    float bl = l12;
+    if (l01 > bl) { best = c01; bl = l01; }
+    if (l02 > bl) { best = c02; bl = l02; }
+    const float inv = (bl > 1e-20f) ? (1.0f / std::sqrt(bl)) : 0.0f;
+    V[idx][0] = best[0] * inv;
+    V[idx][1] = best[1] * inv;
+    V[idx][2] = best[2] * inv;
+  }
+}
+
+bool chol(const float A[6][6], const float b[6], float x[6], int nc) {
+  float L[6][6] = {};
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j <= i; ++j) {
+      float s = A[i][j];
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (s <= 1e-12f) return false;
+        L[i][j] = std::sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  float y[6] = {};
+  for (int i = 0; i < nc; ++i) {
+    float s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+  for (int i = nc - 1; i >= 0; --i) {
+    float s = y[i];
+    for (int k = i + 1; k < nc; ++k) s -= L[k][i] * x[k];
+    x[i] = s / L[i][i];
+  }
+  return true;
+}
+
+void mls_pass(const float* src, const uint8_t* P, float* dst, int rows, int cols,
+              int hw, float sigma, int order_in, float r2cut, int semantics) {
+  std::memcpy(dst, src, (size_t)rows * cols * 3 * sizeof(float));
+  const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+  const int minnb[3] = {3, 5, 8};
+  const int order = std::max(0, std::min(2, order_in));
+#pragma omp parallel for
+  for (int r = 0; r < rows; ++r)
+    for (int x = 0; x < cols; ++x) {
+      const size_t p = (size_t)r * cols + x;
+      if (P[p] == 0) continue;
+      const float pi0 = src[p * 3], pi1 = src[p * 3 + 1], pi2 = src[p * 3 + 2];
+      const int r0 = std::max(0, r - hw), r1 = std::min(rows, r + hw + 1);
+      const int c0 = std::max(0, x - hw), c1 = std::min(cols, x + hw + 1);
+      float cx = 0, cy = 0, cz = 0, wsum = 0;
+      int nv = 0;
+      for (int i = r0; i < r1; ++i)
+        for (int j = c0; j < c1; ++j) {
+          const size_t q = (size_t)i * cols + j;
+          if (P[q] == 0) continue;
+          const float* pk = src + q * 3;
+          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 > r2cut) continue;
+          const float w = std::exp(-d2 * inv_2s2);
+          cx += w * pk[0];
+          cy += w * pk[1];
+          cz += w * pk[2];
+          wsum += w;
+          nv++;
+        }
+      if (nv < minnb[0] || wsum < 1e-12f) continue;
+      const float iw = 1.0f / wsum;
+      cx *= iw;
+      cy *= iw;
+      cz *= iw;
+      float cov[3][3] = {};
+      for (int i = r0; i < r1; ++i)
+        for (int j = c0; j < c1; ++j) {
+          const size_t q = (size_t)i * cols + j;
+          if (P[q] == 0) continue;
+          const float* pk = src + q * 3;
+          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 > r2cut) continue;
+          const float w = std::exp(-d2 * inv_2s2);
+          const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
+          cov[0][0] += w * ex * ex;
+          cov[0][1] += w * ex * ey;
+          cov[0][2] += w * ex * ez;
+          cov[1][1] += w * ey * ey;
+          cov[1][2] += w * ey * ez;
+          cov[2][2] += w * ez * ez;
+        }
+      cov[1][0] = cov[0][1];
+      cov[2][0] = cov[0][2];
+      cov[2][1] = cov[1][2];
+      float ev[3], V[3][3];
+      eigen3(cov, ev, V);
+      const float nx = V[0][0], ny = V[0][1], nz = V[0][2];
+      const float t1x = V[1][0], t1y = V[1][1], t1z = V[1][2];
+      const float t2x = V[2][0], t2y = V[2][1], t2z = V[2][2];
+      int cur;
+      if (semantics == PSGC_SEM_CPU) {
+        cur = (nv >= minnb[order]) ? order : 0;
+        if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
+      } else {
+        cur = order;
+        if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
+      }
+      float* out = dst + p * 3;
+      if (cur == 0) {
+        const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
+        out[0] = pi0 - dot * nx;
+        out[1] = pi1 - dot * ny;
+        out[2] = pi2 - dot * nz;
+        continue;
+      }
+      const int nc = (cur == 1) ? 3 : 6;
+      float AtWA[6][6] = {}, AtWf[6] = {};
+      for (int i = r0; i < r1; ++i)
+        for (int j = c0; j < c1; ++j) {
+          const size_t q = (size_t)i * cols + j;
+          if (P[q] == 0) continue;
+          const float* pk = src + q * 3;
+          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 > r2cut) continue;
+          const float w = std::exp(-d2 * inv_2s2);
+          const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
+          const float u = ex * t1x + ey * t1y + ez * t1z;
+          const float v = ex * t2x + ey * t2y + ez * t2z;
+          const float f = ex * nx + ey * ny + ez * nz;
+          float basis[6] = {1.0f, u, v, 0, 0, 0};
+          if (nc == 6) {
+            basis[3] = u * u;
+            basis[4] = u * v;
+            basis[5] = v * v;
+          }
+          for (int a = 0; a < nc; ++a) {
+            AtWf[a] += w * basis[a] * f;
+            for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
+          }
+        }
+      for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < a; ++b) AtWA[a][b] = AtWA[b][a];
+      float co[6] = {};
+      bool solved = chol(AtWA, AtWf, co, nc);
+      if (!solved && cur == 2) {
+        float A1[6][6] = {}, b1[6] = {};
+        for (int a = 0; a < 3; ++a) {
+          b1[a] = AtWf[a];
+          for (int b = 0; b < 3; ++b) A1[a][b] = AtWA[a][b];
+        }
+        solved = chol(A1, b1, co, 3);
+        if (solved) cur = 1;
+      }
+      if (!solved) {
+        const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
+        out[0] = pi0 - dot * nx;
+        out[1] = pi1 - dot * ny;
+        out[2] = pi2 - dot * nz;
+        continue;
+      }
+      const float ex0 = pi0 - cx, ey0 = pi1 - cy, ez0 = pi2 - cz;
+      const float u0 = ex0 * t1x + ey0 * t1y + ez0 * t1z;
+      const float v0 = ex0 * t2x + ey0 * t2y + ez0 * t2z;
+      float ff = co[0] + co[1] * u0 + co[2] * v0;
+      if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
+      out[0] = cx + u0 * t1x + v0 * t2x + ff * nx;
+      out[1] = cy + u0 * t1y + v0 * t2y + ff * ny;
+      out[2] = cz + u0 * t1z + v0 * t2z + ff * nz;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// applyMLSFilterToPointCloud_CPU (JSLAB_MLS_Filter.cpp:396-415).
+int orc_mls(const float* src, const uint8_t* P, float* dst, int rows, int cols,
+            int window, float sigma, int order, float max_radius, int iterations,
+            int semantics) {
+  const float r2 = (max_radius > 0.0f) ? max_radius * max_radius : 9.0f * sigma * sigma;
+  const int n_iter = std::max(1, iterations);
+  const size_t n3 = (size_t)rows * cols * 3;
+  std::vector<float> cur(src, src + n3);
+  for (int it = 0; it < n_iter; ++it) {
+    mls_pass(cur.data(), P, dst, rows, cols, window, sigma, order, r2, semantics);
+    if (it + 1 < n_iter) std::memcpy(cur.data(), dst, n3 * sizeof(float));
+  }
+  return PSGC_OK;
+}
+
+// JSLAB_PhaseEpipolarReconstructor_V2::startCalculate
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:397-480) on rectified inputs.
+int orc_reconstruct(const psgc_recon_config* cfg, const double* l_phase,
+                    const double* r_phase, const int32_t* l_k,
+                    const int32_t* r_k, int rows, int cols, int pha_f,
+                    const double* Q, const double* R1, float* xyz, uint8_t* P,
+                    float* disp_out, int32_t* match_col) {
+  const size_t npix = (size_t)rows * cols;
+  std::vector<double> disp(npix);
+  orc_stereo_match(l_phase, r_phase, l_k, r_k, rows, cols, pha_f,
+                   cfg->grad_threshold, disp.data(), match_col);
+  // The reference GPU path stores the disparity as f32 (…V2_GPU.cu:382);
+  // the CPU path converts to f32 around the median (…V2.cpp:405-410).
+  std::vector<float> d32(npix), d32m(npix);
+  for (size_t p = 0; p < npix; ++p) d32[p] = (float)disp[p];
+  const bool f32_disp = cfg->use_disp_median_filter || cfg->semantics == PSGC_SEM_GPU;
+  if (cfg->use_disp_median_filter) {
+    orc_median5(d32.data(), d32m.data(), rows, cols);
+    d32.swap(d32m);
+  }
+  if (f32_disp)
+    for (size_t p = 0; p < npix; ++p) disp[p] = (double)d32[p];
+  if (disp_out)
+    for (size_t p = 0; p < npix; ++p) disp_out[p] = (float)disp[p];
+  std::vector<float> xyz0(npix * 3), xyzg(npix * 3), xyzr(npix * 3);
+  std::vector<uint8_t> P0(npix), Pg(npix), Pr(npix);
+  orc_reproject(disp.data(), rows, cols, Q, R1, cfg->min_depth, cfg->max_depth,
+                cfg->use_rectified_coords, xyz0.data(), P0.data());
+  orc_gaussian(xyz0.data(), P0.data(), xyzg.data(), Pg.data(), rows, cols,
+               cfg->gauss_window, cfg->gauss_sigma, cfg->semantics);
+  // CPU semantics: the outlier stage reads the pre-Gaussian mask (…V2.cpp:466).
+  const uint8_t* Pin = cfg->semantics == PSGC_SEM_CPU ? P0.data() : Pg.data();
+  if (cfg->outlier_mode == PSGC_OUTLIER_ROR) {
+    orc_ror(xyzg.data(), Pin, xyzr.data(), Pr.data(), rows, cols, cfg->radius_search,
+            cfg->radius_min_neighbors, cfg->radius_window_size);
+  } else if (cfg->outlier_mode == PSGC_OUTLIER_SOR) {
+    orc_sor(xyzg.data(), Pin, xyzr.data(), Pr.data(), rows, cols, cfg->sor_k,
+            cfg->sor_std_mul, cfg->sor_window, nullptr, nullptr);
+  } else {
+    xyzr = xyzg;
+    std::memcpy(Pr.data(), Pin, npix);
+  }
+  std::memcpy(P, Pr.data(), npix);
+  if (cfg->use_mls) {
+    orc_mls(xyzr.data(), Pr.data(), xyz, rows, cols, cfg->mls_window_size,
+            cfg->mls_spatial_sigma, cfg->mls_polynomial_order, cfg->mls_max_radius,
+            cfg->mls_iterations, cfg->semantics);
+  } else {
+    std::memcpy(xyz, xyzr.data(), npix * 3 * sizeof(float));
+  }
+  return PSGC_OK;
+}
+
+int orc_atan2_many(const double* y, const double* x, double* out, long n) {
+  for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
+  return PSGC_OK;
+}
+
+}  // extern "C"

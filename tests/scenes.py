@@ -1,0 +1,66 @@
+"""Small seeded scenes shared by the tests (rendered on the CPU)."""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from zim_amd import synth
+from zim_amd._capi import DecodeConfig, ReconConfig, SEM_GPU, OUTLIER_SOR
+
+C2_SMALL = dict(rows=192, cols=256)
+
+
+@functools.lru_cache(maxsize=None)
+def c2_small(seed=3, rows=192, cols=256):
+    """Config-2 shaped pair (u16 12-bit, 70/64/59 heterodyne, 4 steps),
+    scaled down so the CPU restatement runs in well under a second."""
+    f = 2400.0 * cols / 2448.0
+    p = synth.render_c2(seed, rows=rows, cols=cols, f=f)
+    left = [x.numpy() for x in p.left]
+    right = [x.numpy() for x in p.right]
+    return p, left, right
+
+
+@functools.lru_cache(maxsize=None)
+def c1_small(seed=5, gray=True):
+    p = synth.render_c1(seed, rows=120, cols=160, with_gray=gray)
+    return p, [x.numpy() for x in p.left], [x.numpy() for x in p.right]
+
+
+def c2_decode_config(**kw):
+    base = dict(freq=(70, 64, 59), shift_num=(4, 4, 4), modulation_threshold=0.3)
+    base.update(kw)
+    return DecodeConfig.default(**base)
+
+
+def c1_gray_config(**kw):
+    base = dict(freq=(50.0, 0.0, 0.0), shift_num=(4, 0, 0), modulation_threshold=0.3,
+                unwrap_mode=1, gray_bits=6, gray_period_px=16.0, gray_min_contrast=20.0)
+    base.update(kw)
+    return DecodeConfig.default(**base)
+
+
+def c2_recon_config(**kw):
+    base = dict(use_mls=1, mls_polynomial_order=2, mls_max_radius=1.5, mls_spatial_sigma=0.5,
+                mls_window_size=6, mls_iterations=2, semantics=SEM_GPU, gauss_window=5,
+                gauss_sigma=1.0, outlier_mode=OUTLIER_SOR, sor_k=30, sor_std_mul=1.0,
+                sor_window=4)
+    base.update(kw)
+    return ReconConfig.default_gpu(**base)
+
+
+def rng_points(rows, cols, seed=0, holes=0.1, z0=700.0):
+    """A smooth organized cloud with holes, for filter-stage tests."""
+    g = np.random.default_rng(seed)
+    ii, jj = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    z = z0 + 20 * np.sin(ii / 17.0) * np.cos(jj / 23.0) + g.normal(0, 0.05, (rows, cols))
+    x = (jj - cols / 2) * z / 2400.0
+    y = (ii - rows / 2) * z / 2400.0
+    xyz = np.stack([x, y, z], -1).astype(np.float32)
+    P = (g.random((rows, cols)) > holes).astype(np.uint8)
+    # a few far outliers
+    out = g.random((rows, cols)) < 0.01
+    xyz[out, 2] += g.normal(0, 5, out.sum()).astype(np.float32)
+    xyz[P == 0] = 0
+    return xyz, P

@@ -1,0 +1,100 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/psgc.h declares, reports a missing device cleanly, and its
+structs and defaults match the Python mirror (which mirrors the reference
+Config structs)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from zim_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(psgc_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_internal_symbols_exported():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", _capi.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    syms = [l.split()[-1] for l in out.splitlines() if " T " in l]
+    assert all(s.startswith("psgc_") for s in syms), [s for s in syms if not s.startswith("psgc_")]
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-device path")
+def test_no_device_is_reported():
+    lib = _capi.load()
+    assert lib.psgc_device_count() == 0
+    h = C.c_void_p()
+    rc = lib.psgc_create(C.byref(h), 0)
+    assert rc == _capi.PSGC_E_NODEVICE
+    assert b"device" in lib.psgc_last_error()
+    assert lib.psgc_warmup() == _capi.PSGC_E_NODEVICE
+
+
+def test_null_context_is_invalid():
+    lib = _capi.load()
+    cfg = _capi.DecodeConfig.default()
+    rc = lib.psgc_decode(None, C.byref(cfg), None, 0, 0, 1, 1, 0, None, None, None, None, 0)
+    assert rc == _capi.PSGC_E_INVALID
+    rc = lib.psgc_reconstruct(None, None, None, None, None, None, 1, 1, 1, None, None, None,
+                              None, None, None, 0)
+    assert rc == _capi.PSGC_E_INVALID
+
+
+def test_struct_defaults_match_reference_configs():
+    lib = _capi.load()
+    d = _capi.DecodeConfig()
+    lib.psgc_decode_config_default(C.byref(d))
+    ref = _capi.DecodeConfig.default()
+    assert bytes(d) == bytes(ref)
+    r = _capi.ReconConfig()
+    lib.psgc_recon_config_default(C.byref(r))
+    assert bytes(r) == bytes(_capi.ReconConfig.default())
+    lib.psgc_recon_config_default_gpu(C.byref(r))
+    assert bytes(r) == bytes(_capi.ReconConfig.default_gpu())
+    # values of JSLAB_PhaseShiftDecoder.h:30-52 and
+    # JSLAB_PhaseEpipolarReconstructor_V2.h:21-54 / _V2_GPU.h:30-60
+    assert ref.modulation_threshold == 1.0 and ref.preblur_sigma == 3.0
+    rc = _capi.ReconConfig.default()
+    assert (rc.grad_threshold, rc.radius_min_neighbors, rc.radius_window_size) == (1.0, 10, 10)
+    assert abs(rc.radius_search - 0.7) < 1e-7 and rc.mls_iterations == 3
+    rg = _capi.ReconConfig.default_gpu()
+    assert (rg.radius_min_neighbors, rg.mls_window_size, rg.mls_iterations) == (5, 25, 2)
+
+
+def test_rectify_ideal_closed_form():
+    lib = _capi.load()
+    Q = np.zeros(16)
+    R1 = np.zeros(9)
+    assert lib.psgc_rectify_ideal(2400.0, 2400.0, 1223.5, 1023.5, 1200.0, 150.0,
+                                  _capi.ptr(Q), _capi.ptr(R1)) == 0
+    assert np.allclose(R1, np.eye(3).ravel())
+    # w = (d - (cx1 - cx2)) / B, Z = f / w
+    d = 500.0
+    w = Q[14] * d + Q[15]
+    assert abs(Q[11] / w - 2400.0 * 150.0 / (d - 23.5)) < 1e-9
+    assert lib.psgc_rectify_ideal(0.0, 1.0, 0, 0, 0, 1.0, _capi.ptr(Q), _capi.ptr(R1)) \
+        == _capi.PSGC_E_INVALID

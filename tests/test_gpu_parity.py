@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path through the C ABI against the CPU restatement
+(oracle/) on the same seeded inputs, stage-isolated as the reference's own
+compare harness does (JSLAB_Demo_Dual.cpp:448-513).
+
+Bars: fringe orders, validity masks, integer match columns and outlier masks
+bit-exact; phase within 1e-5 rad; disparity within 1e-3 px; XYZ within
+1e-3 mm.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import zim_amd
+from zim_amd import _capi
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+PHASE_TOL = 1e-5   # rad
+DISP_TOL = 1e-3    # px
+XYZ_TOL = 1e-3     # mm
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def _decode_both(cfg, frames, ctx):
+    ref = oracle.decode(cfg, frames)
+    got = zim_amd.decode(cfg, [dev(f) for f in frames], ctx)
+    torch.cuda.synchronize()
+    return ref, [host(t) for t in got]
+
+
+def _assert_decode(ref, got):
+    rph, rk, rmod, rv = ref
+    gph, gk, gmod, gv = got
+    assert np.array_equal(gv, rv), f"valid mismatch: {(gv != rv).sum()} px"
+    assert np.array_equal(gk, rk), f"k mismatch: {(gk != rk).sum()} px"
+    assert np.abs(gph - rph).max() <= PHASE_TOL
+    assert np.abs(gmod - rmod).max() <= 1e-12
+    # the shared FP64 routines make these bit-identical in practice
+    print("phase max |diff|", np.abs(gph - rph).max(), "bit-equal:", np.array_equal(gph, rph))
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_decode_c2_u16(ctx, seed):
+    _, left, right = scenes.c2_small(seed)
+    cfg = scenes.c2_decode_config()
+    for frames in (left, right):
+        ref, got = _decode_both(cfg, frames, ctx)
+        _assert_decode(ref, got)
+        assert ref[3].mean() > 0.5  # the scene decodes
+
+
+def test_decode_variants(ctx):
+    _, left, _ = scenes.c2_small(3)
+    for kw in (dict(avg_three_phase=0), dict(post_smooth=0), dict(preblur_sigma=0.0),
+               dict(modulation_threshold=0.6)):
+        cfg = scenes.c2_decode_config(**kw)
+        ref, got = _decode_both(cfg, left, ctx)
+        _assert_decode(ref, got)
+
+
+def test_decode_f64_and_u8_inputs(ctx):
+    _, left, _ = scenes.c2_small(3)
+    cfg = scenes.c2_decode_config()
+    f64 = [f.astype(np.float64) for f in left]
+    ref, got = _decode_both(cfg, f64, ctx)
+    _assert_decode(ref, got)
+    u8 = [(f >> 4).astype(np.uint8) for f in left]
+    ref, got = _decode_both(cfg, u8, ctx)
+    _assert_decode(ref, got)
+
+
+def test_decode_gray_code(ctx):
+    _, left, right = scenes.c1_small(5)
+    cfg = scenes.c1_gray_config()
+    for frames in (left, right):
+        ref, got = _decode_both(cfg, frames, ctx)
+        _assert_decode(ref, got)
+
+
+def test_decode_host_mode_equals_device_mode(ctx):
+    _, left, _ = scenes.c2_small(3)
+    cfg = scenes.c2_decode_config()
+    a = zim_amd.decode(cfg, left, ctx)
+    b = zim_amd.decode(cfg, [dev(f) for f in left], ctx)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, host(y))
+
+
+def _decoded_pair(seed=3):
+    p, left, right = scenes.c2_small(seed)
+    cfg = scenes.c2_decode_config()
+    L = oracle.decode(cfg, left)
+    R = oracle.decode(cfg, right)
+    return p, L, R
+
+
+def test_stereo_match_stage_isolated(ctx):
+    p, L, R = _decoded_pair()
+    ref_d, ref_mc = oracle.stereo_match(L[0], R[0], L[1], R[1], 70, 1.0)
+    rows, cols = L[0].shape
+    d = torch.empty((rows, cols), dtype=torch.float64, device="cuda")
+    mc = torch.empty((rows, cols), dtype=torch.int32, device="cuda")
+    lib = ctx.lib
+    ctx.bind_stream()
+    args = [dev(L[0]), dev(R[0]), dev(L[1]), dev(R[1])]
+    _capi.check(lib.psgc_stereo_match(ctx.h, *[_capi.ptr(a) for a in args], rows, cols, 70, 1.0,
+                                      _capi.ptr(d), _capi.ptr(mc), _capi.MEM_DEVICE))
+    torch.cuda.synchronize()
+    assert np.array_equal(host(mc), ref_mc)
+    assert np.abs(host(d) - ref_d).max() <= DISP_TOL
+    assert (ref_mc >= 0).mean() > 0.3
+
+
+def test_median_reproject(ctx):
+    g = np.random.default_rng(1)
+    disp = (g.random((97, 131)) * 600).astype(np.float32)
+    disp[g.random(disp.shape) < 0.2] = 0
+    ref = oracle.median5(disp)
+    got = np.empty_like(disp)
+    _capi.check(ctx.lib.psgc_median5(ctx.h, _capi.ptr(disp), _capi.ptr(got), 97, 131,
+                                     _capi.MEM_HOST))
+    assert np.array_equal(got, ref)
+    Q = [1, 0, 0, -65, 0, 1, 0, -48, 0, 0, 0, 2400.0, 0, 0, 1 / 150.0, 0]
+    R1 = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    for use_rect in (1, 0):
+        rx, rP = oracle.reproject(ref.astype(np.float64), Q, R1, 0, 1000, use_rect)
+        gx = np.empty_like(rx)
+        gP = np.empty_like(rP)
+        _capi.check(ctx.lib.psgc_reproject(ctx.h, _capi.ptr(ref), 97, 131,
+                                           _capi.ptr(np.array(Q, np.float64)),
+                                           _capi.ptr(np.array(R1, np.float64)), 0.0, 1000.0,
+                                           use_rect, _capi.ptr(gx), _capi.ptr(gP),
+                                           _capi.MEM_HOST))
+        assert np.array_equal(gP, rP)
+        assert np.array_equal(gx, rx)
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+@pytest.mark.parametrize("win,sigma", [(3, 0.7), (5, 1.0)])
+def test_gaussian(ctx, sem, win, sigma):
+    xyz, P = scenes.rng_points(80, 96, seed=2)
+    rx, rP = oracle.gaussian(xyz, P, win, sigma, sem)
+    gx, gP = np.empty_like(rx), np.empty_like(rP)
+    _capi.check(ctx.lib.psgc_filter_gaussian(ctx.h, _capi.ptr(xyz), _capi.ptr(P), _capi.ptr(gx),
+                                             _capi.ptr(gP), 80, 96, win, sigma, sem,
+                                             _capi.MEM_HOST))
+    assert np.array_equal(gP, rP)
+    assert np.array_equal(gx, rx)
+
+
+def test_ror(ctx):
+    xyz, P = scenes.rng_points(80, 96, seed=4)
+    for radius, mn, win in ((0.7, 10, 10), (0.5, 5, 5), (2.0, 3, 3)):
+        rx, rP = oracle.ror(xyz, P, radius, mn, win)
+        gx, gP = np.empty_like(rx), np.empty_like(rP)
+        _capi.check(ctx.lib.psgc_filter_ror(ctx.h, _capi.ptr(xyz), _capi.ptr(P), _capi.ptr(gx),
+                                            _capi.ptr(gP), 80, 96, radius, mn, win,
+                                            _capi.MEM_HOST))
+        assert np.array_equal(gP, rP)
+        assert np.array_equal(gx, rx)
+
+
+@pytest.mark.parametrize("k,half", [(30, 4), (8, 2), (1, 1)])
+def test_sor(ctx, k, half):
+    xyz, P = scenes.rng_points(90, 110, seed=6)
+    rx, rP, rmd, rst = oracle.sor(xyz, P, k, 1.0, half)
+    gx, gP = np.empty_like(rx), np.empty_like(rP)
+    gmd = np.empty_like(rmd)
+    gst = np.zeros(4)
+    _capi.check(ctx.lib.psgc_filter_sor(ctx.h, _capi.ptr(xyz), _capi.ptr(P), _capi.ptr(gx),
+                                        _capi.ptr(gP), 90, 110, k, 1.0, half, _capi.ptr(gmd),
+                                        _capi.ptr(gst), _capi.MEM_HOST))
+    assert np.array_equal(gst, rst), (gst, rst)
+    assert np.array_equal(gmd, rmd)
+    assert np.array_equal(gP, rP)
+    assert np.array_equal(gx, rx)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("sem", [0, 1])
+def test_mls(ctx, order, sem):
+    xyz, P = scenes.rng_points(64, 80, seed=8, holes=0.3)
+    ref = oracle.mls(xyz, P, 4, 0.5, order, 1.5, 2, sem)
+    got = np.empty_like(ref)
+    _capi.check(ctx.lib.psgc_filter_mls(ctx.h, _capi.ptr(xyz), _capi.ptr(P), _capi.ptr(got),
+                                        64, 80, 4, 0.5, order, 1.5, 2, sem, _capi.MEM_HOST))
+    err = np.abs(got - ref).max()
+    print("mls max |diff| mm", err)
+    assert err <= XYZ_TOL
+
+
+@pytest.mark.parametrize("variant", ["cpu_ref", "gpu_ref", "c2"])
+def test_reconstruct_chain(ctx, variant):
+    p, L, R = _decoded_pair(11)
+    if variant == "cpu_ref":
+        cfg = zim_amd.ReconConfig.default(radius_search=1.0, radius_min_neighbors=6)
+    elif variant == "gpu_ref":
+        cfg = zim_amd.ReconConfig.default_gpu(radius_search=1.0, mls_window_size=5)
+    else:
+        cfg = scenes.c2_recon_config()
+    rx, rP, rd, rmc = oracle.reconstruct(cfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1)
+    gx, gP, gd, gmc = zim_amd.reconstruct(cfg, dev(L[0]), dev(R[0]), dev(L[1]), dev(R[1]), 70,
+                                          p.Q, p.R1, ctx)
+    torch.cuda.synchronize()
+    gx, gP, gd, gmc = host(gx), host(gP), host(gd), host(gmc)
+    assert np.array_equal(gmc, rmc)
+    assert np.abs(gd - rd).max() <= DISP_TOL
+    assert np.array_equal(gP, rP), f"mask mismatch {(gP != rP).sum()}"
+    assert np.abs(gx - rx).max() <= XYZ_TOL
+    assert rP.mean() > 0.2
+
+
+def test_error_paths(ctx):
+    cfg = scenes.c2_decode_config(freq=(70, 0, 59))
+    _, left, _ = scenes.c2_small(3)
+    with pytest.raises(zim_amd.PsgcError) as e:
+        zim_amd.decode(cfg, left, ctx)
+    assert e.value.code == _capi.PSGC_E_INVALID
+    cfg = scenes.c2_decode_config()
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.decode(cfg, left[:-1], ctx)
+
+
+def test_tiny_and_degenerate_images(ctx):
+    cfg = scenes.c2_decode_config()
+    for rows, cols in ((1, 1), (1, 7), (5, 1), (3, 3)):
+        g = np.random.default_rng(rows * 100 + cols)
+        frames = [g.integers(0, 4096, (rows, cols)).astype(np.uint16) for _ in range(12)]
+        ref = oracle.decode(cfg, frames)
+        got = zim_amd.decode(cfg, frames, ctx)
+        for r_, g_ in zip(ref, got):
+            assert np.array_equal(r_, g_)
+    # all-constant frames: nothing valid, nothing matched
+    flat = [np.full((16, 16), 1000, np.uint16) for _ in range(12)]
+    ph, k, mod, v = zim_amd.decode(cfg, flat, ctx)
+    assert v.sum() == 0 and k.sum() == 0

@@ -1,0 +1,403 @@
+"""Pins the CPU restatement (oracle/) against what the mathematics fixes:
+closed forms, exhaustive sweeps, brute force and library routines. The
+reference ships no golden vectors or tests, so these checks (plus the
+committed fixtures in tests/golden) are what anchor the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import scenes
+from zim_amd._capi import DecodeConfig
+
+TWO_PI = 2 * math.pi
+
+
+def ulp_diff(a, b):
+    ia = a.view(np.int64).astype(np.int64)
+    ib = b.view(np.int64).astype(np.int64)
+    ia = np.where(ia < 0, np.int64(-(2**63)) - ia, ia)
+    ib = np.where(ib < 0, np.int64(-(2**63)) - ib, ib)
+    return np.abs(ia - ib)
+
+
+# ------------------------------------------------------------------ atan2 --
+
+def test_atan2_within_2ulp_of_libm():
+    g = np.random.default_rng(0)
+    n = 400_000
+    y = g.uniform(-5000, 5000, n) * np.where(g.random(n) < 0.2, 1e-9, 1.0)
+    x = g.uniform(-5000, 5000, n) * np.where(g.random(n) < 0.2, 1e-12, 1.0)
+    got = oracle.atan2(y, x)
+    ref = np.arctan2(y, x)
+    assert ulp_diff(got, ref).max() <= 2
+
+
+def test_atan2_special_values():
+    z = [(0.0, 0.0), (0.0, -0.0), (-0.0, 0.0), (-0.0, -0.0), (1.0, 0.0), (-1.0, 0.0),
+         (1.0, -np.inf), (-1.0, np.inf), (np.inf, np.inf), (-np.inf, -np.inf), (0.0, -3.0),
+         (-0.0, -3.0), (3.0, 3.0), (-3.0, -3.0)]
+    y = np.array([a for a, _ in z])
+    x = np.array([b for _, b in z])
+    got = oracle.atan2(y, x)
+    ref = np.arctan2(y, x)
+    assert np.array_equal(np.signbit(got), np.signbit(ref))
+    assert ulp_diff(got, ref).max() <= 1
+    assert np.isnan(oracle.atan2(np.array([np.nan]), np.array([1.0])))[0]
+
+
+def test_atan_table_correctly_rounded():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "..", "zim_amd", "csrc", "atan_table.inc")
+    vals = [float.fromhex(l.strip().rstrip(",")) for l in open(path) if l.strip().startswith("0x")]
+    assert len(vals) == 65
+    ld = np.arctan(np.arange(65, dtype=np.longdouble) / np.longdouble(64))
+    err = np.abs(np.array(vals, dtype=np.longdouble) - ld)
+    half_ulp = np.array([np.spacing(v) / 2 if v else 0 for v in vals], dtype=np.longdouble)
+    assert np.all(err <= half_ulp * 1.001)
+
+
+# ----------------------------------------------------------------- decode --
+
+def _phase_frames(phi, n, A=1150.0, B=750.0):
+    return [A + B * np.cos(phi + TWO_PI * k / n) for k in range(n)]
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 8, 12])
+def test_wrapped_phase_and_modulation_closed_form(n):
+    # constant images: any normalised blur leaves them (almost) unchanged
+    g = np.random.default_rng(n)
+    for phi in g.uniform(-math.pi, math.pi, 20):
+        frames = [np.full((5, 6), v) for v in _phase_frames(phi, n)]
+        # level 0 carries the phase; levels 1, 2 get a copy (valid anyway)
+        cfg = DecodeConfig.default(freq=(70, 64, 59), shift_num=(n, n, n),
+                                   modulation_threshold=0.3, post_smooth=0, avg_three_phase=0)
+        ph, k, mod, valid = oracle.decode(cfg, frames * 3)
+        assert valid.all()
+        assert np.allclose(mod, 750.0 / 1150.0, rtol=1e-12, atol=0)
+        wrapped = ph - TWO_PI * k
+        d = (wrapped - phi + math.pi) % TWO_PI - math.pi
+        assert np.abs(d).max() < 1e-12
+
+
+def _sweep_frames(freqs, steps, proj_w, xs):
+    """One row, each projector sample repeated 3 times so the 3x3 blur at
+    the middle copy sees a constant neighbourhood."""
+    frames = []
+    for f, n in zip(freqs, steps):
+        phi = TWO_PI * f * xs / proj_w
+        for k in range(n):
+            row = np.repeat(1150.0 + 750.0 * np.cos(phi + TWO_PI * k / n), 3)
+            frames.append(np.tile(row, (3, 1)))
+    return frames
+
+
+@pytest.mark.parametrize("freqs", [(70, 64, 59), (121, 110, 120), (107, 100, 94)])
+@pytest.mark.parametrize("avg", [0, 1])
+def test_heterodyne_exhaustive_sweep(freqs, avg):
+    proj_w = 1280.0
+    xs = np.arange(0, 1280, 0.37)
+    # skip samples that sit exactly on a period boundary of level 0
+    frac = (freqs[0] * xs / proj_w) % 1.0
+    xs = xs[(np.abs(frac) > 1e-6) & (np.abs(frac - 1) > 1e-6)]
+    frames = _sweep_frames(freqs, (4, 4, 4), proj_w, xs)
+    cfg = DecodeConfig.default(freq=freqs, shift_num=(4, 4, 4), modulation_threshold=0.3,
+                               post_smooth=0, avg_three_phase=avg)
+    ph, k, mod, valid = oracle.decode(cfg, frames)
+    mid = slice(1, None, 3)
+    ph, k, valid = ph[1, mid], k[1, mid], valid[1, mid]
+    assert valid.all()
+    truth = TWO_PI * freqs[0] * xs / proj_w
+    assert np.abs(ph - truth).max() < 1e-9
+    exp_k = np.floor(freqs[0] * xs / proj_w + 0.5).astype(np.int64)
+    assert np.array_equal(k, exp_k)
+
+
+def test_post_smooth_is_replicated_3x3_binomial():
+    _, left, _ = scenes.c2_small(3)
+    cfg0 = scenes.c2_decode_config(post_smooth=0)
+    cfg1 = scenes.c2_decode_config(post_smooth=1)
+    raw = oracle.decode(cfg0, left)[0]
+    sm = oracle.decode(cfg1, left)[0]
+    p = np.pad(raw, 1, mode="edge")
+    w = np.array([0.25, 0.5, 0.25])
+    ref = sum(w[a] * w[b] * p[a:a + raw.shape[0], b:b + raw.shape[1]]
+              for a in range(3) for b in range(3))
+    assert np.abs(sm - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("sigma", [3.0, 1.0, 0.0])
+def test_preblur_matches_brute_force_convolution(sigma):
+    _, left, _ = scenes.c2_small(7)
+    frames = [f.astype(np.float64) for f in left]
+    cfg = scenes.c2_decode_config(preblur_sigma=sigma, modulation_threshold=-1.0)
+    _, _, mod, _ = oracle.decode(cfg, frames)
+    if sigma > 0:
+        e = math.exp(-1.0 / (2 * sigma * sigma))
+        w = np.array([e, 1.0, e]) / (1 + 2 * e)
+    else:
+        w = np.array([0.25, 0.5, 0.25])
+    blur = []
+    for f in frames[:4]:
+        p = np.pad(f, 1, mode="edge")
+        blur.append(sum(w[a] * w[b] * p[a:a + f.shape[0], b:b + f.shape[1]]
+                        for a in range(3) for b in range(3)))
+    S = -sum(math.sin(TWO_PI * k / 4) * blur[k] for k in range(4))
+    C = sum(math.cos(TWO_PI * k / 4) * blur[k] for k in range(4))
+    G = sum(blur)
+    ref = (2 * np.sqrt(S * S + C * C) / 4) / (G / 4)
+    assert np.abs(mod - ref).max() <= 1e-12
+
+
+def test_gray_code_all_codes_decode():
+    bits = 6
+    period = 16.0
+    xs = np.arange(0, 64 * period, 1.0) + 0.5 * (np.arange(64 * int(period)) % 2 == 0)
+    n = np.floor(xs / period).astype(np.int64)
+    phi = TWO_PI * xs / period
+    frames = [np.tile(np.repeat(100 + 80 * np.cos(phi + TWO_PI * k / 4), 3), (3, 1))
+              for k in range(4)]
+    g = n ^ (n >> 1)
+    pos, neg = [], []
+    for b in range(bits):
+        bit = (g >> (bits - 1 - b)) & 1
+        pos.append(np.tile(np.repeat(20 + 200.0 * bit, 3), (3, 1)))
+        neg.append(np.tile(np.repeat(20 + 200.0 * (1 - bit), 3), (3, 1)))
+    cfg = scenes.c1_gray_config(post_smooth=0)
+    ph, k, _, valid = oracle.decode(cfg, frames + pos + neg)
+    mid = slice(1, None, 3)
+    assert valid[1, mid].all()
+    assert np.array_equal(k[1, mid], n)
+    assert np.abs(ph[1, mid] - phi).max() < 1e-9
+
+
+def test_gray_code_scene_recovers_projector_phase():
+    p, left, right = scenes.c1_small(5)
+    cfg = scenes.c1_gray_config(post_smooth=0)
+    ph, k, _, valid = oracle.decode(cfg, left)
+    cam, proj = p.cam, p.proj
+    jj = np.arange(cam.cols)[None, :]
+    up = proj.f * ((jj - cam.cx) / cam.f - cam.baseline / (2.0 * p.z_left.numpy())) + proj.cp
+    truth = TWO_PI * up / 16.0
+    m = valid > 0
+    assert m.mean() > 0.9
+    err = np.abs(ph[m] - truth[m])
+    assert np.median(err) < 0.05 and (err > 1.0).mean() < 1e-3
+
+
+# ------------------------------------------------------------------ match --
+
+def _brute_match(lp, rp, lk, rk, pha_f, thr):
+    rows, cols = lp.shape
+    disp = np.zeros_like(lp)
+    mc = -np.ones(lp.shape, np.int32)
+    for i in range(rows):
+        for j in range(cols):
+            k = lk[i, j]
+            if k <= 0 or k > pha_f:
+                continue
+            cand = [c for c in range(cols) if rk[i, c] == k]
+            if not cand:
+                continue
+            diffs = [abs(rp[i, c] - lp[i, j]) for c in cand]
+            b = cand[int(np.argmin(diffs))]       # first minimum wins
+            if min(diffs) >= thr or b < 2 or b >= cols - 1:
+                continue
+            c0, c1, c2 = (rp[i, b - 1] - lp[i, j], rp[i, b] - lp[i, j], rp[i, b + 1] - lp[i, j])
+            den = 2 * (c0 - c1) * (c1 - c2)
+            delta = c1 * (c0 - c2) / den if abs(den) > 1e-9 else 0.0
+            disp[i, j] = j - b - delta
+            mc[i, j] = b
+    return disp, mc
+
+
+def test_stereo_match_brute_force():
+    g = np.random.default_rng(2)
+    rows, cols, pha_f = 7, 60, 6
+    rp = np.cumsum(g.uniform(0.2, 0.6, (rows, cols)), axis=1)
+    lp = rp + g.normal(0, 0.5, (rows, cols)) + 3.0
+    rk = np.floor(rp / TWO_PI).astype(np.int32)
+    lk = np.floor(lp / TWO_PI).astype(np.int32)
+    lk[g.random(lk.shape) < 0.1] = 0
+    rk[g.random(rk.shape) < 0.1] = -1
+    ref_d, ref_mc = _brute_match(lp, rp, lk, rk, pha_f, 1.0)
+    d, mc = oracle.stereo_match(lp, rp, lk, rk, pha_f, 1.0)
+    assert np.array_equal(mc, ref_mc)
+    assert np.abs(d - ref_d).max() < 1e-12
+    assert (mc >= 0).sum() > 50
+
+
+def test_stereo_match_linear_ramp_exact_subpixel():
+    cols = 400
+    a = 0.21
+    for d0 in (37.25, 12.6, 80.03):
+        r = a * np.arange(cols, dtype=np.float64)[None, :]
+        l = a * (np.arange(cols) - d0)[None, :]
+        rk = np.floor(r / TWO_PI).astype(np.int32) + 1
+        lk = np.floor(l / TWO_PI).astype(np.int32) + 1
+        d, mc = oracle.stereo_match(l, r, lk, rk, 100, 1.0)
+        m = mc >= 0
+        assert m.sum() > 200
+        assert np.abs(d[m] - d0).max() < 1e-9
+
+
+# -------------------------------------------------------- median / reproject --
+
+def test_median_brute_force():
+    g = np.random.default_rng(3)
+    src = g.normal(0, 10, (23, 31)).astype(np.float32)
+    p = np.pad(src, 2, mode="edge")
+    ref = np.empty_like(src)
+    for i in range(23):
+        for j in range(31):
+            ref[i, j] = np.median(p[i:i + 5, j:j + 5])
+    assert np.array_equal(oracle.median5(src), ref)
+
+
+def test_reproject_closed_form():
+    f, B, cx1, cx2, cy = 2400.0, 150.0, 60.5, 52.0, 40.0
+    Q = [1, 0, 0, -cx1, 0, 1, 0, -cy, 0, 0, 0, f, 0, 0, 1 / B, -(cx1 - cx2) / B]
+    g = np.random.default_rng(4)
+    disp = g.uniform(400, 720, (50, 70))
+    xyz, P = oracle.reproject(disp, Q, np.eye(3).ravel(), 0, 1000, 1)
+    Z = f * B / (disp - (cx1 - cx2))
+    jj, ii = np.meshgrid(np.arange(70), np.arange(50))
+    X = (jj - cx1) * Z / f
+    Y = (ii - cy) * Z / f
+    assert P.all()
+    assert np.abs(xyz[..., 2] - Z).max() < 1e-4
+    assert np.abs(xyz[..., 0] - X).max() < 1e-4
+    assert np.abs(xyz[..., 1] - Y).max() < 1e-4
+    # rotated output: R1^T applied
+    th = 0.1
+    R1 = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    xr, Pr = oracle.reproject(disp, Q, R1.ravel(), 0, 1000, 0)
+    pts = np.stack([X, Y, Z], -1) @ R1  # row vector times R1 == R1^T p
+    assert np.abs(xr - pts).max() < 1e-3
+
+
+# ----------------------------------------------------------------- filters --
+
+def test_gaussian_weighted_mean_and_masks():
+    xyz, P = scenes.rng_points(40, 50, seed=1)
+    for win, sigma in ((3, 0.7), (5, 1.0)):
+        e = np.exp(-((np.arange(win) - (win - 1) / 2) ** 2) / (2 * sigma * sigma))
+        w1 = (e / e.sum()).astype(np.float32)
+        w2 = np.outer(w1, w1).astype(np.float64)
+        h = (win - 1) // 2
+        for sem in (0, 1):
+            out, oP = oracle.gaussian(xyz, P, win, sigma, sem)
+            for i in range(h, 40 - h):
+                for j in range(h, 50 - h):
+                    if not P[i, j]:
+                        continue
+                    nb = P[i - h:i + h + 1, j - h:j + h + 1].astype(bool)
+                    nx = xyz[i - h:i + h + 1, j - h:j + h + 1]
+                    use = nb & (np.abs(nx[..., 0] - xyz[i, j, 0]) < 3.0)
+                    wsum = w2[use].sum()
+                    if wsum > 0.8 + 1e-6:
+                        mean = (nx[use].astype(np.float64) * w2[use][:, None]).sum(0) / wsum
+                        assert np.abs(out[i, j] - mean).max() < 1e-3
+                        assert oP[i, j] == 1
+                    elif wsum < 0.8 - 1e-6:
+                        assert not out[i, j].any()
+                        assert oP[i, j] == (1 if sem == 0 else 0)
+            # CPU semantics keeps the input mask; borders stay zero
+            if sem == 0:
+                assert np.array_equal(oP, P)
+            assert not out[:h].any() and not out[:, :h].any()
+
+
+def test_ror_brute_force():
+    xyz, P = scenes.rng_points(30, 40, seed=5)
+    for radius, mn, win in ((0.7, 10, 10), (0.4, 4, 5)):
+        out, oP = oracle.ror(xyz, P, radius, mn, win)
+        h = win // 2
+        x64 = xyz.astype(np.float64)
+        for i in range(30):
+            for j in range(40):
+                if not P[i, j]:
+                    assert oP[i, j] == 0
+                    continue
+                r0, r1, c0, c1 = max(0, i - h), min(30, i + h + 1), max(0, j - h), min(40, j + h + 1)
+                nb = P[r0:r1, c0:c1].astype(bool).copy()
+                nb[i - r0, j - c0] = False
+                d2 = ((x64[r0:r1, c0:c1] - x64[i, j]) ** 2).sum(-1)
+                near = (d2 <= radius * radius) & nb
+                if abs(np.sqrt(d2[nb]).min(initial=1e9) - radius) < 1e-4:
+                    continue
+                assert oP[i, j] == (1 if near.sum() >= mn else 0)
+
+
+def test_sor_matches_global_knn_when_window_covers_image():
+    from scipy.spatial import cKDTree
+    xyz, P = scenes.rng_points(24, 30, seed=9, holes=0.2)
+    k = 8
+    out, oP, md, st = oracle.sor(xyz, P, k, 1.0, 40)
+    idx = np.argwhere(P > 0)
+    pts = xyz[P > 0].astype(np.float64)
+    dist, _ = cKDTree(pts).query(pts, k + 1)
+    mean_d = dist[:, 1:].mean(1)
+    assert np.abs(md[P > 0] - mean_d).max() < 1e-5
+    thr = mean_d.mean() + 1.0 * mean_d.std(ddof=1)
+    assert abs(st[3] - thr) < 1e-6
+    keep = mean_d <= thr
+    clear = np.abs(mean_d - thr) > 1e-6
+    assert np.array_equal(oP[idx[:, 0], idx[:, 1]][clear], keep[clear].astype(np.uint8))
+    assert oP[P == 0].sum() == 0
+
+
+def _surface_cloud(fn, rows=40, cols=48, pitch=0.3):
+    g = np.random.default_rng(11)
+    u, v = np.meshgrid(np.arange(cols) * pitch, np.arange(rows) * pitch)
+    u = u + g.uniform(-0.05, 0.05, u.shape)
+    v = v + g.uniform(-0.05, 0.05, v.shape)
+    w = fn(u - u.mean(), v - v.mean())
+    # tilt the local frame so the surface is not axis-aligned
+    th = 0.4
+    x = u
+    y = v * math.cos(th) - w * math.sin(th)
+    z = 700 + v * math.sin(th) + w * math.cos(th)
+    return np.stack([x, y, z], -1).astype(np.float32), np.ones((rows, cols), np.uint8)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_mls_plane_is_fixed_point(order):
+    xyz, P = _surface_cloud(lambda a, b: 0.3 * a - 0.2 * b)
+    out = oracle.mls(xyz, P, 4, 0.5, order, 1.5, 1, 0)
+    assert np.abs(out - xyz).max() < 2e-3
+
+
+def test_mls_quadric_reproduced_by_order2():
+    xyz, P = _surface_cloud(lambda a, b: 0.01 * a * a - 0.008 * a * b + 0.005 * b * b)
+    out = oracle.mls(xyz, P, 5, 0.6, 2, 1.8, 1, 1)
+    inner = (slice(5, -5), slice(5, -5))
+    assert np.abs(out[inner] - xyz[inner]).max() < 2e-3
+    # order 0 (plane) visibly flattens the same surface
+    flat = oracle.mls(xyz, P, 5, 0.6, 0, 1.8, 1, 1)
+    assert np.abs(flat[inner] - xyz[inner]).max() > np.abs(out[inner] - xyz[inner]).max()
+
+
+def test_mls_fallback_rules_differ_between_semantics():
+    # sparse mask: 5..7 neighbours -> CPU rule drops to order 0, GPU to 1
+    xyz, P = _surface_cloud(lambda a, b: 0.01 * a * a)
+    g = np.random.default_rng(0)
+    P = (g.random(P.shape) < 0.25).astype(np.uint8)
+    a = oracle.mls(xyz, P, 2, 0.6, 2, 1.0, 1, 0)
+    b = oracle.mls(xyz, P, 2, 0.6, 2, 1.0, 1, 1)
+    assert not np.array_equal(a, b)
+
+
+def test_reconstruct_recovers_rendered_depth():
+    p, left, right = scenes.c2_small(3)
+    cfg = scenes.c2_decode_config()
+    L = oracle.decode(cfg, left)
+    R = oracle.decode(cfg, right)
+    rc = scenes.c2_recon_config(use_mls=0)
+    xyz, P, disp, mc = oracle.reconstruct(rc, L[0], R[0], L[1], R[1], 70, p.Q, p.R1)
+    z = p.z_left.numpy()
+    m = P > 0
+    assert m.mean() > 0.3
+    err = np.abs(xyz[..., 2][m] - z[m])
+    assert np.median(err) < 0.2 and np.percentile(err, 99) < 1.5

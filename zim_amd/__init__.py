@@ -1,0 +1,239 @@
+"""zim_amd — MI355X-native reconstruction path of PhaseShiftGrayCode3D.
+
+Python host side over the C ABI of libpsgc.so (include/psgc.h). It mirrors
+the reference's operator interface for this path:
+
+  PhaseShiftDecoder.create(config, Version.V2)      JSLAB_PhaseShiftDecoder.h:10-92
+    .setImages(frames) / .startCalculate() / .getOrderMap() / .getModulationMap()
+  PhaseEpipolarReconstructor_V2(config)             JSLAB_PhaseEpipolarReconstructor_V2.h:11-171
+    .setCalibrationParams(...) / .setData(...) / .startCalculate() / .getDisparityMap()
+  applyMLSFilterToPointCloud(...)                   JSLAB_MLS_Filter.h:53-57
+
+Device tensors (torch, on the GPU) are passed to the library as HBM
+pointers and the work is queued on torch's current stream; numpy arrays take
+the host-staging path of the C ABI. There is no CPU fallback: without the
+built library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import (DecodeConfig, ReconConfig, PsgcError, MEM_DEVICE, MEM_HOST,  # noqa: F401
+                    SEM_CPU, SEM_GPU, OUTLIER_NONE, OUTLIER_ROR, OUTLIER_SOR,
+                    UNWRAP_HETERODYNE3, UNWRAP_GRAYCODE)
+
+__all__ = ["Context", "DecodeConfig", "ReconConfig", "PhaseShiftDecoder", "Version",
+           "PhaseEpipolarReconstructor_V2", "applyMLSFilterToPointCloud", "decode",
+           "reconstruct", "PsgcError"]
+
+_SAMPLE = {torch.uint8: _capi.SAMPLE_U8, torch.uint16: _capi.SAMPLE_U16,
+           torch.float64: _capi.SAMPLE_F64, np.dtype(np.uint8): _capi.SAMPLE_U8,
+           np.dtype(np.uint16): _capi.SAMPLE_U16, np.dtype(np.float64): _capi.SAMPLE_F64}
+
+
+class Context:
+    """Owns a psgc_ctx on one device; its work runs on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _capi.load()
+        self.device = device
+        h = C.c_void_p()
+        _capi.check(self.lib.psgc_create(C.byref(h), device))
+        self.h = h
+        self._bound_stream = None
+
+    def bind_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        if self._bound_stream != s.cuda_stream:
+            _capi.check(self.lib.psgc_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+            self._bound_stream = s.cuda_stream
+        return self
+
+    def release(self):
+        _capi.check(self.lib.psgc_release(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.psgc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device].bind_stream()
+
+
+def _is_dev(x):
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+def _out(like_dev, shape, dtype, device):
+    if like_dev:
+        return torch.empty(shape, dtype=dtype, device=device)
+    return np.empty(shape, dtype=torch.empty(0, dtype=dtype).numpy().dtype)
+
+
+def decode(cfg: DecodeConfig, frames, ctx: Context | None = None, want_mod=True,
+           want_valid=True):
+    """Decodes one camera's frame stack. frames: list of 2-D arrays/tensors of
+    one sample type (u8, u16 or f64). Returns (phase f64, k i32, mod f64,
+    valid u8)."""
+    ctx = ctx or default_context()
+    dev = _is_dev(frames[0])
+    if dev:
+        frames = [f if f.stride(-1) == 1 else f.contiguous() for f in frames]
+        st = _SAMPLE[frames[0].dtype]
+        stride = frames[0].stride(0) * frames[0].element_size()
+        ptrs = [f.data_ptr() for f in frames]
+        device = frames[0].device
+    else:
+        frames = [np.ascontiguousarray(f) for f in frames]
+        st = _SAMPLE[frames[0].dtype]
+        stride = 0
+        ptrs = [f.ctypes.data for f in frames]
+        device = None
+    rows, cols = frames[0].shape
+    arr = (C.c_void_p * len(ptrs))(*ptrs)
+    ph = _out(dev, (rows, cols), torch.float64, device)
+    k = _out(dev, (rows, cols), torch.int32, device)
+    md = _out(dev, (rows, cols), torch.float64, device) if want_mod else None
+    vd = _out(dev, (rows, cols), torch.uint8, device) if want_valid else None
+    _capi.check(ctx.lib.psgc_decode(ctx.h, C.byref(cfg), arr, len(ptrs), st, rows, cols, stride,
+                                    _capi.ptr(ph), _capi.ptr(k), _capi.ptr(md), _capi.ptr(vd),
+                                    MEM_DEVICE if dev else MEM_HOST))
+    return ph, k, md, vd
+
+
+def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1, ctx: Context | None = None):
+    """Match -> median -> reproject -> Gaussian -> outlier -> MLS on rectified
+    maps. Returns (xyz f32 HxWx3, P u8, disp f32, match_col i32)."""
+    ctx = ctx or default_context()
+    dev = _is_dev(lp)
+    rows, cols = lp.shape
+    device = lp.device if dev else None
+    xyz = _out(dev, (rows, cols, 3), torch.float32, device)
+    P = _out(dev, (rows, cols), torch.uint8, device)
+    disp = _out(dev, (rows, cols), torch.float32, device)
+    mc = _out(dev, (rows, cols), torch.int32, device)
+    Qa = np.ascontiguousarray(Q, np.float64).reshape(16)
+    Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
+    if not dev:
+        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
+        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
+    _capi.check(ctx.lib.psgc_reconstruct(ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp),
+                                         _capi.ptr(lk), _capi.ptr(rk), rows, cols, int(pha_f),
+                                         _capi.ptr(Qa), _capi.ptr(Ra), _capi.ptr(xyz),
+                                         _capi.ptr(P), _capi.ptr(disp), _capi.ptr(mc),
+                                         MEM_DEVICE if dev else MEM_HOST))
+    return xyz, P, disp, mc
+
+
+# ------------------------------------------------ reference-shaped classes --
+
+class Version(enum.Enum):
+    V1 = 0
+    V2 = 1
+
+
+class PhaseShiftDecoder:
+    """JSLAB_PhaseShiftDecoder (JSLAB_PhaseShiftDecoder.h:10-92) on the GPU."""
+
+    Version = Version
+
+    def __init__(self, config: DecodeConfig, ctx: Context | None = None):
+        self.config = config
+        self.ctx = ctx
+        self._images = None
+        self._k = self._mod = self._valid = None
+
+    @staticmethod
+    def create(config: DecodeConfig, version: Version = Version.V1):
+        """Returns None when any frequency is zero, as the reference factory
+        does (JSLAB_PhaseShiftDecoder.cpp:9-29). V1 has no order map and is
+        superseded; both versions map to the V2 decoder here."""
+        if config.unwrap_mode == UNWRAP_HETERODYNE3 and any(f == 0 for f in config.freq):
+            return None
+        return PhaseShiftDecoder(config)
+
+    def setImages(self, images):
+        n = sum(self.config.shift_num[:3]) if self.config.unwrap_mode == UNWRAP_HETERODYNE3 \
+            else self.config.shift_num[0] + 2 * self.config.gray_bits
+        if len(images) != n:
+            raise ValueError(f"expected {n} images, got {len(images)}")
+        self._images = list(images)
+
+    def startCalculate(self):
+        ph, self._k, self._mod, self._valid = decode(self.config, self._images, self.ctx)
+        return ph
+
+    def getOrderMap(self):
+        return self._k
+
+    def getModulationMap(self):
+        return self._mod
+
+    def getValidMask(self):
+        return self._valid
+
+
+class PhaseEpipolarReconstructor_V2:
+    """JSLAB_PhaseEpipolarReconstructor_V2 (JSLAB_PhaseEpipolarReconstructor_V2.h:
+    11-171) on the GPU, for rectified inputs (the calibration is given as Q
+    and R1; remap is the identity for a pre-rectified pair)."""
+
+    def __init__(self, config: ReconConfig, ctx: Context | None = None):
+        self.config = config
+        self.ctx = ctx
+        self.Q = self.R1 = None
+        self._disp = None
+
+    def setCalibrationParams(self, Q, R1):
+        self.Q = np.asarray(Q, np.float64).reshape(16)
+        self.R1 = np.asarray(R1, np.float64).reshape(9)
+
+    def setData(self, left, right, left_k, right_k, left_mod, pha_f):
+        self._in = (left, right, left_k, right_k, int(pha_f))
+
+    def startCalculate(self):
+        lp, rp, lk, rk, pha_f = self._in
+        xyz, P, self._disp, self._match = reconstruct(self.config, lp, rp, lk, rk, pha_f, self.Q,
+                                                      self.R1, self.ctx)
+        return xyz, P
+
+    def getDisparityMap(self):
+        return self._disp
+
+
+def applyMLSFilterToPointCloud(src_xyz, P, window_size, spatial_sigma, use_gpu=True,
+                               polynomial_order=0, max_radius=0.0, iterations=1,
+                               semantics=SEM_GPU, ctx: Context | None = None):
+    """applyMLSFilterToPointCloud (JSLAB_MLS_Filter.h:53-57); the GPU path
+    always runs (there is no CPU implementation in this package)."""
+    ctx = ctx or default_context()
+    dev = _is_dev(src_xyz)
+    rows, cols = P.shape
+    dst = _out(dev, (rows, cols, 3), torch.float32, src_xyz.device if dev else None)
+    if not dev:
+        src_xyz = np.ascontiguousarray(src_xyz, np.float32)
+        P = np.ascontiguousarray(P, np.uint8)
+    _capi.check(ctx.lib.psgc_filter_mls(ctx.h, _capi.ptr(src_xyz), _capi.ptr(P), _capi.ptr(dst),
+                                        rows, cols, int(window_size), float(spatial_sigma),
+                                        int(polynomial_order), float(max_radius),
+                                        int(iterations), int(semantics),
+                                        MEM_DEVICE if dev else MEM_HOST))
+    return dst

@@ -1,0 +1,837 @@
+// capi.hip — the extern "C" entry points of libpsgc.so (include/psgc.h):
+// argument validation, context-owned buffers, host staging and the
+// orchestration of the per-stage kernels.
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "psgc_internal.h"
+
+namespace psgc {
+
+bool g_prof_on = false;
+
+namespace {
+thread_local char g_err[1024] = "";
+
+struct ProfRec {
+  std::string name;
+  hipEvent_t beg, end;
+};
+std::vector<ProfRec> g_prof;           // closed and open intervals
+std::vector<size_t> g_prof_open;       // indices of open intervals
+
+constexpr double PI = 3.14159265358979323846;
+}  // namespace
+
+void prof_begin(const char* name, hipStream_t s) {
+  ProfRec r;
+  r.name = name;
+  if (hipEventCreate(&r.beg) != hipSuccess || hipEventCreate(&r.end) != hipSuccess) return;
+  hipEventRecord(r.beg, s);
+  g_prof.push_back(r);
+  g_prof_open.push_back(g_prof.size() - 1);
+}
+
+void prof_end(const char* name, hipStream_t s) {
+  for (size_t i = g_prof_open.size(); i-- > 0;) {
+    ProfRec& r = g_prof[g_prof_open[i]];
+    if (r.name == name) {
+      hipEventRecord(r.end, s);
+      g_prof_open.erase(g_prof_open.begin() + i);
+      return;
+    }
+  }
+}
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int ctx_buf(psgc_ctx* ctx, int slot, size_t bytes, void** out) {
+  if ((int)ctx->bufs.size() < S_NUM_SLOTS) ctx->bufs.resize(S_NUM_SLOTS);
+  DevBuf& b = ctx->bufs[slot];
+  if (bytes == 0) bytes = 1;
+  if (b.ptr == nullptr || b.bytes < bytes) {
+    if (b.ptr) {
+      PSGC_HIP(hipStreamSynchronize(ctx->stream));
+      PSGC_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    const size_t want = bytes + bytes / 8;  // headroom for slightly larger calls
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+      b.ptr = nullptr;
+      (void)hipGetLastError();
+      PSGC_FAIL(PSGC_E_ALLOC, "hipMalloc of %zu bytes failed", want);
+    }
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return PSGC_OK;
+}
+
+int ctx_release(psgc_ctx* ctx) {
+  if (ctx->stream) PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& b : ctx->bufs)
+    if (b.ptr) {
+      PSGC_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+  return PSGC_OK;
+}
+
+// getGaussianKernel(win, sigma) in f64 -> f32, outer product in f32.
+void gauss_kernel_f32(int win, float sigma, float* k2) {
+  std::vector<double> k(win);
+  double sg = sigma;
+  if (win == 3 && sg <= 0) {
+    k = {0.25, 0.5, 0.25};
+  } else {
+    if (sg <= 0) sg = ((win - 1) * 0.5 - 1) * 0.3 + 0.8;
+    const double scale2 = -0.5 / (sg * sg);
+    double sum = 0;
+    for (int i = 0; i < win; ++i) {
+      const double x = i - (win - 1) * 0.5;
+      k[i] = std::exp(scale2 * x * x);
+      sum += k[i];
+    }
+    sum = 1.0 / sum;
+    for (int i = 0; i < win; ++i) k[i] *= sum;
+  }
+  std::vector<float> kf(win);
+  for (int i = 0; i < win; ++i) kf[i] = (float)k[i];
+  for (int i = 0; i < win; ++i)
+    for (int j = 0; j < win; ++j) k2[i * win + j] = kf[i] * kf[j];
+}
+
+// getGaussianKernel(3, sigma) in f64: (centre, side).
+void blur3_weights(double sigma, double* c, double* s) {
+  if (sigma <= 0) {
+    *c = 0.5;
+    *s = 0.25;
+    return;
+  }
+  const double scale2 = -0.5 / (sigma * sigma);
+  double k[3], sum = 0;
+  for (int i = 0; i < 3; ++i) {
+    const double x = i - (3 - 1) * 0.5;
+    k[i] = std::exp(scale2 * x * x);
+    sum += k[i];
+  }
+  sum = 1.0 / sum;
+  for (int i = 0; i < 3; ++i) k[i] *= sum;
+  *c = k[1];
+  *s = k[0];
+}
+
+namespace {
+
+int sample_bytes(int t) {
+  return t == PSGC_SAMPLE_U8 ? 1 : (t == PSGC_SAMPLE_U16 ? 2 : 8);
+}
+
+struct DeviceGuard {
+  int rc = PSGC_OK;
+  explicit DeviceGuard(psgc_ctx* ctx) {
+    if (hipSetDevice(ctx->device) != hipSuccess) rc = PSGC_E_HIP;
+  }
+};
+
+#define PSGC_CTX_ENTER(ctx)                                              \
+  do {                                                                   \
+    if (!(ctx)) PSGC_FAIL(PSGC_E_INVALID, "null context");               \
+    if (hipSetDevice((ctx)->device) != hipSuccess)                       \
+      PSGC_FAIL(PSGC_E_HIP, "hipSetDevice(%d) failed", (ctx)->device);    \
+  } while (0)
+
+// Device pointer for a caller buffer: in host mode, a staging slot (filled
+// from the host when `upload`); in device mode, the pointer itself.
+template <typename T>
+int stage_in(psgc_ctx* ctx, int mem, int slot, const T* p, size_t n, const T** out) {
+  if (mem == PSGC_MEM_DEVICE) {
+    *out = p;
+    return PSGC_OK;
+  }
+  void* d;
+  PSGC_TRY(ctx_buf(ctx, slot, n * sizeof(T), &d));
+  PSGC_HIP(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *out = (const T*)d;
+  return PSGC_OK;
+}
+
+template <typename T>
+int stage_out(psgc_ctx* ctx, int mem, int slot, T* p, size_t n, T** out) {
+  if (p == nullptr) {
+    *out = nullptr;
+    return PSGC_OK;
+  }
+  if (mem == PSGC_MEM_DEVICE) {
+    *out = p;
+    return PSGC_OK;
+  }
+  void* d;
+  PSGC_TRY(ctx_buf(ctx, slot, n * sizeof(T), &d));
+  *out = (T*)d;
+  return PSGC_OK;
+}
+
+template <typename T>
+int copy_back(psgc_ctx* ctx, int mem, T* host, const T* dev, size_t n) {
+  if (mem == PSGC_MEM_HOST && host)
+    PSGC_HIP(hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  return PSGC_OK;
+}
+
+int finish(psgc_ctx* ctx, int mem) {
+  if (mem == PSGC_MEM_HOST) PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  return PSGC_OK;
+}
+
+int check_mem(int mem) {
+  if (mem != PSGC_MEM_DEVICE && mem != PSGC_MEM_HOST)
+    PSGC_FAIL(PSGC_E_INVALID, "mem must be PSGC_MEM_DEVICE or PSGC_MEM_HOST");
+  return PSGC_OK;
+}
+
+int check_size(int rows, int cols) {
+  if (rows < 1 || cols < 1 || (int64_t)rows * cols > (int64_t)1 << 31)
+    PSGC_FAIL(PSGC_E_INVALID, "bad image size %d x %d", rows, cols);
+  return PSGC_OK;
+}
+
+// Decode configuration -> kernel arguments (frames filled by the caller).
+int make_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
+                     int cols, int64_t stride, DecodeArgs* a) {
+  std::memset(a, 0, sizeof(*a));
+  if (!cfg) PSGC_FAIL(PSGC_E_INVALID, "null decode config");
+  if (sample_type < PSGC_SAMPLE_U8 || sample_type > PSGC_SAMPLE_F64)
+    PSGC_FAIL(PSGC_E_INVALID, "bad sample type %d", sample_type);
+  if (cfg->enable_hdr)
+    PSGC_FAIL(PSGC_E_UNSUPPORTED, "enable_hdr is decoded on the CPU only");
+  const bool gray = cfg->unwrap_mode == PSGC_UNWRAP_GRAYCODE;
+  if (!gray && cfg->unwrap_mode != PSGC_UNWRAP_HETERODYNE3)
+    PSGC_FAIL(PSGC_E_INVALID, "bad unwrap_mode %d", cfg->unwrap_mode);
+  a->nlev = gray ? 1 : 3;
+  int total = 0;
+  for (int i = 0; i < a->nlev; ++i) {
+    const int n = cfg->shift_num[i];
+    if (n < 3 || n > 16) PSGC_FAIL(PSGC_E_INVALID, "shift_num[%d]=%d outside [3,16]", i, n);
+    a->shift_num[i] = n;
+    a->level_base[i] = total;
+    for (int k = 0; k < n; ++k) {
+      a->sin_tab[total + k] = std::sin(2.0 * PI * k / n);
+      a->cos_tab[total + k] = std::cos(2.0 * PI * k / n);
+    }
+    total += n;
+  }
+  // JSLAB_PhaseShiftDecoder::create refuses a zero frequency
+  // (JSLAB_PhaseShiftDecoder.cpp:11-28).
+  if (!gray) {
+    for (int i = 0; i < 3; ++i)
+      if (cfg->freq[i] == 0) PSGC_FAIL(PSGC_E_INVALID, "freq[%d] == 0", i);
+    a->f12 = std::abs(static_cast<int>(cfg->freq[0] - cfg->freq[1]));
+    a->f23 = std::abs(static_cast<int>(cfg->freq[1] - cfg->freq[2]));
+    if (a->f12 == 0 || a->f23 == 0)
+      PSGC_FAIL(PSGC_E_INVALID, "beat frequencies f12=%d f23=%d must be non-zero", a->f12,
+                a->f23);
+  } else {
+    if (cfg->gray_bits < 1 || cfg->gray_bits > 24)
+      PSGC_FAIL(PSGC_E_INVALID, "gray_bits=%d outside [1,24]", cfg->gray_bits);
+    a->gray_bits = cfg->gray_bits;
+    a->gray_min_contrast = cfg->gray_min_contrast;
+    total += 2 * cfg->gray_bits;
+  }
+  if (n_frames != total)
+    PSGC_FAIL(PSGC_E_INVALID, "expected %d frames, got %d", total, n_frames);
+  if (n_frames > 64) PSGC_FAIL(PSGC_E_INVALID, "at most 64 frames");
+  for (int i = 0; i < 3; ++i) a->freq[i] = cfg->freq[i];
+  a->n_frames = n_frames;
+  a->sample_type = sample_type;
+  a->rows = rows;
+  a->cols = cols;
+  a->stride = stride;
+  a->mod_thresh = cfg->modulation_threshold;
+  a->avg_three = cfg->avg_three_phase ? 1 : 0;
+  blur3_weights(cfg->preblur_sigma, &a->blur_c, &a->blur_s);
+  return PSGC_OK;
+}
+
+}  // namespace
+}  // namespace psgc
+
+using namespace psgc;
+
+extern "C" {
+
+const char* psgc_last_error(void) { return g_err; }
+int psgc_abi_version(void) { return PSGC_ABI_VERSION; }
+
+int psgc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+void psgc_decode_config_default(psgc_decode_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->freq[0] = 121.0;  // driver values, main.cpp:422-423
+  c->freq[1] = 110.0;
+  c->freq[2] = 120.0;
+  c->shift_num[0] = c->shift_num[1] = c->shift_num[2] = 8;
+  c->modulation_threshold = 1.0;
+  c->preblur_sigma = 3.0;
+  c->avg_three_phase = 1;
+  c->post_smooth = 1;
+  c->enable_hdr = 0;
+  c->hdr_saturation_gray = 255.0;
+  c->hdr_min_amplitude = 2.0;
+  c->unwrap_mode = PSGC_UNWRAP_HETERODYNE3;
+}
+
+void psgc_recon_config_default(psgc_recon_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->min_depth = 0;
+  c->max_depth = 1000;
+  c->min_phase_diff = 0.03;
+  c->max_phase_diff = 0.4;
+  c->grad_threshold = 1;
+  c->use_rectified_coords = 1;
+  c->use_disp_median_filter = 1;
+  c->use_full_left_rectification = 1;
+  c->radius_search = 0.7f;
+  c->radius_min_neighbors = 10;
+  c->radius_window_size = 10;
+  c->use_mls = 0;
+  c->mls_window_size = 30;
+  c->mls_spatial_sigma = 0.6f;
+  c->use_gpu_mls = 1;
+  c->mls_polynomial_order = 1;
+  c->mls_max_radius = 0.0f;
+  c->mls_iterations = 3;
+  c->semantics = PSGC_SEM_CPU;
+  c->gauss_window = 3;
+  c->gauss_sigma = 0.7f;
+  c->outlier_mode = PSGC_OUTLIER_ROR;
+  c->sor_k = 30;
+  c->sor_std_mul = 1.0f;
+  c->sor_window = 4;
+}
+
+void psgc_recon_config_default_gpu(psgc_recon_config* c) {
+  if (!c) return;
+  psgc_recon_config_default(c);
+  c->radius_search = 0.05f;
+  c->radius_min_neighbors = 5;
+  c->radius_window_size = 5;
+  c->use_mls = 1;
+  c->mls_window_size = 25;
+  c->mls_spatial_sigma = 0.5f;
+  c->mls_iterations = 2;
+  c->semantics = PSGC_SEM_GPU;
+}
+
+int psgc_warmup(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    PSGC_FAIL(PSGC_E_NODEVICE, "no HIP device");
+  }
+  PSGC_HIP(hipFree(nullptr));
+  return PSGC_OK;
+}
+
+int psgc_create(psgc_ctx** out, int device) {
+  if (!out) PSGC_FAIL(PSGC_E_INVALID, "null output pointer");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    PSGC_FAIL(PSGC_E_NODEVICE, "no HIP device");
+  }
+  if (device < 0 || device >= n) PSGC_FAIL(PSGC_E_NODEVICE, "device %d of %d", device, n);
+  PSGC_HIP(hipSetDevice(device));
+  psgc_ctx* ctx = new psgc_ctx();
+  ctx->device = device;
+  if (hipStreamCreate(&ctx->own_stream) != hipSuccess) {
+    delete ctx;
+    PSGC_FAIL(PSGC_E_HIP, "hipStreamCreate failed");
+  }
+  ctx->stream = ctx->own_stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+  ctx->bufs.resize(S_NUM_SLOTS);
+  *out = ctx;
+  return PSGC_OK;
+}
+
+int psgc_destroy(psgc_ctx* ctx) {
+  if (!ctx) return PSGC_OK;
+  hipSetDevice(ctx->device);
+  int rc = ctx_release(ctx);
+  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return rc;
+}
+
+int psgc_release(psgc_ctx* ctx) {
+  PSGC_CTX_ENTER(ctx);
+  return ctx_release(ctx);
+}
+
+int psgc_set_stream(psgc_ctx* ctx, void* s) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return PSGC_OK;
+}
+
+int psgc_synchronize(psgc_ctx* ctx) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  return PSGC_OK;
+}
+
+int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const* frames,
+                int n_frames, int sample_type, int rows, int cols, int64_t row_stride_bytes,
+                double* phase, int32_t* k, double* mod, uint8_t* valid, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!frames || !phase || !k) PSGC_FAIL(PSGC_E_INVALID, "null frames/phase/k");
+  DecodeArgs a;
+  const int sb = sample_bytes(sample_type);
+  if (row_stride_bytes == 0) row_stride_bytes = (int64_t)cols * sb;
+  PSGC_TRY(make_decode_args(cfg, n_frames, sample_type, rows, cols, row_stride_bytes, &a));
+  if (row_stride_bytes < (int64_t)cols * sb)
+    PSGC_FAIL(PSGC_E_INVALID, "row stride %lld below %d", (long long)row_stride_bytes, cols * sb);
+  for (int f = 0; f < n_frames; ++f)
+    if (!frames[f]) PSGC_FAIL(PSGC_E_INVALID, "frame %d is null", f);
+  const size_t npix = (size_t)rows * cols;
+  if (mem == PSGC_MEM_HOST) {
+    const size_t plane = (size_t)cols * sb * rows;
+    void* d;
+    PSGC_TRY(ctx_buf(ctx, S_FRAMES, plane * n_frames, &d));
+    for (int f = 0; f < n_frames; ++f) {
+      PSGC_HIP(hipMemcpy2DAsync((char*)d + plane * f, (size_t)cols * sb, frames[f],
+                                (size_t)row_stride_bytes, (size_t)cols * sb, rows,
+                                hipMemcpyHostToDevice, ctx->stream));
+      a.frames[f] = (char*)d + plane * f;
+    }
+    a.stride = (int64_t)cols * sb;
+  } else {
+    for (int f = 0; f < n_frames; ++f) a.frames[f] = frames[f];
+  }
+  double *d_phase, *d_mod, *d_raw;
+  int32_t* d_k;
+  uint8_t* d_valid;
+  PSGC_TRY(stage_out(ctx, mem, S_PHASE, phase, npix, &d_phase));
+  PSGC_TRY(stage_out(ctx, mem, S_K, k, npix, &d_k));
+  PSGC_TRY(stage_out(ctx, mem, S_MOD, mod, npix, &d_mod));
+  const bool gray = a.nlev == 1;
+  if (valid) {
+    PSGC_TRY(stage_out(ctx, mem, S_VALID, valid, npix, &d_valid));
+  } else if (gray) {
+    void* v;
+    PSGC_TRY(ctx_buf(ctx, S_VALID, npix, &v));
+    d_valid = (uint8_t*)v;
+  } else {
+    d_valid = nullptr;
+  }
+  const bool smooth = cfg->post_smooth != 0;
+  if (smooth || gray) {
+    void* v;
+    PSGC_TRY(ctx_buf(ctx, S_PHASE_RAW, npix * sizeof(double), &v));
+    d_raw = (double*)v;
+  } else {
+    d_raw = d_phase;
+  }
+  PSGC_TRY(launch_decode_plain(a, d_raw, d_k, d_mod, d_valid, ctx->stream));
+  if (gray) {
+    // jump correction reads (raw, k) and writes (phase, k)
+    void* kt;
+    PSGC_TRY(ctx_buf(ctx, S_TMP0, npix * sizeof(int32_t), &kt));
+    PSGC_HIP(hipMemcpyAsync(kt, d_k, npix * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    PSGC_TRY(launch_gray_fix(a, d_raw, d_phase, (const int32_t*)kt, d_k, d_valid, ctx->stream));
+    if (smooth) {
+      PSGC_HIP(hipMemcpyAsync(d_raw, d_phase, npix * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+      PSGC_TRY(launch_post_smooth(d_raw, d_phase, rows, cols, ctx->stream));
+    }
+  } else if (smooth) {
+    PSGC_TRY(launch_post_smooth(d_raw, d_phase, rows, cols, ctx->stream));
+  }
+  PSGC_TRY(copy_back(ctx, mem, phase, d_phase, npix));
+  PSGC_TRY(copy_back(ctx, mem, k, d_k, npix));
+  PSGC_TRY(copy_back(ctx, mem, mod, d_mod, npix));
+  PSGC_TRY(copy_back(ctx, mem, valid, d_valid, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_stereo_match(psgc_ctx* ctx, const double* l_phase, const double* r_phase,
+                      const int32_t* l_k, const int32_t* r_k, int rows, int cols, int pha_f,
+                      double grad_threshold, double* disp, int32_t* match_col, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!l_phase || !r_phase || !l_k || !r_k || !disp)
+    PSGC_FAIL(PSGC_E_INVALID, "null input/output");
+  if (pha_f < 0 || pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pha_f=%d", pha_f);
+  const size_t npix = (size_t)rows * cols;
+  const double *lp, *rp;
+  const int32_t *lk, *rk;
+  PSGC_TRY(stage_in(ctx, mem, S_LPH, l_phase, npix, &lp));
+  PSGC_TRY(stage_in(ctx, mem, S_RPH, r_phase, npix, &rp));
+  PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
+  PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  double* d_disp;
+  int32_t* d_mc;
+  PSGC_TRY(stage_out(ctx, mem, S_DISP64, disp, npix, &d_disp));
+  PSGC_TRY(stage_out(ctx, mem, S_MATCH, match_col, npix, &d_mc));
+  void* span;
+  PSGC_TRY(ctx_buf(ctx, S_SPAN, (size_t)rows * (pha_f + 1) * 2 * sizeof(int32_t), &span));
+  PSGC_TRY(launch_match_plain(lp, rp, lk, rk, rows, cols, pha_f, grad_threshold,
+                              (int32_t*)span, d_disp, nullptr, d_mc, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, disp, d_disp, npix));
+  PSGC_TRY(copy_back(ctx, mem, match_col, d_mc, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_median5(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!src || !dst) PSGC_FAIL(PSGC_E_INVALID, "null src/dst");
+  const size_t npix = (size_t)rows * cols;
+  const float* s;
+  float* d;
+  PSGC_TRY(stage_in(ctx, mem, S_DISP, src, npix, &s));
+  PSGC_TRY(stage_out(ctx, mem, S_DISP2, dst, npix, &d));
+  if (s == d) PSGC_FAIL(PSGC_E_INVALID, "median5 cannot run in place");
+  PSGC_TRY(launch_median5(s, d, rows, cols, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, dst, d, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_reproject(psgc_ctx* ctx, const float* disp, int rows, int cols, const double Q[16],
+                   const double R1[9], double min_depth, double max_depth,
+                   int use_rectified_coords, float* xyz, uint8_t* P, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!disp || !Q || !R1 || !xyz || !P) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const size_t npix = (size_t)rows * cols;
+  double qr[25];
+  std::memcpy(qr, Q, 16 * sizeof(double));
+  std::memcpy(qr + 16, R1, 9 * sizeof(double));
+  const float* dd;
+  float* dx;
+  uint8_t* dp;
+  PSGC_TRY(stage_in(ctx, mem, S_DISP, disp, npix, &dd));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ0, xyz, npix * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P0, P, npix, &dp));
+  PSGC_TRY(launch_reproject(dd, nullptr, rows, cols, qr, min_depth, max_depth,
+                            use_rectified_coords, dx, dp, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, xyz, dx, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, P, dp, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_filter_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst_xyz,
+                         uint8_t* dst_P, int rows, int cols, int win, float sigma, int semantics,
+                         int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!xyz || !P || !dst_xyz || !dst_P) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (win < 1 || win > 7 || (win & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "win=%d", win);
+  const size_t npix = (size_t)rows * cols;
+  float k2[49];
+  gauss_kernel_f32(win, sigma, k2);
+  const float* sx;
+  const uint8_t* sp;
+  float* dx;
+  uint8_t* dp;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, npix * 3, &sx));
+  PSGC_TRY(stage_in(ctx, mem, S_P0, P, npix, &sp));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, dst_xyz, npix * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P1, dst_P, npix, &dp));
+  PSGC_TRY(launch_gaussian(sx, sp, dx, dp, rows, cols, win, k2, semantics, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, dst_P, dp, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_filter_ror(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst_xyz,
+                    uint8_t* dst_P, int rows, int cols, float radius, int min_neighbors,
+                    int window_size, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!xyz || !P || !dst_xyz || !dst_P) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (window_size < 0 || window_size > 512) PSGC_FAIL(PSGC_E_INVALID, "window=%d", window_size);
+  const size_t npix = (size_t)rows * cols;
+  const float* sx;
+  const uint8_t* sp;
+  float* dx;
+  uint8_t* dp;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, npix * 3, &sx));
+  PSGC_TRY(stage_in(ctx, mem, S_P0, P, npix, &sp));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, dst_xyz, npix * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P1, dst_P, npix, &dp));
+  PSGC_TRY(launch_ror(sx, sp, dx, dp, rows, cols, radius, min_neighbors, window_size,
+                      ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, dst_P, dp, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_filter_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst_xyz,
+                    uint8_t* dst_P, int rows, int cols, int k, float std_mul, int half_window,
+                    double* mean_dist, double stats[4], int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!xyz || !P || !dst_xyz || !dst_P) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (half_window < 1 || half_window > 64) PSGC_FAIL(PSGC_E_INVALID, "half_window=%d", half_window);
+  const size_t npix = (size_t)rows * cols;
+  const float* sx;
+  const uint8_t* sp;
+  float* dx;
+  uint8_t* dp;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, npix * 3, &sx));
+  PSGC_TRY(stage_in(ctx, mem, S_P0, P, npix, &sp));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, dst_xyz, npix * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P1, dst_P, npix, &dp));
+  double* md;
+  if (mean_dist && mem == PSGC_MEM_DEVICE) {
+    md = mean_dist;
+  } else {
+    void* v;
+    PSGC_TRY(ctx_buf(ctx, S_MEANDIST, npix * sizeof(double), &v));
+    md = (double*)v;
+  }
+  void *part, *st;
+  PSGC_TRY(ctx_buf(ctx, S_PARTIAL, sor_partial_doubles(npix) * sizeof(double), &part));
+  PSGC_TRY(ctx_buf(ctx, S_STATS, 4 * sizeof(double), &st));
+  PSGC_TRY(launch_sor(sx, sp, dx, dp, rows, cols, k, std_mul, half_window, md, (double*)part,
+                      (double*)st, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, dst_P, dp, npix));
+  if (mean_dist && mem == PSGC_MEM_HOST) PSGC_TRY(copy_back(ctx, mem, mean_dist, md, npix));
+  if (stats) {
+    // statistics are always returned to host memory
+    PSGC_HIP(hipMemcpyAsync(stats, st, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return finish(ctx, mem);
+}
+
+int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t* P, float* dst_xyz,
+                    int rows, int cols, int window_size, float spatial_sigma,
+                    int polynomial_order, float max_radius, int iterations, int semantics,
+                    int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!src_xyz || !P || !dst_xyz) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (window_size < 0 || window_size > 256) PSGC_FAIL(PSGC_E_INVALID, "window=%d", window_size);
+  if (!(spatial_sigma > 0)) PSGC_FAIL(PSGC_E_INVALID, "spatial_sigma must be > 0");
+  const size_t npix = (size_t)rows * cols;
+  const float r2 = (max_radius > 0.0f) ? max_radius * max_radius
+                                       : 9.0f * spatial_sigma * spatial_sigma;
+  const int order = polynomial_order < 0 ? 0 : (polynomial_order > 2 ? 2 : polynomial_order);
+  const int n_iter = iterations > 1 ? iterations : 1;
+  const float* sx;
+  const uint8_t* sp;
+  float* dx;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, src_xyz, npix * 3, &sx));
+  PSGC_TRY(stage_in(ctx, mem, S_P0, P, npix, &sp));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, dst_xyz, npix * 3, &dx));
+  if (sx == dx) PSGC_FAIL(PSGC_E_INVALID, "MLS cannot run in place");
+  // ping-pong: the last pass must land in dx
+  void* tmp = nullptr;
+  if (n_iter > 1) PSGC_TRY(ctx_buf(ctx, S_XYZ2, npix * 3 * sizeof(float), &tmp));
+  const float* in = sx;
+  for (int it = 0; it < n_iter; ++it) {
+    const int remaining = n_iter - 1 - it;
+    float* out = (remaining % 2 == 0) ? dx : (float*)tmp;
+    PSGC_TRY(launch_mls(in, sp, out, rows, cols, window_size, spatial_sigma, order, r2,
+                        semantics, ctx->stream));
+    in = out;
+  }
+  PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
+  return finish(ctx, mem);
+}
+
+int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
+                     const double* r_phase, const int32_t* l_k, const int32_t* r_k, int rows,
+                     int cols, int pha_f, const double Q[16], const double R1[9], float* xyz,
+                     uint8_t* P, float* disp, int32_t* match_col, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!cfg || !l_phase || !r_phase || !l_k || !r_k || !Q || !R1 || !xyz || !P)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (pha_f < 0 || pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pha_f=%d", pha_f);
+  const int gw = cfg->gauss_window;
+  if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
+  if (cfg->outlier_mode < PSGC_OUTLIER_NONE || cfg->outlier_mode > PSGC_OUTLIER_SOR)
+    PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", cfg->outlier_mode);
+  if (cfg->use_mls && !(cfg->mls_spatial_sigma > 0))
+    PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+  const size_t npix = (size_t)rows * cols;
+  const double *lp, *rp;
+  const int32_t *lk, *rk;
+  PSGC_TRY(stage_in(ctx, mem, S_LPH, l_phase, npix, &lp));
+  PSGC_TRY(stage_in(ctx, mem, S_RPH, r_phase, npix, &rp));
+  PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
+  PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  void *span, *d64, *d32, *d32m, *x0, *x1, *x2, *p0, *p1, *p2;
+  PSGC_TRY(ctx_buf(ctx, S_SPAN, (size_t)rows * (pha_f + 1) * 2 * sizeof(int32_t), &span));
+  PSGC_TRY(ctx_buf(ctx, S_DISP64, npix * sizeof(double), &d64));
+  PSGC_TRY(ctx_buf(ctx, S_DISP, npix * sizeof(float), &d32));
+  PSGC_TRY(ctx_buf(ctx, S_DISP2, npix * sizeof(float), &d32m));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ0, npix * 3 * sizeof(float), &x0));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ1, npix * 3 * sizeof(float), &x1));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ2, npix * 3 * sizeof(float), &x2));
+  PSGC_TRY(ctx_buf(ctx, S_P0, npix, &p0));
+  PSGC_TRY(ctx_buf(ctx, S_P1, npix, &p1));
+  PSGC_TRY(ctx_buf(ctx, S_P2, npix, &p2));
+  int32_t* d_mc;
+  PSGC_TRY(stage_out(ctx, mem, S_MATCH, match_col, npix, &d_mc));
+  hipStream_t s = ctx->stream;
+  PSGC_TRY(launch_match_plain(lp, rp, lk, rk, rows, cols, pha_f, cfg->grad_threshold,
+                              (int32_t*)span, (double*)d64, (float*)d32, d_mc, s));
+  const float* dfinal = (const float*)d32;
+  if (cfg->use_disp_median_filter) {
+    PSGC_TRY(launch_median5((const float*)d32, (float*)d32m, rows, cols, s));
+    dfinal = (const float*)d32m;
+  }
+  double qr[25];
+  std::memcpy(qr, Q, 16 * sizeof(double));
+  std::memcpy(qr + 16, R1, 9 * sizeof(double));
+  // CPU semantics without the median reprojects the f64 disparity
+  // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:401-430).
+  const bool use64 = !cfg->use_disp_median_filter && cfg->semantics == PSGC_SEM_CPU;
+  PSGC_TRY(launch_reproject(dfinal, use64 ? (const double*)d64 : nullptr, rows, cols, qr,
+                            cfg->min_depth, cfg->max_depth, cfg->use_rectified_coords,
+                            (float*)x0, (uint8_t*)p0, s));
+  float k2[49];
+  gauss_kernel_f32(gw, cfg->gauss_sigma, k2);
+  PSGC_TRY(launch_gaussian((const float*)x0, (const uint8_t*)p0, (float*)x1, (uint8_t*)p1, rows,
+                           cols, gw, k2, cfg->semantics, s));
+  const uint8_t* pin = cfg->semantics == PSGC_SEM_CPU ? (const uint8_t*)p0 : (const uint8_t*)p1;
+  float* xo = (float*)x2;
+  uint8_t* po = (uint8_t*)p2;
+  if (cfg->outlier_mode == PSGC_OUTLIER_ROR) {
+    PSGC_TRY(launch_ror((const float*)x1, pin, xo, po, rows, cols, cfg->radius_search,
+                        cfg->radius_min_neighbors, cfg->radius_window_size, s));
+  } else if (cfg->outlier_mode == PSGC_OUTLIER_SOR) {
+    void *md, *part, *st;
+    PSGC_TRY(ctx_buf(ctx, S_MEANDIST, npix * sizeof(double), &md));
+    PSGC_TRY(ctx_buf(ctx, S_PARTIAL, sor_partial_doubles(npix) * sizeof(double), &part));
+    PSGC_TRY(ctx_buf(ctx, S_STATS, 4 * sizeof(double), &st));
+    PSGC_TRY(launch_sor((const float*)x1, pin, xo, po, rows, cols, cfg->sor_k, cfg->sor_std_mul,
+                        cfg->sor_window, (double*)md, (double*)part, (double*)st, s));
+  } else {
+    xo = (float*)x1;
+    po = (uint8_t*)pin;
+  }
+  float* d_xyz;
+  uint8_t* d_P;
+  PSGC_TRY(stage_out(ctx, mem, S_TMP0, xyz, npix * 3, &d_xyz));
+  PSGC_TRY(stage_out(ctx, mem, S_TMP1, P, npix, &d_P));
+  if (cfg->use_mls) {
+    const float r2 = (cfg->mls_max_radius > 0.0f)
+                         ? cfg->mls_max_radius * cfg->mls_max_radius
+                         : 9.0f * cfg->mls_spatial_sigma * cfg->mls_spatial_sigma;
+    const int order = cfg->mls_polynomial_order < 0 ? 0
+                      : (cfg->mls_polynomial_order > 2 ? 2 : cfg->mls_polynomial_order);
+    const int n_iter = cfg->mls_iterations > 1 ? cfg->mls_iterations : 1;
+    // ping-pong between x0 (free now) and the output
+    const float* in = xo;
+    for (int it = 0; it < n_iter; ++it) {
+      const int remaining = n_iter - 1 - it;
+      float* out = (remaining % 2 == 0) ? d_xyz : (float*)x0;
+      if (out == in) out = (float*)x1;
+      PSGC_TRY(launch_mls(in, po, out, rows, cols, cfg->mls_window_size, cfg->mls_spatial_sigma,
+                          order, r2, cfg->semantics, s));
+      in = out;
+    }
+    if (in != d_xyz)
+      PSGC_HIP(hipMemcpyAsync(d_xyz, in, npix * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  } else {
+    PSGC_HIP(hipMemcpyAsync(d_xyz, xo, npix * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  PSGC_HIP(hipMemcpyAsync(d_P, po, npix, hipMemcpyDeviceToDevice, s));
+  if (disp) {
+    if (mem == PSGC_MEM_HOST)
+      PSGC_HIP(hipMemcpyAsync(disp, dfinal, npix * sizeof(float), hipMemcpyDeviceToHost, s));
+    else
+      PSGC_HIP(hipMemcpyAsync(disp, dfinal, npix * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, P, d_P, npix));
+  PSGC_TRY(copy_back(ctx, mem, match_col, d_mc, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_profile_enable(int on) {
+  for (auto& r : g_prof) {
+    hipEventDestroy(r.beg);
+    hipEventDestroy(r.end);
+  }
+  g_prof.clear();
+  g_prof_open.clear();
+  g_prof_on = on != 0;
+  return PSGC_OK;
+}
+
+int psgc_profile_read(const char* name, int64_t* launches, double* total_ms) {
+  if (!name || !launches || !total_ms) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  PSGC_HIP(hipDeviceSynchronize());
+  int64_t n = 0;
+  double ms = 0;
+  for (auto& r : g_prof) {
+    if (r.name != name) continue;
+    float t = 0;
+    if (hipEventElapsedTime(&t, r.beg, r.end) == hipSuccess) {
+      ms += t;
+      ++n;
+    }
+  }
+  *launches = n;
+  *total_ms = ms;
+  return PSGC_OK;
+}
+
+int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1, double cx2, double baseline,
+                       double Q[16], double R1[9]) {
+  if (!Q || !R1) PSGC_FAIL(PSGC_E_INVALID, "null Q/R1");
+  if (!(fx > 0) || !(fy > 0) || baseline == 0) PSGC_FAIL(PSGC_E_INVALID, "bad intrinsics");
+  (void)fy;
+  // Q of cv::stereoRectify (flags 0) for R = I, T = (-B, 0, 0):
+  // [1 0 0 -cx1; 0 1 0 -cy1; 0 0 0 f; 0 0 -1/Tx (cx1-cx2)/Tx], Tx = -B.
+  const double Tx = -baseline;
+  const double q[16] = {1, 0, 0, -cx1, 0, 1, 0, -cy1, 0, 0, 0, fx, 0, 0, -1.0 / Tx,
+                        (cx1 - cx2) / Tx};
+  std::memcpy(Q, q, sizeof(q));
+  const double r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::memcpy(R1, r, sizeof(r));
+  return PSGC_OK;
+}
+
+}  // extern "C"

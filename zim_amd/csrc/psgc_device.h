@@ -1,0 +1,379 @@
+// psgc_device.h — per-pixel device routines shared by the plain and the
+// tiled kernels. Each routine states the reference lines it follows; the
+// operation order is the reference CPU path's, evaluated without
+// floating-point contraction.
+#pragma once
+
+#include "psgc_internal.h"
+#include "psgc_math.h"
+
+namespace psgc {
+
+// ---------------------------------------------------------------- decode --
+
+__device__ inline double load_sample(const DecodeArgs& a, int f, int r, int x) {
+  const char* row = (const char*)a.frames[f] + (size_t)r * a.stride;
+  if (a.sample_type == PSGC_SAMPLE_U16) return (double)((const uint16_t*)row)[x];
+  if (a.sample_type == PSGC_SAMPLE_U8) return (double)((const uint8_t*)row)[x];
+  return ((const double*)row)[x];
+}
+
+// 3x3 pre-blur at (r, x), replicated border: row pass c*x + s*(l + r) on rows
+// r-1, r, r+1, then the same combination down the column
+// (cv::GaussianBlur in JSLAB_ImageTypeConvertBodyV2,
+// JSLAB_PhaseShiftDecoder_V2.cpp:22-26).
+__device__ inline double blurred_sample(const DecodeArgs& a, int f, int r, int x) {
+  const int xl = x > 0 ? x - 1 : 0, xr = x < a.cols - 1 ? x + 1 : a.cols - 1;
+  const int ru = r > 0 ? r - 1 : 0, rd = r < a.rows - 1 ? r + 1 : a.rows - 1;
+  const double c = a.blur_c, s = a.blur_s;
+  const double hu = c * load_sample(a, f, ru, x) +
+                    s * (load_sample(a, f, ru, xl) + load_sample(a, f, ru, xr));
+  const double hm = c * load_sample(a, f, r, x) +
+                    s * (load_sample(a, f, r, xl) + load_sample(a, f, r, xr));
+  const double hd = c * load_sample(a, f, rd, x) +
+                    s * (load_sample(a, f, rd, xl) + load_sample(a, f, rd, xr));
+  return c * hm + s * (hu + hd);
+}
+
+// Wrapped phase and normalized modulation of one level from its sums
+// S = -sum sin(d_k) I_k, C = sum cos(d_k) I_k, G = sum I_k
+// (JSLAB_PhaseShiftDecoder_V2.cpp:99-117). Returns false when the pixel is
+// rejected (modulation below the threshold).
+__device__ inline bool level_finish(double S, double C, double G, int n, double thr,
+                                    double* wrap, double* mod) {
+  *wrap = psgc::atan2(S, C);
+  double b = S * S + C * C;
+  const double mean = G / n;
+  b = 2.0 * sqrt(b) / n;
+  double nm = 0.0;
+  if (mean > 1e-3) nm = b / mean;
+  *mod = nm;
+  return !(nm < thr);
+}
+
+// Three-frequency heterodyne unwrapping and fringe order
+// (JSLAB_PhaseShiftDecoder_V2.cpp:124-179).
+__device__ inline void heterodyne(const DecodeArgs& a, const double wrap[3], double* ph,
+                                  int32_t* k) {
+  const double f0 = a.freq[0], f1 = a.freq[1], f2 = a.freq[2];
+  double phase12, phase23, phase123;
+  if (wrap[0] > wrap[1]) phase12 = wrap[0] - wrap[1];
+  else phase12 = wrap[0] - wrap[1] + 2 * kPi;
+  if (f1 > f2) {
+    if (wrap[1] > wrap[2]) phase23 = wrap[1] - wrap[2];
+    else phase23 = wrap[1] - wrap[2] + 2 * kPi;
+  } else {
+    if (wrap[2] > wrap[1]) phase23 = wrap[2] - wrap[1];
+    else phase23 = wrap[2] - wrap[1] + 2 * kPi;
+  }
+  if (phase12 > phase23) phase123 = phase12 - phase23;
+  else phase123 = phase12 - phase23 + 2 * kPi;
+  const double f12 = (double)a.f12, f23 = (double)a.f23;
+  const double u12 = phase12 + round_half_away((phase123 * f12 - phase12) / (2 * kPi)) * 2 * kPi;
+  const double kv = round_half_away((u12 * f0 / f12 - wrap[0]) / (2 * kPi));
+  const double u1 = wrap[0] + kv * 2 * kPi;
+  const double u23 = phase23 + round_half_away((phase123 * f23 - phase23) / (2 * kPi)) * 2 * kPi;
+  double u2 = wrap[1] + round_half_away((u23 * f1 / f23 - wrap[1]) / (2 * kPi)) * 2 * kPi;
+  u2 = (u2 * f0) / f1;
+  double u3 = wrap[2] + round_half_away((u23 * f2 / f23 - wrap[2]) / (2 * kPi)) * 2 * kPi;
+  u3 = (u3 * f0) / f2;
+  *ph = a.avg_three ? (u1 + u2 + u3) / 3.0 : u1;
+  *k = (int32_t)kv;
+}
+
+// Gray-code order with complement frames (MSB first) combined with the
+// wrapped phase moved to [0, 2 pi). Returns false for low contrast.
+__device__ inline bool gray_order(const DecodeArgs& a, int r, int x, double wrap0, double* ph,
+                                  int32_t* k) {
+  const int nphase = a.shift_num[0];
+  unsigned g = 0;
+  for (int b = 0; b < a.gray_bits; ++b) {
+    const double v = load_sample(a, nphase + b, r, x);
+    const double c = load_sample(a, nphase + a.gray_bits + b, r, x);
+    const double d = v - c;
+    if (fabs(d) < a.gray_min_contrast) return false;
+    g = (g << 1) | (d > 0 ? 1u : 0u);
+  }
+  unsigned n = g;
+  for (unsigned s = 1; s < 32; s <<= 1) n ^= n >> s;
+  const double w = wrap0 < 0 ? wrap0 + 2 * kPi : wrap0;
+  *ph = w + 2 * kPi * (double)n;
+  *k = (int32_t)n;
+  return true;
+}
+
+// k-th smallest of v[0..n) (n small), by partial selection.
+__device__ inline double select_kth(double* v, int n, int kth) {
+  for (int i = 0; i <= kth; ++i) {
+    int mi = i;
+    for (int t = i + 1; t < n; ++t)
+      if (v[t] < v[mi]) mi = t;
+    const double tmp = v[i];
+    v[i] = v[mi];
+    v[mi] = tmp;
+  }
+  return v[kth];
+}
+
+// ----------------------------------------------------------------- match --
+
+// Parabolic sub-pixel root around the integer match `best` of left column j
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003). `rrow` is the right
+// phase row.
+__device__ inline double subpixel_disparity(const double* rrow, int best, double lv, int j) {
+  const double c0 = rrow[best - 1] - lv;
+  const double c1 = rrow[best] - lv;
+  const double c2 = rrow[best + 1] - lv;
+  const double num = c1 * (c0 - c2);
+  const double den = 2.0 * (c0 - c1) * (c1 - c2);
+  double delta = 0.0;
+  if (fabs(den) > 1e-9) delta = num / den;
+  return (double)j - (double)best - delta;
+}
+
+// Q reprojection of one pixel (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+// 423-459). Returns the mask value; out receives the f32 point or zeros.
+__device__ inline uint8_t reproject_point(const double* q, const double* r1, int i, int j,
+                                          double d, double min_depth, double max_depth,
+                                          int use_rect, float out[3]) {
+  out[0] = out[1] = out[2] = 0.f;
+  if (fabs(d) < 1e-5) return 0;
+  const double w = q[14] * d + q[15];
+  double x = j + q[3];
+  double y = i + q[7];
+  double z = q[11];
+  const double inv_w = 1.0 / w;
+  x *= inv_w;
+  y *= inv_w;
+  z *= inv_w;
+  if (use_rect) {
+    if (z > min_depth && z < max_depth) {
+      out[0] = (float)x;
+      out[1] = (float)y;
+      out[2] = (float)z;
+      return 1;
+    }
+    return 0;
+  }
+  const double rx = r1[0] * x + r1[3] * y + r1[6] * z;
+  const double ry = r1[1] * x + r1[4] * y + r1[7] * z;
+  const double rz = r1[2] * x + r1[5] * y + r1[8] * z;
+  if (rz > min_depth && rz < max_depth) {
+    out[0] = (float)rx;
+    out[1] = (float)ry;
+    out[2] = (float)rz;
+    return 1;
+  }
+  return 0;
+}
+
+// Squared distance in f32, evaluated as dx*dx + dy*dy + dz*dz
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:171-174).
+__device__ inline float dist2(float px, float py, float pz, const float* q) {
+  const float dx = px - q[0];
+  const float dy = py - q[1];
+  const float dz = pz - q[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// ------------------------------------------------------------------- MLS --
+
+// symmetricEigen3x3 (JSLAB_MLS_Filter.cpp:11-106): ascending eigenvalues,
+// eigenvectors as rows of V.
+__device__ inline void eigen3(const float C[3][3], float ev[3], float V[3][3]) {
+  const float p1 = C[0][1] * C[0][1] + C[0][2] * C[0][2] + C[1][2] * C[1][2];
+  if (p1 < 1e-20f) {
+    ev[0] = C[0][0];
+    ev[1] = C[1][1];
+    ev[2] = C[2][2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.f : 0.f;
+    for (int i = 0; i < 2; ++i)
+      for (int j = i + 1; j < 3; ++j)
+        if (ev[i] > ev[j]) {
+          float t = ev[i];
+          ev[i] = ev[j];
+          ev[j] = t;
+          for (int c = 0; c < 3; ++c) {
+            t = V[i][c];
+            V[i][c] = V[j][c];
+            V[j][c] = t;
+          }
+        }
+    return;
+  }
+  const float trace = C[0][0] + C[1][1] + C[2][2];
+  const float q = trace / 3.0f;
+  float B[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B[i][j] = C[i][j];
+  B[0][0] -= q;
+  B[1][1] -= q;
+  B[2][2] -= q;
+  const float p2 = B[0][0] * B[0][0] + B[1][1] * B[1][1] + B[2][2] * B[2][2] + 2.0f * p1;
+  const float p = sqrtf(p2 / 6.0f);
+  const float inv_p = (p > 1e-15f) ? (1.0f / p) : 0.0f;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B[i][j] *= inv_p;
+  const float detB = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                     B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                     B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
+  float rr = detB * 0.5f;
+  rr = fmaxf(-1.0f, fminf(1.0f, rr));
+  const float phi = acosf(rr) / 3.0f;
+  const float PIf = 3.14159265358979323846f;
+  ev[0] = q + 2.0f * p * cosf(phi + 2.0f * PIf / 3.0f);
+  ev[2] = q + 2.0f * p * cosf(phi);
+  ev[1] = trace - ev[0] - ev[2];
+  for (int idx = 0; idx < 3; ++idx) {
+    float A[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) A[i][j] = C[i][j];
+    A[0][0] -= ev[idx];
+    A[1][1] -= ev[idx];
+    A[2][2] -= ev[idx];
+    const float c01[3] = {A[0][1] * A[1][2] - A[0][2] * A[1][1],
+                          A[0][2] * A[1][0] - A[0][0] * A[1][2],
+                          A[0][0] * A[1][1] - A[0][1] * A[1][0]};
+    const float c02[3] = {A[0][1] * A[2][2] - A[0][2] * A[2][1],
+                          A[0][2] * A[2][0] - A[0][0] * A[2][2],
+                          A[0][0] * A[2][1] - A[0][1] * A[2][0]};
+    const float c12[3] = {A[1][1] * A[2][2] - A[1][2] * A[2][1],
+                          A[1][2] * A[2][0] - A[1][0] * A[2][2],
+                          A[1][0] * A[2][1] - A[1][1] * A[2][0]};
+    const float l01 = c01[0] * c01[0] + c01[1] * c01[1] + c01[2] * c01[2];
+    const float l02 = c02[0] * c02[0] + c02[1] * c02[1] + c02[2] * c02[2];
+    const float l12 = c12[0] * c12[0] + c12[1] * c12[1] + c12[2] * c12[2];
+    const float* best = c12;
+    float bl = l12;
+    if (l01 > bl) { best = c01; bl = l01; }
+    if (l02 > bl) { best = c02; bl = l02; }
+    const float inv = (bl > 1e-20f) ? (1.0f / sqrtf(bl)) : 0.0f;
+    V[idx][0] = best[0] * inv;
+    V[idx][1] = best[1] * inv;
+    V[idx][2] = best[2] * inv;
+  }
+}
+
+// choleskySolve (JSLAB_MLS_Filter.cpp:113-156).
+__device__ inline bool chol_solve(const float A[6][6], const float b[6], float x[6], int nc) {
+  float L[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) L[i][j] = 0.f;
+  for (int i = 0; i < nc; ++i)
+    for (int j = 0; j <= i; ++j) {
+      float s = A[i][j];
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (s <= 1e-12f) return false;
+        L[i][j] = sqrtf(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  float y[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < nc; ++i) {
+    float s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+  for (int i = nc - 1; i >= 0; --i) {
+    float s = y[i];
+    for (int k = i + 1; k < nc; ++k) s -= L[k][i] * x[k];
+    x[i] = s / L[i][i];
+  }
+  return true;
+}
+
+// Effective polynomial order after the neighbour-count fallback:
+// CPU rule JSLAB_MLS_Filter.cpp:277-279, GPU rule JSLAB_MLS_Filter.cu:283-286.
+__device__ inline int mls_order(int order, int nv, int semantics) {
+  const int minnb0 = 3, minnb1 = 5, minnb2 = 8;
+  const int need = order == 2 ? minnb2 : (order == 1 ? minnb1 : minnb0);
+  int cur;
+  if (semantics == PSGC_SEM_CPU) cur = (nv >= need) ? order : 0;
+  else cur = order;
+  if (cur > 0 && nv < (cur == 2 ? minnb2 : minnb1)) cur = (nv >= minnb1) ? 1 : 0;
+  return cur;
+}
+
+// Projection of point pi onto the polynomial fitted in its local frame, from
+// centroid, covariance frame and normal equations
+// (JSLAB_MLS_Filter.cpp:283-388). Returns the projected point in out.
+struct MlsFit {
+  float cx, cy, cz;
+  float n[3], t1[3], t2[3];
+};
+
+__device__ inline void mls_frame(float cov[3][3], MlsFit* f) {
+  cov[1][0] = cov[0][1];
+  cov[2][0] = cov[0][2];
+  cov[2][1] = cov[1][2];
+  float ev[3], V[3][3];
+  eigen3(cov, ev, V);
+  for (int c = 0; c < 3; ++c) {
+    f->n[c] = V[0][c];
+    f->t1[c] = V[1][c];
+    f->t2[c] = V[2][c];
+  }
+}
+
+__device__ inline void mls_plane(const MlsFit& f, const float pi[3], float out[3]) {
+  const float dot = (pi[0] - f.cx) * f.n[0] + (pi[1] - f.cy) * f.n[1] + (pi[2] - f.cz) * f.n[2];
+  out[0] = pi[0] - dot * f.n[0];
+  out[1] = pi[1] - dot * f.n[1];
+  out[2] = pi[2] - dot * f.n[2];
+}
+
+// Solve (with the order-2 -> 1 fallback of JSLAB_MLS_Filter.cpp:350-374) and
+// evaluate at the query point.
+__device__ inline void mls_solve_eval(const MlsFit& f, float AtWA[6][6], float AtWf[6], int cur,
+                                      const float pi[3], float out[3]) {
+  const int nc = cur == 1 ? 3 : 6;
+  for (int a = 0; a < nc; ++a)
+    for (int b = 0; b < a; ++b) AtWA[a][b] = AtWA[b][a];
+  float co[6] = {0, 0, 0, 0, 0, 0};
+  bool solved = chol_solve(AtWA, AtWf, co, nc);
+  if (!solved && cur == 2) {
+    float A1[6][6], b1[6];
+    for (int a = 0; a < 6; ++a) {
+      b1[a] = a < 3 ? AtWf[a] : 0.f;
+      for (int b = 0; b < 6; ++b) A1[a][b] = (a < 3 && b < 3) ? AtWA[a][b] : 0.f;
+    }
+    solved = chol_solve(A1, b1, co, 3);
+    if (solved) cur = 1;
+  }
+  if (!solved) {
+    mls_plane(f, pi, out);
+    return;
+  }
+  const float ex0 = pi[0] - f.cx, ey0 = pi[1] - f.cy, ez0 = pi[2] - f.cz;
+  const float u0 = ex0 * f.t1[0] + ey0 * f.t1[1] + ez0 * f.t1[2];
+  const float v0 = ex0 * f.t2[0] + ey0 * f.t2[1] + ez0 * f.t2[2];
+  float ff = co[0] + co[1] * u0 + co[2] * v0;
+  if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
+  out[0] = f.cx + u0 * f.t1[0] + v0 * f.t2[0] + ff * f.n[0];
+  out[1] = f.cy + u0 * f.t1[1] + v0 * f.t2[1] + ff * f.n[1];
+  out[2] = f.cz + u0 * f.t1[2] + v0 * f.t2[2] + ff * f.n[2];
+}
+
+// Accumulates one neighbour pk (weight w) into the normal equations.
+__device__ inline void mls_accum_poly(const MlsFit& f, const float* pk, float w, int nc,
+                                      float AtWA[6][6], float AtWf[6]) {
+  const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+  const float u = ex * f.t1[0] + ey * f.t1[1] + ez * f.t1[2];
+  const float v = ex * f.t2[0] + ey * f.t2[1] + ez * f.t2[2];
+  const float fv = ex * f.n[0] + ey * f.n[1] + ez * f.n[2];
+  float basis[6] = {1.0f, u, v, 0.f, 0.f, 0.f};
+  if (nc == 6) {
+    basis[3] = u * u;
+    basis[4] = u * v;
+    basis[5] = v * v;
+  }
+  for (int a = 0; a < nc; ++a) {
+    AtWf[a] += w * basis[a] * fv;
+    for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
+  }
+}
+
+}  // namespace psgc

@@ -1,0 +1,147 @@
+// psgc_internal.h — context, error plumbing and kernel entry declarations
+// shared by the translation units of libpsgc.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "psgc.h"
+
+namespace psgc {
+
+void set_error(const char* fmt, ...);
+
+#define PSGC_FAIL(code, ...)        \
+  do {                              \
+    ::psgc::set_error(__VA_ARGS__); \
+    return (code);                  \
+  } while (0)
+
+#define PSGC_HIP(expr)                                                        \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      PSGC_FAIL(PSGC_E_HIP, "%s failed: %s (%s:%d)", #expr,                  \
+                hipGetErrorString(_e), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define PSGC_TRY(expr)           \
+  do {                           \
+    int _rc = (expr);            \
+    if (_rc != PSGC_OK) return _rc; \
+  } while (0)
+
+// Launch-error check right after a kernel launch.
+#define PSGC_LAUNCHED() PSGC_HIP(hipGetLastError())
+
+// One growable device allocation owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Buffer slots of a context (staging for host-pointer calls and the
+// intermediates of the multi-stage calls).
+enum Slot : int {
+  S_FRAMES = 0, S_PHASE, S_PHASE_RAW, S_K, S_MOD, S_VALID,
+  S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
+  S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
+  S_STATS, S_QR, S_TMP0, S_TMP1, S_NUM_SLOTS
+};
+
+}  // namespace psgc
+
+struct psgc_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<psgc::DevBuf> bufs;
+  int cu_count = 0;
+};
+
+namespace psgc {
+
+// Returns a device buffer of at least `bytes` for `slot` (grown with a
+// stream synchronisation so that in-flight kernels never see a freed block).
+int ctx_buf(psgc_ctx* ctx, int slot, size_t bytes, void** out);
+int ctx_release(psgc_ctx* ctx);
+
+// ---- kernel launchers (all asynchronous on `s`, device pointers) --------
+
+struct DecodeArgs {
+  const void* frames[64];  // device planes
+  int n_frames;
+  int sample_type;
+  int rows, cols;
+  int64_t stride;          // row stride in bytes
+  int nlev;                // 3 heterodyne, 1 Gray code
+  int shift_num[3];
+  int level_base[3];
+  double sin_tab[48];      // concatenated per level
+  double cos_tab[48];
+  double freq[3];
+  int f12, f23;
+  double mod_thresh;
+  int avg_three;
+  double blur_c, blur_s;   // pre-blur weights (centre, side)
+  int gray_bits;
+  double gray_min_contrast;
+  int hdr;                 // HDR fit is not on the device path
+};
+
+int launch_decode_plain(const DecodeArgs& a, double* phase_raw, int32_t* k,
+                        double* mod, uint8_t* valid, hipStream_t s);
+int launch_gray_fix(const DecodeArgs& a, const double* phase_in, double* phase_out,
+                    const int32_t* k_in, int32_t* k_out, const uint8_t* valid,
+                    hipStream_t s);
+int launch_post_smooth(const double* in, double* out, int rows, int cols,
+                       hipStream_t s);
+
+int launch_match_plain(const double* lp, const double* rp, const int32_t* lk,
+                       const int32_t* rk, int rows, int cols, int pha_f,
+                       double grad_thr, int32_t* span, double* disp64,
+                       float* disp32, int32_t* match_col, hipStream_t s);
+int launch_median5(const float* src, float* dst, int rows, int cols,
+                   hipStream_t s);
+int launch_reproject(const float* disp32, const double* disp64, int rows,
+                     int cols, const double* QR, double min_depth,
+                     double max_depth, int use_rect, float* xyz, uint8_t* P,
+                     hipStream_t s);
+int launch_gaussian(const float* xyz, const uint8_t* P, float* dst,
+                    uint8_t* dstP, int rows, int cols, int win,
+                    const float* k2_host, int semantics, hipStream_t s);
+int launch_ror(const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
+               int rows, int cols, float radius, int min_nb, int window,
+               hipStream_t s);
+int launch_sor(const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
+               int rows, int cols, int k, float std_mul, int half,
+               double* mean_dist, double* partial, double* stats_dev,
+               hipStream_t s);
+int launch_sor_finish(const float* xyz, double* md, float* dst, uint8_t* dstP,
+                      size_t npix, float std_mul, double* partial, double* stats,
+                      hipStream_t s);
+size_t sor_partial_doubles(size_t npix);
+int launch_mls(const float* src, const uint8_t* P, float* dst, int rows,
+               int cols, int hw, float sigma, int order, float r2cut,
+               int semantics, hipStream_t s);
+
+// Kernel timing with HIP events on the launch stream (off unless enabled
+// through psgc_profile_enable). Every launcher brackets its kernels with
+// PSGC_PROF_BEGIN/END under a kernel-family name.
+void prof_begin(const char* name, hipStream_t s);
+void prof_end(const char* name, hipStream_t s);
+extern bool g_prof_on;
+#define PSGC_PROF_BEGIN(name, s) \
+  do { if (::psgc::g_prof_on) ::psgc::prof_begin(name, s); } while (0)
+#define PSGC_PROF_END(name, s) \
+  do { if (::psgc::g_prof_on) ::psgc::prof_end(name, s); } while (0)
+
+// Host-side helpers.
+void gauss_kernel_f32(int win, float sigma, float* k2 /* win*win */);
+void blur3_weights(double sigma, double* c, double* s);
+
+}  // namespace psgc
