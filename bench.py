@@ -1,19 +1,22 @@
 """Throughput of the reconstruction path on one MI355X.
 
-Workload (full-size single-pair shape, batched): `--pairs` independently
-seeded 2448x2048 rectified pairs, three-frequency heterodyne 4-step fringes
+Workload: `--pairs` independently seeded 2448x2048 rectified pairs (the
+full-size single-pair shape), three-frequency heterodyne 4-step fringes
 (70/64/59 across a 1280-px projector), uint16 12-bit, rendered on the GPU
 before timing. One step = decode L + decode R + reconstruct (order match,
 5x5 median, Q reprojection, 5x5 Gaussian sigma 1, SOR k=30, MLS order 2 at
-1.5 mm) of every pair, frames already resident in HBM.
+1.5 mm, 2 iterations) of every pair, frames resident in HBM.
 
-Prints one JSON line: metric, value (Mpts/s over all pairs, one point =
-one left rectified pixel), per-path numbers, the dominant kernel's HBM
-roofline and the CPU restatement's throughput on the host.
+value = Mpts/s of the batched pipeline (one hipGraph replay per step; one
+point = one left rectified pixel). The plain per-pair GPU path (one
+launch per stage per pair, one-thread-per-pixel kernels, a sync after
+each pair, as the reference GPU path is structured) is timed on the same
+pairs beside it. Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,37 +33,35 @@ ROWS, COLS = 2048, 2448
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs", type=int, default=16)
-    ap.add_argument("--path", choices=["plain"], default="plain")
+    ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--plain-pairs", type=int, default=8,
+                    help="pairs timed through the plain per-pair path (subset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample-rows", type=int, default=ROWS)
     return ap.parse_args()
 
 
-def algorithmic_bytes(family: str, npix: int) -> float:
-    """Compulsory HBM bytes of one launch of a kernel family over npix pixels
-    (inputs read once, outputs written once)."""
-    per = {
-        "decode": 12 * 2 + 8 + 4 + 8 + 0,   # 12 u16 frames -> phase f64, k i32, mod f64
-        "post_smooth": 8 + 8,
-        "match": 8 + 8 + 4 + 4 + 8 + 4 + 4,  # l/r phase, l/r k -> d64, d32, col
-        "median5": 4 + 4,
-        "reproject": 4 + 12 + 1,
-        "gaussian": 13 + 13,
-        "sor_knn": 13 + 8,
-        "sor_stats": 8 + 12 + 13,
-        "mls": 13 + 12,
-    }[family]
-    return float(per) * npix
+# Compulsory HBM bytes per pixel of one launch of each kernel family
+# (inputs read once, outputs written once), for the tiled batch pipeline.
+ALG_BYTES_PER_PX = {
+    "decode": 12 * 2 + 8 + 4,         # 12 u16 frames -> raw phase f64 + order i32
+    "match": 2 * (8 + 4) + 4,         # l/r raw phase + order -> f32 disparity
+    "median5": 4 + 4,
+    "repro_gauss": 4 + 12 + 1,        # disparity -> Gaussian xyz + mask
+    "sor_knn": 12 + 1 + 8,            # xyz + mask -> mean distance f64
+    "sor_stats": 8,
+    "mls": 12 + 1 + 8 + 12,           # xyz, mask, mean distance -> xyz
+}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(pairs shard across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pairs shard "
+              "across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -71,92 +72,117 @@ def main():
 
     dev = torch.device("cuda:0")
     ctx = zim_amd.Context(0)
+    lib = ctx.lib
     dcfg = scenes.c2_decode_config()
     rcfg = scenes.c2_recon_config()
 
-    print(f"[bench] rendering {args.pairs} pairs of {COLS}x{ROWS} on the GPU", file=sys.stderr,
-          flush=True)
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    log(f"rendering {args.pairs} pairs of {COLS}x{ROWS} on the GPU")
     t0 = time.time()
-    pairs = []
-    for i in range(args.pairs):
-        p = synth.render_c2(1000 + i, device=dev)
-        pairs.append(p)
+    pairs = [synth.render_c2(1000 + i, device=dev) for i in range(args.pairs)]
     torch.cuda.synchronize()
-    print(f"[bench] rendered in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-
+    log(f"rendered in {time.time() - t0:.1f}s")
     npix = ROWS * COLS
-    outs = []
-    for _ in pairs:
-        outs.append(dict(
-            lph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
-            rph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
-            lk=torch.empty((ROWS, COLS), dtype=torch.int32, device=dev),
-            rk=torch.empty((ROWS, COLS), dtype=torch.int32, device=dev),
-            mod=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
-            xyz=torch.empty((ROWS, COLS, 3), dtype=torch.float32, device=dev),
-            P=torch.empty((ROWS, COLS), dtype=torch.uint8, device=dev),
-        ))
-    lib = ctx.lib
-    import ctypes as C
-    Qa = [np.ascontiguousarray(p.Q, np.float64) for p in pairs]
-    Ra = [np.ascontiguousarray(p.R1, np.float64) for p in pairs]
 
-    def run_plain():
-        ctx.bind_stream()
-        for p, o, q, r in zip(pairs, outs, Qa, Ra):
-            for frames, ph, k in ((p.left, o["lph"], o["lk"]), (p.right, o["rph"], o["rk"])):
-                arr = (C.c_void_p * len(frames))(*[f.data_ptr() for f in frames])
-                _capi.check(lib.psgc_decode(ctx.h, C.byref(dcfg), arr, len(frames),
-                                            _capi.SAMPLE_U16, ROWS, COLS, COLS * 2,
-                                            _capi.ptr(ph), _capi.ptr(k), _capi.ptr(o["mod"]),
-                                            None, _capi.MEM_DEVICE))
-            _capi.check(lib.psgc_reconstruct(ctx.h, C.byref(rcfg), _capi.ptr(o["lph"]),
-                                             _capi.ptr(o["rph"]), _capi.ptr(o["lk"]),
-                                             _capi.ptr(o["rk"]), ROWS, COLS, 70, _capi.ptr(q),
-                                             _capi.ptr(r), _capi.ptr(o["xyz"]),
-                                             _capi.ptr(o["P"]), None, None, _capi.MEM_DEVICE))
-
-    run = run_plain
+    # ---------------------------------------------------------- batched path
+    t0 = time.time()
+    batch = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rcfg, pha_f=70,
+                                Q=p.Q, R1=p.R1) for p in pairs], ctx, graph=True, autotune=True)
+    torch.cuda.synchronize()
+    grids = {s: batch.grid(s) for s in ("decode", "match", "median5", "repro_gauss", "sor_knn",
+                                        "mls")}
+    log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}")
     for w in range(args.warmup):
-        run()
+        batch.run()
         torch.cuda.synchronize()
-        print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-
-    families = ["decode", "post_smooth", "match", "median5", "reproject", "gaussian", "sor_knn",
-                "sor_stats", "mls"]
-    lib.psgc_profile_enable(1)
+    log("warmup done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        run()
+    for _ in range(args.steps):
+        batch.run()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = npix * args.pairs / (ms_per_step * 1e-3) / 1e6
+    valid = sum(int((o["P"] != 0).sum().item()) for o in batch.outputs)
+    log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} Mpts/s")
+
+    # per-kernel durations: eager replay of the same launches with HIP events
+    lib.psgc_profile_enable(1)
+    prof_steps = 2
+    for _ in range(prof_steps):
+        batch.run_eager()
+    torch.cuda.synchronize()
     prof = {}
-    for fam in families:
+    for fam in list(ALG_BYTES_PER_PX) + ["gray_fix", "ror"]:
         n = C.c_int64()
         ms = C.c_double()
         _capi.check(lib.psgc_profile_read(fam.encode(), C.byref(n), C.byref(ms)))
         if n.value:
             prof[fam] = (n.value, ms.value)
     lib.psgc_profile_enable(0)
-    print(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step", file=sys.stderr,
-          flush=True)
-
-    pts = npix * args.pairs
-    value = pts / (ms_per_step * 1e-3) / 1e6
-    valid = sum(int((o["P"] != 0).sum().item()) for o in outs)
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_n, dom_ms) = dom
-    # launches of the dominant family per step: decode runs per camera, mls per iteration
+    stage_ms = {k: round(v[1] / prof_steps, 3) for k, v in prof.items()}
+    dom_name, (dom_n, dom_ms) = max(prof.items(), key=lambda kv: kv[1][1])
     avg_ms = dom_ms / dom_n
-    alg = algorithmic_bytes(dom_name, npix)
+    # one launch of a batched family covers every pair (MLS: one iteration)
+    alg = ALG_BYTES_PER_PX[dom_name] * npix * args.pairs * (2 if dom_name == "decode" else 1)
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    breakdown = {k: round(v[1] / args.steps, 3) for k, v in prof.items()}
+    log(f"stage ms/step {stage_ms}")
+
+    # ----------------------------------------------------- plain per-pair path
+    plain = None
+    if not args.no_plain:
+        ctx.set_kernels("plain")
+        ctx.bind_stream()
+        sub = pairs[:args.plain_pairs]
+        bufs = [dict(lph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
+                     rph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
+                     lk=torch.empty((ROWS, COLS), dtype=torch.int32, device=dev),
+                     rk=torch.empty((ROWS, COLS), dtype=torch.int32, device=dev),
+                     mod=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
+                     xyz=torch.empty((ROWS, COLS, 3), dtype=torch.float32, device=dev),
+                     P=torch.empty((ROWS, COLS), dtype=torch.uint8, device=dev)) for _ in sub]
+        Qa = [np.ascontiguousarray(p.Q, np.float64) for p in sub]
+        Ra = [np.ascontiguousarray(p.R1, np.float64) for p in sub]
+
+        def run_plain():
+            for p, o, q, r in zip(sub, bufs, Qa, Ra):
+                for frames, ph, k in ((p.left, o["lph"], o["lk"]), (p.right, o["rph"], o["rk"])):
+                    arr = (C.c_void_p * len(frames))(*[f.data_ptr() for f in frames])
+                    _capi.check(lib.psgc_decode(ctx.h, C.byref(dcfg), arr, len(frames),
+                                                _capi.SAMPLE_U16, ROWS, COLS, COLS * 2,
+                                                _capi.ptr(ph), _capi.ptr(k), _capi.ptr(o["mod"]),
+                                                None, _capi.MEM_DEVICE))
+                _capi.check(lib.psgc_reconstruct(ctx.h, C.byref(rcfg), _capi.ptr(o["lph"]),
+                                                 _capi.ptr(o["rph"]), _capi.ptr(o["lk"]),
+                                                 _capi.ptr(o["rk"]), ROWS, COLS, 70,
+                                                 _capi.ptr(q), _capi.ptr(r), _capi.ptr(o["xyz"]),
+                                                 _capi.ptr(o["P"]), None, None,
+                                                 _capi.MEM_DEVICE))
+                _capi.check(lib.psgc_synchronize(ctx.h))
+
+        run_plain()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_plain()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1e3
+        plain_val = npix * len(sub) / (plain_ms * 1e-3) / 1e6
+        # the two paths must agree on the pairs both ran
+        mism = 0
+        maxerr = 0.0
+        for o, bo in zip(bufs, batch.outputs):
+            mism += int((o["P"] != bo["P"]).sum().item())
+            m = bo["P"] != 0
+            if m.any():
+                maxerr = max(maxerr, float((o["xyz"][m] - bo["xyz"][m]).abs().max().item()))
+        plain = {"mpts_per_s": round(plain_val, 3), "ms_per_pair": round(plain_ms / len(sub), 3),
+                 "pairs": len(sub), "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr}
+        ctx.set_kernels("tiled")
+        log(f"plain per-pair: {plain}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(pairs[0], dcfg, rcfg, args.cpu_sample_rows)
+        cpu = cpu_baseline(pairs[0], dcfg, rcfg, args.cpu_sample_rows, log)
 
     line = {
         "metric": METRIC,
@@ -169,34 +195,36 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 decode/match/triangulation, f32 cloud filters",
+        "dtype": "f64",
         "data": "synthetic (seeded bump height fields rendered on the GPU)",
         "config": {"workload": f"{args.pairs} seeded 2448x2048 pairs, 3-freq 70/64/59 4-step "
-                               "u16, SOR k=30, MLS order 2 r=1.5 mm, Gaussian 5x5 s=1",
-                   "pairs": args.pairs, "path": args.path, "rows": ROWS, "cols": COLS},
-        "valid_mpts_per_s": round(valid / args.steps / (ms_per_step * 1e-3) / 1e6 * args.steps, 3)
-        if False else round(valid / (ms_per_step * 1e-3) / 1e6, 3),
-        "stage_ms_per_step": breakdown,
+                               "u16 12-bit; median 5x5, Gaussian 5x5 s=1, SOR k=30 (9x9 window), "
+                               "MLS order 2 r=1.5 mm (13x13 window, 2 iterations)",
+                   "pairs": args.pairs, "rows": ROWS, "cols": COLS, "path": "batched graph"},
+        "valid_mpts_per_s": round(valid / (ms_per_step * 1e-3) / 1e6, 3),
+        "per_pair_plain": plain,
+        "speedup_vs_per_pair": round(value / plain["mpts_per_s"], 2) if plain else None,
+        "stage_ms_per_step": stage_ms,
+        "autotuned_grids": grids,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "avg_launch_ms": round(avg_ms, 4),
-                     "alg_bytes_per_launch": alg},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    batch.close()
 
 
-def cpu_baseline(pair, dcfg, rcfg, sample_rows):
-    """The CPU restatement of the reference path (oracle/), timed on the
-    host's cores over one pair (rows [0, sample_rows))."""
-    import numpy as np
-
+def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
+    """The CPU restatement of the reference path (oracle/, OpenMP), timed on
+    this host's cores over a row band of one pair."""
     import oracle
     cores = os.cpu_count() or 1
     left = [f[:sample_rows].cpu().numpy() for f in pair.left]
     right = [f[:sample_rows].cpu().numpy() for f in pair.right]
-    print(f"[bench] cpu baseline: one pair, {sample_rows} rows, {cores} threads", file=sys.stderr,
-          flush=True)
+    log(f"cpu baseline: one pair, rows [0,{sample_rows}), {cores} threads")
+    oracle.decode(dcfg, [f[:8] for f in left])  # load + thread pool warm-up
     t0 = time.perf_counter()
     L = oracle.decode(dcfg, left)
     R = oracle.decode(dcfg, right)
@@ -204,7 +232,8 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows):
     dt = time.perf_counter() - t0
     pts = sample_rows * COLS
     return {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
-            "sample": f"1 pair, {sample_rows}x{COLS} px, decode L+R + reconstruct, {dt:.1f} s"}
+            "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
+                      f"decode L+R + reconstruct, {dt:.1f} s"}
 
 
 if __name__ == "__main__":

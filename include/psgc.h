@@ -279,6 +279,81 @@ PSGC_API int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1,
                        double cx2, double baseline, double Q[16],
                        double R1[9]);
 
+/* ---- kernel selection -----------------------------------------------------
+ * PSGC_KERNELS_TILED (default): LDS-tiled kernels. PSGC_KERNELS_PLAIN: one
+ * thread per pixel in 16x16 workgroups reading HBM directly, one launch per
+ * stage, which is the structure of the reference GPU path and serves as the
+ * per-pair baseline. Both produce the same integer outputs. */
+enum { PSGC_KERNELS_TILED = 0, PSGC_KERNELS_PLAIN = 1 };
+PSGC_API int psgc_set_kernels(psgc_ctx* ctx, int which);
+
+/* ---- batched pipeline ------------------------------------------------------
+ * Decode + reconstruct of many stereo pairs of uneven size, step count and
+ * unwrapping schedule, one launch per stage for the whole batch (persistent,
+ * XCD-aware grids over the concatenated tiles of all pairs), optionally
+ * captured once into a hipGraph that psgc_batch_run replays. No counterpart
+ * in the reference, which processes one pair per call
+ * (JSLAB_DecodePhase_GPU + JSLAB_ReconstructPipeline_GPU). All pointers in a
+ * pair descriptor are device pointers and must stay valid while the batch
+ * exists; outputs are overwritten by every run. */
+typedef struct psgc_pair_desc {
+    const void* left[64];          /* frames of the left camera            */
+    const void* right[64];         /* frames of the right camera           */
+    int32_t n_frames;              /* per camera                           */
+    int32_t sample_type;           /* PSGC_SAMPLE_*                        */
+    int64_t row_stride_bytes;      /* 0 = dense                            */
+    int32_t rows, cols;
+    psgc_decode_config decode;
+    psgc_recon_config recon;
+    int32_t pha_f;                 /* fringe orders (reference: freq[0])   */
+    double Q[16];
+    double R1[9];
+    float* xyz;                    /* rows*cols*3 f32                      */
+    uint8_t* P;                    /* rows*cols u8                         */
+    float* disp;                   /* optional f32 disparity (after median) */
+} psgc_pair_desc;
+
+enum {
+    PSGC_BATCH_GRAPH = 1,          /* capture the launches into a hipGraph  */
+    PSGC_BATCH_AUTOTUNE = 2        /* time launch shapes per resolution     */
+};
+
+typedef struct psgc_batch psgc_batch;
+PSGC_API int psgc_batch_create(psgc_ctx* ctx, const psgc_pair_desc* pairs, int n_pairs,
+                               int flags, psgc_batch** out);
+PSGC_API int psgc_batch_run(psgc_batch* batch);
+PSGC_API int psgc_batch_destroy(psgc_batch* batch);
+/* Runs the batch's launches directly on the stream (no graph), so that
+ * per-kernel instrumentation (psgc_profile_*) can see each launch. */
+PSGC_API int psgc_batch_run_eager(psgc_batch* batch);
+/* Launch shape chosen for stage `stage` ("decode", "match", "median5",
+ * "repro_gauss", "sor_knn", "mls"): workgroups of the persistent grid. */
+PSGC_API int psgc_batch_grid(psgc_batch* batch, const char* stage, int* grid);
+
+/* ---- differentiable triangulation ----------------------------------------
+ * Rectification of raw phase maps under a calibration theta, the sub-pixel
+ * root (JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003) with the integer
+ * match held fixed, and the Q triangulation (:430-438), in FP64, with its
+ * hand-written adjoint. No counterpart in the reference (it has no
+ * gradients); used for calibration refinement.
+ *   theta[9]  = f, cx1, cy1, cx2, cy2, baseline, wx, wy, wz (rad; right
+ *               rectification rotation Rz(wz) Ry(wy) Rx(wx))
+ *   nominal[5]= f0, c0Lx, c0Ly, c0Rx, c0Ry: intrinsics of the raw maps
+ *   match_col = integer right column per left rectified pixel (-1: none)
+ *   xyz (rows*cols*3 f64, rectified left frame), valid (u8, may be NULL)
+ * The backward pass takes dL/dxyz and returns dL/dtheta (9 f64) and, when
+ * the pointers are non-NULL, dL/dphi for both raw maps (f64, overwritten).
+ * theta and nominal are host arrays; grad_theta lives where `mem` says. */
+PSGC_API int psgc_adjoint_forward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
+                                  const double* phL, const double* phR,
+                                  const int32_t* match_col, int rows, int cols, double* xyz,
+                                  uint8_t* valid, int mem);
+PSGC_API int psgc_adjoint_backward(psgc_ctx* ctx, const double theta[9],
+                                   const double nominal[5], const double* phL,
+                                   const double* phR, const int32_t* match_col, int rows,
+                                   int cols, const double* grad_xyz, double* grad_theta,
+                                   double* grad_phL, double* grad_phR, int mem);
+
 /* ---- instrumentation ------------------------------------------------------
  * Per-kernel-family timing with hipEvents recorded on the launch stream.
  * psgc_profile_enable(1) clears and starts recording; psgc_profile_read

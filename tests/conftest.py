@@ -16,3 +16,11 @@ def pytest_configure(config):
 def ctx():
     import zim_amd
     return zim_amd.Context(0)
+
+
+@pytest.fixture(scope="session", params=["tiled", "plain"])
+def kctx(request):
+    import zim_amd
+    c = zim_amd.Context(0)
+    c.set_kernels(request.param)
+    return c

@@ -98,3 +98,29 @@ def test_rectify_ideal_closed_form():
     assert abs(Q[11] / w - 2400.0 * 150.0 / (d - 23.5)) < 1e-9
     assert lib.psgc_rectify_ideal(0.0, 1.0, 0, 0, 0, 1.0, _capi.ptr(Q), _capi.ptr(R1)) \
         == _capi.PSGC_E_INVALID
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """include/psgc.h is valid C99 and C++17 and links against the library
+    (the integration header compiles without OpenCV too)."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.join(ROOT, "zim_amd")
+    c_src = tmp_path / "t.c"
+    c_src.write_text('#include "psgc.h"\n#include <stdio.h>\nint main(void){\n'
+                     '  psgc_decode_config d; psgc_decode_config_default(&d);\n'
+                     '  printf("%d %d\\n", psgc_abi_version(), d.shift_num[0]);\n'
+                     '  return psgc_device_count() < 0; }\n')
+    exe = tmp_path / "t"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", f"-I{inc}", str(c_src), "-o",
+                        str(exe), f"-L{libdir}", "-l:libpsgc.so", f"-Wl,-rpath,{libdir}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.split() == ["1", "8"], out
+    cpp = tmp_path / "t.cpp"
+    cpp.write_text('#include "psgc_jslab.hpp"\nint main(){ return psgc_abi_version() == 1 ? 0 : 1; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{inc}", str(cpp), "-o",
+                        str(tmp_path / "tpp"), f"-L{libdir}", "-l:libpsgc.so",
+                        f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -49,7 +49,8 @@ def _assert_decode(ref, got):
 
 
 @pytest.mark.parametrize("seed", [3, 11])
-def test_decode_c2_u16(ctx, seed):
+def test_decode_c2_u16(kctx, seed):
+    ctx = kctx
     _, left, right = scenes.c2_small(seed)
     cfg = scenes.c2_decode_config()
     for frames in (left, right):
@@ -58,7 +59,8 @@ def test_decode_c2_u16(ctx, seed):
         assert ref[3].mean() > 0.5  # the scene decodes
 
 
-def test_decode_variants(ctx):
+def test_decode_variants(kctx):
+    ctx = kctx
     _, left, _ = scenes.c2_small(3)
     for kw in (dict(avg_three_phase=0), dict(post_smooth=0), dict(preblur_sigma=0.0),
                dict(modulation_threshold=0.6)):
@@ -67,7 +69,8 @@ def test_decode_variants(ctx):
         _assert_decode(ref, got)
 
 
-def test_decode_f64_and_u8_inputs(ctx):
+def test_decode_f64_and_u8_inputs(kctx):
+    ctx = kctx
     _, left, _ = scenes.c2_small(3)
     cfg = scenes.c2_decode_config()
     f64 = [f.astype(np.float64) for f in left]
@@ -78,7 +81,8 @@ def test_decode_f64_and_u8_inputs(ctx):
     _assert_decode(ref, got)
 
 
-def test_decode_gray_code(ctx):
+def test_decode_gray_code(kctx):
+    ctx = kctx
     _, left, right = scenes.c1_small(5)
     cfg = scenes.c1_gray_config()
     for frames in (left, right):
@@ -199,7 +203,8 @@ def test_mls(ctx, order, sem):
 
 
 @pytest.mark.parametrize("variant", ["cpu_ref", "gpu_ref", "c2"])
-def test_reconstruct_chain(ctx, variant):
+def test_reconstruct_chain(kctx, variant):
+    ctx = kctx
     p, L, R = _decoded_pair(11)
     if variant == "cpu_ref":
         cfg = zim_amd.ReconConfig.default(radius_search=1.0, radius_min_neighbors=6)
@@ -230,7 +235,8 @@ def test_error_paths(ctx):
         zim_amd.decode(cfg, left[:-1], ctx)
 
 
-def test_tiny_and_degenerate_images(ctx):
+def test_tiny_and_degenerate_images(kctx):
+    ctx = kctx
     cfg = scenes.c2_decode_config()
     for rows, cols in ((1, 1), (1, 7), (5, 1), (3, 3)):
         g = np.random.default_rng(rows * 100 + cols)
@@ -243,3 +249,50 @@ def test_tiny_and_degenerate_images(ctx):
     flat = [np.full((16, 16), 1000, np.uint16) for _ in range(12)]
     ph, k, mod, v = zim_amd.decode(cfg, flat, ctx)
     assert v.sum() == 0 and k.sum() == 0
+
+
+def _pair_inputs(seed, rows, cols):
+    p, left, right = scenes.c2_small(seed, rows=rows, cols=cols)
+    return p, [dev(f) for f in left], [dev(f) for f in right], left, right
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_uneven_pairs_match_oracle(ctx, graph):
+    """Two pairs of different sizes (and a CPU-semantics ROR pair) in one
+    batch; each must equal the CPU restatement run on that pair alone."""
+    dcfg = scenes.c2_decode_config()
+    specs = [(21, 96, 128, scenes.c2_recon_config(mls_window_size=4)),
+             (22, 80, 200, zim_amd.ReconConfig.default(radius_search=1.0, radius_min_neighbors=6)),
+             (23, 64, 96, scenes.c2_recon_config(use_mls=0))]
+    pairs, refs = [], []
+    for seed, rows, cols, rcfg in specs:
+        p, L, R, lh, rh = _pair_inputs(seed, rows, cols)
+        pairs.append(dict(left=L, right=R, decode=dcfg, recon=rcfg, pha_f=70, Q=p.Q, R1=p.R1,
+                          disp=True))
+        Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+        refs.append(oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1))
+    b = zim_amd.Batch(pairs, ctx, graph=graph)
+    for rep in range(2):   # replays must be identical
+        outs = b.run()
+        torch.cuda.synchronize()
+        for out, (rx, rP, rd, _) in zip(outs, refs):
+            assert np.array_equal(host(out["P"]), rP), f"mask mismatch {(host(out['P']) != rP).sum()}"
+            assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
+            assert np.abs(host(out["disp"]) - rd).max() <= DISP_TOL
+    b.close()
+
+
+def test_batch_autotune_keeps_results(ctx):
+    dcfg = scenes.c2_decode_config()
+    p, L, R, lh, rh = _pair_inputs(31, 96, 128)
+    rcfg = scenes.c2_recon_config(mls_window_size=4)
+    b = zim_amd.Batch([dict(left=L, right=R, decode=dcfg, recon=rcfg, pha_f=70, Q=p.Q, R1=p.R1)],
+                      ctx, graph=True, autotune=True)
+    out = b.run()[0]
+    torch.cuda.synchronize()
+    Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+    rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
+    assert np.array_equal(host(out["P"]), rP)
+    assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
+    assert b.grid("mls") > 0 and b.grid("decode") > 0
+    b.close()

@@ -401,3 +401,34 @@ def test_reconstruct_recovers_rendered_depth():
     assert m.mean() > 0.3
     err = np.abs(xyz[..., 2][m] - z[m])
     assert np.median(err) < 0.2 and np.percentile(err, 99) < 1.5
+
+
+def test_adjoint_reference_model_at_true_calibration_equals_reprojection():
+    """At theta0 the rectification remaps are identities, so the FP64
+    differentiable model must reproduce the oracle's sub-pixel root and Q
+    reprojection of the same integer matches (pre-median)."""
+    import torch
+    from tests import adjoint_ref
+    from zim_amd import synth
+    p = synth.render_c4(4, rows=96, cols=160, f=1600.0 * 160 / 1280.0)
+    cfg = scenes.c2_decode_config()
+    L = oracle.decode(cfg, [f.numpy() for f in p.left])
+    R = oracle.decode(cfg, [f.numpy() for f in p.right])
+    disp, mc = oracle.stereo_match(L[0], R[0], L[1], R[1], 70, 1.0)
+    cam = p.cam
+    theta0 = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy, cam.baseline, 0, 0, 0],
+                          dtype=torch.float64)
+    nominal = [cam.f, cam.cx, cam.cy, cam.cx, cam.cy]
+    xyz, ok = adjoint_ref.forward(theta0, nominal, torch.from_numpy(L[0]), torch.from_numpy(R[0]),
+                                  torch.from_numpy(mc))
+    rx, rP = oracle.reproject(disp, p.Q, p.R1, 0, 1e9, 1)
+    m = ok.numpy() & (rP > 0)
+    assert m.sum() > 0.3 * m.size
+    assert np.abs(xyz.numpy()[m] - rx[m]).max() < 1e-3
+    # and it is differentiable: autograd of the plane loss is finite
+    th = theta0.clone().requires_grad_(True)
+    loss = adjoint_ref.plane_loss(*adjoint_ref.forward(th, nominal, torch.from_numpy(L[0]),
+                                                       torch.from_numpy(R[0]),
+                                                       torch.from_numpy(mc)))
+    loss.backward()
+    assert torch.isfinite(th.grad).all()

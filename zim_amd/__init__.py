@@ -27,7 +27,7 @@ from ._capi import (DecodeConfig, ReconConfig, PsgcError, MEM_DEVICE, MEM_HOST, 
                     SEM_CPU, SEM_GPU, OUTLIER_NONE, OUTLIER_ROR, OUTLIER_SOR,
                     UNWRAP_HETERODYNE3, UNWRAP_GRAYCODE)
 
-__all__ = ["Context", "DecodeConfig", "ReconConfig", "PhaseShiftDecoder", "Version",
+__all__ = ["Batch", "Context", "DecodeConfig", "ReconConfig", "PhaseShiftDecoder", "Version",
            "PhaseEpipolarReconstructor_V2", "applyMLSFilterToPointCloud", "decode",
            "reconstruct", "PsgcError"]
 
@@ -46,6 +46,13 @@ class Context:
         _capi.check(self.lib.psgc_create(C.byref(h), device))
         self.h = h
         self._bound_stream = None
+
+    def set_kernels(self, which: str):
+        """"tiled" (LDS-tiled kernels, the default) or "plain" (one thread
+        per pixel, the reference GPU path's structure)."""
+        code = {"tiled": _capi.KERNELS_TILED, "plain": _capi.KERNELS_PLAIN}[which]
+        _capi.check(self.lib.psgc_set_kernels(self.h, code))
+        return self
 
     def bind_stream(self):
         s = torch.cuda.current_stream(self.device)
@@ -141,6 +148,80 @@ def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1, ctx: Context | N
                                          _capi.ptr(P), _capi.ptr(disp), _capi.ptr(mc),
                                          MEM_DEVICE if dev else MEM_HOST))
     return xyz, P, disp, mc
+
+
+class Batch:
+    """Batched decode + reconstruct of many stereo pairs (psgc_batch_*).
+
+    pairs: list of dicts with keys left, right (lists of device frame
+    tensors), decode (DecodeConfig), recon (ReconConfig), pha_f, Q, R1 and
+    optionally disp=True. Outputs (xyz, P[, disp]) are allocated here and
+    overwritten by every run()."""
+
+    def __init__(self, pairs, ctx: Context | None = None, graph=True, autotune=False):
+        self.ctx = ctx or default_context()
+        self.ctx.bind_stream()
+        n = len(pairs)
+        descs = (_capi.PairDesc * n)()
+        self.outputs = []
+        self._keep = []
+        for i, p in enumerate(pairs):
+            L, R = p["left"], p["right"]
+            rows, cols = L[0].shape
+            d = descs[i]
+            for j, f in enumerate(L):
+                d.left[j] = f.data_ptr()
+            for j, f in enumerate(R):
+                d.right[j] = f.data_ptr()
+            d.n_frames = len(L)
+            d.sample_type = _SAMPLE[L[0].dtype]
+            d.row_stride_bytes = L[0].stride(0) * L[0].element_size()
+            d.rows, d.cols = rows, cols
+            d.decode = p["decode"]
+            d.recon = p["recon"]
+            d.pha_f = int(p["pha_f"])
+            d.Q[:] = [float(v) for v in np.asarray(p["Q"]).reshape(16)]
+            d.R1[:] = [float(v) for v in np.asarray(p["R1"]).reshape(9)]
+            dev = L[0].device
+            out = {"xyz": torch.empty((rows, cols, 3), dtype=torch.float32, device=dev),
+                   "P": torch.empty((rows, cols), dtype=torch.uint8, device=dev)}
+            if p.get("disp"):
+                out["disp"] = torch.empty((rows, cols), dtype=torch.float32, device=dev)
+            d.xyz = out["xyz"].data_ptr()
+            d.P = out["P"].data_ptr()
+            d.disp = out["disp"].data_ptr() if "disp" in out else None
+            self.outputs.append(out)
+            self._keep.append((L, R))
+        flags = (_capi.BATCH_GRAPH if graph else 0) | (_capi.BATCH_AUTOTUNE if autotune else 0)
+        h = C.c_void_p()
+        _capi.check(self.ctx.lib.psgc_batch_create(self.ctx.h, descs, n, flags, C.byref(h)))
+        self.h = h
+
+    def run(self):
+        self.ctx.bind_stream()
+        _capi.check(self.ctx.lib.psgc_batch_run(self.h))
+        return self.outputs
+
+    def run_eager(self):
+        self.ctx.bind_stream()
+        _capi.check(self.ctx.lib.psgc_batch_run_eager(self.h))
+        return self.outputs
+
+    def grid(self, stage: str) -> int:
+        g = C.c_int()
+        _capi.check(self.ctx.lib.psgc_batch_grid(self.h, stage.encode(), C.byref(g)))
+        return g.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.psgc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------ reference-shaped classes --

@@ -134,6 +134,33 @@ class ReconConfig(C.Structure):
         return c
 
 
+KERNELS_TILED = 0
+KERNELS_PLAIN = 1
+
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
+
+
+class PairDesc(C.Structure):
+    _fields_ = [
+        ("left", C.c_void_p * 64),
+        ("right", C.c_void_p * 64),
+        ("n_frames", C.c_int32),
+        ("sample_type", C.c_int32),
+        ("row_stride_bytes", C.c_int64),
+        ("rows", C.c_int32),
+        ("cols", C.c_int32),
+        ("decode", DecodeConfig),
+        ("recon", ReconConfig),
+        ("pha_f", C.c_int32),
+        ("Q", C.c_double * 16),
+        ("R1", C.c_double * 9),
+        ("xyz", C.c_void_p),
+        ("P", C.c_void_p),
+        ("disp", C.c_void_p),
+    ]
+
+
 def ptr(x):
     """Address of a numpy array, a torch tensor, or None."""
     if x is None:
@@ -186,6 +213,14 @@ def load(path: str | None = None):
                             i32),
         "psgc_rectify_ideal": ([dbl, dbl, dbl, dbl, dbl, dbl, vp, vp], i32),
         "psgc_profile_enable": ([i32], i32),
+        "psgc_set_kernels": ([vp, i32], i32),
+        "psgc_adjoint_forward": ([vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, i32], i32),
+        "psgc_adjoint_backward": ([vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, i32], i32),
+        "psgc_batch_create": ([vp, vp, i32, i32, C.POINTER(vp)], i32),
+        "psgc_batch_run": ([vp], i32),
+        "psgc_batch_run_eager": ([vp], i32),
+        "psgc_batch_destroy": ([vp], i32),
+        "psgc_batch_grid": ([vp, C.c_char_p, C.POINTER(C.c_int)], i32),
         "psgc_profile_read": ([C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)], i32),
     }
     for name, (args, res) in sig.items():

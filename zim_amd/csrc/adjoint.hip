@@ -1,0 +1,304 @@
+// adjoint.hip — differentiable rectification + sub-pixel root + Q
+// triangulation, and its hand-written adjoint.
+//
+// Model (no reference counterpart; it differentiates the reference's
+// sub-pixel root, JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003, and Q
+// reprojection, :430-438, through a calibration-dependent rectification):
+//   theta = (f, cx1, cy1, cx2, cy2, B, wx, wy, wz). The raw phase maps were
+//   taken by cameras with intrinsics (f0, c0L) and (f0, c0R) (`nominal`).
+//   Left rectified pixel (j, i) samples the raw left map at
+//     u = f0/f (j - cx1) + c0Lx,  v = f0/f (i - cy1) + c0Ly;
+//   right rectified pixel (x, i) samples the raw right map at the
+//   projection of R(w) K_R^-1 (x, i, 1), R(w) = Rz(wz) Ry(wy) Rx(wx),
+//   K_R = [f 0 cx2; 0 f cy2; 0 0 1], with the raw intrinsics (f0, c0R).
+//   Bilinear sampling; taps outside the map invalidate the point.
+//   With the integer match b fixed, c_m = phiR(i, b-1+m) - phiL(i, j),
+//   delta = c1 (c0 - c2) / (2 (c0 - c1)(c1 - c2)), d = j - b - delta and
+//   X = (j - cx1) B / D, Y = (i - cy1) B / D, Z = f B / D, D = d - (cx1 - cx2)
+//   (Q with R1 = I). Everything is FP64.
+// The adjoint accumulates dL/dtheta (per-workgroup reduction, then f64
+// atomics) and scatters dL/dphi into the raw maps with f64 atomics.
+
+#include "psgc_internal.h"
+
+namespace psgc {
+namespace {
+
+struct Theta {
+  double f, cx1, cy1, cx2, cy2, B, wx, wy, wz;
+};
+struct Nominal {
+  double f0, cLx, cLy, cRx, cRy;
+};
+
+__device__ inline void rot(const Theta& t, double R[3][3], double dR[3][3][3]) {
+  const double cx = cos(t.wx), sx = sin(t.wx), cy = cos(t.wy), sy = sin(t.wy), cz = cos(t.wz),
+               sz = sin(t.wz);
+  const double Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+  const double Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+  const double Rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+  const double dRx[3][3] = {{0, 0, 0}, {0, -sx, -cx}, {0, cx, -sx}};
+  const double dRy[3][3] = {{-sy, 0, cy}, {0, 0, 0}, {-cy, 0, -sy}};
+  const double dRz[3][3] = {{-sz, -cz, 0}, {cz, -sz, 0}, {0, 0, 0}};
+  auto mul3 = [](const double A[3][3], const double Bm[3][3], const double Cm[3][3],
+                 double O[3][3]) {
+    double T[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0;
+        for (int k = 0; k < 3; ++k) s += Bm[i][k] * Cm[k][j];
+        T[i][j] = s;
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0;
+        for (int k = 0; k < 3; ++k) s += A[i][k] * T[k][j];
+        O[i][j] = s;
+      }
+  };
+  mul3(Rz, Ry, Rx, R);
+  mul3(Rz, Ry, dRx, dR[0]);
+  mul3(Rz, dRy, Rx, dR[1]);
+  mul3(dRz, Ry, Rx, dR[2]);
+}
+
+// Bilinear sample of map m at (u, v); returns false if a tap is outside.
+// w: tap weights, idx: tap indices, du/dv: partial derivatives.
+struct Bil {
+  double val, du, dv;
+  double w[4];
+  size_t idx[4];
+};
+
+__device__ inline bool bilinear(const double* m, int rows, int cols, double u, double v, Bil* b) {
+  if (!(u >= 0.0 && u <= (double)(cols - 1) && v >= 0.0 && v <= (double)(rows - 1))) return false;
+  int x0 = (int)floor(u), y0 = (int)floor(v);
+  if (x0 > cols - 2) x0 = cols >= 2 ? cols - 2 : 0;
+  if (y0 > rows - 2) y0 = rows >= 2 ? rows - 2 : 0;
+  const int x1 = min(x0 + 1, cols - 1), y1 = min(y0 + 1, rows - 1);
+  const double fu = (double)x0, fv = (double)y0;
+  const double a = u - fu, c = v - fv;
+  b->idx[0] = (size_t)y0 * cols + x0;
+  b->idx[1] = (size_t)y0 * cols + x1;
+  b->idx[2] = (size_t)y1 * cols + x0;
+  b->idx[3] = (size_t)y1 * cols + x1;
+  const double p00 = m[b->idx[0]], p10 = m[b->idx[1]], p01 = m[b->idx[2]], p11 = m[b->idx[3]];
+  b->w[0] = (1 - a) * (1 - c);
+  b->w[1] = a * (1 - c);
+  b->w[2] = (1 - a) * c;
+  b->w[3] = a * c;
+  b->val = b->w[0] * p00 + b->w[1] * p10 + b->w[2] * p01 + b->w[3] * p11;
+  b->du = (1 - c) * (p10 - p00) + c * (p11 - p01);
+  b->dv = (1 - a) * (p01 - p00) + a * (p11 - p10);
+  return true;
+}
+
+// Right remap of rectified (x, i): raw coordinates and their derivatives
+// with respect to theta (indices 0..8).
+__device__ inline void right_map(const Theta& t, const Nominal& n, const double R[3][3],
+                                 const double dR[3][3][3], double x, double i, double* u,
+                                 double* v, double du[9], double dv[9]) {
+  const double r[3] = {(x - t.cx2) / t.f, (i - t.cy2) / t.f, 1.0};
+  double q[3];
+  for (int a = 0; a < 3; ++a) q[a] = R[a][0] * r[0] + R[a][1] * r[1] + R[a][2] * r[2];
+  *u = n.f0 * q[0] / q[2] + n.cRx;
+  *v = n.f0 * q[1] / q[2] + n.cRy;
+  // d(u, v)/dq
+  const double iz = 1.0 / q[2];
+  const double duq[3] = {n.f0 * iz, 0.0, -n.f0 * q[0] * iz * iz};
+  const double dvq[3] = {0.0, n.f0 * iz, -n.f0 * q[1] * iz * iz};
+  for (int k = 0; k < 9; ++k) du[k] = dv[k] = 0.0;
+  // dr/df = -(r0, r1, 0)/f, dr/dcx2 = (-1/f, 0, 0), dr/dcy2 = (0, -1/f, 0)
+  const double drs[3][3] = {{-r[0] / t.f, -r[1] / t.f, 0.0}, {-1.0 / t.f, 0, 0}, {0, -1.0 / t.f, 0}};
+  const int idx[3] = {0, 3, 4};
+  for (int s = 0; s < 3; ++s) {
+    double dq[3];
+    for (int a = 0; a < 3; ++a) dq[a] = R[a][0] * drs[s][0] + R[a][1] * drs[s][1] + R[a][2] * drs[s][2];
+    du[idx[s]] = duq[0] * dq[0] + duq[1] * dq[1] + duq[2] * dq[2];
+    dv[idx[s]] = dvq[0] * dq[0] + dvq[1] * dq[1] + dvq[2] * dq[2];
+  }
+  for (int s = 0; s < 3; ++s) {
+    double dq[3];
+    for (int a = 0; a < 3; ++a)
+      dq[a] = dR[s][a][0] * r[0] + dR[s][a][1] * r[1] + dR[s][a][2] * r[2];
+    du[6 + s] = duq[0] * dq[0] + duq[1] * dq[1] + duq[2] * dq[2];
+    dv[6 + s] = dvq[0] * dq[0] + dvq[1] * dq[1] + dvq[2] * dq[2];
+  }
+}
+
+struct PointEval {
+  bool ok;
+  double xyz[3];
+  // for the adjoint
+  double dd_dc[3];        // d d / d c_m
+  double dxyz_dd[3];
+  double dxyz_dth[3][9];  // direct dependence through Q
+  Bil bl, br[3];
+  double dul[9], dvl[9], dur[3][9], dvr[3][9];
+};
+
+__device__ inline void eval_point(const Theta& t, const Nominal& n, const double R[3][3],
+                                  const double dR[3][3][3], const double* phL, const double* phR,
+                                  int rows, int cols, int i, int j, int b, PointEval* e) {
+  e->ok = false;
+  if (b < 1 || b > cols - 2) return;
+  const double ul = n.f0 / t.f * (j - t.cx1) + n.cLx;
+  const double vl = n.f0 / t.f * (i - t.cy1) + n.cLy;
+  if (!bilinear(phL, rows, cols, ul, vl, &e->bl)) return;
+  for (int k = 0; k < 9; ++k) e->dul[k] = e->dvl[k] = 0.0;
+  e->dul[0] = -n.f0 / (t.f * t.f) * (j - t.cx1);
+  e->dvl[0] = -n.f0 / (t.f * t.f) * (i - t.cy1);
+  e->dul[1] = -n.f0 / t.f;
+  e->dvl[2] = -n.f0 / t.f;
+  double c[3];
+  for (int m = 0; m < 3; ++m) {
+    double u, v;
+    right_map(t, n, R, dR, (double)(b - 1 + m), (double)i, &u, &v, e->dur[m], e->dvr[m]);
+    if (!bilinear(phR, rows, cols, u, v, &e->br[m])) return;
+    c[m] = e->br[m].val - e->bl.val;
+  }
+  const double N = c[1] * (c[0] - c[2]);
+  const double Dn = 2.0 * (c[0] - c[1]) * (c[1] - c[2]);
+  if (!(fabs(Dn) > 1e-9)) return;
+  const double delta = N / Dn;
+  const double dN[3] = {c[1], c[0] - c[2], -c[1]};
+  const double dD[3] = {2.0 * (c[1] - c[2]), 2.0 * (c[0] - 2.0 * c[1] + c[2]), -2.0 * (c[0] - c[1])};
+  for (int m = 0; m < 3; ++m) e->dd_dc[m] = -(dN[m] * Dn - N * dD[m]) / (Dn * Dn);
+  const double d = (double)j - (double)b - delta;
+  const double D = d - (t.cx1 - t.cx2);
+  if (!(D > 0.0)) return;
+  const double X = (j - t.cx1) * t.B / D, Y = (i - t.cy1) * t.B / D, Z = t.f * t.B / D;
+  e->xyz[0] = X;
+  e->xyz[1] = Y;
+  e->xyz[2] = Z;
+  const double iD = 1.0 / D, iD2 = iD * iD;
+  e->dxyz_dd[0] = -(j - t.cx1) * t.B * iD2;
+  e->dxyz_dd[1] = -(i - t.cy1) * t.B * iD2;
+  e->dxyz_dd[2] = -t.f * t.B * iD2;
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < 9; ++k) e->dxyz_dth[a][k] = 0.0;
+  // f
+  e->dxyz_dth[2][0] = t.B * iD;
+  // cx1: D depends on -cx1
+  e->dxyz_dth[0][1] = -t.B * iD + (j - t.cx1) * t.B * iD2;
+  e->dxyz_dth[1][1] = (i - t.cy1) * t.B * iD2;
+  e->dxyz_dth[2][1] = t.f * t.B * iD2;
+  // cy1
+  e->dxyz_dth[1][2] = -t.B * iD;
+  // cx2
+  e->dxyz_dth[0][3] = -(j - t.cx1) * t.B * iD2;
+  e->dxyz_dth[1][3] = -(i - t.cy1) * t.B * iD2;
+  e->dxyz_dth[2][3] = -t.f * t.B * iD2;
+  // B
+  e->dxyz_dth[0][5] = (j - t.cx1) * iD;
+  e->dxyz_dth[1][5] = (i - t.cy1) * iD;
+  e->dxyz_dth[2][5] = t.f * iD;
+  e->ok = true;
+}
+
+__global__ void adj_forward_kernel(Theta t, Nominal n, const double* __restrict__ phL,
+                                   const double* __restrict__ phR,
+                                   const int32_t* __restrict__ mc, int rows, int cols,
+                                   double* __restrict__ xyz, uint8_t* __restrict__ valid) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (size_t)rows * cols) return;
+  const int i = (int)(p / cols), j = (int)(p % cols);
+  double R[3][3], dR[3][3][3];
+  rot(t, R, dR);
+  PointEval e;
+  eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
+  xyz[p * 3 + 0] = e.ok ? e.xyz[0] : 0.0;
+  xyz[p * 3 + 1] = e.ok ? e.xyz[1] : 0.0;
+  xyz[p * 3 + 2] = e.ok ? e.xyz[2] : 0.0;
+  if (valid) valid[p] = e.ok ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) adj_backward_kernel(
+    Theta t, Nominal n, const double* __restrict__ phL, const double* __restrict__ phR,
+    const int32_t* __restrict__ mc, int rows, int cols, const double* __restrict__ gxyz,
+    double* __restrict__ gth, double* __restrict__ gphL, double* __restrict__ gphR) {
+  __shared__ double red[9][256];
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < (size_t)rows * cols) {
+    const int i = (int)(p / cols), j = (int)(p % cols);
+    double R[3][3], dR[3][3][3];
+    rot(t, R, dR);
+    PointEval e;
+    eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
+    if (e.ok) {
+      const double g[3] = {gxyz[p * 3], gxyz[p * 3 + 1], gxyz[p * 3 + 2]};
+      const double gd = g[0] * e.dxyz_dd[0] + g[1] * e.dxyz_dd[1] + g[2] * e.dxyz_dd[2];
+      for (int k = 0; k < 9; ++k)
+        acc[k] = g[0] * e.dxyz_dth[0][k] + g[1] * e.dxyz_dth[1][k] + g[2] * e.dxyz_dth[2][k];
+      double gL = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        const double gc = gd * e.dd_dc[m];  // dL/dc_m
+        gL -= gc;
+        // through the right sample
+        for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * e.dur[m][k] + e.br[m].dv * e.dvr[m][k]);
+        if (gphR)
+          for (int q = 0; q < 4; ++q) atomicAdd(&gphR[e.br[m].idx[q]], gc * e.br[m].w[q]);
+      }
+      for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * e.dul[k] + e.bl.dv * e.dvl[k]);
+      if (gphL)
+        for (int q = 0; q < 4; ++q) atomicAdd(&gphL[e.bl.idx[q]], gL * e.bl.w[q]);
+    }
+  }
+  for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int h = 128; h >= 1; h /= 2) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 9; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) atomicAdd(&gth[threadIdx.x], red[threadIdx.x][0]);
+}
+
+__global__ void zero_f64_kernel(double* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+int launch_adj_forward(const double* theta, const double* nominal, const double* phL,
+                       const double* phR, const int32_t* mc, int rows, int cols, double* xyz,
+                       uint8_t* valid, hipStream_t s) {
+  Theta t{theta[0], theta[1], theta[2], theta[3], theta[4], theta[5], theta[6], theta[7], theta[8]};
+  Nominal n{nominal[0], nominal[1], nominal[2], nominal[3], nominal[4]};
+  const size_t np = (size_t)rows * cols;
+  PSGC_PROF_BEGIN("adj_forward", s);
+  adj_forward_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(t, n, phL, phR, mc, rows, cols,
+                                                                  xyz, valid);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("adj_forward", s);
+  return PSGC_OK;
+}
+
+int launch_adj_backward(const double* theta, const double* nominal, const double* phL,
+                        const double* phR, const int32_t* mc, int rows, int cols,
+                        const double* gxyz, double* gth, double* gphL, double* gphR,
+                        hipStream_t s) {
+  Theta t{theta[0], theta[1], theta[2], theta[3], theta[4], theta[5], theta[6], theta[7], theta[8]};
+  Nominal n{nominal[0], nominal[1], nominal[2], nominal[3], nominal[4]};
+  const size_t np = (size_t)rows * cols;
+  PSGC_PROF_BEGIN("adj_backward", s);
+  zero_f64_kernel<<<1, 64, 0, s>>>(gth, 9);
+  PSGC_LAUNCHED();
+  if (gphL) {
+    zero_f64_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(gphL, np);
+    PSGC_LAUNCHED();
+  }
+  if (gphR) {
+    zero_f64_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(gphR, np);
+    PSGC_LAUNCHED();
+  }
+  adj_backward_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(t, n, phL, phR, mc, rows, cols,
+                                                                   gxyz, gth, gphL, gphR);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("adj_backward", s);
+  return PSGC_OK;
+}
+
+}  // namespace psgc

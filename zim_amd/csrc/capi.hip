@@ -206,8 +206,10 @@ int check_size(int rows, int cols) {
   return PSGC_OK;
 }
 
+}  // namespace
+
 // Decode configuration -> kernel arguments (frames filled by the caller).
-int make_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
+int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
                      int cols, int64_t stride, DecodeArgs* a) {
   std::memset(a, 0, sizeof(*a));
   if (!cfg) PSGC_FAIL(PSGC_E_INVALID, "null decode config");
@@ -263,7 +265,6 @@ int make_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_typ
   return PSGC_OK;
 }
 
-}  // namespace
 }  // namespace psgc
 
 using namespace psgc;
@@ -390,6 +391,14 @@ int psgc_release(psgc_ctx* ctx) {
   return ctx_release(ctx);
 }
 
+int psgc_set_kernels(psgc_ctx* ctx, int which) {
+  PSGC_CTX_ENTER(ctx);
+  if (which != PSGC_KERNELS_TILED && which != PSGC_KERNELS_PLAIN)
+    PSGC_FAIL(PSGC_E_INVALID, "kernels=%d", which);
+  ctx->kernels = which;
+  return PSGC_OK;
+}
+
 int psgc_set_stream(psgc_ctx* ctx, void* s) {
   PSGC_CTX_ENTER(ctx);
   PSGC_HIP(hipStreamSynchronize(ctx->stream));
@@ -413,7 +422,7 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
   DecodeArgs a;
   const int sb = sample_bytes(sample_type);
   if (row_stride_bytes == 0) row_stride_bytes = (int64_t)cols * sb;
-  PSGC_TRY(make_decode_args(cfg, n_frames, sample_type, rows, cols, row_stride_bytes, &a));
+  PSGC_TRY(build_decode_args(cfg, n_frames, sample_type, rows, cols, row_stride_bytes, &a));
   if (row_stride_bytes < (int64_t)cols * sb)
     PSGC_FAIL(PSGC_E_INVALID, "row stride %lld below %d", (long long)row_stride_bytes, cols * sb);
   for (int f = 0; f < n_frames; ++f)
@@ -457,7 +466,10 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
   } else {
     d_raw = d_phase;
   }
-  PSGC_TRY(launch_decode_plain(a, d_raw, d_k, d_mod, d_valid, ctx->stream));
+  if (ctx->kernels == PSGC_KERNELS_PLAIN)
+    PSGC_TRY(launch_decode_plain(a, d_raw, d_k, d_mod, d_valid, ctx->stream));
+  else
+    PSGC_TRY(tiled_decode(ctx, a, d_raw, d_k, d_mod, d_valid, ctx->stream));
   if (gray) {
     // jump correction reads (raw, k) and writes (phase, k)
     void* kt;
@@ -699,6 +711,22 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   PSGC_TRY(stage_in(ctx, mem, S_RPH, r_phase, npix, &rp));
   PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
   PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  if (ctx->kernels == PSGC_KERNELS_TILED) {
+    float *d_xyz, *d_disp;
+    uint8_t* d_P;
+    int32_t* d_mc;
+    PSGC_TRY(stage_out(ctx, mem, S_TMP0, xyz, npix * 3, &d_xyz));
+    PSGC_TRY(stage_out(ctx, mem, S_TMP1, P, npix, &d_P));
+    PSGC_TRY(stage_out(ctx, mem, S_DISP2, disp, npix, &d_disp));
+    PSGC_TRY(stage_out(ctx, mem, S_MATCH, match_col, npix, &d_mc));
+    PSGC_TRY(tiled_reconstruct(ctx, *cfg, lp, rp, lk, rk, rows, cols, pha_f, Q, R1, d_xyz, d_P,
+                               d_disp, d_mc, ctx->stream));
+    PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, npix * 3));
+    PSGC_TRY(copy_back(ctx, mem, P, d_P, npix));
+    PSGC_TRY(copy_back(ctx, mem, disp, d_disp, npix));
+    PSGC_TRY(copy_back(ctx, mem, match_col, d_mc, npix));
+    return finish(ctx, mem);
+  }
   void *span, *d64, *d32, *d32m, *x0, *x1, *x2, *p0, *p1, *p2;
   PSGC_TRY(ctx_buf(ctx, S_SPAN, (size_t)rows * (pha_f + 1) * 2 * sizeof(int32_t), &span));
   PSGC_TRY(ctx_buf(ctx, S_DISP64, npix * sizeof(double), &d64));
@@ -786,6 +814,59 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, npix * 3));
   PSGC_TRY(copy_back(ctx, mem, P, d_P, npix));
   PSGC_TRY(copy_back(ctx, mem, match_col, d_mc, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_adjoint_forward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
+                         const double* phL, const double* phR, const int32_t* match_col,
+                         int rows, int cols, double* xyz, uint8_t* valid, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!theta || !nominal || !phL || !phR || !match_col || !xyz)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (!(theta[0] > 0) || !(nominal[0] > 0)) PSGC_FAIL(PSGC_E_INVALID, "focal must be > 0");
+  const size_t np = (size_t)rows * cols;
+  const double *l, *r;
+  const int32_t* m;
+  PSGC_TRY(stage_in(ctx, mem, S_LPH, phL, np, &l));
+  PSGC_TRY(stage_in(ctx, mem, S_RPH, phR, np, &r));
+  PSGC_TRY(stage_in(ctx, mem, S_MATCH, match_col, np, &m));
+  double* dx;
+  uint8_t* dv;
+  PSGC_TRY(stage_out(ctx, mem, S_ADJ0, xyz, np * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P0, valid, np, &dv));
+  PSGC_TRY(launch_adj_forward(theta, nominal, l, r, m, rows, cols, dx, dv, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, xyz, dx, np * 3));
+  PSGC_TRY(copy_back(ctx, mem, valid, dv, np));
+  return finish(ctx, mem);
+}
+
+int psgc_adjoint_backward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
+                          const double* phL, const double* phR, const int32_t* match_col,
+                          int rows, int cols, const double* grad_xyz, double* grad_theta,
+                          double* grad_phL, double* grad_phR, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!theta || !nominal || !phL || !phR || !match_col || !grad_xyz || !grad_theta)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (!(theta[0] > 0) || !(nominal[0] > 0)) PSGC_FAIL(PSGC_E_INVALID, "focal must be > 0");
+  const size_t np = (size_t)rows * cols;
+  const double *l, *r, *g;
+  const int32_t* m;
+  PSGC_TRY(stage_in(ctx, mem, S_LPH, phL, np, &l));
+  PSGC_TRY(stage_in(ctx, mem, S_RPH, phR, np, &r));
+  PSGC_TRY(stage_in(ctx, mem, S_MATCH, match_col, np, &m));
+  PSGC_TRY(stage_in(ctx, mem, S_ADJ0, grad_xyz, np * 3, &g));
+  double *gt, *gl, *gr;
+  PSGC_TRY(stage_out(ctx, mem, S_STATS, grad_theta, 9, &gt));
+  PSGC_TRY(stage_out(ctx, mem, S_ADJ1, grad_phL, np, &gl));
+  PSGC_TRY(stage_out(ctx, mem, S_ADJ2, grad_phR, np, &gr));
+  PSGC_TRY(launch_adj_backward(theta, nominal, l, r, m, rows, cols, g, gt, gl, gr, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, grad_theta, gt, 9));
+  PSGC_TRY(copy_back(ctx, mem, grad_phL, gl, np));
+  PSGC_TRY(copy_back(ctx, mem, grad_phR, gr, np));
   return finish(ctx, mem);
 }
 

@@ -1,0 +1,684 @@
+// pipeline.hip — the batched reconstruction pipeline (psgc_batch_*) and the
+// tiled single-pair paths of psgc_decode / psgc_reconstruct.
+//
+// A plan holds, per stage, one table of work descriptors covering every pair
+// of the batch, the intermediates of every pair in one device arena, and the
+// persistent grid size of every stage. Running the plan issues one launch
+// per stage for the whole batch (plus a few tiny per-pair statistics
+// kernels), so a captured hipGraph replays the full workload with one call.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "psgc_internal.h"
+#include "psgc_tiled.h"
+
+namespace psgc {
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  void* take(size_t bytes) {
+    used = (used + kAlign - 1) / kAlign * kAlign;
+    void* p = base + used;
+    used += bytes;
+    return p;
+  }
+};
+
+// Per-pair operations that run as small per-pair kernels after the batched
+// stages (statistics and mask application).
+struct PairPost {
+  int outlier;              // PSGC_OUTLIER_*
+  int use_mls;
+  int rows, cols;
+  // ROR
+  const float* ror_src;
+  const uint8_t* ror_P;
+  float* ror_dst;
+  uint8_t* ror_Pdst;
+  float radius;
+  int min_nb, window;
+  // SOR statistics
+  const float* sor_src;
+  double* md;
+  double* partial;
+  double* stats;
+  float std_mul;
+  float* sor_dst;           // when no MLS: final xyz
+  uint8_t* sor_Pdst;        // when no MLS: final mask
+  // copies when neither an outlier stage writes the outputs nor MLS runs
+  const float* copy_xyz_src;
+  const uint8_t* copy_P_src;
+  float* copy_xyz_dst;
+  uint8_t* copy_P_dst;
+  // disparity output copy (no median)
+  const float* disp_src;
+  float* disp_dst;
+};
+
+template <typename J>
+struct Table {
+  std::vector<J> host;
+  J* dev = nullptr;
+  int64_t total = 0;
+};
+
+}  // namespace
+
+struct Plan {
+  psgc_ctx* ctx = nullptr;
+  int n = 0;
+  Arena arena;
+  Table<DecodeJob> dec;
+  Table<MatchJob> mat;
+  Table<CloudJob> med, rg, sor;
+  std::vector<Table<CloudJob>> mls;
+  std::vector<PairPost> post;
+  std::vector<DecodeArgs> gray_fix;       // gray-code jump correction per camera
+  std::vector<const double*> gray_in_ph;
+  std::vector<double*> gray_out_ph;
+  std::vector<const int32_t*> gray_in_k;
+  std::vector<int32_t*> gray_out_k;
+  std::vector<const uint8_t*> gray_valid;
+  int max_cols = 0, max_knum = 0, max_ghalf = 0, max_sor_half = 0, max_sor_k = 0, max_mls_hw = 0;
+  std::map<std::string, int> grid;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~Plan() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (arena.base) hipFree(arena.base);
+  }
+};
+
+namespace {
+
+int tiles(int n, int t) { return (n + t - 1) / t; }
+
+int upload_tables(Plan* pl) {
+  // device copies of the tables live at the end of the arena
+  auto put = [&](auto& tab) -> int {
+    using J = typename std::decay_t<decltype(tab.host)>::value_type;
+    if (tab.host.empty()) return PSGC_OK;
+    tab.dev = (J*)pl->arena.take(tab.host.size() * sizeof(J));
+    PSGC_HIP(hipMemcpy(tab.dev, tab.host.data(), tab.host.size() * sizeof(J),
+                       hipMemcpyHostToDevice));
+    return PSGC_OK;
+  };
+  PSGC_TRY(put(pl->dec));
+  PSGC_TRY(put(pl->mat));
+  PSGC_TRY(put(pl->med));
+  PSGC_TRY(put(pl->rg));
+  PSGC_TRY(put(pl->sor));
+  for (auto& t : pl->mls) PSGC_TRY(put(t));
+  return PSGC_OK;
+}
+
+size_t table_bytes(const Plan* pl) {
+  size_t b = 0;
+  auto add = [&](const auto& tab) {
+    using J = typename std::decay_t<decltype(tab.host)>::value_type;
+    b += tab.host.size() * sizeof(J) + kAlign;
+  };
+  add(pl->dec);
+  add(pl->mat);
+  add(pl->med);
+  add(pl->rg);
+  add(pl->sor);
+  for (auto& t : pl->mls) add(t);
+  return b;
+}
+
+int default_grid(psgc_ctx* ctx, int per_cu) {
+  const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
+  return cu * per_cu;
+}
+
+}  // namespace
+
+// Input of one pair to the reconstruct part of a plan.
+struct ReconInput {
+  const double* lph;   // phase maps (raw if smooth)
+  const double* rph;
+  const int32_t* lk;
+  const int32_t* rk;
+  int smooth;          // post-smooth still to apply (fused into the match)
+};
+
+// Builds the reconstruct stages of pair i (after decode) into the plan.
+// Buffers come from the arena (sizes counted by a first dry pass when
+// `count` is set: then only arena.used advances).
+static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int rows, int cols,
+                            int pha_f, const double* Q, const double* R1, const ReconInput& in,
+                            float* xyz_out, uint8_t* P_out, float* disp_out, int32_t* mc_out,
+                            double* d64_out) {
+  Arena& A = pl->arena;
+  const size_t np = (size_t)rows * cols;
+  const int gw = rc.gauss_window;
+  if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
+  if (rc.outlier_mode < PSGC_OUTLIER_NONE || rc.outlier_mode > PSGC_OUTLIER_SOR)
+    PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", rc.outlier_mode);
+  if (rc.outlier_mode == PSGC_OUTLIER_SOR) {
+    if (rc.sor_k < 1 || rc.sor_k > 64) PSGC_FAIL(PSGC_E_INVALID, "sor_k=%d", rc.sor_k);
+    if (rc.sor_window < 1 || rc.sor_window > 8)
+      PSGC_FAIL(PSGC_E_UNSUPPORTED, "sor_window=%d above the tiled limit 8", rc.sor_window);
+  }
+  if (rc.use_mls) {
+    if (!(rc.mls_spatial_sigma > 0)) PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+    if (rc.mls_window_size < 0 || rc.mls_window_size > 8)
+      PSGC_FAIL(PSGC_E_UNSUPPORTED, "mls_window_size=%d above the tiled limit 8",
+                rc.mls_window_size);
+  }
+  if (match_tiled_lds(cols, pha_f) > (size_t)match_tiled_max_lds())
+    PSGC_FAIL(PSGC_E_UNSUPPORTED, "row of %d px with %d orders exceeds the LDS budget", cols,
+              pha_f);
+  const bool cpu = rc.semantics == PSGC_SEM_CPU;
+  const bool use64 = !rc.use_disp_median_filter && cpu;
+
+  // ---- match
+  float* disp = disp_out && !rc.use_disp_median_filter ? disp_out : (float*)A.take(np * 4);
+  double* d64 = d64_out ? d64_out : (use64 ? (double*)A.take(np * 8) : nullptr);
+  MatchJob mj{};
+  mj.lp = in.lph;
+  mj.rp = in.rph;
+  mj.lk = in.lk;
+  mj.rk = in.rk;
+  mj.rows = rows;
+  mj.cols = cols;
+  mj.pha_f = pha_f;
+  mj.smooth = in.smooth;
+  mj.grad_thr = rc.grad_threshold;
+  mj.disp = disp;
+  mj.disp64 = d64;
+  mj.match_col = mc_out;
+  mj.tile_base = pl->mat.total;
+  pl->mat.total += rows;
+  pl->mat.host.push_back(mj);
+  pl->max_cols = std::max(pl->max_cols, cols);
+  pl->max_knum = std::max(pl->max_knum, pha_f);
+
+  CloudJob base{};
+  base.rows = rows;
+  base.cols = cols;
+  base.tiles_x = tiles(cols, 16);
+  base.tiles_y = tiles(rows, 16);
+  const int64_t ntile = (int64_t)base.tiles_x * base.tiles_y;
+
+  // ---- median
+  const float* dsrc = disp;
+  if (rc.use_disp_median_filter) {
+    CloudJob c = base;
+    c.tile_base = pl->med.total;
+    pl->med.total += ntile;
+    c.disp_in = disp;
+    c.disp_med = disp_out ? disp_out : (float*)A.take(np * 4);
+    dsrc = c.disp_med;
+    pl->med.host.push_back(c);
+  }
+
+  // ---- reproject + Gaussian
+  float* xyzg = (float*)A.take(np * 12);
+  uint8_t* Pg = (uint8_t*)A.take(np);
+  uint8_t* P0 = cpu ? (uint8_t*)A.take(np) : nullptr;
+  {
+    CloudJob c = base;
+    c.tile_base = pl->rg.total;
+    pl->rg.total += ntile;
+    c.disp_src = dsrc;
+    c.disp_src64 = use64 ? d64 : nullptr;
+    std::memcpy(c.q, Q, sizeof(c.q));
+    std::memcpy(c.r1, R1, sizeof(c.r1));
+    c.min_depth = rc.min_depth;
+    c.max_depth = rc.max_depth;
+    c.use_rect = rc.use_rectified_coords;
+    c.semantics = rc.semantics;
+    c.gwin = gw;
+    gauss_kernel_f32(gw, rc.gauss_sigma, c.gk);
+    c.xyz0 = nullptr;
+    c.P0 = P0;
+    c.xyzg = xyzg;
+    c.Pg = Pg;
+    pl->rg.host.push_back(c);
+    pl->max_ghalf = std::max(pl->max_ghalf, (gw - 1) / 2);
+  }
+  // CPU semantics: the outlier stage reads the pre-Gaussian mask.
+  const uint8_t* Pin = cpu ? P0 : Pg;
+
+  PairPost pp{};
+  pp.outlier = rc.outlier_mode;
+  pp.use_mls = rc.use_mls;
+  pp.rows = rows;
+  pp.cols = cols;
+  if (disp_out && !rc.use_disp_median_filter) pp.disp_dst = nullptr;  // written by the match
+
+  const float* mls_src = xyzg;
+  const uint8_t* mls_mask = Pin;
+  const double* mls_md = nullptr;
+  const double* mls_thr = nullptr;
+  uint8_t* mls_Pout = P_out;
+  float* t0 = nullptr;
+  float* t1 = rc.use_mls && rc.mls_iterations > 1 ? (float*)A.take(np * 12) : nullptr;
+
+  if (rc.outlier_mode == PSGC_OUTLIER_ROR) {
+    float* dst = rc.use_mls ? (t0 = (float*)A.take(np * 12)) : xyz_out;
+    pp.ror_src = xyzg;
+    pp.ror_P = Pin;
+    pp.ror_dst = dst;
+    pp.ror_Pdst = P_out;
+    pp.radius = rc.radius_search;
+    pp.min_nb = rc.radius_min_neighbors;
+    pp.window = rc.radius_window_size;
+    mls_src = dst;
+    mls_mask = P_out;
+    mls_Pout = nullptr;
+  } else if (rc.outlier_mode == PSGC_OUTLIER_SOR) {
+    double* md = (double*)A.take(np * 8);
+    double* part = (double*)A.take(sor_partial_doubles(np) * 8);
+    double* st = (double*)A.take(4 * 8);
+    CloudJob c = base;
+    c.tile_base = pl->sor.total;
+    pl->sor.total += ntile;
+    c.sor_xyz = xyzg;
+    c.sor_P = Pin;
+    c.sor_k = rc.sor_k;
+    c.sor_half = rc.sor_window;
+    c.md = md;
+    pl->sor.host.push_back(c);
+    pl->max_sor_half = std::max(pl->max_sor_half, rc.sor_window);
+    pl->max_sor_k = std::max(pl->max_sor_k, rc.sor_k);
+    pp.sor_src = xyzg;
+    pp.md = md;
+    pp.partial = part;
+    pp.stats = st;
+    pp.std_mul = rc.sor_std_mul;
+    if (!rc.use_mls) {
+      pp.sor_dst = xyz_out;
+      pp.sor_Pdst = P_out;
+    }
+    mls_md = md;
+    mls_thr = st + 3;
+  } else if (!rc.use_mls) {
+    pp.copy_xyz_src = xyzg;
+    pp.copy_P_src = Pin;
+    pp.copy_xyz_dst = xyz_out;
+    pp.copy_P_dst = P_out;
+  }
+
+  if (rc.use_mls) {
+    const int n_it = rc.mls_iterations > 1 ? rc.mls_iterations : 1;
+    const int order = std::max(0, std::min(2, rc.mls_polynomial_order));
+    const float r2 = (rc.mls_max_radius > 0.0f) ? rc.mls_max_radius * rc.mls_max_radius
+                                                : 9.0f * rc.mls_spatial_sigma * rc.mls_spatial_sigma;
+    if ((int)pl->mls.size() < n_it) pl->mls.resize(n_it);
+    const float* src = mls_src;
+    for (int it = 0; it < n_it; ++it) {
+      CloudJob c = base;
+      Table<CloudJob>& tab = pl->mls[it];
+      c.tile_base = tab.total;
+      tab.total += ntile;
+      float* dst = (it == n_it - 1) ? xyz_out : ((it % 2 == 0) ? t1 : t0);
+      if (dst == nullptr || dst == src) dst = (src == t1) ? (t0 ? t0 : (t0 = (float*)A.take(np * 12))) : t1;
+      c.mls_src = src;
+      c.mls_P = it == 0 ? mls_mask : P_out;
+      c.mls_md = it == 0 ? mls_md : nullptr;
+      c.mls_thr = it == 0 ? mls_thr : nullptr;
+      c.mls_dst = dst;
+      c.mls_Pout = it == 0 ? mls_Pout : nullptr;
+      c.mls_hw = rc.mls_window_size;
+      c.mls_order = order;
+      c.mls_sem = rc.semantics;
+      c.mls_inv2s2 = 1.0f / (2.0f * rc.mls_spatial_sigma * rc.mls_spatial_sigma);
+      c.mls_r2 = r2;
+      tab.host.push_back(c);
+      src = dst;
+    }
+    pl->max_mls_hw = std::max(pl->max_mls_hw, rc.mls_window_size);
+  }
+  pl->post.push_back(pp);
+  (void)i;
+  return PSGC_OK;
+}
+
+static int run_plan(Plan* pl, hipStream_t s) {
+  auto g = [&](const char* st, int per_cu) {
+    auto it = pl->grid.find(st);
+    return it != pl->grid.end() ? it->second : default_grid(pl->ctx, per_cu);
+  };
+  if (!pl->dec.host.empty())
+    PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
+                                 g("decode", 4), s));
+  for (size_t c = 0; c < pl->gray_fix.size(); ++c)
+    PSGC_TRY(launch_gray_fix(pl->gray_fix[c], pl->gray_in_ph[c], pl->gray_out_ph[c],
+                             pl->gray_in_k[c], pl->gray_out_k[c], pl->gray_valid[c], s));
+  if (!pl->mat.host.empty())
+    PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
+                                g("match", 4), pl->max_cols, pl->max_knum, s));
+  if (!pl->med.host.empty())
+    PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
+                                 g("median5", 8), s));
+  if (!pl->rg.host.empty())
+    PSGC_TRY(launch_repro_gauss_tiled(pl->rg.dev, (int)pl->rg.host.size(), pl->rg.total,
+                                      g("repro_gauss", 8), pl->max_ghalf, s));
+  if (!pl->sor.host.empty())
+    PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
+                                  g("sor_knn", 8), pl->max_sor_half, pl->max_sor_k, s));
+  for (const PairPost& pp : pl->post) {
+    const size_t np = (size_t)pp.rows * pp.cols;
+    if (pp.outlier == PSGC_OUTLIER_ROR) {
+      PSGC_TRY(launch_ror(pp.ror_src, pp.ror_P, pp.ror_dst, pp.ror_Pdst, pp.rows, pp.cols,
+                          pp.radius, pp.min_nb, pp.window, s));
+    } else if (pp.outlier == PSGC_OUTLIER_SOR) {
+      if (pp.use_mls)
+        PSGC_TRY(launch_sor_stats(pp.md, np, pp.std_mul, pp.partial, pp.stats, s));
+      else
+        PSGC_TRY(launch_sor_finish(pp.sor_src, pp.md, pp.sor_dst, pp.sor_Pdst, np, pp.std_mul,
+                                   pp.partial, pp.stats, s));
+    } else if (pp.copy_xyz_dst) {
+      PSGC_HIP(hipMemcpyAsync(pp.copy_xyz_dst, pp.copy_xyz_src, np * 12, hipMemcpyDeviceToDevice,
+                              s));
+      PSGC_HIP(hipMemcpyAsync(pp.copy_P_dst, pp.copy_P_src, np, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  for (auto& t : pl->mls)
+    if (!t.host.empty())
+      PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 4), pl->max_mls_hw,
+                                s));
+  return PSGC_OK;
+}
+
+// Times one stage of the plan over candidate persistent grid sizes and keeps
+// the fastest (the plan's inputs of that stage must already be valid).
+static int autotune(Plan* pl, hipStream_t s) {
+  const int cu = pl->ctx->cu_count > 0 ? pl->ctx->cu_count : 256;
+  const int cands[] = {1, 2, 4, 8, 16};
+  hipEvent_t a, b;
+  PSGC_HIP(hipEventCreate(&a));
+  PSGC_HIP(hipEventCreate(&b));
+  struct St {
+    const char* name;
+    bool on;
+  };
+  const St stages[] = {{"decode", !pl->dec.host.empty()},    {"match", !pl->mat.host.empty()},
+                       {"median5", !pl->med.host.empty()},   {"repro_gauss", !pl->rg.host.empty()},
+                       {"sor_knn", !pl->sor.host.empty()},   {"mls", !pl->mls.empty()}};
+  for (const St& st : stages) {
+    if (!st.on) continue;
+    float best_ms = 1e30f;
+    int best_g = 0;
+    for (int pc : cands) {
+      const int grid = cu * pc;
+      float tot = 0;
+      for (int rep = 0; rep < 2; ++rep) {
+        PSGC_HIP(hipEventRecord(a, s));
+        const std::string n = st.name;
+        if (n == "decode")
+          PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
+                                       grid, s));
+        else if (n == "match")
+          PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total, grid,
+                                      pl->max_cols, pl->max_knum, s));
+        else if (n == "median5")
+          PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
+                                       grid, s));
+        else if (n == "repro_gauss")
+          PSGC_TRY(launch_repro_gauss_tiled(pl->rg.dev, (int)pl->rg.host.size(), pl->rg.total,
+                                            grid, pl->max_ghalf, s));
+        else if (n == "sor_knn")
+          PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
+                                        grid, pl->max_sor_half, pl->max_sor_k, s));
+        else
+          PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
+                                    pl->mls[0].total, grid, pl->max_mls_hw, s));
+        PSGC_HIP(hipEventRecord(b, s));
+        PSGC_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        PSGC_HIP(hipEventElapsedTime(&ms, a, b));
+        if (rep > 0) tot += ms;  // first repetition warms caches
+      }
+      if (tot < best_ms) {
+        best_ms = tot;
+        best_g = grid;
+      }
+    }
+    pl->grid[st.name] = best_g;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  // the MLS tuning pass overwrote iteration 0's output only, which every
+  // run recomputes first; nothing else was modified
+  return PSGC_OK;
+}
+
+// Allocates the arena: a dry build counts bytes, the real build takes them.
+template <typename Build>
+static int build_plan(Plan* pl, Build&& build) {
+  Plan dry;
+  dry.ctx = pl->ctx;
+  dry.arena.base = nullptr;
+  PSGC_TRY(build(&dry));
+  const size_t bytes = dry.arena.used + table_bytes(&dry) + 4 * kAlign;
+  if (hipMalloc(&pl->arena.base, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    pl->arena.base = nullptr;
+    PSGC_FAIL(PSGC_E_ALLOC, "batch arena of %zu bytes", bytes);
+  }
+  pl->arena.size = bytes;
+  PSGC_TRY(build(pl));
+  return upload_tables(pl);
+}
+
+// Decode stages of one camera into the plan; returns the phase/k the match
+// should read.
+static int plan_decode(Plan* pl, const psgc_decode_config& dc, const void* const* frames,
+                       int n_frames, int sample_type, int rows, int cols, int64_t stride,
+                       double* phase_raw, int32_t* k, double* mod, uint8_t* valid,
+                       const double** ph_for_match, const int32_t** k_for_match) {
+  DecodeJob j{};
+  PSGC_TRY(build_decode_args(&dc, n_frames, sample_type, rows, cols, stride, &j.a));
+  for (int f = 0; f < n_frames; ++f) {
+    if (!frames[f]) PSGC_FAIL(PSGC_E_INVALID, "frame %d is null", f);
+    j.a.frames[f] = frames[f];
+  }
+  const size_t np = (size_t)rows * cols;
+  const bool gray = j.a.nlev == 1;
+  j.phase = phase_raw;
+  j.k = k;
+  j.mod = mod;
+  j.valid = valid;
+  if (gray && !valid) j.valid = (uint8_t*)pl->arena.take(np);
+  j.tiles_x = tiles(cols, 256);
+  j.tiles_y = tiles(rows, 4);
+  j.tile_base = pl->dec.total;
+  pl->dec.total += (int64_t)j.tiles_x * j.tiles_y;
+  pl->dec.host.push_back(j);
+  *ph_for_match = phase_raw;
+  *k_for_match = k;
+  if (gray) {
+    double* ph2 = (double*)pl->arena.take(np * 8);
+    int32_t* k2 = (int32_t*)pl->arena.take(np * 4);
+    pl->gray_fix.push_back(j.a);
+    pl->gray_in_ph.push_back(phase_raw);
+    pl->gray_out_ph.push_back(ph2);
+    pl->gray_in_k.push_back(k);
+    pl->gray_out_k.push_back(k2);
+    pl->gray_valid.push_back(j.valid);
+    *ph_for_match = ph2;
+    *k_for_match = k2;
+  }
+  return PSGC_OK;
+}
+
+}  // namespace psgc
+
+using namespace psgc;
+
+struct psgc_batch {
+  psgc_ctx* ctx = nullptr;
+  Plan plan;
+  bool graph = false;
+};
+
+extern "C" {
+
+int psgc_batch_create(psgc_ctx* ctx, const psgc_pair_desc* pairs, int n, int flags,
+                      psgc_batch** out) {
+  if (!ctx || !pairs || n < 1 || !out) PSGC_FAIL(PSGC_E_INVALID, "bad batch arguments");
+  *out = nullptr;
+  PSGC_HIP(hipSetDevice(ctx->device));
+  for (int i = 0; i < n; ++i) {
+    const psgc_pair_desc& p = pairs[i];
+    if (p.rows < 1 || p.cols < 1 || !p.xyz || !p.P)
+      PSGC_FAIL(PSGC_E_INVALID, "pair %d: bad size or null output", i);
+    if (p.n_frames < 1 || p.n_frames > 64) PSGC_FAIL(PSGC_E_INVALID, "pair %d: n_frames", i);
+    if (p.pha_f < 0 || p.pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pair %d: pha_f", i);
+  }
+  psgc_batch* b = new psgc_batch();
+  b->ctx = ctx;
+  b->plan.ctx = ctx;
+  auto build = [&](Plan* pl) -> int {
+    for (int i = 0; i < n; ++i) {
+      const psgc_pair_desc& p = pairs[i];
+      const size_t np = (size_t)p.rows * p.cols;
+      const int sb = p.sample_type == PSGC_SAMPLE_U8 ? 1 : (p.sample_type == PSGC_SAMPLE_U16 ? 2 : 8);
+      const int64_t stride = p.row_stride_bytes ? p.row_stride_bytes : (int64_t)p.cols * sb;
+      double* lph = (double*)pl->arena.take(np * 8);
+      double* rph = (double*)pl->arena.take(np * 8);
+      int32_t* lk = (int32_t*)pl->arena.take(np * 4);
+      int32_t* rk = (int32_t*)pl->arena.take(np * 4);
+      ReconInput in{};
+      PSGC_TRY(plan_decode(pl, p.decode, p.left, p.n_frames, p.sample_type, p.rows, p.cols,
+                           stride, lph, lk, nullptr, nullptr, &in.lph, &in.lk));
+      PSGC_TRY(plan_decode(pl, p.decode, p.right, p.n_frames, p.sample_type, p.rows, p.cols,
+                           stride, rph, rk, nullptr, nullptr, &in.rph, &in.rk));
+      in.smooth = p.decode.post_smooth ? 1 : 0;
+      PSGC_TRY(plan_reconstruct(pl, i, p.recon, p.rows, p.cols, p.pha_f, p.Q, p.R1, in, p.xyz,
+                                p.P, p.disp, nullptr, nullptr));
+    }
+    return PSGC_OK;
+  };
+  int rc = build_plan(&b->plan, build);
+  if (rc != PSGC_OK) {
+    delete b;
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  if (flags & PSGC_BATCH_AUTOTUNE) {
+    rc = run_plan(&b->plan, s);  // makes every stage's inputs valid
+    if (rc == PSGC_OK) rc = autotune(&b->plan, s);
+    if (rc != PSGC_OK) {
+      delete b;
+      return rc;
+    }
+  }
+  if (flags & PSGC_BATCH_GRAPH) {
+    const bool prof = g_prof_on;
+    g_prof_on = false;  // no event records inside the captured graph
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      rc = run_plan(&b->plan, s);
+      hipGraph_t graph = nullptr;
+      hipError_t e2 = hipStreamEndCapture(s, &graph);
+      if (rc == PSGC_OK && e2 == hipSuccess) {
+        b->plan.graph = graph;
+        e = hipGraphInstantiate(&b->plan.exec, graph, nullptr, nullptr, 0);
+      } else {
+        e = e2;
+        if (graph) hipGraphDestroy(graph);
+      }
+    }
+    g_prof_on = prof;
+    if (rc != PSGC_OK || e != hipSuccess) {
+      delete b;
+      if (rc != PSGC_OK) return rc;
+      PSGC_FAIL(PSGC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    b->graph = true;
+  }
+  *out = b;
+  return PSGC_OK;
+}
+
+int psgc_batch_run(psgc_batch* b) {
+  if (!b) PSGC_FAIL(PSGC_E_INVALID, "null batch");
+  PSGC_HIP(hipSetDevice(b->ctx->device));
+  if (b->graph) {
+    PSGC_HIP(hipGraphLaunch(b->plan.exec, b->ctx->stream));
+    return PSGC_OK;
+  }
+  return run_plan(&b->plan, b->ctx->stream);
+}
+
+int psgc_batch_run_eager(psgc_batch* b) {
+  if (!b) PSGC_FAIL(PSGC_E_INVALID, "null batch");
+  PSGC_HIP(hipSetDevice(b->ctx->device));
+  return run_plan(&b->plan, b->ctx->stream);
+}
+
+int psgc_batch_destroy(psgc_batch* b) {
+  if (!b) return PSGC_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return PSGC_OK;
+}
+
+int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
+  if (!b || !stage || !grid) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  auto it = b->plan.grid.find(stage);
+  *grid = it != b->plan.grid.end() ? it->second : 0;
+  return PSGC_OK;
+}
+
+}  // extern "C"
+
+namespace psgc {
+
+// Tiled single-pair paths used by psgc_decode / psgc_reconstruct.
+int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t* k, double* mod,
+                 uint8_t* valid, hipStream_t s) {
+  DecodeJob j{};
+  j.a = a;
+  j.phase = phase_raw;
+  j.k = k;
+  j.mod = mod;
+  j.valid = valid;
+  j.tiles_x = tiles(a.cols, 256);
+  j.tiles_y = tiles(a.rows, 4);
+  j.tile_base = 0;
+  void* d;
+  PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(DecodeJob), &d));
+  PSGC_HIP(hipMemcpyAsync(d, &j, sizeof(DecodeJob), hipMemcpyHostToDevice, s));
+  return launch_decode_tiled((const DecodeJob*)d, 1, (int64_t)j.tiles_x * j.tiles_y,
+                             default_grid(ctx, 4), s);
+}
+
+int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
+                      const double* rp, const int32_t* lk, const int32_t* rk, int rows, int cols,
+                      int pha_f, const double* Q, const double* R1, float* xyz, uint8_t* P,
+                      float* disp, int32_t* mc, hipStream_t s) {
+  Plan* pl = new Plan();
+  pl->ctx = ctx;
+  ReconInput in{lp, rp, lk, rk, 0};
+  auto build = [&](Plan* p) -> int {
+    return plan_reconstruct(p, 0, rc, rows, cols, pha_f, Q, R1, in, xyz, P, disp, mc, nullptr);
+  };
+  int r = build_plan(pl, build);
+  if (r == PSGC_OK) r = run_plan(pl, s);
+  // the plan's arena must outlive the queued kernels
+  if (r == PSGC_OK && hipStreamSynchronize(s) != hipSuccess) r = PSGC_E_HIP;
+  delete pl;
+  return r;
+}
+
+}  // namespace psgc

@@ -659,6 +659,20 @@ int launch_sor_finish(const float* xyz, double* md, float* dst, uint8_t* dstP, s
   return PSGC_OK;
 }
 
+int launch_sor_stats(const double* md, size_t npix, float std_mul, double* partial,
+                     double* stats, hipStream_t s) {
+  const size_t nchunks = (npix + kChunk - 1) / kChunk;
+  size_t np2 = 1;
+  while (np2 < nchunks) np2 *= 2;
+  PSGC_PROF_BEGIN("sor_stats", s);
+  sor_chunk_kernel<<<(unsigned)nchunks, 256, 0, s>>>(md, npix, partial, np2);
+  PSGC_LAUNCHED();
+  sor_final_kernel<<<1, 1024, 0, s>>>(partial, nchunks, np2, std_mul, stats);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("sor_stats", s);
+  return PSGC_OK;
+}
+
 size_t sor_partial_doubles(size_t npix) {
   const size_t nchunks = (npix + kChunk - 1) / kChunk;
   size_t np2 = 1;

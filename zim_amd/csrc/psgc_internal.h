@@ -50,7 +50,7 @@ enum Slot : int {
   S_FRAMES = 0, S_PHASE, S_PHASE_RAW, S_K, S_MOD, S_VALID,
   S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
-  S_STATS, S_QR, S_TMP0, S_TMP1, S_NUM_SLOTS
+  S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_NUM_SLOTS
 };
 
 }  // namespace psgc
@@ -61,6 +61,7 @@ struct psgc_ctx {
   hipStream_t stream = nullptr;
   std::vector<psgc::DevBuf> bufs;
   int cu_count = 0;
+  int kernels = PSGC_KERNELS_TILED;
 };
 
 namespace psgc {
@@ -125,6 +126,16 @@ int launch_sor_finish(const float* xyz, double* md, float* dst, uint8_t* dstP,
                       size_t npix, float std_mul, double* partial, double* stats,
                       hipStream_t s);
 size_t sor_partial_doubles(size_t npix);
+int launch_sor_stats(const double* md, size_t npix, float std_mul, double* partial,
+                     double* stats, hipStream_t s);
+
+// Tiled single-image / single-pair paths (pipeline.hip).
+int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t* k,
+                 double* mod, uint8_t* valid, hipStream_t s);
+int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
+                      const double* rp, const int32_t* lk, const int32_t* rk, int rows,
+                      int cols, int pha_f, const double* Q, const double* R1, float* xyz,
+                      uint8_t* P, float* disp, int32_t* mc, hipStream_t s);
 int launch_mls(const float* src, const uint8_t* P, float* dst, int rows,
                int cols, int hw, float sigma, int order, float r2cut,
                int semantics, hipStream_t s);
@@ -139,6 +150,18 @@ extern bool g_prof_on;
   do { if (::psgc::g_prof_on) ::psgc::prof_begin(name, s); } while (0)
 #define PSGC_PROF_END(name, s) \
   do { if (::psgc::g_prof_on) ::psgc::prof_end(name, s); } while (0)
+
+int launch_adj_forward(const double* theta, const double* nominal, const double* phL,
+                       const double* phR, const int32_t* mc, int rows, int cols, double* xyz,
+                       uint8_t* valid, hipStream_t s);
+int launch_adj_backward(const double* theta, const double* nominal, const double* phL,
+                        const double* phR, const int32_t* mc, int rows, int cols,
+                        const double* gxyz, double* gth, double* gphL, double* gphR,
+                        hipStream_t s);
+
+// Validates a decode configuration and fills everything but the frames.
+int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
+                      int cols, int64_t stride, DecodeArgs* a);
 
 // Host-side helpers.
 void gauss_kernel_f32(int win, float sigma, float* k2 /* win*win */);

@@ -1,0 +1,652 @@
+// tiled.hip — LDS-tiled, batch-capable kernels of the pipeline.
+//
+// Same per-pixel arithmetic as the plain kernels (psgc_device.h, reference
+// CPU operation order, no contraction) so the integer outputs stay
+// bit-identical; what changes is the data movement:
+//   decode     one thread per column, 4 rows per thread, each raw frame tile
+//              (6 x 258 samples) staged once through LDS (double-buffered),
+//              the horizontal blur pass done once per sample;
+//   match      one workgroup per row: the right phase row (post-smoothed on
+//              the fly from the raw map) and its orders in LDS, per-row order
+//              spans built with LDS atomics, the argmin scan reads LDS only;
+//   median     20x20 tile in LDS, branch-free 202-op selection network;
+//   reproject+Gaussian fused on a 16x16 tile with its halo in LDS;
+//   SOR        k-nearest search over the LDS tile, ring order (near first);
+//   MLS        points and mask of the tile plus halo in LDS, the in-radius
+//              window entries kept as a bit mask so passes 2 and 3 skip the
+//              distance test, fast exp2-based weight.
+
+#include <climits>
+
+#include "psgc_device.h"
+#include "psgc_internal.h"
+#include "psgc_math.h"
+#include "psgc_tiled.h"
+
+namespace psgc {
+namespace {
+
+// ----------------------------------------------------------------- decode --
+
+constexpr int DC = 256;  // tile width (threads)
+constexpr int DR = 4;    // output rows per thread
+constexpr int DLR = DR + 2, DLC = DC + 2;
+
+__device__ inline void decode_load(const DecodeArgs& a, int f, int y0, int x0,
+                                   double (*buf)[DLC]) {
+  for (int rr = 0; rr < DLR; ++rr) {
+    const int gy = min(max(y0 - 1 + rr, 0), a.rows - 1);
+    for (int cc = threadIdx.x; cc < DLC; cc += DC) {
+      const int gx = min(max(x0 - 1 + cc, 0), a.cols - 1);
+      buf[rr][cc] = load_sample(a, f, gy, gx);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __restrict__ jobs,
+                                                          int njobs, int64_t total) {
+  __shared__ double buf[2][DLR][DLC];
+  const XcdRange xr = xcd_range(total);
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const DecodeJob& job = jobs[j];
+    const DecodeArgs& a = job.a;
+    const int64_t lt = t - job.tile_base;
+    const int tx = (int)(lt % job.tiles_x), ty = (int)(lt / job.tiles_x);
+    const int x0 = tx * DC, y0 = ty * DR;
+    const int x = x0 + threadIdx.x;
+    const double c = a.blur_c, s = a.blur_s;
+
+    double S[DR], Cs[DR], G[DR], wrap[3][DR], mod0[DR];
+    bool ok[DR];
+    for (int q = 0; q < DR; ++q) {
+      ok[q] = true;
+      mod0[q] = 0.0;
+      wrap[0][q] = wrap[1][q] = wrap[2][q] = 0.0;
+    }
+    int nframes = 0;
+    for (int i = 0; i < a.nlev; ++i) nframes += a.shift_num[i];
+
+    decode_load(a, 0, y0, x0, buf[0]);
+    __syncthreads();
+    int level = 0, kin = 0;
+    for (int q = 0; q < DR; ++q) S[q] = Cs[q] = G[q] = 0.0;
+    for (int f = 0; f < nframes; ++f) {
+      if (f + 1 < nframes) decode_load(a, f + 1, y0, x0, buf[(f + 1) & 1]);
+      const double(*b)[DLC] = buf[f & 1];
+      const int tc = threadIdx.x + 1;
+      double h[DLR];
+#pragma unroll
+      for (int rr = 0; rr < DLR; ++rr) h[rr] = c * b[rr][tc] + s * (b[rr][tc - 1] + b[rr][tc + 1]);
+      const double st = a.sin_tab[f], ct = a.cos_tab[f];
+#pragma unroll
+      for (int q = 0; q < DR; ++q) {
+        const double v = c * h[q + 1] + s * (h[q] + h[q + 2]);
+        S[q] -= st * v;
+        Cs[q] += ct * v;
+        G[q] += v;
+      }
+      if (++kin == a.shift_num[level]) {
+#pragma unroll
+        for (int q = 0; q < DR; ++q) {
+          double m;
+          const bool lv = level_finish(S[q], Cs[q], G[q], a.shift_num[level], a.mod_thresh,
+                                       &wrap[level][q], &m);
+          if (level == 0) mod0[q] = m;
+          ok[q] = ok[q] && lv;
+          S[q] = Cs[q] = G[q] = 0.0;
+        }
+        kin = 0;
+        ++level;
+      }
+      __syncthreads();
+    }
+    if (x < a.cols) {
+      for (int q = 0; q < DR; ++q) {
+        const int y = y0 + q;
+        if (y >= a.rows) break;
+        const size_t p = (size_t)y * a.cols + x;
+        double ph = 0.0;
+        int32_t k = 0;
+        bool v = ok[q];
+        if (v) {
+          double w3[3] = {wrap[0][q], wrap[1][q], wrap[2][q]};
+          if (a.nlev == 1) v = gray_order(a, y, x, w3[0], &ph, &k);
+          else heterodyne(a, w3, &ph, &k);
+        }
+        if (!v) {
+          ph = 0.0;
+          k = 0;
+        }
+        job.phase[p] = ph;
+        job.k[p] = k;
+        if (job.mod) job.mod[p] = mod0[q];
+        if (job.valid) job.valid[p] = v ? 255 : 0;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ match --
+
+constexpr int MT = 256;
+
+__device__ inline double smooth_at(const double* m, int rows, int cols, int r, int x) {
+  const int xl = x > 0 ? x - 1 : 0, xr = x < cols - 1 ? x + 1 : cols - 1;
+  const int ru = r > 0 ? r - 1 : 0, rd = r < rows - 1 ? r + 1 : rows - 1;
+  const double* u = m + (size_t)ru * cols;
+  const double* c = m + (size_t)r * cols;
+  const double* d = m + (size_t)rd * cols;
+  const double hu = 0.5 * u[x] + 0.25 * (u[xl] + u[xr]);
+  const double hm = 0.5 * c[x] + 0.25 * (c[xl] + c[xr]);
+  const double hd = 0.5 * d[x] + 0.25 * (d[xl] + d[xr]);
+  return 0.5 * hm + 0.25 * (hu + hd);
+}
+
+__global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restrict__ jobs,
+                                                         int njobs, int64_t total) {
+  extern __shared__ double lds[];
+  const XcdRange xr = xcd_range(total);
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const MatchJob& m = jobs[j];
+    const int r = (int)(t - m.tile_base);
+    const int cols = m.cols;
+    const int knum = m.pha_f > 0 ? m.pha_f : 0;
+    double* sR = lds;
+    int32_t* kR = (int32_t*)(sR + cols);
+    int32_t* lo = kR + cols;
+    int32_t* hi = lo + (knum + 1);
+    const size_t row = (size_t)r * cols;
+    for (int k = threadIdx.x; k <= knum; k += MT) {
+      lo[k] = INT_MAX;
+      hi[k] = -1;
+    }
+    for (int c = threadIdx.x; c < cols; c += MT) {
+      sR[c] = m.smooth ? smooth_at(m.rp, m.rows, cols, r, c) : m.rp[row + c];
+      kR[c] = m.rk[row + c];
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < cols; c += MT) {
+      const int k = kR[c];
+      if (k >= 0 && k <= knum) {
+        atomicMin(&lo[k], c);
+        atomicMax(&hi[k], c);
+      }
+    }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < cols; jj += MT) {
+      double d = 0.0;
+      int best = -1;
+      const int k = m.lk[row + jj];
+      if (k > 0 && k <= m.pha_f) {
+        const int a = lo[k], b = hi[k];
+        const double lv = m.smooth ? smooth_at(m.lp, m.rows, cols, r, jj) : m.lp[row + jj];
+        double min_cost = 1e9;
+        for (int c = a; c <= b; ++c) {
+          if (kR[c] != k) continue;
+          const double diff = fabs(sR[c] - lv);
+          if (diff < min_cost) {
+            min_cost = diff;
+            best = c;
+          }
+        }
+        if (best >= 0 && min_cost < m.grad_thr && best >= 2 && best < cols - 1) {
+          d = subpixel_disparity(sR, best, lv, jj);
+        } else {
+          best = -1;
+        }
+      }
+      m.disp[row + jj] = (float)d;
+      if (m.disp64) m.disp64[row + jj] = d;
+      if (m.match_col) m.match_col[row + jj] = best;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ cloud tiles --
+
+constexpr int CT = 16;  // 16x16 output tile, 256 threads
+
+__device__ inline float median25(const float* v) {
+  const float t0 = v[0], t1 = v[1], t2 = v[2], t3 = v[3], t4 = v[4], t5 = v[5], t6 = v[6],
+              t7 = v[7], t8 = v[8], t9 = v[9], t10 = v[10], t11 = v[11], t12 = v[12],
+              t13 = v[13], t14 = v[14], t15 = v[15], t16 = v[16], t17 = v[17], t18 = v[18],
+              t19 = v[19], t20 = v[20], t21 = v[21], t22 = v[22], t23 = v[23], t24 = v[24];
+#include "median25.inc"
+  return PSGC_MEDIAN25_RESULT;
+}
+
+__global__ void __launch_bounds__(256) median_tiled_kernel(const CloudJob* __restrict__ jobs,
+                                                           int njobs, int64_t total) {
+  __shared__ float tile[CT + 4][CT + 5];
+  const XcdRange xr = xcd_range(total);
+  const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const CloudJob& cj = jobs[j];
+    const int64_t lt = t - cj.tile_base;
+    const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
+    for (int idx = threadIdx.x; idx < (CT + 4) * (CT + 4); idx += 256) {
+      const int rr = idx / (CT + 4), cc = idx % (CT + 4);
+      const int gy = min(max(y0 - 2 + rr, 0), cj.rows - 1);
+      const int gx = min(max(x0 - 2 + cc, 0), cj.cols - 1);
+      tile[rr][cc] = cj.disp_in[(size_t)gy * cj.cols + gx];
+    }
+    __syncthreads();
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < cj.cols && y < cj.rows) {
+      float v[25];
+#pragma unroll
+      for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) v[dy * 5 + dx] = tile[ly + dy][lx + dx];
+      cj.disp_med[(size_t)y * cj.cols + x] = median25(v);
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int GH_MAX = 3;  // Gaussian half window <= 3 (win <= 7)
+constexpr int GL = CT + 2 * GH_MAX;
+
+__global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* __restrict__ jobs,
+                                                                int njobs, int64_t total) {
+  __shared__ float px[GL][GL + 1], py[GL][GL + 1], pz[GL][GL + 1];
+  __shared__ uint8_t pm[GL][GL + 1];
+  const XcdRange xr = xcd_range(total);
+  const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const CloudJob& cj = jobs[j];
+    const int64_t lt = t - cj.tile_base;
+    const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
+    const int h = (cj.gwin - 1) / 2;
+    const int L = CT + 2 * h;
+    for (int idx = threadIdx.x; idx < L * L; idx += 256) {
+      const int rr = idx / L, cc = idx % L;
+      const int gy = y0 - h + rr, gx = x0 - h + cc;
+      float o[3] = {0.f, 0.f, 0.f};
+      uint8_t v = 0;
+      if (gy >= 0 && gy < cj.rows && gx >= 0 && gx < cj.cols) {
+        const size_t p = (size_t)gy * cj.cols + gx;
+        const double dd = cj.disp_src64 ? cj.disp_src64[p] : (double)cj.disp_src[p];
+        v = reproject_point(cj.q, cj.r1, gy, gx, dd, cj.min_depth,
+                            cj.max_depth, cj.use_rect, o);
+        // the tile's own pixels publish the reprojection
+        if (rr >= h && rr < h + CT && cc >= h && cc < h + CT) {
+          if (cj.xyz0) {
+            cj.xyz0[p * 3 + 0] = o[0];
+            cj.xyz0[p * 3 + 1] = o[1];
+            cj.xyz0[p * 3 + 2] = o[2];
+          }
+          if (cj.P0) cj.P0[p] = v;
+        }
+      }
+      px[rr][cc] = o[0];
+      py[rr][cc] = o[1];
+      pz[rr][cc] = o[2];
+      pm[rr][cc] = v;
+    }
+    __syncthreads();
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < cj.cols && y < cj.rows) {
+      const size_t p = (size_t)y * cj.cols + x;
+      const int cy = ly + h, cx = lx + h;
+      float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+      uint8_t op = cj.semantics == PSGC_SEM_CPU ? pm[cy][cx] : 0;
+      if (y >= h && y < cj.rows - h && x >= h && x < cj.cols - h && pm[cy][cx] != 0) {
+        const float ccx = px[cy][cx];
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, cnt = 0.f;
+        for (int ik = -h; ik <= h; ++ik)
+          for (int jk = -h; jk <= h; ++jk) {
+            const int ry = cy + ik, rx = cx + jk;
+            if (pm[ry][rx] == 0) continue;
+            const float qx = px[ry][rx];
+            if (fabsf(qx - ccx) < 3.0f) {
+              const float w = cj.gk[(ik + h) * cj.gwin + (jk + h)];
+              s0 += qx * w;
+              s1 += py[ry][rx] * w;
+              s2 += pz[ry][rx] * w;
+              cnt += w;
+            }
+          }
+        if (cnt > 0.8f) {
+          if (cj.semantics == PSGC_SEM_CPU) {
+            const float inv = 1.f / cnt;
+            o0 = s0 * inv;
+            o1 = s1 * inv;
+            o2 = s2 * inv;
+          } else {
+            o0 = s0 / cnt;
+            o1 = s1 / cnt;
+            o2 = s2 / cnt;
+            op = 1;
+          }
+        }
+      }
+      cj.xyzg[p * 3 + 0] = o0;
+      cj.xyzg[p * 3 + 1] = o1;
+      cj.xyzg[p * 3 + 2] = o2;
+      cj.Pg[p] = op;
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------- SOR --
+
+constexpr int SH_MAX = 8;
+constexpr int SL = CT + 2 * SH_MAX;
+
+// Loads points and mask of the tile plus a halo of `h` into LDS (mask 0
+// outside the image).
+template <int LD>
+__device__ inline void load_cloud_tile(const float* xyz, const uint8_t* P, const double* md,
+                                       const double* thr, int rows, int cols, int y0, int x0,
+                                       int h, float (*X)[LD + 1], float (*Y)[LD + 1],
+                                       float (*Z)[LD + 1], uint8_t (*M)[LD + 1]) {
+  const int L = CT + 2 * h;
+  const double th = thr ? *thr : 0.0;
+  for (int idx = threadIdx.x; idx < L * L; idx += 256) {
+    const int rr = idx / L, cc = idx % L;
+    const int gy = y0 - h + rr, gx = x0 - h + cc;
+    float a = 0.f, b = 0.f, c = 0.f;
+    uint8_t v = 0;
+    if (gy >= 0 && gy < rows && gx >= 0 && gx < cols) {
+      const size_t p = (size_t)gy * cols + gx;
+      v = P[p] != 0;
+      if (v && md) {
+        const double d = md[p];
+        v = d >= 0.0 && d <= th;
+      }
+      a = xyz[p * 3 + 0];
+      b = xyz[p * 3 + 1];
+      c = xyz[p * 3 + 2];
+    }
+    X[rr][cc] = a;
+    Y[rr][cc] = b;
+    Z[rr][cc] = c;
+    M[rr][cc] = v;
+  }
+}
+
+// Keeps the KMAX smallest values seen (ascending in best[0..cnt)), capacity k.
+template <int KMAX>
+__device__ inline void keep_smallest(float (&best)[KMAX], int& cnt, int k, float v) {
+  if (cnt == k && !(v < best[k - 1])) return;
+#pragma unroll
+  for (int t = 0; t < KMAX; ++t) {
+    if (t < cnt) {
+      const float b = best[t];
+      const bool sw = v < b;
+      best[t] = sw ? v : b;
+      v = sw ? b : v;
+    } else if (t == cnt) {
+      best[t] = v;
+    }
+  }
+  if (cnt < k) ++cnt;
+}
+
+template <int KMAX>
+__global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __restrict__ jobs,
+                                                            int njobs, int64_t total) {
+  __shared__ float X[SL][SL + 1], Y[SL][SL + 1], Z[SL][SL + 1];
+  __shared__ uint8_t M[SL][SL + 1];
+  const XcdRange xr = xcd_range(total);
+  const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const CloudJob& cj = jobs[j];
+    const int64_t lt = t - cj.tile_base;
+    const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
+    const int h = cj.sor_half;
+    load_cloud_tile<SL>(cj.sor_xyz, cj.sor_P, nullptr, nullptr, cj.rows, cj.cols, y0, x0, h, X, Y,
+                        Z, M);
+    __syncthreads();
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < cj.cols && y < cj.rows) {
+      const int cy = ly + h, cx = lx + h;
+      double out = -1.0;
+      if (M[cy][cx]) {
+        const float qx = X[cy][cx], qy = Y[cy][cx], qz = Z[cy][cx];
+        float best[KMAX];
+        int cnt = 0;
+        const int k = cj.sor_k;
+        // rings of growing Chebyshev radius: near candidates first
+        for (int rad = 1; rad <= h; ++rad) {
+          for (int e = 0; e < 8 * rad; ++e) {
+            int dy, dx;
+            const int side = e / (2 * rad), off = e % (2 * rad);
+            if (side == 0) { dy = -rad; dx = -rad + off; }
+            else if (side == 1) { dx = rad; dy = -rad + off; }
+            else if (side == 2) { dy = rad; dx = rad - off; }
+            else { dx = -rad; dy = rad - off; }
+            const int ry = cy + dy, rx = cx + dx;
+            if (!M[ry][rx]) continue;
+            const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+            keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
+          }
+        }
+        if (cnt >= k) {
+          double s = 0.0;
+#pragma unroll
+          for (int m = 0; m < KMAX; ++m)
+            if (m < k) s += sqrt((double)best[m]);
+          out = s / k;
+        }
+      }
+      cj.md[(size_t)y * cj.cols + x] = out;
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------- MLS --
+
+constexpr int MH_MAX = 8;
+constexpr int ML = CT + 2 * MH_MAX;
+constexpr int MW = 2 * MH_MAX + 1;
+
+__global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restrict__ jobs,
+                                                        int njobs, int64_t total) {
+  __shared__ float X[ML][ML + 1], Y[ML][ML + 1], Z[ML][ML + 1];
+  __shared__ uint8_t M[ML][ML + 1];
+  const XcdRange xr = xcd_range(total);
+  const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
+    const int j = find_job(jobs, njobs, t);
+    const CloudJob& cj = jobs[j];
+    const int64_t lt = t - cj.tile_base;
+    const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
+    const int hw = cj.mls_hw;
+    load_cloud_tile<ML>(cj.mls_src, cj.mls_P, cj.mls_md, cj.mls_thr, cj.rows, cj.cols, y0, x0,
+                        hw, X, Y, Z, M);
+    __syncthreads();
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < cj.cols && y < cj.rows) {
+      const size_t p = (size_t)y * cj.cols + x;
+      const int cy = ly + hw, cx = lx + hw;
+      const float pi[3] = {X[cy][cx], Y[cy][cx], Z[cy][cx]};
+      float out[3] = {pi[0], pi[1], pi[2]};
+      const bool mine = M[cy][cx] != 0;
+      if (cj.mls_Pout) cj.mls_Pout[p] = mine ? 1 : 0;
+      if (mine) {
+        // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
+        // within the cutoff radius
+        uint32_t bits[MW];
+#pragma unroll
+        for (int w = 0; w < MW; ++w) bits[w] = 0u;
+        MlsFit f;
+        f.cx = f.cy = f.cz = 0.f;
+        float wsum = 0.f;
+        int nv = 0;
+        const float inv2s2 = cj.mls_inv2s2, r2 = cj.mls_r2;
+        const int W = 2 * hw + 1;
+        for (int dy = 0; dy < W; ++dy) {
+          const int ry = cy - hw + dy;
+          uint32_t row_bits = 0u;
+          for (int dx = 0; dx < W; ++dx) {
+            const int rx = cx - hw + dx;
+            if (!M[ry][rx]) continue;
+            const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+            const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+            if (d2 > r2) continue;
+            row_bits |= 1u << dx;
+            const float w = __expf(-d2 * inv2s2);
+            f.cx += w * pk[0];
+            f.cy += w * pk[1];
+            f.cz += w * pk[2];
+            wsum += w;
+            nv++;
+          }
+          bits[dy] = row_bits;
+        }
+        if (nv >= 3 && !(wsum < 1e-12f)) {
+          const float iw = 1.0f / wsum;
+          f.cx *= iw;
+          f.cy *= iw;
+          f.cz *= iw;
+          float cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          for (int dy = 0; dy < W; ++dy) {
+            const int ry = cy - hw + dy;
+            uint32_t b = bits[dy];
+            while (b) {
+              const int rx = cx - hw + __builtin_ctz(b);
+              b &= b - 1;
+              const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+              const float w = __expf(-d2 * inv2s2);
+              const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+              cov[0][0] += w * ex * ex;
+              cov[0][1] += w * ex * ey;
+              cov[0][2] += w * ex * ez;
+              cov[1][1] += w * ey * ey;
+              cov[1][2] += w * ey * ez;
+              cov[2][2] += w * ez * ez;
+            }
+          }
+          mls_frame(cov, &f);
+          const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
+          if (cur == 0) {
+            mls_plane(f, pi, out);
+          } else {
+            const int nc = cur == 1 ? 3 : 6;
+            float AtWA[6][6], AtWf[6];
+            for (int a = 0; a < 6; ++a) {
+              AtWf[a] = 0.f;
+              for (int b = 0; b < 6; ++b) AtWA[a][b] = 0.f;
+            }
+            for (int dy = 0; dy < W; ++dy) {
+              const int ry = cy - hw + dy;
+              uint32_t b = bits[dy];
+              while (b) {
+                const int rx = cx - hw + __builtin_ctz(b);
+                b &= b - 1;
+                const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+                const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+                const float w = __expf(-d2 * inv2s2);
+                mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
+              }
+            }
+            mls_solve_eval(f, AtWA, AtWf, cur, pi, out);
+          }
+        }
+      }
+      cj.mls_dst[p * 3 + 0] = out[0];
+      cj.mls_dst[p * 3 + 1] = out[1];
+      cj.mls_dst[p * 3 + 2] = out[2];
+    }
+    __syncthreads();
+  }
+}
+
+int round_grid(int64_t total, int grid) {
+  int64_t g = grid;
+  if (g > total) g = total;
+  if (g < 1) g = 1;
+  g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+  return (int)g;
+}
+
+}  // namespace
+
+size_t match_tiled_lds(int cols, int knum) {
+  return (size_t)cols * (sizeof(double) + sizeof(int32_t)) + (size_t)(knum + 1) * 2 * sizeof(int32_t);
+}
+
+int match_tiled_max_lds() { return 64 * 1024; }
+
+int launch_decode_tiled(const DecodeJob* jobs, int njobs, int64_t total, int grid,
+                        hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  PSGC_PROF_BEGIN("decode", s);
+  decode_tiled_kernel<<<round_grid(total, grid), DC, 0, s>>>(jobs, njobs, total);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("decode", s);
+  return PSGC_OK;
+}
+
+int launch_match_tiled(const MatchJob* jobs, int njobs, int64_t total, int grid, int max_cols,
+                       int max_knum, hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  const size_t lds = match_tiled_lds(max_cols, max_knum);
+  if (lds > (size_t)match_tiled_max_lds())
+    PSGC_FAIL(PSGC_E_UNSUPPORTED, "row match needs %zu B of LDS", lds);
+  PSGC_PROF_BEGIN("match", s);
+  match_tiled_kernel<<<round_grid(total, grid), MT, lds, s>>>(jobs, njobs, total);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("match", s);
+  return PSGC_OK;
+}
+
+int launch_median_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  PSGC_PROF_BEGIN("median5", s);
+  median_tiled_kernel<<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("median5", s);
+  return PSGC_OK;
+}
+
+int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid,
+                             int max_half, hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  if (max_half > GH_MAX) PSGC_FAIL(PSGC_E_INVALID, "Gaussian window above 7");
+  PSGC_PROF_BEGIN("repro_gauss", s);
+  repro_gauss_tiled_kernel<<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("repro_gauss", s);
+  return PSGC_OK;
+}
+
+int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_half,
+                         int max_k, hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  if (max_half > SH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "SOR half window above %d", SH_MAX);
+  const int g = round_grid(total, grid);
+  PSGC_PROF_BEGIN("sor_knn", s);
+  if (max_k <= 8) sor_knn_tiled_kernel<8><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 16) sor_knn_tiled_kernel<16><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 32) sor_knn_tiled_kernel<32><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 64) sor_knn_tiled_kernel<64><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else PSGC_FAIL(PSGC_E_INVALID, "sor k=%d above 64", max_k);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("sor_knn", s);
+  return PSGC_OK;
+}
+
+int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
+                     hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  if (max_hw > MH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "MLS half window above %d", MH_MAX);
+  PSGC_PROF_BEGIN("mls", s);
+  mls_tiled_kernel<<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("mls", s);
+  return PSGC_OK;
+}
+
+}  // namespace psgc

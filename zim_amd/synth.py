@@ -271,3 +271,20 @@ def _render(cam, proj, z, seed, device, freqs, steps, ambient, modulation,
                              gen_r, noise, max_val, dtype)
     return Pair(cam, proj, left, right, z, tuple(freqs), tuple(steps), q_matrix(cam),
                 list(IDENTITY3))
+
+
+def render_c4(seed: int, device="cpu", rows=1024, cols=1280, f=1600.0, baseline=120.0,
+              z0=700.0) -> Pair:
+    """Differentiation workload: a flat plane target at z0 (slightly tilted,
+    seeded) seen by a 1280x1024 pair with the true calibration (f = 1600 px,
+    B = 120 mm), three-frequency 4-step fringes, uint16 12-bit."""
+    cam = Camera(rows, cols, f, baseline)
+    proj = Projector(1280, 1200.0 * cols / 1280.0)
+    g = torch.Generator().manual_seed(seed)
+    tilt = (torch.rand(2, generator=g, dtype=torch.float64) - 0.5) * 0.2  # mm per px
+    ii = torch.arange(rows, dtype=torch.float64, device=device).view(-1, 1)
+    jj = torch.arange(cols, dtype=torch.float64, device=device).view(1, -1)
+    z = z0 + tilt[0].item() * (jj - cam.cx) + tilt[1].item() * (ii - cam.cy)
+    z = z.expand(rows, cols).clone()
+    return _render(cam, proj, z, seed, device, (70, 64, 59), (4, 4, 4), ambient=400.0,
+                   modulation=750.0, noise=4.0, poisson=True, max_val=4095, dtype=torch.uint16)
