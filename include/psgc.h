@@ -330,6 +330,29 @@ PSGC_API int psgc_batch_run_eager(psgc_batch* batch);
  * "repro_gauss", "sor_knn", "mls"): workgroups of the persistent grid. */
 PSGC_API int psgc_batch_grid(psgc_batch* batch, const char* stage, int* grid);
 
+/* ---- rectification, compaction and output ---------------------------------
+ * psgc_rectify_maps: cv::initUndistortRectifyMap for zero distortion
+ *   (JSLAB_PhaseEpipolarReconstructor_V2.cpp:333-338): for each rectified
+ *   pixel, raw = K_raw R^T K_rect^-1 (u, v, 1); f32 maps rows x cols.
+ * psgc_remap: cv::remap of the reference's setData (…V2.cpp:368-395):
+ *   kind 0 = f64 map, INTER_LINEAR (exact bilinear weights); kind 1 = i32
+ *   map, INTER_NEAREST; BORDER_CONSTANT 0.
+ * psgc_compact: valid points (P != 0) packed row-major into out_xyz (n*3
+ *   f32) and out_gray (n u8, 255 when gray is NULL; may be NULL); *count
+ *   (host) receives n. Order = the reference PLY writers' (main.cpp:85-162).
+ * psgc_write_ply: binary little-endian PLY, float x y z and, when gray is
+ *   given, uchar red green blue (main.cpp:132-162). Host arrays. */
+PSGC_API int psgc_rectify_maps(psgc_ctx* ctx, const double K_raw[9], const double R[9],
+                               const double K_rect[9], int rows, int cols, float* map_x,
+                               float* map_y, int mem);
+PSGC_API int psgc_remap(psgc_ctx* ctx, int kind, const void* src, int src_rows, int src_cols,
+                        const float* map_x, const float* map_y, int rows, int cols, void* dst,
+                        int mem);
+PSGC_API int psgc_compact(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
+                          const uint8_t* gray, int rows, int cols, float* out_xyz,
+                          uint8_t* out_gray, int64_t* count, int mem);
+PSGC_API int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
+
 /* ---- differentiable triangulation ----------------------------------------
  * Rectification of raw phase maps under a calibration theta, the sub-pixel
  * root (JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003) with the integer

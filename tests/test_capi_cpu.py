@@ -124,3 +124,24 @@ def test_header_compiles_as_c_and_links(tmp_path):
                         str(tmp_path / "tpp"), f"-L{libdir}", "-l:libpsgc.so",
                         f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_write_ply_binary_layout(tmp_path):
+    """Binary little-endian PLY as main.cpp:132-162 writes it."""
+    lib = _capi.load()
+    pts = np.array([[1.5, -2.0, 700.25], [0.0, 1.0, 650.0]], np.float32)
+    gray = np.array([10, 200], np.uint8)
+    path = str(tmp_path / "a.ply")
+    assert lib.psgc_write_ply(path.encode(), _capi.ptr(pts), _capi.ptr(gray), 2) == 0
+    data = open(path, "rb").read()
+    head, body = data.split(b"end_header\n", 1)
+    assert b"format binary_little_endian 1.0" in head and b"element vertex 2" in head
+    assert b"property uchar red" in head
+    rec = np.frombuffer(body, dtype=np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"),
+                                              ("r", "u1"), ("g", "u1"), ("b", "u1")]))
+    assert rec["x"].tolist() == [1.5, 0.0] and rec["z"].tolist() == [700.25, 650.0]
+    assert rec["g"].tolist() == [10, 200]
+    path2 = str(tmp_path / "b.ply")
+    assert lib.psgc_write_ply(path2.encode(), _capi.ptr(pts), None, 2) == 0
+    body2 = open(path2, "rb").read().split(b"end_header\n", 1)[1]
+    assert np.array_equal(np.frombuffer(body2, "<f4").reshape(2, 3), pts)

@@ -817,6 +817,106 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   return finish(ctx, mem);
 }
 
+int psgc_rectify_maps(psgc_ctx* ctx, const double K_raw[9], const double R[9],
+                      const double K_rect[9], int rows, int cols, float* map_x, float* map_y,
+                      int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!K_raw || !R || !K_rect || !map_x || !map_y) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  // K_rect^-1 (upper triangular 3x3 with K[8] = 1 assumed general)
+  const double* K = K_rect;
+  const double det = K[0] * (K[4] * K[8] - K[5] * K[7]) - K[1] * (K[3] * K[8] - K[5] * K[6]) +
+                     K[2] * (K[3] * K[7] - K[4] * K[6]);
+  if (det == 0) PSGC_FAIL(PSGC_E_INVALID, "singular K_rect");
+  double Ki[9] = {(K[4] * K[8] - K[5] * K[7]) / det, (K[2] * K[7] - K[1] * K[8]) / det,
+                  (K[1] * K[5] - K[2] * K[4]) / det, (K[5] * K[6] - K[3] * K[8]) / det,
+                  (K[0] * K[8] - K[2] * K[6]) / det, (K[2] * K[3] - K[0] * K[5]) / det,
+                  (K[3] * K[7] - K[4] * K[6]) / det, (K[1] * K[6] - K[0] * K[7]) / det,
+                  (K[0] * K[4] - K[1] * K[3]) / det};
+  double T[9], M[9];
+  for (int a = 0; a < 3; ++a)  // T = R^T Ki
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += R[k * 3 + a] * Ki[k * 3 + b];
+      T[a * 3 + b] = s;
+    }
+  for (int a = 0; a < 3; ++a)  // M = K_raw T
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += K_raw[a * 3 + k] * T[k * 3 + b];
+      M[a * 3 + b] = s;
+    }
+  void* md;
+  PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(M), &md));
+  PSGC_HIP(hipMemcpyAsync(md, M, sizeof(M), hipMemcpyHostToDevice, ctx->stream));
+  const size_t np = (size_t)rows * cols;
+  float *mx, *my;
+  PSGC_TRY(stage_out(ctx, mem, S_MAPX, map_x, np, &mx));
+  PSGC_TRY(stage_out(ctx, mem, S_MAPY, map_y, np, &my));
+  PSGC_TRY(launch_rect_maps(rows, cols, (const double*)md, mx, my, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, map_x, mx, np));
+  PSGC_TRY(copy_back(ctx, mem, map_y, my, np));
+  // the small M buffer must stay valid until the kernel ran
+  PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  return finish(ctx, mem);
+}
+
+int psgc_remap(psgc_ctx* ctx, int kind, const void* src, int src_rows, int src_cols,
+               const float* map_x, const float* map_y, int rows, int cols, void* dst, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  PSGC_TRY(check_size(src_rows, src_cols));
+  if (!src || !map_x || !map_y || !dst) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (kind != 0 && kind != 1) PSGC_FAIL(PSGC_E_INVALID, "kind=%d", kind);
+  const size_t es = kind == 0 ? 8 : 4;
+  const size_t np = (size_t)rows * cols, ns = (size_t)src_rows * src_cols;
+  const char* s;
+  const float *mx, *my;
+  char* d;
+  PSGC_TRY(stage_in(ctx, mem, S_SRC, (const char*)src, ns * es, &s));
+  PSGC_TRY(stage_in(ctx, mem, S_MAPX, map_x, np, &mx));
+  PSGC_TRY(stage_in(ctx, mem, S_MAPY, map_y, np, &my));
+  PSGC_TRY(stage_out(ctx, mem, S_DST, (char*)dst, np * es, &d));
+  PSGC_TRY(launch_remap(kind, s, d, mx, my, rows, cols, src_rows, src_cols, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, (char*)dst, d, np * es));
+  return finish(ctx, mem);
+}
+
+int psgc_compact(psgc_ctx* ctx, const float* xyz, const uint8_t* P, const uint8_t* gray, int rows,
+                 int cols, float* out_xyz, uint8_t* out_gray, int64_t* count, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!xyz || !P || !out_xyz || !count) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const size_t np = (size_t)rows * cols;
+  const float* x;
+  const uint8_t *p, *g = nullptr;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, np * 3, &x));
+  PSGC_TRY(stage_in(ctx, mem, S_P0, P, np, &p));
+  if (gray) PSGC_TRY(stage_in(ctx, mem, S_P1, gray, np, &g));
+  float* ox;
+  uint8_t* og;
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, out_xyz, np * 3, &ox));
+  PSGC_TRY(stage_out(ctx, mem, S_P2, out_gray, np, &og));
+  void* scratch;
+  PSGC_TRY(ctx_buf(ctx, S_CNT, compact_scratch_int64(np) * sizeof(int64_t), &scratch));
+  int64_t* total = (int64_t*)scratch + compact_scratch_int64(np) - 1;
+  PSGC_TRY(launch_compact(x, p, g, np, (int64_t*)scratch, ox, og, total, ctx->stream));
+  PSGC_HIP(hipMemcpyAsync(count, total, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  const size_t n = (size_t)*count;
+  PSGC_TRY(copy_back(ctx, mem, out_xyz, ox, n * 3));
+  PSGC_TRY(copy_back(ctx, mem, out_gray, og, n));
+  return finish(ctx, mem);
+}
+
+int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n) {
+  if (!path || (n > 0 && !xyz) || n < 0) PSGC_FAIL(PSGC_E_INVALID, "bad arguments");
+  return write_ply(path, xyz, gray, n);
+}
+
 int psgc_adjoint_forward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
                          const double* phL, const double* phR, const int32_t* match_col,
                          int rows, int cols, double* xyz, uint8_t* valid, int mem) {

@@ -1,0 +1,231 @@
+// compact.hip — device-side rectification maps and remap, valid-point
+// compaction, and the binary PLY writer.
+//
+// remap:    cv::remap semantics of the reference's setData
+//           (JSLAB_PhaseEpipolarReconstructor_V2.cpp:368-395): phase and
+//           modulation INTER_LINEAR, orders INTER_NEAREST, BORDER_CONSTANT 0.
+//           Bilinear weights are exact FP64 (OpenCV quantises them to 1/32 px;
+//           that quantisation is not reproduced).
+// maps:     initUndistortRectifyMap for zero distortion
+//           (…V2.cpp:333-338): raw = K_raw R^T K_rect^-1 (u, v, 1).
+// compact:  stable row-major compaction of the points with P != 0 (the
+//           order of the reference's PLY writers, main.cpp:85-162), by
+//           wavefront ballot + per-block prefix sums.
+// PLY:      binary little-endian, float x y z [+ uchar red green blue]
+//           (main.cpp:132-162).
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "psgc_internal.h"
+
+namespace psgc {
+namespace {
+
+__global__ void rect_maps_kernel(int rows, int cols, const double* __restrict__ Mdev,
+                                 float* __restrict__ mx, float* __restrict__ my) {
+  // M = K_raw R^T K_rect^-1 (3x3 row-major)
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y * blockDim.y + threadIdx.y;
+  if (u >= cols || v >= rows) return;
+  const double x = Mdev[0] * u + Mdev[1] * v + Mdev[2];
+  const double y = Mdev[3] * u + Mdev[4] * v + Mdev[5];
+  const double w = Mdev[6] * u + Mdev[7] * v + Mdev[8];
+  const size_t p = (size_t)v * cols + u;
+  mx[p] = (float)(x / w);
+  my[p] = (float)(y / w);
+}
+
+template <typename T>
+__device__ inline T sample_linear(const T* src, int rows, int cols, float fx, float fy) {
+  const double x = fx, y = fy;
+  const double x0f = floor(x), y0f = floor(y);
+  const int x0 = (int)x0f, y0 = (int)y0f;
+  const double a = x - x0f, b = y - y0f;
+  auto at = [&](int yy, int xx) -> double {
+    return (xx >= 0 && xx < cols && yy >= 0 && yy < rows) ? (double)src[(size_t)yy * cols + xx]
+                                                           : 0.0;
+  };
+  const double v = (1 - a) * (1 - b) * at(y0, x0) + a * (1 - b) * at(y0, x0 + 1) +
+                   (1 - a) * b * at(y0 + 1, x0) + a * b * at(y0 + 1, x0 + 1);
+  return (T)v;
+}
+
+__global__ void remap_f64_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                 const float* __restrict__ mx, const float* __restrict__ my,
+                                 int rows, int cols, int src_rows, int src_cols) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (size_t)rows * cols) return;
+  dst[p] = sample_linear(src, src_rows, src_cols, mx[p], my[p]);
+}
+
+__global__ void remap_i32_nearest_kernel(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                                         const float* __restrict__ mx,
+                                         const float* __restrict__ my, int rows, int cols,
+                                         int src_rows, int src_cols) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (size_t)rows * cols) return;
+  // cv::remap INTER_NEAREST rounds the coordinates to the nearest integer
+  const int x = (int)rintf(mx[p]), y = (int)rintf(my[p]);
+  dst[p] = (x >= 0 && x < src_cols && y >= 0 && y < src_rows) ? src[(size_t)y * src_cols + x] : 0;
+}
+
+constexpr int CB = 256;   // threads per block
+constexpr int CPT = 4;    // pixels per thread
+constexpr int CPB = CB * CPT;
+
+__global__ void __launch_bounds__(CB) compact_count_kernel(const uint8_t* __restrict__ P,
+                                                           size_t n, int64_t* __restrict__ counts) {
+  __shared__ int wsum[CB / 64];
+  const size_t base = (size_t)blockIdx.x * CPB;
+  int c = 0;
+  for (int q = 0; q < CPT; ++q) {
+    const size_t p = base + (size_t)q * CB + threadIdx.x;
+    c += (p < n && P[p] != 0) ? 1 : 0;
+  }
+  // wavefront sum
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_down(c, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int w = 0; w < CB / 64; ++w) s += wsum[w];
+    counts[blockIdx.x] = s;
+  }
+}
+
+// Exclusive scan of the block counts in one workgroup (sequential chunks).
+__global__ void __launch_bounds__(1024) compact_scan_kernel(int64_t* __restrict__ counts,
+                                                            size_t nblocks,
+                                                            int64_t* __restrict__ total) {
+  __shared__ int64_t part[1024];
+  const size_t per = (nblocks + 1023) / 1024;
+  const size_t b0 = threadIdx.x * per;
+  int64_t s = 0;
+  for (size_t i = b0; i < b0 + per && i < nblocks; ++i) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t acc = 0;
+    for (int t = 0; t < 1024; ++t) {
+      const int64_t v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
+    *total = acc;
+  }
+  __syncthreads();
+  int64_t acc = part[threadIdx.x];
+  for (size_t i = b0; i < b0 + per && i < nblocks; ++i) {
+    const int64_t v = counts[i];
+    counts[i] = acc;
+    acc += v;
+  }
+}
+
+// Scatter: pixel order inside a block is q-major over the CB threads, so the
+// prefix is built per sub-row of CB pixels with wavefront ballots.
+__global__ void __launch_bounds__(CB) compact_scatter_kernel(
+    const float* __restrict__ xyz, const uint8_t* __restrict__ P, const uint8_t* __restrict__ gray,
+    size_t n, const int64_t* __restrict__ offsets, float* __restrict__ out,
+    uint8_t* __restrict__ out_gray) {
+  __shared__ int wcnt[CB / 64];
+  __shared__ int64_t running;
+  const size_t base = (size_t)blockIdx.x * CPB;
+  if (threadIdx.x == 0) running = offsets[blockIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int q = 0; q < CPT; ++q) {
+    const size_t p = base + (size_t)q * CB + threadIdx.x;
+    const bool v = p < n && P[p] != 0;
+    const unsigned long long m = __ballot(v);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int64_t off = running;
+    for (int w = 0; w < wave; ++w) off += wcnt[w];
+    if (v) {
+      const int64_t o = off + before;
+      out[o * 3 + 0] = xyz[p * 3 + 0];
+      out[o * 3 + 1] = xyz[p * 3 + 1];
+      out[o * 3 + 2] = xyz[p * 3 + 2];
+      if (out_gray) out_gray[o] = gray ? gray[p] : 255;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < CB / 64; ++w) s += wcnt[w];
+      running += s;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+int launch_rect_maps(int rows, int cols, const double* M_dev, float* mx, float* my,
+                     hipStream_t s) {
+  dim3 b(16, 16), g((cols + 15) / 16, (rows + 15) / 16);
+  rect_maps_kernel<<<g, b, 0, s>>>(rows, cols, M_dev, mx, my);
+  PSGC_LAUNCHED();
+  return PSGC_OK;
+}
+
+int launch_remap(int kind, const void* src, void* dst, const float* mx, const float* my, int rows,
+                 int cols, int src_rows, int src_cols, hipStream_t s) {
+  const size_t n = (size_t)rows * cols;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  PSGC_PROF_BEGIN("remap", s);
+  if (kind == 0)
+    remap_f64_kernel<<<g, 256, 0, s>>>((const double*)src, (double*)dst, mx, my, rows, cols,
+                                        src_rows, src_cols);
+  else
+    remap_i32_nearest_kernel<<<g, 256, 0, s>>>((const int32_t*)src, (int32_t*)dst, mx, my, rows,
+                                               cols, src_rows, src_cols);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("remap", s);
+  return PSGC_OK;
+}
+
+size_t compact_scratch_int64(size_t n) { return (n + CPB - 1) / CPB + 1; }
+
+int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size_t n,
+                   int64_t* scratch, float* out, uint8_t* out_gray, int64_t* total,
+                   hipStream_t s) {
+  const size_t nb = (n + CPB - 1) / CPB;
+  PSGC_PROF_BEGIN("compact", s);
+  compact_count_kernel<<<(unsigned)nb, CB, 0, s>>>(P, n, scratch);
+  PSGC_LAUNCHED();
+  compact_scan_kernel<<<1, 1024, 0, s>>>(scratch, nb, total);
+  PSGC_LAUNCHED();
+  compact_scatter_kernel<<<(unsigned)nb, CB, 0, s>>>(xyz, P, gray, n, scratch, out, out_gray);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("compact", s);
+  return PSGC_OK;
+}
+
+int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n) {
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) PSGC_FAIL(PSGC_E_INVALID, "cannot open %s", path);
+  std::fprintf(fp, "ply\nformat binary_little_endian 1.0\nelement vertex %lld\n", (long long)n);
+  std::fprintf(fp, "property float x\nproperty float y\nproperty float z\n");
+  if (gray) std::fprintf(fp, "property uchar red\nproperty uchar green\nproperty uchar blue\n");
+  std::fprintf(fp, "end_header\n");
+  bool ok = true;
+  if (!gray) {
+    ok = std::fwrite(xyz, sizeof(float) * 3, (size_t)n, fp) == (size_t)n;
+  } else {
+    std::vector<unsigned char> rec(15);
+    for (int64_t i = 0; i < n && ok; ++i) {
+      std::memcpy(rec.data(), xyz + i * 3, 12);
+      rec[12] = rec[13] = rec[14] = gray[i];
+      ok = std::fwrite(rec.data(), 1, 15, fp) == 15;
+    }
+  }
+  if (std::fclose(fp) != 0) ok = false;
+  if (!ok) PSGC_FAIL(PSGC_E_INVALID, "write to %s failed", path);
+  return PSGC_OK;
+}
+
+}  // namespace psgc

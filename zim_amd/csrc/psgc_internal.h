@@ -50,7 +50,8 @@ enum Slot : int {
   S_FRAMES = 0, S_PHASE, S_PHASE_RAW, S_K, S_MOD, S_VALID,
   S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
-  S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_NUM_SLOTS
+  S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_MAPX, S_MAPY, S_SRC, S_DST, S_CNT,
+  S_NUM_SLOTS
 };
 
 }  // namespace psgc
@@ -158,6 +159,16 @@ int launch_adj_backward(const double* theta, const double* nominal, const double
                         const double* phR, const int32_t* mc, int rows, int cols,
                         const double* gxyz, double* gth, double* gphL, double* gphR,
                         hipStream_t s);
+
+int launch_rect_maps(int rows, int cols, const double* M_dev, float* mx, float* my,
+                     hipStream_t s);
+int launch_remap(int kind, const void* src, void* dst, const float* mx, const float* my, int rows,
+                 int cols, int src_rows, int src_cols, hipStream_t s);
+size_t compact_scratch_int64(size_t n);
+int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size_t n,
+                   int64_t* scratch, float* out, uint8_t* out_gray, int64_t* total,
+                   hipStream_t s);
+int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
 
 // Validates a decode configuration and fills everything but the frames.
 int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
