@@ -13,10 +13,12 @@ C2_SMALL = dict(rows=192, cols=256)
 
 @functools.lru_cache(maxsize=None)
 def c2_small(seed=3, rows=192, cols=256):
-    """Config-2 shaped pair (u16 12-bit, 70/64/59 heterodyne, 4 steps),
-    scaled down so the CPU restatement runs in well under a second."""
-    f = 2400.0 * cols / 2448.0
-    p = synth.render_c2(seed, rows=rows, cols=cols, f=f)
+    """Config-2 shaped pair (u16 12-bit, 70/64/59 heterodyne, 4 steps) cut
+    down so the CPU restatement runs in well under a second: the full-size
+    focal length (so the 0.2-0.4 mm pixel pitch and the filter radii keep
+    their meaning) with a short 20 mm baseline (so the disparity fits the
+    narrow image)."""
+    p = synth.render_c2(seed, rows=rows, cols=cols, f=2400.0, baseline=20.0)
     left = [x.numpy() for x in p.left]
     right = [x.numpy() for x in p.right]
     return p, left, right

@@ -124,7 +124,8 @@ def test_stereo_match_stage_isolated(ctx):
     assert (ref_mc >= 0).mean() > 0.3
 
 
-def test_median_reproject(ctx):
+def test_median_reproject(kctx):
+    ctx = kctx
     g = np.random.default_rng(1)
     disp = (g.random((97, 131)) * 600).astype(np.float32)
     disp[g.random(disp.shape) < 0.2] = 0
@@ -174,7 +175,8 @@ def test_ror(ctx):
 
 
 @pytest.mark.parametrize("k,half", [(30, 4), (8, 2), (1, 1)])
-def test_sor(ctx, k, half):
+def test_sor(kctx, k, half):
+    ctx = kctx
     xyz, P = scenes.rng_points(90, 110, seed=6)
     rx, rP, rmd, rst = oracle.sor(xyz, P, k, 1.0, half)
     gx, gP = np.empty_like(rx), np.empty_like(rP)
@@ -191,7 +193,8 @@ def test_sor(ctx, k, half):
 
 @pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("sem", [0, 1])
-def test_mls(ctx, order, sem):
+def test_mls(kctx, order, sem):
+    ctx = kctx
     xyz, P = scenes.rng_points(64, 80, seed=8, holes=0.3)
     ref = oracle.mls(xyz, P, 4, 0.5, order, 1.5, 2, sem)
     got = np.empty_like(ref)

@@ -6,19 +6,32 @@ xyz differentiable with respect to theta (9 calibration parameters: f, cx1,
 cy1, cx2, cy2, baseline, wx, wy, wz) and to both raw phase maps. The
 autograd node is a torch::autograd::Function in the C++ extension
 (csrc/torch_ext.cpp, built by zim_amd._build.build_torch()).
+
+`refine_plane` re-rectifies the raw phase and order maps under the current
+calibration on the GPU (psgc_rectify_maps + psgc_remap), re-runs the
+order-constrained match (psgc_stereo_match), holds that integer match fixed
+and descends the plane-flatness objective along the adjoint gradient with a
+backtracking step. A single plane constrains neither scale nor focal length
+nor a pure disparity offset (all map planes to planes), so by default only
+the principal points and the rectification rotation are free.
 """
 from __future__ import annotations
 
+import ctypes as C
 import importlib.util
+import math
 import os
 
+import numpy as np
 import torch
 
-from . import Context, default_context
+from . import Context, _capi, default_context
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXT_DIR = os.path.join(HERE, "_build", "torch")
 _ext = None
+
+THETA_NAMES = ("f", "cx1", "cy1", "cx2", "cy2", "baseline", "wx", "wy", "wz")
 
 
 def _load():
@@ -27,7 +40,6 @@ def _load():
         path = os.path.join(EXT_DIR, "psgc_torch.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} not built: run zim_amd._build.build_torch()")
-        from . import _capi
         _capi.load()  # libpsgc.so first (the extension links against it)
         spec = importlib.util.spec_from_file_location("psgc_torch", path)
         mod = importlib.util.module_from_spec(spec)
@@ -53,30 +65,110 @@ def plane_loss(xyz, valid):
     return torch.linalg.eigvalsh(e.T @ e)[0]
 
 
-THETA_NAMES = ("f", "cx1", "cy1", "cx2", "cy2", "baseline", "wx", "wy", "wz")
+def plane_flatness(xyz, valid):
+    """Scale-invariant flatness: smallest eigenvalue over the trace of the
+    scatter matrix of the valid points."""
+    m = valid.bool()
+    pts = xyz[m]
+    e = pts - pts.mean(0)
+    ev = torch.linalg.eigvalsh(e.T @ e)
+    return ev[0] / ev.sum()
 
 
-def refine_plane(theta0, phL, phR, match_col, nominal, iters=60, lr=None, ctx=None,
-                 free=None):
-    """Plane-target calibration refinement: minimises the plane-fit residual
-    of the reconstructed target over the free calibration parameters by
-    gradient descent with per-parameter scales (Adam). Returns (theta,
-    history of losses)."""
-    scale = torch.tensor([10.0, 1.0, 1.0, 1.0, 1.0, 0.5, 1e-3, 1e-3, 1e-3], dtype=torch.float64)
-    lr = lr if lr is not None else 0.05
-    free = free if free is not None else list(range(9))
+def default_scale(theta0):
+    """Perturbation scale of each parameter: 1 % focal, 3 px principal
+    points, 0.5 % baseline, 0.05 deg rotations."""
+    t = torch.as_tensor(theta0, dtype=torch.float64)
+    return torch.tensor([0.01 * float(t[0]), 3.0, 3.0, 3.0, 3.0, 0.005 * float(t[5]),
+                         math.radians(0.05), math.radians(0.05), math.radians(0.05)],
+                        dtype=torch.float64)
+
+
+def _rot(wx, wy, wz):
+    cx, sx, cy, sy, cz, sz = (math.cos(wx), math.sin(wx), math.cos(wy), math.sin(wy),
+                              math.cos(wz), math.sin(wz))
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def rectified_match(theta, nominal, phL, phR, kL, kR, pha_f, grad_threshold=1.0,
+                    ctx: Context | None = None):
+    """Rectifies the raw maps under theta on the GPU (bilinear phase,
+    nearest order) and returns the integer match column per left pixel."""
+    ctx = (ctx or default_context()).bind_stream()
+    lib = ctx.lib
+    f, cx1, cy1, cx2, cy2, B, wx, wy, wz = [float(v) for v in theta]
+    f0, cLx, cLy, cRx, cRy = [float(v) for v in nominal]
+    rows, cols = phL.shape
+    dev = phL.device
+    K = lambda ff, a, b: np.array([[ff, 0, a], [0, ff, b], [0, 0, 1.0]])  # noqa: E731
+    out = []
+    for Kraw, R, Krect, ph, k in ((K(f0, cLx, cLy), np.eye(3), K(f, cx1, cy1), phL, kL),
+                                  (K(f0, cRx, cRy), _rot(wx, wy, wz).T, K(f, cx2, cy2), phR, kR)):
+        mx = torch.empty((rows, cols), dtype=torch.float32, device=dev)
+        my = torch.empty_like(mx)
+        _capi.check(lib.psgc_rectify_maps(ctx.h, _capi.ptr(np.ascontiguousarray(Kraw)),
+                                          _capi.ptr(np.ascontiguousarray(R)),
+                                          _capi.ptr(np.ascontiguousarray(Krect)), rows, cols,
+                                          _capi.ptr(mx), _capi.ptr(my), _capi.MEM_DEVICE))
+        phr = torch.empty_like(ph)
+        kr = torch.empty_like(k)
+        _capi.check(lib.psgc_remap(ctx.h, 0, _capi.ptr(ph), rows, cols, _capi.ptr(mx),
+                                   _capi.ptr(my), rows, cols, _capi.ptr(phr), _capi.MEM_DEVICE))
+        _capi.check(lib.psgc_remap(ctx.h, 1, _capi.ptr(k), rows, cols, _capi.ptr(mx),
+                                   _capi.ptr(my), rows, cols, _capi.ptr(kr), _capi.MEM_DEVICE))
+        out.append((phr, kr))
+    disp = torch.empty((rows, cols), dtype=torch.float64, device=dev)
+    mc = torch.empty((rows, cols), dtype=torch.int32, device=dev)
+    (pl, kl), (pr, kr) = out
+    _capi.check(lib.psgc_stereo_match(ctx.h, _capi.ptr(pl), _capi.ptr(pr), _capi.ptr(kl),
+                                      _capi.ptr(kr), rows, cols, int(pha_f), grad_threshold,
+                                      _capi.ptr(disp), _capi.ptr(mc), _capi.MEM_DEVICE))
+    return mc
+
+
+def refine_plane(theta_start, theta_ref, phL, phR, kL, kR, nominal, pha_f, outer=6, inner=6,
+                 step=0.5, free=(1, 2, 3, 4, 6, 7, 8), ctx: Context | None = None):
+    """Plane-target refinement by backtracking gradient descent in the
+    normalized coordinates z = (theta - theta_ref) / default_scale. Returns
+    (theta, history of (flatness before, flatness after) per outer step)."""
+    ctx = ctx or default_context()
+    scale = default_scale(theta_ref)
+    ref = torch.as_tensor(theta_ref, dtype=torch.float64)
     mask = torch.zeros(9, dtype=torch.float64)
-    mask[free] = 1.0
-    theta0 = torch.as_tensor(theta0, dtype=torch.float64)
-    z = torch.zeros(9, dtype=torch.float64, requires_grad=True)
-    opt = torch.optim.Adam([z], lr=lr)
+    mask[list(free)] = 1.0
+    z = ((torch.as_tensor(theta_start, dtype=torch.float64) - ref) / scale).clone()
     hist = []
-    for _ in range(iters):
-        opt.zero_grad()
-        theta = theta0 + z * scale * mask
-        xyz, valid = triangulate(theta, phL, phR, match_col, nominal, ctx)
-        loss = plane_loss(xyz, valid)
-        loss.backward()
-        opt.step()
-        hist.append(float(loss.detach()))
-    return (theta0 + z.detach() * scale * mask), hist
+
+    def obj(zz, mc, grad=False):
+        th = (ref + zz * scale).requires_grad_(grad)
+        v = plane_flatness(*triangulate(th, phL, phR, mc, nominal, ctx))
+        if grad:
+            v.backward()
+            return v.detach(), th.grad * scale * mask
+        return v.detach()
+
+    for _ in range(outer):
+        mc = rectified_match(ref + z * scale, nominal, phL, phR, kL, kR, pha_f, ctx=ctx)
+        start = None
+        for _ in range(inner):
+            f0, g = obj(z, mc, grad=True)
+            if start is None:
+                start = float(f0)
+            gn = float(g.norm())
+            if gn == 0.0:
+                break
+            d = -g / gn
+            a = step
+            while a > 1e-4:
+                f1 = obj(z + a * d, mc)
+                if f1 < f0:
+                    z = z + a * d
+                    break
+                a *= 0.5
+            else:
+                break
+        hist.append((start, float(obj(z, mc))))
+    return ref + z * scale, hist

@@ -532,7 +532,10 @@ int psgc_median5(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols
   PSGC_TRY(stage_in(ctx, mem, S_DISP, src, npix, &s));
   PSGC_TRY(stage_out(ctx, mem, S_DISP2, dst, npix, &d));
   if (s == d) PSGC_FAIL(PSGC_E_INVALID, "median5 cannot run in place");
-  PSGC_TRY(launch_median5(s, d, rows, cols, ctx->stream));
+  if (ctx->kernels == PSGC_KERNELS_TILED)
+    PSGC_TRY(tiled_median(ctx, s, d, rows, cols, ctx->stream));
+  else
+    PSGC_TRY(launch_median5(s, d, rows, cols, ctx->stream));
   PSGC_TRY(copy_back(ctx, mem, dst, d, npix));
   return finish(ctx, mem);
 }
@@ -638,8 +641,12 @@ int psgc_filter_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* ds
   void *part, *st;
   PSGC_TRY(ctx_buf(ctx, S_PARTIAL, sor_partial_doubles(npix) * sizeof(double), &part));
   PSGC_TRY(ctx_buf(ctx, S_STATS, 4 * sizeof(double), &st));
-  PSGC_TRY(launch_sor(sx, sp, dx, dp, rows, cols, k, std_mul, half_window, md, (double*)part,
-                      (double*)st, ctx->stream));
+  if (ctx->kernels == PSGC_KERNELS_TILED)
+    PSGC_TRY(tiled_sor(ctx, sx, sp, dx, dp, rows, cols, k, std_mul, half_window, md, (double*)part,
+                       (double*)st, ctx->stream));
+  else
+    PSGC_TRY(launch_sor(sx, sp, dx, dp, rows, cols, k, std_mul, half_window, md, (double*)part,
+                        (double*)st, ctx->stream));
   PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
   PSGC_TRY(copy_back(ctx, mem, dst_P, dp, npix));
   if (mean_dist && mem == PSGC_MEM_HOST) PSGC_TRY(copy_back(ctx, mem, mean_dist, md, npix));
@@ -676,6 +683,12 @@ int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t* P, float
   // ping-pong: the last pass must land in dx
   void* tmp = nullptr;
   if (n_iter > 1) PSGC_TRY(ctx_buf(ctx, S_XYZ2, npix * 3 * sizeof(float), &tmp));
+  if (ctx->kernels == PSGC_KERNELS_TILED) {
+    PSGC_TRY(tiled_mls(ctx, sx, sp, dx, (float*)tmp, rows, cols, window_size, spatial_sigma, order,
+                       r2, n_iter, semantics, ctx->stream));
+    PSGC_TRY(copy_back(ctx, mem, dst_xyz, dx, npix * 3));
+    return finish(ctx, mem);
+  }
   const float* in = sx;
   for (int it = 0; it < n_iter; ++it) {
     const int remaining = n_iter - 1 - it;

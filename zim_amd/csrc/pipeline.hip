@@ -91,6 +91,7 @@ struct Plan {
   std::vector<int32_t*> gray_out_k;
   std::vector<const uint8_t*> gray_valid;
   int max_cols = 0, max_knum = 0, max_ghalf = 0, max_sor_half = 0, max_sor_k = 0, max_mls_hw = 0;
+  int min_sor_k = 1 << 30;
   std::map<std::string, int> grid;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -297,6 +298,7 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     pl->sor.host.push_back(c);
     pl->max_sor_half = std::max(pl->max_sor_half, rc.sor_window);
     pl->max_sor_k = std::max(pl->max_sor_k, rc.sor_k);
+    pl->min_sor_k = std::min(pl->min_sor_k, rc.sor_k);
     pp.sor_src = xyzg;
     pp.md = md;
     pp.partial = part;
@@ -372,7 +374,8 @@ static int run_plan(Plan* pl, hipStream_t s) {
                                       g("repro_gauss", 8), pl->max_ghalf, s));
   if (!pl->sor.host.empty())
     PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
-                                  g("sor_knn", 8), pl->max_sor_half, pl->max_sor_k, s));
+                                  g("sor_knn", 8), pl->max_sor_half, pl->min_sor_k,
+                                  pl->max_sor_k, s));
   for (const PairPost& pp : pl->post) {
     const size_t np = (size_t)pp.rows * pp.cols;
     if (pp.outlier == PSGC_OUTLIER_ROR) {
@@ -436,7 +439,8 @@ static int autotune(Plan* pl, hipStream_t s) {
                                             grid, pl->max_ghalf, s));
         else if (n == "sor_knn")
           PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
-                                        grid, pl->max_sor_half, pl->max_sor_k, s));
+                                        grid, pl->max_sor_half, pl->min_sor_k,
+                                        pl->max_sor_k, s));
         else
           PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
                                     pl->mls[0].total, grid, pl->max_mls_hw, s));
@@ -679,6 +683,94 @@ int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* 
   if (r == PSGC_OK && hipStreamSynchronize(s) != hipSuccess) r = PSGC_E_HIP;
   delete pl;
   return r;
+}
+
+}  // namespace psgc
+
+namespace psgc {
+
+namespace {
+int upload_job(psgc_ctx* ctx, const CloudJob& j, CloudJob** out, hipStream_t s) {
+  void* d;
+  PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(CloudJob), &d));
+  PSGC_HIP(hipMemcpyAsync(d, &j, sizeof(CloudJob), hipMemcpyHostToDevice, s));
+  *out = (CloudJob*)d;
+  return PSGC_OK;
+}
+
+CloudJob base_job(int rows, int cols) {
+  CloudJob c{};
+  c.rows = rows;
+  c.cols = cols;
+  c.tiles_x = tiles(cols, 16);
+  c.tiles_y = tiles(rows, 16);
+  c.tile_base = 0;
+  return c;
+}
+}  // namespace
+
+int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols, hipStream_t s) {
+  CloudJob c = base_job(rows, cols);
+  c.disp_in = src;
+  c.disp_med = dst;
+  CloudJob* d;
+  PSGC_TRY(upload_job(ctx, c, &d, s));
+  PSGC_TRY(launch_median_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 8), s));
+  return hipStreamSynchronize(s) == hipSuccess ? PSGC_OK : PSGC_E_HIP;
+}
+
+int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
+              int rows, int cols, int k, float std_mul, int half, double* md, double* partial,
+              double* stats, hipStream_t s) {
+  if (half > 8) return launch_sor(xyz, P, dst, dstP, rows, cols, k, std_mul, half, md, partial,
+                                  stats, s);
+  if (k < 1 || k > 64) PSGC_FAIL(PSGC_E_INVALID, "sor k=%d outside [1, 64]", k);
+  CloudJob c = base_job(rows, cols);
+  c.sor_xyz = xyz;
+  c.sor_P = P;
+  c.sor_k = k;
+  c.sor_half = half;
+  c.md = md;
+  CloudJob* d;
+  PSGC_TRY(upload_job(ctx, c, &d, s));
+  PSGC_TRY(launch_sor_knn_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 8), half,
+                                k, k, s));
+  PSGC_TRY(launch_sor_finish(xyz, md, dst, dstP, (size_t)rows * cols, std_mul, partial, stats, s));
+  return hipStreamSynchronize(s) == hipSuccess ? PSGC_OK : PSGC_E_HIP;
+}
+
+int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, float* tmp, int rows,
+              int cols, int hw, float sigma, int order, float r2, int iterations, int semantics,
+              hipStream_t s) {
+  if (hw > 8) {
+    const float* in = src;
+    for (int it = 0; it < iterations; ++it) {
+      float* out = ((iterations - 1 - it) % 2 == 0) ? dst : tmp;
+      PSGC_TRY(launch_mls(in, P, out, rows, cols, hw, sigma, order, r2, semantics, s));
+      in = out;
+    }
+    return PSGC_OK;
+  }
+  const float* in = src;
+  for (int it = 0; it < iterations; ++it) {
+    float* out = ((iterations - 1 - it) % 2 == 0) ? dst : tmp;
+    CloudJob c = base_job(rows, cols);
+    c.mls_src = in;
+    c.mls_P = P;
+    c.mls_dst = out;
+    c.mls_hw = hw;
+    c.mls_order = order;
+    c.mls_sem = semantics;
+    c.mls_inv2s2 = 1.0f / (2.0f * sigma * sigma);
+    c.mls_r2 = r2;
+    CloudJob* d;
+    PSGC_TRY(upload_job(ctx, c, &d, s));
+    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 4), hw, s));
+    // the job buffer is reused by the next iteration
+    PSGC_HIP(hipStreamSynchronize(s));
+    in = out;
+  }
+  return PSGC_OK;
 }
 
 }  // namespace psgc

@@ -170,6 +170,15 @@ int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size
                    hipStream_t s);
 int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
 
+int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols,
+                 hipStream_t s);
+int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
+              int rows, int cols, int k, float std_mul, int half, double* md, double* partial,
+              double* stats, hipStream_t s);
+int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, float* tmp,
+              int rows, int cols, int hw, float sigma, int order, float r2, int iterations,
+              int semantics, hipStream_t s);
+
 // Validates a decode configuration and fills everything but the frames.
 int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_type, int rows,
                       int cols, int64_t stride, DecodeArgs* a);

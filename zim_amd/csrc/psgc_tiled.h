@@ -114,7 +114,7 @@ int launch_median_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid
 int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid,
                              int max_half, hipStream_t s);
 int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_half,
-                         int max_k, hipStream_t s);
+                         int min_k, int max_k, hipStream_t s);
 int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
                      hipStream_t s);
 

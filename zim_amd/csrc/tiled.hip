@@ -390,7 +390,31 @@ __device__ inline void keep_smallest(float (&best)[KMAX], int& cnt, int k, float
   if (cnt < k) ++cnt;
 }
 
-template <int KMAX>
+// Exactly K smallest, list prefilled with +inf: every insertion is the same
+// branch-free carry chain; the point is eligible iff the list filled up.
+template <int K>
+__device__ inline void keep_smallest_fixed(float (&best)[K], float v) {
+  if (!(v < best[K - 1])) return;
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const float b = best[t];
+    const bool sw = v < b;
+    best[t] = sw ? v : b;
+    v = sw ? b : v;
+  }
+}
+
+// Ring position e (0 <= e < 8 rad) -> offset on the Chebyshev ring `rad`.
+__device__ inline void ring_offset(int rad, int e, int* dy, int* dx) {
+  const int side = e / (2 * rad), off = e - side * (2 * rad);
+  if (side == 0) { *dy = -rad; *dx = -rad + off; }
+  else if (side == 1) { *dx = rad; *dy = -rad + off; }
+  else if (side == 2) { *dy = rad; *dx = rad - off; }
+  else { *dx = -rad; *dy = rad - off; }
+}
+
+// KFIX > 0: k == KFIX known at compile time; otherwise k <= KMAX at run time.
+template <int KMAX, int KFIX>
 __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                             int njobs, int64_t total) {
   __shared__ float X[SL][SL + 1], Y[SL][SL + 1], Z[SL][SL + 1];
@@ -412,30 +436,45 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
       double out = -1.0;
       if (M[cy][cx]) {
         const float qx = X[cy][cx], qy = Y[cy][cx], qz = Z[cy][cx];
-        float best[KMAX];
-        int cnt = 0;
-        const int k = cj.sor_k;
-        // rings of growing Chebyshev radius: near candidates first
-        for (int rad = 1; rad <= h; ++rad) {
-          for (int e = 0; e < 8 * rad; ++e) {
-            int dy, dx;
-            const int side = e / (2 * rad), off = e % (2 * rad);
-            if (side == 0) { dy = -rad; dx = -rad + off; }
-            else if (side == 1) { dx = rad; dy = -rad + off; }
-            else if (side == 2) { dy = rad; dx = rad - off; }
-            else { dx = -rad; dy = rad - off; }
-            const int ry = cy + dy, rx = cx + dx;
-            if (!M[ry][rx]) continue;
-            const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-            keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
-          }
-        }
-        if (cnt >= k) {
-          double s = 0.0;
+        if constexpr (KFIX > 0) {
+          float best[KFIX];
 #pragma unroll
-          for (int m = 0; m < KMAX; ++m)
-            if (m < k) s += sqrt((double)best[m]);
-          out = s / k;
+          for (int m = 0; m < KFIX; ++m) best[m] = __builtin_huge_valf();
+          for (int rad = 1; rad <= h; ++rad)
+            for (int e = 0; e < 8 * rad; ++e) {
+              int dy, dx;
+              ring_offset(rad, e, &dy, &dx);
+              const int ry = cy + dy, rx = cx + dx;
+              if (!M[ry][rx]) continue;
+              const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
+            }
+          if (best[KFIX - 1] < __builtin_huge_valf()) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < KFIX; ++m) s += sqrt((double)best[m]);
+            out = s / KFIX;
+          }
+        } else {
+          float best[KMAX];
+          int cnt = 0;
+          const int k = cj.sor_k;
+          for (int rad = 1; rad <= h; ++rad)
+            for (int e = 0; e < 8 * rad; ++e) {
+              int dy, dx;
+              ring_offset(rad, e, &dy, &dx);
+              const int ry = cy + dy, rx = cx + dx;
+              if (!M[ry][rx]) continue;
+              const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
+            }
+          if (cnt >= k) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < KMAX; ++m)
+              if (m < k) s += sqrt((double)best[m]);
+            out = s / k;
+          }
         }
       }
       cj.md[(size_t)y * cj.cols + x] = out;
@@ -473,22 +512,28 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
       float out[3] = {pi[0], pi[1], pi[2]};
       const bool mine = M[cy][cx] != 0;
       if (cj.mls_Pout) cj.mls_Pout[p] = mine ? 1 : 0;
+      // points removed by SOR enter MLS as zeros (the SOR output), and MLS
+      // copies its input for points outside the mask
+      if (!mine && cj.mls_md) out[0] = out[1] = out[2] = 0.f;
       if (mine) {
         // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
-        // within the cutoff radius
+        // within the cutoff radius (the same uncontracted d^2 test as the
+        // CPU restatement, so the neighbour set and every order fallback
+        // decision are identical)
         uint32_t bits[MW];
 #pragma unroll
         for (int w = 0; w < MW; ++w) bits[w] = 0u;
-        MlsFit f;
-        f.cx = f.cy = f.cz = 0.f;
-        float wsum = 0.f;
+        // weighted moments about the query point: W, m1 = sum w e,
+        // M2 = sum w e e^T with e = pk - pi
+        float W = 0.f, m1x = 0.f, m1y = 0.f, m1z = 0.f;
+        float mxx = 0.f, mxy = 0.f, mxz = 0.f, myy = 0.f, myz = 0.f, mzz = 0.f;
         int nv = 0;
         const float inv2s2 = cj.mls_inv2s2, r2 = cj.mls_r2;
-        const int W = 2 * hw + 1;
-        for (int dy = 0; dy < W; ++dy) {
+        const int Wd = 2 * hw + 1;
+        for (int dy = 0; dy < Wd; ++dy) {
           const int ry = cy - hw + dy;
           uint32_t row_bits = 0u;
-          for (int dx = 0; dx < W; ++dx) {
+          for (int dx = 0; dx < Wd; ++dx) {
             const int rx = cx - hw + dx;
             if (!M[ry][rx]) continue;
             const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
@@ -496,59 +541,76 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             if (d2 > r2) continue;
             row_bits |= 1u << dx;
             const float w = __expf(-d2 * inv2s2);
-            f.cx += w * pk[0];
-            f.cy += w * pk[1];
-            f.cz += w * pk[2];
-            wsum += w;
+            const float ex = pk[0] - pi[0], ey = pk[1] - pi[1], ez = pk[2] - pi[2];
+            const float wx = w * ex, wy = w * ey, wz = w * ez;
+            W += w;
+            m1x += wx;
+            m1y += wy;
+            m1z += wz;
+            mxx = __builtin_fmaf(wx, ex, mxx);
+            mxy = __builtin_fmaf(wx, ey, mxy);
+            mxz = __builtin_fmaf(wx, ez, mxz);
+            myy = __builtin_fmaf(wy, ey, myy);
+            myz = __builtin_fmaf(wy, ez, myz);
+            mzz = __builtin_fmaf(wz, ez, mzz);
             nv++;
           }
           bits[dy] = row_bits;
         }
-        if (nv >= 3 && !(wsum < 1e-12f)) {
-          const float iw = 1.0f / wsum;
-          f.cx *= iw;
-          f.cy *= iw;
-          f.cz *= iw;
-          float cov[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-          for (int dy = 0; dy < W; ++dy) {
-            const int ry = cy - hw + dy;
-            uint32_t b = bits[dy];
-            while (b) {
-              const int rx = cx - hw + __builtin_ctz(b);
-              b &= b - 1;
-              const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-              const float d2 = dist2(pi[0], pi[1], pi[2], pk);
-              const float w = __expf(-d2 * inv2s2);
-              const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-              cov[0][0] += w * ex * ex;
-              cov[0][1] += w * ex * ey;
-              cov[0][2] += w * ex * ez;
-              cov[1][1] += w * ey * ey;
-              cov[1][2] += w * ey * ez;
-              cov[2][2] += w * ez * ez;
-            }
-          }
+        if (nv >= 3 && !(W < 1e-12f)) {
+          const float iw = 1.0f / W;
+          const float dcx = m1x * iw, dcy = m1y * iw, dcz = m1z * iw;
+          MlsFit f;
+          f.cx = pi[0] + dcx;
+          f.cy = pi[1] + dcy;
+          f.cz = pi[2] + dcz;
+          // sum w (pk - c)(pk - c)^T = M2 - m1 m1^T / W
+          float cov[3][3] = {{mxx - m1x * dcx, mxy - m1x * dcy, mxz - m1x * dcz},
+                             {0.f, myy - m1y * dcy, myz - m1y * dcz},
+                             {0.f, 0.f, mzz - m1z * dcz}};
           mls_frame(cov, &f);
           const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
           if (cur == 0) {
             mls_plane(f, pi, out);
           } else {
-            const int nc = cur == 1 ? 3 : 6;
             float AtWA[6][6], AtWf[6];
             for (int a = 0; a < 6; ++a) {
               AtWf[a] = 0.f;
               for (int b = 0; b < 6; ++b) AtWA[a][b] = 0.f;
             }
-            for (int dy = 0; dy < W; ++dy) {
+            // frame axes relative to the query point: e_c = (pk - pi) - dc
+            for (int dy = 0; dy < Wd; ++dy) {
               const int ry = cy - hw + dy;
               uint32_t b = bits[dy];
               while (b) {
                 const int rx = cx - hw + __builtin_ctz(b);
                 b &= b - 1;
                 const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-                const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+                const float ex0 = pk[0] - pi[0], ey0 = pk[1] - pi[1], ez0 = pk[2] - pi[2];
+                const float d2 = __builtin_fmaf(ex0, ex0, __builtin_fmaf(ey0, ey0, ez0 * ez0));
                 const float w = __expf(-d2 * inv2s2);
-                mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
+                const float ex = ex0 - dcx, ey = ey0 - dcy, ez = ez0 - dcz;
+                const float u = __builtin_fmaf(ex, f.t1[0], __builtin_fmaf(ey, f.t1[1], ez * f.t1[2]));
+                const float v = __builtin_fmaf(ex, f.t2[0], __builtin_fmaf(ey, f.t2[1], ez * f.t2[2]));
+                const float fv = __builtin_fmaf(ex, f.n[0], __builtin_fmaf(ey, f.n[1], ez * f.n[2]));
+                const float bs[6] = {1.0f, u, v, u * u, u * v, v * v};
+                if (cur == 1) {
+#pragma unroll
+                  for (int a = 0; a < 3; ++a) {
+                    const float wb = w * bs[a];
+                    AtWf[a] = __builtin_fmaf(wb, fv, AtWf[a]);
+#pragma unroll
+                    for (int c2 = a; c2 < 3; ++c2) AtWA[a][c2] = __builtin_fmaf(wb, bs[c2], AtWA[a][c2]);
+                  }
+                } else {
+#pragma unroll
+                  for (int a = 0; a < 6; ++a) {
+                    const float wb = w * bs[a];
+                    AtWf[a] = __builtin_fmaf(wb, fv, AtWf[a]);
+#pragma unroll
+                    for (int c2 = a; c2 < 6; ++c2) AtWA[a][c2] = __builtin_fmaf(wb, bs[c2], AtWA[a][c2]);
+                  }
+                }
               }
             }
             mls_solve_eval(f, AtWA, AtWf, cur, pi, out);
@@ -623,15 +685,17 @@ int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int
 }
 
 int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_half,
-                         int max_k, hipStream_t s) {
+                         int min_k, int max_k, hipStream_t s) {
   if (total <= 0) return PSGC_OK;
   if (max_half > SH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "SOR half window above %d", SH_MAX);
   const int g = round_grid(total, grid);
   PSGC_PROF_BEGIN("sor_knn", s);
-  if (max_k <= 8) sor_knn_tiled_kernel<8><<<g, 256, 0, s>>>(jobs, njobs, total);
-  else if (max_k <= 16) sor_knn_tiled_kernel<16><<<g, 256, 0, s>>>(jobs, njobs, total);
-  else if (max_k <= 32) sor_knn_tiled_kernel<32><<<g, 256, 0, s>>>(jobs, njobs, total);
-  else if (max_k <= 64) sor_knn_tiled_kernel<64><<<g, 256, 0, s>>>(jobs, njobs, total);
+  if (min_k == max_k && max_k == 30) sor_knn_tiled_kernel<30, 30><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (min_k == max_k && max_k == 20) sor_knn_tiled_kernel<20, 20><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 8) sor_knn_tiled_kernel<8, 0><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 16) sor_knn_tiled_kernel<16, 0><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 32) sor_knn_tiled_kernel<32, 0><<<g, 256, 0, s>>>(jobs, njobs, total);
+  else if (max_k <= 64) sor_knn_tiled_kernel<64, 0><<<g, 256, 0, s>>>(jobs, njobs, total);
   else PSGC_FAIL(PSGC_E_INVALID, "sor k=%d above 64", max_k);
   PSGC_LAUNCHED();
   PSGC_PROF_END("sor_knn", s);

@@ -227,7 +227,7 @@ def render_c2(seed: int, device="cpu", rows=2048, cols=2448, f=2400.0,
     a 1280-px projector), uint16 12-bit, ambient 400, modulation 750,
     Poisson + N(0, 4 DN) noise."""
     cam = Camera(rows, cols, f, baseline)
-    proj = Projector(1280, 1000.0 * cols / 2448.0)
+    proj = Projector(1280, 1000.0 * f / 2400.0)  # projector/camera magnification of 1000/2400
     z = depth_bumps(cam, seed, device)
     return _render(cam, proj, z, seed, device, freqs, steps, ambient=400.0,
                    modulation=750.0, noise=4.0, poisson=True, max_val=4095,
@@ -274,7 +274,7 @@ def _render(cam, proj, z, seed, device, freqs, steps, ambient, modulation,
 
 
 def render_c4(seed: int, device="cpu", rows=1024, cols=1280, f=1600.0, baseline=120.0,
-              z0=700.0) -> Pair:
+              z0=700.0, noise=True) -> Pair:
     """Differentiation workload: a flat plane target at z0 (slightly tilted,
     seeded) seen by a 1280x1024 pair with the true calibration (f = 1600 px,
     B = 120 mm), three-frequency 4-step fringes, uint16 12-bit."""
@@ -287,4 +287,5 @@ def render_c4(seed: int, device="cpu", rows=1024, cols=1280, f=1600.0, baseline=
     z = z0 + tilt[0].item() * (jj - cam.cx) + tilt[1].item() * (ii - cam.cy)
     z = z.expand(rows, cols).clone()
     return _render(cam, proj, z, seed, device, (70, 64, 59), (4, 4, 4), ambient=400.0,
-                   modulation=750.0, noise=4.0, poisson=True, max_val=4095, dtype=torch.uint16)
+                   modulation=750.0, noise=4.0 if noise else 0.0, poisson=noise, max_val=4095,
+                   dtype=torch.uint16 if noise else torch.float64)
