@@ -220,7 +220,7 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
     """The CPU restatement of the reference path (oracle/, OpenMP), timed on
     this host's cores over a row band of one pair."""
     import oracle
-    cores = os.cpu_count() or 1
+    cores = oracle.num_threads()
     left = [f[:sample_rows].cpu().numpy() for f in pair.left]
     right = [f[:sample_rows].cpu().numpy() for f in pair.right]
     log(f"cpu baseline: one pair, rows [0,{sample_rows}), {cores} threads")

@@ -35,6 +35,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 #include "psgc.h"
 #include "psgc_math.h"
 
@@ -989,5 +991,8 @@ int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
   return PSGC_OK;
 }
+
+// Threads the OpenMP loops above run on (the timed CPU baseline's core count).
+int orc_num_threads() { return omp_get_max_threads(); }
 
 }  // extern "C"

@@ -519,14 +519,17 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
         // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
         // within the cutoff radius (the same uncontracted d^2 test as the
         // CPU restatement, so the neighbour set and every order fallback
-        // decision are identical)
+        // decision are identical). The three accumulation passes keep the
+        // CPU filter's arithmetic (JSLAB_MLS_Filter.cpp:196-340): weighted
+        // centroid, covariance about it, then the normal equations; an
+        // ill-conditioned fit amplifies any reassociation far beyond the
+        // 1e-3 mm output tolerance.
         uint32_t bits[MW];
 #pragma unroll
         for (int w = 0; w < MW; ++w) bits[w] = 0u;
-        // weighted moments about the query point: W, m1 = sum w e,
-        // M2 = sum w e e^T with e = pk - pi
-        float W = 0.f, m1x = 0.f, m1y = 0.f, m1z = 0.f;
-        float mxx = 0.f, mxy = 0.f, mxz = 0.f, myy = 0.f, myz = 0.f, mzz = 0.f;
+        MlsFit f;
+        f.cx = f.cy = f.cz = 0.f;
+        float wsum = 0.f;
         int nv = 0;
         const float inv2s2 = cj.mls_inv2s2, r2 = cj.mls_r2;
         const int Wd = 2 * hw + 1;
@@ -540,45 +543,49 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
             if (d2 > r2) continue;
             row_bits |= 1u << dx;
-            const float w = __expf(-d2 * inv2s2);
-            const float ex = pk[0] - pi[0], ey = pk[1] - pi[1], ez = pk[2] - pi[2];
-            const float wx = w * ex, wy = w * ey, wz = w * ez;
-            W += w;
-            m1x += wx;
-            m1y += wy;
-            m1z += wz;
-            mxx = __builtin_fmaf(wx, ex, mxx);
-            mxy = __builtin_fmaf(wx, ey, mxy);
-            mxz = __builtin_fmaf(wx, ez, mxz);
-            myy = __builtin_fmaf(wy, ey, myy);
-            myz = __builtin_fmaf(wy, ez, myz);
-            mzz = __builtin_fmaf(wz, ez, mzz);
+            const float w = expf(-d2 * inv2s2);
+            f.cx += w * pk[0];
+            f.cy += w * pk[1];
+            f.cz += w * pk[2];
+            wsum += w;
             nv++;
           }
           bits[dy] = row_bits;
         }
-        if (nv >= 3 && !(W < 1e-12f)) {
-          const float iw = 1.0f / W;
-          const float dcx = m1x * iw, dcy = m1y * iw, dcz = m1z * iw;
-          MlsFit f;
-          f.cx = pi[0] + dcx;
-          f.cy = pi[1] + dcy;
-          f.cz = pi[2] + dcz;
-          // sum w (pk - c)(pk - c)^T = M2 - m1 m1^T / W
-          float cov[3][3] = {{mxx - m1x * dcx, mxy - m1x * dcy, mxz - m1x * dcz},
-                             {0.f, myy - m1y * dcy, myz - m1y * dcz},
-                             {0.f, 0.f, mzz - m1z * dcz}};
+        if (nv >= 3 && !(wsum < 1e-12f)) {
+          const float iw = 1.0f / wsum;
+          f.cx *= iw;
+          f.cy *= iw;
+          f.cz *= iw;
+          float cov[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+          for (int dy = 0; dy < Wd; ++dy) {
+            const int ry = cy - hw + dy;
+            uint32_t b = bits[dy];
+            while (b) {
+              const int rx = cx - hw + __builtin_ctz(b);
+              b &= b - 1;
+              const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
+              const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+              cov[0][0] += w * ex * ex;
+              cov[0][1] += w * ex * ey;
+              cov[0][2] += w * ex * ez;
+              cov[1][1] += w * ey * ey;
+              cov[1][2] += w * ey * ez;
+              cov[2][2] += w * ez * ez;
+            }
+          }
           mls_frame(cov, &f);
           const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
           if (cur == 0) {
             mls_plane(f, pi, out);
           } else {
+            const int nc = cur == 1 ? 3 : 6;
             float AtWA[6][6], AtWf[6];
             for (int a = 0; a < 6; ++a) {
               AtWf[a] = 0.f;
               for (int b = 0; b < 6; ++b) AtWA[a][b] = 0.f;
             }
-            // frame axes relative to the query point: e_c = (pk - pi) - dc
             for (int dy = 0; dy < Wd; ++dy) {
               const int ry = cy - hw + dy;
               uint32_t b = bits[dy];
@@ -586,31 +593,8 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
                 const int rx = cx - hw + __builtin_ctz(b);
                 b &= b - 1;
                 const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-                const float ex0 = pk[0] - pi[0], ey0 = pk[1] - pi[1], ez0 = pk[2] - pi[2];
-                const float d2 = __builtin_fmaf(ex0, ex0, __builtin_fmaf(ey0, ey0, ez0 * ez0));
-                const float w = __expf(-d2 * inv2s2);
-                const float ex = ex0 - dcx, ey = ey0 - dcy, ez = ez0 - dcz;
-                const float u = __builtin_fmaf(ex, f.t1[0], __builtin_fmaf(ey, f.t1[1], ez * f.t1[2]));
-                const float v = __builtin_fmaf(ex, f.t2[0], __builtin_fmaf(ey, f.t2[1], ez * f.t2[2]));
-                const float fv = __builtin_fmaf(ex, f.n[0], __builtin_fmaf(ey, f.n[1], ez * f.n[2]));
-                const float bs[6] = {1.0f, u, v, u * u, u * v, v * v};
-                if (cur == 1) {
-#pragma unroll
-                  for (int a = 0; a < 3; ++a) {
-                    const float wb = w * bs[a];
-                    AtWf[a] = __builtin_fmaf(wb, fv, AtWf[a]);
-#pragma unroll
-                    for (int c2 = a; c2 < 3; ++c2) AtWA[a][c2] = __builtin_fmaf(wb, bs[c2], AtWA[a][c2]);
-                  }
-                } else {
-#pragma unroll
-                  for (int a = 0; a < 6; ++a) {
-                    const float wb = w * bs[a];
-                    AtWf[a] = __builtin_fmaf(wb, fv, AtWf[a]);
-#pragma unroll
-                    for (int c2 = a; c2 < 6; ++c2) AtWA[a][c2] = __builtin_fmaf(wb, bs[c2], AtWA[a][c2]);
-                  }
-                }
+                const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
+                mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
               }
             }
             mls_solve_eval(f, AtWA, AtWf, cur, pi, out);
