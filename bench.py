@@ -181,8 +181,22 @@ def main():
         log(f"plain per-pair: {plain}")
 
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(pairs[0], dcfg, rcfg, args.cpu_sample_rows, log)
+        cpu, ref = cpu_baseline(pairs[0], dcfg, rcfg, args.cpu_sample_rows, log)
+        if ref is not None:
+            # the oracle's output for the whole of pair 0 checks the batched one
+            rx, rP = ref
+            out = batch.outputs[0]
+            gP = out["P"].cpu().numpy()
+            gx = out["xyz"].cpu().numpy()
+            m = rP != 0
+            parity = {"pair": 0, "points": int(m.sum()),
+                      "mask_mismatch_px": int((gP != rP).sum()),
+                      "xyz_max_diff_mm": float(np.abs(gx[m] - rx[m]).max()) if m.any() else 0.0,
+                      "xyz_p99999_diff_mm": float(np.quantile(np.abs(gx[m] - rx[m]).max(-1),
+                                                              0.99999)) if m.any() else 0.0}
+            log(f"full-pair parity vs oracle: {parity}")
 
     line = {
         "metric": METRIC,
@@ -211,6 +225,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg},
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
     }
     print(json.dumps(line), flush=True)
     batch.close()
@@ -218,7 +233,8 @@ def main():
 
 def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
     """The CPU restatement of the reference path (oracle/, OpenMP), timed on
-    this host's cores over a row band of one pair."""
+    this host's cores over a row band of one pair. For the whole pair it also
+    returns the oracle's (xyz, P), which check the batched output."""
     import oracle
     cores = oracle.num_threads()
     left = [f[:sample_rows].cpu().numpy() for f in pair.left]
@@ -228,12 +244,13 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
     t0 = time.perf_counter()
     L = oracle.decode(dcfg, left)
     R = oracle.decode(dcfg, right)
-    oracle.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, pair.Q, pair.R1)
+    rx, rP, _, _ = oracle.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, pair.Q, pair.R1)
     dt = time.perf_counter() - t0
     pts = sample_rows * COLS
-    return {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
-            "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
-                      f"decode L+R + reconstruct, {dt:.1f} s"}
+    res = {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
+           "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
+                     f"decode L+R + reconstruct, {dt:.1f} s"}
+    return res, ((rx, rP) if sample_rows == ROWS else None)
 
 
 if __name__ == "__main__":

@@ -66,3 +66,48 @@ def rng_points(rows, cols, seed=0, holes=0.1, z0=700.0):
     xyz[out, 2] += g.normal(0, 5, out.sum()).astype(np.float32)
     xyz[P == 0] = 0
     return xyz, P
+
+
+HDR_ROWS = ((120.0, 100.0), (150.0, 140.0), (400.0, 100.0))  # (A, B) per row block
+
+
+def hdr_sweep(cols=2000, block=3, steps=8, freqs=(70.0, 64.0, 59.0), width=1280.0, sat=255.0):
+    """Noise-free f64 three-frequency fringes I_k = A + B cos(phi_f + 2 pi k / N)
+    over a sweep of projector columns x in [0, width), phi_f = 2 pi f x / width,
+    in three blocks of `block` rows with the (A, B) of HDR_ROWS: unclipped,
+    partly clipped (at most 2 of 8 samples reach `sat`) and saturated (every
+    sample reaches `sat`). Samples at or above `sat` are set to `sat`.
+    Returns (frames, phi0 per column, fringe order per column)."""
+    x = (np.arange(cols, dtype=np.float64) + 0.37) * width / cols
+    frames = []
+    for f in freqs:
+        phi = 2 * np.pi * f * x / width
+        for k in range(steps):
+            img = np.empty((3 * block, cols))
+            for b, (A, B) in enumerate(HDR_ROWS):
+                img[b * block:(b + 1) * block] = A + B * np.cos(phi + 2 * np.pi * k / steps)
+            frames.append(np.minimum(img, sat))
+    phi0 = 2 * np.pi * freqs[0] * x / width
+    order = np.round(phi0 / (2 * np.pi)).astype(np.int32)
+    return frames, phi0, order
+
+
+def hdr_config(**kw):
+    base = dict(freq=(70, 64, 59), shift_num=(8, 8, 8), modulation_threshold=0.3, enable_hdr=1,
+                hdr_saturation_gray=255.0, hdr_min_amplitude=2.0)
+    base.update(kw)
+    return DecodeConfig.default(**base)
+
+
+@functools.lru_cache(maxsize=None)
+def hdr_small(seed=6, rows=192, cols=256):
+    """An over-exposed 8-step config-2 shaped pair in uint8: the 12-bit
+    intensities scaled by 255/1500 and clipped, so that bright fringe samples
+    saturate at 255 (the case the HDR fit exists for)."""
+    p = synth.render_c2(seed, rows=rows, cols=cols, f=2400.0, baseline=20.0, steps=(8, 8, 8))
+
+    def to_u8(t):
+        v = np.round(t.numpy().astype(np.float64) * (255.0 / 1500.0))
+        return np.clip(v, 0, 255).astype(np.uint8)
+
+    return p, [to_u8(x) for x in p.left], [to_u8(x) for x in p.right]

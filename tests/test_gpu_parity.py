@@ -299,3 +299,87 @@ def test_batch_autotune_keeps_results(ctx):
     assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
     assert b.grid("mls") > 0 and b.grid("decode") > 0
     b.close()
+
+
+# ------------------------------------------------------------------- HDR --
+
+@pytest.mark.parametrize("smooth", [0, 1])
+def test_decode_hdr_sweep(kctx, smooth):
+    """Clipped noise-free fringes (f64): the device HDR fit against the
+    oracle's, including the all-saturated rows (invalid, order -1)."""
+    frames, _, _ = scenes.hdr_sweep(cols=640)
+    cfg = scenes.hdr_config(post_smooth=smooth)
+    ref, got = _decode_both(cfg, frames, kctx)
+    _assert_decode(ref, got)
+    assert (got[1][6:] == -1).all()
+
+
+def test_decode_hdr_saturated_u8(kctx):
+    _, left, right = scenes.hdr_small()
+    cfg = scenes.hdr_config()
+    for frames in (left, right):
+        assert np.mean([(f == 255).mean() for f in frames]) > 0.05  # really over-exposed
+        ref, got = _decode_both(cfg, frames, kctx)
+        _assert_decode(ref, got)
+        assert ref[3].mean() > 0.5
+
+
+def test_batch_mixed_schedules_match_oracle(ctx):
+    """One captured batch of uneven pairs with different schedules: 3-step
+    and 8-step heterodyne (uint16), a 6-bit Gray-code pair (uint8) and an
+    over-exposed 8-step pair decoded with the HDR fit (uint8). Each pair must
+    equal the oracle run on it alone, on every replay."""
+    from zim_amd import synth
+    specs = []
+    for seed, rows, cols, steps in ((41, 64, 96, 3), (42, 80, 120, 8)):
+        p = synth.render_c2(seed, rows=rows, cols=cols, f=2400.0, baseline=20.0,
+                            steps=(steps,) * 3)
+        specs.append((p, [x.numpy() for x in p.left], [x.numpy() for x in p.right],
+                      scenes.c2_decode_config(shift_num=(steps,) * 3),
+                      scenes.c2_recon_config(mls_window_size=4), 70))
+    p, lh, rh = scenes.c1_small(5)
+    specs.append((p, lh, rh, scenes.c1_gray_config(),
+                  scenes.c2_recon_config(use_mls=0, sor_k=20), 50))
+    p, lh, rh = scenes.hdr_small()
+    specs.append((p, lh, rh, scenes.hdr_config(), scenes.c2_recon_config(mls_window_size=4), 70))
+    pairs, refs = [], []
+    for p, lh, rh, dcfg, rcfg, pha_f in specs:
+        pairs.append(dict(left=[dev(f) for f in lh], right=[dev(f) for f in rh], decode=dcfg,
+                          recon=rcfg, pha_f=pha_f, Q=p.Q, R1=p.R1, disp=True))
+        Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+        refs.append(oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], pha_f, p.Q, p.R1))
+    b = zim_amd.Batch(pairs, ctx, graph=True)
+    for rep in range(3):
+        outs = b.run()
+        torch.cuda.synchronize()
+        for i, (out, (rx, rP, rd, _)) in enumerate(zip(outs, refs)):
+            assert rP.sum() > 0, i
+            assert np.array_equal(host(out["P"]), rP), (rep, i, (host(out["P"]) != rP).sum())
+            assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL, (rep, i)
+            assert np.abs(host(out["disp"]) - rd).max() <= DISP_TOL, (rep, i)
+    b.close()
+
+
+def test_full_size_pair_matches_oracle(ctx):
+    """One full 2448x2048 config-2 pair (rendered on the GPU, decoded and
+    reconstructed by the batched pipeline) against the oracle on the same
+    frames: masks bit-exact, XYZ within 1e-3 mm over ~4 M points."""
+    from zim_amd import synth
+    p = synth.render_c2(1000, device="cuda")
+    dcfg = scenes.c2_decode_config()
+    rcfg = scenes.c2_recon_config()
+    b = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rcfg, pha_f=70,
+                            Q=p.Q, R1=p.R1)], ctx, graph=False)
+    out = b.run()[0]
+    torch.cuda.synchronize()
+    lh = [host(f) for f in p.left]
+    rh = [host(f) for f in p.right]
+    Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+    rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
+    gP, gx = host(out["P"]), host(out["xyz"])
+    m = rP != 0
+    print("points", int(m.sum()), "max |dxyz|", np.abs(gx[m] - rx[m]).max())
+    assert m.sum() > 2_000_000
+    assert np.array_equal(gP, rP), f"mask mismatch {(gP != rP).sum()}"
+    assert np.abs(gx - rx).max() <= XYZ_TOL
+    b.close()

@@ -432,3 +432,68 @@ def test_adjoint_reference_model_at_true_calibration_equals_reprojection():
                                                        torch.from_numpy(mc)))
     loss.backward()
     assert torch.isfinite(th.grad).all()
+
+
+# -------------------------------------------------------------- HDR fit --
+
+def test_hdr_fit_recovers_clipped_fringes():
+    """Closed form: noise-free samples I_k = A + B cos(phi + 2 pi k/N) with the
+    saturated ones clipped. The least-squares fit over the unsaturated subset
+    is exact whenever 3 or more samples remain, so the unwrapped phase, the
+    fringe order and m = B/A come back to rounding; a pixel with fewer than 3
+    unsaturated samples is invalid with order -1 (the HDR initial value).
+    The plain sums over the same clipped samples are biased (control)."""
+    frames, phi0, order = scenes.hdr_sweep()
+    cfg = scenes.hdr_config(post_smooth=0)
+    ph, k, mod, valid = oracle.decode(cfg, frames)
+    for b, (A, B) in enumerate(scenes.HDR_ROWS[:2]):
+        for r in range(3 * b, 3 * b + 3):
+            assert (valid[r] == 255).all()
+            assert np.array_equal(k[r], order)
+            assert np.abs(ph[r] - phi0).max() < 1e-9
+            assert np.abs(mod[r] - B / A).max() < 1e-12
+    assert (valid[6:] == 0).all() and (k[6:] == -1).all() and (ph[6:] == 0).all()
+    # control: the blurred sums. A symmetric blur keeps a sinusoid's phase
+    # away from the replicated first/last column and rows of a block agree.
+    plain = scenes.hdr_config(post_smooth=0, enable_hdr=0, preblur_sigma=0.0)
+    ph2, _, _, valid2 = oracle.decode(plain, frames)
+    assert np.abs(ph2[0, 1:-1] - phi0[1:-1]).max() < 1e-9   # unclipped row: exact
+    assert np.abs(ph2[4, 1:-1] - phi0[1:-1]).max() > 1e-3   # clipped row: biased
+
+
+def test_hdr_subset_weights_are_a_pseudo_inverse():
+    """With one sample clipped the fit uses N-1 equations; rotating the
+    phase so that a different sample saturates must give the same phase
+    error (zero): every subset's weights reproduce (A, C, S) exactly."""
+    N = 8
+    frames = []
+    cols = 64
+    phi = np.linspace(-np.pi + 0.01, np.pi - 0.01, cols)
+    A, B, sat = 100.0, 90.0, 185.0   # exactly the samples within ~19 deg of the peak clip
+    for f in (70.0, 64.0, 59.0):
+        for kk in range(N):
+            img = np.tile(A + B * np.cos(phi + 2 * np.pi * kk / N), (3, 1))
+            frames.append(np.minimum(img, sat))
+    cfg = scenes.hdr_config(post_smooth=0, hdr_saturation_gray=sat, avg_three_phase=0)
+    ph, k, mod, valid = oracle.decode(cfg, frames)
+    assert (valid == 255).all()
+    # all three levels carry the same phase: phase12 = phase23 = 2 pi, and the
+    # unwrapped phase equals the wrapped one plus 2 pi k
+    wrapped = ph - 2 * np.pi * k
+    assert np.abs(wrapped[1] - phi).max() < 1e-10
+    assert np.abs(mod[1] - B / A).max() < 1e-12
+
+
+def test_hdr_min_amplitude_gate():
+    """The fit rejects a pixel whose amplitude |(C, S)| = B is not above
+    hdr_min_amplitude, independently of the modulation threshold (0 here)."""
+    cols = 32
+    phi = np.linspace(-3.0, 3.0, cols)
+    for B, ok in ((1.9, False), (2.1, True)):
+        frames = [np.tile(100.0 + B * np.cos(phi + 2 * np.pi * kk / 4), (3, 1))
+                  for _ in range(3) for kk in range(4)]
+        cfg = scenes.hdr_config(post_smooth=0, shift_num=(4, 4, 4), modulation_threshold=0.0)
+        _, k, _, valid = oracle.decode(cfg, frames)
+        assert (valid == (255 if ok else 0)).all()
+        if not ok:
+            assert (k == -1).all()

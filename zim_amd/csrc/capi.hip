@@ -215,8 +215,6 @@ int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_ty
   if (!cfg) PSGC_FAIL(PSGC_E_INVALID, "null decode config");
   if (sample_type < PSGC_SAMPLE_U8 || sample_type > PSGC_SAMPLE_F64)
     PSGC_FAIL(PSGC_E_INVALID, "bad sample type %d", sample_type);
-  if (cfg->enable_hdr)
-    PSGC_FAIL(PSGC_E_UNSUPPORTED, "enable_hdr is decoded on the CPU only");
   const bool gray = cfg->unwrap_mode == PSGC_UNWRAP_GRAYCODE;
   if (!gray && cfg->unwrap_mode != PSGC_UNWRAP_HETERODYNE3)
     PSGC_FAIL(PSGC_E_INVALID, "bad unwrap_mode %d", cfg->unwrap_mode);
@@ -262,7 +260,111 @@ int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_ty
   a->mod_thresh = cfg->modulation_threshold;
   a->avg_three = cfg->avg_three_phase ? 1 : 0;
   blur3_weights(cfg->preblur_sigma, &a->blur_c, &a->blur_s);
+  if (cfg->enable_hdr) {
+    // JSLAB_HDRPhaseFit asserts 3 <= steps <= 12 (JSLAB_HDRPhaseFit.h:15);
+    // saturation is classified on the raw samples, so there is no pre-blur
+    // (JSLAB_PhaseShiftDecoder_V2.cpp:262-265).
+    for (int i = 0; i < a->nlev; ++i)
+      if (a->shift_num[i] > 12)
+        PSGC_FAIL(PSGC_E_INVALID, "HDR fit needs shift_num[%d]=%d <= 12", i, a->shift_num[i]);
+    a->hdr = 1;
+    a->hdr_sat = cfg->hdr_saturation_gray;
+    a->hdr_min_amp = cfg->hdr_min_amplitude;
+    a->blur_c = 1.0;
+    a->blur_s = 0.0;
+  }
   return PSGC_OK;
+}
+
+// ---- HDR pseudo-inverse tables (JSLAB_HDRPhaseFit.h:13-40) ---------------
+// For every subset (bit mask) of the N equally spaced steps with at least 3
+// members, the design rows (1, cos d_k, -sin d_k) give the Gram matrix G; the
+// least-squares weights of sample k are G^-1 row_k (the pseudo-inverse
+// column that the reference takes from cv::invert(DECOMP_SVD)). G^-1 is the
+// adjugate over the determinant. Subsets whose Gram determinant is not above
+// 1e-20 trace^3 are rank deficient and stay marked unusable (the reference's
+// singular-value test; with distinct angles every subset of 3 or more has
+// full rank). Layout per level: weights (2^N * N * 3 doubles), then flags
+// (2^N bytes), each block 256-byte aligned.
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+static void hdr_level_sizes(int n, size_t* wbytes, size_t* okbytes) {
+  *wbytes = align256(((size_t)1 << n) * n * 3 * sizeof(double));
+  *okbytes = align256((size_t)1 << n);
+}
+
+size_t hdr_table_bytes(const DecodeArgs& a) {
+  size_t b = 0;
+  for (int i = 0; i < a.nlev; ++i) {
+    size_t wb, ob;
+    hdr_level_sizes(a.shift_num[i], &wb, &ob);
+    b += wb + ob;
+  }
+  return b;
+}
+
+void hdr_table_fill(const DecodeArgs& a, char* host) {
+  for (int i = 0; i < a.nlev; ++i) {
+    const int n = a.shift_num[i];
+    size_t wb, ob;
+    hdr_level_sizes(n, &wb, &ob);
+    double* W = (double*)host;
+    uint8_t* ok = (uint8_t*)(host + wb);
+    std::memset(host, 0, wb + ob);
+    double cs[16], sn[16];
+    for (int k = 0; k < n; ++k) {
+      const double d = 2.0 * PI * k / n;
+      cs[k] = std::cos(d);
+      sn[k] = -std::sin(d);
+    }
+    for (unsigned mask = 0; mask < (1u << n); ++mask) {
+      if (__builtin_popcount(mask) < 3) continue;
+      double G[3][3] = {};
+      for (int k = 0; k < n; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const double row[3] = {1.0, cs[k], sn[k]};
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) G[r][c] += row[r] * row[c];
+      }
+      const double det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
+                         G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                         G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+      const double tr = G[0][0] + G[1][1] + G[2][2];
+      if (!(det > 1e-20 * tr * tr * tr)) continue;
+      const double inv[3][3] = {
+          {(G[1][1] * G[2][2] - G[1][2] * G[2][1]) / det,
+           (G[0][2] * G[2][1] - G[0][1] * G[2][2]) / det,
+           (G[0][1] * G[1][2] - G[0][2] * G[1][1]) / det},
+          {(G[1][2] * G[2][0] - G[1][0] * G[2][2]) / det,
+           (G[0][0] * G[2][2] - G[0][2] * G[2][0]) / det,
+           (G[0][2] * G[1][0] - G[0][0] * G[1][2]) / det},
+          {(G[1][0] * G[2][1] - G[1][1] * G[2][0]) / det,
+           (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det,
+           (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det}};
+      for (int k = 0; k < n; ++k) {
+        if (!(mask >> k & 1u)) continue;
+        const double row[3] = {1.0, cs[k], sn[k]};
+        for (int r = 0; r < 3; ++r) {
+          double v = 0.0;
+          for (int c = 0; c < 3; ++c) v += inv[r][c] * row[c];
+          W[((size_t)mask * n + k) * 3 + r] = v;
+        }
+      }
+      ok[mask] = 1;
+    }
+    host += wb + ob;
+  }
+}
+
+void hdr_table_bind(DecodeArgs* a, const char* dev) {
+  for (int i = 0; i < a->nlev; ++i) {
+    size_t wb, ob;
+    hdr_level_sizes(a->shift_num[i], &wb, &ob);
+    a->hdr_w[i] = (const double*)dev;
+    a->hdr_ok[i] = (const uint8_t*)(dev + wb);
+    dev += wb + ob;
+  }
 }
 
 }  // namespace psgc
@@ -466,10 +568,32 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
   } else {
     d_raw = d_phase;
   }
-  if (ctx->kernels == PSGC_KERNELS_PLAIN)
+  if (a.hdr) {
+    // the HDR-weighted post-smooth needs the validity map
+    if (!d_valid) {
+      void* v;
+      PSGC_TRY(ctx_buf(ctx, S_VALID, npix, &v));
+      d_valid = (uint8_t*)v;
+    }
+    // tables are rebuilt per call: wait for earlier work that may read them
+    const size_t tb = hdr_table_bytes(a);
+    void* dt;
+    PSGC_TRY(ctx_buf(ctx, S_HDR, tb, &dt));
+    std::vector<char> host(tb);
+    hdr_table_fill(a, host.data());
+    PSGC_HIP(hipStreamSynchronize(ctx->stream));
+    PSGC_HIP(hipMemcpy(dt, host.data(), tb, hipMemcpyHostToDevice));
+    hdr_table_bind(&a, (const char*)dt);
+  }
+  // the HDR fit needs a level's raw samples together: one thread per pixel
+  if (ctx->kernels == PSGC_KERNELS_PLAIN || a.hdr)
     PSGC_TRY(launch_decode_plain(a, d_raw, d_k, d_mod, d_valid, ctx->stream));
   else
     PSGC_TRY(tiled_decode(ctx, a, d_raw, d_k, d_mod, d_valid, ctx->stream));
+  auto post_smooth = [&](const double* in, double* out) -> int {
+    return a.hdr ? launch_post_smooth_valid(in, d_valid, out, rows, cols, ctx->stream)
+                 : launch_post_smooth(in, out, rows, cols, ctx->stream);
+  };
   if (gray) {
     // jump correction reads (raw, k) and writes (phase, k)
     void* kt;
@@ -480,10 +604,10 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
     if (smooth) {
       PSGC_HIP(hipMemcpyAsync(d_raw, d_phase, npix * sizeof(double), hipMemcpyDeviceToDevice,
                               ctx->stream));
-      PSGC_TRY(launch_post_smooth(d_raw, d_phase, rows, cols, ctx->stream));
+      PSGC_TRY(post_smooth(d_raw, d_phase));
     }
   } else if (smooth) {
-    PSGC_TRY(launch_post_smooth(d_raw, d_phase, rows, cols, ctx->stream));
+    PSGC_TRY(post_smooth(d_raw, d_phase));
   }
   PSGC_TRY(copy_back(ctx, mem, phase, d_phase, npix));
   PSGC_TRY(copy_back(ctx, mem, k, d_k, npix));

@@ -66,6 +66,29 @@ struct PairPost {
   float* disp_dst;
 };
 
+// A camera decoded by the one-thread-per-pixel kernel (the HDR fit).
+struct PlainDecode {
+  DecodeArgs a;
+  double* phase;
+  int32_t* k;
+  double* mod;
+  uint8_t* valid;
+};
+
+// Validity-weighted post-smooth of an HDR camera.
+struct HdrSmooth {
+  const double* in;
+  const uint8_t* valid;
+  double* out;
+  int rows, cols;
+};
+
+// Host data copied into the arena once the plan is built.
+struct Blob {
+  char* dev;
+  std::vector<char> host;
+};
+
 template <typename J>
 struct Table {
   std::vector<J> host;
@@ -90,6 +113,9 @@ struct Plan {
   std::vector<const int32_t*> gray_in_k;
   std::vector<int32_t*> gray_out_k;
   std::vector<const uint8_t*> gray_valid;
+  std::vector<PlainDecode> plain_dec;
+  std::vector<HdrSmooth> hdr_smooth;
+  std::vector<Blob> blobs;
   int max_cols = 0, max_knum = 0, max_ghalf = 0, max_sor_half = 0, max_sor_k = 0, max_mls_hw = 0;
   int min_sor_k = 1 << 30;
   std::map<std::string, int> grid;
@@ -117,6 +143,8 @@ int upload_tables(Plan* pl) {
                        hipMemcpyHostToDevice));
     return PSGC_OK;
   };
+  for (const Blob& b : pl->blobs)
+    PSGC_HIP(hipMemcpy(b.dev, b.host.data(), b.host.size(), hipMemcpyHostToDevice));
   PSGC_TRY(put(pl->dec));
   PSGC_TRY(put(pl->mat));
   PSGC_TRY(put(pl->med));
@@ -357,12 +385,16 @@ static int run_plan(Plan* pl, hipStream_t s) {
     auto it = pl->grid.find(st);
     return it != pl->grid.end() ? it->second : default_grid(pl->ctx, per_cu);
   };
+  for (const PlainDecode& d : pl->plain_dec)
+    PSGC_TRY(launch_decode_plain(d.a, d.phase, d.k, d.mod, d.valid, s));
   if (!pl->dec.host.empty())
     PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
                                  g("decode", 4), s));
   for (size_t c = 0; c < pl->gray_fix.size(); ++c)
     PSGC_TRY(launch_gray_fix(pl->gray_fix[c], pl->gray_in_ph[c], pl->gray_out_ph[c],
                              pl->gray_in_k[c], pl->gray_out_k[c], pl->gray_valid[c], s));
+  for (const HdrSmooth& h : pl->hdr_smooth)
+    PSGC_TRY(launch_post_smooth_valid(h.in, h.valid, h.out, h.rows, h.cols, s));
   if (!pl->mat.host.empty())
     PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
                                 g("match", 4), pl->max_cols, pl->max_knum, s));
@@ -483,11 +515,13 @@ static int build_plan(Plan* pl, Build&& build) {
 }
 
 // Decode stages of one camera into the plan; returns the phase/k the match
-// should read.
+// should read and whether the post-smooth is still to apply (fused into the
+// match).
 static int plan_decode(Plan* pl, const psgc_decode_config& dc, const void* const* frames,
                        int n_frames, int sample_type, int rows, int cols, int64_t stride,
                        double* phase_raw, int32_t* k, double* mod, uint8_t* valid,
-                       const double** ph_for_match, const int32_t** k_for_match) {
+                       const double** ph_for_match, const int32_t** k_for_match,
+                       int* smooth_in_match) {
   DecodeJob j{};
   PSGC_TRY(build_decode_args(&dc, n_frames, sample_type, rows, cols, stride, &j.a));
   for (int f = 0; f < n_frames; ++f) {
@@ -496,18 +530,31 @@ static int plan_decode(Plan* pl, const psgc_decode_config& dc, const void* const
   }
   const size_t np = (size_t)rows * cols;
   const bool gray = j.a.nlev == 1;
+  const bool hdr = j.a.hdr != 0;
   j.phase = phase_raw;
   j.k = k;
   j.mod = mod;
   j.valid = valid;
-  if (gray && !valid) j.valid = (uint8_t*)pl->arena.take(np);
-  j.tiles_x = tiles(cols, 256);
-  j.tiles_y = tiles(rows, 4);
-  j.tile_base = pl->dec.total;
-  pl->dec.total += (int64_t)j.tiles_x * j.tiles_y;
-  pl->dec.host.push_back(j);
+  if ((gray || hdr) && !valid) j.valid = (uint8_t*)pl->arena.take(np);
   *ph_for_match = phase_raw;
   *k_for_match = k;
+  *smooth_in_match = dc.post_smooth ? 1 : 0;
+  if (hdr) {
+    // the fit needs a level's raw samples together: one thread per pixel
+    Blob b;
+    b.host.resize(hdr_table_bytes(j.a));
+    hdr_table_fill(j.a, b.host.data());
+    b.dev = (char*)pl->arena.take(b.host.size());
+    hdr_table_bind(&j.a, b.dev);
+    pl->blobs.push_back(std::move(b));
+    pl->plain_dec.push_back(PlainDecode{j.a, j.phase, j.k, j.mod, j.valid});
+  } else {
+    j.tiles_x = tiles(cols, 256);
+    j.tiles_y = tiles(rows, 4);
+    j.tile_base = pl->dec.total;
+    pl->dec.total += (int64_t)j.tiles_x * j.tiles_y;
+    pl->dec.host.push_back(j);
+  }
   if (gray) {
     double* ph2 = (double*)pl->arena.take(np * 8);
     int32_t* k2 = (int32_t*)pl->arena.take(np * 4);
@@ -519,6 +566,12 @@ static int plan_decode(Plan* pl, const psgc_decode_config& dc, const void* const
     pl->gray_valid.push_back(j.valid);
     *ph_for_match = ph2;
     *k_for_match = k2;
+  }
+  if (hdr && dc.post_smooth) {
+    double* sm = (double*)pl->arena.take(np * 8);
+    pl->hdr_smooth.push_back(HdrSmooth{*ph_for_match, j.valid, sm, rows, cols});
+    *ph_for_match = sm;
+    *smooth_in_match = 0;
   }
   return PSGC_OK;
 }
@@ -561,11 +614,12 @@ int psgc_batch_create(psgc_ctx* ctx, const psgc_pair_desc* pairs, int n, int fla
       int32_t* lk = (int32_t*)pl->arena.take(np * 4);
       int32_t* rk = (int32_t*)pl->arena.take(np * 4);
       ReconInput in{};
+      int sl = 0, sr = 0;
       PSGC_TRY(plan_decode(pl, p.decode, p.left, p.n_frames, p.sample_type, p.rows, p.cols,
-                           stride, lph, lk, nullptr, nullptr, &in.lph, &in.lk));
+                           stride, lph, lk, nullptr, nullptr, &in.lph, &in.lk, &sl));
       PSGC_TRY(plan_decode(pl, p.decode, p.right, p.n_frames, p.sample_type, p.rows, p.cols,
-                           stride, rph, rk, nullptr, nullptr, &in.rph, &in.rk));
-      in.smooth = p.decode.post_smooth ? 1 : 0;
+                           stride, rph, rk, nullptr, nullptr, &in.rph, &in.rk, &sr));
+      in.smooth = sl;  // both cameras share the decode configuration
       PSGC_TRY(plan_reconstruct(pl, i, p.recon, p.rows, p.cols, p.pha_f, p.Q, p.R1, in, p.xyz,
                                 p.P, p.disp, nullptr, nullptr));
     }

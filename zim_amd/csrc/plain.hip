@@ -43,14 +43,25 @@ __global__ void decode_plain_kernel(DecodeArgs a, double* __restrict__ phase_raw
     const int n = a.shift_num[i];
     const int base = a.level_base[i];
     double ssum = 0, csum = 0, gsum = 0;
-    for (int k = 0; k < n; ++k) {
-      const double v = blurred_sample(a, base + k, r, x);
-      ssum -= a.sin_tab[base + k] * v;
-      csum += a.cos_tab[base + k] * v;
-      gsum += v;
+    double m = 0.0;
+    if (a.hdr) {
+      double beta[3];
+      ok = hdr_fit(a, i, r, x, beta);
+      if (ok) {
+        csum = beta[1] * n / 2.0;
+        ssum = beta[2] * n / 2.0;
+        gsum = beta[0] * n;
+        ok = level_finish(ssum, csum, gsum, n, a.mod_thresh, &wrap[i], &m);
+      }
+    } else {
+      for (int k = 0; k < n; ++k) {
+        const double v = blurred_sample(a, base + k, r, x);
+        ssum -= a.sin_tab[base + k] * v;
+        csum += a.cos_tab[base + k] * v;
+        gsum += v;
+      }
+      ok = level_finish(ssum, csum, gsum, n, a.mod_thresh, &wrap[i], &m);
     }
-    double m;
-    ok = level_finish(ssum, csum, gsum, n, a.mod_thresh, &wrap[i], &m);
     if (i == 0) mod0 = m;
   }
   if (modmap) modmap[p] = mod0;
@@ -61,8 +72,10 @@ __global__ void decode_plain_kernel(DecodeArgs a, double* __restrict__ phase_raw
     else heterodyne(a, wrap, &ph, &k);
   }
   if (!ok) {
+    // with HDR the order map starts at -1 (JSLAB_PhaseShiftDecoder_V2.cpp:
+    // 199-239), otherwise at 0
     ph = 0.0;
-    k = 0;
+    k = a.hdr ? -1 : 0;
   }
   phase_raw[p] = ph;
   kmap[p] = k;
@@ -102,6 +115,35 @@ __global__ void gray_fix_kernel(int rows, int cols, const double* __restrict__ p
 }
 
 // Post-smooth: 3x3 [1/4 1/2 1/4]^2, replicated border, separable order.
+// HDR post-smooth (JSLAB_PhaseShiftDecoder_V2.cpp:282-290): the [1/4,1/2,1/4]^2
+// blur of the phase over the blur of the validity weights (1 for valid, 0
+// otherwise, floored at 1e-12); invalid pixels become 0.
+__global__ void post_smooth_valid_kernel(const double* __restrict__ in,
+                                         const uint8_t* __restrict__ valid,
+                                         double* __restrict__ out, int rows, int cols) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y * blockDim.y + threadIdx.y;
+  if (r >= rows || x >= cols) return;
+  const size_t p = (size_t)r * cols + x;
+  if (!valid[p]) {
+    out[p] = 0.0;
+    return;
+  }
+  const int xl = x > 0 ? x - 1 : 0, xr = x < cols - 1 ? x + 1 : cols - 1;
+  const int ru = r > 0 ? r - 1 : 0, rd = r < rows - 1 ? r + 1 : rows - 1;
+  const size_t u = (size_t)ru * cols, m = (size_t)r * cols, d = (size_t)rd * cols;
+  auto wv = [&](size_t q) { return valid[q] ? 1.0 : 0.0; };
+  const double hu = 0.5 * in[u + x] + 0.25 * (in[u + xl] + in[u + xr]);
+  const double hm = 0.5 * in[m + x] + 0.25 * (in[m + xl] + in[m + xr]);
+  const double hd = 0.5 * in[d + x] + 0.25 * (in[d + xl] + in[d + xr]);
+  const double ph = 0.5 * hm + 0.25 * (hu + hd);
+  const double wu = 0.5 * wv(u + x) + 0.25 * (wv(u + xl) + wv(u + xr));
+  const double wm = 0.5 * wv(m + x) + 0.25 * (wv(m + xl) + wv(m + xr));
+  const double wd = 0.5 * wv(d + x) + 0.25 * (wv(d + xl) + wv(d + xr));
+  const double w = 0.5 * wm + 0.25 * (wu + wd);
+  out[p] = ph / (w > 1e-12 ? w : 1e-12);
+}
+
 __global__ void post_smooth_kernel(const double* __restrict__ in, double* __restrict__ out,
                                    int rows, int cols) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -559,6 +601,16 @@ int launch_gray_fix(const DecodeArgs& a, const double* phase_in, double* phase_o
                                                                    phase_out, k_in, k_out, valid);
   PSGC_LAUNCHED();
   PSGC_PROF_END("gray_fix", s);
+  return PSGC_OK;
+}
+
+int launch_post_smooth_valid(const double* in, const uint8_t* valid, double* out, int rows,
+                             int cols, hipStream_t s) {
+  PSGC_PROF_BEGIN("post_smooth", s);
+  post_smooth_valid_kernel<<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(in, valid, out, rows,
+                                                                       cols);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("post_smooth", s);
   return PSGC_OK;
 }
 

@@ -51,6 +51,35 @@ __device__ inline bool level_finish(double S, double C, double G, int n, double 
   return !(nm < thr);
 }
 
+// HDR least-squares fit of level i at (r, x) (JSLAB_HDRPhaseFit.h:43-53 as
+// used at JSLAB_PhaseShiftDecoder_V2.cpp:73-89): the raw samples with
+// 0 <= I < saturation select a subset, whose pseudo-inverse columns give
+// beta = (A, C, S) of I = A + C cos d - S sin d. Returns false when the
+// subset is rank deficient, beta is not finite, the amplitude |(C, S)| is not
+// above hdr_min_amp or A is not above 1e-3.
+__device__ inline bool hdr_fit(const DecodeArgs& a, int i, int r, int x, double beta[3]) {
+  const int n = a.shift_num[i];
+  const int base = a.level_base[i];
+  double I[12];
+  unsigned mask = 0;
+  for (int k = 0; k < n; ++k) {
+    I[k] = load_sample(a, base + k, r, x);
+    if (I[k] >= 0.0 && I[k] < a.hdr_sat) mask |= 1u << k;
+  }
+  if (!a.hdr_ok[i][mask]) return false;
+  const double* w = a.hdr_w[i] + (size_t)mask * n * 3;
+  beta[0] = beta[1] = beta[2] = 0.0;
+  for (int k = 0; k < n; ++k)
+    if (mask >> k & 1u) {
+      beta[0] += w[k * 3 + 0] * I[k];
+      beta[1] += w[k * 3 + 1] * I[k];
+      beta[2] += w[k * 3 + 2] * I[k];
+    }
+  if (!(isfinite(beta[0]) && isfinite(beta[1]) && isfinite(beta[2]))) return false;
+  if (hypot(beta[1], beta[2]) <= a.hdr_min_amp || beta[0] <= 1e-3) return false;
+  return true;
+}
+
 // Three-frequency heterodyne unwrapping and fringe order
 // (JSLAB_PhaseShiftDecoder_V2.cpp:124-179).
 __device__ inline void heterodyne(const DecodeArgs& a, const double wrap[3], double* ph,
@@ -325,6 +354,20 @@ __device__ inline void mls_plane(const MlsFit& f, const float pi[3], float out[3
   out[2] = pi[2] - dot * f.n[2];
 }
 
+// Evaluates the fitted polynomial at the query point and maps it back to
+// world coordinates (JSLAB_MLS_Filter.cpp:377-388).
+__device__ inline void mls_eval_poly(const MlsFit& f, const float co[6], int cur,
+                                     const float pi[3], float out[3]) {
+  const float ex0 = pi[0] - f.cx, ey0 = pi[1] - f.cy, ez0 = pi[2] - f.cz;
+  const float u0 = ex0 * f.t1[0] + ey0 * f.t1[1] + ez0 * f.t1[2];
+  const float v0 = ex0 * f.t2[0] + ey0 * f.t2[1] + ez0 * f.t2[2];
+  float ff = co[0] + co[1] * u0 + co[2] * v0;
+  if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
+  out[0] = f.cx + u0 * f.t1[0] + v0 * f.t2[0] + ff * f.n[0];
+  out[1] = f.cy + u0 * f.t1[1] + v0 * f.t2[1] + ff * f.n[1];
+  out[2] = f.cz + u0 * f.t1[2] + v0 * f.t2[2] + ff * f.n[2];
+}
+
 // Solve (with the order-2 -> 1 fallback of JSLAB_MLS_Filter.cpp:350-374) and
 // evaluate at the query point.
 __device__ inline void mls_solve_eval(const MlsFit& f, float AtWA[6][6], float AtWf[6], int cur,
@@ -347,14 +390,7 @@ __device__ inline void mls_solve_eval(const MlsFit& f, float AtWA[6][6], float A
     mls_plane(f, pi, out);
     return;
   }
-  const float ex0 = pi[0] - f.cx, ey0 = pi[1] - f.cy, ez0 = pi[2] - f.cz;
-  const float u0 = ex0 * f.t1[0] + ey0 * f.t1[1] + ez0 * f.t1[2];
-  const float v0 = ex0 * f.t2[0] + ey0 * f.t2[1] + ez0 * f.t2[2];
-  float ff = co[0] + co[1] * u0 + co[2] * v0;
-  if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
-  out[0] = f.cx + u0 * f.t1[0] + v0 * f.t2[0] + ff * f.n[0];
-  out[1] = f.cy + u0 * f.t1[1] + v0 * f.t2[1] + ff * f.n[1];
-  out[2] = f.cz + u0 * f.t1[2] + v0 * f.t2[2] + ff * f.n[2];
+  mls_eval_poly(f, co, cur, pi, out);
 }
 
 // Accumulates one neighbour pk (weight w) into the normal equations.
@@ -374,6 +410,95 @@ __device__ inline void mls_accum_poly(const MlsFit& f, const float* pk, float w,
     AtWf[a] += w * basis[a] * fv;
     for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
   }
+}
+
+// ---- fixed-size variants (register resident) ------------------------------
+// The same arithmetic as chol_solve / mls_accum_poly / mls_solve_eval with
+// the system size a compile-time constant, so every array index is static and
+// the normal equations stay in registers instead of private memory. Only the
+// upper triangle of A is accumulated; A[j][i] (j <= i) is the value the
+// reference reads after mirroring it into the lower triangle.
+
+// Cholesky solve of the leading N x N block of A (N <= NA).
+template <int N, int NA>
+__device__ inline bool chol_solve_fixed(const float (&A)[NA][NA], const float (&b)[NA],
+                                        float x[6]) {
+  float L[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      float s = A[j][i];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (s <= 1e-12f) return false;
+        L[i][j] = sqrtf(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  float y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    float s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s / L[i][i];
+  }
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    float s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < N; ++k) s -= L[k][i] * x[k];
+    x[i] = s / L[i][i];
+  }
+  return true;
+}
+
+// One neighbour into the NC-term normal equations. For NC = 6 the leading
+// 3 x 3 block receives exactly the terms of the NC = 3 system, so a point
+// whose order fell back to 1 solves that block.
+template <int NC>
+__device__ inline void mls_accum_fixed(const MlsFit& f, const float pk[3], float w,
+                                       float (&A)[NC][NC], float (&bv)[NC]) {
+  const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+  const float u = ex * f.t1[0] + ey * f.t1[1] + ez * f.t1[2];
+  const float v = ex * f.t2[0] + ey * f.t2[1] + ez * f.t2[2];
+  const float fv = ex * f.n[0] + ey * f.n[1] + ez * f.n[2];
+  const float basis[6] = {1.0f, u, v, u * u, u * v, v * v};
+#pragma unroll
+  for (int a = 0; a < NC; ++a) {
+    const float wa = w * basis[a];  // w * basis[a] * x evaluates as (w * basis[a]) * x
+    bv[a] += wa * fv;
+#pragma unroll
+    for (int b = a; b < NC; ++b) A[a][b] += wa * basis[b];
+  }
+}
+
+// mls_solve_eval for an NC-term accumulation (NC = 6 serves orders 1 and 2).
+template <int NC>
+__device__ inline void mls_solve_eval_fixed(const MlsFit& f, const float (&A)[NC][NC],
+                                            const float (&bv)[NC], int cur, const float pi[3],
+                                            float out[3]) {
+  float co[6] = {0, 0, 0, 0, 0, 0};
+  bool solved;
+  if (NC == 6 && cur == 2) {
+    solved = chol_solve_fixed<NC, NC>(A, bv, co);
+    if (!solved) {
+      // the reference's fallback starts from zeroed coefficients
+      co[0] = co[1] = co[2] = co[3] = co[4] = co[5] = 0.f;
+      solved = chol_solve_fixed<3, NC>(A, bv, co);
+      if (solved) cur = 1;
+    }
+  } else {
+    solved = chol_solve_fixed<3, NC>(A, bv, co);
+  }
+  if (!solved) {
+    mls_plane(f, pi, out);
+    return;
+  }
+  mls_eval_poly(f, co, cur, pi, out);
 }
 
 }  // namespace psgc

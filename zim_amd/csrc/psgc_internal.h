@@ -51,6 +51,7 @@ enum Slot : int {
   S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
   S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_MAPX, S_MAPY, S_SRC, S_DST, S_CNT,
+  S_HDR,
   S_NUM_SLOTS
 };
 
@@ -92,8 +93,20 @@ struct DecodeArgs {
   double blur_c, blur_s;   // pre-blur weights (centre, side)
   int gray_bits;
   double gray_min_contrast;
-  int hdr;                 // HDR fit is not on the device path
+  // HDR: per level, a least-squares fit of I = A + C cos d - S sin d over the
+  // unsaturated raw samples instead of the blurred sums
+  int hdr;
+  double hdr_sat, hdr_min_amp;
+  const double* hdr_w[3];  // [mask][k][3] pseudo-inverse columns (device)
+  const uint8_t* hdr_ok[3];  // [mask] 1 if the subset has full rank (device)
 };
+
+// HDR pseudo-inverse tables of every level of `a`, one host/device blob:
+// hdr_table_bytes sizes it, hdr_table_fill writes it on the host and
+// hdr_table_bind points a->hdr_w / a->hdr_ok into its device copy.
+size_t hdr_table_bytes(const DecodeArgs& a);
+void hdr_table_fill(const DecodeArgs& a, char* host);
+void hdr_table_bind(DecodeArgs* a, const char* dev);
 
 int launch_decode_plain(const DecodeArgs& a, double* phase_raw, int32_t* k,
                         double* mod, uint8_t* valid, hipStream_t s);
@@ -102,6 +115,8 @@ int launch_gray_fix(const DecodeArgs& a, const double* phase_in, double* phase_o
                     hipStream_t s);
 int launch_post_smooth(const double* in, double* out, int rows, int cols,
                        hipStream_t s);
+int launch_post_smooth_valid(const double* in, const uint8_t* valid, double* out, int rows,
+                             int cols, hipStream_t s);
 
 int launch_match_plain(const double* lp, const double* rp, const int32_t* lk,
                        const int32_t* rk, int rows, int cols, int pha_f,

@@ -489,6 +489,34 @@ constexpr int MH_MAX = 8;
 constexpr int ML = CT + 2 * MH_MAX;
 constexpr int MW = 2 * MH_MAX + 1;
 
+// Pass 3 of the MLS fit over the in-radius entries (bit masks per window
+// row): the NC-term normal equations in registers, then the solve.
+template <int NC>
+__device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int Wd, int ry0,
+                                     int rx0, const float (*X)[ML + 1], const float (*Y)[ML + 1],
+                                     const float (*Z)[ML + 1], const float pi[3], float inv2s2,
+                                     int cur, float out[3]) {
+  float A[NC][NC], bv[NC];
+#pragma unroll
+  for (int a = 0; a < NC; ++a) {
+    bv[a] = 0.f;
+#pragma unroll
+    for (int b = 0; b < NC; ++b) A[a][b] = 0.f;
+  }
+  for (int dy = 0; dy < Wd; ++dy) {
+    const int ry = ry0 + dy;
+    uint32_t b = bits[dy];
+    while (b) {
+      const int rx = rx0 + __builtin_ctz(b);
+      b &= b - 1;
+      const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+      const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
+      mls_accum_fixed<NC>(f, pk, w, A, bv);
+    }
+  }
+  mls_solve_eval_fixed<NC>(f, A, bv, cur, pi, out);
+}
+
 __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                         int njobs, int64_t total) {
   __shared__ float X[ML][ML + 1], Y[ML][ML + 1], Z[ML][ML + 1];
@@ -577,28 +605,12 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
           }
           mls_frame(cov, &f);
           const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
-          if (cur == 0) {
+          if (cur == 0)
             mls_plane(f, pi, out);
-          } else {
-            const int nc = cur == 1 ? 3 : 6;
-            float AtWA[6][6], AtWf[6];
-            for (int a = 0; a < 6; ++a) {
-              AtWf[a] = 0.f;
-              for (int b = 0; b < 6; ++b) AtWA[a][b] = 0.f;
-            }
-            for (int dy = 0; dy < Wd; ++dy) {
-              const int ry = cy - hw + dy;
-              uint32_t b = bits[dy];
-              while (b) {
-                const int rx = cx - hw + __builtin_ctz(b);
-                b &= b - 1;
-                const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-                const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
-                mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
-              }
-            }
-            mls_solve_eval(f, AtWA, AtWf, cur, pi, out);
-          }
+          else if (cj.mls_order >= 2)  // uniform per tile
+            mls_fit_fixed<6>(f, bits, Wd, cy - hw, cx - hw, X, Y, Z, pi, inv2s2, cur, out);
+          else
+            mls_fit_fixed<3>(f, bits, Wd, cy - hw, cx - hw, X, Y, Z, pi, inv2s2, cur, out);
         }
       }
       cj.mls_dst[p * 3 + 0] = out[0];
