@@ -37,8 +37,21 @@ def lib():
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
         L.orc_num_threads.restype = i32
+        L.orc_check_f32.argtypes = [i32, flt, flt, i32, vp, vp, vp, vp]
         _lib = L
     return _lib
+
+
+def check_f32(which: str, lo: float, hi: float, stride: int = 1):
+    """Compares psgc::{exp,acos,cos}_f32 with the C library over every
+    `stride`-th float in [lo, hi]; returns (max ulp vs the float library
+    routine, #differences from it, #differences from the double routine
+    rounded to float, #floats tested)."""
+    out = [C.c_int64() for _ in range(4)]
+    rc = lib().orc_check_f32({"exp": 0, "acos": 1, "cos": 2}[which], lo, hi, stride,
+                             *[C.byref(o) for o in out])
+    assert rc == 0
+    return tuple(o.value for o in out)
 
 
 def num_threads() -> int:

@@ -25,7 +25,11 @@
 //     the reciprocal of their sum, rounded to the kernel type; for ksize 3 and
 //     sigma <= 0 the fixed [1/4, 1/2, 1/4];
 //   - cv::medianBlur 5 on f32: exact 13th order statistic, BORDER_REPLICATE.
-// atan2 is psgc::atan2 (shared with the device, within 2 ulp of libm).
+// atan2 is psgc::atan2 (shared with the device, within 2 ulp of libm). The
+// float std::exp / std::acos / std::cos of the MLS filter are
+// psgc::exp_f32 / acos_f32 / cos_f32 (shared with the device; evaluated in
+// double and rounded once, checked exhaustively against the C library by
+// orc_check_f32).
 //
 // Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
 
@@ -718,10 +722,10 @@ void eigen3(const float C[3][3], float ev[3], float V[3][3]) {
                      B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
   float r = detB * 0.5f;
   r = std::max(-1.0f, std::min(1.0f, r));
-  const float phi = std::acos(r) / 3.0f;
+  const float phi = psgc::acos_f32(r) / 3.0f;
   const float PIf = 3.14159265358979323846f;
-  ev[0] = q + 2.0f * p * std::cos(phi + 2.0f * PIf / 3.0f);
-  ev[2] = q + 2.0f * p * std::cos(phi);
+  ev[0] = q + 2.0f * p * psgc::cos_f32(phi + 2.0f * PIf / 3.0f);
+  ev[2] = q + 2.0f * p * psgc::cos_f32(phi);
   ev[1] = trace - ev[0] - ev[2];
   for (int idx = 0; idx < 3; ++idx) {
     float A[3][3];
@@ -804,7 +808,7 @@ void mls_pass(const float* src, const uint8_t* P, float* dst, int rows, int cols
           const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
           const float d2 = dx * dx + dy * dy + dz * dz;
           if (d2 > r2cut) continue;
-          const float w = std::exp(-d2 * inv_2s2);
+          const float w = psgc::exp_f32(-d2 * inv_2s2);
           cx += w * pk[0];
           cy += w * pk[1];
           cz += w * pk[2];
@@ -825,7 +829,7 @@ void mls_pass(const float* src, const uint8_t* P, float* dst, int rows, int cols
           const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
           const float d2 = dx * dx + dy * dy + dz * dz;
           if (d2 > r2cut) continue;
-          const float w = std::exp(-d2 * inv_2s2);
+          const float w = psgc::exp_f32(-d2 * inv_2s2);
           const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
           cov[0][0] += w * ex * ex;
           cov[0][1] += w * ex * ey;
@@ -868,7 +872,7 @@ void mls_pass(const float* src, const uint8_t* P, float* dst, int rows, int cols
           const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
           const float d2 = dx * dx + dy * dy + dz * dz;
           if (d2 > r2cut) continue;
-          const float w = std::exp(-d2 * inv_2s2);
+          const float w = psgc::exp_f32(-d2 * inv_2s2);
           const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
           const float u = ex * t1x + ey * t1y + ez * t1z;
           const float v = ex * t2x + ey * t2y + ez * t2z;
@@ -989,6 +993,50 @@ int orc_reconstruct(const psgc_recon_config* cfg, const double* l_phase,
 
 int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
+  return PSGC_OK;
+}
+
+// Exhaustive comparison of psgc::{exp,acos,cos}_f32 (which = 0, 1, 2) with
+// the C library over every `stride`-th float in [lo, hi]: the largest ulp
+// distance from the float library routine, how many results differ from it,
+// and how many differ from the double routine rounded to float (a correctly
+// rounded reference in all but rare double-rounding cases).
+int orc_check_f32(int which, float lo, float hi, int stride, int64_t* max_ulp,
+                  int64_t* n_diff_libf, int64_t* n_diff_cr, int64_t* n_total) {
+  auto key = [](float f) -> int64_t {
+    int32_t i;
+    std::memcpy(&i, &f, 4);
+    return i < 0 ? -(int64_t)(i & 0x7fffffff) : (int64_t)i;
+  };
+  auto from_key = [](int64_t k) -> float {
+    int32_t i = k < 0 ? (int32_t)((-k) | 0x80000000LL) : (int32_t)k;
+    float f;
+    std::memcpy(&f, &i, 4);
+    return f;
+  };
+  const int64_t k0 = key(lo), k1 = key(hi);
+  int64_t mu = 0, nd = 0, nc = 0, nt = 0;
+#pragma omp parallel for reduction(max : mu) reduction(+ : nd, nc, nt)
+  for (int64_t k = k0; k <= k1; k += stride) {
+    const float x = from_key(k);
+    float ours, libf, cr;
+    if (which == 0) {
+      ours = psgc::exp_f32(x); libf = std::exp(x); cr = (float)std::exp((double)x);
+    } else if (which == 1) {
+      ours = psgc::acos_f32(x); libf = std::acos(x); cr = (float)std::acos((double)x);
+    } else {
+      ours = psgc::cos_f32(x); libf = std::cos(x); cr = (float)std::cos((double)x);
+    }
+    const int64_t d = std::llabs(key(ours) - key(libf));
+    if (d > mu) mu = d;
+    nd += ours != libf;
+    nc += ours != cr;
+    ++nt;
+  }
+  *max_ulp = mu;
+  *n_diff_libf = nd;
+  *n_diff_cr = nc;
+  *n_total = nt;
   return PSGC_OK;
 }
 

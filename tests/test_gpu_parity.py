@@ -326,8 +326,9 @@ def test_decode_hdr_saturated_u8(kctx):
 
 def test_batch_mixed_schedules_match_oracle(ctx):
     """One captured batch of uneven pairs with different schedules: 3-step
-    and 8-step heterodyne (uint16), a 6-bit Gray-code pair (uint8) and an
-    over-exposed 8-step pair decoded with the HDR fit (uint8). Each pair must
+    and 8-step heterodyne (uint16), the 640x480 6-bit Gray-code pair of the
+    CPU configuration (uint8, CPU semantics, SOR k=20) and an over-exposed
+    8-step pair decoded with the HDR fit (uint8). Each pair must
     equal the oracle run on it alone, on every replay."""
     from zim_amd import synth
     specs = []
@@ -336,10 +337,12 @@ def test_batch_mixed_schedules_match_oracle(ctx):
                             steps=(steps,) * 3)
         specs.append((p, [x.numpy() for x in p.left], [x.numpy() for x in p.right],
                       scenes.c2_decode_config(shift_num=(steps,) * 3),
-                      scenes.c2_recon_config(mls_window_size=4), 70))
-    p, lh, rh = scenes.c1_small(5)
-    specs.append((p, lh, rh, scenes.c1_gray_config(),
-                  scenes.c2_recon_config(use_mls=0, sor_k=20), 50))
+                      scenes.c2_recon_config(mls_window_size=4, max_depth=2000.0), 70))
+    p = synth.render_c1(5)   # the CPU configuration: 640x480, 6-bit Gray code, SOR k=20
+    specs.append((p, [x.numpy() for x in p.left], [x.numpy() for x in p.right],
+                  scenes.c1_gray_config(),
+                  zim_amd.ReconConfig.default(outlier_mode=_capi.OUTLIER_SOR, sor_k=20,
+                                              sor_std_mul=1.0, sor_window=4), 50))
     p, lh, rh = scenes.hdr_small()
     specs.append((p, lh, rh, scenes.hdr_config(), scenes.c2_recon_config(mls_window_size=4), 70))
     pairs, refs = [], []

@@ -497,3 +497,26 @@ def test_hdr_min_amplitude_gate():
         assert (valid == (255 if ok else 0)).all()
         if not ok:
             assert (k == -1).all()
+
+
+# ------------------------------------------- float exp / acos / cos (MLS) --
+
+@pytest.mark.parametrize("which,lo,hi,stride", [
+    ("exp", -20.0, 0.0, 5),        # every weight argument -d^2/(2 sigma^2) of the filter
+    ("exp", -104.0, 89.0, 97),     # the whole finite range
+    ("acos", -1.0, 1.0, 5),
+    ("cos", 0.0, 3.1415927, 5),    # the eigen solver's arguments lie in [0, pi]
+    ("cos", -2e4, 2e4, 211),
+])
+def test_shared_float_functions_against_libm(which, lo, hi, stride):
+    """psgc::exp_f32 / acos_f32 / cos_f32 (shared by the oracle and the
+    device so that the MLS filter is bit-identical on both) against the C
+    library over every `stride`-th float of the range: never more than 1 ulp
+    from the float routine, and equal to the double routine rounded to float
+    (the correctly rounded result) in all but a handful of cases. A mistake
+    in a coefficient or in the range reduction shows up as many-ulp errors."""
+    max_ulp, n_libf, n_cr, n_tot = oracle.check_f32(which, lo, hi, stride)
+    assert n_tot > 1_000_000
+    assert max_ulp <= 1
+    assert n_cr <= 2, (n_cr, n_tot)
+    assert n_libf < 0.01 * n_tot

@@ -434,7 +434,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
         const float* pk = src + q * 3;
         const float d2 = dist2(pi[0], pi[1], pi[2], pk);
         if (d2 > r2cut) continue;
-        const float w = expf(-d2 * inv2s2);
+        const float w = exp_f32(-d2 * inv2s2);
         f.cx += w * pk[0];
         f.cy += w * pk[1];
         f.cz += w * pk[2];
@@ -454,7 +454,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
           const float* pk = src + q * 3;
           const float d2 = dist2(pi[0], pi[1], pi[2], pk);
           if (d2 > r2cut) continue;
-          const float w = expf(-d2 * inv2s2);
+          const float w = exp_f32(-d2 * inv2s2);
           const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
           cov[0][0] += w * ex * ex;
           cov[0][1] += w * ex * ey;
@@ -481,7 +481,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
             const float* pk = src + q * 3;
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
             if (d2 > r2cut) continue;
-            const float w = expf(-d2 * inv2s2);
+            const float w = exp_f32(-d2 * inv2s2);
             mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
           }
         mls_solve_eval(f, AtWA, AtWf, cur, pi, out);

@@ -249,10 +249,10 @@ __device__ inline void eigen3(const float C[3][3], float ev[3], float V[3][3]) {
                      B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
   float rr = detB * 0.5f;
   rr = fmaxf(-1.0f, fminf(1.0f, rr));
-  const float phi = acosf(rr) / 3.0f;
+  const float phi = acos_f32(rr) / 3.0f;
   const float PIf = 3.14159265358979323846f;
-  ev[0] = q + 2.0f * p * cosf(phi + 2.0f * PIf / 3.0f);
-  ev[2] = q + 2.0f * p * cosf(phi);
+  ev[0] = q + 2.0f * p * cos_f32(phi + 2.0f * PIf / 3.0f);
+  ev[2] = q + 2.0f * p * cos_f32(phi);
   ev[1] = trace - ev[0] - ev[2];
   for (int idx = 0; idx < 3; ++idx) {
     float A[3][3];

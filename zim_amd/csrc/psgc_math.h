@@ -1,8 +1,10 @@
-// psgc_math.h — deterministic FP64 scalar routines shared by the HIP kernels
-// and the host code, so that host and device produce bit-identical results.
+// psgc_math.h — deterministic scalar routines shared by the HIP kernels and
+// the host code, so that host and device produce bit-identical results: the
+// FP64 atan2 of the decode and the float exp / acos / cos of the MLS filter.
 //
-// Every routine uses only IEEE-754 basic operations (+ - * /, sqrt, round,
-// comparisons), each of which is correctly rounded on x86-64 and on gfx950.
+// Every routine uses only IEEE-754 basic operations (+ - * /, sqrt, rounding
+// to an integer, scaling by a power of two, conversions, comparisons), each
+// of which is exact or correctly rounded on x86-64 and on gfx950.
 // Translation units that include this header must be compiled without
 // floating-point contraction (-ffp-contract=off) so that no mul+add pair is
 // fused differently on the two sides.
@@ -76,5 +78,89 @@ PSGC_HD inline double atan2(double y, double x) {
 
 // C round(): halves away from zero.
 PSGC_HD inline double round_half_away(double v) { return __builtin_round(v); }
+
+// ---- float exp / acos / cos ---------------------------------------------
+// Each is evaluated in double and rounded once to float, so the result is
+// the correctly rounded float unless the exact value lies within ~1e-13
+// (relative) of a rounding boundary; a C library's correctly rounded or
+// sub-ulp float routine agrees with it except in such cases.
+
+constexpr double kInvLn2 = 0x1.71547652b82fep+0;  // 1/ln 2
+constexpr double kLn2Hi = 0x1.62e42p-1;           // ln 2, 20 significant bits
+constexpr double kLn2Lo = 0x1.fdf473de6af28p-22;  // ln 2 - kLn2Hi
+constexpr double k2OverPi = 0x1.45f306dc9c883p-1; // 2/pi
+constexpr double kPio2_1 = 0x1.921fap+0;          // pi/2 = P1 + P2 + P3, P1 and P2
+constexpr double kPio2_2 = 0x1.54442p-20;         // with 20 significant bits so that
+constexpr double kPio2_3 = 0x1.a308d313198a3p-41; // n * P1, n * P2 are exact
+
+// e^x: x = k ln 2 + r with |r| <= ln2/2 (Cody and Waite's split of ln 2, k*kLn2Hi
+// exact), e^r by its Taylor series through r^11 (first omitted term below
+// 7e-15 e^r), scaled by 2^k.
+PSGC_HD inline float exp_f32(float xf) {
+  if (__builtin_isnan(xf)) return xf;
+  if (xf > 89.0f) return __builtin_huge_valf();
+  if (xf < -104.0f) return 0.0f;
+  const double x = xf;
+  const double k = __builtin_rint(x * kInvLn2);
+  const double r = (x - k * kLn2Hi) - k * kLn2Lo;
+  double p = 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return (float)__builtin_ldexp(p, (int)k);
+}
+
+// acos(x) = atan2(sqrt((1 - x)(1 + x)), x) in double (the product of two
+// float-derived factors is exact, the square root correctly rounded).
+PSGC_HD inline float acos_f32(float xf) {
+  if (__builtin_isnan(xf) || xf > 1.0f || xf < -1.0f) return __builtin_nanf("");
+  const double x = xf;
+  return (float)atan2(__builtin_sqrt((1.0 - x) * (1.0 + x)), x);
+}
+
+// cos(x) for |x| < 2^20: x = n pi/2 + r, |r| <= pi/4, with the three-part
+// pi/2 above; then +-cos r or +-sin r by their Taylor series through r^16 /
+// r^17 (first omitted terms below 5e-17).
+PSGC_HD inline float cos_f32(float xf) {
+  if (__builtin_isnan(xf) || __builtin_isinf(xf)) return __builtin_nanf("");
+  const double x = __builtin_fabs((double)xf);
+  const double n = __builtin_rint(x * k2OverPi);
+  const double r = ((x - n * kPio2_1) - n * kPio2_2) - n * kPio2_3;
+  const double r2 = r * r;
+  const int q = (int)__builtin_fmod(n, 4.0);
+  double v;
+  if (q == 0 || q == 2) {
+    double c = 1.0 / 20922789888000.0;  // 1/16!
+    c = c * r2 - 1.0 / 87178291200.0;
+    c = c * r2 + 1.0 / 479001600.0;
+    c = c * r2 - 1.0 / 3628800.0;
+    c = c * r2 + 1.0 / 40320.0;
+    c = c * r2 - 1.0 / 720.0;
+    c = c * r2 + 1.0 / 24.0;
+    c = c * r2 - 0.5;
+    v = c * r2 + 1.0;
+  } else {
+    double sn = 1.0 / 355687428096000.0;  // 1/17!
+    sn = sn * r2 - 1.0 / 1307674368000.0;
+    sn = sn * r2 + 1.0 / 6227020800.0;
+    sn = sn * r2 - 1.0 / 39916800.0;
+    sn = sn * r2 + 1.0 / 362880.0;
+    sn = sn * r2 - 1.0 / 5040.0;
+    sn = sn * r2 + 1.0 / 120.0;
+    sn = sn * r2 - 1.0 / 6.0;
+    v = r + (r * r2) * sn;
+  }
+  // cos(n pi/2 + r) = cos r, -sin r, -cos r, sin r for n mod 4 = 0..3
+  if (q == 1 || q == 2) v = -v;
+  return (float)v;
+}
 
 }  // namespace psgc

@@ -510,7 +510,7 @@ __device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int 
       const int rx = rx0 + __builtin_ctz(b);
       b &= b - 1;
       const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-      const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
+      const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
       mls_accum_fixed<NC>(f, pk, w, A, bv);
     }
   }
@@ -571,7 +571,7 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
             if (d2 > r2) continue;
             row_bits |= 1u << dx;
-            const float w = expf(-d2 * inv2s2);
+            const float w = exp_f32(-d2 * inv2s2);
             f.cx += w * pk[0];
             f.cy += w * pk[1];
             f.cz += w * pk[2];
@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
               const int rx = cx - hw + __builtin_ctz(b);
               b &= b - 1;
               const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
-              const float w = expf(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
+              const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
               const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
               cov[0][0] += w * ex * ex;
               cov[0][1] += w * ex * ey;
