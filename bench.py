@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpts/s end-to-end 5MP 3-freq, batched vs per-pair, at ≤1e-3 mm vs FP64"
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 1 << 30
 ROWS, COLS = 2048, 2448
 
 
@@ -41,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample-rows", type=int, default=ROWS)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the measurement, copy 1 GiB device-to-device once (a kernel "
+                         "of known traffic that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -179,6 +183,15 @@ def main():
                  "pairs": len(sub), "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr}
         ctx.set_kernels("tiled")
         log(f"plain per-pair: {plain}")
+
+    if args.copy_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {COPY_PROBE_BYTES} bytes read and written once")
+        del src, dst
 
     cpu = None
     parity = None

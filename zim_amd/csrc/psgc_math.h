@@ -2,9 +2,10 @@
 // the host code, so that host and device produce bit-identical results: the
 // FP64 atan2 of the decode and the float exp / acos / cos of the MLS filter.
 //
-// Every routine uses only IEEE-754 basic operations (+ - * /, sqrt, rounding
-// to an integer, scaling by a power of two, conversions, comparisons), each
-// of which is exact or correctly rounded on x86-64 and on gfx950.
+// Every routine uses only IEEE-754 basic operations (+ - * /, fused
+// multiply-add where written explicitly, sqrt, rounding to an integer,
+// scaling by a power of two, conversions, comparisons), each of which is
+// exact or correctly rounded on x86-64 (with FMA3) and on gfx950.
 // Translation units that include this header must be compiled without
 // floating-point contraction (-ffp-contract=off) so that no mul+add pair is
 // fused differently on the two sides.
@@ -95,26 +96,27 @@ constexpr double kPio2_3 = 0x1.a308d313198a3p-41; // n * P1, n * P2 are exact
 
 // e^x: x = k ln 2 + r with |r| <= ln2/2 (Cody and Waite's split of ln 2, k*kLn2Hi
 // exact), e^r by its Taylor series through r^11 (first omitted term below
-// 7e-15 e^r), scaled by 2^k.
+// 7e-15 e^r) in Horner form with fused multiply-adds (one rounding each, the
+// same on both sides), scaled by 2^k.
 PSGC_HD inline float exp_f32(float xf) {
   if (__builtin_isnan(xf)) return xf;
   if (xf > 89.0f) return __builtin_huge_valf();
   if (xf < -104.0f) return 0.0f;
   const double x = xf;
   const double k = __builtin_rint(x * kInvLn2);
-  const double r = (x - k * kLn2Hi) - k * kLn2Lo;
+  const double r = __builtin_fma(-k, kLn2Lo, __builtin_fma(-k, kLn2Hi, x));
   double p = 1.0 / 39916800.0;
-  p = p * r + 1.0 / 3628800.0;
-  p = p * r + 1.0 / 362880.0;
-  p = p * r + 1.0 / 40320.0;
-  p = p * r + 1.0 / 5040.0;
-  p = p * r + 1.0 / 720.0;
-  p = p * r + 1.0 / 120.0;
-  p = p * r + 1.0 / 24.0;
-  p = p * r + 1.0 / 6.0;
-  p = p * r + 0.5;
-  p = p * r + 1.0;
-  p = p * r + 1.0;
+  p = __builtin_fma(p, r, 1.0 / 3628800.0);
+  p = __builtin_fma(p, r, 1.0 / 362880.0);
+  p = __builtin_fma(p, r, 1.0 / 40320.0);
+  p = __builtin_fma(p, r, 1.0 / 5040.0);
+  p = __builtin_fma(p, r, 1.0 / 720.0);
+  p = __builtin_fma(p, r, 1.0 / 120.0);
+  p = __builtin_fma(p, r, 1.0 / 24.0);
+  p = __builtin_fma(p, r, 1.0 / 6.0);
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r, 1.0);
+  p = __builtin_fma(p, r, 1.0);
   return (float)__builtin_ldexp(p, (int)k);
 }
 
@@ -139,24 +141,24 @@ PSGC_HD inline float cos_f32(float xf) {
   double v;
   if (q == 0 || q == 2) {
     double c = 1.0 / 20922789888000.0;  // 1/16!
-    c = c * r2 - 1.0 / 87178291200.0;
-    c = c * r2 + 1.0 / 479001600.0;
-    c = c * r2 - 1.0 / 3628800.0;
-    c = c * r2 + 1.0 / 40320.0;
-    c = c * r2 - 1.0 / 720.0;
-    c = c * r2 + 1.0 / 24.0;
-    c = c * r2 - 0.5;
-    v = c * r2 + 1.0;
+    c = __builtin_fma(c, r2, -1.0 / 87178291200.0);
+    c = __builtin_fma(c, r2, 1.0 / 479001600.0);
+    c = __builtin_fma(c, r2, -1.0 / 3628800.0);
+    c = __builtin_fma(c, r2, 1.0 / 40320.0);
+    c = __builtin_fma(c, r2, -1.0 / 720.0);
+    c = __builtin_fma(c, r2, 1.0 / 24.0);
+    c = __builtin_fma(c, r2, -0.5);
+    v = __builtin_fma(c, r2, 1.0);
   } else {
     double sn = 1.0 / 355687428096000.0;  // 1/17!
-    sn = sn * r2 - 1.0 / 1307674368000.0;
-    sn = sn * r2 + 1.0 / 6227020800.0;
-    sn = sn * r2 - 1.0 / 39916800.0;
-    sn = sn * r2 + 1.0 / 362880.0;
-    sn = sn * r2 - 1.0 / 5040.0;
-    sn = sn * r2 + 1.0 / 120.0;
-    sn = sn * r2 - 1.0 / 6.0;
-    v = r + (r * r2) * sn;
+    sn = __builtin_fma(sn, r2, -1.0 / 1307674368000.0);
+    sn = __builtin_fma(sn, r2, 1.0 / 6227020800.0);
+    sn = __builtin_fma(sn, r2, -1.0 / 39916800.0);
+    sn = __builtin_fma(sn, r2, 1.0 / 362880.0);
+    sn = __builtin_fma(sn, r2, -1.0 / 5040.0);
+    sn = __builtin_fma(sn, r2, 1.0 / 120.0);
+    sn = __builtin_fma(sn, r2, -1.0 / 6.0);
+    v = __builtin_fma(r * r2, sn, r);
   }
   // cos(n pi/2 + r) = cos r, -sin r, -cos r, sin r for n mod 4 = 0..3
   if (q == 1 || q == 2) v = -v;

@@ -345,30 +345,25 @@ constexpr int SL = CT + 2 * SH_MAX;
 template <int LD>
 __device__ inline void load_cloud_tile(const float* xyz, const uint8_t* P, const double* md,
                                        const double* thr, int rows, int cols, int y0, int x0,
-                                       int h, float (*X)[LD + 1], float (*Y)[LD + 1],
-                                       float (*Z)[LD + 1], uint8_t (*M)[LD + 1]) {
+                                       int h, float4 (*T)[LD + 1]) {
+  // T[r][c] = (x, y, z, 1) for a valid point, (x, y, z, 0) otherwise, so one
+  // 16-byte LDS read returns a point and its mask
   const int L = CT + 2 * h;
   const double th = thr ? *thr : 0.0;
   for (int idx = threadIdx.x; idx < L * L; idx += 256) {
     const int rr = idx / L, cc = idx % L;
     const int gy = y0 - h + rr, gx = x0 - h + cc;
-    float a = 0.f, b = 0.f, c = 0.f;
-    uint8_t v = 0;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (gy >= 0 && gy < rows && gx >= 0 && gx < cols) {
       const size_t p = (size_t)gy * cols + gx;
-      v = P[p] != 0;
-      if (v && md) {
+      bool ok = P[p] != 0;
+      if (ok && md) {
         const double d = md[p];
-        v = d >= 0.0 && d <= th;
+        ok = d >= 0.0 && d <= th;
       }
-      a = xyz[p * 3 + 0];
-      b = xyz[p * 3 + 1];
-      c = xyz[p * 3 + 2];
+      v = make_float4(xyz[p * 3 + 0], xyz[p * 3 + 1], xyz[p * 3 + 2], ok ? 1.f : 0.f);
     }
-    X[rr][cc] = a;
-    Y[rr][cc] = b;
-    Z[rr][cc] = c;
-    M[rr][cc] = v;
+    T[rr][cc] = v;
   }
 }
 
@@ -417,8 +412,7 @@ __device__ inline void ring_offset(int rad, int e, int* dy, int* dx) {
 template <int KMAX, int KFIX>
 __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                             int njobs, int64_t total) {
-  __shared__ float X[SL][SL + 1], Y[SL][SL + 1], Z[SL][SL + 1];
-  __shared__ uint8_t M[SL][SL + 1];
+  __shared__ float4 T[SL][SL + 1];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
@@ -427,15 +421,15 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
     const int h = cj.sor_half;
-    load_cloud_tile<SL>(cj.sor_xyz, cj.sor_P, nullptr, nullptr, cj.rows, cj.cols, y0, x0, h, X, Y,
-                        Z, M);
+    load_cloud_tile<SL>(cj.sor_xyz, cj.sor_P, nullptr, nullptr, cj.rows, cj.cols, y0, x0, h, T);
     __syncthreads();
     const int x = x0 + lx, y = y0 + ly;
     if (x < cj.cols && y < cj.rows) {
       const int cy = ly + h, cx = lx + h;
       double out = -1.0;
-      if (M[cy][cx]) {
-        const float qx = X[cy][cx], qy = Y[cy][cx], qz = Z[cy][cx];
+      const float4 q = T[cy][cx];
+      if (q.w != 0.f) {
+        const float qx = q.x, qy = q.y, qz = q.z;
         if constexpr (KFIX > 0) {
           float best[KFIX];
 #pragma unroll
@@ -444,9 +438,9 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
             for (int e = 0; e < 8 * rad; ++e) {
               int dy, dx;
               ring_offset(rad, e, &dy, &dx);
-              const int ry = cy + dy, rx = cx + dx;
-              if (!M[ry][rx]) continue;
-              const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              const float4 e4 = T[cy + dy][cx + dx];
+              if (e4.w == 0.f) continue;
+              const float p3[3] = {e4.x, e4.y, e4.z};
               keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
             }
           if (best[KFIX - 1] < __builtin_huge_valf()) {
@@ -463,9 +457,9 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
             for (int e = 0; e < 8 * rad; ++e) {
               int dy, dx;
               ring_offset(rad, e, &dy, &dx);
-              const int ry = cy + dy, rx = cx + dx;
-              if (!M[ry][rx]) continue;
-              const float p3[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              const float4 e4 = T[cy + dy][cx + dx];
+              if (e4.w == 0.f) continue;
+              const float p3[3] = {e4.x, e4.y, e4.z};
               keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
             }
           if (cnt >= k) {
@@ -493,9 +487,8 @@ constexpr int MW = 2 * MH_MAX + 1;
 // row): the NC-term normal equations in registers, then the solve.
 template <int NC>
 __device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int Wd, int ry0,
-                                     int rx0, const float (*X)[ML + 1], const float (*Y)[ML + 1],
-                                     const float (*Z)[ML + 1], const float pi[3], float inv2s2,
-                                     int cur, float out[3]) {
+                                     int rx0, const float4 (*T)[ML + 1], const float pi[3],
+                                     float inv2s2, int cur, float out[3]) {
   float A[NC][NC], bv[NC];
 #pragma unroll
   for (int a = 0; a < NC; ++a) {
@@ -507,9 +500,9 @@ __device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int 
     const int ry = ry0 + dy;
     uint32_t b = bits[dy];
     while (b) {
-      const int rx = rx0 + __builtin_ctz(b);
+      const float4 e4 = T[ry][rx0 + __builtin_ctz(b)];
       b &= b - 1;
-      const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+      const float pk[3] = {e4.x, e4.y, e4.z};
       const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
       mls_accum_fixed<NC>(f, pk, w, A, bv);
     }
@@ -519,8 +512,7 @@ __device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int 
 
 __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                         int njobs, int64_t total) {
-  __shared__ float X[ML][ML + 1], Y[ML][ML + 1], Z[ML][ML + 1];
-  __shared__ uint8_t M[ML][ML + 1];
+  __shared__ float4 T[ML][ML + 1];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
@@ -530,15 +522,16 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
     const int hw = cj.mls_hw;
     load_cloud_tile<ML>(cj.mls_src, cj.mls_P, cj.mls_md, cj.mls_thr, cj.rows, cj.cols, y0, x0,
-                        hw, X, Y, Z, M);
+                        hw, T);
     __syncthreads();
     const int x = x0 + lx, y = y0 + ly;
     if (x < cj.cols && y < cj.rows) {
       const size_t p = (size_t)y * cj.cols + x;
       const int cy = ly + hw, cx = lx + hw;
-      const float pi[3] = {X[cy][cx], Y[cy][cx], Z[cy][cx]};
+      const float4 q = T[cy][cx];
+      const float pi[3] = {q.x, q.y, q.z};
       float out[3] = {pi[0], pi[1], pi[2]};
-      const bool mine = M[cy][cx] != 0;
+      const bool mine = q.w != 0.f;
       if (cj.mls_Pout) cj.mls_Pout[p] = mine ? 1 : 0;
       // points removed by SOR enter MLS as zeros (the SOR output), and MLS
       // copies its input for points outside the mask
@@ -565,9 +558,9 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
           const int ry = cy - hw + dy;
           uint32_t row_bits = 0u;
           for (int dx = 0; dx < Wd; ++dx) {
-            const int rx = cx - hw + dx;
-            if (!M[ry][rx]) continue;
-            const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+            const float4 e4 = T[ry][cx - hw + dx];
+            if (e4.w == 0.f) continue;
+            const float pk[3] = {e4.x, e4.y, e4.z};
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
             if (d2 > r2) continue;
             row_bits |= 1u << dx;
@@ -590,9 +583,9 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             const int ry = cy - hw + dy;
             uint32_t b = bits[dy];
             while (b) {
-              const int rx = cx - hw + __builtin_ctz(b);
+              const float4 e4 = T[ry][cx - hw + __builtin_ctz(b)];
               b &= b - 1;
-              const float pk[3] = {X[ry][rx], Y[ry][rx], Z[ry][rx]};
+              const float pk[3] = {e4.x, e4.y, e4.z};
               const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
               const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
               cov[0][0] += w * ex * ex;
@@ -608,9 +601,9 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
           if (cur == 0)
             mls_plane(f, pi, out);
           else if (cj.mls_order >= 2)  // uniform per tile
-            mls_fit_fixed<6>(f, bits, Wd, cy - hw, cx - hw, X, Y, Z, pi, inv2s2, cur, out);
+            mls_fit_fixed<6>(f, bits, Wd, cy - hw, cx - hw, T, pi, inv2s2, cur, out);
           else
-            mls_fit_fixed<3>(f, bits, Wd, cy - hw, cx - hw, X, Y, Z, pi, inv2s2, cur, out);
+            mls_fit_fixed<3>(f, bits, Wd, cy - hw, cx - hw, T, pi, inv2s2, cur, out);
         }
       }
       cj.mls_dst[p * 3 + 0] = out[0];
