@@ -238,6 +238,34 @@ PSGC_API int psgc_reproject(psgc_ctx* ctx, const float* disp, int rows, int cols
                    double max_depth, int use_rectified_coords, float* xyz,
                    uint8_t* P, int mem);
 
+/* Per-pixel polynomial triangulation: evaluateV2CalibPoint and the point
+ * loop of startCalculate_v2 (JSLAB_PhaseEpipolarReconstructor_V2.cpp:13-76,
+ * 493-545). coef_x/y/z: the _rx/_ry/_rz calibration maps
+ * (JSLAB_StereoCalibParams_V2.cpp:36-41), rows*cols*4 f32 each (CV_32FC4):
+ * markers -1..-4 in the fourth term of x or z select the form of
+ * (x, y, z)(d). disp f64; R1 rotates to camera coordinates when
+ * use_rectified_coords is 0; the depth gate is the Q path's. */
+PSGC_API int psgc_reproject_poly(psgc_ctx* ctx, const double* disp, int rows, int cols,
+                                 const float* coef_x, const float* coef_y, const float* coef_z,
+                                 const double R1[9], double min_depth, double max_depth,
+                                 int use_rectified_coords, float* xyz, uint8_t* P, int mem);
+
+/* startCalculate_v2 (JSLAB_PhaseEpipolarReconstructor_V2.cpp:482-593):
+ * psgc_reconstruct with the per-pixel polynomial triangulation in place of
+ * the Q reprojection (same configuration, outputs and staging rules). */
+PSGC_API int psgc_reconstruct_poly(psgc_ctx* ctx, const psgc_recon_config* cfg,
+                                   const double* l_phase, const double* r_phase,
+                                   const int32_t* l_k, const int32_t* r_k, int rows, int cols,
+                                   int pha_f, const float* coef_x, const float* coef_y,
+                                   const float* coef_z, const double R1[9], float* xyz,
+                                   uint8_t* P, float* disp, int32_t* match_col, int mem);
+
+/* Reads one coefficient map file (JSLAB_StereoCalibParams_V2::loadMatBinary,
+ * JSLAB_StereoCalibParams_V2.cpp:4-17): int32 type, rows, cols, then the
+ * row-major data; the type must be CV_32FC4 (29). With out == NULL only
+ * *rows and *cols are set; otherwise out receives rows*cols*4 f32 (host). */
+PSGC_API int psgc_load_coef_bin(const char* path, int* rows, int* cols, float* out);
+
 /* Point-cloud Gaussian (JSLAB_PhaseEpipolarReconstructor_V2.cpp:79-124):
  * window `win` (odd), sigma; with PSGC_SEM_GPU dst_P receives the updated
  * mask, with PSGC_SEM_CPU dst_P receives a copy of P. */
@@ -311,6 +339,8 @@ typedef struct psgc_pair_desc {
     float* xyz;                    /* rows*cols*3 f32                      */
     uint8_t* P;                    /* rows*cols u8                         */
     float* disp;                   /* optional f32 disparity (after median) */
+    const float* coef[3];          /* optional: x/y/z coefficient maps (see
+                                      psgc_reconstruct_poly); all NULL = Q  */
 } psgc_pair_desc;
 
 enum {

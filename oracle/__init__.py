@@ -36,6 +36,9 @@ def lib():
         L.orc_reconstruct.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
+        L.orc_reproject_poly.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
+        L.orc_reconstruct_poly.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
+                                           vp, vp, vp, vp]
         L.orc_num_threads.restype = i32
         L.orc_check_f32.argtypes = [i32, flt, flt, i32, vp, vp, vp, vp]
         _lib = L
@@ -162,6 +165,38 @@ def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1):
     Q, R1 = _c(Q, np.float64), _c(R1, np.float64)
     lib().orc_reconstruct(C.byref(cfg), _p(lp), _p(rp), _p(lk), _p(rk), rows, cols, pha_f,
                           _p(Q), _p(R1), _p(xyz), _p(P), _p(disp), _p(mc))
+    return xyz, P, disp, mc
+
+
+def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1):
+    """coef: (cx, cy, cz), each rows x cols x 4 float32 (the _rx/_ry/_rz maps)."""
+    disp = _c(disp, np.float64)
+    rows, cols = disp.shape
+    cx, cy, cz = [_c(c, np.float32) for c in coef]
+    for c in (cx, cy, cz):
+        assert c.shape == (rows, cols, 4)
+    xyz = np.zeros((rows, cols, 3), np.float32)
+    P = np.zeros((rows, cols), np.uint8)
+    R1 = _c(R1, np.float64)
+    lib().orc_reproject_poly(_p(disp), rows, cols, _p(cx), _p(cy), _p(cz), _p(R1), min_depth,
+                             max_depth, use_rect, _p(xyz), _p(P))
+    return xyz, P
+
+
+def reconstruct_poly(cfg: ReconConfig, lp, rp, lk, rk, pha_f, coef, R1):
+    lp, rp = _c(lp, np.float64), _c(rp, np.float64)
+    lk, rk = _c(lk, np.int32), _c(rk, np.int32)
+    rows, cols = lp.shape
+    cx, cy, cz = [_c(c, np.float32) for c in coef]
+    for c in (cx, cy, cz):
+        assert c.shape == (rows, cols, 4)
+    xyz = np.zeros((rows, cols, 3), np.float32)
+    P = np.zeros((rows, cols), np.uint8)
+    disp = np.zeros((rows, cols), np.float32)
+    mc = np.zeros((rows, cols), np.int32)
+    R1 = _c(R1, np.float64)
+    lib().orc_reconstruct_poly(C.byref(cfg), _p(lp), _p(rp), _p(lk), _p(rk), rows, cols, pha_f,
+                               _p(cx), _p(cy), _p(cz), _p(R1), _p(xyz), _p(P), _p(disp), _p(mc))
     return xyz, P, disp, mc
 
 

@@ -505,6 +505,104 @@ int orc_reproject(const double* disp, int rows, int cols, const double* Q,
   return PSGC_OK;
 }
 
+// startCalculate_v2's per-pixel triangulation (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+// 493-545) with evaluateV2CalibPoint (:13-76). cx/cy/cz: rows*cols*4 floats
+// (the _rx/_ry/_rz CV_32FC4 maps).
+int orc_reproject_poly(const double* disp, int rows, int cols, const float* cxm,
+                       const float* cym, const float* czm, const double* R1,
+                       double min_depth, double max_depth, int use_rect, float* xyz,
+                       uint8_t* P) {
+  const float kInvDepth = -1.0f, kFittedRay = -2.0f, kZLinearXY = -3.0f, kRational = -4.0f;
+  auto eval = [&](const float* cx, const float* cy, const float* cz, double d, double& x,
+                  double& y, double& z) -> bool {
+    if (std::abs(cz[0]) <= 1e-9 && std::abs(cz[1]) <= 1e-9 && std::abs(cz[2]) <= 1e-9 &&
+        std::abs(cz[3]) <= 1e-9)
+      return false;
+    if (std::abs(cx[3] - kInvDepth) < 1e-6f) {
+      double d2 = d * d;
+      double inv_z = cz[0] + cz[1] * d + cz[2] * d2 + cz[3] * d2 * d;
+      if (!std::isfinite(inv_z) || std::abs(inv_z) < 1e-12) return false;
+      z = 1.0 / inv_z;
+      x = cx[0] * z;
+      y = cy[0] * z;
+      return std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+    }
+    if (std::abs(cx[3] - kFittedRay) < 1e-6f) {
+      double d2 = d * d;
+      if (std::abs(cz[3] - kInvDepth) < 1e-6f) {
+        double inv_z = cz[0] + cz[1] * d + cz[2] * d2;
+        if (!std::isfinite(inv_z) || std::abs(inv_z) < 1e-12) return false;
+        z = 1.0 / inv_z;
+      } else {
+        z = cz[0] + cz[1] * d + cz[2] * d2 + cz[3] * d2 * d;
+      }
+      x = cx[0] * z;
+      y = cy[0] * z;
+      return std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+    }
+    if (std::abs(cx[3] - kZLinearXY) < 1e-6f) {
+      if (std::abs(cz[3] - kRational) < 1e-6f) {
+        double denom = cz[1] * d + 1.0;
+        if (!std::isfinite(denom) || std::abs(denom) < 1e-12) return false;
+        z = cz[0] / denom;
+      } else {
+        double d2 = d * d;
+        double d3 = d2 * d;
+        z = cz[0] + cz[1] * d + cz[2] * d2 + cz[3] * d3;
+      }
+      x = cx[0] * z + cx[1];
+      y = cy[0] * z + cy[1];
+      return std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+    }
+    if (std::abs(cz[3] - kInvDepth) < 1e-6f) {
+      double inv_z = cz[0] + cz[1] * d + cz[2] * d * d;
+      if (!std::isfinite(inv_z) || std::abs(inv_z) < 1e-12) return false;
+      z = 1.0 / inv_z;
+      x = cx[0] * z;
+      y = cy[0] * z;
+      return std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+    }
+    double d2 = d * d;
+    double d3 = d2 * d;
+    x = cx[0] + cx[1] * d + cx[2] * d2 + cx[3] * d3;
+    y = cy[0] + cy[1] * d + cy[2] * d2 + cy[3] * d3;
+    z = cz[0] + cz[1] * d + cz[2] * d2 + cz[3] * d3;
+    return std::isfinite(x) && std::isfinite(y) && std::isfinite(z);
+  };
+#pragma omp parallel for
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j) {
+      const size_t p = (size_t)i * cols + j;
+      xyz[p * 3 + 0] = xyz[p * 3 + 1] = xyz[p * 3 + 2] = 0.f;
+      P[p] = 0;
+      const double d = disp[p];
+      if (std::abs(d) < 1e-5) continue;
+      const float* cz = czm + p * 4;
+      if (!(std::abs(cz[0]) > 1e-9 || std::abs(cz[1]) > 1e-9 || std::abs(cz[2]) > 1e-9 ||
+            std::abs(cz[3]) > 1e-9))
+        continue;
+      double x = 0.0, y = 0.0, z = 0.0;
+      if (!eval(cxm + p * 4, cym + p * 4, cz, d, x, y, z)) continue;
+      const double rx = R1[0] * x + R1[3] * y + R1[6] * z;
+      const double ry = R1[1] * x + R1[4] * y + R1[7] * z;
+      const double rz = R1[2] * x + R1[5] * y + R1[8] * z;
+      if (use_rect) {
+        if (z > min_depth && z < max_depth) {
+          xyz[p * 3 + 0] = (float)x;
+          xyz[p * 3 + 1] = (float)y;
+          xyz[p * 3 + 2] = (float)z;
+          P[p] = 1;
+        }
+      } else if (rz > min_depth && rz < max_depth) {
+        xyz[p * 3 + 0] = (float)rx;
+        xyz[p * 3 + 1] = (float)ry;
+        xyz[p * 3 + 2] = (float)rz;
+        P[p] = 1;
+      }
+    }
+  return PSGC_OK;
+}
+
 // applyGaussianFilterToPointCloudV2 (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
 // 79-124); semantics GPU = the reference kernel's mask update and division
 // (JSLAB_PhaseEpipolarReconstructor_V2_GPU.cu:220-274).
@@ -940,11 +1038,38 @@ int orc_mls(const float* src, const uint8_t* P, float* dst, int rows, int cols,
 
 // JSLAB_PhaseEpipolarReconstructor_V2::startCalculate
 // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:397-480) on rectified inputs.
+static int reconstruct_impl(const psgc_recon_config* cfg, const double* l_phase,
+                            const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                            int rows, int cols, int pha_f, const double* Q,
+                            const float* const* coef, const double* R1, float* xyz,
+                            uint8_t* P, float* disp_out, int32_t* match_col);
+
 int orc_reconstruct(const psgc_recon_config* cfg, const double* l_phase,
                     const double* r_phase, const int32_t* l_k,
                     const int32_t* r_k, int rows, int cols, int pha_f,
                     const double* Q, const double* R1, float* xyz, uint8_t* P,
                     float* disp_out, int32_t* match_col) {
+  return reconstruct_impl(cfg, l_phase, r_phase, l_k, r_k, rows, cols, pha_f, Q, nullptr, R1,
+                          xyz, P, disp_out, match_col);
+}
+
+// startCalculate_v2 (JSLAB_PhaseEpipolarReconstructor_V2.cpp:482-593): the same
+// chain with the per-pixel polynomial triangulation in place of Q.
+int orc_reconstruct_poly(const psgc_recon_config* cfg, const double* l_phase,
+                         const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                         int rows, int cols, int pha_f, const float* cx, const float* cy,
+                         const float* cz, const double* R1, float* xyz, uint8_t* P,
+                         float* disp_out, int32_t* match_col) {
+  const float* coef[3] = {cx, cy, cz};
+  return reconstruct_impl(cfg, l_phase, r_phase, l_k, r_k, rows, cols, pha_f, nullptr, coef, R1,
+                          xyz, P, disp_out, match_col);
+}
+
+static int reconstruct_impl(const psgc_recon_config* cfg, const double* l_phase,
+                            const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                            int rows, int cols, int pha_f, const double* Q,
+                            const float* const* coef, const double* R1, float* xyz,
+                            uint8_t* P, float* disp_out, int32_t* match_col) {
   const size_t npix = (size_t)rows * cols;
   std::vector<double> disp(npix);
   orc_stereo_match(l_phase, r_phase, l_k, r_k, rows, cols, pha_f,
@@ -964,8 +1089,12 @@ int orc_reconstruct(const psgc_recon_config* cfg, const double* l_phase,
     for (size_t p = 0; p < npix; ++p) disp_out[p] = (float)disp[p];
   std::vector<float> xyz0(npix * 3), xyzg(npix * 3), xyzr(npix * 3);
   std::vector<uint8_t> P0(npix), Pg(npix), Pr(npix);
-  orc_reproject(disp.data(), rows, cols, Q, R1, cfg->min_depth, cfg->max_depth,
-                cfg->use_rectified_coords, xyz0.data(), P0.data());
+  if (coef)
+    orc_reproject_poly(disp.data(), rows, cols, coef[0], coef[1], coef[2], R1, cfg->min_depth,
+                       cfg->max_depth, cfg->use_rectified_coords, xyz0.data(), P0.data());
+  else
+    orc_reproject(disp.data(), rows, cols, Q, R1, cfg->min_depth, cfg->max_depth,
+                  cfg->use_rectified_coords, xyz0.data(), P0.data());
   orc_gaussian(xyz0.data(), P0.data(), xyzg.data(), Pg.data(), rows, cols,
                cfg->gauss_window, cfg->gauss_sigma, cfg->semantics);
   // CPU semantics: the outlier stage reads the pre-Gaussian mask (…V2.cpp:466).

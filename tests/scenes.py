@@ -111,3 +111,42 @@ def hdr_small(seed=6, rows=192, cols=256):
         return np.clip(v, 0, 255).astype(np.uint8)
 
     return p, [to_u8(x) for x in p.left], [to_u8(x) for x in p.right]
+
+
+def q_ideal(f, cx1, cy, cx2, B):
+    """Q of an ideal rectified pair (cv::stereoRectify's layout): w = (d -
+    (cx1 - cx2)) / B, X = (j - cx1)/w, Y = (i - cy)/w, Z = f/w."""
+    return np.array([[1, 0, 0, -cx1], [0, 1, 0, -cy], [0, 0, 0, f],
+                     [0, 0, 1.0 / B, -(cx1 - cx2) / B]], np.float64)
+
+
+def poly_coefs_equivalent_to_q(rows, cols, f, cx1, cy, cx2, B, form):
+    """Coefficient maps whose polynomial triangulation is the ideal Q
+    reprojection, in one of three exact forms:
+      'inv_cubic'  cx.w = -1: 1/z = (d - D)/(f B), x = (j - cx1)/f z, y = (i - cy)/f z
+      'rational'   cx.w = -3, cz.w = -4: z = (-f B / D) / (1 - d / D)
+      'inv_quad'   cz.w = -1: 1/z as above (quadratic term 0)
+    with D = cx1 - cx2 (non-zero for 'rational')."""
+    ii, jj = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    D = cx1 - cx2
+    cx = np.zeros((rows, cols, 4), np.float32)
+    cy_ = np.zeros((rows, cols, 4), np.float32)
+    cz = np.zeros((rows, cols, 4), np.float32)
+    cx[..., 0] = (jj - cx1) / f
+    cy_[..., 0] = (ii - cy) / f
+    if form == "inv_cubic":
+        cx[..., 3] = -1.0
+        cz[..., 0] = -D / (f * B)
+        cz[..., 1] = 1.0 / (f * B)
+    elif form == "rational":
+        cx[..., 3] = -3.0
+        cz[..., 0] = -f * B / D
+        cz[..., 1] = -1.0 / D
+        cz[..., 3] = -4.0
+    elif form == "inv_quad":
+        cz[..., 0] = -D / (f * B)
+        cz[..., 1] = 1.0 / (f * B)
+        cz[..., 3] = -1.0
+    else:
+        raise ValueError(form)
+    return cx, cy_, cz

@@ -145,3 +145,52 @@ def test_write_ply_binary_layout(tmp_path):
     assert lib.psgc_write_ply(path2.encode(), _capi.ptr(pts), None, 2) == 0
     body2 = open(path2, "rb").read().split(b"end_header\n", 1)[1]
     assert np.array_equal(np.frombuffer(body2, "<f4").reshape(2, 3), pts)
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors in zim_amd/_capi.py have the C structs' sizes and
+    field offsets (a drifted field would silently corrupt batch descriptors)."""
+    import ctypes as C
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    fields = {"psgc_decode_config": _capi.DecodeConfig, "psgc_recon_config": _capi.ReconConfig,
+              "psgc_pair_desc": _capi.PairDesc}
+    lines = ['#include "psgc.h"', "#include <stdio.h>", "#include <stddef.h>", "int main(void){"]
+    for cname, cls in fields.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "l.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "l"
+    r = subprocess.run(["gcc", "-std=c99", f"-I{inc}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, cls in fields.items():
+        assert got[(cname, "sizeof")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_load_coef_bin(tmp_path):
+    """The calibration coefficient file of JSLAB_StereoCalibParams_V2::
+    loadMatBinary: int32 type (CV_32FC4 = 29), rows, cols, row-major data."""
+    import zim_amd
+    g = np.random.default_rng(0)
+    a = g.normal(size=(5, 7, 4)).astype(np.float32)
+    path = tmp_path / "c_rz.bin"
+    path.write_bytes(np.array([29, 5, 7], "<i4").tobytes() + a.tobytes())
+    assert np.array_equal(zim_amd.load_coef_bin(path), a)
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(np.array([5, 5, 7], "<i4").tobytes() + a.tobytes())   # CV_32FC1
+    with pytest.raises(_capi.PsgcError):
+        zim_amd.load_coef_bin(bad)
+    short = tmp_path / "short.bin"
+    short.write_bytes(np.array([29, 5, 7], "<i4").tobytes() + a.tobytes()[:-4])
+    with pytest.raises(_capi.PsgcError):
+        zim_amd.load_coef_bin(short)

@@ -386,3 +386,102 @@ def test_full_size_pair_matches_oracle(ctx):
     assert np.array_equal(gP, rP), f"mask mismatch {(gP != rP).sum()}"
     assert np.abs(gx - rx).max() <= XYZ_TOL
     b.close()
+
+
+# ----------------------------------------------- polynomial triangulation --
+
+def _mixed_poly_maps(rows, cols, seed=2):
+    """Coefficient maps whose rows cycle through every form (the three
+    Q-equivalent ones and three polynomial ones) and the masking cases."""
+    f, cx1, cy, cx2, B = 1600.0, cols / 2 + 0.5, rows / 2 - 0.5, cols / 2 - 4.0, 120.0
+    forms = [scenes.poly_coefs_equivalent_to_q(rows, cols, f, cx1, cy, cx2, B, fm)
+             for fm in ("inv_cubic", "rational", "inv_quad")]
+    g = np.random.default_rng(seed)
+    cx = g.uniform(-1, 1, (rows, cols, 4)).astype(np.float32) * np.float32([1, 1e-2, 1e-4, 1e-6])
+    cy = g.uniform(-1, 1, (rows, cols, 4)).astype(np.float32) * np.float32([1, 1e-2, 1e-4, 1e-6])
+    cz = np.stack([np.full((rows, cols), 700.0), g.uniform(-2, 2, (rows, cols)),
+                   g.uniform(-1e-2, 1e-2, (rows, cols)), g.uniform(-1e-4, 1e-4, (rows, cols))],
+                  -1).astype(np.float32)
+    out = [cx.copy(), cy.copy(), cz.copy()]
+    for r in range(rows):
+        k = r % 7
+        if k < 3:
+            for a in range(3):
+                out[a][r] = forms[k][a][r]
+        elif k == 3:
+            out[0][r, :, 3] = -2.0
+        elif k == 4:
+            out[0][r, :, 3] = -3.0
+        elif k == 5:
+            out[2][r, ::3] = 0.0
+    return out
+
+
+def test_reproject_poly_stage(ctx):
+    rows, cols = 70, 90
+    g = np.random.default_rng(3)
+    d = g.uniform(20.0, 90.0, (rows, cols))
+    d[g.random((rows, cols)) < 0.1] = 0.0
+    coef = _mixed_poly_maps(rows, cols)
+    a = np.radians(2.0)
+    R1 = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    for use_rect in (1, 0):
+        rx, rP = oracle.reproject_poly(d, coef, R1, 0.0, 1500.0, use_rect)
+        gx, gP = zim_amd.reproject_poly(dev(d), [dev(c) for c in coef], R1, 0.0, 1500.0, use_rect,
+                                        ctx=ctx)
+        torch.cuda.synchronize()
+        assert rP.sum() > 0.5 * rP.size
+        assert np.array_equal(host(gP), rP)
+        assert np.array_equal(host(gx), rx)     # same double arithmetic: bit-identical
+
+
+@pytest.mark.parametrize("variant", ["gpu_ref", "c2"])
+def test_reconstruct_poly_chain(kctx, variant):
+    """startCalculate_v2 on device (both kernel sets) against the oracle."""
+    p, L, R = _decoded_pair(3)
+    cam = p.cam
+    rows, cols = L[0].shape
+    coef = scenes.poly_coefs_equivalent_to_q(rows, cols, cam.f, cam.cx, cam.cy, cam.cx,
+                                             cam.baseline, "inv_cubic")
+    rc = (zim_amd.ReconConfig.default_gpu() if variant == "gpu_ref"
+          else scenes.c2_recon_config(mls_window_size=4))
+    rx, rP, rd, rmc = oracle.reconstruct_poly(rc, L[0], R[0], L[1], R[1], 70, coef, p.R1)
+    gx, gP, gd, gmc = zim_amd.reconstruct_poly(
+        rc, *[dev(a) for a in (L[0], R[0], L[1], R[1])], 70, [dev(c) for c in coef], p.R1,
+        ctx=kctx)
+    torch.cuda.synchronize()
+    assert rP.sum() > 1000
+    assert np.array_equal(host(gmc), rmc)
+    assert np.array_equal(host(gP), rP)
+    assert np.abs(host(gx) - rx).max() <= XYZ_TOL
+    assert np.abs(host(gd) - rd).max() <= DISP_TOL
+
+
+def test_batch_with_poly_pair(ctx):
+    """A batch mixing a Q pair and a polynomial-triangulation pair."""
+    dcfg = scenes.c2_decode_config()
+    rcfg = scenes.c2_recon_config(mls_window_size=4)
+    pairs, refs = [], []
+    for k, seed in enumerate((51, 52)):
+        p, Ld, Rd, lh, rh = _pair_inputs(seed, 96, 128)
+        Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+        spec = dict(left=Ld, right=Rd, decode=dcfg, recon=rcfg, pha_f=70, Q=p.Q, R1=p.R1)
+        if k == 1:
+            cam = p.cam
+            coef = scenes.poly_coefs_equivalent_to_q(96, 128, cam.f, cam.cx, cam.cy, cam.cx,
+                                                     cam.baseline, "inv_quad")
+            spec["coef"] = [dev(c) for c in coef]
+            refs.append(oracle.reconstruct_poly(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, coef,
+                                                p.R1))
+        else:
+            refs.append(oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1))
+        pairs.append(spec)
+    b = zim_amd.Batch(pairs, ctx, graph=True)
+    for rep in range(2):
+        outs = b.run()
+        torch.cuda.synchronize()
+        for out, (rx, rP, _, _) in zip(outs, refs):
+            assert rP.sum() > 0
+            assert np.array_equal(host(out["P"]), rP)
+            assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
+    b.close()

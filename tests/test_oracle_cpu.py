@@ -503,7 +503,8 @@ def test_hdr_min_amplitude_gate():
 
 @pytest.mark.parametrize("which,lo,hi,stride", [
     ("exp", -20.0, 0.0, 5),        # every weight argument -d^2/(2 sigma^2) of the filter
-    ("exp", -104.0, 89.0, 97),     # the whole finite range
+    ("exp", -104.0, 89.0, 97),     # the range where the result is neither 0 nor inf
+    ("exp", -3.4e38, 3.4e38, 997),  # every finite float (the clamped tails)
     ("acos", -1.0, 1.0, 5),
     ("cos", 0.0, 3.1415927, 5),    # the eigen solver's arguments lie in [0, pi]
     ("cos", -2e4, 2e4, 211),
@@ -520,3 +521,97 @@ def test_shared_float_functions_against_libm(which, lo, hi, stride):
     assert max_ulp <= 1
     assert n_cr <= 2, (n_cr, n_tot)
     assert n_libf < 0.01 * n_tot
+
+
+# ---------------------------------------- polynomial triangulation (v2) --
+
+def _disparity_field(rows, cols, seed=0, lo=40.0, hi=90.0):
+    g = np.random.default_rng(seed)
+    d = g.uniform(lo, hi, (rows, cols))
+    d[g.random((rows, cols)) < 0.1] = 0.0          # unmatched pixels
+    return d
+
+
+@pytest.mark.parametrize("form", ["inv_cubic", "rational", "inv_quad"])
+@pytest.mark.parametrize("use_rect", [1, 0])
+def test_poly_triangulation_reproduces_q(form, use_rect):
+    """Closed form: each exact polynomial form of the ideal Q reprojection
+    gives the Q reprojection's points (within the rounding of the f32
+    coefficients, < 1e-6 relative) and the same mask, in rectified or in
+    camera (R1^T) coordinates."""
+    rows, cols, f, cx1, cy, cx2, B = 40, 56, 1600.0, 27.5, 19.5, 22.0, 120.0
+    d = _disparity_field(rows, cols)
+    Q = scenes.q_ideal(f, cx1, cy, cx2, B)
+    a = np.radians(3.0)
+    R1 = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    coef = scenes.poly_coefs_equivalent_to_q(rows, cols, f, cx1, cy, cx2, B, form)
+    xq, Pq = oracle.reproject(d, Q, R1, 0.0, 1e4, use_rect)
+    xp, Pp = oracle.reproject_poly(d, coef, R1, 0.0, 1e4, use_rect)
+    assert np.array_equal(Pq, Pp) and Pq.sum() > 0.8 * Pq.size
+    m = Pq != 0
+    rel = np.abs(xp[m] - xq[m]) / np.abs(xq[m]).max(axis=-1, keepdims=True)
+    assert rel.max() < 1e-6
+
+
+def test_poly_triangulation_polynomial_forms():
+    """The forms that are polynomials in d: chosen cubics for x, y, z (no
+    marker), a cubic depth on a fitted ray (cx.w = -2) and a cubic depth with
+    z-linear x/y (cx.w = -3), against direct FP64 evaluation; plus the
+    masking rules: all-zero cz, |d| < 1e-5, a singular inverse depth and the
+    depth gate."""
+    rows, cols = 6, 5
+    d = np.full((rows, cols), 50.0)
+    d[0, 0] = 5e-6                                   # below the disparity floor
+    g = np.random.default_rng(1)
+    cx = g.uniform(-1, 1, (rows, cols, 4)).astype(np.float32) * [1, 1e-2, 1e-4, 1e-6]
+    cy = g.uniform(-1, 1, (rows, cols, 4)).astype(np.float32) * [1, 1e-2, 1e-4, 1e-6]
+    cz = np.zeros((rows, cols, 4), np.float32)
+    cz[..., 0] = 700.0
+    cz[..., 1] = g.uniform(-2, 2, (rows, cols))
+    cz[..., 2] = g.uniform(-1e-2, 1e-2, (rows, cols))
+    cz[..., 3] = g.uniform(-1e-4, 1e-4, (rows, cols))
+    cx, cy = cx.astype(np.float32), cy.astype(np.float32)
+    cx[1, :, 3] = -2.0                               # fitted ray, depth cubic
+    cx[2, :, 3] = -3.0                               # z-linear x/y, depth cubic
+    cz[3, 0] = 0.0                                   # no calibration here
+    cz[3, 1] = [0.0, 0.0, 0.0, -1.0]                 # inverse depth 0 -> rejected
+    cx[3, 1, 3] = 0.0
+    cz[4, 2, 0] = 2000.0                             # beyond the depth gate
+    xyz, P = oracle.reproject_poly(d, (cx, cy, cz), np.eye(3), 0.0, 1000.0, 1)
+    dd = 50.0
+    for i in range(rows):
+        for j in range(cols):
+            X, Y, Z = (c.astype(np.float64) for c in (cx[i, j], cy[i, j], cz[i, j]))
+            if (i, j) in ((0, 0), (3, 0), (3, 1), (4, 2)):
+                assert P[i, j] == 0, (i, j)
+                continue
+            z = Z[0] + Z[1] * dd + Z[2] * dd * dd + Z[3] * (dd * dd * dd)
+            if i == 1:
+                x, y = X[0] * z, Y[0] * z
+            elif i == 2:
+                x, y = X[0] * z + X[1], Y[0] * z + Y[1]
+            else:
+                x = X[0] + X[1] * dd + X[2] * dd * dd + X[3] * (dd * dd * dd)
+                y = Y[0] + Y[1] * dd + Y[2] * dd * dd + Y[3] * (dd * dd * dd)
+            assert P[i, j] == 1, (i, j)
+            assert np.allclose(xyz[i, j], np.float32([x, y, z]), rtol=1e-6, atol=1e-6), (i, j)
+
+
+def test_poly_reconstruct_equals_q_reconstruct_chain():
+    """startCalculate_v2 with Q-equivalent coefficients runs the same filter
+    chain as startCalculate: same mask, points within 1e-3 mm."""
+    p, left, right = scenes.c2_small(3)
+    dcfg = scenes.c2_decode_config()
+    L, R = oracle.decode(dcfg, left), oracle.decode(dcfg, right)
+    cam = p.cam
+    rows, cols = L[0].shape
+    coef = scenes.poly_coefs_equivalent_to_q(rows, cols, cam.f, cam.cx, cam.cy, cam.cx,
+                                             cam.baseline, "inv_cubic")
+    rc = scenes.c2_recon_config(use_mls=0)
+    xq, Pq, dq, _ = oracle.reconstruct(rc, L[0], R[0], L[1], R[1], 70, p.Q, p.R1)
+    xp, Pp, dp, _ = oracle.reconstruct_poly(rc, L[0], R[0], L[1], R[1], 70, coef, p.R1)
+    assert np.array_equal(dq, dp)
+    assert Pq.sum() > 1000
+    assert (Pq != Pp).sum() <= 2                     # a float-rounding flip at a gate at most
+    m = (Pq != 0) & (Pp != 0)
+    assert np.abs(xq[m] - xp[m]).max() < 1e-3

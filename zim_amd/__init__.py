@@ -150,13 +150,89 @@ def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1, ctx: Context | N
     return xyz, P, disp, mc
 
 
+def _coef_maps(coef, dev, rows, cols):
+    out = []
+    for c in coef:
+        if dev:
+            assert c.is_cuda and c.dtype == torch.float32 and c.is_contiguous()
+            assert tuple(c.shape) == (rows, cols, 4)
+            out.append(c)
+        else:
+            a = np.ascontiguousarray(c, np.float32)
+            assert a.shape == (rows, cols, 4)
+            out.append(a)
+    return out
+
+
+def reconstruct_poly(cfg: ReconConfig, lp, rp, lk, rk, pha_f, coef, R1,
+                     ctx: Context | None = None):
+    """startCalculate_v2: reconstruct with the per-pixel polynomial
+    triangulation. coef = (cx, cy, cz), each rows x cols x 4 float32 (the
+    _rx/_ry/_rz calibration maps). Returns (xyz, P, disp, match_col)."""
+    ctx = ctx or default_context()
+    dev = _is_dev(lp)
+    rows, cols = lp.shape
+    device = lp.device if dev else None
+    xyz = _out(dev, (rows, cols, 3), torch.float32, device)
+    P = _out(dev, (rows, cols), torch.uint8, device)
+    disp = _out(dev, (rows, cols), torch.float32, device)
+    mc = _out(dev, (rows, cols), torch.int32, device)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
+    Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
+    if not dev:
+        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
+        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
+    _capi.check(ctx.lib.psgc_reconstruct_poly(ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp),
+                                              _capi.ptr(lk), _capi.ptr(rk), rows, cols,
+                                              int(pha_f), _capi.ptr(cx), _capi.ptr(cy),
+                                              _capi.ptr(cz), _capi.ptr(Ra), _capi.ptr(xyz),
+                                              _capi.ptr(P), _capi.ptr(disp), _capi.ptr(mc),
+                                              MEM_DEVICE if dev else MEM_HOST))
+    return xyz, P, disp, mc
+
+
+def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1,
+                   ctx: Context | None = None):
+    """Per-pixel polynomial triangulation of an f64 disparity map."""
+    ctx = ctx or default_context()
+    dev = _is_dev(disp)
+    rows, cols = disp.shape
+    device = disp.device if dev else None
+    xyz = _out(dev, (rows, cols, 3), torch.float32, device)
+    P = _out(dev, (rows, cols), torch.uint8, device)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
+    if not dev:
+        disp = np.ascontiguousarray(disp, np.float64)
+    Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
+    _capi.check(ctx.lib.psgc_reproject_poly(ctx.h, _capi.ptr(disp), rows, cols, _capi.ptr(cx),
+                                            _capi.ptr(cy), _capi.ptr(cz), _capi.ptr(Ra),
+                                            float(min_depth), float(max_depth), int(use_rect),
+                                            _capi.ptr(xyz), _capi.ptr(P),
+                                            MEM_DEVICE if dev else MEM_HOST))
+    return xyz, P
+
+
+def load_coef_bin(path):
+    """One _rx/_ry/_rz coefficient file (int32 type, rows, cols + CV_32FC4
+    data, JSLAB_StereoCalibParams_V2.cpp:4-17) as a rows x cols x 4 float32
+    array."""
+    lib = _capi.load()
+    r, c = C.c_int(), C.c_int()
+    _capi.check(lib.psgc_load_coef_bin(str(path).encode(), C.byref(r), C.byref(c), None))
+    out = np.empty((r.value, c.value, 4), np.float32)
+    _capi.check(lib.psgc_load_coef_bin(str(path).encode(), C.byref(r), C.byref(c),
+                                       _capi.ptr(out)))
+    return out
+
+
 class Batch:
     """Batched decode + reconstruct of many stereo pairs (psgc_batch_*).
 
     pairs: list of dicts with keys left, right (lists of device frame
     tensors), decode (DecodeConfig), recon (ReconConfig), pha_f, Q, R1 and
-    optionally disp=True. Outputs (xyz, P[, disp]) are allocated here and
-    overwritten by every run()."""
+    optionally disp=True and coef=(cx, cy, cz) device coefficient maps (the
+    polynomial triangulation instead of Q). Outputs (xyz, P[, disp]) are
+    allocated here and overwritten by every run()."""
 
     def __init__(self, pairs, ctx: Context | None = None, graph=True, autotune=False):
         self.ctx = ctx or default_context()
@@ -190,6 +266,11 @@ class Batch:
             d.xyz = out["xyz"].data_ptr()
             d.P = out["P"].data_ptr()
             d.disp = out["disp"].data_ptr() if "disp" in out else None
+            if p.get("coef") is not None:
+                cm = _coef_maps(p["coef"], True, rows, cols)
+                for a in range(3):
+                    d.coef[a] = cm[a].data_ptr()
+                self._keep.append(cm)
             self.outputs.append(out)
             self._keep.append((L, R))
         flags = (_capi.BATCH_GRAPH if graph else 0) | (_capi.BATCH_AUTOTUNE if autotune else 0)

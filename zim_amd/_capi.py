@@ -158,6 +158,7 @@ class PairDesc(C.Structure):
         ("xyz", C.c_void_p),
         ("P", C.c_void_p),
         ("disp", C.c_void_p),
+        ("coef", C.c_void_p * 3),
     ]
 
 
@@ -206,6 +207,11 @@ def load(path: str | None = None):
         "psgc_stereo_match": ([vp, vp, vp, vp, vp, i32, i32, i32, dbl, vp, vp, i32], i32),
         "psgc_median5": ([vp, vp, vp, i32, i32, i32], i32),
         "psgc_reproject": ([vp, vp, i32, i32, vp, vp, dbl, dbl, i32, vp, vp, i32], i32),
+        "psgc_reproject_poly": ([vp, vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp, i32],
+                                i32),
+        "psgc_reconstruct_poly": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp,
+                                   vp, vp, vp, i32], i32),
+        "psgc_load_coef_bin": ([C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), vp], i32),
         "psgc_filter_gaussian": ([vp, vp, vp, vp, vp, i32, i32, i32, flt, i32, i32], i32),
         "psgc_filter_ror": ([vp, vp, vp, vp, vp, i32, i32, flt, i32, i32, i32], i32),
         "psgc_filter_sor": ([vp, vp, vp, vp, vp, i32, i32, i32, flt, i32, vp, vp, i32], i32),

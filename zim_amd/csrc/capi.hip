@@ -3,6 +3,7 @@
 // orchestration of the per-stage kernels.
 
 #include <cmath>
+#include <cstdio>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
@@ -825,15 +826,25 @@ int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t* P, float
   return finish(ctx, mem);
 }
 
-int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
+}  // extern "C"
+
+namespace psgc {
+namespace {
+
+// psgc_reconstruct / psgc_reconstruct_poly: Q reprojection when coef_h is
+// NULL, else the polynomial triangulation with the three maps coef_h[0..2].
+int reconstruct_impl(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
                      const double* r_phase, const int32_t* l_k, const int32_t* r_k, int rows,
-                     int cols, int pha_f, const double Q[16], const double R1[9], float* xyz,
-                     uint8_t* P, float* disp, int32_t* match_col, int mem) {
+                     int cols, int pha_f, const double* Q, const float* const* coef_h,
+                     const double* R1, float* xyz, uint8_t* P, float* disp, int32_t* match_col,
+                     int mem) {
   PSGC_CTX_ENTER(ctx);
   PSGC_TRY(check_mem(mem));
   PSGC_TRY(check_size(rows, cols));
-  if (!cfg || !l_phase || !r_phase || !l_k || !r_k || !Q || !R1 || !xyz || !P)
+  if (!cfg || !l_phase || !r_phase || !l_k || !r_k || (!Q && !coef_h) || !R1 || !xyz || !P)
     PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (coef_h && (!coef_h[0] || !coef_h[1] || !coef_h[2]))
+    PSGC_FAIL(PSGC_E_INVALID, "null coefficient map");
   if (pha_f < 0 || pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pha_f=%d", pha_f);
   const int gw = cfg->gauss_window;
   if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
@@ -848,6 +859,11 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   PSGC_TRY(stage_in(ctx, mem, S_RPH, r_phase, npix, &rp));
   PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
   PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  const float* coef[3] = {nullptr, nullptr, nullptr};
+  if (coef_h)
+    for (int a = 0; a < 3; ++a)
+      PSGC_TRY(stage_in(ctx, mem, S_COEF0 + a, coef_h[a], npix * 4, &coef[a]));
+  const float* const* coef_d = coef_h ? coef : nullptr;
   if (ctx->kernels == PSGC_KERNELS_TILED) {
     float *d_xyz, *d_disp;
     uint8_t* d_P;
@@ -856,8 +872,8 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
     PSGC_TRY(stage_out(ctx, mem, S_TMP1, P, npix, &d_P));
     PSGC_TRY(stage_out(ctx, mem, S_DISP2, disp, npix, &d_disp));
     PSGC_TRY(stage_out(ctx, mem, S_MATCH, match_col, npix, &d_mc));
-    PSGC_TRY(tiled_reconstruct(ctx, *cfg, lp, rp, lk, rk, rows, cols, pha_f, Q, R1, d_xyz, d_P,
-                               d_disp, d_mc, ctx->stream));
+    PSGC_TRY(tiled_reconstruct(ctx, *cfg, lp, rp, lk, rk, rows, cols, pha_f, Q, coef_d, R1,
+                               d_xyz, d_P, d_disp, d_mc, ctx->stream));
     PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, npix * 3));
     PSGC_TRY(copy_back(ctx, mem, P, d_P, npix));
     PSGC_TRY(copy_back(ctx, mem, disp, d_disp, npix));
@@ -885,15 +901,21 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
     PSGC_TRY(launch_median5((const float*)d32, (float*)d32m, rows, cols, s));
     dfinal = (const float*)d32m;
   }
-  double qr[25];
-  std::memcpy(qr, Q, 16 * sizeof(double));
-  std::memcpy(qr + 16, R1, 9 * sizeof(double));
   // CPU semantics without the median reprojects the f64 disparity
   // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:401-430).
   const bool use64 = !cfg->use_disp_median_filter && cfg->semantics == PSGC_SEM_CPU;
-  PSGC_TRY(launch_reproject(dfinal, use64 ? (const double*)d64 : nullptr, rows, cols, qr,
-                            cfg->min_depth, cfg->max_depth, cfg->use_rectified_coords,
-                            (float*)x0, (uint8_t*)p0, s));
+  if (coef_d) {
+    PSGC_TRY(launch_reproject_poly(dfinal, use64 ? (const double*)d64 : nullptr, rows, cols,
+                                   coef_d, R1, cfg->min_depth, cfg->max_depth,
+                                   cfg->use_rectified_coords, (float*)x0, (uint8_t*)p0, s));
+  } else {
+    double qr[25];
+    std::memcpy(qr, Q, 16 * sizeof(double));
+    std::memcpy(qr + 16, R1, 9 * sizeof(double));
+    PSGC_TRY(launch_reproject(dfinal, use64 ? (const double*)d64 : nullptr, rows, cols, qr,
+                              cfg->min_depth, cfg->max_depth, cfg->use_rectified_coords,
+                              (float*)x0, (uint8_t*)p0, s));
+  }
   float k2[49];
   gauss_kernel_f32(gw, cfg->gauss_sigma, k2);
   PSGC_TRY(launch_gaussian((const float*)x0, (const uint8_t*)p0, (float*)x1, (uint8_t*)p1, rows,
@@ -952,6 +974,85 @@ int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   PSGC_TRY(copy_back(ctx, mem, P, d_P, npix));
   PSGC_TRY(copy_back(ctx, mem, match_col, d_mc, npix));
   return finish(ctx, mem);
+}
+
+}  // namespace
+}  // namespace psgc
+
+extern "C" {
+
+int psgc_reconstruct(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
+                     const double* r_phase, const int32_t* l_k, const int32_t* r_k, int rows,
+                     int cols, int pha_f, const double Q[16], const double R1[9], float* xyz,
+                     uint8_t* P, float* disp, int32_t* match_col, int mem) {
+  if (!Q) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  return reconstruct_impl(ctx, cfg, l_phase, r_phase, l_k, r_k, rows, cols, pha_f, Q, nullptr,
+                          R1, xyz, P, disp, match_col, mem);
+}
+
+int psgc_reconstruct_poly(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
+                          const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                          int rows, int cols, int pha_f, const float* coef_x,
+                          const float* coef_y, const float* coef_z, const double R1[9],
+                          float* xyz, uint8_t* P, float* disp, int32_t* match_col, int mem) {
+  const float* coef[3] = {coef_x, coef_y, coef_z};
+  return reconstruct_impl(ctx, cfg, l_phase, r_phase, l_k, r_k, rows, cols, pha_f, nullptr,
+                          coef, R1, xyz, P, disp, match_col, mem);
+}
+
+int psgc_reproject_poly(psgc_ctx* ctx, const double* disp, int rows, int cols,
+                        const float* coef_x, const float* coef_y, const float* coef_z,
+                        const double R1[9], double min_depth, double max_depth,
+                        int use_rectified_coords, float* xyz, uint8_t* P, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!disp || !coef_x || !coef_y || !coef_z || !R1 || !xyz || !P)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const size_t npix = (size_t)rows * cols;
+  const double* dd;
+  const float* c[3];
+  float* dx;
+  uint8_t* dp;
+  PSGC_TRY(stage_in(ctx, mem, S_DISP64, disp, npix, &dd));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF0, coef_x, npix * 4, &c[0]));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF1, coef_y, npix * 4, &c[1]));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF2, coef_z, npix * 4, &c[2]));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ0, xyz, npix * 3, &dx));
+  PSGC_TRY(stage_out(ctx, mem, S_P0, P, npix, &dp));
+  PSGC_TRY(launch_reproject_poly(nullptr, dd, rows, cols, c, R1, min_depth, max_depth,
+                                 use_rectified_coords, dx, dp, ctx->stream));
+  PSGC_TRY(copy_back(ctx, mem, xyz, dx, npix * 3));
+  PSGC_TRY(copy_back(ctx, mem, P, dp, npix));
+  return finish(ctx, mem);
+}
+
+int psgc_load_coef_bin(const char* path, int* rows, int* cols, float* out) {
+  if (!path || !rows || !cols) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) PSGC_FAIL(PSGC_E_INVALID, "cannot open %s", path);
+  int32_t hdr[3];
+  const bool ok_hdr = std::fread(hdr, sizeof(int32_t), 3, fp) == 3;
+  if (!ok_hdr || hdr[1] < 1 || hdr[2] < 1 || (int64_t)hdr[1] * hdr[2] > ((int64_t)1 << 31)) {
+    std::fclose(fp);
+    PSGC_FAIL(PSGC_E_INVALID, "%s: bad header", path);
+  }
+  constexpr int kCV_32FC4 = 29;  // CV_MAKETYPE(CV_32F, 4)
+  if (hdr[0] != kCV_32FC4) {
+    std::fclose(fp);
+    PSGC_FAIL(PSGC_E_INVALID, "%s: type %d is not CV_32FC4", path, hdr[0]);
+  }
+  *rows = hdr[1];
+  *cols = hdr[2];
+  if (out) {
+    const size_t n = (size_t)hdr[1] * hdr[2] * 4;
+    const bool ok = std::fread(out, sizeof(float), n, fp) == n;
+    std::fclose(fp);
+    if (!ok) PSGC_FAIL(PSGC_E_INVALID, "%s: truncated data", path);
+  } else {
+    std::fclose(fp);
+  }
+  return PSGC_OK;
 }
 
 int psgc_rectify_maps(psgc_ctx* ctx, const double K_raw[9], const double R[9],

@@ -189,7 +189,8 @@ struct ReconInput {
 // Buffers come from the arena (sizes counted by a first dry pass when
 // `count` is set: then only arena.used advances).
 static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int rows, int cols,
-                            int pha_f, const double* Q, const double* R1, const ReconInput& in,
+                            int pha_f, const double* Q, const float* const* coef,
+                            const double* R1, const ReconInput& in,
                             float* xyz_out, uint8_t* P_out, float* disp_out, int32_t* mc_out,
                             double* d64_out) {
   Arena& A = pl->arena;
@@ -266,7 +267,11 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     pl->rg.total += ntile;
     c.disp_src = dsrc;
     c.disp_src64 = use64 ? d64 : nullptr;
-    std::memcpy(c.q, Q, sizeof(c.q));
+    if (coef) {
+      for (int a = 0; a < 3; ++a) c.coef[a] = (const float4*)coef[a];
+    } else {
+      std::memcpy(c.q, Q, sizeof(c.q));
+    }
     std::memcpy(c.r1, R1, sizeof(c.r1));
     c.min_depth = rc.min_depth;
     c.max_depth = rc.max_depth;
@@ -620,7 +625,8 @@ int psgc_batch_create(psgc_ctx* ctx, const psgc_pair_desc* pairs, int n, int fla
       PSGC_TRY(plan_decode(pl, p.decode, p.right, p.n_frames, p.sample_type, p.rows, p.cols,
                            stride, rph, rk, nullptr, nullptr, &in.rph, &in.rk, &sr));
       in.smooth = sl;  // both cameras share the decode configuration
-      PSGC_TRY(plan_reconstruct(pl, i, p.recon, p.rows, p.cols, p.pha_f, p.Q, p.R1, in, p.xyz,
+      PSGC_TRY(plan_reconstruct(pl, i, p.recon, p.rows, p.cols, p.pha_f, p.Q,
+                                p.coef[2] ? p.coef : nullptr, p.R1, in, p.xyz,
                                 p.P, p.disp, nullptr, nullptr));
     }
     return PSGC_OK;
@@ -723,13 +729,14 @@ int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t*
 
 int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
                       const double* rp, const int32_t* lk, const int32_t* rk, int rows, int cols,
-                      int pha_f, const double* Q, const double* R1, float* xyz, uint8_t* P,
-                      float* disp, int32_t* mc, hipStream_t s) {
+                      int pha_f, const double* Q, const float* const* coef, const double* R1,
+                      float* xyz, uint8_t* P, float* disp, int32_t* mc, hipStream_t s) {
   Plan* pl = new Plan();
   pl->ctx = ctx;
   ReconInput in{lp, rp, lk, rk, 0};
   auto build = [&](Plan* p) -> int {
-    return plan_reconstruct(p, 0, rc, rows, cols, pha_f, Q, R1, in, xyz, P, disp, mc, nullptr);
+    return plan_reconstruct(p, 0, rc, rows, cols, pha_f, Q, coef, R1, in, xyz, P, disp, mc,
+                            nullptr);
   };
   int r = build_plan(pl, build);
   if (r == PSGC_OK) r = run_plan(pl, s);

@@ -271,6 +271,27 @@ __global__ void reproject_plain_kernel(const float* __restrict__ disp32,
   P[p] = v;
 }
 
+__global__ void reproject_poly_plain_kernel(const float* __restrict__ disp32,
+                                            const double* __restrict__ disp64, int rows,
+                                            int cols, const float4* __restrict__ cx,
+                                            const float4* __restrict__ cy,
+                                            const float4* __restrict__ cz, QR m,
+                                            double min_depth, double max_depth, int use_rect,
+                                            float* __restrict__ xyz, uint8_t* __restrict__ P) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= rows || j >= cols) return;
+  const size_t p = (size_t)i * cols + j;
+  const double d = disp64 ? disp64[p] : (double)disp32[p];
+  float o[3];
+  const uint8_t v =
+      reproject_poly(cx[p], cy[p], cz[p], m.r1, d, min_depth, max_depth, use_rect, o);
+  xyz[p * 3 + 0] = o[0];
+  xyz[p * 3 + 1] = o[1];
+  xyz[p * 3 + 2] = o[2];
+  P[p] = v;
+}
+
 // ------------------------------------------------------------ Gaussian ----
 
 struct GaussK {
@@ -656,6 +677,22 @@ int launch_reproject(const float* disp32, const double* disp64, int rows, int co
   PSGC_PROF_BEGIN("reproject", s);
   reproject_plain_kernel<<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(
       disp32, disp64, rows, cols, m, min_depth, max_depth, use_rect, xyz, P);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("reproject", s);
+  return PSGC_OK;
+}
+
+int launch_reproject_poly(const float* disp32, const double* disp64, int rows, int cols,
+                          const float* const* coef, const double* R1, double min_depth,
+                          double max_depth, int use_rect, float* xyz, uint8_t* P,
+                          hipStream_t s) {
+  QR m;
+  for (int i = 0; i < 16; ++i) m.q[i] = 0.0;
+  for (int i = 0; i < 9; ++i) m.r1[i] = R1[i];
+  PSGC_PROF_BEGIN("reproject", s);
+  reproject_poly_plain_kernel<<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(
+      disp32, disp64, rows, cols, (const float4*)coef[0], (const float4*)coef[1],
+      (const float4*)coef[2], m, min_depth, max_depth, use_rect, xyz, P);
   PSGC_LAUNCHED();
   PSGC_PROF_END("reproject", s);
   return PSGC_OK;

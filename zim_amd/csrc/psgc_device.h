@@ -196,6 +196,87 @@ __device__ inline uint8_t reproject_point(const double* q, const double* r1, int
   return 0;
 }
 
+// Per-pixel polynomial triangulation (evaluateV2CalibPoint and the point
+// loop of startCalculate_v2, JSLAB_PhaseEpipolarReconstructor_V2.cpp:13-76,
+// 493-545). c = (cx, cy, cz): the pixel's three 4-term coefficient vectors;
+// markers in cx.w / cz.w select the form: cx.w = -1 inverse-depth cubic,
+// -2 fitted ray (cz.w = -1: inverse-depth quadratic, else depth cubic), -3
+// z-linear x/y (cz.w = -4: rational depth, else depth cubic); otherwise
+// cz.w = -1 inverse-depth quadratic, else cubics in d for x, y and z. The
+// expressions keep the reference's operation order (d^2 * d versus d3,
+// cz2 * d * d versus cz2 * d2) so the results are the CPU path's bits.
+__device__ inline bool poly_marker(float v, float m) { return fabsf(v - m) < 1e-6f; }
+
+__device__ inline uint8_t reproject_poly(const float4 cx, const float4 cy, const float4 cz,
+                                         const double* r1, double d, double min_depth,
+                                         double max_depth, int use_rect, float out[3]) {
+  out[0] = out[1] = out[2] = 0.f;
+  if (fabs(d) < 1e-5) return 0;
+  if (!(fabsf(cz.x) > 1e-9 || fabsf(cz.y) > 1e-9 || fabsf(cz.z) > 1e-9 || fabsf(cz.w) > 1e-9))
+    return 0;
+  double x, y, z;
+  const double d2 = d * d;
+  if (poly_marker(cx.w, -1.0f)) {
+    const double inv_z = cz.x + cz.y * d + cz.z * d2 + cz.w * d2 * d;
+    if (!isfinite(inv_z) || fabs(inv_z) < 1e-12) return 0;
+    z = 1.0 / inv_z;
+    x = cx.x * z;
+    y = cy.x * z;
+  } else if (poly_marker(cx.w, -2.0f)) {
+    if (poly_marker(cz.w, -1.0f)) {
+      const double inv_z = cz.x + cz.y * d + cz.z * d2;
+      if (!isfinite(inv_z) || fabs(inv_z) < 1e-12) return 0;
+      z = 1.0 / inv_z;
+    } else {
+      z = cz.x + cz.y * d + cz.z * d2 + cz.w * d2 * d;
+    }
+    x = cx.x * z;
+    y = cy.x * z;
+  } else if (poly_marker(cx.w, -3.0f)) {
+    if (poly_marker(cz.w, -4.0f)) {
+      const double denom = cz.y * d + 1.0;
+      if (!isfinite(denom) || fabs(denom) < 1e-12) return 0;
+      z = cz.x / denom;
+    } else {
+      const double d3 = d2 * d;
+      z = cz.x + cz.y * d + cz.z * d2 + cz.w * d3;
+    }
+    x = cx.x * z + cx.y;
+    y = cy.x * z + cy.y;
+  } else if (poly_marker(cz.w, -1.0f)) {
+    const double inv_z = cz.x + cz.y * d + cz.z * d * d;
+    if (!isfinite(inv_z) || fabs(inv_z) < 1e-12) return 0;
+    z = 1.0 / inv_z;
+    x = cx.x * z;
+    y = cy.x * z;
+  } else {
+    const double d3 = d2 * d;
+    x = cx.x + cx.y * d + cx.z * d2 + cx.w * d3;
+    y = cy.x + cy.y * d + cy.z * d2 + cy.w * d3;
+    z = cz.x + cz.y * d + cz.z * d2 + cz.w * d3;
+  }
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) return 0;
+  if (use_rect) {
+    if (z > min_depth && z < max_depth) {
+      out[0] = (float)x;
+      out[1] = (float)y;
+      out[2] = (float)z;
+      return 1;
+    }
+    return 0;
+  }
+  const double rx = r1[0] * x + r1[3] * y + r1[6] * z;
+  const double ry = r1[1] * x + r1[4] * y + r1[7] * z;
+  const double rz = r1[2] * x + r1[5] * y + r1[8] * z;
+  if (rz > min_depth && rz < max_depth) {
+    out[0] = (float)rx;
+    out[1] = (float)ry;
+    out[2] = (float)rz;
+    return 1;
+  }
+  return 0;
+}
+
 // Squared distance in f32, evaluated as dx*dx + dy*dy + dz*dz
 // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:171-174).
 __device__ inline float dist2(float px, float py, float pz, const float* q) {

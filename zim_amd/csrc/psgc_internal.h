@@ -51,7 +51,7 @@ enum Slot : int {
   S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
   S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_MAPX, S_MAPY, S_SRC, S_DST, S_CNT,
-  S_HDR,
+  S_HDR, S_COEF0, S_COEF1, S_COEF2,
   S_NUM_SLOTS
 };
 
@@ -128,6 +128,11 @@ int launch_reproject(const float* disp32, const double* disp64, int rows,
                      int cols, const double* QR, double min_depth,
                      double max_depth, int use_rect, float* xyz, uint8_t* P,
                      hipStream_t s);
+// coef: three device maps of rows*cols float4 (x, y, z coefficient vectors).
+int launch_reproject_poly(const float* disp32, const double* disp64, int rows, int cols,
+                          const float* const* coef, const double* R1, double min_depth,
+                          double max_depth, int use_rect, float* xyz, uint8_t* P,
+                          hipStream_t s);
 int launch_gaussian(const float* xyz, const uint8_t* P, float* dst,
                     uint8_t* dstP, int rows, int cols, int win,
                     const float* k2_host, int semantics, hipStream_t s);
@@ -148,10 +153,12 @@ int launch_sor_stats(const double* md, size_t npix, float std_mul, double* parti
 // Tiled single-image / single-pair paths (pipeline.hip).
 int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t* k,
                  double* mod, uint8_t* valid, hipStream_t s);
+// coef: NULL for the Q reprojection, else the three device coefficient maps.
 int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
                       const double* rp, const int32_t* lk, const int32_t* rk, int rows,
-                      int cols, int pha_f, const double* Q, const double* R1, float* xyz,
-                      uint8_t* P, float* disp, int32_t* mc, hipStream_t s);
+                      int cols, int pha_f, const double* Q, const float* const* coef,
+                      const double* R1, float* xyz, uint8_t* P, float* disp, int32_t* mc,
+                      hipStream_t s);
 int launch_mls(const float* src, const uint8_t* P, float* dst, int rows,
                int cols, int hw, float sigma, int order, float r2cut,
                int semantics, hipStream_t s);

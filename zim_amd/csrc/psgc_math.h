@@ -94,30 +94,44 @@ constexpr double kPio2_1 = 0x1.921fap+0;          // pi/2 = P1 + P2 + P3, P1 and
 constexpr double kPio2_2 = 0x1.54442p-20;         // with 20 significant bits so that
 constexpr double kPio2_3 = 0x1.a308d313198a3p-41; // n * P1, n * P2 are exact
 
+// Fused multiply-add a * b + c with one rounding. On the device it is the
+// three-operand v_fma_f64 (the compiler otherwise copies a loop-invariant
+// addend into the destination of the two-operand form before every use);
+// the result is the same IEEE fma on both sides.
+PSGC_HD inline double fma_rn(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+#else
+  return __builtin_fma(a, b, c);
+#endif
+}
+
 // e^x: x = k ln 2 + r with |r| <= ln2/2 (Cody and Waite's split of ln 2, k*kLn2Hi
 // exact), e^r by its Taylor series through r^11 (first omitted term below
 // 7e-15 e^r) in Horner form with fused multiply-adds (one rounding each, the
 // same on both sides), scaled by 2^k.
+// Arguments are clamped to [-104, 89] rather than branched on: e^-104 rounds to
+// float 0 and e^89 to float infinity, the values of the unclamped tails.
 PSGC_HD inline float exp_f32(float xf) {
-  if (__builtin_isnan(xf)) return xf;
-  if (xf > 89.0f) return __builtin_huge_valf();
-  if (xf < -104.0f) return 0.0f;
-  const double x = xf;
+  const double x = __builtin_fminf(__builtin_fmaxf(xf, -104.0f), 89.0f);
   const double k = __builtin_rint(x * kInvLn2);
   const double r = __builtin_fma(-k, kLn2Lo, __builtin_fma(-k, kLn2Hi, x));
   double p = 1.0 / 39916800.0;
-  p = __builtin_fma(p, r, 1.0 / 3628800.0);
-  p = __builtin_fma(p, r, 1.0 / 362880.0);
-  p = __builtin_fma(p, r, 1.0 / 40320.0);
-  p = __builtin_fma(p, r, 1.0 / 5040.0);
-  p = __builtin_fma(p, r, 1.0 / 720.0);
-  p = __builtin_fma(p, r, 1.0 / 120.0);
-  p = __builtin_fma(p, r, 1.0 / 24.0);
-  p = __builtin_fma(p, r, 1.0 / 6.0);
+  p = fma_rn(p, r, 1.0 / 3628800.0);
+  p = fma_rn(p, r, 1.0 / 362880.0);
+  p = fma_rn(p, r, 1.0 / 40320.0);
+  p = fma_rn(p, r, 1.0 / 5040.0);
+  p = fma_rn(p, r, 1.0 / 720.0);
+  p = fma_rn(p, r, 1.0 / 120.0);
+  p = fma_rn(p, r, 1.0 / 24.0);
+  p = fma_rn(p, r, 1.0 / 6.0);
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   p = __builtin_fma(p, r, 1.0);
-  return (float)__builtin_ldexp(p, (int)k);
+  const float res = (float)__builtin_ldexp(p, (int)k);
+  return __builtin_isnan(xf) ? xf : res;
 }
 
 // acos(x) = atan2(sqrt((1 - x)(1 + x)), x) in double (the product of two

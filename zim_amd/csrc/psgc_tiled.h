@@ -53,6 +53,7 @@ struct CloudJob {
   const float* disp_src;     // disparity to reproject (f32)
   const double* disp_src64;  // if set, reproject this f64 disparity instead
   double q[16], r1[9];
+  const float4* coef[3];     // if set: per-pixel polynomial triangulation instead of q
   double min_depth, max_depth;
   int use_rect, semantics, gwin;
   float gk[49];
@@ -76,10 +77,12 @@ struct CloudJob {
   float mls_inv2s2, mls_r2;
 };
 
-// Flat tile -> (job, local tile) by linear search (batches are small).
+// Flat tile -> job by forward linear search from `from`, the job of this
+// workgroup's previous tile: a workgroup visits its tiles in increasing
+// order, so over a whole launch it walks the job table once.
 template <typename J>
-__device__ inline int find_job(const J* jobs, int njobs, int64_t t) {
-  int j = 0;
+__device__ inline int find_job(const J* jobs, int njobs, int64_t t, int from = 0) {
+  int j = from;
   while (j + 1 < njobs && jobs[j + 1].tile_base <= t) ++j;
   return j;
 }

@@ -32,23 +32,49 @@ constexpr int DC = 256;  // tile width (threads)
 constexpr int DR = 4;    // output rows per thread
 constexpr int DLR = DR + 2, DLC = DC + 2;
 
-__device__ inline void decode_load(const DecodeArgs& a, int f, int y0, int x0,
-                                   double (*buf)[DLC]) {
+// The (DR + 2) x (DC + 2) sample tile of frame f (replicated border), in
+// two steps so that the global loads of the next frame are in flight while
+// the current one is computed: decode_fetch issues them into registers,
+// decode_store writes the registers to LDS. Thread t holds column t of every
+// tile row; threads t < 2 DLR also hold one of the 2 x DLR samples past
+// column DC (row t / 2, column DC + t % 2).
+struct FrameRegs {
+  double v[DLR];
+  double e;
+};
+
+constexpr int kDecodeExtra = (DLC - DC) * DLR;
+
+__device__ inline void decode_fetch(const DecodeArgs& a, int f, int y0, int x0, FrameRegs* r) {
+  const int t = threadIdx.x;
+  const int gx = min(max(x0 - 1 + t, 0), a.cols - 1);
+#pragma unroll
   for (int rr = 0; rr < DLR; ++rr) {
     const int gy = min(max(y0 - 1 + rr, 0), a.rows - 1);
-    for (int cc = threadIdx.x; cc < DLC; cc += DC) {
-      const int gx = min(max(x0 - 1 + cc, 0), a.cols - 1);
-      buf[rr][cc] = load_sample(a, f, gy, gx);
-    }
+    r->v[rr] = load_sample(a, f, gy, gx);
   }
+  if (t < kDecodeExtra) {
+    const int gy = min(max(y0 - 1 + t / 2, 0), a.rows - 1);
+    const int gxe = min(max(x0 - 1 + DC + t % 2, 0), a.cols - 1);
+    r->e = load_sample(a, f, gy, gxe);
+  }
+}
+
+__device__ inline void decode_store(const FrameRegs& r, double (*buf)[DLC]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int rr = 0; rr < DLR; ++rr) buf[rr][t] = r.v[rr];
+  if (t < kDecodeExtra) buf[t / 2][DC + t % 2] = r.e;
 }
 
 __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __restrict__ jobs,
                                                           int njobs, int64_t total) {
   __shared__ double buf[2][DLR][DLC];
   const XcdRange xr = xcd_range(total);
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const DecodeJob& job = jobs[j];
     const DecodeArgs& a = job.a;
     const int64_t lt = t - job.tile_base;
@@ -67,12 +93,14 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
     int nframes = 0;
     for (int i = 0; i < a.nlev; ++i) nframes += a.shift_num[i];
 
-    decode_load(a, 0, y0, x0, buf[0]);
+    FrameRegs pre;
+    decode_fetch(a, 0, y0, x0, &pre);
+    decode_store(pre, buf[0]);
     __syncthreads();
     int level = 0, kin = 0;
     for (int q = 0; q < DR; ++q) S[q] = Cs[q] = G[q] = 0.0;
     for (int f = 0; f < nframes; ++f) {
-      if (f + 1 < nframes) decode_load(a, f + 1, y0, x0, buf[(f + 1) & 1]);
+      if (f + 1 < nframes) decode_fetch(a, f + 1, y0, x0, &pre);
       const double(*b)[DLC] = buf[f & 1];
       const int tc = threadIdx.x + 1;
       double h[DLR];
@@ -99,6 +127,9 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
         kin = 0;
         ++level;
       }
+      // buf[(f + 1) & 1] was last read in iteration f - 1, before that
+      // iteration's barrier
+      if (f + 1 < nframes) decode_store(pre, buf[(f + 1) & 1]);
       __syncthreads();
     }
     if (x < a.cols) {
@@ -147,8 +178,10 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
                                                          int njobs, int64_t total) {
   extern __shared__ double lds[];
   const XcdRange xr = xcd_range(total);
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const MatchJob& m = jobs[j];
     const int r = (int)(t - m.tile_base);
     const int cols = m.cols;
@@ -223,8 +256,10 @@ __global__ void __launch_bounds__(256) median_tiled_kernel(const CloudJob* __res
   __shared__ float tile[CT + 4][CT + 5];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const CloudJob& cj = jobs[j];
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
@@ -257,8 +292,10 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
   __shared__ uint8_t pm[GL][GL + 1];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const CloudJob& cj = jobs[j];
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
@@ -272,8 +309,10 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
       if (gy >= 0 && gy < cj.rows && gx >= 0 && gx < cj.cols) {
         const size_t p = (size_t)gy * cj.cols + gx;
         const double dd = cj.disp_src64 ? cj.disp_src64[p] : (double)cj.disp_src[p];
-        v = reproject_point(cj.q, cj.r1, gy, gx, dd, cj.min_depth,
-                            cj.max_depth, cj.use_rect, o);
+        v = cj.coef[2] ? reproject_poly(cj.coef[0][p], cj.coef[1][p], cj.coef[2][p], cj.r1, dd,
+                                        cj.min_depth, cj.max_depth, cj.use_rect, o)
+                       : reproject_point(cj.q, cj.r1, gy, gx, dd, cj.min_depth,
+                                         cj.max_depth, cj.use_rect, o);
         // the tile's own pixels publish the reprojection
         if (rr >= h && rr < h + CT && cc >= h && cc < h + CT) {
           if (cj.xyz0) {
@@ -415,8 +454,10 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
   __shared__ float4 T[SL][SL + 1];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const CloudJob& cj = jobs[j];
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
@@ -515,8 +556,10 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
   __shared__ float4 T[ML][ML + 1];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
+  int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
-    const int j = find_job(jobs, njobs, t);
+    const int j = find_job(jobs, njobs, t, jprev);
+    jprev = j;
     const CloudJob& cj = jobs[j];
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
