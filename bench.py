@@ -211,6 +211,20 @@ def main():
                                                               0.99999)) if m.any() else 0.0}
             log(f"full-pair parity vs oracle: {parity}")
 
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE on this workload (profiles/c2_64pair_traffic.*)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "c2_64pair_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tj = json.load(fh)
+        if (tj.get("workload") == {"pairs": args.pairs, "rows": ROWS, "cols": COLS}
+                and dom_name in tj.get("kernels", {})):
+            traffic = int(tj["kernels"][dom_name]["traffic_bytes"])
+    # whole pipeline: frames of both cameras read once, xyz + mask written once
+    e2e_bytes = (24 * 2 + 13) * npix * args.pairs
+    e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
+
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -235,8 +249,10 @@ def main():
         "autotuned_grids": grids,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg,
+                     "end_to_end": {"alg_bytes_per_pt": 61, "achieved": round(e2e_gbs, 2),
+                                    "frac": round(e2e_gbs / HBM_PEAK_GBS, 5)}},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
     }

@@ -260,6 +260,36 @@ PSGC_API int psgc_reconstruct_poly(psgc_ctx* ctx, const psgc_recon_config* cfg,
                                    const float* coef_z, const double R1[9], float* xyz,
                                    uint8_t* P, float* disp, int32_t* match_col, int mem);
 
+/* ROI-restricted reconstruction: startCalculate_v2_roi(xyz, P, RectifiedROI)
+ * (JSLAB_PhaseEpipolarReconstructor_V2.cpp:693-914). left_rect and
+ * right_rect = {x, y, width, height} in rectified pixels with the same y and
+ * height (RectifiedROI, …V2.h:97-102); mask = width x height u8 of
+ * left_rect (0: outside the ROI). The full-size maps are read in place.
+ * The left crop is matched against the right crop under the mask (edge test
+ * in right-crop columns, disparity from global columns), then median,
+ * polynomial triangulation with the coefficients at global pixels,
+ * Gaussian and the outlier stage; with ROR, a result that keeps fewer than
+ * 10 % of the points is replaced by the Gaussian points and the pre-ROR
+ * mask (…V2.cpp:889-901); then MLS. xyz, P and disp (f32, after the
+ * median; may be NULL) have left_rect's size. */
+PSGC_API int psgc_reconstruct_poly_roi(psgc_ctx* ctx, const psgc_recon_config* cfg,
+                                       const double* l_phase, const double* r_phase,
+                                       const int32_t* l_k, const int32_t* r_k, int rows,
+                                       int cols, int pha_f, const int32_t left_rect[4],
+                                       const int32_t right_rect[4], const uint8_t* mask,
+                                       const float* coef_x, const float* coef_y,
+                                       const float* coef_z, const double R1[9], float* xyz,
+                                       uint8_t* P, float* disp, int mem);
+
+/* The right search window of prepareRectifiedROI (…V2.cpp:629-669): the
+ * fringe orders 1..pha_f present under the mask in left_rect, then the first
+ * and last column of the right order map (left_rect's rows) carrying one of
+ * them, padded by 32 px and clipped to the image; with none, the full rows.
+ * right_rect (host) receives {x, y, width, height}. Synchronous. */
+PSGC_API int psgc_roi_right_window(psgc_ctx* ctx, const int32_t* l_k, const int32_t* r_k,
+                                   int rows, int cols, int pha_f, const int32_t left_rect[4],
+                                   const uint8_t* mask, int32_t right_rect[4], int mem);
+
 /* Reads one coefficient map file (JSLAB_StereoCalibParams_V2::loadMatBinary,
  * JSLAB_StereoCalibParams_V2.cpp:4-17): int32 type, rows, cols, then the
  * row-major data; the type must be CV_32FC4 (29). With out == NULL only

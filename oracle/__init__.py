@@ -37,6 +37,9 @@ def lib():
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
         L.orc_reproject_poly.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
+        L.orc_roi_right_window.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
+        L.orc_reconstruct_poly_roi.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp,
+                                               vp, vp, vp, vp, vp, vp, vp]
         L.orc_reconstruct_poly.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
                                            vp, vp, vp, vp]
         L.orc_num_threads.restype = i32
@@ -198,6 +201,37 @@ def reconstruct_poly(cfg: ReconConfig, lp, rp, lk, rk, pha_f, coef, R1):
     lib().orc_reconstruct_poly(C.byref(cfg), _p(lp), _p(rp), _p(lk), _p(rk), rows, cols, pha_f,
                                _p(cx), _p(cy), _p(cz), _p(R1), _p(xyz), _p(P), _p(disp), _p(mc))
     return xyz, P, disp, mc
+
+
+def roi_right_window(lk, rk, pha_f, left_rect, mask):
+    lk, rk = _c(lk, np.int32), _c(rk, np.int32)
+    rows, cols = lk.shape
+    lr = np.asarray(left_rect, np.int32)
+    mask = _c(mask, np.uint8)
+    assert mask.shape == (lr[3], lr[2])
+    rr = np.zeros(4, np.int32)
+    lib().orc_roi_right_window(_p(lk), _p(rk), rows, cols, pha_f, _p(lr), _p(mask), _p(rr))
+    return rr
+
+
+def reconstruct_poly_roi(cfg: ReconConfig, lp, rp, lk, rk, pha_f, left_rect, right_rect, mask,
+                         coef, R1):
+    """startCalculate_v2_roi; outputs have the left rectangle's size."""
+    lp, rp = _c(lp, np.float64), _c(rp, np.float64)
+    lk, rk = _c(lk, np.int32), _c(rk, np.int32)
+    rows, cols = lp.shape
+    lr, rr = np.asarray(left_rect, np.int32), np.asarray(right_rect, np.int32)
+    mask = _c(mask, np.uint8)
+    assert mask.shape == (lr[3], lr[2])
+    cx, cy, cz = [_c(c, np.float32) for c in coef]
+    xyz = np.zeros((lr[3], lr[2], 3), np.float32)
+    P = np.zeros((lr[3], lr[2]), np.uint8)
+    disp = np.zeros((lr[3], lr[2]), np.float32)
+    R1 = _c(R1, np.float64)
+    lib().orc_reconstruct_poly_roi(C.byref(cfg), _p(lp), _p(rp), _p(lk), _p(rk), rows, cols,
+                                   pha_f, _p(lr), _p(rr), _p(mask), _p(cx), _p(cy), _p(cz),
+                                   _p(R1), _p(xyz), _p(P), _p(disp))
+    return xyz, P, disp
 
 
 def atan2(y, x):

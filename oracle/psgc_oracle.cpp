@@ -1120,6 +1120,163 @@ static int reconstruct_impl(const psgc_recon_config* cfg, const double* l_phase,
   return PSGC_OK;
 }
 
+// The right search window of prepareRectifiedROI
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:629-669): fringe orders 1..pha_f
+// present under the mask inside left_rect = {x, y, w, h}, then the first and
+// last right column carrying one of them in those rows, padded by 32 and
+// clipped; no such column: the full rows.
+int orc_roi_right_window(const int32_t* l_k, const int32_t* r_k, int rows, int cols,
+                         int pha_f, const int32_t* lr, const uint8_t* mask, int32_t* rr) {
+  std::vector<uint8_t> needed((size_t)std::max(pha_f + 1, 1), 0);
+  for (int y = 0; y < lr[3]; ++y)
+    for (int x = 0; x < lr[2]; ++x) {
+      if (mask[(size_t)y * lr[2] + x] == 0) continue;
+      const int k = l_k[(size_t)(lr[1] + y) * cols + lr[0] + x];
+      if (k > 0 && k <= pha_f) needed[k] = 1;
+    }
+  int mn = cols, mx = -1;
+  for (int y = lr[1]; y < lr[1] + lr[3]; ++y) {
+    if (y < 0 || y >= rows) continue;
+    for (int x = 0; x < cols; ++x) {
+      const int k = r_k[(size_t)y * cols + x];
+      if (k > 0 && k <= pha_f && needed[k]) {
+        mn = std::min(mn, x);
+        mx = std::max(mx, x);
+      }
+    }
+  }
+  const int pad = 32;
+  if (mx >= mn) {
+    const int x0 = std::max(0, mn - pad), x1 = std::min(cols - 1, mx + pad);
+    rr[0] = x0;
+    rr[2] = x1 - x0 + 1;
+  } else {
+    rr[0] = 0;
+    rr[2] = cols;
+  }
+  rr[1] = lr[1];
+  rr[3] = lr[3];
+  return PSGC_OK;
+}
+
+// startCalculate_v2_roi(xyz, P, RectifiedROI)
+// (JSLAB_PhaseEpipolarReconstructor_V2.cpp:693-914): the match of the left
+// crop against the right crop (same rows), only under the mask, with the
+// edge test in right-crop columns and the disparity from global columns;
+// median on the crop; polynomial triangulation with the coefficients at
+// global pixels; Gaussian; ROR, kept only if it leaves at least 10 % of the
+// points (else the pre-ROR mask and the Gaussian points) — or SOR / none as
+// configured; MLS. Outputs are crop-sized (lr[2] x lr[3]).
+int orc_reconstruct_poly_roi(const psgc_recon_config* cfg, const double* l_phase,
+                             const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                             int rows, int cols, int pha_f, const int32_t* lr,
+                             const int32_t* rr, const uint8_t* mask, const float* cxm,
+                             const float* cym, const float* czm, const double* R1,
+                             float* xyz, uint8_t* P, float* disp_out) {
+  (void)rows;
+  const int lx0 = lr[0], y0 = lr[1], lw = lr[2], h = lr[3];
+  const int rx0 = rr[0], rw = rr[2];
+  const int knum = pha_f;
+  const size_t n = (size_t)lw * h;
+  std::vector<double> disp(n, 0.0);
+#pragma omp parallel for
+  for (int r = 0; r < h; ++r) {
+    const double* lp = l_phase + (size_t)(y0 + r) * cols + lx0;
+    const double* rp = r_phase + (size_t)(y0 + r) * cols + rx0;
+    const int32_t* lk = l_k + (size_t)(y0 + r) * cols + lx0;
+    const int32_t* rk = r_k + (size_t)(y0 + r) * cols + rx0;
+    const uint8_t* mk = mask + (size_t)r * lw;
+    std::vector<std::vector<int>> table(knum + 1);
+    for (int j = 0; j < rw; ++j)
+      if (rk[j] >= 0 && rk[j] <= knum) table[rk[j]].push_back(j);
+    for (int j = 0; j < lw; ++j) {
+      if (mk[j] == 0) continue;
+      const int k = lk[j];
+      if (k > pha_f || k <= 0 || k > knum) continue;
+      if (table[k].empty()) continue;
+      double min_cost = 1e9;
+      int best = -1;
+      const double lv = lp[j];
+      for (int c : table[k]) {
+        const double diff = std::abs(rp[c] - lv);
+        if (diff < min_cost) {
+          min_cost = diff;
+          best = c;
+        }
+      }
+      if (min_cost >= cfg->grad_threshold) continue;
+      if (best < 2 || best >= rw - 1) continue;
+      const double c0 = rp[best - 1] - lv, c1 = rp[best] - lv, c2 = rp[best + 1] - lv;
+      const double num = c1 * (c0 - c2);
+      const double den = 2.0 * (c0 - c1) * (c1 - c2);
+      double delta = 0.0;
+      if (std::abs(den) > 1e-9) delta = num / den;
+      disp[(size_t)r * lw + j] =
+          static_cast<double>(lx0 + j) - static_cast<double>(rx0 + best) - delta;
+    }
+  }
+  std::vector<float> d32(n), d32m(n);
+  for (size_t p = 0; p < n; ++p) d32[p] = (float)disp[p];
+  const bool f32_disp = cfg->use_disp_median_filter || cfg->semantics == PSGC_SEM_GPU;
+  if (cfg->use_disp_median_filter) {
+    orc_median5(d32.data(), d32m.data(), h, lw);
+    d32.swap(d32m);
+  }
+  if (f32_disp)
+    for (size_t p = 0; p < n; ++p) disp[p] = (double)d32[p];
+  if (disp_out)
+    for (size_t p = 0; p < n; ++p) disp_out[p] = (float)disp[p];
+  // triangulation of the masked crop pixels with the coefficients at their
+  // global positions: gather the crop's coefficients, then the pixel loop
+  std::vector<float> cx(n * 4), cy(n * 4), cz(n * 4);
+  for (int r = 0; r < h; ++r)
+    for (int j = 0; j < lw; ++j) {
+      const size_t g = ((size_t)(y0 + r) * cols + lx0 + j) * 4, c = ((size_t)r * lw + j) * 4;
+      for (int a = 0; a < 4; ++a) {
+        cx[c + a] = cxm[g + a];
+        cy[c + a] = cym[g + a];
+        cz[c + a] = czm[g + a];
+      }
+    }
+  std::vector<double> dmask(disp);
+  for (size_t p = 0; p < n; ++p)
+    if (mask[p] == 0) dmask[p] = 0.0;
+  std::vector<float> xyz0(n * 3), xyzg(n * 3), xyzr(n * 3);
+  std::vector<uint8_t> P0(n), Pg(n), Pr(n);
+  orc_reproject_poly(dmask.data(), h, lw, cx.data(), cy.data(), cz.data(), R1, cfg->min_depth,
+                     cfg->max_depth, cfg->use_rectified_coords, xyz0.data(), P0.data());
+  orc_gaussian(xyz0.data(), P0.data(), xyzg.data(), Pg.data(), h, lw, cfg->gauss_window,
+               cfg->gauss_sigma, cfg->semantics);
+  const uint8_t* Pin = cfg->semantics == PSGC_SEM_CPU ? P0.data() : Pg.data();
+  if (cfg->outlier_mode == PSGC_OUTLIER_ROR) {
+    orc_ror(xyzg.data(), Pin, xyzr.data(), Pr.data(), h, lw, cfg->radius_search,
+            cfg->radius_min_neighbors, cfg->radius_window_size);
+    int64_t raw = 0, kept = 0;
+    for (size_t p = 0; p < n; ++p) {
+      raw += Pin[p] != 0;
+      kept += Pr[p] != 0;
+    }
+    if (raw > 0 && (double)kept < (double)raw * 0.1) {
+      xyzr = xyzg;
+      std::memcpy(Pr.data(), Pin, n);
+    }
+  } else if (cfg->outlier_mode == PSGC_OUTLIER_SOR) {
+    orc_sor(xyzg.data(), Pin, xyzr.data(), Pr.data(), h, lw, cfg->sor_k, cfg->sor_std_mul,
+            cfg->sor_window, nullptr, nullptr);
+  } else {
+    xyzr = xyzg;
+    std::memcpy(Pr.data(), Pin, n);
+  }
+  std::memcpy(P, Pr.data(), n);
+  if (cfg->use_mls)
+    orc_mls(xyzr.data(), Pr.data(), xyz, h, lw, cfg->mls_window_size, cfg->mls_spatial_sigma,
+            cfg->mls_polynomial_order, cfg->mls_max_radius, cfg->mls_iterations,
+            cfg->semantics);
+  else
+    std::memcpy(xyz, xyzr.data(), n * 3 * sizeof(float));
+  return PSGC_OK;
+}
+
 int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
   return PSGC_OK;

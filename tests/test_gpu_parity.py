@@ -426,8 +426,8 @@ def test_reproject_poly_stage(ctx):
     a = np.radians(2.0)
     R1 = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
     for use_rect in (1, 0):
-        rx, rP = oracle.reproject_poly(d, coef, R1, 0.0, 1500.0, use_rect)
-        gx, gP = zim_amd.reproject_poly(dev(d), [dev(c) for c in coef], R1, 0.0, 1500.0, use_rect,
+        rx, rP = oracle.reproject_poly(d, coef, R1, 0.0, 2e4, use_rect)
+        gx, gP = zim_amd.reproject_poly(dev(d), [dev(c) for c in coef], R1, 0.0, 2e4, use_rect,
                                         ctx=ctx)
         torch.cuda.synchronize()
         assert rP.sum() > 0.5 * rP.size
@@ -443,8 +443,10 @@ def test_reconstruct_poly_chain(kctx, variant):
     rows, cols = L[0].shape
     coef = scenes.poly_coefs_equivalent_to_q(rows, cols, cam.f, cam.cx, cam.cy, cam.cx,
                                              cam.baseline, "inv_cubic")
-    rc = (zim_amd.ReconConfig.default_gpu() if variant == "gpu_ref"
-          else scenes.c2_recon_config(mls_window_size=4))
+    # the reference GPU defaults with a radius that fits the 0.3 mm pixel
+    # pitch and an MLS window the tiled kernels hold (as in the Q chain test)
+    rc = (zim_amd.ReconConfig.default_gpu(radius_search=1.0, mls_window_size=5)
+          if variant == "gpu_ref" else scenes.c2_recon_config(mls_window_size=4))
     rx, rP, rd, rmc = oracle.reconstruct_poly(rc, L[0], R[0], L[1], R[1], 70, coef, p.R1)
     gx, gP, gd, gmc = zim_amd.reconstruct_poly(
         rc, *[dev(a) for a in (L[0], R[0], L[1], R[1])], 70, [dev(c) for c in coef], p.R1,
@@ -485,3 +487,40 @@ def test_batch_with_poly_pair(ctx):
             assert np.array_equal(host(out["P"]), rP)
             assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
     b.close()
+
+
+# ------------------------------------------------ ROI-restricted reconstruction --
+
+@pytest.mark.parametrize("variant", ["ror_mls", "sor", "ror_fallback"])
+def test_reconstruct_poly_roi(ctx, variant):
+    """startCalculate_v2_roi on device against the oracle, with the right
+    window from the device (checked against the oracle's)."""
+    p, L, R = _decoded_pair(3)
+    cam = p.cam
+    rows, cols = L[0].shape
+    coef = scenes.poly_coefs_equivalent_to_q(rows, cols, cam.f, cam.cx, cam.cy, cam.cx,
+                                             cam.baseline, "inv_cubic")
+    lr = [30, 17, 180, 150]
+    g = np.random.default_rng(4)
+    mask = np.ones((150, 180), np.uint8)
+    mask[g.random((150, 180)) < 0.05] = 0
+    rr_ref = oracle.roi_right_window(L[1], R[1], 70, lr, mask)
+    rr = zim_amd.roi_right_window(dev(L[1]), dev(R[1]), 70, lr, dev(mask), ctx=ctx)
+    assert rr.tolist() == rr_ref.tolist()
+    if variant == "ror_mls":
+        rc = zim_amd.ReconConfig.default(radius_search=1.0, radius_min_neighbors=6, use_mls=1,
+                                         mls_window_size=4, mls_iterations=2)
+    elif variant == "sor":
+        rc = scenes.c2_recon_config(mls_window_size=4)
+    else:
+        rc = zim_amd.ReconConfig.default(radius_search=1e-4, radius_min_neighbors=8)
+    rx, rP, rd = oracle.reconstruct_poly_roi(rc, L[0], R[0], L[1], R[1], 70, lr, rr, mask, coef,
+                                             p.R1)
+    gx, gP, gd = zim_amd.reconstruct_poly_roi(rc, *[dev(a) for a in (L[0], R[0], L[1], R[1])], 70,
+                                              lr, rr, dev(mask), [dev(c) for c in coef], p.R1,
+                                              ctx=ctx)
+    torch.cuda.synchronize()
+    assert rP.sum() > 1000
+    assert np.array_equal(host(gd), rd)
+    assert np.array_equal(host(gP), rP)
+    assert np.abs(host(gx) - rx).max() <= XYZ_TOL

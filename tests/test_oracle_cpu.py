@@ -615,3 +615,101 @@ def test_poly_reconstruct_equals_q_reconstruct_chain():
     assert (Pq != Pp).sum() <= 2                     # a float-rounding flip at a gate at most
     m = (Pq != 0) & (Pp != 0)
     assert np.abs(xq[m] - xp[m]).max() < 1e-3
+
+
+# --------------------------------------------- ROI-restricted reconstruction --
+
+def _poly_scene(seed=3):
+    p, left, right = scenes.c2_small(seed)
+    dcfg = scenes.c2_decode_config()
+    L, R = oracle.decode(dcfg, left), oracle.decode(dcfg, right)
+    cam = p.cam
+    rows, cols = L[0].shape
+    coef = scenes.poly_coefs_equivalent_to_q(rows, cols, cam.f, cam.cx, cam.cy, cam.cx,
+                                             cam.baseline, "inv_cubic")
+    return p, L, R, coef
+
+
+@pytest.mark.parametrize("cfg", ["sor_mls", "ror"])
+def test_roi_covering_the_image_equals_v2(cfg):
+    """An ROI of the whole image with an all-ones mask and full right rows is
+    startCalculate_v2: identical disparity, mask and points."""
+    p, L, R, coef = _poly_scene()
+    rows, cols = L[0].shape
+    rc = (scenes.c2_recon_config(mls_window_size=4) if cfg == "sor_mls"
+          else zim_amd_recon_default(radius_search=1.0, radius_min_neighbors=6))
+    full = [0, 0, cols, rows]
+    xr, Pr, dr = oracle.reconstruct_poly_roi(rc, L[0], R[0], L[1], R[1], 70, full, full,
+                                             np.ones((rows, cols), np.uint8), coef, p.R1)
+    xv, Pv, dv, _ = oracle.reconstruct_poly(rc, L[0], R[0], L[1], R[1], 70, coef, p.R1)
+    assert Pv.sum() > 1000
+    assert np.array_equal(dr, dv) and np.array_equal(Pr, Pv) and np.array_equal(xr, xv)
+
+
+def zim_amd_recon_default(**kw):
+    from zim_amd._capi import ReconConfig
+    return ReconConfig.default(**kw)
+
+
+def test_roi_crop_disparity_equals_full_image_disparity():
+    """The match is row-local and uses global columns, so a left crop matched
+    against full right rows reproduces the full-image disparity on every
+    masked crop pixel (bit for bit, no median); masked-out pixels are 0."""
+    p, L, R, coef = _poly_scene()
+    rows, cols = L[0].shape
+    rc = scenes.c2_recon_config(use_mls=0, use_disp_median_filter=0, outlier_mode=0)
+    lr = [37, 21, 101, 83]
+    rr = [0, 21, cols, 83]
+    g = np.random.default_rng(0)
+    mask = (g.random((83, 101)) < 0.8).astype(np.uint8)
+    _, _, dr = oracle.reconstruct_poly_roi(rc, L[0], R[0], L[1], R[1], 70, lr, rr, mask, coef,
+                                           p.R1)
+    dfull, _ = oracle.stereo_match(L[0], R[0], L[1], R[1], 70)
+    crop = dfull[21:21 + 83, 37:37 + 101].astype(np.float32)
+    assert (crop != 0).sum() > 1000
+    assert np.array_equal(dr[mask != 0], crop[mask != 0])
+    assert (dr[mask == 0] == 0).all()
+
+
+def test_roi_right_window_brute_force():
+    """prepareRectifiedROI's window against a direct numpy computation."""
+    p, L, R, _ = _poly_scene()
+    rows, cols = L[0].shape
+    g = np.random.default_rng(1)
+    for lr in ([40, 30, 60, 50], [0, 0, cols, rows], [200, 100, 56, 92]):
+        mask = (g.random((lr[3], lr[2])) < 0.6).astype(np.uint8)
+        got = oracle.roi_right_window(L[1], R[1], 70, lr, mask)
+        lk = L[1][lr[1]:lr[1] + lr[3], lr[0]:lr[0] + lr[2]]
+        need = np.unique(lk[(mask != 0) & (lk > 0) & (lk <= 70)])
+        rk = R[1][lr[1]:lr[1] + lr[3]]
+        xs = np.nonzero(np.isin(rk, need).any(axis=0))[0]
+        if len(xs):
+            x0, x1 = max(0, xs.min() - 32), min(cols - 1, xs.max() + 32)
+            want = [x0, lr[1], x1 - x0 + 1, lr[3]]
+        else:
+            want = [0, lr[1], cols, lr[3]]
+        assert got.tolist() == want, (lr, got, want)
+
+
+def test_roi_ror_fallback_when_ror_removes_nearly_all():
+    """A ROR that keeps fewer than 10 % of the points is undone: the result
+    equals the same chain with no outlier stage."""
+    p, L, R, coef = _poly_scene()
+    rows, cols = L[0].shape
+    lr = [20, 10, 200, 160]
+    rr = oracle.roi_right_window(L[1], R[1], 70, lr, np.ones((160, 200), np.uint8))
+    mask = np.ones((160, 200), np.uint8)
+    base = dict(use_mls=0)
+    harsh = zim_amd_recon_default(radius_search=1e-4, radius_min_neighbors=8, **base)
+    none = zim_amd_recon_default(outlier_mode=0, **base)
+    xh, Ph, _ = oracle.reconstruct_poly_roi(harsh, L[0], R[0], L[1], R[1], 70, lr, rr, mask, coef,
+                                            p.R1)
+    xn, Pn, _ = oracle.reconstruct_poly_roi(none, L[0], R[0], L[1], R[1], 70, lr, rr, mask, coef,
+                                            p.R1)
+    assert Pn.sum() > 1000
+    assert np.array_equal(Ph, Pn) and np.array_equal(xh, xn)
+    # a mild ROR is kept
+    mild = zim_amd_recon_default(radius_search=1.0, radius_min_neighbors=6, **base)
+    xm, Pm, _ = oracle.reconstruct_poly_roi(mild, L[0], R[0], L[1], R[1], 70, lr, rr, mask, coef,
+                                            p.R1)
+    assert 0.1 * Pn.sum() <= Pm.sum() < Pn.sum()

@@ -212,6 +212,51 @@ def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1,
     return xyz, P
 
 
+def roi_right_window(lk, rk, pha_f, left_rect, mask, ctx: Context | None = None):
+    """prepareRectifiedROI's right search window for left_rect = (x, y, w, h)
+    and its mask; returns right_rect as a 4-element int32 array."""
+    ctx = ctx or default_context()
+    dev = _is_dev(lk)
+    rows, cols = lk.shape
+    lr = np.asarray(left_rect, np.int32)
+    if not dev:
+        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
+        mask = np.ascontiguousarray(mask, np.uint8)
+    rr = np.zeros(4, np.int32)
+    _capi.check(ctx.lib.psgc_roi_right_window(ctx.h, _capi.ptr(lk), _capi.ptr(rk), rows, cols,
+                                              int(pha_f), _capi.ptr(lr), _capi.ptr(mask),
+                                              _capi.ptr(rr), MEM_DEVICE if dev else MEM_HOST))
+    return rr
+
+
+def reconstruct_poly_roi(cfg: ReconConfig, lp, rp, lk, rk, pha_f, left_rect, right_rect, mask,
+                         coef, R1, ctx: Context | None = None):
+    """startCalculate_v2_roi: reconstruct only the ROI (left_rect, right_rect
+    = (x, y, w, h), mask of left_rect's size). Returns crop-sized
+    (xyz, P, disp)."""
+    ctx = ctx or default_context()
+    dev = _is_dev(lp)
+    rows, cols = lp.shape
+    lr, rr = np.asarray(left_rect, np.int32), np.asarray(right_rect, np.int32)
+    device = lp.device if dev else None
+    h, w = int(lr[3]), int(lr[2])
+    xyz = _out(dev, (h, w, 3), torch.float32, device)
+    P = _out(dev, (h, w), torch.uint8, device)
+    disp = _out(dev, (h, w), torch.float32, device)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
+    if not dev:
+        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
+        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
+        mask = np.ascontiguousarray(mask, np.uint8)
+    Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
+    _capi.check(ctx.lib.psgc_reconstruct_poly_roi(
+        ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp), _capi.ptr(lk), _capi.ptr(rk), rows,
+        cols, int(pha_f), _capi.ptr(lr), _capi.ptr(rr), _capi.ptr(mask), _capi.ptr(cx),
+        _capi.ptr(cy), _capi.ptr(cz), _capi.ptr(Ra), _capi.ptr(xyz), _capi.ptr(P),
+        _capi.ptr(disp), MEM_DEVICE if dev else MEM_HOST))
+    return xyz, P, disp
+
+
 def load_coef_bin(path):
     """One _rx/_ry/_rz coefficient file (int32 type, rows, cols + CV_32FC4
     data, JSLAB_StereoCalibParams_V2.cpp:4-17) as a rows x cols x 4 float32

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpsgc.so")
 
-SOURCES = ["capi.hip", "plain.hip", "tiled.hip", "pipeline.hip", "adjoint.hip", "compact.hip"]
+SOURCES = ["capi.hip", "plain.hip", "tiled.hip", "pipeline.hip", "adjoint.hip", "compact.hip",
+           "roi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PSGC_OFFLOAD_ARCH", "gfx950")

@@ -212,6 +212,9 @@ def load(path: str | None = None):
         "psgc_reconstruct_poly": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, vp,
                                    vp, vp, vp, i32], i32),
         "psgc_load_coef_bin": ([C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), vp], i32),
+        "psgc_roi_right_window": ([vp, vp, vp, i32, i32, i32, vp, vp, vp, i32], i32),
+        "psgc_reconstruct_poly_roi": ([vp, vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
+                                       vp, vp, vp, vp, vp, vp, i32], i32),
         "psgc_filter_gaussian": ([vp, vp, vp, vp, vp, i32, i32, i32, flt, i32, i32], i32),
         "psgc_filter_ror": ([vp, vp, vp, vp, vp, i32, i32, flt, i32, i32, i32], i32),
         "psgc_filter_sor": ([vp, vp, vp, vp, vp, i32, i32, i32, flt, i32, vp, vp, i32], i32),

@@ -1027,6 +1027,195 @@ int psgc_reproject_poly(psgc_ctx* ctx, const double* disp, int rows, int cols,
   return finish(ctx, mem);
 }
 
+}  // extern "C"
+
+namespace psgc {
+namespace {
+
+int check_roi(int rows, int cols, const int32_t* lr, const int32_t* rr) {
+  if (!lr) PSGC_FAIL(PSGC_E_INVALID, "null left_rect");
+  if (lr[2] < 1 || lr[3] < 1 || lr[0] < 0 || lr[1] < 0 || lr[0] + lr[2] > cols ||
+      lr[1] + lr[3] > rows)
+    PSGC_FAIL(PSGC_E_INVALID, "left_rect {%d,%d,%d,%d} outside %dx%d", lr[0], lr[1], lr[2],
+              lr[3], cols, rows);
+  if (rr && (rr[1] != lr[1] || rr[3] != lr[3] || rr[2] < 1 || rr[0] < 0 ||
+             rr[0] + rr[2] > cols))
+    PSGC_FAIL(PSGC_E_INVALID, "right_rect {%d,%d,%d,%d} must share left_rect's rows", rr[0],
+              rr[1], rr[2], rr[3]);
+  return PSGC_OK;
+}
+
+}  // namespace
+}  // namespace psgc
+
+extern "C" {
+
+int psgc_roi_right_window(psgc_ctx* ctx, const int32_t* l_k, const int32_t* r_k, int rows,
+                          int cols, int pha_f, const int32_t left_rect[4], const uint8_t* mask,
+                          int32_t right_rect[4], int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!l_k || !r_k || !mask || !right_rect) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (pha_f < 0 || pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pha_f=%d", pha_f);
+  PSGC_TRY(check_roi(rows, cols, left_rect, nullptr));
+  const size_t npix = (size_t)rows * cols;
+  const size_t nroi = (size_t)left_rect[2] * left_rect[3];
+  const int32_t *lk, *rk;
+  const uint8_t* mk;
+  PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
+  PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  PSGC_TRY(stage_in(ctx, mem, S_MASK, mask, nroi, &mk));
+  void* buf;
+  PSGC_TRY(ctx_buf(ctx, S_CNT, ((size_t)pha_f + 3) * sizeof(int32_t), &buf));
+  int32_t* mnmx = (int32_t*)buf;
+  int32_t* needed = mnmx + 2;
+  const int32_t init[2] = {cols, -1};
+  PSGC_HIP(hipMemsetAsync(needed, 0, ((size_t)pha_f + 1) * sizeof(int32_t), ctx->stream));
+  PSGC_HIP(hipMemcpyAsync(mnmx, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  PSGC_TRY(launch_roi_right_window(lk, rk, rows, cols, pha_f, left_rect, mk, needed, mnmx,
+                                   ctx->stream));
+  int32_t got[2];
+  PSGC_HIP(hipMemcpyAsync(got, mnmx, sizeof(got), hipMemcpyDeviceToHost, ctx->stream));
+  PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  const int pad = 32;
+  if (got[1] >= got[0]) {
+    const int x0 = got[0] - pad > 0 ? got[0] - pad : 0;
+    const int x1 = got[1] + pad < cols - 1 ? got[1] + pad : cols - 1;
+    right_rect[0] = x0;
+    right_rect[2] = x1 - x0 + 1;
+  } else {
+    right_rect[0] = 0;
+    right_rect[2] = cols;
+  }
+  right_rect[1] = left_rect[1];
+  right_rect[3] = left_rect[3];
+  return PSGC_OK;
+}
+
+int psgc_reconstruct_poly_roi(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* l_phase,
+                              const double* r_phase, const int32_t* l_k, const int32_t* r_k,
+                              int rows, int cols, int pha_f, const int32_t left_rect[4],
+                              const int32_t right_rect[4], const uint8_t* mask,
+                              const float* coef_x, const float* coef_y, const float* coef_z,
+                              const double R1[9], float* xyz, uint8_t* P, float* disp, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  PSGC_TRY(check_size(rows, cols));
+  if (!cfg || !l_phase || !r_phase || !l_k || !r_k || !mask || !coef_x || !coef_y || !coef_z ||
+      !R1 || !xyz || !P)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (pha_f < 0 || pha_f > 65535) PSGC_FAIL(PSGC_E_INVALID, "pha_f=%d", pha_f);
+  PSGC_TRY(check_roi(rows, cols, left_rect, right_rect));
+  if (!right_rect) PSGC_FAIL(PSGC_E_INVALID, "null right_rect");
+  const int gw = cfg->gauss_window;
+  if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
+  if (cfg->outlier_mode < PSGC_OUTLIER_NONE || cfg->outlier_mode > PSGC_OUTLIER_SOR)
+    PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", cfg->outlier_mode);
+  if (cfg->use_mls && !(cfg->mls_spatial_sigma > 0))
+    PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+  const size_t npix = (size_t)rows * cols;
+  const int lw = left_rect[2], h = left_rect[3];
+  const size_t n = (size_t)lw * h;
+  hipStream_t s = ctx->stream;
+  const double *lp, *rp;
+  const int32_t *lk, *rk;
+  const uint8_t* mk;
+  const float* coef[3];
+  PSGC_TRY(stage_in(ctx, mem, S_LPH, l_phase, npix, &lp));
+  PSGC_TRY(stage_in(ctx, mem, S_RPH, r_phase, npix, &rp));
+  PSGC_TRY(stage_in(ctx, mem, S_LK, l_k, npix, &lk));
+  PSGC_TRY(stage_in(ctx, mem, S_RK, r_k, npix, &rk));
+  PSGC_TRY(stage_in(ctx, mem, S_MASK, mask, n, &mk));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF0, coef_x, npix * 4, &coef[0]));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF1, coef_y, npix * 4, &coef[1]));
+  PSGC_TRY(stage_in(ctx, mem, S_COEF2, coef_z, npix * 4, &coef[2]));
+  void *span, *d64, *d32, *d32m, *x0, *x1, *x2, *x3, *p0, *p1, *p2, *p3;
+  PSGC_TRY(ctx_buf(ctx, S_SPAN, (size_t)h * (pha_f + 1) * 2 * sizeof(int32_t), &span));
+  PSGC_TRY(ctx_buf(ctx, S_DISP64, n * sizeof(double), &d64));
+  PSGC_TRY(ctx_buf(ctx, S_DISP, n * sizeof(float), &d32));
+  PSGC_TRY(ctx_buf(ctx, S_DISP2, n * sizeof(float), &d32m));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ0, n * 12, &x0));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ1, n * 12, &x1));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ2, n * 12, &x2));
+  PSGC_TRY(ctx_buf(ctx, S_XYZ3, n * 12, &x3));
+  PSGC_TRY(ctx_buf(ctx, S_P0, n, &p0));
+  PSGC_TRY(ctx_buf(ctx, S_P1, n, &p1));
+  PSGC_TRY(ctx_buf(ctx, S_P2, n, &p2));
+  PSGC_TRY(ctx_buf(ctx, S_P3, n, &p3));
+  PSGC_TRY(launch_roi_match(lp, rp, lk, rk, cols, left_rect, right_rect, mk, pha_f,
+                            cfg->grad_threshold, (int32_t*)span, (double*)d64, (float*)d32, s));
+  const float* dfinal = (const float*)d32;
+  if (cfg->use_disp_median_filter) {
+    PSGC_TRY(launch_median5((const float*)d32, (float*)d32m, h, lw, s));
+    dfinal = (const float*)d32m;
+  }
+  const bool use64 = !cfg->use_disp_median_filter && cfg->semantics == PSGC_SEM_CPU;
+  PSGC_TRY(launch_roi_poly(use64 ? (const double*)d64 : nullptr, dfinal, mk, cols, left_rect,
+                           coef, R1, cfg->min_depth, cfg->max_depth, cfg->use_rectified_coords,
+                           (float*)x0, (uint8_t*)p0, s));
+  float k2[49];
+  gauss_kernel_f32(gw, cfg->gauss_sigma, k2);
+  PSGC_TRY(launch_gaussian((const float*)x0, (const uint8_t*)p0, (float*)x1, (uint8_t*)p1, h, lw,
+                           gw, k2, cfg->semantics, s));
+  const uint8_t* pin = cfg->semantics == PSGC_SEM_CPU ? (const uint8_t*)p0 : (const uint8_t*)p1;
+  const float* xo = (const float*)x1;
+  const uint8_t* po = pin;
+  if (cfg->outlier_mode == PSGC_OUTLIER_ROR) {
+    PSGC_TRY(launch_ror((const float*)x1, pin, (float*)x2, (uint8_t*)p2, h, lw,
+                        cfg->radius_search, cfg->radius_min_neighbors, cfg->radius_window_size,
+                        s));
+    void* cnt;
+    PSGC_TRY(ctx_buf(ctx, S_CNT, 2 * sizeof(unsigned long long), &cnt));
+    PSGC_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), s));
+    PSGC_TRY(launch_roi_ror_select((const float*)x1, pin, (const float*)x2, (const uint8_t*)p2,
+                                   n, (unsigned long long*)cnt, (float*)x3, (uint8_t*)p3, s));
+    xo = (const float*)x3;
+    po = (const uint8_t*)p3;
+  } else if (cfg->outlier_mode == PSGC_OUTLIER_SOR) {
+    void *md, *part, *st;
+    PSGC_TRY(ctx_buf(ctx, S_MEANDIST, n * sizeof(double), &md));
+    PSGC_TRY(ctx_buf(ctx, S_PARTIAL, sor_partial_doubles(n) * sizeof(double), &part));
+    PSGC_TRY(ctx_buf(ctx, S_STATS, 4 * sizeof(double), &st));
+    PSGC_TRY(launch_sor((const float*)x1, pin, (float*)x2, (uint8_t*)p2, h, lw, cfg->sor_k,
+                        cfg->sor_std_mul, cfg->sor_window, (double*)md, (double*)part,
+                        (double*)st, s));
+    xo = (const float*)x2;
+    po = (const uint8_t*)p2;
+  }
+  float* d_xyz;
+  uint8_t* d_P;
+  PSGC_TRY(stage_out(ctx, mem, S_TMP0, xyz, n * 3, &d_xyz));
+  PSGC_TRY(stage_out(ctx, mem, S_TMP1, P, n, &d_P));
+  if (cfg->use_mls) {
+    const float r2 = (cfg->mls_max_radius > 0.0f)
+                         ? cfg->mls_max_radius * cfg->mls_max_radius
+                         : 9.0f * cfg->mls_spatial_sigma * cfg->mls_spatial_sigma;
+    const int order = cfg->mls_polynomial_order < 0 ? 0
+                      : (cfg->mls_polynomial_order > 2 ? 2 : cfg->mls_polynomial_order);
+    const int n_iter = cfg->mls_iterations > 1 ? cfg->mls_iterations : 1;
+    // ping-pong between x0 (free now) and the output
+    const float* in = xo;
+    for (int it = 0; it < n_iter; ++it) {
+      const int remaining = n_iter - 1 - it;
+      float* out = (remaining % 2 == 0) ? d_xyz : (float*)x0;
+      PSGC_TRY(launch_mls(in, po, out, h, lw, cfg->mls_window_size, cfg->mls_spatial_sigma,
+                          order, r2, cfg->semantics, s));
+      in = out;
+    }
+  } else {
+    PSGC_HIP(hipMemcpyAsync(d_xyz, xo, n * 12, hipMemcpyDeviceToDevice, s));
+  }
+  PSGC_HIP(hipMemcpyAsync(d_P, po, n, hipMemcpyDeviceToDevice, s));
+  if (disp)
+    PSGC_HIP(hipMemcpyAsync(disp, dfinal, n * sizeof(float),
+                            mem == PSGC_MEM_HOST ? hipMemcpyDeviceToHost
+                                                 : hipMemcpyDeviceToDevice, s));
+  PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, n * 3));
+  PSGC_TRY(copy_back(ctx, mem, P, d_P, n));
+  return finish(ctx, mem);
+}
+
 int psgc_load_coef_bin(const char* path, int* rows, int* cols, float* out) {
   if (!path || !rows || !cols) PSGC_FAIL(PSGC_E_INVALID, "null argument");
   FILE* fp = std::fopen(path, "rb");

@@ -149,7 +149,7 @@ __device__ inline double select_kth(double* v, int n, int kth) {
 // Parabolic sub-pixel root around the integer match `best` of left column j
 // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003). `rrow` is the right
 // phase row.
-__device__ inline double subpixel_disparity(const double* rrow, int best, double lv, int j) {
+__device__ inline double subpixel_delta(const double* rrow, int best, double lv) {
   const double c0 = rrow[best - 1] - lv;
   const double c1 = rrow[best] - lv;
   const double c2 = rrow[best + 1] - lv;
@@ -157,7 +157,11 @@ __device__ inline double subpixel_disparity(const double* rrow, int best, double
   const double den = 2.0 * (c0 - c1) * (c1 - c2);
   double delta = 0.0;
   if (fabs(den) > 1e-9) delta = num / den;
-  return (double)j - (double)best - delta;
+  return delta;
+}
+
+__device__ inline double subpixel_disparity(const double* rrow, int best, double lv, int j) {
+  return (double)j - (double)best - subpixel_delta(rrow, best, lv);
 }
 
 // Q reprojection of one pixel (JSLAB_PhaseEpipolarReconstructor_V2.cpp:

@@ -51,7 +51,7 @@ enum Slot : int {
   S_LPH, S_RPH, S_LK, S_RK, S_DISP64, S_DISP, S_DISP2, S_MATCH,
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
   S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_MAPX, S_MAPY, S_SRC, S_DST, S_CNT,
-  S_HDR, S_COEF0, S_COEF1, S_COEF2,
+  S_HDR, S_COEF0, S_COEF1, S_COEF2, S_XYZ3, S_P3, S_MASK,
   S_NUM_SLOTS
 };
 
@@ -149,6 +149,22 @@ int launch_sor_finish(const float* xyz, double* md, float* dst, uint8_t* dstP,
 size_t sor_partial_doubles(size_t npix);
 int launch_sor_stats(const double* md, size_t npix, float std_mul, double* partial,
                      double* stats, hipStream_t s);
+
+// ROI-restricted reconstruction (roi.hip); lr / rr = {x, y, w, h} on the host.
+int launch_roi_right_window(const int32_t* lk, const int32_t* rk, int rows, int cols, int pha_f,
+                            const int32_t* lr, const uint8_t* mask, int32_t* needed,
+                            int32_t* mnmx, hipStream_t s);
+int launch_roi_match(const double* lp, const double* rp, const int32_t* lk, const int32_t* rk,
+                     int cols, const int32_t* lr, const int32_t* rr, const uint8_t* mask,
+                     int pha_f, double grad_thr, int32_t* span, double* disp64, float* disp32,
+                     hipStream_t s);
+int launch_roi_poly(const double* disp64, const float* disp32, const uint8_t* mask, int cols,
+                    const int32_t* lr, const float* const* coef, const double* R1,
+                    double min_depth, double max_depth, int use_rect, float* xyz, uint8_t* P,
+                    hipStream_t s);
+int launch_roi_ror_select(const float* xyz_g, const uint8_t* P_in, const float* xyz_r,
+                          const uint8_t* P_r, size_t n, unsigned long long* cnt, float* xyz,
+                          uint8_t* P, hipStream_t s);
 
 // Tiled single-image / single-pair paths (pipeline.hip).
 int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t* k,
