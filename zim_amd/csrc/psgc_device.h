@@ -9,13 +9,32 @@
 
 namespace psgc {
 
+// Global-memory accesses through pointers taken from work descriptors or
+// argument structs. Without the address space the compiler emits flat
+// accesses, which also count against the LDS wait counter, so a kernel's LDS
+// reads would wait for its outstanding global loads and stores.
+template <typename T>
+__device__ inline T ld_global(const T* p, size_t i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <typename T>
+__device__ inline void st_global(T* p, size_t i, T v) {
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+// float4 (a class with constructors) through the plain 4-wide vector type
+__device__ inline float4 ld_global(const float4* p, size_t i) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f v = ((const __attribute__((address_space(1))) v4f*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // ---------------------------------------------------------------- decode --
 
 __device__ inline double load_sample(const DecodeArgs& a, int f, int r, int x) {
   const char* row = (const char*)a.frames[f] + (size_t)r * a.stride;
-  if (a.sample_type == PSGC_SAMPLE_U16) return (double)((const uint16_t*)row)[x];
-  if (a.sample_type == PSGC_SAMPLE_U8) return (double)((const uint8_t*)row)[x];
-  return ((const double*)row)[x];
+  if (a.sample_type == PSGC_SAMPLE_U16) return (double)ld_global((const uint16_t*)row, x);
+  if (a.sample_type == PSGC_SAMPLE_U8) return (double)ld_global((const uint8_t*)row, x);
+  return ld_global((const double*)row, x);
 }
 
 // 3x3 pre-blur at (r, x), replicated border: row pass c*x + s*(l + r) on rows

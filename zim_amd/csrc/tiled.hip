@@ -149,10 +149,10 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
           ph = 0.0;
           k = 0;
         }
-        job.phase[p] = ph;
-        job.k[p] = k;
-        if (job.mod) job.mod[p] = mod0[q];
-        if (job.valid) job.valid[p] = v ? 255 : 0;
+        st_global(job.phase, p, ph);
+        st_global(job.k, p, k);
+        if (job.mod) st_global(job.mod, p, mod0[q]);
+        if (job.valid) st_global(job.valid, p, (uint8_t)(v ? 255 : 0));
       }
     }
   }
@@ -168,9 +168,9 @@ __device__ inline double smooth_at(const double* m, int rows, int cols, int r, i
   const double* u = m + (size_t)ru * cols;
   const double* c = m + (size_t)r * cols;
   const double* d = m + (size_t)rd * cols;
-  const double hu = 0.5 * u[x] + 0.25 * (u[xl] + u[xr]);
-  const double hm = 0.5 * c[x] + 0.25 * (c[xl] + c[xr]);
-  const double hd = 0.5 * d[x] + 0.25 * (d[xl] + d[xr]);
+  const double hu = 0.5 * ld_global(u, x) + 0.25 * (ld_global(u, xl) + ld_global(u, xr));
+  const double hm = 0.5 * ld_global(c, x) + 0.25 * (ld_global(c, xl) + ld_global(c, xr));
+  const double hd = 0.5 * ld_global(d, x) + 0.25 * (ld_global(d, xl) + ld_global(d, xr));
   return 0.5 * hm + 0.25 * (hu + hd);
 }
 
@@ -196,8 +196,8 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
       hi[k] = -1;
     }
     for (int c = threadIdx.x; c < cols; c += MT) {
-      sR[c] = m.smooth ? smooth_at(m.rp, m.rows, cols, r, c) : m.rp[row + c];
-      kR[c] = m.rk[row + c];
+      sR[c] = m.smooth ? smooth_at(m.rp, m.rows, cols, r, c) : ld_global(m.rp, row + c);
+      kR[c] = ld_global(m.rk, row + c);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < cols; c += MT) {
@@ -211,10 +211,11 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
     for (int jj = threadIdx.x; jj < cols; jj += MT) {
       double d = 0.0;
       int best = -1;
-      const int k = m.lk[row + jj];
+      const int k = ld_global(m.lk, row + jj);
       if (k > 0 && k <= m.pha_f) {
         const int a = lo[k], b = hi[k];
-        const double lv = m.smooth ? smooth_at(m.lp, m.rows, cols, r, jj) : m.lp[row + jj];
+        const double lv =
+            m.smooth ? smooth_at(m.lp, m.rows, cols, r, jj) : ld_global(m.lp, row + jj);
         double min_cost = 1e9;
         for (int c = a; c <= b; ++c) {
           if (kR[c] != k) continue;
@@ -230,9 +231,9 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
           best = -1;
         }
       }
-      m.disp[row + jj] = (float)d;
-      if (m.disp64) m.disp64[row + jj] = d;
-      if (m.match_col) m.match_col[row + jj] = best;
+      st_global(m.disp, row + jj, (float)d);
+      if (m.disp64) st_global(m.disp64, row + jj, d);
+      if (m.match_col) st_global(m.match_col, row + jj, (int32_t)best);
     }
     __syncthreads();
   }
@@ -267,7 +268,7 @@ __global__ void __launch_bounds__(256) median_tiled_kernel(const CloudJob* __res
       const int rr = idx / (CT + 4), cc = idx % (CT + 4);
       const int gy = min(max(y0 - 2 + rr, 0), cj.rows - 1);
       const int gx = min(max(x0 - 2 + cc, 0), cj.cols - 1);
-      tile[rr][cc] = cj.disp_in[(size_t)gy * cj.cols + gx];
+      tile[rr][cc] = ld_global(cj.disp_in, (size_t)gy * cj.cols + gx);
     }
     __syncthreads();
     const int x = x0 + lx, y = y0 + ly;
@@ -277,7 +278,7 @@ __global__ void __launch_bounds__(256) median_tiled_kernel(const CloudJob* __res
       for (int dy = 0; dy < 5; ++dy)
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) v[dy * 5 + dx] = tile[ly + dy][lx + dx];
-      cj.disp_med[(size_t)y * cj.cols + x] = median25(v);
+      st_global(cj.disp_med, (size_t)y * cj.cols + x, median25(v));
     }
     __syncthreads();
   }
@@ -308,19 +309,21 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
       uint8_t v = 0;
       if (gy >= 0 && gy < cj.rows && gx >= 0 && gx < cj.cols) {
         const size_t p = (size_t)gy * cj.cols + gx;
-        const double dd = cj.disp_src64 ? cj.disp_src64[p] : (double)cj.disp_src[p];
-        v = cj.coef[2] ? reproject_poly(cj.coef[0][p], cj.coef[1][p], cj.coef[2][p], cj.r1, dd,
+        const double dd =
+            cj.disp_src64 ? ld_global(cj.disp_src64, p) : (double)ld_global(cj.disp_src, p);
+        v = cj.coef[2] ? reproject_poly(ld_global(cj.coef[0], p), ld_global(cj.coef[1], p),
+                                        ld_global(cj.coef[2], p), cj.r1, dd,
                                         cj.min_depth, cj.max_depth, cj.use_rect, o)
                        : reproject_point(cj.q, cj.r1, gy, gx, dd, cj.min_depth,
                                          cj.max_depth, cj.use_rect, o);
         // the tile's own pixels publish the reprojection
         if (rr >= h && rr < h + CT && cc >= h && cc < h + CT) {
           if (cj.xyz0) {
-            cj.xyz0[p * 3 + 0] = o[0];
-            cj.xyz0[p * 3 + 1] = o[1];
-            cj.xyz0[p * 3 + 2] = o[2];
+            st_global(cj.xyz0, p * 3 + 0, o[0]);
+            st_global(cj.xyz0, p * 3 + 1, o[1]);
+            st_global(cj.xyz0, p * 3 + 2, o[2]);
           }
-          if (cj.P0) cj.P0[p] = v;
+          if (cj.P0) st_global(cj.P0, p, v);
         }
       }
       px[rr][cc] = o[0];
@@ -365,10 +368,10 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
           }
         }
       }
-      cj.xyzg[p * 3 + 0] = o0;
-      cj.xyzg[p * 3 + 1] = o1;
-      cj.xyzg[p * 3 + 2] = o2;
-      cj.Pg[p] = op;
+      st_global(cj.xyzg, p * 3 + 0, o0);
+      st_global(cj.xyzg, p * 3 + 1, o1);
+      st_global(cj.xyzg, p * 3 + 2, o2);
+      st_global(cj.Pg, p, op);
     }
     __syncthreads();
   }
@@ -388,19 +391,20 @@ __device__ inline void load_cloud_tile(const float* xyz, const uint8_t* P, const
   // T[r][c] = (x, y, z, 1) for a valid point, (x, y, z, 0) otherwise, so one
   // 16-byte LDS read returns a point and its mask
   const int L = CT + 2 * h;
-  const double th = thr ? *thr : 0.0;
+  const double th = thr ? ld_global(thr, 0) : 0.0;
   for (int idx = threadIdx.x; idx < L * L; idx += 256) {
     const int rr = idx / L, cc = idx % L;
     const int gy = y0 - h + rr, gx = x0 - h + cc;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (gy >= 0 && gy < rows && gx >= 0 && gx < cols) {
       const size_t p = (size_t)gy * cols + gx;
-      bool ok = P[p] != 0;
+      bool ok = ld_global(P, p) != 0;
       if (ok && md) {
-        const double d = md[p];
+        const double d = ld_global(md, p);
         ok = d >= 0.0 && d <= th;
       }
-      v = make_float4(xyz[p * 3 + 0], xyz[p * 3 + 1], xyz[p * 3 + 2], ok ? 1.f : 0.f);
+      v = make_float4(ld_global(xyz, p * 3 + 0), ld_global(xyz, p * 3 + 1),
+                      ld_global(xyz, p * 3 + 2), ok ? 1.f : 0.f);
     }
     T[rr][cc] = v;
   }
@@ -426,15 +430,21 @@ __device__ inline void keep_smallest(float (&best)[KMAX], int& cnt, int k, float
 
 // Exactly K smallest, list prefilled with +inf: every insertion is the same
 // branch-free carry chain; the point is eligible iff the list filled up.
+// Keeps the K smallest values in ascending order: a new value replaces the
+// largest and bubbles down until it is in place. The window is scanned in
+// rings of growing radius, so most values land at or near the end and the
+// bubble stops after a step or two (a wave leaves the chain when all its
+// lanes have).
 template <int K>
 __device__ inline void keep_smallest_fixed(float (&best)[K], float v) {
   if (!(v < best[K - 1])) return;
+  best[K - 1] = v;
 #pragma unroll
-  for (int t = 0; t < K; ++t) {
-    const float b = best[t];
-    const bool sw = v < b;
-    best[t] = sw ? v : b;
-    v = sw ? b : v;
+  for (int t = K - 1; t > 0; --t) {
+    const float a = best[t - 1], b = best[t];
+    if (!(b < a)) break;
+    best[t - 1] = b;
+    best[t] = a;
   }
 }
 
@@ -512,7 +522,7 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
           }
         }
       }
-      cj.md[(size_t)y * cj.cols + x] = out;
+      st_global(cj.md, (size_t)y * cj.cols + x, out);
     }
     __syncthreads();
   }
@@ -575,7 +585,7 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
       const float pi[3] = {q.x, q.y, q.z};
       float out[3] = {pi[0], pi[1], pi[2]};
       const bool mine = q.w != 0.f;
-      if (cj.mls_Pout) cj.mls_Pout[p] = mine ? 1 : 0;
+      if (cj.mls_Pout) st_global(cj.mls_Pout, p, (uint8_t)(mine ? 1 : 0));
       // points removed by SOR enter MLS as zeros (the SOR output), and MLS
       // copies its input for points outside the mask
       if (!mine && cj.mls_md) out[0] = out[1] = out[2] = 0.f;
@@ -649,9 +659,9 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             mls_fit_fixed<3>(f, bits, Wd, cy - hw, cx - hw, T, pi, inv2s2, cur, out);
         }
       }
-      cj.mls_dst[p * 3 + 0] = out[0];
-      cj.mls_dst[p * 3 + 1] = out[1];
-      cj.mls_dst[p * 3 + 2] = out[2];
+      st_global(cj.mls_dst, p * 3 + 0, out[0]);
+      st_global(cj.mls_dst, p * 3 + 1, out[1]);
+      st_global(cj.mls_dst, p * 3 + 2, out[2]);
     }
     __syncthreads();
   }
