@@ -448,13 +448,18 @@ __device__ inline void keep_smallest_fixed(float (&best)[K], float v) {
   }
 }
 
-// Ring position e (0 <= e < 8 rad) -> offset on the Chebyshev ring `rad`.
-__device__ inline void ring_offset(int rad, int e, int* dy, int* dx) {
-  const int side = e / (2 * rad), off = e - side * (2 * rad);
-  if (side == 0) { *dy = -rad; *dx = -rad + off; }
-  else if (side == 1) { *dx = rad; *dy = -rad + off; }
-  else if (side == 2) { *dy = rad; *dx = rad - off; }
-  else { *dx = -rad; *dy = rad - off; }
+// Visits the window entries at Chebyshev distance 1..h from (cy, cx), ring by
+// ring (near first), each ring as four runs: the top row left to right, the
+// right column downwards, the bottom row right to left, the left column
+// upwards. Loop bounds are uniform, so no per-entry index arithmetic.
+template <typename F>
+__device__ inline void for_each_ring_entry(int h, int cy, int cx, F&& visit) {
+  for (int rad = 1; rad <= h; ++rad) {
+    for (int o = -rad; o < rad; ++o) visit(cy - rad, cx + o);
+    for (int o = -rad; o < rad; ++o) visit(cy + o, cx + rad);
+    for (int o = rad; o > -rad; --o) visit(cy + rad, cx + o);
+    for (int o = rad; o > -rad; --o) visit(cy + o, cx - rad);
+  }
 }
 
 // KFIX > 0: k == KFIX known at compile time; otherwise k <= KMAX at run time.
@@ -485,15 +490,12 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
           float best[KFIX];
 #pragma unroll
           for (int m = 0; m < KFIX; ++m) best[m] = __builtin_huge_valf();
-          for (int rad = 1; rad <= h; ++rad)
-            for (int e = 0; e < 8 * rad; ++e) {
-              int dy, dx;
-              ring_offset(rad, e, &dy, &dx);
-              const float4 e4 = T[cy + dy][cx + dx];
-              if (e4.w == 0.f) continue;
-              const float p3[3] = {e4.x, e4.y, e4.z};
-              keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
-            }
+          for_each_ring_entry(h, cy, cx, [&](int ry, int rx) {
+            const float4 e4 = T[ry][rx];
+            if (e4.w == 0.f) return;
+            const float p3[3] = {e4.x, e4.y, e4.z};
+            keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
+          });
           if (best[KFIX - 1] < __builtin_huge_valf()) {
             double s = 0.0;
 #pragma unroll
@@ -504,15 +506,12 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
           float best[KMAX];
           int cnt = 0;
           const int k = cj.sor_k;
-          for (int rad = 1; rad <= h; ++rad)
-            for (int e = 0; e < 8 * rad; ++e) {
-              int dy, dx;
-              ring_offset(rad, e, &dy, &dx);
-              const float4 e4 = T[cy + dy][cx + dx];
-              if (e4.w == 0.f) continue;
-              const float p3[3] = {e4.x, e4.y, e4.z};
-              keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
-            }
+          for_each_ring_entry(h, cy, cx, [&](int ry, int rx) {
+            const float4 e4 = T[ry][rx];
+            if (e4.w == 0.f) return;
+            const float p3[3] = {e4.x, e4.y, e4.z};
+            keep_smallest<KMAX>(best, cnt, k, dist2(qx, qy, qz, p3));
+          });
           if (cnt >= k) {
             double s = 0.0;
 #pragma unroll
