@@ -387,8 +387,15 @@ PSGC_API int psgc_batch_destroy(psgc_batch* batch);
  * per-kernel instrumentation (psgc_profile_*) can see each launch. */
 PSGC_API int psgc_batch_run_eager(psgc_batch* batch);
 /* Launch shape chosen for stage `stage` ("decode", "match", "median5",
- * "repro_gauss", "sor_knn", "mls"): workgroups of the persistent grid. */
+ * "repro_gauss", "sor_knn", "mls"): workgroups of the persistent grid;
+ * "decode_rows": rows per decode tile (2 or 4). 0 = the stage's default grid. */
 PSGC_API int psgc_batch_grid(psgc_batch* batch, const char* stage, int* grid);
+/* Pins a launch shape (same stage names; grids 1..2^20 workgroups, decode_rows
+ * 2 or 4), e.g. one saved from an earlier autotuned batch of the same
+ * resolutions. A captured graph is re-captured. Results do not depend on the
+ * launch shape. No counterpart in the reference (its launches have fixed
+ * shapes). */
+PSGC_API int psgc_batch_set_grid(psgc_batch* batch, const char* stage, int value);
 
 /* ---- rectification, compaction and output ---------------------------------
  * psgc_rectify_maps: cv::initUndistortRectifyMap for zero distortion

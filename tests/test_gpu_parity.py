@@ -298,6 +298,24 @@ def test_batch_autotune_keeps_results(ctx):
     assert np.array_equal(host(out["P"]), rP)
     assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
     assert b.grid("mls") > 0 and b.grid("decode") > 0
+    assert b.grid("decode_rows") in (2, 4)
+    # pinned shapes (the other decode tile height, odd grid sizes) re-capture
+    # the graph and leave the results unchanged
+    rows = 6 - b.grid("decode_rows")
+    for stage, v in (("decode_rows", rows), ("decode", 37), ("match", 300), ("mls", 257),
+                     ("sor_knn", 1)):
+        b.set_grid(stage, v)
+        assert b.grid(stage) == v
+    for run in (b.run, b.run_eager):
+        out["xyz"].fill_(float("nan"))
+        out["P"].fill_(7)
+        run()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(out["P"]), rP)
+        assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
+    for stage, v in (("decode_rows", 3), ("decode", 0), ("nope", 64)):
+        with pytest.raises(zim_amd.PsgcError):
+            b.set_grid(stage, v)
     b.close()
 
 

@@ -338,6 +338,11 @@ class Batch:
         _capi.check(self.ctx.lib.psgc_batch_grid(self.h, stage.encode(), C.byref(g)))
         return g.value
 
+    def set_grid(self, stage: str, value: int):
+        """Pins a stage's launch shape (re-captures the graph)."""
+        self.ctx.bind_stream()
+        _capi.check(self.ctx.lib.psgc_batch_set_grid(self.h, stage.encode(), int(value)))
+
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.psgc_batch_destroy(self.h)

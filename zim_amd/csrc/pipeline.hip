@@ -118,7 +118,8 @@ struct Plan {
   std::vector<Blob> blobs;
   int max_cols = 0, max_knum = 0, max_ghalf = 0, max_sor_half = 0, max_sor_k = 0, max_mls_hw = 0;
   int min_sor_k = 1 << 30;
-  std::map<std::string, int> grid;
+  std::map<std::string, int> grid;  // per stage; "decode_rows": the decode tile height
+  int dec_rows = 4;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -132,6 +133,18 @@ struct Plan {
 namespace {
 
 int tiles(int n, int t) { return (n + t - 1) / t; }
+
+// Lays the decode tiles out as kDecodeTileCols x tile_rows per job.
+void layout_decode(Plan* pl, int tile_rows) {
+  pl->dec_rows = tile_rows;
+  pl->dec.total = 0;
+  for (DecodeJob& j : pl->dec.host) {
+    j.tiles_x = tiles(j.a.cols, kDecodeTileCols);
+    j.tiles_y = tiles(j.a.rows, tile_rows);
+    j.tile_base = pl->dec.total;
+    pl->dec.total += (int64_t)j.tiles_x * j.tiles_y;
+  }
+}
 
 int upload_tables(Plan* pl) {
   // device copies of the tables live at the end of the arena
@@ -394,7 +407,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
     PSGC_TRY(launch_decode_plain(d.a, d.phase, d.k, d.mod, d.valid, s));
   if (!pl->dec.host.empty())
     PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
-                                 g("decode", 4), s));
+                                 g("decode", 4), pl->dec_rows, s));
   for (size_t c = 0; c < pl->gray_fix.size(); ++c)
     PSGC_TRY(launch_gray_fix(pl->gray_fix[c], pl->gray_in_ph[c], pl->gray_out_ph[c],
                              pl->gray_in_k[c], pl->gray_out_k[c], pl->gray_valid[c], s));
@@ -437,11 +450,24 @@ static int run_plan(Plan* pl, hipStream_t s) {
   return PSGC_OK;
 }
 
-// Times one stage of the plan over candidate persistent grid sizes and keeps
-// the fastest (the plan's inputs of that stage must already be valid).
+// Re-lays the decode job table for a tile height and uploads it (the stream
+// is drained first: no launch in flight reads the table).
+static int relayout_decode(Plan* pl, int tile_rows, hipStream_t s) {
+  PSGC_HIP(hipStreamSynchronize(s));
+  layout_decode(pl, tile_rows);
+  PSGC_HIP(hipMemcpy(pl->dec.dev, pl->dec.host.data(), pl->dec.host.size() * sizeof(DecodeJob),
+                     hipMemcpyHostToDevice));
+  return PSGC_OK;
+}
+
+// Times each stage of the plan over candidate persistent grid sizes (and, for
+// the decode, tile heights) and keeps the fastest; the plan's inputs of every
+// stage must already be valid.
 static int autotune(Plan* pl, hipStream_t s) {
   const int cu = pl->ctx->cu_count > 0 ? pl->ctx->cu_count : 256;
   const int cands[] = {1, 2, 4, 8, 16};
+  constexpr int ncand = sizeof(cands) / sizeof(cands[0]);
+  const int dec_rows[] = {4, 2};
   hipEvent_t a, b;
   PSGC_HIP(hipEventCreate(&a));
   PSGC_HIP(hipEventCreate(&b));
@@ -454,17 +480,19 @@ static int autotune(Plan* pl, hipStream_t s) {
                        {"sor_knn", !pl->sor.host.empty()},   {"mls", !pl->mls.empty()}};
   for (const St& st : stages) {
     if (!st.on) continue;
+    const bool dec = std::string(st.name) == "decode";
     float best_ms = 1e30f;
-    int best_g = 0;
-    for (int pc : cands) {
-      const int grid = cu * pc;
+    int best_g = 0, best_rows = pl->dec_rows;
+    for (int ci = 0; ci < ncand * (dec ? 2 : 1); ++ci) {
+      const int grid = cu * cands[ci % ncand];
+      if (dec && ci % ncand == 0) PSGC_TRY(relayout_decode(pl, dec_rows[ci / ncand], s));
       float tot = 0;
       for (int rep = 0; rep < 2; ++rep) {
         PSGC_HIP(hipEventRecord(a, s));
         const std::string n = st.name;
         if (n == "decode")
           PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
-                                       grid, s));
+                                       grid, pl->dec_rows, s));
         else if (n == "match")
           PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total, grid,
                                       pl->max_cols, pl->max_knum, s));
@@ -490,9 +518,14 @@ static int autotune(Plan* pl, hipStream_t s) {
       if (tot < best_ms) {
         best_ms = tot;
         best_g = grid;
+        best_rows = pl->dec_rows;
       }
     }
     pl->grid[st.name] = best_g;
+    if (dec) {
+      PSGC_TRY(relayout_decode(pl, best_rows, s));
+      pl->grid["decode_rows"] = best_rows;
+    }
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
@@ -554,11 +587,8 @@ static int plan_decode(Plan* pl, const psgc_decode_config& dc, const void* const
     pl->blobs.push_back(std::move(b));
     pl->plain_dec.push_back(PlainDecode{j.a, j.phase, j.k, j.mod, j.valid});
   } else {
-    j.tiles_x = tiles(cols, 256);
-    j.tiles_y = tiles(rows, 4);
-    j.tile_base = pl->dec.total;
-    pl->dec.total += (int64_t)j.tiles_x * j.tiles_y;
     pl->dec.host.push_back(j);
+    layout_decode(pl, pl->dec_rows);
   }
   if (gray) {
     double* ph2 = (double*)pl->arena.take(np * 8);
@@ -590,6 +620,38 @@ struct psgc_batch {
   Plan plan;
   bool graph = false;
 };
+
+// Captures the plan's launches into a graph (replacing an earlier one).
+static int capture(psgc_batch* b) {
+  hipStream_t s = b->ctx->stream;
+  Plan& pl = b->plan;
+  if (pl.exec) hipGraphExecDestroy(pl.exec);
+  if (pl.graph) hipGraphDestroy(pl.graph);
+  pl.exec = nullptr;
+  pl.graph = nullptr;
+  b->graph = false;
+  const bool prof = g_prof_on;
+  g_prof_on = false;  // no event records inside the captured graph
+  int rc = PSGC_OK;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = run_plan(&pl, s);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    if (rc == PSGC_OK && e2 == hipSuccess) {
+      pl.graph = graph;
+      e = hipGraphInstantiate(&pl.exec, graph, nullptr, nullptr, 0);
+    } else {
+      e = e2;
+      if (graph) hipGraphDestroy(graph);
+    }
+  }
+  g_prof_on = prof;
+  if (rc != PSGC_OK) return rc;
+  if (e != hipSuccess) PSGC_FAIL(PSGC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  b->graph = true;
+  return PSGC_OK;
+}
 
 extern "C" {
 
@@ -646,28 +708,11 @@ int psgc_batch_create(psgc_ctx* ctx, const psgc_pair_desc* pairs, int n, int fla
     }
   }
   if (flags & PSGC_BATCH_GRAPH) {
-    const bool prof = g_prof_on;
-    g_prof_on = false;  // no event records inside the captured graph
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      rc = run_plan(&b->plan, s);
-      hipGraph_t graph = nullptr;
-      hipError_t e2 = hipStreamEndCapture(s, &graph);
-      if (rc == PSGC_OK && e2 == hipSuccess) {
-        b->plan.graph = graph;
-        e = hipGraphInstantiate(&b->plan.exec, graph, nullptr, nullptr, 0);
-      } else {
-        e = e2;
-        if (graph) hipGraphDestroy(graph);
-      }
-    }
-    g_prof_on = prof;
-    if (rc != PSGC_OK || e != hipSuccess) {
+    rc = capture(b);
+    if (rc != PSGC_OK) {
       delete b;
-      if (rc != PSGC_OK) return rc;
-      PSGC_FAIL(PSGC_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      return rc;
     }
-    b->graph = true;
   }
   *out = b;
   return PSGC_OK;
@@ -699,8 +744,37 @@ int psgc_batch_destroy(psgc_batch* b) {
 
 int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
   if (!b || !stage || !grid) PSGC_FAIL(PSGC_E_INVALID, "null argument");
-  auto it = b->plan.grid.find(stage);
+  const std::string st = stage;
+  if (st == "decode_rows") {
+    *grid = b->plan.dec_rows;
+    return PSGC_OK;
+  }
+  auto it = b->plan.grid.find(st);
   *grid = it != b->plan.grid.end() ? it->second : 0;
+  return PSGC_OK;
+}
+
+int psgc_batch_set_grid(psgc_batch* b, const char* stage, int value) {
+  if (!b || !stage) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const std::string st = stage;
+  static const char* const kStages[] = {"decode", "match", "median5", "repro_gauss",
+                                        "sor_knn", "mls"};
+  PSGC_HIP(hipSetDevice(b->ctx->device));
+  if (st == "decode_rows") {
+    if (value != 2 && value != 4)
+      PSGC_FAIL(PSGC_E_INVALID, "decode_rows must be 2 or 4 (got %d)", value);
+    if (!b->plan.dec.host.empty()) PSGC_TRY(relayout_decode(&b->plan, value, b->ctx->stream));
+    else b->plan.dec_rows = value;
+    b->plan.grid["decode_rows"] = value;
+  } else {
+    bool known = false;
+    for (const char* k : kStages) known = known || st == k;
+    if (!known) PSGC_FAIL(PSGC_E_INVALID, "unknown stage '%s'", stage);
+    if (value < 1 || value > (1 << 20))
+      PSGC_FAIL(PSGC_E_INVALID, "grid of %d workgroups for '%s' (1..2^20)", value, stage);
+    b->plan.grid[st] = value;
+  }
+  if (b->graph) PSGC_TRY(capture(b));
   return PSGC_OK;
 }
 
@@ -717,14 +791,14 @@ int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t*
   j.k = k;
   j.mod = mod;
   j.valid = valid;
-  j.tiles_x = tiles(a.cols, 256);
+  j.tiles_x = tiles(a.cols, kDecodeTileCols);
   j.tiles_y = tiles(a.rows, 4);
   j.tile_base = 0;
   void* d;
   PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(DecodeJob), &d));
   PSGC_HIP(hipMemcpyAsync(d, &j, sizeof(DecodeJob), hipMemcpyHostToDevice, s));
   return launch_decode_tiled((const DecodeJob*)d, 1, (int64_t)j.tiles_x * j.tiles_y,
-                             default_grid(ctx, 4), s);
+                             default_grid(ctx, 4), 4, s);
 }
 
 int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,

@@ -109,8 +109,11 @@ __device__ inline XcdRange xcd_range(int64_t total) {
 }
 
 // Launchers (asynchronous on s; jobs in device memory).
+// Decode tiles are kDecodeTileCols x tile_rows (tile_rows 2 or 4: output rows
+// per thread); the jobs' tiles_x / tiles_y / tile_base must be laid out for it.
+constexpr int kDecodeTileCols = 256;
 int launch_decode_tiled(const DecodeJob* jobs_dev, int njobs, int64_t total_tiles, int grid,
-                        hipStream_t s);
+                        int tile_rows, hipStream_t s);
 int launch_match_tiled(const MatchJob* jobs_dev, int njobs, int64_t total_rows, int grid,
                        int max_cols, int max_knum, hipStream_t s);
 int launch_median_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, hipStream_t s);

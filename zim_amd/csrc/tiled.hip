@@ -29,47 +29,94 @@ namespace {
 // ----------------------------------------------------------------- decode --
 
 constexpr int DC = 256;  // tile width (threads)
-constexpr int DR = 4;    // output rows per thread
-constexpr int DLR = DR + 2, DLC = DC + 2;
+constexpr int DLC = DC + 2;
 
-// The (DR + 2) x (DC + 2) sample tile of frame f (replicated border), in
-// two steps so that the global loads of the next frame are in flight while
-// the current one is computed: decode_fetch issues them into registers,
-// decode_store writes the registers to LDS. Thread t holds column t of every
-// tile row; threads t < 2 DLR also hold one of the 2 x DLR samples past
-// column DC (row t / 2, column DC + t % 2).
+// A tile is DC columns by DRT rows (DRT output rows per thread, chosen by the
+// batch autotuner); its frame samples span (DRT + 2) x (DC + 2) with the
+// replicated border. Thread t holds column t of every tile row; threads
+// t < 2 (DRT + 2) also hold one of the samples past column DC (row t / 2,
+// column DC + t % 2).
+template <int DRT>
 struct FrameRegs {
-  double v[DLR];
+  double v[DRT + 2];
   double e;
 };
 
-constexpr int kDecodeExtra = (DLC - DC) * DLR;
+template <int DRT>
+constexpr int decode_extra() { return (DLC - DC) * (DRT + 2); }
 
-__device__ inline void decode_fetch(const DecodeArgs& a, int f, int y0, int x0, FrameRegs* r) {
+// f64 frames: decode_fetch issues the next frame's loads into registers,
+// decode_store writes them to LDS after the current frame is computed.
+template <int DRT>
+__device__ inline void decode_fetch(const DecodeArgs& a, int f, int y0, int x0,
+                                    FrameRegs<DRT>* r) {
   const int t = threadIdx.x;
   const int gx = min(max(x0 - 1 + t, 0), a.cols - 1);
 #pragma unroll
-  for (int rr = 0; rr < DLR; ++rr) {
+  for (int rr = 0; rr < DRT + 2; ++rr) {
     const int gy = min(max(y0 - 1 + rr, 0), a.rows - 1);
     r->v[rr] = load_sample(a, f, gy, gx);
   }
-  if (t < kDecodeExtra) {
+  if (t < decode_extra<DRT>()) {
     const int gy = min(max(y0 - 1 + t / 2, 0), a.rows - 1);
     const int gxe = min(max(x0 - 1 + DC + t % 2, 0), a.cols - 1);
     r->e = load_sample(a, f, gy, gxe);
   }
 }
 
-__device__ inline void decode_store(const FrameRegs& r, double (*buf)[DLC]) {
+template <int DRT>
+__device__ inline void decode_store(const FrameRegs<DRT>& r, double (*buf)[DLC]) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int rr = 0; rr < DLR; ++rr) buf[rr][t] = r.v[rr];
-  if (t < kDecodeExtra) buf[t / 2][DC + t % 2] = r.e;
+  for (int rr = 0; rr < DRT + 2; ++rr) buf[rr][t] = r.v[rr];
+  if (t < decode_extra<DRT>()) buf[t / 2][DC + t % 2] = r.e;
 }
 
-__global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __restrict__ jobs,
-                                                          int njobs, int64_t total) {
-  __shared__ double buf[2][DLR][DLC];
+// Integer frames (u8, u16) are staged DCH frames at a time as raw 16-bit
+// samples, so a tile pays the global-load latency once per chunk instead of
+// once per frame; f64 frames use two double buffers. Both layouts share one
+// LDS block.
+constexpr int DCH = 12;
+
+template <int DRT>
+constexpr size_t decode_lds_bytes() {
+  const size_t raw = (size_t)DCH * (DRT + 2) * DLC * sizeof(uint16_t);
+  const size_t f64 = (size_t)2 * (DRT + 2) * DLC * sizeof(double);
+  return raw > f64 ? raw : f64;
+}
+
+__device__ inline uint16_t load_raw16(const DecodeArgs& a, int f, int r, int x) {
+  const char* row = (const char*)a.frames[f] + (size_t)r * a.stride;
+  if (a.sample_type == PSGC_SAMPLE_U16) return ld_global((const uint16_t*)row, x);
+  return ld_global((const uint8_t*)row, x);
+}
+
+// Frame f's (DRT + 2) x (DC + 2) tile (replicated border) as raw samples.
+template <int DRT>
+__device__ inline void decode_stage_raw(const DecodeArgs& a, int f, int y0, int x0,
+                                        uint16_t (*raw)[DLC]) {
+  const int t = threadIdx.x;
+  const int gx = min(max(x0 - 1 + t, 0), a.cols - 1);
+  uint16_t v[DRT + 2];
+#pragma unroll
+  for (int rr = 0; rr < DRT + 2; ++rr)
+    v[rr] = load_raw16(a, f, min(max(y0 - 1 + rr, 0), a.rows - 1), gx);
+  uint16_t e = 0;
+  if (t < decode_extra<DRT>())
+    e = load_raw16(a, f, min(max(y0 - 1 + t / 2, 0), a.rows - 1),
+                   min(max(x0 - 1 + DC + t % 2, 0), a.cols - 1));
+#pragma unroll
+  for (int rr = 0; rr < DRT + 2; ++rr) raw[rr][t] = v[rr];
+  if (t < decode_extra<DRT>()) raw[t / 2][DC + t % 2] = e;
+}
+
+// Four waves per SIMD (at most 128 VGPRs) measure faster than three without
+// the bound, despite a few spilled scalars (4-row tiles).
+template <int DRT>
+__global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __restrict__ jobs,
+                                                             int njobs, int64_t total) {
+  constexpr int DLR = DRT + 2;
+  __shared__ __align__(16) unsigned char lds[decode_lds_bytes<DRT>()];
   const XcdRange xr = xcd_range(total);
   int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
@@ -79,36 +126,29 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
     const DecodeArgs& a = job.a;
     const int64_t lt = t - job.tile_base;
     const int tx = (int)(lt % job.tiles_x), ty = (int)(lt / job.tiles_x);
-    const int x0 = tx * DC, y0 = ty * DR;
+    const int x0 = tx * DC, y0 = ty * DRT;
     const int x = x0 + threadIdx.x;
     const double c = a.blur_c, s = a.blur_s;
+    const int tc = threadIdx.x + 1;
 
-    double S[DR], Cs[DR], G[DR], wrap[3][DR], mod0[DR];
-    bool ok[DR];
-    for (int q = 0; q < DR; ++q) {
+    double S[DRT], Cs[DRT], G[DRT], wrap[3][DRT], mod0[DRT];
+    bool ok[DRT];
+    for (int q = 0; q < DRT; ++q) {
       ok[q] = true;
       mod0[q] = 0.0;
       wrap[0][q] = wrap[1][q] = wrap[2][q] = 0.0;
+      S[q] = Cs[q] = G[q] = 0.0;
     }
     int nframes = 0;
     for (int i = 0; i < a.nlev; ++i) nframes += a.shift_num[i];
-
-    FrameRegs pre;
-    decode_fetch(a, 0, y0, x0, &pre);
-    decode_store(pre, buf[0]);
-    __syncthreads();
     int level = 0, kin = 0;
-    for (int q = 0; q < DR; ++q) S[q] = Cs[q] = G[q] = 0.0;
-    for (int f = 0; f < nframes; ++f) {
-      if (f + 1 < nframes) decode_fetch(a, f + 1, y0, x0, &pre);
-      const double(*b)[DLC] = buf[f & 1];
-      const int tc = threadIdx.x + 1;
-      double h[DLR];
-#pragma unroll
-      for (int rr = 0; rr < DLR; ++rr) h[rr] = c * b[rr][tc] + s * (b[rr][tc - 1] + b[rr][tc + 1]);
+
+    // one frame's blurred samples into the level sums (h: the horizontal
+    // pass of the tile rows at this thread's column)
+    auto accumulate = [&](int f, const double (&h)[DLR]) {
       const double st = a.sin_tab[f], ct = a.cos_tab[f];
 #pragma unroll
-      for (int q = 0; q < DR; ++q) {
+      for (int q = 0; q < DRT; ++q) {
         const double v = c * h[q + 1] + s * (h[q] + h[q + 2]);
         S[q] -= st * v;
         Cs[q] += ct * v;
@@ -116,7 +156,7 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
       }
       if (++kin == a.shift_num[level]) {
 #pragma unroll
-        for (int q = 0; q < DR; ++q) {
+        for (int q = 0; q < DRT; ++q) {
           double m;
           const bool lv = level_finish(S[q], Cs[q], G[q], a.shift_num[level], a.mod_thresh,
                                        &wrap[level][q], &m);
@@ -127,13 +167,46 @@ __global__ void __launch_bounds__(DC) decode_tiled_kernel(const DecodeJob* __res
         kin = 0;
         ++level;
       }
-      // buf[(f + 1) & 1] was last read in iteration f - 1, before that
-      // iteration's barrier
-      if (f + 1 < nframes) decode_store(pre, buf[(f + 1) & 1]);
+    };
+
+    if (a.sample_type != PSGC_SAMPLE_F64) {
+      uint16_t(*raw)[DLR][DLC] = (uint16_t(*)[DLR][DLC])lds;
+      for (int f0 = 0; f0 < nframes; f0 += DCH) {
+        const int nf = min(DCH, nframes - f0);
+        for (int g = 0; g < nf; ++g) decode_stage_raw<DRT>(a, f0 + g, y0, x0, raw[g]);
+        __syncthreads();
+        for (int g = 0; g < nf; ++g) {
+          const uint16_t(*b)[DLC] = raw[g];
+          double h[DLR];
+#pragma unroll
+          for (int rr = 0; rr < DLR; ++rr)
+            h[rr] = c * (double)b[rr][tc] + s * ((double)b[rr][tc - 1] + (double)b[rr][tc + 1]);
+          accumulate(f0 + g, h);
+        }
+        __syncthreads();  // the chunk's readers are done before it is refilled
+      }
+    } else {
+      double(*buf)[DLR][DLC] = (double(*)[DLR][DLC])lds;
+      FrameRegs<DRT> pre;
+      decode_fetch<DRT>(a, 0, y0, x0, &pre);
+      decode_store<DRT>(pre, buf[0]);
       __syncthreads();
+      for (int f = 0; f < nframes; ++f) {
+        if (f + 1 < nframes) decode_fetch<DRT>(a, f + 1, y0, x0, &pre);
+        const double(*b)[DLC] = buf[f & 1];
+        double h[DLR];
+#pragma unroll
+        for (int rr = 0; rr < DLR; ++rr)
+          h[rr] = c * b[rr][tc] + s * (b[rr][tc - 1] + b[rr][tc + 1]);
+        accumulate(f, h);
+        // buf[(f + 1) & 1] was last read in iteration f - 1, before that
+        // iteration's barrier
+        if (f + 1 < nframes) decode_store<DRT>(pre, buf[(f + 1) & 1]);
+        __syncthreads();
+      }
     }
     if (x < a.cols) {
-      for (int q = 0; q < DR; ++q) {
+      for (int q = 0; q < DRT; ++q) {
         const int y = y0 + q;
         if (y >= a.rows) break;
         const size_t p = (size_t)y * a.cols + x;
@@ -683,10 +756,15 @@ size_t match_tiled_lds(int cols, int knum) {
 int match_tiled_max_lds() { return 64 * 1024; }
 
 int launch_decode_tiled(const DecodeJob* jobs, int njobs, int64_t total, int grid,
-                        hipStream_t s) {
+                        int tile_rows, hipStream_t s) {
   if (total <= 0) return PSGC_OK;
+  if (tile_rows != 2 && tile_rows != 4)
+    PSGC_FAIL(PSGC_E_INVALID, "decode tile of %d rows (2 or 4)", tile_rows);
   PSGC_PROF_BEGIN("decode", s);
-  decode_tiled_kernel<<<round_grid(total, grid), DC, 0, s>>>(jobs, njobs, total);
+  if (tile_rows == 2)
+    decode_tiled_kernel<2><<<round_grid(total, grid), DC, 0, s>>>(jobs, njobs, total);
+  else
+    decode_tiled_kernel<4><<<round_grid(total, grid), DC, 0, s>>>(jobs, njobs, total);
   PSGC_LAUNCHED();
   PSGC_PROF_END("decode", s);
   return PSGC_OK;
