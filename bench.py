@@ -94,7 +94,7 @@ def main():
                                 Q=p.Q, R1=p.R1) for p in pairs], ctx, graph=True, autotune=True)
     torch.cuda.synchronize()
     grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "median5",
-                                        "repro_gauss", "sor_knn", "mls")}
+                                        "repro_gauss", "sor_knn", "mls", "mls_variant")}
     log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}")
     for w in range(args.warmup):
         batch.run()

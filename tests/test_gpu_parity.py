@@ -302,8 +302,9 @@ def test_batch_autotune_keeps_results(ctx):
     # pinned shapes (the other decode tile height, odd grid sizes) re-capture
     # the graph and leave the results unchanged
     rows = 6 - b.grid("decode_rows")
-    for stage, v in (("decode_rows", rows), ("decode", 37), ("match", 300), ("mls", 257),
-                     ("sor_knn", 1)):
+    variant = 1 - b.grid("mls_variant")
+    for stage, v in (("decode_rows", rows), ("mls_variant", variant), ("decode", 37),
+                     ("match", 300), ("mls", 257), ("sor_knn", 1)):
         b.set_grid(stage, v)
         assert b.grid(stage) == v
     for run in (b.run, b.run_eager):
@@ -313,9 +314,56 @@ def test_batch_autotune_keeps_results(ctx):
         torch.cuda.synchronize()
         assert np.array_equal(host(out["P"]), rP)
         assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
-    for stage, v in (("decode_rows", 3), ("decode", 0), ("nope", 64)):
+    for stage, v in (("decode_rows", 3), ("mls_variant", 2), ("decode", 0), ("nope", 64)):
         with pytest.raises(zim_amd.PsgcError):
             b.set_grid(stage, v)
+    b.close()
+
+
+def _inradius_counts(xyz, P, hw, r):
+    """In-radius valid neighbours (the point itself included) of every valid
+    point over the (2 hw + 1)^2 window, with the MLS filter's float32
+    squared distance."""
+    rows, cols = P.shape
+    x = xyz.astype(np.float32)
+    ok = P != 0
+    cnt = np.zeros((rows, cols), np.int32)
+    for dy in range(-hw, hw + 1):
+        for dx in range(-hw, hw + 1):
+            ys, yd = slice(max(dy, 0), rows + min(dy, 0)), slice(max(-dy, 0), rows + min(-dy, 0))
+            xs, xd = slice(max(dx, 0), cols + min(dx, 0)), slice(max(-dx, 0), cols + min(-dx, 0))
+            d = x[yd, xd] - x[ys, xs]
+            d2 = d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2]
+            cnt[yd, xd] += (ok[yd, xd] & ok[ys, xs] & (d2 <= np.float32(r * r))).astype(np.int32)
+    return np.where(ok, cnt, 0)
+
+
+def test_batch_mls_variants_match_oracle(ctx):
+    """Both MLS kernels, pinned in turn on one captured batch: weights
+    evaluated in every pass, and weights kept in LDS between the passes
+    (points with more than 24 in-radius neighbours evaluate the rest
+    again); each against the CPU restatement."""
+    dcfg = scenes.c2_decode_config()
+    # 13 x 13 window; a 2.5 mm radius on this ~1 m deep scene gives about
+    # half the points more than 24 neighbours (the full-size 5 MP scene holds
+    # 40-60 within 1.5 mm)
+    rcfg = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0)
+    p, L, R, lh, rh = _pair_inputs(41, 128, 192)
+    Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+    rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
+    nx, nP, _, _ = oracle.reconstruct(scenes.c2_recon_config(use_mls=0), Lr[0], Rr[0], Lr[1],
+                                      Rr[1], 70, p.Q, p.R1)
+    cnt = _inradius_counts(nx, nP, 6, 2.5)
+    print("in-radius neighbours: max", cnt.max(), "points above 24:", int((cnt > 24).sum()))
+    assert (cnt > 24).sum() > 100 and ((cnt > 3) & (cnt <= 24)).sum() > 100
+    b = zim_amd.Batch([dict(left=L, right=R, decode=dcfg, recon=rcfg, pha_f=70, Q=p.Q, R1=p.R1)],
+                      ctx, graph=True)
+    for variant in (1, 0, 1):
+        b.set_grid("mls_variant", variant)
+        out = b.run()[0]
+        torch.cuda.synchronize()
+        assert np.array_equal(host(out["P"]), rP)
+        assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
     b.close()
 
 

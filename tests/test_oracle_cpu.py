@@ -519,7 +519,8 @@ def test_shared_float_functions_against_libm(which, lo, hi, stride):
     max_ulp, n_libf, n_cr, n_tot = oracle.check_f32(which, lo, hi, stride)
     assert n_tot > 1_000_000
     assert max_ulp <= 1
-    assert n_cr <= 2, (n_cr, n_tot)
+    # exp_f32 is correctly rounded on every float (checked exhaustively)
+    assert n_cr <= (0 if which == "exp" else 2), (n_cr, n_tot)
     assert n_libf < 0.01 * n_tot
 
 

@@ -120,6 +120,7 @@ struct Plan {
   int min_sor_k = 1 << 30;
   std::map<std::string, int> grid;  // per stage; "decode_rows": the decode tile height
   int dec_rows = 4;
+  int mls_variant = kMlsPlain;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -446,7 +447,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
   for (auto& t : pl->mls)
     if (!t.host.empty())
       PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 4), pl->max_mls_hw,
-                                s));
+                                pl->mls_variant, s));
   return PSGC_OK;
 }
 
@@ -461,13 +462,11 @@ static int relayout_decode(Plan* pl, int tile_rows, hipStream_t s) {
 }
 
 // Times each stage of the plan over candidate persistent grid sizes (and, for
-// the decode, tile heights) and keeps the fastest; the plan's inputs of every
-// stage must already be valid.
+// the decode, tile heights; for the MLS, kernel variants) and keeps the
+// fastest; the plan's inputs of every stage must already be valid.
 static int autotune(Plan* pl, hipStream_t s) {
   const int cu = pl->ctx->cu_count > 0 ? pl->ctx->cu_count : 256;
   const int cands[] = {1, 2, 4, 8, 16};
-  constexpr int ncand = sizeof(cands) / sizeof(cands[0]);
-  const int dec_rows[] = {4, 2};
   hipEvent_t a, b;
   PSGC_HIP(hipEventCreate(&a));
   PSGC_HIP(hipEventCreate(&b));
@@ -480,51 +479,62 @@ static int autotune(Plan* pl, hipStream_t s) {
                        {"sor_knn", !pl->sor.host.empty()},   {"mls", !pl->mls.empty()}};
   for (const St& st : stages) {
     if (!st.on) continue;
-    const bool dec = std::string(st.name) == "decode";
+    const std::string n = st.name;
+    // second tuning dimension: decode tile rows / MLS kernel variant
+    std::vector<int> shapes{0};
+    if (n == "decode") shapes = {4, 2};
+    if (n == "mls") {
+      shapes = {kMlsPlain};
+      if (pl->max_mls_hw <= kMlsCachedMaxHalf) shapes.push_back(kMlsCached);
+    }
     float best_ms = 1e30f;
-    int best_g = 0, best_rows = pl->dec_rows;
-    for (int ci = 0; ci < ncand * (dec ? 2 : 1); ++ci) {
-      const int grid = cu * cands[ci % ncand];
-      if (dec && ci % ncand == 0) PSGC_TRY(relayout_decode(pl, dec_rows[ci / ncand], s));
-      float tot = 0;
-      for (int rep = 0; rep < 2; ++rep) {
-        PSGC_HIP(hipEventRecord(a, s));
-        const std::string n = st.name;
-        if (n == "decode")
-          PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
-                                       grid, pl->dec_rows, s));
-        else if (n == "match")
-          PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total, grid,
-                                      pl->max_cols, pl->max_knum, s));
-        else if (n == "median5")
-          PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
-                                       grid, s));
-        else if (n == "repro_gauss")
-          PSGC_TRY(launch_repro_gauss_tiled(pl->rg.dev, (int)pl->rg.host.size(), pl->rg.total,
-                                            grid, pl->max_ghalf, s));
-        else if (n == "sor_knn")
-          PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
-                                        grid, pl->max_sor_half, pl->min_sor_k,
-                                        pl->max_sor_k, s));
-        else
-          PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
-                                    pl->mls[0].total, grid, pl->max_mls_hw, s));
-        PSGC_HIP(hipEventRecord(b, s));
-        PSGC_HIP(hipEventSynchronize(b));
-        float ms = 0;
-        PSGC_HIP(hipEventElapsedTime(&ms, a, b));
-        if (rep > 0) tot += ms;  // first repetition warms caches
-      }
-      if (tot < best_ms) {
-        best_ms = tot;
-        best_g = grid;
-        best_rows = pl->dec_rows;
+    int best_g = 0, best_shape = shapes[0];
+    for (int shape : shapes) {
+      if (n == "decode") PSGC_TRY(relayout_decode(pl, shape, s));
+      for (int pc : cands) {
+        const int grid = cu * pc;
+        float tot = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+          PSGC_HIP(hipEventRecord(a, s));
+          if (n == "decode")
+            PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
+                                         grid, shape, s));
+          else if (n == "match")
+            PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
+                                        grid, pl->max_cols, pl->max_knum, s));
+          else if (n == "median5")
+            PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
+                                         grid, s));
+          else if (n == "repro_gauss")
+            PSGC_TRY(launch_repro_gauss_tiled(pl->rg.dev, (int)pl->rg.host.size(), pl->rg.total,
+                                              grid, pl->max_ghalf, s));
+          else if (n == "sor_knn")
+            PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
+                                          grid, pl->max_sor_half, pl->min_sor_k,
+                                          pl->max_sor_k, s));
+          else
+            PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
+                                      pl->mls[0].total, grid, pl->max_mls_hw, shape, s));
+          PSGC_HIP(hipEventRecord(b, s));
+          PSGC_HIP(hipEventSynchronize(b));
+          float ms = 0;
+          PSGC_HIP(hipEventElapsedTime(&ms, a, b));
+          if (rep > 0) tot += ms;  // first repetition warms caches
+        }
+        if (tot < best_ms) {
+          best_ms = tot;
+          best_g = grid;
+          best_shape = shape;
+        }
       }
     }
-    pl->grid[st.name] = best_g;
-    if (dec) {
-      PSGC_TRY(relayout_decode(pl, best_rows, s));
-      pl->grid["decode_rows"] = best_rows;
+    pl->grid[n] = best_g;
+    if (n == "decode") {
+      PSGC_TRY(relayout_decode(pl, best_shape, s));
+      pl->grid["decode_rows"] = best_shape;
+    } else if (n == "mls") {
+      pl->mls_variant = best_shape;
+      pl->grid["mls_variant"] = best_shape;
     }
   }
   hipEventDestroy(a);
@@ -749,6 +759,10 @@ int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
     *grid = b->plan.dec_rows;
     return PSGC_OK;
   }
+  if (st == "mls_variant") {
+    *grid = b->plan.mls_variant;
+    return PSGC_OK;
+  }
   auto it = b->plan.grid.find(st);
   *grid = it != b->plan.grid.end() ? it->second : 0;
   return PSGC_OK;
@@ -766,6 +780,14 @@ int psgc_batch_set_grid(psgc_batch* b, const char* stage, int value) {
     if (!b->plan.dec.host.empty()) PSGC_TRY(relayout_decode(&b->plan, value, b->ctx->stream));
     else b->plan.dec_rows = value;
     b->plan.grid["decode_rows"] = value;
+  } else if (st == "mls_variant") {
+    if (value != kMlsPlain && value != kMlsCached)
+      PSGC_FAIL(PSGC_E_INVALID, "mls_variant must be 0 or 1 (got %d)", value);
+    if (value == kMlsCached && b->plan.max_mls_hw > kMlsCachedMaxHalf)
+      PSGC_FAIL(PSGC_E_UNSUPPORTED, "mls_variant 1 serves MLS half windows up to %d (batch has %d)",
+                kMlsCachedMaxHalf, b->plan.max_mls_hw);
+    b->plan.mls_variant = value;
+    b->plan.grid["mls_variant"] = value;
   } else {
     bool known = false;
     for (const char* k : kStages) known = known || st == k;
@@ -900,7 +922,8 @@ int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, flo
     c.mls_r2 = r2;
     CloudJob* d;
     PSGC_TRY(upload_job(ctx, c, &d, s));
-    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 4), hw, s));
+    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 4), hw,
+                              kMlsPlain, s));
     // the job buffer is reused by the next iteration
     PSGC_HIP(hipStreamSynchronize(s));
     in = out;

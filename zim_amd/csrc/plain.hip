@@ -435,6 +435,10 @@ __global__ void sor_knn_plain_kernel(const float* __restrict__ src, const uint8_
 __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* __restrict__ P,
                                  float* __restrict__ dst, int rows, int cols, int hw,
                                  float inv2s2, int order, float r2cut, int semantics) {
+  __shared__ double etab[64];  // 2^(j/64) for exp_f32
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  if (tid < 64) etab[tid] = kExp2Tab64[tid];
+  __syncthreads();
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y * blockDim.y + threadIdx.y;
   if (r >= rows || x >= cols) return;
@@ -455,7 +459,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
         const float* pk = src + q * 3;
         const float d2 = dist2(pi[0], pi[1], pi[2], pk);
         if (d2 > r2cut) continue;
-        const float w = exp_f32(-d2 * inv2s2);
+        const float w = exp_f32(-d2 * inv2s2, etab);
         f.cx += w * pk[0];
         f.cy += w * pk[1];
         f.cz += w * pk[2];
@@ -475,7 +479,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
           const float* pk = src + q * 3;
           const float d2 = dist2(pi[0], pi[1], pi[2], pk);
           if (d2 > r2cut) continue;
-          const float w = exp_f32(-d2 * inv2s2);
+          const float w = exp_f32(-d2 * inv2s2, etab);
           const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
           cov[0][0] += w * ex * ex;
           cov[0][1] += w * ex * ey;
@@ -502,7 +506,7 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
             const float* pk = src + q * 3;
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
             if (d2 > r2cut) continue;
-            const float w = exp_f32(-d2 * inv2s2);
+            const float w = exp_f32(-d2 * inv2s2, etab);
             mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
           }
         mls_solve_eval(f, AtWA, AtWf, cur, pi, out);

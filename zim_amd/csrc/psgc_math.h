@@ -108,31 +108,61 @@ PSGC_HD inline double fma_rn(double a, double b, double c) {
 #endif
 }
 
-// e^x: x = k ln 2 + r with |r| <= ln2/2 (Cody and Waite's split of ln 2, k*kLn2Hi
-// exact), e^r by its Taylor series through r^11 (first omitted term below
-// 7e-15 e^r) in Horner form with fused multiply-adds (one rounding each, the
-// same on both sides), scaled by 2^k.
+// 2^(j/64), j = 0..63, each the double nearest to the exact value.
+constexpr double kExp2Tab64[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,
+    0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0,
+    0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0,
+    0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0,
+    0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0,
+    0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0,
+    0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0,
+    0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0,
+    0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0,
+    0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0,
+    0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0,
+    0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0,
+    0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0,
+    0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0,
+    0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0,
+    0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
+    0x1.fa7c1819e90d8p+0
+};
+
+// e^x: x = (64 e + j) ln2/64 + r with |r| <= ln2/128 (Cody and Waite's split
+// of ln 2 / 64; the product of k = 64 e + j with its leading part is exact),
+// e^x = 2^e 2^(j/64) e^r, e^r by its Taylor series through r^5 (first omitted
+// term below 4e-17 e^r) in Horner form with fused multiply-adds (one rounding
+// each, the same on both sides). k = rint(64 x / ln 2) is formed in float (any
+// integer within a few units of 64 x / ln 2 keeps |r| far inside the series'
+// range). The double result is within a few units of 2^-53 of e^x, and over
+// every float its rounding to float equals the correctly rounded e^x.
 // Arguments are clamped to [-104, 89] rather than branched on: e^-104 rounds to
 // float 0 and e^89 to float infinity, the values of the unclamped tails.
-PSGC_HD inline float exp_f32(float xf) {
-  const double x = __builtin_fminf(__builtin_fmaxf(xf, -104.0f), 89.0f);
-  const double k = __builtin_rint(x * kInvLn2);
-  const double r = __builtin_fma(-k, kLn2Lo, __builtin_fma(-k, kLn2Hi, x));
-  double p = 1.0 / 39916800.0;
-  p = fma_rn(p, r, 1.0 / 3628800.0);
-  p = fma_rn(p, r, 1.0 / 362880.0);
-  p = fma_rn(p, r, 1.0 / 40320.0);
-  p = fma_rn(p, r, 1.0 / 5040.0);
-  p = fma_rn(p, r, 1.0 / 720.0);
-  p = fma_rn(p, r, 1.0 / 120.0);
-  p = fma_rn(p, r, 1.0 / 24.0);
+// `tab` holds kExp2Tab64 (on the device a copy in shared memory, so that the
+// per-lane lookup is one LDS read).
+PSGC_HD inline float exp_f32(float xf, const double* tab) {
+  const float xc = __builtin_fminf(__builtin_fmaxf(xf, -104.0f), 89.0f);
+  const float kf = __builtin_rintf(xc * (float)(64.0 * kInvLn2));
+  const int ki = (int)kf;
+  const double t = tab[ki & 63];
+  const double x = xc, k = kf;
+  const double r = __builtin_fma(-k, kLn2Lo / 64.0, __builtin_fma(-k, kLn2Hi / 64.0, x));
+  double p = fma_rn(1.0 / 120.0, r, 1.0 / 24.0);
   p = fma_rn(p, r, 1.0 / 6.0);
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   p = __builtin_fma(p, r, 1.0);
-  const float res = (float)__builtin_ldexp(p, (int)k);
+  const float res = (float)__builtin_ldexp(p * t, ki >> 6);
   return __builtin_isnan(xf) ? xf : res;
 }
+
+PSGC_HD inline float exp_f32(float xf) { return exp_f32(xf, kExp2Tab64); }
 
 // acos(x) = atan2(sqrt((1 - x)(1 + x)), x) in double (the product of two
 // float-derived factors is exact, the square root correctly rounded).

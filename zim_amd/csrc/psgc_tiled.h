@@ -121,8 +121,14 @@ int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int
                              int max_half, hipStream_t s);
 int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_half,
                          int min_k, int max_k, hipStream_t s);
+// MLS kernel variants: kMlsPlain serves half windows up to 8 and evaluates
+// each weight in all three passes; kMlsCached (half windows up to 6) keeps the
+// first 24 in-radius weights of each point in LDS between the passes (the rest
+// are evaluated again). Results are identical.
+enum { kMlsPlain = 0, kMlsCached = 1 };
+constexpr int kMlsCachedMaxHalf = 6;
 int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
-                     hipStream_t s);
+                     int variant, hipStream_t s);
 
 // Shared-memory bytes of the row-match kernel for a given width and order count.
 size_t match_tiled_lds(int cols, int knum);

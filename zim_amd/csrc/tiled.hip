@@ -603,39 +603,65 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
 // --------------------------------------------------------------------- MLS --
 
 constexpr int MH_MAX = 8;
-constexpr int ML = CT + 2 * MH_MAX;
-constexpr int MW = 2 * MH_MAX + 1;
+// Variant with a per-thread weight cache: windows up to 13 x 13, the first
+// MKC in-radius weights of each point kept in LDS between the passes.
+constexpr int MH_CACHED = kMlsCachedMaxHalf;
+// 24 entries keep four workgroups per CU resident (LDS 13 KB tile + 24 KB
+// cache each), the occupancy the plain variant runs at.
+constexpr int MKC = 24;
 
-// Pass 3 of the MLS fit over the in-radius entries (bit masks per window
-// row): the NC-term normal equations in registers, then the solve.
-template <int NC>
-__device__ inline void mls_fit_fixed(const MlsFit& f, const uint32_t* bits, int Wd, int ry0,
-                                     int rx0, const float4 (*T)[ML + 1], const float pi[3],
-                                     float inv2s2, int cur, float out[3]) {
-  float A[NC][NC], bv[NC];
-#pragma unroll
-  for (int a = 0; a < NC; ++a) {
-    bv[a] = 0.f;
-#pragma unroll
-    for (int b = 0; b < NC; ++b) A[a][b] = 0.f;
-  }
-  for (int dy = 0; dy < Wd; ++dy) {
+// Visits the in-radius entries of window rows [dy0, dy1) (bit masks per row)
+// in order, with their weights: from the cache (entries kc, kc + 1, ... of
+// this thread's column) when CACHED, else the pass-1 expression evaluated
+// again (bit-identical either way).
+template <bool CACHED, int LD, typename F>
+__device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0, int rx0,
+                                const float4 (*T)[LD + 1], const float (*Wc)[256], int& kc,
+                                const double* etab, const float pi[3], float inv2s2, F&& visit) {
+  for (int dy = dy0; dy < dy1; ++dy) {
     const int ry = ry0 + dy;
     uint32_t b = bits[dy];
     while (b) {
       const float4 e4 = T[ry][rx0 + __builtin_ctz(b)];
       b &= b - 1;
       const float pk[3] = {e4.x, e4.y, e4.z};
-      const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
-      mls_accum_fixed<NC>(f, pk, w, A, bv);
+      float w;
+      if (CACHED) w = Wc[kc][threadIdx.x];
+      else w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
+      ++kc;
+      visit(pk, w);
     }
   }
-  mls_solve_eval_fixed<NC>(f, A, bv, cur, pi, out);
 }
 
+// All in-radius entries of the window in order: rows [0, dyc) from the cache.
+template <int LD, typename F>
+__device__ inline void mls_entries(const uint32_t* bits, int dyc, int Wd, int ry0, int rx0,
+                                   const float4 (*T)[LD + 1], const float (*Wc)[256],
+                                   const double* etab, const float pi[3], float inv2s2,
+                                   F&& visit) {
+  int kc = 0;
+  if (dyc > 0) mls_rows<true, LD>(bits, 0, dyc, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
+  mls_rows<false, LD>(bits, dyc, Wd, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
+}
+
+__device__ inline int wave_min(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// HM: largest half window served; KC: weight-cache entries per thread (0:
+// every pass evaluates its weights).
+template <int HM, int KC>
 __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                         int njobs, int64_t total) {
-  __shared__ float4 T[ML][ML + 1];
+  constexpr int LD = CT + 2 * HM;
+  constexpr int MWD = 2 * HM + 1;
+  __shared__ float4 T[LD][LD + 1];
+  __shared__ float Wc[KC > 0 ? KC : 1][256];
+  __shared__ double etab[64];  // 2^(j/64) for exp_f32 (published by the first tile's barrier)
+  if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
   int jprev = 0;
@@ -646,91 +672,106 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
     const int hw = cj.mls_hw;
-    load_cloud_tile<ML>(cj.mls_src, cj.mls_P, cj.mls_md, cj.mls_thr, cj.rows, cj.cols, y0, x0,
+    load_cloud_tile<LD>(cj.mls_src, cj.mls_P, cj.mls_md, cj.mls_thr, cj.rows, cj.cols, y0, x0,
                         hw, T);
     __syncthreads();
     const int x = x0 + lx, y = y0 + ly;
-    if (x < cj.cols && y < cj.rows) {
-      const size_t p = (size_t)y * cj.cols + x;
-      const int cy = ly + hw, cx = lx + hw;
-      const float4 q = T[cy][cx];
-      const float pi[3] = {q.x, q.y, q.z};
-      float out[3] = {pi[0], pi[1], pi[2]};
+    const bool inimg = x < cj.cols && y < cj.rows;
+    const size_t p = (size_t)y * cj.cols + x;
+    const int cy = ly + hw, cx = lx + hw;
+    const int Wd = 2 * hw + 1;
+    const float inv2s2 = cj.mls_inv2s2, r2 = cj.mls_r2;
+    const float4 q = T[cy][cx];
+    const float pi[3] = {q.x, q.y, q.z};
+    float out[3] = {pi[0], pi[1], pi[2]};
+    // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
+    // within the cutoff radius (the same uncontracted d^2 test as the CPU
+    // restatement, so the neighbour set and every order fallback decision
+    // are identical). The three accumulation passes keep the CPU filter's
+    // arithmetic (JSLAB_MLS_Filter.cpp:196-340): weighted centroid,
+    // covariance about it, then the normal equations; an ill-conditioned fit
+    // amplifies any reassociation far beyond the 1e-3 mm output tolerance.
+    uint32_t bits[MWD];
+#pragma unroll
+    for (int w = 0; w < MWD; ++w) bits[w] = 0u;
+    MlsFit f;
+    f.cx = f.cy = f.cz = 0.f;
+    float wsum = 0.f;
+    int nv = 0;
+    int rows_fit = 0;  // leading window rows whose weights all sit in the cache
+    bool fit = false;
+    if (inimg) {
       const bool mine = q.w != 0.f;
       if (cj.mls_Pout) st_global(cj.mls_Pout, p, (uint8_t)(mine ? 1 : 0));
       // points removed by SOR enter MLS as zeros (the SOR output), and MLS
       // copies its input for points outside the mask
       if (!mine && cj.mls_md) out[0] = out[1] = out[2] = 0.f;
       if (mine) {
-        // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
-        // within the cutoff radius (the same uncontracted d^2 test as the
-        // CPU restatement, so the neighbour set and every order fallback
-        // decision are identical). The three accumulation passes keep the
-        // CPU filter's arithmetic (JSLAB_MLS_Filter.cpp:196-340): weighted
-        // centroid, covariance about it, then the normal equations; an
-        // ill-conditioned fit amplifies any reassociation far beyond the
-        // 1e-3 mm output tolerance.
-        uint32_t bits[MW];
-#pragma unroll
-        for (int w = 0; w < MW; ++w) bits[w] = 0u;
-        MlsFit f;
-        f.cx = f.cy = f.cz = 0.f;
-        float wsum = 0.f;
-        int nv = 0;
-        const float inv2s2 = cj.mls_inv2s2, r2 = cj.mls_r2;
-        const int Wd = 2 * hw + 1;
         for (int dy = 0; dy < Wd; ++dy) {
           const int ry = cy - hw + dy;
           uint32_t row_bits = 0u;
-          for (int dx = 0; dx < Wd; ++dx) {
+#pragma unroll
+          for (int dx = 0; dx < MWD; ++dx) {
+            if (dx >= Wd) break;
+            // one 16-byte read: point and mask (an invalid entry's distance
+            // is computed and discarded)
             const float4 e4 = T[ry][cx - hw + dx];
-            if (e4.w == 0.f) continue;
             const float pk[3] = {e4.x, e4.y, e4.z};
             const float d2 = dist2(pi[0], pi[1], pi[2], pk);
-            if (d2 > r2) continue;
-            row_bits |= 1u << dx;
-            const float w = exp_f32(-d2 * inv2s2);
-            f.cx += w * pk[0];
-            f.cy += w * pk[1];
-            f.cz += w * pk[2];
-            wsum += w;
-            nv++;
-          }
-          bits[dy] = row_bits;
-        }
-        if (nv >= 3 && !(wsum < 1e-12f)) {
-          const float iw = 1.0f / wsum;
-          f.cx *= iw;
-          f.cy *= iw;
-          f.cz *= iw;
-          float cov[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-          for (int dy = 0; dy < Wd; ++dy) {
-            const int ry = cy - hw + dy;
-            uint32_t b = bits[dy];
-            while (b) {
-              const float4 e4 = T[ry][cx - hw + __builtin_ctz(b)];
-              b &= b - 1;
-              const float pk[3] = {e4.x, e4.y, e4.z};
-              const float w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2);
-              const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-              cov[0][0] += w * ex * ex;
-              cov[0][1] += w * ex * ey;
-              cov[0][2] += w * ex * ez;
-              cov[1][1] += w * ey * ey;
-              cov[1][2] += w * ey * ez;
-              cov[2][2] += w * ez * ez;
+            if ((e4.w != 0.f) & (d2 <= r2)) {
+              row_bits |= 1u << dx;
+              const float w = exp_f32(-d2 * inv2s2, etab);
+              if (KC > 0 && nv < KC) Wc[nv][threadIdx.x] = w;
+              f.cx += w * pk[0];
+              f.cy += w * pk[1];
+              f.cz += w * pk[2];
+              wsum += w;
+              nv++;
             }
           }
-          mls_frame(cov, &f);
-          const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
-          if (cur == 0)
-            mls_plane(f, pi, out);
-          else if (cj.mls_order >= 2)  // uniform per tile
-            mls_fit_fixed<6>(f, bits, Wd, cy - hw, cx - hw, T, pi, inv2s2, cur, out);
-          else
-            mls_fit_fixed<3>(f, bits, Wd, cy - hw, cx - hw, T, pi, inv2s2, cur, out);
+          bits[dy] = row_bits;
+          if (nv <= KC) rows_fit = dy + 1;
         }
+        fit = nv >= 3 && !(wsum < 1e-12f);
       }
+    }
+    // rows served from the cache: the same for every lane of the wave (every
+    // lane is active here), so the passes below do not diverge on it
+    int dyc = 0;
+    if (KC > 0) dyc = __builtin_amdgcn_readfirstlane(wave_min(fit ? rows_fit : MWD));
+    if (fit) {
+      const float iw = 1.0f / wsum;
+      f.cx *= iw;
+      f.cy *= iw;
+      f.cz *= iw;
+      float cov[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+      mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                      [&](const float pk[3], float w) {
+                        const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+                        cov[0][0] += w * ex * ex;
+                        cov[0][1] += w * ex * ey;
+                        cov[0][2] += w * ex * ez;
+                        cov[1][1] += w * ey * ey;
+                        cov[1][2] += w * ey * ez;
+                        cov[2][2] += w * ez * ez;
+                      });
+      mls_frame(cov, &f);
+      const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
+      if (cur == 0) {
+        mls_plane(f, pi, out);
+      } else if (cj.mls_order >= 2) {  // uniform per tile
+        float A[6][6] = {}, bv[6] = {};
+        mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                        [&](const float pk[3], float w) { mls_accum_fixed<6>(f, pk, w, A, bv); });
+        mls_solve_eval_fixed<6>(f, A, bv, cur, pi, out);
+      } else {
+        float A[3][3] = {}, bv[3] = {};
+        mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                        [&](const float pk[3], float w) { mls_accum_fixed<3>(f, pk, w, A, bv); });
+        mls_solve_eval_fixed<3>(f, A, bv, cur, pi, out);
+      }
+    }
+    if (inimg) {
       st_global(cj.mls_dst, p * 3 + 0, out[0]);
       st_global(cj.mls_dst, p * 3 + 1, out[1]);
       st_global(cj.mls_dst, p * 3 + 2, out[2]);
@@ -822,11 +863,18 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
 }
 
 int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
-                     hipStream_t s) {
+                     int variant, hipStream_t s) {
   if (total <= 0) return PSGC_OK;
   if (max_hw > MH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "MLS half window above %d", MH_MAX);
+  if (variant != kMlsPlain && variant != kMlsCached)
+    PSGC_FAIL(PSGC_E_INVALID, "MLS variant %d", variant);
+  if (variant == kMlsCached && max_hw > MH_CACHED)
+    PSGC_FAIL(PSGC_E_UNSUPPORTED, "cached MLS variant: half window above %d", MH_CACHED);
   PSGC_PROF_BEGIN("mls", s);
-  mls_tiled_kernel<<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
+  if (variant == kMlsCached)
+    mls_tiled_kernel<MH_CACHED, MKC><<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
+  else
+    mls_tiled_kernel<MH_MAX, 0><<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
   PSGC_LAUNCHED();
   PSGC_PROF_END("mls", s);
   return PSGC_OK;
