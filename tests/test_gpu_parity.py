@@ -174,8 +174,10 @@ def test_ror(ctx):
         assert np.array_equal(gx, rx)
 
 
-@pytest.mark.parametrize("k,half", [(30, 4), (8, 2), (1, 1)])
+@pytest.mark.parametrize("k,half", [(30, 4), (20, 3), (30, 2), (8, 2), (1, 1)])
 def test_sor(kctx, k, half):
+    """(30, 2): a 5 x 5 window never holds 30 neighbours, every point is
+    dropped."""
     ctx = kctx
     xyz, P = scenes.rng_points(90, 110, seed=6)
     rx, rP, rmd, rst = oracle.sor(xyz, P, k, 1.0, half)

@@ -22,6 +22,7 @@
 #include "psgc_internal.h"
 #include "psgc_math.h"
 #include "psgc_tiled.h"
+#include "psgc_select.h"
 
 namespace psgc {
 namespace {
@@ -526,8 +527,8 @@ __device__ inline void keep_smallest_fixed(float (&best)[K], float v) {
 // right column downwards, the bottom row right to left, the left column
 // upwards. Loop bounds are uniform, so no per-entry index arithmetic.
 template <typename F>
-__device__ inline void for_each_ring_entry(int h, int cy, int cx, F&& visit) {
-  for (int rad = 1; rad <= h; ++rad) {
+__device__ inline void for_each_ring_entry(int h, int cy, int cx, F&& visit, int rad0 = 1) {
+  for (int rad = rad0; rad <= h; ++rad) {
     for (int o = -rad; o < rad; ++o) visit(cy - rad, cx + o);
     for (int o = -rad; o < rad; ++o) visit(cy + o, cx + rad);
     for (int o = rad; o > -rad; --o) visit(cy + rad, cx + o);
@@ -560,15 +561,48 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
       if (q.w != 0.f) {
         const float qx = q.x, qy = q.y, qz = q.z;
         if constexpr (KFIX > 0) {
+          // Rings 1 and 2 (24 entries; invalid ones +inf) sorted by a fixed
+          // network, their KFIX smallest kept; the rest of the window enters
+          // one by one. The selected multiset, and so the ascending list, is
+          // the same in any order. A window of fewer than KFIX entries never
+          // holds KFIX neighbours (and KFIX > 8 means such a window has h < 2).
+          static_assert(KFIX > 8, "the fixed-k selection prefills rings 1-2");
+          constexpr int NP = 24;
+          constexpr int NB = KFIX > NP ? KFIX : NP;
+          const bool enough = (2 * h + 1) * (2 * h + 1) - 1 >= KFIX;
+          float cand[NB];
+#pragma unroll
+          for (int i = 0; i < NB; ++i) cand[i] = __builtin_huge_valf();
+          if (enough) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+              // ring(i) <= 2 <= h: inside the tile's halo
+              const float4 e4 = T[cy + kRing.dy[i]][cx + kRing.dx[i]];
+              const float p3[3] = {e4.x, e4.y, e4.z};
+              const float d2 = dist2(qx, qy, qz, p3);
+              // a NaN distance never enters the list (as with the insertion below)
+              cand[i] = (e4.w != 0.f) & (d2 < __builtin_huge_valf()) ? d2 : __builtin_huge_valf();
+            }
+          }
+          constexpr SortNet net = make_sort_net(NP);
+#pragma unroll
+          for (int c = 0; c < net.n; ++c) {
+            const float a = cand[net.a[c]], b = cand[net.b[c]];
+            cand[net.a[c]] = __builtin_fminf(a, b);
+            cand[net.b[c]] = __builtin_fmaxf(a, b);
+          }
           float best[KFIX];
 #pragma unroll
-          for (int m = 0; m < KFIX; ++m) best[m] = __builtin_huge_valf();
-          for_each_ring_entry(h, cy, cx, [&](int ry, int rx) {
-            const float4 e4 = T[ry][rx];
-            if (e4.w == 0.f) return;
-            const float p3[3] = {e4.x, e4.y, e4.z};
-            keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
-          });
+          for (int i = 0; i < KFIX; ++i) best[i] = cand[i];
+          for_each_ring_entry(
+              enough ? h : 0, cy, cx,
+              [&](int ry, int rx) {
+                const float4 e4 = T[ry][rx];
+                if (e4.w == 0.f) return;
+                const float p3[3] = {e4.x, e4.y, e4.z};
+                keep_smallest_fixed<KFIX>(best, dist2(qx, qy, qz, p3));
+              },
+              3);
           if (best[KFIX - 1] < __builtin_huge_valf()) {
             double s = 0.0;
 #pragma unroll
