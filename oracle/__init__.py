@@ -44,6 +44,7 @@ def lib():
                                            vp, vp, vp, vp]
         L.orc_num_threads.restype = i32
         L.orc_check_f32.argtypes = [i32, flt, flt, i32, vp, vp, vp, vp]
+        L.orc_check_sort_net.argtypes = [i32, vp]
         _lib = L
     return _lib
 
@@ -58,6 +59,14 @@ def check_f32(which: str, lo: float, hi: float, stride: int = 1):
                              *[C.byref(o) for o in out])
     assert rc == 0
     return tuple(o.value for o in out)
+
+
+def check_sort_net(k: int) -> int:
+    """Inputs of zeros and ones (all 2^k) that the device's k-input sorting
+    network leaves unsorted."""
+    out = C.c_int64()
+    assert lib().orc_check_sort_net(k, C.byref(out)) == 0
+    return out.value
 
 
 def num_threads() -> int:

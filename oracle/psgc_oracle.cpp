@@ -43,6 +43,7 @@
 
 #include "psgc.h"
 #include "psgc_math.h"
+#include "psgc_select.h"
 
 namespace {
 
@@ -1279,6 +1280,38 @@ int orc_reconstruct_poly_roi(const psgc_recon_config* cfg, const double* l_phase
 
 int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
+  return PSGC_OK;
+}
+
+// The fixed sorting network of the device's k-nearest selection
+// (psgc::make_sort_net(k), k <= 26) on every input of zeros and ones (2^k of
+// them, 64 at a time as bit lanes): the number left unsorted. By the 0-1
+// principle a comparator network that sorts these sorts every input.
+int orc_check_sort_net(int k, int64_t* unsorted) {
+  if (k < 2 || k > 26 || !unsorted) return PSGC_E_INVALID;
+  const psgc::SortNet net = psgc::make_sort_net(k);
+  const int64_t words = k <= 6 ? 1 : (int64_t)1 << (k - 6);
+  int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+  for (int64_t w = 0; w < words; ++w) {
+    static const uint64_t lane_bits[6] = {0xAAAAAAAAAAAAAAAAull, 0xCCCCCCCCCCCCCCCCull,
+                                          0xF0F0F0F0F0F0F0F0ull, 0xFF00FF00FF00FF00ull,
+                                          0xFFFF0000FFFF0000ull, 0xFFFFFFFF00000000ull};
+    uint64_t v[26];
+    for (int i = 0; i < k; ++i)
+      v[i] = i < 6 ? lane_bits[i] : (((w >> (i - 6)) & 1) ? ~0ull : 0ull);
+    for (int c = 0; c < net.n; ++c) {
+      const uint64_t a = v[net.a[c]], b = v[net.b[c]];
+      v[net.a[c]] = a & b;  // min
+      v[net.b[c]] = a | b;  // max
+    }
+    uint64_t ok = ~0ull;
+    for (int i = 0; i + 1 < k; ++i) ok &= ~(v[i] & ~v[i + 1]);
+    // lanes beyond 2^k (k < 6) repeat earlier inputs; count distinct ones only
+    const uint64_t live = k >= 6 ? ~0ull : ((1ull << (1 << k)) - 1);
+    bad += __builtin_popcountll(~ok & live);
+  }
+  *unsorted = bad;
   return PSGC_OK;
 }
 

@@ -524,6 +524,17 @@ def test_shared_float_functions_against_libm(which, lo, hi, stride):
     assert n_libf < 0.01 * n_tot
 
 
+# ------------------------------------------- k-nearest sorting network --
+
+@pytest.mark.parametrize("k", [2, 5, 8, 16, 20, 24])
+def test_sort_network_sorts_every_input(k):
+    """The SOR kernel sorts the first window rings with psgc::make_sort_net
+    (Batcher's odd-even merge sort truncated to k inputs). Every input of
+    zeros and ones comes out sorted, hence every input (0-1 principle); a
+    dropped or misplaced comparator leaves some unsorted."""
+    assert oracle.check_sort_net(k) == 0
+
+
 # ---------------------------------------- polynomial triangulation (v2) --
 
 def _disparity_field(rows, cols, seed=0, lo=40.0, hi=90.0):
