@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the measurement, copy 1 GiB device-to-device once (a kernel "
                          "of known traffic that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--c5", action="store_true",
+                    help="instead: the merged-cloud filters (16 views of one 2448x2048 scene "
+                         "merged into one unorganized cloud; SOR k=50, MLS order 2 at 1 mm, "
+                         "Gaussian), timed per filter")
+    ap.add_argument("--c5-views", type=int, default=16)
     return ap.parse_args()
 
 
@@ -73,6 +78,9 @@ def main():
     import zim_amd
     from zim_amd import _capi, synth
     from tests import scenes
+
+    if args.c5:
+        return run_c5(args)
 
     dev = torch.device("cuda:0")
     ctx = zim_amd.Context(0)
@@ -258,6 +266,74 @@ def main():
     }
     print(json.dumps(line), flush=True)
     batch.close()
+
+
+def run_c5(args):
+    """Merged-cloud filters: --c5-views views of one 2448x2048 C2 scene (the
+    same surface, independent noise) reconstructed on the GPU without
+    filters, their valid points merged into one unorganized cloud (about
+    3.4 M points per view) resident in HBM; then SOR (k = 50, exact k
+    nearest), MLS (order 2, r = 1.0 mm, sigma 0.5 mm, one iteration) and a
+    Gaussian (sigma 0.5 mm, r = 1.0 mm) over it, each timed with the grid
+    build included. The kNN voxel comes from a timing sweep (the autotuner)."""
+    import torch
+
+    import zim_amd
+    from zim_amd import synth
+    from tests import scenes
+
+    log = lambda m: print(f"[bench-c5] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    dev = torch.device("cuda:0")
+    ctx = zim_amd.Context(0)
+    dcfg = scenes.c2_decode_config()
+    rcfg = scenes.c2_recon_config(use_mls=0, outlier_mode=0)
+    t0 = time.time()
+    parts = []
+    for v in range(args.c5_views):
+        p = synth.render_c2(2000, device=dev, noise_seed=3000 + v)
+        L = zim_amd.decode(dcfg, p.left, ctx)
+        R = zim_amd.decode(dcfg, p.right, ctx)
+        xyz, P, _, _ = zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1, ctx)
+        parts.append(xyz.reshape(-1, 3)[P.reshape(-1) != 0].clone())
+        del p, L, R, xyz, P
+    cloud = torch.cat(parts).contiguous()
+    del parts
+    torch.cuda.synchronize()
+    n = cloud.shape[0]
+    log(f"{args.c5_views} views merged: {n} points in {time.time() - t0:.1f}s")
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best * 1e3
+
+    # voxel autotune for the k-nearest search: one timed SOR per candidate
+    tune = {}
+    for vox in (0.25, 0.35, 0.5, 0.7):
+        tune[vox] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, vox, 4, ctx), reps=1)
+    voxel = min(tune, key=tune.get)
+    log(f"kNN voxel sweep (ms): {tune} -> {voxel}")
+    res = {}
+    res["sor_ms"] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, voxel, 4, ctx))
+    keep, md, st = zim_amd.cloud_sor(cloud, None, 50, 1.0, voxel, 4, ctx)
+    res["mls_ms"] = timed(lambda: zim_amd.cloud_mls(cloud, keep, 0.5, 2, 1.0, 1, 1, 1.0, ctx),
+                          reps=1)
+    res["gauss_ms"] = timed(lambda: zim_amd.cloud_gaussian(cloud, keep, 0.5, 1.0, 1.0, ctx))
+    torch.cuda.synchronize()
+    total = res["sor_ms"] + res["mls_ms"] + res["gauss_ms"]
+    line = {"metric": "merged-cloud filters Mpts/s (SOR k=50 + MLS order 2 r=1 mm + Gaussian)",
+            "value": round(n / (total * 1e-3) / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1,
+            "points": n, "views": args.c5_views, "voxel_knn": voxel, "voxel_sweep_ms": tune,
+            "kept_after_sor": int(keep.sum().item()), "sor_stats": [float(x) for x in st.cpu()],
+            **{k: round(v, 3) for k, v in res.items()}, "dtype": "f32",
+            "data": "synthetic (16 noisy views of one seeded bump height field)"}
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):

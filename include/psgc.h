@@ -328,6 +328,39 @@ PSGC_API int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t*
                     float spatial_sigma, int polynomial_order,
                     float max_radius, int iterations, int semantics, int mem);
 
+/* ---- unorganized clouds (a merged multi-view cloud) -----------------------
+ * No counterpart in the reference, which filters organized clouds only; the
+ * per-point rules are its organized filters' with the window replaced by a
+ * neighbour search. xyz: n points (f32 x, y, z interleaved); valid: one u8
+ * per point or NULL (all). A point takes part when valid and finite; the
+ * others pass through (MLS, Gaussian) or are removed (SOR).
+ * Neighbours come from a voxel grid of edge `voxel` (points sorted by cell;
+ * a workgroup stages the 3x3x3 cells around one cell in LDS); sums run over
+ * the cells dz, dy, dx = -1..1 (z outermost), each cell's points in index
+ * order, so results are deterministic.
+ *
+ * psgc_cloud_sor: mean distance to the k (1..64) nearest other points,
+ *   exact: the search widens ring by ring (R = 1..rmax cells, rmax <= 16)
+ *   until the k-th squared distance is at most 0.99999 (R voxel)^2; a point
+ *   not settled by rmax is not eligible. keep[i] = eligible and mean_dist <=
+ *   mean + std_mul * sd over the eligible points (sample sd; the reduction
+ *   of psgc_filter_sor). mean_dist (required): f64 per point, 0 where not
+ *   eligible; stats: {count, mean, sd, threshold}. keep may be NULL.
+ * psgc_cloud_mls: the MLS fit of psgc_filter_mls (orders, fallback,
+ *   semantics) over the points within max_radius (3 sigma if 0), the point
+ *   itself included; voxel >= that radius. Each iteration rebuilds the grid
+ *   from the moved points.
+ * psgc_cloud_gaussian: weighted mean of the points within radius (itself
+ *   included), weights exp(-d^2 / (2 sigma^2)), FP32; voxel >= radius. */
+PSGC_API int psgc_cloud_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                            int k, float std_mul, float voxel, int rmax, uint8_t* keep,
+                            double* mean_dist, double* stats, int mem);
+PSGC_API int psgc_cloud_mls(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                            float spatial_sigma, int polynomial_order, float max_radius,
+                            int iterations, int semantics, float voxel, float* out, int mem);
+PSGC_API int psgc_cloud_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                                 float sigma, float radius, float voxel, float* out, int mem);
+
 /* Closed-form rectification for an ideal, already rectified pair (zero
  * distortion, R = I, translation along x): fills Q (4x4) and R1 (3x3) as
  * cv::stereoRectify would with flags 0, plus the full-left canvas shift of

@@ -45,6 +45,11 @@ def lib():
         L.orc_num_threads.restype = i32
         L.orc_check_f32.argtypes = [i32, flt, flt, i32, vp, vp, vp, vp]
         L.orc_check_sort_net.argtypes = [i32, vp]
+        i64 = C.c_int64
+        L.orc_cloud_sor.argtypes = [vp, vp, i64, i32, flt, flt, i32, vp, vp, vp]
+        L.orc_cloud_mls.argtypes = [vp, vp, i64, flt, i32, flt, i32, i32, flt, vp]
+        L.orc_cloud_gaussian.argtypes = [vp, vp, i64, flt, flt, flt, vp]
+        L.orc_cloud_brute.argtypes = [vp, vp, i64, i32, flt, vp, vp]
         _lib = L
     return _lib
 
@@ -67,6 +72,53 @@ def check_sort_net(k: int) -> int:
     out = C.c_int64()
     assert lib().orc_check_sort_net(k, C.byref(out)) == 0
     return out.value
+
+
+def _pts(xyz, valid):
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    v = None if valid is None else np.ascontiguousarray(valid, np.uint8).reshape(-1)
+    return xyz, v
+
+
+def cloud_sor(xyz, valid, k, std_mul, voxel, rmax=4):
+    """SOR on an unorganized cloud (N x 3 f32): (keep u8, mean distance f64,
+    stats {count, mean, sd, threshold})."""
+    xyz, v = _pts(xyz, valid)
+    n = len(xyz)
+    keep = np.zeros(n, np.uint8)
+    md = np.zeros(n, np.float64)
+    st = np.zeros(4, np.float64)
+    rc = lib().orc_cloud_sor(_p(xyz), _p(v), n, k, std_mul, voxel, rmax, _p(keep), _p(md), _p(st))
+    assert rc == 0, rc
+    return keep, md, st
+
+
+def cloud_mls(xyz, valid, sigma, order, max_radius, iterations, semantics, voxel):
+    xyz, v = _pts(xyz, valid)
+    out = np.empty_like(xyz)
+    rc = lib().orc_cloud_mls(_p(xyz), _p(v), len(xyz), sigma, order, max_radius, iterations,
+                             semantics, voxel, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def cloud_gaussian(xyz, valid, sigma, radius, voxel):
+    xyz, v = _pts(xyz, valid)
+    out = np.empty_like(xyz)
+    rc = lib().orc_cloud_gaussian(_p(xyz), _p(v), len(xyz), sigma, radius, voxel, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def cloud_brute(xyz, valid, k, radius):
+    """O(N^2) pins: k smallest squared distances to other points (+inf pads)
+    and the number of points within radius (self included)."""
+    xyz, v = _pts(xyz, valid)
+    n = len(xyz)
+    d2 = np.empty((n, k), np.float32)
+    cnt = np.empty(n, np.int32)
+    assert lib().orc_cloud_brute(_p(xyz), _p(v), n, k, radius, _p(d2), _p(cnt)) == 0
+    return d2, cnt
 
 
 def num_threads() -> int:

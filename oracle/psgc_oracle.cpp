@@ -44,6 +44,7 @@
 #include "psgc.h"
 #include "psgc_math.h"
 #include "psgc_select.h"
+#include "psgc_voxel.h"
 
 namespace {
 
@@ -703,6 +704,53 @@ static double tree_sum(std::vector<double> s) {
   return s.empty() ? 0.0 : s[0];
 }
 
+}  // extern "C"
+
+namespace {
+
+// Threshold statistics of SOR over the eligible points (ok): chunks of 1024
+// consecutive points tree-summed, the chunk partials tree-summed over the
+// next power of two; sample standard deviation. st = {count, mean, sd,
+// mean + std_mul sd}.
+void sor_stats_of(const double* md, const uint8_t* ok, size_t npix, float std_mul, double st[4]) {
+  const size_t chunk = 1024;
+  const size_t nchunks = (npix + chunk - 1) / chunk;
+  size_t np2 = 1;
+  while (np2 < nchunks) np2 *= 2;
+  std::vector<double> cs(np2, 0.0), cq(np2, 0.0);
+  std::vector<long long> cn(nchunks, 0);
+  for (size_t c = 0; c < nchunks; ++c) {
+    std::vector<double> s(chunk, 0.0), q(chunk, 0.0);
+    for (size_t t = 0; t < chunk; ++t) {
+      const size_t p = c * chunk + t;
+      if (p < npix && ok[p]) {
+        s[t] = md[p];
+        q[t] = md[p] * md[p];
+        cn[c]++;
+      }
+    }
+    cs[c] = tree_sum(s);
+    cq[c] = tree_sum(q);
+  }
+  long long n = 0;
+  for (auto v : cn) n += v;
+  const double sum = tree_sum(cs), sq = tree_sum(cq);
+  double mean = 0, sd = 0;
+  if (n > 0) mean = sum / (double)n;
+  if (n > 1) {
+    const double var = (sq - sum * sum / (double)n) / ((double)n - 1.0);
+    sd = std::sqrt(std::max(var, 0.0));
+  }
+  st[0] = (double)n;
+  st[1] = mean;
+  st[2] = sd;
+  st[3] = mean + (double)std_mul * sd;
+}
+
+}  // namespace
+
+extern "C" {
+
 // Statistical outlier removal (PCL StatisticalOutlierRemoval definition on
 // an organized window); see psgc.h for the exact rule. Reduction order:
 // chunks of 1024 consecutive pixels are tree-summed, the chunk partials are
@@ -738,35 +786,12 @@ int orc_sor(const float* src, const uint8_t* P, float* dst, uint8_t* dstP,
       ok[p] = 1;
     }
   }
-  const size_t chunk = 1024;
-  const size_t nchunks = (npix + chunk - 1) / chunk;
-  size_t np2 = 1;
-  while (np2 < nchunks) np2 *= 2;
-  std::vector<double> cs(np2, 0.0), cq(np2, 0.0);
-  std::vector<long long> cn(nchunks, 0);
-  for (size_t c = 0; c < nchunks; ++c) {
-    std::vector<double> s(chunk, 0.0), q(chunk, 0.0);
-    for (size_t t = 0; t < chunk; ++t) {
-      const size_t p = c * chunk + t;
-      if (p < npix && ok[p]) {
-        s[t] = md[p];
-        q[t] = md[p] * md[p];
-        cn[c]++;
-      }
-    }
-    cs[c] = tree_sum(s);
-    cq[c] = tree_sum(q);
-  }
-  long long n = 0;
-  for (auto v : cn) n += v;
-  const double sum = tree_sum(cs), sq = tree_sum(cq);
-  double mean = 0, sd = 0;
-  if (n > 0) mean = sum / (double)n;
-  if (n > 1) {
-    const double var = (sq - sum * sum / (double)n) / ((double)n - 1.0);
-    sd = std::sqrt(std::max(var, 0.0));
-  }
-  const double thr = mean + (double)std_mul * sd;
+  double st[4];
+  sor_stats_of(md.data(), ok.data(), npix, std_mul, st);
+  const double thr = st[3];
+  const long long n = (long long)st[0];
+  const double mean = st[1], sd = st[2];
+
   for (size_t p = 0; p < npix; ++p) {
     const bool keep = ok[p] && md[p] <= thr;
     dstP[p] = keep ? 1 : 0;
@@ -883,138 +908,147 @@ bool chol(const float A[6][6], const float b[6], float x[6], int nc) {
   return true;
 }
 
+// One point of the MLS filter. each(visit) calls visit(pk) for every
+// candidate neighbour pk (float[3]) in a fixed order; the radius test, the
+// three accumulation passes and the fit follow JSLAB_MLS_Filter.cpp:196-389.
+// out receives the projected point, or pi where the fit is skipped.
+template <typename Each>
+void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int order,
+               int semantics, float out[3]) {
+  const int minnb[3] = {3, 5, 8};
+  const float pi0 = pi[0], pi1 = pi[1], pi2 = pi[2];
+  out[0] = pi0;
+  out[1] = pi1;
+  out[2] = pi2;
+  float cx = 0, cy = 0, cz = 0, wsum = 0;
+  int nv = 0;
+  each([&](const float* pk) {
+    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > r2cut) return;
+    const float w = psgc::exp_f32(-d2 * inv_2s2);
+    cx += w * pk[0];
+    cy += w * pk[1];
+    cz += w * pk[2];
+    wsum += w;
+    nv++;
+  });
+  if (nv < minnb[0] || wsum < 1e-12f) return;
+  const float iw = 1.0f / wsum;
+  cx *= iw;
+  cy *= iw;
+  cz *= iw;
+  float cov[3][3] = {};
+  each([&](const float* pk) {
+    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > r2cut) return;
+    const float w = psgc::exp_f32(-d2 * inv_2s2);
+    const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
+    cov[0][0] += w * ex * ex;
+    cov[0][1] += w * ex * ey;
+    cov[0][2] += w * ex * ez;
+    cov[1][1] += w * ey * ey;
+    cov[1][2] += w * ey * ez;
+    cov[2][2] += w * ez * ez;
+  });
+  cov[1][0] = cov[0][1];
+  cov[2][0] = cov[0][2];
+  cov[2][1] = cov[1][2];
+  float ev[3], V[3][3];
+  eigen3(cov, ev, V);
+  const float nx = V[0][0], ny = V[0][1], nz = V[0][2];
+  const float t1x = V[1][0], t1y = V[1][1], t1z = V[1][2];
+  const float t2x = V[2][0], t2y = V[2][1], t2z = V[2][2];
+  int cur;
+  if (semantics == PSGC_SEM_CPU) {
+    cur = (nv >= minnb[order]) ? order : 0;
+    if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
+  } else {
+    cur = order;
+    if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
+  }
+  if (cur == 0) {
+    const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
+    out[0] = pi0 - dot * nx;
+    out[1] = pi1 - dot * ny;
+    out[2] = pi2 - dot * nz;
+    return;
+  }
+  const int nc = (cur == 1) ? 3 : 6;
+  float AtWA[6][6] = {}, AtWf[6] = {};
+  each([&](const float* pk) {
+    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > r2cut) return;
+    const float w = psgc::exp_f32(-d2 * inv_2s2);
+    const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
+    const float u = ex * t1x + ey * t1y + ez * t1z;
+    const float v = ex * t2x + ey * t2y + ez * t2z;
+    const float f = ex * nx + ey * ny + ez * nz;
+    float basis[6] = {1.0f, u, v, 0, 0, 0};
+    if (nc == 6) {
+      basis[3] = u * u;
+      basis[4] = u * v;
+      basis[5] = v * v;
+    }
+    for (int a = 0; a < nc; ++a) {
+      AtWf[a] += w * basis[a] * f;
+      for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
+    }
+  });
+  for (int a = 0; a < nc; ++a)
+    for (int b = 0; b < a; ++b) AtWA[a][b] = AtWA[b][a];
+  float co[6] = {};
+  bool solved = chol(AtWA, AtWf, co, nc);
+  if (!solved && cur == 2) {
+    float A1[6][6] = {}, b1[6] = {};
+    for (int a = 0; a < 3; ++a) {
+      b1[a] = AtWf[a];
+      for (int b = 0; b < 3; ++b) A1[a][b] = AtWA[a][b];
+    }
+    solved = chol(A1, b1, co, 3);
+    if (solved) cur = 1;
+  }
+  if (!solved) {
+    const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
+    out[0] = pi0 - dot * nx;
+    out[1] = pi1 - dot * ny;
+    out[2] = pi2 - dot * nz;
+    return;
+  }
+  const float ex0 = pi0 - cx, ey0 = pi1 - cy, ez0 = pi2 - cz;
+  const float u0 = ex0 * t1x + ey0 * t1y + ez0 * t1z;
+  const float v0 = ex0 * t2x + ey0 * t2y + ez0 * t2z;
+  float ff = co[0] + co[1] * u0 + co[2] * v0;
+  if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
+  out[0] = cx + u0 * t1x + v0 * t2x + ff * nx;
+  out[1] = cy + u0 * t1y + v0 * t2y + ff * ny;
+  out[2] = cz + u0 * t1z + v0 * t2z + ff * nz;
+}
+
+// One MLS iteration over an organized cloud: the window of +-hw pixels,
+// row by row (JSLAB_MLS_Filter.cpp:176-389).
 void mls_pass(const float* src, const uint8_t* P, float* dst, int rows, int cols,
               int hw, float sigma, int order_in, float r2cut, int semantics) {
   std::memcpy(dst, src, (size_t)rows * cols * 3 * sizeof(float));
   const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
-  const int minnb[3] = {3, 5, 8};
   const int order = std::max(0, std::min(2, order_in));
 #pragma omp parallel for
   for (int r = 0; r < rows; ++r)
     for (int x = 0; x < cols; ++x) {
       const size_t p = (size_t)r * cols + x;
       if (P[p] == 0) continue;
-      const float pi0 = src[p * 3], pi1 = src[p * 3 + 1], pi2 = src[p * 3 + 2];
       const int r0 = std::max(0, r - hw), r1 = std::min(rows, r + hw + 1);
       const int c0 = std::max(0, x - hw), c1 = std::min(cols, x + hw + 1);
-      float cx = 0, cy = 0, cz = 0, wsum = 0;
-      int nv = 0;
-      for (int i = r0; i < r1; ++i)
-        for (int j = c0; j < c1; ++j) {
-          const size_t q = (size_t)i * cols + j;
-          if (P[q] == 0) continue;
-          const float* pk = src + q * 3;
-          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 > r2cut) continue;
-          const float w = psgc::exp_f32(-d2 * inv_2s2);
-          cx += w * pk[0];
-          cy += w * pk[1];
-          cz += w * pk[2];
-          wsum += w;
-          nv++;
-        }
-      if (nv < minnb[0] || wsum < 1e-12f) continue;
-      const float iw = 1.0f / wsum;
-      cx *= iw;
-      cy *= iw;
-      cz *= iw;
-      float cov[3][3] = {};
-      for (int i = r0; i < r1; ++i)
-        for (int j = c0; j < c1; ++j) {
-          const size_t q = (size_t)i * cols + j;
-          if (P[q] == 0) continue;
-          const float* pk = src + q * 3;
-          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 > r2cut) continue;
-          const float w = psgc::exp_f32(-d2 * inv_2s2);
-          const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
-          cov[0][0] += w * ex * ex;
-          cov[0][1] += w * ex * ey;
-          cov[0][2] += w * ex * ez;
-          cov[1][1] += w * ey * ey;
-          cov[1][2] += w * ey * ez;
-          cov[2][2] += w * ez * ez;
-        }
-      cov[1][0] = cov[0][1];
-      cov[2][0] = cov[0][2];
-      cov[2][1] = cov[1][2];
-      float ev[3], V[3][3];
-      eigen3(cov, ev, V);
-      const float nx = V[0][0], ny = V[0][1], nz = V[0][2];
-      const float t1x = V[1][0], t1y = V[1][1], t1z = V[1][2];
-      const float t2x = V[2][0], t2y = V[2][1], t2z = V[2][2];
-      int cur;
-      if (semantics == PSGC_SEM_CPU) {
-        cur = (nv >= minnb[order]) ? order : 0;
-        if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
-      } else {
-        cur = order;
-        if (cur > 0 && nv < minnb[cur]) cur = (nv >= minnb[1]) ? 1 : 0;
-      }
-      float* out = dst + p * 3;
-      if (cur == 0) {
-        const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
-        out[0] = pi0 - dot * nx;
-        out[1] = pi1 - dot * ny;
-        out[2] = pi2 - dot * nz;
-        continue;
-      }
-      const int nc = (cur == 1) ? 3 : 6;
-      float AtWA[6][6] = {}, AtWf[6] = {};
-      for (int i = r0; i < r1; ++i)
-        for (int j = c0; j < c1; ++j) {
-          const size_t q = (size_t)i * cols + j;
-          if (P[q] == 0) continue;
-          const float* pk = src + q * 3;
-          const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 > r2cut) continue;
-          const float w = psgc::exp_f32(-d2 * inv_2s2);
-          const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
-          const float u = ex * t1x + ey * t1y + ez * t1z;
-          const float v = ex * t2x + ey * t2y + ez * t2z;
-          const float f = ex * nx + ey * ny + ez * nz;
-          float basis[6] = {1.0f, u, v, 0, 0, 0};
-          if (nc == 6) {
-            basis[3] = u * u;
-            basis[4] = u * v;
-            basis[5] = v * v;
+      auto each = [&](auto&& visit) {
+        for (int i = r0; i < r1; ++i)
+          for (int j = c0; j < c1; ++j) {
+            const size_t q = (size_t)i * cols + j;
+            if (P[q] != 0) visit(src + q * 3);
           }
-          for (int a = 0; a < nc; ++a) {
-            AtWf[a] += w * basis[a] * f;
-            for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
-          }
-        }
-      for (int a = 0; a < nc; ++a)
-        for (int b = 0; b < a; ++b) AtWA[a][b] = AtWA[b][a];
-      float co[6] = {};
-      bool solved = chol(AtWA, AtWf, co, nc);
-      if (!solved && cur == 2) {
-        float A1[6][6] = {}, b1[6] = {};
-        for (int a = 0; a < 3; ++a) {
-          b1[a] = AtWf[a];
-          for (int b = 0; b < 3; ++b) A1[a][b] = AtWA[a][b];
-        }
-        solved = chol(A1, b1, co, 3);
-        if (solved) cur = 1;
-      }
-      if (!solved) {
-        const float dot = (pi0 - cx) * nx + (pi1 - cy) * ny + (pi2 - cz) * nz;
-        out[0] = pi0 - dot * nx;
-        out[1] = pi1 - dot * ny;
-        out[2] = pi2 - dot * nz;
-        continue;
-      }
-      const float ex0 = pi0 - cx, ey0 = pi1 - cy, ez0 = pi2 - cz;
-      const float u0 = ex0 * t1x + ey0 * t1y + ez0 * t1z;
-      const float v0 = ex0 * t2x + ey0 * t2y + ez0 * t2z;
-      float ff = co[0] + co[1] * u0 + co[2] * v0;
-      if (cur == 2) ff += co[3] * u0 * u0 + co[4] * u0 * v0 + co[5] * v0 * v0;
-      out[0] = cx + u0 * t1x + v0 * t2x + ff * nx;
-      out[1] = cy + u0 * t1y + v0 * t2y + ff * ny;
-      out[2] = cz + u0 * t1z + v0 * t2z + ff * nz;
+      };
+      mls_point(src + p * 3, each, inv_2s2, r2cut, order, semantics, dst + p * 3);
     }
 }
 
@@ -1280,6 +1314,233 @@ int orc_reconstruct_poly_roi(const psgc_recon_config* cfg, const double* l_phase
 
 int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
+  return PSGC_OK;
+}
+
+// ---- Unorganized clouds: voxel-grid filters -------------------------------
+// Not in the reference (which filters organized clouds only). The grid:
+// psgc_voxel.h's frame over the participating points' bounding box, points
+// ordered by cell key, ties by index (a stable sort). Candidate neighbours
+// of a point: the cells of the (2R+1)^3 block around its cell, in the order
+// dz, dy, dx = -R..R (z outermost), each cell's points in grid order. Sums
+// run in that order on both sides.
+
+}  // extern "C"
+
+namespace {
+
+struct CloudGrid {
+  psgc::VoxelFrame f;
+  std::vector<int64_t> order;    // point index by grid position
+  std::vector<uint64_t> ukey;    // occupied cells, ascending
+  std::vector<int64_t> start, count;
+  std::vector<uint64_t> key_of;  // cell key of each point (participating only)
+};
+
+bool cloud_ok(const float* xyz, const uint8_t* valid, int64_t i) {
+  return (!valid || valid[i] != 0) &&
+         psgc::voxel_point_ok(xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]);
+}
+
+int build_grid(const float* xyz, const uint8_t* valid, int64_t n, float voxel, CloudGrid* g) {
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int64_t np = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!cloud_ok(xyz, valid, i)) continue;
+    ++np;
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = std::min(mn[c], xyz[i * 3 + c]);
+      mx[c] = std::max(mx[c], xyz[i * 3 + c]);
+    }
+  }
+  if (np == 0) {
+    mn[0] = mn[1] = mn[2] = mx[0] = mx[1] = mx[2] = 0.0f;
+  }
+  if (!psgc::voxel_frame(mn, mx, voxel, &g->f)) return PSGC_E_INVALID;
+  g->key_of.assign(n, 0);
+  std::vector<std::pair<uint64_t, int64_t>> kv;
+  kv.reserve(np);
+  for (int64_t i = 0; i < n; ++i) {
+    if (!cloud_ok(xyz, valid, i)) continue;
+    const uint64_t k = psgc::voxel_key_of(g->f, xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]);
+    g->key_of[i] = k;
+    kv.emplace_back(k, i);
+  }
+  std::sort(kv.begin(), kv.end());
+  g->order.resize(kv.size());
+  g->ukey.clear();
+  g->start.clear();
+  g->count.clear();
+  for (size_t j = 0; j < kv.size(); ++j) {
+    g->order[j] = kv[j].second;
+    if (j == 0 || kv[j].first != kv[j - 1].first) {
+      g->ukey.push_back(kv[j].first);
+      g->start.push_back((int64_t)j);
+      g->count.push_back(0);
+    }
+    g->count.back()++;
+  }
+  return PSGC_OK;
+}
+
+// visit(index) for every point of the (2R+1)^3 block of cells around the
+// cell of p (canonical order; cells outside the frame skipped)
+template <typename F>
+void grid_block(const CloudGrid& g, const float* p, int R, F&& visit) {
+  const int64_t ix = psgc::voxel_coord(p[0], g.f.ox, g.f.inv);
+  const int64_t iy = psgc::voxel_coord(p[1], g.f.oy, g.f.inv);
+  const int64_t iz = psgc::voxel_coord(p[2], g.f.oz, g.f.inv);
+  for (int dz = -R; dz <= R; ++dz)
+    for (int dy = -R; dy <= R; ++dy)
+      for (int dx = -R; dx <= R; ++dx) {
+        const int64_t cx = ix + dx, cy = iy + dy, cz = iz + dz;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= g.f.X || cy >= g.f.Y || cz >= g.f.Z) continue;
+        const uint64_t kk = psgc::voxel_key(g.f, cx, cy, cz);
+        auto it = std::lower_bound(g.ukey.begin(), g.ukey.end(), kk);
+        if (it == g.ukey.end() || *it != kk) continue;
+        const size_t c = (size_t)(it - g.ukey.begin());
+        for (int64_t j = g.start[c]; j < g.start[c] + g.count[c]; ++j) visit(g.order[j]);
+      }
+}
+
+inline float d2_of(const float* a, const float* b) {
+  const float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+}  // namespace
+
+extern "C" {
+
+// SOR on an unorganized cloud: mean distance to the k nearest other points
+// (exact: the search widens ring by ring, R = 1..rmax cells, until the k-th
+// distance is below R voxels; a point whose k-th nearest lies farther than
+// rmax voxels is not eligible and is removed), threshold mean + std_mul sd
+// over the eligible points, keep[i] = eligible and md <= threshold.
+// md: 0 where not eligible.
+int orc_cloud_sor(const float* xyz, const uint8_t* valid, int64_t n, int k, float std_mul,
+                  float voxel, int rmax, uint8_t* keep, double* md_out, double* stats) {
+  if (n < 0 || k < 1 || rmax < 1) return PSGC_E_INVALID;
+  CloudGrid g;
+  const int rc = build_grid(xyz, valid, n, voxel, &g);
+  if (rc != PSGC_OK) return rc;
+  std::vector<double> md(n, 0.0);
+  std::vector<uint8_t> ok(n, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    if (!cloud_ok(xyz, valid, i)) continue;
+    std::vector<float> d2s;
+    for (int R = 1; R <= rmax; ++R) {
+      d2s.clear();
+      grid_block(g, xyz + i * 3, R, [&](int64_t j) {
+        if (j != i) d2s.push_back(d2_of(xyz + i * 3, xyz + j * 3));
+      });
+      if ((int)d2s.size() < k) continue;
+      std::nth_element(d2s.begin(), d2s.begin() + (k - 1), d2s.end());
+      const float lim = psgc::cloud_knn_limit(voxel, R);
+      if (!(d2s[k - 1] <= lim)) continue;
+      std::sort(d2s.begin(), d2s.begin() + k);
+      double s = 0;
+      for (int m = 0; m < k; ++m) s += std::sqrt((double)d2s[m]);
+      md[i] = s / k;
+      ok[i] = 1;
+      break;
+    }
+  }
+  double st[4];
+  sor_stats_of(md.data(), ok.data(), (size_t)n, std_mul, st);
+  for (int64_t i = 0; i < n; ++i) {
+    if (keep) keep[i] = (ok[i] && md[i] <= st[3]) ? 1 : 0;
+    if (md_out) md_out[i] = md[i];
+  }
+  if (stats) std::memcpy(stats, st, sizeof(st));
+  return PSGC_OK;
+}
+
+// MLS on an unorganized cloud: the organized filter's fit (orders, fallback,
+// semantics) with the neighbours within `max_radius` (3 sigma if 0) of the
+// point, itself included, found in the 3x3x3 cells around it (voxel >=
+// radius). Each iteration rebuilds the grid from the moved points.
+int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma, int order,
+                  float max_radius, int iterations, int semantics, float voxel, float* out) {
+  const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
+  if (n < 0 || !(sigma > 0.0f) || voxel < r) return PSGC_E_INVALID;
+  const float r2 = r * r;
+  const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+  const int ord = std::max(0, std::min(2, order));
+  std::vector<float> cur(xyz, xyz + n * 3);
+  std::memcpy(out, xyz, (size_t)n * 3 * sizeof(float));
+  for (int it = 0; it < std::max(1, iterations); ++it) {
+    CloudGrid g;
+    const int rc = build_grid(cur.data(), valid, n, voxel, &g);
+    if (rc != PSGC_OK) return rc;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+      if (!cloud_ok(cur.data(), valid, i)) continue;
+      auto each = [&](auto&& visit) {
+        grid_block(g, cur.data() + i * 3, 1, [&](int64_t j) { visit(cur.data() + j * 3); });
+      };
+      mls_point(cur.data() + i * 3, each, inv_2s2, r2, ord, semantics, out + i * 3);
+    }
+    std::memcpy(cur.data(), out, (size_t)n * 3 * sizeof(float));
+  }
+  return PSGC_OK;
+}
+
+// Gaussian smoothing of an unorganized cloud: the weighted mean of the points
+// within `radius` (the point itself included), weights exp(-d^2 / 2 sigma^2).
+int orc_cloud_gaussian(const float* xyz, const uint8_t* valid, int64_t n, float sigma,
+                       float radius, float voxel, float* out) {
+  if (n < 0 || !(sigma > 0.0f) || !(radius > 0.0f) || voxel < radius) return PSGC_E_INVALID;
+  CloudGrid g;
+  const int rc = build_grid(xyz, valid, n, voxel, &g);
+  if (rc != PSGC_OK) return rc;
+  const float r2 = radius * radius;
+  const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+  std::memcpy(out, xyz, (size_t)n * 3 * sizeof(float));
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    if (!cloud_ok(xyz, valid, i)) continue;
+    float sx = 0, sy = 0, sz = 0, sw = 0;
+    grid_block(g, xyz + i * 3, 1, [&](int64_t j) {
+      const float* pk = xyz + j * 3;
+      const float d2 = d2_of(xyz + i * 3, pk);
+      if (d2 > r2) return;
+      const float w = psgc::exp_f32(-d2 * inv_2s2);
+      sx += w * pk[0];
+      sy += w * pk[1];
+      sz += w * pk[2];
+      sw += w;
+    });
+    out[i * 3] = sx / sw;
+    out[i * 3 + 1] = sy / sw;
+    out[i * 3 + 2] = sz / sw;
+  }
+  return PSGC_OK;
+}
+
+// Brute-force pins for small clouds: for every participating point, its k
+// smallest squared distances to the other participating points (ascending;
+// +inf where fewer), and the count of points within `radius` (itself
+// included).
+int orc_cloud_brute(const float* xyz, const uint8_t* valid, int64_t n, int k, float radius,
+                    float* knn_d2, int32_t* within) {
+  const float r2 = radius * radius;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    std::vector<float> d2s;
+    int32_t cnt = 0;
+    if (cloud_ok(xyz, valid, i))
+      for (int64_t j = 0; j < n; ++j) {
+        if (!cloud_ok(xyz, valid, j)) continue;
+        const float d2 = d2_of(xyz + i * 3, xyz + j * 3);
+        if (d2 <= r2) ++cnt;
+        if (j != i) d2s.push_back(d2);
+      }
+    std::sort(d2s.begin(), d2s.end());
+    for (int m = 0; m < k; ++m) knn_d2[i * k + m] = m < (int)d2s.size() ? d2s[m] : INFINITY;
+    within[i] = cnt;
+  }
   return PSGC_OK;
 }
 

@@ -1,0 +1,107 @@
+"""The CPU restatement of the unorganized-cloud filters (voxel grid) pinned
+by brute force and closed forms: exact k nearest distances, the radius
+neighbour sets, and the organized MLS filter on a regular grid where the
+pixel window and the radius pick the same neighbours."""
+import numpy as np
+import pytest
+
+import oracle
+from tests import scenes
+
+
+def _rng_cloud(n, seed, holes=0.0, spread=4.0):
+    g = np.random.default_rng(seed)
+    xy = g.uniform(0, spread, (n, 2))
+    z = 700 + 0.3 * np.sin(xy[:, 0]) * np.cos(0.7 * xy[:, 1]) + g.normal(0, 0.02, n)
+    xyz = np.column_stack([xy, z]).astype(np.float32)
+    valid = (g.random(n) >= holes).astype(np.uint8)
+    return xyz, valid
+
+
+@pytest.mark.parametrize("k,voxel,rmax", [(8, 0.2, 4), (30, 0.15, 4), (30, 0.12, 3), (1, 1.0, 1)])
+def test_cloud_sor_is_exact_knn(k, voxel, rmax):
+    """Eligible points are exactly those whose k-th nearest lies within rmax
+    voxels, and their mean distance is the mean of the brute-force k nearest
+    distances, summed ascending: bit-exact."""
+    xyz, valid = _rng_cloud(1500, 3, holes=0.1)
+    keep, md, st = oracle.cloud_sor(xyz, valid, k, 1.0, voxel, rmax)
+    d2, _ = oracle.cloud_brute(xyz, valid, k, 0.0)
+    lim = np.float32(voxel * rmax) ** 2 * np.float32(0.99999)
+    eligible = (valid != 0) & (d2[:, k - 1] <= lim)
+    assert eligible.sum() > 100
+    ref = np.zeros(len(xyz))
+    for i in np.nonzero(eligible)[0]:
+        s = 0.0
+        for v in d2[i]:
+            s += np.sqrt(np.float64(v))
+        ref[i] = s / k
+    assert np.array_equal(md, ref)
+    mean = ref[eligible].mean()
+    sd = ref[eligible].std(ddof=1)
+    assert st[0] == eligible.sum()
+    assert abs(st[1] - mean) < 1e-12 and abs(st[2] - sd) < 1e-12
+    assert np.array_equal(keep != 0, eligible & (ref <= st[3]))
+
+
+def test_cloud_gaussian_brute_force():
+    xyz, valid = _rng_cloud(1200, 5, holes=0.05)
+    sigma, radius = 0.15, 0.4
+    got = oracle.cloud_gaussian(xyz, valid, sigma, radius, 0.45)
+    x = xyz.astype(np.float64)
+    ok = valid != 0
+    for i in range(0, len(xyz), 7):
+        if not ok[i]:
+            assert np.array_equal(got[i], xyz[i])
+            continue
+        d2 = ((x - x[i]) ** 2).sum(1)
+        m = ok & (d2 <= radius * radius * (1 - 1e-6))
+        w = np.exp(-d2[m] / (2 * sigma * sigma))
+        ref = (w[:, None] * x[m]).sum(0) / w.sum()
+        # FP32 weighted sums of ~30 terms around z = 700 mm: 1e-6 relative
+        assert np.abs(got[i] - ref).max() < 1e-6 * 700
+
+
+def test_cloud_mls_matches_organized_window():
+    """Points on a regular unit grid in x, y with a gentle height: a 5 x 5
+    pixel window with a 2.9 cutoff and a 2.9 radius select the same
+    neighbours, so the organized filter (the reference's) and the cloud
+    filter agree up to the order of summation. Order 1: on a lattice the two
+    in-plane eigenvalues nearly coincide, and an order-2 fit amplifies the
+    reordering far more (the quadric test below pins order 2)."""
+    order = 1
+    rows, cols = 40, 48
+    ii, jj = np.mgrid[0:rows, 0:cols].astype(np.float64)
+    z = 0.4 * np.sin(jj / 6.0) * np.cos(ii / 9.0)
+    org = np.stack([jj, ii, z], -1).astype(np.float32)
+    P = np.ones((rows, cols), np.uint8)
+    ref = oracle.mls(org, P, 2, 1.2, order, 2.9, 1, 1)
+    got = oracle.cloud_mls(org.reshape(-1, 3), None, 1.2, order, 2.9, 1, 1, 3.0).reshape(rows, cols, 3)
+    d = np.abs(got - ref).max(-1)
+    assert d.max() < 5e-4 and np.median(d) < 1e-6
+    assert np.abs(got - org).max() > 1e-3   # the filter moved points
+
+
+def test_cloud_mls_fixed_points_and_iterations():
+    """A plane is a fixed point (order 1); a quadric height field is
+    reproduced by order 2 far better than by order 1 (not exactly: in the
+    tilted local frame the surface picks up cubic terms); invalid points pass
+    through unchanged."""
+    g = np.random.default_rng(2)
+    xy = g.uniform(-3, 3, (2000, 2))
+    plane = np.column_stack([xy, 10 + 0.2 * xy[:, 0] - 0.1 * xy[:, 1]]).astype(np.float32)
+    out = oracle.cloud_mls(plane, None, 0.5, 1, 1.2, 2, 1, 1.2)
+    assert np.abs(out - plane).max() < 2e-5
+    quad = np.column_stack([xy, 0.05 * xy[:, 0] ** 2 - 0.03 * xy[:, 0] * xy[:, 1]
+                            + 0.02 * xy[:, 1] ** 2]).astype(np.float32)
+    valid = (g.random(2000) > 0.1).astype(np.uint8)
+    out = oracle.cloud_mls(quad, valid, 0.5, 2, 1.2, 1, 1, 1.2)
+    out1 = oracle.cloud_mls(quad, valid, 0.5, 1, 1.2, 1, 1, 1.2)
+    e2, e1 = np.abs(out - quad).max(), np.abs(out1 - quad).max()
+    assert e2 < 5e-4 and e2 < 0.2 * e1, (e2, e1)
+    assert np.array_equal(out[valid == 0], quad[valid == 0])
+
+
+def test_cloud_voxel_below_radius_is_rejected():
+    xyz, _ = _rng_cloud(50, 1)
+    with pytest.raises(AssertionError):
+        oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.2)

@@ -1,0 +1,124 @@
+"""Unorganized-cloud filters on the device (voxel grid, LDS neighbour
+tiles) against the CPU restatement: SOR masks and mean distances
+bit-exact, MLS and Gaussian within 1e-3 mm (bit-identical in practice: the
+same candidate order and arithmetic), on random clouds, on a merged cloud
+of several noisy views of one C2 scene, and on edge cases."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import zim_amd
+from tests import scenes
+from tests.test_cloud_cpu import _rng_cloud
+
+pytestmark = pytest.mark.gpu
+
+XYZ_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return zim_amd.Context(0)
+
+
+def _sor_both(ctx, xyz, valid, k, voxel, rmax, dev=True):
+    ref = oracle.cloud_sor(xyz, valid, k, 1.0, voxel, rmax)
+    if dev:
+        got = zim_amd.cloud_sor(torch.from_numpy(xyz).cuda(),
+                                None if valid is None else torch.from_numpy(valid).cuda(),
+                                k, 1.0, voxel, rmax, ctx)
+        torch.cuda.synchronize()
+        got = [g.cpu().numpy() for g in got]
+    else:
+        got = zim_amd.cloud_sor(xyz, valid, k, 1.0, voxel, rmax, ctx)
+    return ref, got
+
+
+def _assert_sor(ref, got):
+    assert np.array_equal(got[1], ref[1]), np.abs(got[1] - ref[1]).max()
+    assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[2], ref[2])
+
+
+@pytest.mark.parametrize("k,voxel,rmax", [(8, 0.2, 4), (30, 0.15, 4), (30, 0.12, 3), (20, 0.3, 1),
+                                          (50, 0.2, 4), (13, 0.25, 2)])
+def test_cloud_sor_random(ctx, k, voxel, rmax):
+    xyz, valid = _rng_cloud(6000, 11, holes=0.1)
+    ref, got = _sor_both(ctx, xyz, valid, k, voxel, rmax)
+    _assert_sor(ref, got)
+    assert ref[0].sum() > 100
+
+
+def test_cloud_sor_host_mode_and_edge_cases(ctx):
+    xyz, valid = _rng_cloud(3000, 12)
+    xyz[5] = np.nan                      # not finite: removed
+    xyz[7] = xyz[8]                      # a duplicate: distance 0
+    xyz[9:20] = xyz[9]                   # a cluster of 11 identical points
+    ref, got = _sor_both(ctx, xyz, None, 10, 0.2, 4, dev=False)
+    _assert_sor(ref, got)
+    assert got[0][5] == 0 and got[1][5] == 0.0
+    # nothing participates / empty cloud
+    ref, got = _sor_both(ctx, xyz[:50], np.zeros(50, np.uint8), 5, 0.2, 4)
+    _assert_sor(ref, got)
+    assert got[2][0] == 0
+    keep, md, st = zim_amd.cloud_sor(np.zeros((0, 3), np.float32), None, 5, 1.0, 0.2, 4, ctx)
+    assert len(keep) == 0 and st[0] == 0
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.cloud_sor(xyz, None, 65, 1.0, 0.2, 4, ctx)
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.cloud_sor(xyz, None, 5, 1.0, 1e-9, 4, ctx)   # 10^27 cells
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("sem", [0, 1])
+def test_cloud_mls_random(ctx, order, sem):
+    xyz, valid = _rng_cloud(5000, 13, holes=0.1)
+    ref = oracle.cloud_mls(xyz, valid, 0.2, order, 0.35, 2, sem, 0.4)
+    got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), torch.from_numpy(valid).cuda(), 0.2,
+                            order, 0.35, 2, sem, 0.4, ctx)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    err = np.abs(got - ref).max()
+    print("cloud mls max |diff| mm", err, "bit-equal:", np.array_equal(got, ref))
+    assert err <= XYZ_TOL
+    assert np.array_equal(got[valid == 0], xyz[valid == 0])
+
+
+def test_cloud_gaussian_random(ctx):
+    xyz, valid = _rng_cloud(5000, 14, holes=0.05)
+    ref = oracle.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.45)
+    got = zim_amd.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.45, ctx)
+    err = np.abs(got - ref).max()
+    print("cloud gaussian max |diff| mm", err, "bit-equal:", np.array_equal(got, ref))
+    assert err <= XYZ_TOL
+
+
+def _merged_views(n_views=3, rows=96, cols=128):
+    """Several views of one C2 scene (the same surface, independent noise),
+    reconstructed on the GPU without filters and merged into one list."""
+    dcfg = scenes.c2_decode_config()
+    rcfg = scenes.c2_recon_config(use_mls=0, outlier_mode=0)
+    ctx = zim_amd.Context(0)
+    pts = []
+    for v in range(n_views):
+        p = zim_amd.synth.render_c2(7, device="cuda", rows=rows, cols=cols, f=2400.0,
+                                     baseline=20.0, noise_seed=100 + v)
+        L = zim_amd.decode(dcfg, p.left, ctx)
+        R = zim_amd.decode(dcfg, p.right, ctx)
+        xyz, P, _, _ = zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1, ctx)
+        torch.cuda.synchronize()
+        x, m = xyz.cpu().numpy().reshape(-1, 3), P.cpu().numpy().reshape(-1) != 0
+        pts.append(x[m])
+    return np.concatenate(pts).astype(np.float32)
+
+
+def test_merged_views_filters(ctx):
+    xyz = _merged_views()
+    assert len(xyz) > 6000
+    ref, got = _sor_both(ctx, xyz, None, 50, 1.0, 4)
+    _assert_sor(ref, got)
+    ref = oracle.cloud_mls(xyz, None, 0.5, 2, 1.0, 1, 1, 1.0)
+    got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), None, 0.5, 2, 1.0, 1, 1, 1.0, ctx)
+    torch.cuda.synchronize()
+    assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL

@@ -212,6 +212,65 @@ def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1,
     return xyz, P
 
 
+def _cloud_in(xyz, valid):
+    dev = _is_dev(xyz)
+    if dev:
+        xyz = xyz.reshape(-1, 3).contiguous()
+        if valid is not None:
+            valid = valid.reshape(-1).to(torch.uint8).contiguous()
+    else:
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        if valid is not None:
+            valid = np.ascontiguousarray(valid, np.uint8).reshape(-1)
+    return dev, xyz, valid
+
+
+def cloud_sor(xyz, valid=None, k=30, std_mul=1.0, voxel=1.0, rmax=4, ctx: Context | None = None):
+    """SOR on an unorganized cloud (N x 3 f32): exact k-nearest mean distances
+    from a voxel grid; returns (keep u8, mean_dist f64, stats f64[4])."""
+    ctx = ctx or default_context()
+    dev, xyz, valid = _cloud_in(xyz, valid)
+    n = xyz.shape[0]
+    device = xyz.device if dev else None
+    keep = _out(dev, (n,), torch.uint8, device)
+    md = _out(dev, (n,), torch.float64, device)
+    st = _out(dev, (4,), torch.float64, device)
+    _capi.check(ctx.lib.psgc_cloud_sor(ctx.h, _capi.ptr(xyz), _capi.ptr(valid), n, int(k),
+                                       float(std_mul), float(voxel), int(rmax), _capi.ptr(keep),
+                                       _capi.ptr(md), _capi.ptr(st), MEM_DEVICE if dev else MEM_HOST))
+    return keep, md, st
+
+
+def cloud_mls(xyz, valid=None, sigma=0.5, order=2, max_radius=1.0, iterations=1,
+              semantics=SEM_GPU, voxel=None, ctx: Context | None = None):
+    """MLS projection of an unorganized cloud over the neighbours within
+    max_radius (voxel defaults to that radius)."""
+    ctx = ctx or default_context()
+    dev, xyz, valid = _cloud_in(xyz, valid)
+    n = xyz.shape[0]
+    r = max_radius if max_radius > 0 else 3.0 * sigma
+    out = _out(dev, (n, 3), torch.float32, xyz.device if dev else None)
+    _capi.check(ctx.lib.psgc_cloud_mls(ctx.h, _capi.ptr(xyz), _capi.ptr(valid), n, float(sigma),
+                                       int(order), float(max_radius), int(iterations),
+                                       int(semantics), float(voxel if voxel else r), _capi.ptr(out),
+                                       MEM_DEVICE if dev else MEM_HOST))
+    return out
+
+
+def cloud_gaussian(xyz, valid=None, sigma=0.5, radius=1.5, voxel=None,
+                   ctx: Context | None = None):
+    """Gaussian-weighted mean of each point's neighbours within radius."""
+    ctx = ctx or default_context()
+    dev, xyz, valid = _cloud_in(xyz, valid)
+    n = xyz.shape[0]
+    out = _out(dev, (n, 3), torch.float32, xyz.device if dev else None)
+    _capi.check(ctx.lib.psgc_cloud_gaussian(ctx.h, _capi.ptr(xyz), _capi.ptr(valid), n,
+                                            float(sigma), float(radius),
+                                            float(voxel if voxel else radius), _capi.ptr(out),
+                                            MEM_DEVICE if dev else MEM_HOST))
+    return out
+
+
 def roi_right_window(lk, rk, pha_f, left_rect, mask, ctx: Context | None = None):
     """prepareRectifiedROI's right search window for left_rect = (x, y, w, h)
     and its mask; returns right_rect as a 4-element int32 array."""

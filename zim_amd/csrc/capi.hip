@@ -714,6 +714,65 @@ int psgc_filter_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t* P, floa
   return finish(ctx, mem);
 }
 
+int psgc_cloud_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
+                   float std_mul, float voxel, int rmax, uint8_t* keep, double* mean_dist,
+                   double* stats, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  if (!xyz || !mean_dist || !stats || n < 0) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const float* sx;
+  const uint8_t* sv = nullptr;
+  uint8_t* dk;
+  double *dmd, *dst;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, (size_t)n * 3, &sx));
+  if (valid) PSGC_TRY(stage_in(ctx, mem, S_P0, valid, (size_t)n, &sv));
+  PSGC_TRY(stage_out(ctx, mem, S_P1, keep, (size_t)n, &dk));
+  PSGC_TRY(stage_out(ctx, mem, S_MEANDIST, mean_dist, (size_t)n, &dmd));
+  PSGC_TRY(stage_out(ctx, mem, S_STATS, stats, 4, &dst));
+  PSGC_TRY(cloud_sor_run(ctx, sx, sv, n, k, std_mul, voxel, rmax, dk, dmd, dst));
+  PSGC_TRY(copy_back(ctx, mem, keep, dk, (size_t)n));
+  PSGC_TRY(copy_back(ctx, mem, mean_dist, dmd, (size_t)n));
+  PSGC_TRY(copy_back(ctx, mem, stats, dst, 4));
+  return finish(ctx, mem);
+}
+
+int psgc_cloud_mls(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                   float spatial_sigma, int polynomial_order, float max_radius, int iterations,
+                   int semantics, float voxel, float* out, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  if (!xyz || !out || n < 0) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (iterations < 1 || iterations > 64) PSGC_FAIL(PSGC_E_INVALID, "iterations=%d", iterations);
+  const float* sx;
+  const uint8_t* sv = nullptr;
+  float* dout;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, (size_t)n * 3, &sx));
+  if (valid) PSGC_TRY(stage_in(ctx, mem, S_P0, valid, (size_t)n, &sv));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, out, (size_t)n * 3, &dout));
+  void* tmp = nullptr;
+  if (iterations > 1) PSGC_TRY(ctx_buf(ctx, S_XYZ2, (size_t)n * 12 + 16, &tmp));
+  PSGC_TRY(cloud_mls_run(ctx, sx, sv, n, spatial_sigma, polynomial_order, max_radius, iterations,
+                         semantics, voxel, dout, (float*)tmp));
+  PSGC_TRY(copy_back(ctx, mem, out, dout, (size_t)n * 3));
+  return finish(ctx, mem);
+}
+
+int psgc_cloud_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                        float sigma, float radius, float voxel, float* out, int mem) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_mem(mem));
+  if (!xyz || !out || n < 0) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  const float* sx;
+  const uint8_t* sv = nullptr;
+  float* dout;
+  PSGC_TRY(stage_in(ctx, mem, S_XYZ0, xyz, (size_t)n * 3, &sx));
+  if (valid) PSGC_TRY(stage_in(ctx, mem, S_P0, valid, (size_t)n, &sv));
+  PSGC_TRY(stage_out(ctx, mem, S_XYZ1, out, (size_t)n * 3, &dout));
+  PSGC_TRY(cloud_gaussian_run(ctx, sx, sv, n, sigma, radius, voxel, dout));
+  PSGC_TRY(copy_back(ctx, mem, out, dout, (size_t)n * 3));
+  return finish(ctx, mem);
+}
+
 int psgc_filter_ror(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst_xyz,
                     uint8_t* dst_P, int rows, int cols, float radius, int min_neighbors,
                     int window_size, int mem) {

@@ -1,0 +1,639 @@
+// Filters for unorganized point clouds (a merged multi-view cloud): SOR on
+// the exact k nearest neighbours, MLS and Gaussian smoothing on the
+// neighbours within a radius. Not in the reference, which filters organized
+// clouds only (JSLAB_MLS_Filter.cpp, JSLAB_PhaseEpipolarReconstructor_V2.cpp);
+// the per-point arithmetic is the organized filters'.
+//
+// Grid: cells of edge `voxel` (psgc_voxel.h); the points are sorted by cell
+// key with a stable radix sort (ties keep index order), the occupied cells
+// found by run-length encoding. A workgroup takes one occupied cell: the
+// 3x3x3 block of cells around it is looked up once (binary search over the
+// sorted keys), its points streamed through LDS in chunks of 1024, and every
+// thread scans each chunk for one point of the centre cell. Candidates come
+// in a fixed order (cells dz, dy, dx = -1..1, points in grid order), the
+// order of the CPU restatement.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "psgc_device.h"
+#include "psgc_internal.h"
+#include "psgc_math.h"
+#include "psgc_tiled.h"
+#include "psgc_voxel.h"
+
+namespace psgc {
+namespace {
+
+constexpr int CB = 256;    // threads per workgroup
+constexpr int CCH = 1024;  // candidates per LDS chunk (16 KB)
+
+struct CellTab {
+  const uint64_t* ukey;
+  const int32_t* start;
+  const int32_t* count;
+  int32_t ncells;
+};
+
+__device__ inline int32_t cell_find(const CellTab& t, uint64_t key) {
+  int32_t lo = 0, hi = t.ncells;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (ld_global(t.ukey, mid) < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < t.ncells && ld_global(t.ukey, lo) == key) ? lo : -1;
+}
+
+__device__ inline bool point_ok(const float* xyz, const uint8_t* valid, int64_t i) {
+  return (!valid || ld_global(valid, i) != 0) &&
+         voxel_point_ok(ld_global(xyz, i * 3), ld_global(xyz, i * 3 + 1), ld_global(xyz, i * 3 + 2));
+}
+
+// ---------------------------------------------------------------- grid ----
+
+// Per-workgroup bounding box and count of the participating points.
+__global__ void __launch_bounds__(CB) cloud_bbox_kernel(const float* __restrict__ xyz,
+                                                        const uint8_t* __restrict__ valid,
+                                                        int64_t n, float* __restrict__ part) {
+  __shared__ float red[7][CB];
+  float v[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.f};
+  for (int64_t i = (int64_t)blockIdx.x * CB + threadIdx.x; i < n; i += (int64_t)gridDim.x * CB) {
+    if (!point_ok(xyz, valid, i)) continue;
+    for (int c = 0; c < 3; ++c) {
+      const float x = ld_global(xyz, i * 3 + c);
+      v[c] = fminf(v[c], x);
+      v[3 + c] = fmaxf(v[3 + c], x);
+    }
+    v[6] += 1.f;
+  }
+  for (int c = 0; c < 7; ++c) red[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int h = CB / 2; h >= 1; h /= 2) {
+    if (threadIdx.x < h)
+      for (int c = 0; c < 7; ++c) {
+        const float a = red[c][threadIdx.x], b = red[c][threadIdx.x + h];
+        red[c][threadIdx.x] = c < 3 ? fminf(a, b) : (c < 6 ? fmaxf(a, b) : a + b);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) st_global(part, (size_t)blockIdx.x * 8 + threadIdx.x, red[threadIdx.x][0]);
+}
+
+// Cell key per point (participating ones; the others get `sentinel`, which
+// sorts after every cell) and the identity permutation.
+__global__ void cloud_keys_kernel(const float* __restrict__ xyz, const uint8_t* __restrict__ valid,
+                                  int64_t n, VoxelFrame f, uint64_t sentinel,
+                                  uint64_t* __restrict__ key, int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = sentinel;
+  if (point_ok(xyz, valid, i))
+    k = voxel_key_of(f, ld_global(xyz, i * 3), ld_global(xyz, i * 3 + 1), ld_global(xyz, i * 3 + 2));
+  st_global(key, i, k);
+  st_global(idx, i, (int32_t)i);
+}
+
+// Sorted points with their index in the fourth lane.
+__global__ void cloud_gather_kernel(const float* __restrict__ xyz, const int32_t* __restrict__ idx,
+                                    int32_t np, float4* __restrict__ pts) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const int32_t i = ld_global(idx, j);
+  pts[j] = make_float4(ld_global(xyz, (int64_t)i * 3), ld_global(xyz, (int64_t)i * 3 + 1),
+                       ld_global(xyz, (int64_t)i * 3 + 2), __int_as_float(i));
+}
+
+// ------------------------------------------------------- block staging ----
+
+struct BlockCells {
+  int32_t st[27], pre[28];
+};
+
+// The 27 cells around `key` (absent ones empty) and the running counts.
+__device__ inline void load_block(const CellTab& t, const VoxelFrame& f, uint64_t key,
+                                  BlockCells* b) {
+  __shared__ int32_t cnt[27];
+  const int d = threadIdx.x;
+  if (d < 27) {
+    const int64_t off = (d % 3 - 1) + f.X * ((d / 3) % 3 - 1 + f.Y * (int64_t)(d / 9 - 1));
+    const int32_t c = cell_find(t, (uint64_t)((int64_t)key + off));
+    b->st[d] = c >= 0 ? ld_global(t.start, c) : 0;
+    cnt[d] = c >= 0 ? ld_global(t.count, c) : 0;
+  }
+  __syncthreads();
+  if (d == 0) {
+    int32_t s = 0;
+    for (int i = 0; i < 27; ++i) {
+      b->pre[i] = s;
+      s += cnt[i];
+    }
+    b->pre[27] = s;
+  }
+  __syncthreads();
+}
+
+// Candidates [cb, cb + len) of the block into tile (cooperative).
+__device__ inline void stage_chunk(const float4* __restrict__ pts, const BlockCells* b, int32_t cb,
+                                   int32_t len, float4* tile) {
+  for (int32_t j = threadIdx.x; j < len; j += CB) {
+    const int32_t g = cb + j;
+    int lo = 0, hi = 26;  // last cell with pre <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b->pre[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    tile[j] = pts[b->st[lo] + (g - b->pre[lo])];
+  }
+}
+
+// ------------------------------------------------------------------ kNN ----
+
+// K > 0: exactly K nearest kept in ascending order (a value replaces the
+// largest and bubbles down); K == 0: k <= 64 at run time.
+template <int K>
+struct Knn {
+  static constexpr int N = K > 0 ? K : 64;
+  float best[N];
+  int cnt;
+  __device__ void init() {
+#pragma unroll
+    for (int i = 0; i < N; ++i) best[i] = INFINITY;
+    cnt = 0;
+  }
+  __device__ void add(float v, int k) {
+    if (K > 0) {
+      ++cnt;
+      if (!(v < best[N - 1])) return;
+      best[N - 1] = v;
+#pragma unroll
+      for (int t = N - 1; t > 0; --t) {
+        const float a = best[t - 1], b = best[t];
+        if (!(b < a)) break;
+        best[t - 1] = b;
+        best[t] = a;
+      }
+    } else {
+      const int have = cnt < k ? cnt : k;
+      ++cnt;
+      if (have == k && !(v < best[k - 1])) return;
+#pragma unroll
+      for (int t = 0; t < N; ++t) {
+        if (t < have) {
+          const float b = best[t];
+          const bool sw = v < b;
+          best[t] = sw ? v : b;
+          v = sw ? b : v;
+        } else if (t == have) {
+          best[t] = v;
+        }
+      }
+    }
+  }
+  // The k-th smallest when there are k candidates, else +inf.
+  __device__ float kth(int k) const {
+    if (cnt < k) return INFINITY;
+    float r = best[0];
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+      if (t == k - 1) r = best[t];
+    return r;
+  }
+  __device__ double mean_dist(int k) const {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < N; ++t)
+      if (t < k) s += sqrt((double)best[t]);
+    return s / k;
+  }
+};
+
+// First pass: the 3x3x3 block. Points it cannot settle (fewer than k
+// candidates, or a k-th distance beyond one voxel) go to the retry list.
+template <int K>
+__global__ void __launch_bounds__(CB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
+                                                       VoxelFrame f, int k, float lim,
+                                                       double* __restrict__ md,
+                                                       int32_t* __restrict__ retry,
+                                                       int32_t* __restrict__ nretry) {
+  __shared__ float4 tile[CCH];
+  __shared__ BlockCells bc;
+  const XcdRange xr = xcd_range(t.ncells);
+  for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
+    load_block(t, f, ld_global(t.ukey, c), &bc);
+    const int32_t M = bc.pre[27];
+    const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
+    for (int32_t qb = 0; qb < cn; qb += CB) {
+      const bool active = qb + (int32_t)threadIdx.x < cn;
+      const int32_t q = c0 + qb + threadIdx.x;
+      const float4 qp = active ? pts[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+      const int qi = __float_as_int(qp.w);
+      Knn<K> sel;
+      sel.init();
+      for (int32_t cb = 0; cb < M; cb += CCH) {
+        const int32_t len = min(CCH, M - cb);
+        __syncthreads();  // the previous chunk's readers are done
+        stage_chunk(pts, &bc, cb, len, tile);
+        __syncthreads();
+        if (active)
+          for (int32_t j = 0; j < len; ++j) {
+            const float4 p = tile[j];
+            if (__float_as_int(p.w) == qi) continue;
+            const float pk[3] = {p.x, p.y, p.z};
+            sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
+          }
+      }
+      if (active) {
+        if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
+        else st_global(retry, atomicAdd(nretry, 1), q);
+      }
+    }
+    __syncthreads();  // block cells are reloaded for the next cell
+  }
+}
+
+// Retry: for each listed point (grid position), the (2R+1)^3 blocks for
+// R = 2..rmax, read straight from the sorted points, until the k-th distance
+// is below R voxels. Unsettled points keep md = -1 (not eligible).
+template <int K>
+__global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __restrict__ pts,
+                                                             CellTab t, VoxelFrame f, int k,
+                                                             int rmax,
+                                                             const int32_t* __restrict__ retry,
+                                                             const int32_t* __restrict__ nretry,
+                                                             double* __restrict__ md) {
+  const int32_t nr = ld_global(nretry, 0);
+  for (int32_t r = blockIdx.x * CB + threadIdx.x; r < nr; r += gridDim.x * CB) {
+    const float4 qp = pts[ld_global(retry, r)];
+    const int qi = __float_as_int(qp.w);
+    const int64_t ix = voxel_coord(qp.x, f.ox, f.inv), iy = voxel_coord(qp.y, f.oy, f.inv),
+                  iz = voxel_coord(qp.z, f.oz, f.inv);
+    for (int R = 2; R <= rmax; ++R) {
+      Knn<K> sel;
+      sel.init();
+      for (int dz = -R; dz <= R; ++dz)
+        for (int dy = -R; dy <= R; ++dy)
+          for (int dx = -R; dx <= R; ++dx) {
+            const int64_t cx = ix + dx, cy = iy + dy, cz = iz + dz;
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= f.X || cy >= f.Y || cz >= f.Z) continue;
+            const int32_t c = cell_find(t, voxel_key(f, cx, cy, cz));
+            if (c < 0) continue;
+            const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
+            for (int32_t j = s0; j < s1; ++j) {
+              const float4 p = pts[j];
+              if (__float_as_int(p.w) == qi) continue;
+              const float pk[3] = {p.x, p.y, p.z};
+              sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
+            }
+          }
+      if (sel.kth(k) <= cloud_knn_limit(f.voxel, R)) {
+        st_global(md, qi, sel.mean_dist(k));
+        break;
+      }
+    }
+  }
+}
+
+__global__ void fill_f64_kernel(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) st_global(p, i, v);
+}
+
+__global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
+                                       const double* __restrict__ stats,
+                                       uint8_t* __restrict__ keep) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = ld_global(md, i);
+  if (keep) st_global(keep, i, (uint8_t)(v >= 0.0 && v <= ld_global(stats, 3) ? 1 : 0));
+  if (v < 0.0) st_global(md, i, 0.0);
+}
+
+// ---------------------------------------------------------- MLS, Gauss ----
+
+template <int NC>
+__global__ void __launch_bounds__(CB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
+                                                       VoxelFrame f, float r2, float inv2s2,
+                                                       int order, int semantics,
+                                                       float* __restrict__ out) {
+  __shared__ float4 tile[CCH];
+  __shared__ BlockCells bc;
+  __shared__ double etab[64];
+  if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
+  const XcdRange xr = xcd_range(t.ncells);
+  for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
+    load_block(t, f, ld_global(t.ukey, c), &bc);  // its barriers also publish etab
+    const int32_t M = bc.pre[27];
+    const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
+    for (int32_t qb = 0; qb < cn; qb += CB) {
+      const bool active = qb + (int32_t)threadIdx.x < cn;
+      const float4 qp = active ? pts[c0 + qb + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float pi[3] = {qp.x, qp.y, qp.z};
+      // every lane takes part in the staging; the passes' arithmetic is the
+      // organized filter's (centroid, covariance, normal equations)
+      auto sweep = [&](bool on, auto&& visit) {
+        for (int32_t cb = 0; cb < M; cb += CCH) {
+          const int32_t len = min(CCH, M - cb);
+          __syncthreads();
+          stage_chunk(pts, &bc, cb, len, tile);
+          __syncthreads();
+          if (on)
+            for (int32_t j = 0; j < len; ++j) {
+              const float4 p = tile[j];
+              const float pk[3] = {p.x, p.y, p.z};
+              const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+              if (d2 <= r2) visit(pk, exp_f32(-d2 * inv2s2, etab));
+            }
+        }
+      };
+      MlsFit fit;
+      fit.cx = fit.cy = fit.cz = 0.f;
+      float wsum = 0.f;
+      int nv = 0;
+      sweep(active, [&](const float pk[3], float w) {
+        fit.cx += w * pk[0];
+        fit.cy += w * pk[1];
+        fit.cz += w * pk[2];
+        wsum += w;
+        nv++;
+      });
+      const bool go = active && nv >= 3 && !(wsum < 1e-12f);
+      float o[3] = {pi[0], pi[1], pi[2]};
+      float cov[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+      if (go) {
+        const float iw = 1.0f / wsum;
+        fit.cx *= iw;
+        fit.cy *= iw;
+        fit.cz *= iw;
+      }
+      sweep(go, [&](const float pk[3], float w) {
+        const float ex = pk[0] - fit.cx, ey = pk[1] - fit.cy, ez = pk[2] - fit.cz;
+        cov[0][0] += w * ex * ex;
+        cov[0][1] += w * ex * ey;
+        cov[0][2] += w * ex * ez;
+        cov[1][1] += w * ey * ey;
+        cov[1][2] += w * ey * ez;
+        cov[2][2] += w * ez * ez;
+      });
+      int cur = 0;
+      if (go) {
+        mls_frame(cov, &fit);
+        cur = mls_order(order, nv, semantics);
+      }
+      float A[NC][NC] = {}, bv[NC] = {};
+      sweep(go && cur > 0, [&](const float pk[3], float w) { mls_accum_fixed<NC>(fit, pk, w, A, bv); });
+      if (go) {
+        if (cur == 0) mls_plane(fit, pi, o);
+        else mls_solve_eval_fixed<NC>(fit, A, bv, cur, pi, o);
+        const int qi = __float_as_int(qp.w);
+        st_global(out, (int64_t)qi * 3 + 0, o[0]);
+        st_global(out, (int64_t)qi * 3 + 1, o[1]);
+        st_global(out, (int64_t)qi * 3 + 2, o[2]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(CB) cloud_gauss_kernel(const float4* __restrict__ pts, CellTab t,
+                                                         VoxelFrame f, float r2, float inv2s2,
+                                                         float* __restrict__ out) {
+  __shared__ float4 tile[CCH];
+  __shared__ BlockCells bc;
+  __shared__ double etab[64];
+  if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
+  const XcdRange xr = xcd_range(t.ncells);
+  for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
+    load_block(t, f, ld_global(t.ukey, c), &bc);
+    const int32_t M = bc.pre[27];
+    const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
+    for (int32_t qb = 0; qb < cn; qb += CB) {
+      const bool active = qb + (int32_t)threadIdx.x < cn;
+      const float4 qp = active ? pts[c0 + qb + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float sx = 0.f, sy = 0.f, sz = 0.f, sw = 0.f;
+      for (int32_t cb = 0; cb < M; cb += CCH) {
+        const int32_t len = min(CCH, M - cb);
+        __syncthreads();
+        stage_chunk(pts, &bc, cb, len, tile);
+        __syncthreads();
+        if (active)
+          for (int32_t j = 0; j < len; ++j) {
+            const float4 p = tile[j];
+            const float pk[3] = {p.x, p.y, p.z};
+            const float d2 = dist2(qp.x, qp.y, qp.z, pk);
+            if (d2 > r2) continue;
+            const float w = exp_f32(-d2 * inv2s2, etab);
+            sx += w * pk[0];
+            sy += w * pk[1];
+            sz += w * pk[2];
+            sw += w;
+          }
+      }
+      if (active) {
+        const int qi = __float_as_int(qp.w);
+        st_global(out, (int64_t)qi * 3 + 0, sx / sw);
+        st_global(out, (int64_t)qi * 3 + 1, sy / sw);
+        st_global(out, (int64_t)qi * 3 + 2, sz / sw);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+unsigned blocks_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
+
+int grid_for_cells(psgc_ctx* ctx, int32_t ncells) {
+  const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
+  const int64_t g = std::min<int64_t>((int64_t)cu * 8, ncells);
+  return (int)std::max<int64_t>(kNumXcd, (g + kNumXcd - 1) / kNumXcd * kNumXcd);
+}
+
+}  // namespace
+
+// The grid of the participating points of xyz (device pointers).
+int cloud_grid_build(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                     float voxel, CloudGridDev* g) {
+  hipStream_t s = ctx->stream;
+  if (n < 0 || n > ((int64_t)1 << 31) - 1) PSGC_FAIL(PSGC_E_INVALID, "cloud of %lld points", (long long)n);
+  if (!(voxel > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "voxel must be > 0");
+  const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + CB - 1) / CB));
+  void *misc, *k0, *k1, *i0, *i1;
+  PSGC_TRY(ctx_buf(ctx, S_CL_MISC, (size_t)nb * 8 * sizeof(float) + 64, &misc));
+  float* part = (float*)misc;
+  if (n > 0) {
+    cloud_bbox_kernel<<<nb, CB, 0, s>>>(xyz, valid, n, part);
+    PSGC_LAUNCHED();
+  }
+  std::vector<float> hp((size_t)nb * 8, 0.f);
+  if (n > 0) PSGC_HIP(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  PSGC_HIP(hipStreamSynchronize(s));
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double np = 0;
+  for (int b = 0; b < (n > 0 ? nb : 0); ++b) {
+    for (int c = 0; c < 3; ++c) {
+      mn[c] = std::min(mn[c], hp[b * 8 + c]);
+      mx[c] = std::max(mx[c], hp[b * 8 + 3 + c]);
+    }
+    np += hp[b * 8 + 6];
+  }
+  if (np == 0) mn[0] = mn[1] = mn[2] = mx[0] = mx[1] = mx[2] = 0.f;
+  if (!voxel_frame(mn, mx, voxel, &g->f))
+    PSGC_FAIL(PSGC_E_INVALID, "voxel %g too small for the cloud's extent", (double)voxel);
+  g->np = (int32_t)np;
+  g->ncells = 0;
+  if (g->np == 0) return PSGC_OK;
+  const int bits = voxel_key_bits(g->f) + 1;  // one more: the sentinel sorts last
+  const uint64_t sentinel = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+  PSGC_TRY(ctx_buf(ctx, S_CL_KEY0, (size_t)n * 8, &k0));
+  PSGC_TRY(ctx_buf(ctx, S_CL_KEY1, (size_t)n * 8, &k1));
+  PSGC_TRY(ctx_buf(ctx, S_CL_IDX0, (size_t)n * 4, &i0));
+  PSGC_TRY(ctx_buf(ctx, S_CL_IDX1, (size_t)n * 4, &i1));
+  cloud_keys_kernel<<<blocks_for(n, 256), 256, 0, s>>>(xyz, valid, n, g->f, sentinel, (uint64_t*)k0,
+                                                       (int32_t*)i0);
+  PSGC_LAUNCHED();
+  // sort, run-length encode, scan; one scratch block sized for the largest
+  size_t tb_sort = 0, tb_rle = 0, tb_scan = 0;
+  PSGC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb_sort, (uint64_t*)k0, (uint64_t*)k1,
+                                              (int32_t*)i0, (int32_t*)i1, (int)n, 0, bits, s));
+  void *uk, *st, *ct;
+  PSGC_TRY(ctx_buf(ctx, S_CL_UKEY, (size_t)g->np * 8, &uk));
+  PSGC_TRY(ctx_buf(ctx, S_CL_START, (size_t)g->np * 4 + 4, &st));
+  PSGC_TRY(ctx_buf(ctx, S_CL_COUNT, (size_t)g->np * 4 + 4, &ct));
+  int32_t* nruns = (int32_t*)((char*)misc + (size_t)nb * 8 * sizeof(float));
+  PSGC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb_rle, (uint64_t*)k1, (uint64_t*)uk,
+                                                 (int32_t*)ct, nruns, g->np, s));
+  PSGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, (int32_t*)ct, (int32_t*)st, g->np, s));
+  void* tmp;
+  size_t tb = std::max(tb_sort, std::max(tb_rle, tb_scan));
+  PSGC_TRY(ctx_buf(ctx, S_CL_TEMP, tb, &tmp));
+  PSGC_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, (uint64_t*)k0, (uint64_t*)k1, (int32_t*)i0,
+                                              (int32_t*)i1, (int)n, 0, bits, s));
+  tb = std::max(tb_sort, std::max(tb_rle, tb_scan));
+  PSGC_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, tb, (uint64_t*)k1, (uint64_t*)uk,
+                                                 (int32_t*)ct, nruns, g->np, s));
+  int32_t hn = 0;
+  PSGC_HIP(hipMemcpyAsync(&hn, nruns, 4, hipMemcpyDeviceToHost, s));
+  PSGC_HIP(hipStreamSynchronize(s));
+  g->ncells = hn;
+  tb = std::max(tb_sort, std::max(tb_rle, tb_scan));
+  PSGC_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, (int32_t*)ct, (int32_t*)st, hn, s));
+  void* pts;
+  PSGC_TRY(ctx_buf(ctx, S_CL_PTS, (size_t)g->np * sizeof(float4), &pts));
+  cloud_gather_kernel<<<blocks_for(g->np, 256), 256, 0, s>>>(xyz, (const int32_t*)i1, g->np,
+                                                             (float4*)pts);
+  PSGC_LAUNCHED();
+  g->pts = (const float4*)pts;
+  g->ukey = (const uint64_t*)uk;
+  g->start = (const int32_t*)st;
+  g->count = (const int32_t*)ct;
+  return PSGC_OK;
+}
+
+int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
+                  float std_mul, float voxel, int rmax, uint8_t* keep, double* md, double* stats) {
+  if (k < 1 || k > 64) PSGC_FAIL(PSGC_E_INVALID, "k=%d (1..64)", k);
+  if (rmax < 1 || rmax > 16) PSGC_FAIL(PSGC_E_INVALID, "rmax=%d (1..16)", rmax);
+  CloudGridDev g;
+  PSGC_TRY(cloud_grid_build(ctx, xyz, valid, n, voxel, &g));
+  hipStream_t s = ctx->stream;
+  if (n > 0) {
+    fill_f64_kernel<<<blocks_for(n, 256), 256, 0, s>>>(md, n, -1.0);
+    PSGC_LAUNCHED();
+  }
+  if (g.ncells > 0) {
+    void* rl;
+    PSGC_TRY(ctx_buf(ctx, S_CL_RETRY, (size_t)g.np * 4 + 64, &rl));
+    int32_t* nretry = (int32_t*)rl;
+    int32_t* retry = nretry + 16;
+    PSGC_HIP(hipMemsetAsync(nretry, 0, 4, s));
+    const CellTab t{g.ukey, g.start, g.count, g.ncells};
+    const int grid = grid_for_cells(ctx, g.ncells);
+    const float lim = cloud_knn_limit(voxel, 1);
+    PSGC_PROF_BEGIN("cloud_knn", s);
+    if (k == 20) cloud_knn_kernel<20><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    else if (k == 30) cloud_knn_kernel<30><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    else if (k == 50) cloud_knn_kernel<50><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    else cloud_knn_kernel<0><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    PSGC_LAUNCHED();
+    PSGC_PROF_END("cloud_knn", s);
+    if (rmax >= 2) {
+      const int rg = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
+      PSGC_PROF_BEGIN("cloud_knn_retry", s);
+      if (k == 20) cloud_knn_retry_kernel<20><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
+      else if (k == 30) cloud_knn_retry_kernel<30><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
+      else if (k == 50) cloud_knn_retry_kernel<50><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
+      else cloud_knn_retry_kernel<0><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
+      PSGC_LAUNCHED();
+      PSGC_PROF_END("cloud_knn_retry", s);
+    }
+  }
+  if (n == 0) {  // no points: zero statistics
+    PSGC_HIP(hipMemsetAsync(stats, 0, 4 * sizeof(double), s));
+    return PSGC_OK;
+  }
+  void* part;
+  PSGC_TRY(ctx_buf(ctx, S_PARTIAL, sor_partial_doubles((size_t)n) * sizeof(double), &part));
+  PSGC_TRY(launch_sor_stats(md, (size_t)n, std_mul, (double*)part, stats, s));
+  {
+    cloud_sor_apply_kernel<<<blocks_for(n, 256), 256, 0, s>>>(md, n, stats, keep);
+    PSGC_LAUNCHED();
+  }
+  return PSGC_OK;
+}
+
+int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, float sigma,
+                  int order, float max_radius, int iterations, int semantics, float voxel,
+                  float* out, float* tmp) {
+  const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
+  if (!(sigma > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma must be > 0");
+  if (voxel < r) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below the radius %g", (double)voxel, (double)r);
+  const float r2 = r * r, inv2s2 = 1.0f / (2.0f * sigma * sigma);
+  const int ord = std::max(0, std::min(2, order));
+  hipStream_t s = ctx->stream;
+  const int nit = std::max(1, iterations);
+  // iteration i reads src and writes dst (out last); every point starts as
+  // its input, so points the fit skips pass through
+  const float* src = xyz;
+  for (int it = 0; it < nit; ++it) {
+    float* dst = ((nit - 1 - it) % 2 == 0) ? out : tmp;
+    CloudGridDev g;
+    PSGC_TRY(cloud_grid_build(ctx, src, valid, n, voxel, &g));
+    if (n > 0) PSGC_HIP(hipMemcpyAsync(dst, src, (size_t)n * 12, hipMemcpyDeviceToDevice, s));
+    if (g.ncells > 0) {
+      const CellTab t{g.ukey, g.start, g.count, g.ncells};
+      const int grid = grid_for_cells(ctx, g.ncells);
+      PSGC_PROF_BEGIN("cloud_mls", s);
+      if (ord >= 2) cloud_mls_kernel<6><<<grid, CB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+      else cloud_mls_kernel<3><<<grid, CB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+      PSGC_LAUNCHED();
+      PSGC_PROF_END("cloud_mls", s);
+    }
+    src = dst;
+  }
+  return PSGC_OK;
+}
+
+int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                       float sigma, float radius, float voxel, float* out) {
+  if (!(sigma > 0.0f) || !(radius > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma and radius must be > 0");
+  if (voxel < radius) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below the radius %g", (double)voxel, (double)radius);
+  CloudGridDev g;
+  PSGC_TRY(cloud_grid_build(ctx, xyz, valid, n, voxel, &g));
+  hipStream_t s = ctx->stream;
+  if (n > 0) PSGC_HIP(hipMemcpyAsync(out, xyz, (size_t)n * 12, hipMemcpyDeviceToDevice, s));
+  if (g.ncells > 0) {
+    const CellTab t{g.ukey, g.start, g.count, g.ncells};
+    PSGC_PROF_BEGIN("cloud_gauss", s);
+    cloud_gauss_kernel<<<grid_for_cells(ctx, g.ncells), CB, 0, s>>>(
+        g.pts, t, g.f, radius * radius, 1.0f / (2.0f * sigma * sigma), out);
+    PSGC_LAUNCHED();
+    PSGC_PROF_END("cloud_gauss", s);
+  }
+  return PSGC_OK;
+}
+
+}  // namespace psgc

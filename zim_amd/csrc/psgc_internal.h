@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "psgc.h"
+#include "psgc_voxel.h"
 
 namespace psgc {
 
@@ -52,6 +53,10 @@ enum Slot : int {
   S_XYZ0, S_XYZ1, S_XYZ2, S_P0, S_P1, S_P2, S_SPAN, S_MEANDIST, S_PARTIAL,
   S_STATS, S_QR, S_TMP0, S_TMP1, S_ADJ0, S_ADJ1, S_ADJ2, S_MAPX, S_MAPY, S_SRC, S_DST, S_CNT,
   S_HDR, S_COEF0, S_COEF1, S_COEF2, S_XYZ3, S_P3, S_MASK,
+  // unorganized-cloud grid: keys and indices (sort in/out), sorted points,
+  // occupied cells, sort/scan scratch, small counters, retry list
+  S_CL_KEY0, S_CL_KEY1, S_CL_IDX0, S_CL_IDX1, S_CL_PTS, S_CL_UKEY, S_CL_START, S_CL_COUNT,
+  S_CL_TEMP, S_CL_MISC, S_CL_RETRY,
   S_NUM_SLOTS
 };
 
@@ -224,5 +229,28 @@ int build_decode_args(const psgc_decode_config* cfg, int n_frames, int sample_ty
 // Host-side helpers.
 void gauss_kernel_f32(int win, float sigma, float* k2 /* win*win */);
 void blur3_weights(double sigma, double* c, double* s);
+
+// ---- unorganized clouds (cloud.hip) --------------------------------------
+
+// Grid of a cloud's participating points: sorted points (x, y, z, index),
+// occupied cells (ascending keys, first position, count).
+struct CloudGridDev {
+  VoxelFrame f;
+  int32_t np = 0, ncells = 0;
+  const float4* pts = nullptr;
+  const uint64_t* ukey = nullptr;
+  const int32_t* start = nullptr;
+  const int32_t* count = nullptr;
+};
+int cloud_grid_build(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                     float voxel, CloudGridDev* g);
+int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
+                  float std_mul, float voxel, int rmax, uint8_t* keep, double* md, double* stats);
+// tmp: n x 3 floats of scratch when iterations > 1
+int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, float sigma,
+                  int order, float max_radius, int iterations, int semantics, float voxel,
+                  float* out, float* tmp);
+int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                       float sigma, float radius, float voxel, float* out);
 
 }  // namespace psgc

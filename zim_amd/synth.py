@@ -221,17 +221,19 @@ class Pair:
 
 
 def render_c2(seed: int, device="cpu", rows=2048, cols=2448, f=2400.0,
-              baseline=150.0, freqs=(70, 64, 59), steps=(4, 4, 4)) -> Pair:
+              baseline=150.0, freqs=(70, 64, 59), steps=(4, 4, 4),
+              noise_seed: int | None = None) -> Pair:
     """Full-size single-pair workload: 2448x2048, bump height field on a plane
     at 500-900 mm, three-frequency heterodyne 4-step fringes (70/64/59 across
     a 1280-px projector), uint16 12-bit, ambient 400, modulation 750,
-    Poisson + N(0, 4 DN) noise."""
+    Poisson + N(0, 4 DN) noise. noise_seed (default: seed) seeds the noise
+    alone, so several views of one surface differ only in their noise."""
     cam = Camera(rows, cols, f, baseline)
     proj = Projector(1280, 1000.0 * f / 2400.0)  # projector/camera magnification of 1000/2400
     z = depth_bumps(cam, seed, device)
-    return _render(cam, proj, z, seed, device, freqs, steps, ambient=400.0,
-                   modulation=750.0, noise=4.0, poisson=True, max_val=4095,
-                   dtype=torch.uint16)
+    return _render(cam, proj, z, seed if noise_seed is None else noise_seed, device, freqs,
+                   steps, ambient=400.0, modulation=750.0, noise=4.0, poisson=True,
+                   max_val=4095, dtype=torch.uint16)
 
 
 def render_c1(seed: int, device="cpu", rows=480, cols=640, with_gray=True) -> Pair:
