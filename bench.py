@@ -315,7 +315,7 @@ def run_c5(args):
 
     # voxel autotune for the k-nearest search: one timed SOR per candidate
     tune = {}
-    for vox in (0.25, 0.35, 0.5, 0.7):
+    for vox in (0.3, 0.4, 0.5, 0.7, 1.0):
         tune[vox] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, vox, 4, ctx), reps=1)
     voxel = min(tune, key=tune.get)
     log(f"kNN voxel sweep (ms): {tune} -> {voxel}")

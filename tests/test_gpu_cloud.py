@@ -41,8 +41,10 @@ def _assert_sor(ref, got):
     assert np.array_equal(got[2], ref[2])
 
 
+# the mean points per occupied cell selects the kNN and Gaussian workgroup
+# width: voxels of 0.2-0.45 give one wave, 0.6 two, 1.0 four on these clouds
 @pytest.mark.parametrize("k,voxel,rmax", [(8, 0.2, 4), (30, 0.15, 4), (30, 0.12, 3), (20, 0.3, 1),
-                                          (50, 0.2, 4), (13, 0.25, 2)])
+                                          (50, 0.2, 4), (13, 0.25, 2), (30, 0.6, 2), (50, 1.0, 2)])
 def test_cloud_sor_random(ctx, k, voxel, rmax):
     xyz, valid = _rng_cloud(6000, 11, holes=0.1)
     ref, got = _sor_both(ctx, xyz, valid, k, voxel, rmax)
@@ -70,13 +72,13 @@ def test_cloud_sor_host_mode_and_edge_cases(ctx):
         zim_amd.cloud_sor(xyz, None, 5, 1.0, 1e-9, 4, ctx)   # 10^27 cells
 
 
-@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("order,voxel", [(0, 0.4), (1, 0.4), (2, 0.4), (2, 0.6), (2, 1.0)])
 @pytest.mark.parametrize("sem", [0, 1])
-def test_cloud_mls_random(ctx, order, sem):
+def test_cloud_mls_random(ctx, order, voxel, sem):
     xyz, valid = _rng_cloud(5000, 13, holes=0.1)
-    ref = oracle.cloud_mls(xyz, valid, 0.2, order, 0.35, 2, sem, 0.4)
+    ref = oracle.cloud_mls(xyz, valid, 0.2, order, 0.35, 2, sem, voxel)
     got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), torch.from_numpy(valid).cuda(), 0.2,
-                            order, 0.35, 2, sem, 0.4, ctx)
+                            order, 0.35, 2, sem, voxel, ctx)
     torch.cuda.synchronize()
     got = got.cpu().numpy()
     err = np.abs(got - ref).max()
@@ -85,10 +87,11 @@ def test_cloud_mls_random(ctx, order, sem):
     assert np.array_equal(got[valid == 0], xyz[valid == 0])
 
 
-def test_cloud_gaussian_random(ctx):
+@pytest.mark.parametrize("voxel", [0.45, 0.6, 1.0])
+def test_cloud_gaussian_random(ctx, voxel):
     xyz, valid = _rng_cloud(5000, 14, holes=0.05)
-    ref = oracle.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.45)
-    got = zim_amd.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.45, ctx)
+    ref = oracle.cloud_gaussian(xyz, valid, 0.15, 0.4, voxel)
+    got = zim_amd.cloud_gaussian(xyz, valid, 0.15, 0.4, voxel, ctx)
     err = np.abs(got - ref).max()
     print("cloud gaussian max |diff| mm", err, "bit-equal:", np.array_equal(got, ref))
     assert err <= XYZ_TOL

@@ -8,8 +8,8 @@
 // key with a stable radix sort (ties keep index order), the occupied cells
 // found by run-length encoding. A workgroup takes one occupied cell: the
 // 3x3x3 block of cells around it is looked up once (binary search over the
-// sorted keys), its points streamed through LDS in chunks of 1024, and every
-// thread scans each chunk for one point of the centre cell. Candidates come
+// sorted keys), its points streamed through LDS in chunks (512 or 1024),
+// and every thread scans each chunk for one point of the centre cell. Candidates come
 // in a fixed order (cells dz, dy, dx = -1..1, points in grid order), the
 // order of the CPU restatement.
 
@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "psgc_device.h"
 #include "psgc_internal.h"
@@ -29,7 +30,6 @@ namespace psgc {
 namespace {
 
 constexpr int CB = 256;    // threads per workgroup
-constexpr int CCH = 1024;  // candidates per LDS chunk (16 KB)
 
 struct CellTab {
   const uint64_t* ukey;
@@ -113,19 +113,28 @@ struct BlockCells {
   int32_t st[27], pre[28];
 };
 
-// The 27 cells around `key` (absent ones empty) and the running counts.
+// The 27 cells ordered by distance class from the centre: itself, the 6
+// face neighbours, the 12 edge neighbours, the 8 corners (index dz*9 + dy*3
+// + dx with offsets shifted by one).
+__constant__ int8_t kNearFirst[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11,
+                                      15, 17, 19, 21, 23, 25, 0, 2, 6, 8, 18, 20, 24, 26};
+
+// The 27 cells around `key` (absent ones empty) and the running counts; in
+// the canonical order (dz, dy, dx = -1..1), or nearest class first.
+template <bool NEAR_FIRST = false>
 __device__ inline void load_block(const CellTab& t, const VoxelFrame& f, uint64_t key,
                                   BlockCells* b) {
   __shared__ int32_t cnt[27];
-  const int d = threadIdx.x;
-  if (d < 27) {
+  if (threadIdx.x < 27) {
+    const int d = NEAR_FIRST ? kNearFirst[threadIdx.x] : (int)threadIdx.x;
+    const int slot = threadIdx.x;
     const int64_t off = (d % 3 - 1) + f.X * ((d / 3) % 3 - 1 + f.Y * (int64_t)(d / 9 - 1));
     const int32_t c = cell_find(t, (uint64_t)((int64_t)key + off));
-    b->st[d] = c >= 0 ? ld_global(t.start, c) : 0;
-    cnt[d] = c >= 0 ? ld_global(t.count, c) : 0;
+    b->st[slot] = c >= 0 ? ld_global(t.start, c) : 0;
+    cnt[slot] = c >= 0 ? ld_global(t.count, c) : 0;
   }
   __syncthreads();
-  if (d == 0) {
+  if (threadIdx.x == 0) {
     int32_t s = 0;
     for (int i = 0; i < 27; ++i) {
       b->pre[i] = s;
@@ -136,10 +145,11 @@ __device__ inline void load_block(const CellTab& t, const VoxelFrame& f, uint64_
   __syncthreads();
 }
 
-// Candidates [cb, cb + len) of the block into tile (cooperative).
+// Candidates [cb, cb + len) of the block into tile (cooperative, TB threads).
+template <int TB>
 __device__ inline void stage_chunk(const float4* __restrict__ pts, const BlockCells* b, int32_t cb,
                                    int32_t len, float4* tile) {
-  for (int32_t j = threadIdx.x; j < len; j += CB) {
+  for (int32_t j = threadIdx.x; j < len; j += TB) {
     const int32_t g = cb + j;
     int lo = 0, hi = 26;  // last cell with pre <= g
     while (lo < hi) {
@@ -214,30 +224,37 @@ struct Knn {
 
 // First pass: the 3x3x3 block. Points it cannot settle (fewer than k
 // candidates, or a k-th distance beyond one voxel) go to the retry list.
-template <int K>
-__global__ void __launch_bounds__(CB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
+// TB threads per workgroup (one cell at a time), chunks of chunk_of(TB).
+constexpr int chunk_of(int tb) { return tb >= 256 ? 1024 : 512; }
+
+template <int K, int TB>
+__global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, int k, float lim,
                                                        double* __restrict__ md,
                                                        int32_t* __restrict__ retry,
                                                        int32_t* __restrict__ nretry) {
-  __shared__ float4 tile[CCH];
+  constexpr int CH = chunk_of(TB);
+  __shared__ float4 tile[CH];
   __shared__ BlockCells bc;
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
-    load_block(t, f, ld_global(t.ukey, c), &bc);
+    // nearest cells first: the k-th distance falls early and later
+    // candidates rarely enter the list (the selected multiset does not
+    // depend on the order)
+    load_block<true>(t, f, ld_global(t.ukey, c), &bc);
     const int32_t M = bc.pre[27];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
-    for (int32_t qb = 0; qb < cn; qb += CB) {
+    for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
       const int32_t q = c0 + qb + threadIdx.x;
       const float4 qp = active ? pts[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
       const int qi = __float_as_int(qp.w);
       Knn<K> sel;
       sel.init();
-      for (int32_t cb = 0; cb < M; cb += CCH) {
-        const int32_t len = min(CCH, M - cb);
+      for (int32_t cb = 0; cb < M; cb += CH) {
+        const int32_t len = min(CH, M - cb);
         __syncthreads();  // the previous chunk's readers are done
-        stage_chunk(pts, &bc, cb, len, tile);
+        stage_chunk<TB>(pts, &bc, cb, len, tile);
         __syncthreads();
         if (active)
           for (int32_t j = 0; j < len; ++j) {
@@ -315,12 +332,13 @@ __global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
 
 // ---------------------------------------------------------- MLS, Gauss ----
 
-template <int NC>
-__global__ void __launch_bounds__(CB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
+template <int NC, int TB>
+__global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, float r2, float inv2s2,
                                                        int order, int semantics,
                                                        float* __restrict__ out) {
-  __shared__ float4 tile[CCH];
+  constexpr int CH = chunk_of(TB);
+  __shared__ float4 tile[CH];
   __shared__ BlockCells bc;
   __shared__ double etab[64];
   if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
@@ -329,17 +347,17 @@ __global__ void __launch_bounds__(CB) cloud_mls_kernel(const float4* __restrict_
     load_block(t, f, ld_global(t.ukey, c), &bc);  // its barriers also publish etab
     const int32_t M = bc.pre[27];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
-    for (int32_t qb = 0; qb < cn; qb += CB) {
+    for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
       const float4 qp = active ? pts[c0 + qb + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
       const float pi[3] = {qp.x, qp.y, qp.z};
       // every lane takes part in the staging; the passes' arithmetic is the
       // organized filter's (centroid, covariance, normal equations)
       auto sweep = [&](bool on, auto&& visit) {
-        for (int32_t cb = 0; cb < M; cb += CCH) {
-          const int32_t len = min(CCH, M - cb);
+        for (int32_t cb = 0; cb < M; cb += CH) {
+          const int32_t len = min(CH, M - cb);
           __syncthreads();
-          stage_chunk(pts, &bc, cb, len, tile);
+          stage_chunk<TB>(pts, &bc, cb, len, tile);
           __syncthreads();
           if (on)
             for (int32_t j = 0; j < len; ++j) {
@@ -399,10 +417,12 @@ __global__ void __launch_bounds__(CB) cloud_mls_kernel(const float4* __restrict_
   }
 }
 
-__global__ void __launch_bounds__(CB) cloud_gauss_kernel(const float4* __restrict__ pts, CellTab t,
+template <int TB>
+__global__ void __launch_bounds__(TB) cloud_gauss_kernel(const float4* __restrict__ pts, CellTab t,
                                                          VoxelFrame f, float r2, float inv2s2,
                                                          float* __restrict__ out) {
-  __shared__ float4 tile[CCH];
+  constexpr int CH = chunk_of(TB);
+  __shared__ float4 tile[CH];
   __shared__ BlockCells bc;
   __shared__ double etab[64];
   if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
@@ -411,14 +431,14 @@ __global__ void __launch_bounds__(CB) cloud_gauss_kernel(const float4* __restric
     load_block(t, f, ld_global(t.ukey, c), &bc);
     const int32_t M = bc.pre[27];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
-    for (int32_t qb = 0; qb < cn; qb += CB) {
+    for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
       const float4 qp = active ? pts[c0 + qb + threadIdx.x] : make_float4(0.f, 0.f, 0.f, 0.f);
       float sx = 0.f, sy = 0.f, sz = 0.f, sw = 0.f;
-      for (int32_t cb = 0; cb < M; cb += CCH) {
-        const int32_t len = min(CCH, M - cb);
+      for (int32_t cb = 0; cb < M; cb += CH) {
+        const int32_t len = min(CH, M - cb);
         __syncthreads();
-        stage_chunk(pts, &bc, cb, len, tile);
+        stage_chunk<TB>(pts, &bc, cb, len, tile);
         __syncthreads();
         if (active)
           for (int32_t j = 0; j < len; ++j) {
@@ -444,11 +464,29 @@ __global__ void __launch_bounds__(CB) cloud_gauss_kernel(const float4* __restric
   }
 }
 
+// Workgroup width per cell from the mean points per occupied cell: one wave
+// while cells hold tens of points (wider workgroups would leave most lanes
+// idle: 2.5x slower kNN at 256 threads), wider as they fill (fewer passes of
+// the block through LDS per cell).
+int width_for(const CloudGridDev& g) {
+  const double avg = g.ncells > 0 ? (double)g.np / g.ncells : 0.0;
+  return avg > 160.0 ? 256 : (avg > 80.0 ? 128 : 64);
+}
+
+// Calls f with the workgroup width as a compile-time constant.
+template <typename F>
+void with_width(int tb, F&& f) {
+  if (tb == 256) f(std::integral_constant<int, 256>{});
+  else if (tb == 128) f(std::integral_constant<int, 128>{});
+  else f(std::integral_constant<int, 64>{});
+}
+
 unsigned blocks_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
 
-int grid_for_cells(psgc_ctx* ctx, int32_t ncells) {
+// Persistent grid: up to 2048 threads' worth of workgroups per CU.
+int grid_for_cells(psgc_ctx* ctx, int32_t ncells, int tb) {
   const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
-  const int64_t g = std::min<int64_t>((int64_t)cu * 8, ncells);
+  const int64_t g = std::min<int64_t>((int64_t)cu * (2048 / tb), ncells);
   return (int)std::max<int64_t>(kNumXcd, (g + kNumXcd - 1) / kNumXcd * kNumXcd);
 }
 
@@ -551,13 +589,17 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     int32_t* retry = nretry + 16;
     PSGC_HIP(hipMemsetAsync(nretry, 0, 4, s));
     const CellTab t{g.ukey, g.start, g.count, g.ncells};
-    const int grid = grid_for_cells(ctx, g.ncells);
+    const int tbv = width_for(g);
+    const int grid = grid_for_cells(ctx, g.ncells, tbv);
     const float lim = cloud_knn_limit(voxel, 1);
     PSGC_PROF_BEGIN("cloud_knn", s);
-    if (k == 20) cloud_knn_kernel<20><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-    else if (k == 30) cloud_knn_kernel<30><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-    else if (k == 50) cloud_knn_kernel<50><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-    else cloud_knn_kernel<0><<<grid, CB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    with_width(tbv, [&](auto w) {
+      constexpr int TB = decltype(w)::value;
+      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+    });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_knn", s);
     if (rmax >= 2) {
@@ -605,10 +647,16 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     if (n > 0) PSGC_HIP(hipMemcpyAsync(dst, src, (size_t)n * 12, hipMemcpyDeviceToDevice, s));
     if (g.ncells > 0) {
       const CellTab t{g.ukey, g.start, g.count, g.ncells};
-      const int grid = grid_for_cells(ctx, g.ncells);
+      // one wave per cell measures fastest for MLS at every density tried
+      // (its per-candidate work dwarfs the repeated staging)
+      const int tbv = 64;
+      const int grid = grid_for_cells(ctx, g.ncells, tbv);
       PSGC_PROF_BEGIN("cloud_mls", s);
-      if (ord >= 2) cloud_mls_kernel<6><<<grid, CB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-      else cloud_mls_kernel<3><<<grid, CB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+      with_width(tbv, [&](auto w) {
+        constexpr int TB = decltype(w)::value;
+        if (ord >= 2) cloud_mls_kernel<6, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+        else cloud_mls_kernel<3, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+      });
       PSGC_LAUNCHED();
       PSGC_PROF_END("cloud_mls", s);
     }
@@ -628,8 +676,12 @@ int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, in
   if (g.ncells > 0) {
     const CellTab t{g.ukey, g.start, g.count, g.ncells};
     PSGC_PROF_BEGIN("cloud_gauss", s);
-    cloud_gauss_kernel<<<grid_for_cells(ctx, g.ncells), CB, 0, s>>>(
-        g.pts, t, g.f, radius * radius, 1.0f / (2.0f * sigma * sigma), out);
+    const int tbv = std::max(128, width_for(g));
+    with_width(tbv, [&](auto w) {
+      constexpr int TB = decltype(w)::value;
+      cloud_gauss_kernel<TB><<<grid_for_cells(ctx, g.ncells, TB), TB, 0, s>>>(
+          g.pts, t, g.f, radius * radius, 1.0f / (2.0f * sigma * sigma), out);
+    });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_gauss", s);
   }
