@@ -479,6 +479,28 @@ PSGC_API int psgc_adjoint_backward(psgc_ctx* ctx, const double theta[9],
                                    int cols, const double* grad_xyz, double* grad_theta,
                                    double* grad_phL, double* grad_phR, int mem);
 
+/* Many views (stereo pairs) in one launch each way, e.g. per-view calibration
+ * refinement of a multi-view scan. Per view, the arguments of
+ * psgc_adjoint_forward / psgc_adjoint_backward; device pointers only (on the
+ * context's stream; the call returns when the launch is done). grad_theta:
+ * 9 doubles per view. No counterpart in the reference. */
+typedef struct psgc_adjoint_view {
+    double theta[9];
+    double nominal[5];
+    const double* phL;
+    const double* phR;
+    const int32_t* match_col;
+    int32_t rows, cols;
+    double* xyz;              /* forward: rows*cols*3                     */
+    uint8_t* valid;           /* forward, optional                        */
+    const double* grad_xyz;   /* backward input                           */
+    double* grad_theta;       /* backward: 9                              */
+    double* grad_phL;         /* backward, optional                       */
+    double* grad_phR;         /* backward, optional                       */
+} psgc_adjoint_view;
+PSGC_API int psgc_adjoint_forward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n);
+PSGC_API int psgc_adjoint_backward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n);
+
 /* ---- instrumentation ------------------------------------------------------
  * Per-kernel-family timing with hipEvents recorded on the launch stream.
  * psgc_profile_enable(1) clears and starts recording; psgc_profile_read

@@ -56,6 +56,31 @@ def triangulate(theta, phL, phR, match_col, nominal, ctx: Context | None = None)
     return xyz, valid
 
 
+def triangulate_views(thetas, phLs, phRs, match_cols, nominal, ctx: Context | None = None):
+    """triangulate for many views (pairs) in one forward and one backward
+    launch: thetas [V, 9]; per view a raw phase pair and its match columns
+    (sizes may differ); nominal [5] shared or [V, 5]. Returns a list of
+    (xyz, valid) per view, differentiable like triangulate."""
+    ctx = ctx or default_context()
+    V = len(phLs)
+    stream = torch.cuda.current_stream(phLs[0].device).cuda_stream
+    nom = torch.as_tensor(nominal, dtype=torch.float64)
+    if nom.dim() == 1:
+        nom = nom.expand(V, 5)
+    sizes = torch.tensor([list(p.shape) for p in phLs], dtype=torch.int64)
+    L = torch.cat([p.reshape(-1) for p in phLs])
+    R = torch.cat([p.reshape(-1) for p in phRs])
+    M = torch.cat([m.reshape(-1) for m in match_cols])
+    xyz, valid = _load().triangulate_batch(thetas, L, R, M, nom.contiguous(), sizes,
+                                           ctx.h.value, stream)
+    out, off = [], 0
+    for p in phLs:
+        n = p.numel()
+        out.append((xyz[off:off + n].view(*p.shape, 3), valid[off:off + n].view(*p.shape)))
+        off += n
+    return out
+
+
 def plane_loss(xyz, valid):
     """Sum of squared residuals of the valid points to their least-squares
     plane: the smallest eigenvalue of the scatter matrix."""
@@ -171,4 +196,63 @@ def refine_plane(theta_start, theta_ref, phL, phR, kL, kR, nominal, pha_f, outer
             else:
                 break
         hist.append((start, float(obj(z, mc))))
+    return ref + z * scale, hist
+
+
+def refine_planes(thetas_start, thetas_ref, phLs, phRs, kLs, kRs, nominal, pha_f, outer=6,
+                  inner=6, step=0.5, free=(1, 2, 3, 4, 6, 7, 8), ctx: Context | None = None):
+    """refine_plane for many views at once (nominal [5] shared or [V, 5]):
+    each inner step is one batched
+    forward + backward over all views (their flatness objectives are
+    separate, so one summed backward yields every view's gradient), and the
+    backtracking line search runs for all views together (a trial is one
+    batched forward). thetas [V, 9]. Returns (thetas, per view the history
+    of (flatness before, flatness after) per outer step)."""
+    ctx = ctx or default_context()
+    V = len(phLs)
+    nom = torch.as_tensor(nominal, dtype=torch.float64)
+    if nom.dim() == 1:
+        nom = nom.expand(V, 5)
+    ref = torch.as_tensor(thetas_ref, dtype=torch.float64).reshape(V, 9)
+    scale = torch.stack([default_scale(r) for r in ref])
+    mask = torch.zeros(9, dtype=torch.float64)
+    mask[list(free)] = 1.0
+    z = ((torch.as_tensor(thetas_start, dtype=torch.float64).reshape(V, 9) - ref) / scale).clone()
+    hist = [[] for _ in range(V)]
+
+    def objs(zz, mcs, grad=False):
+        th = (ref + zz * scale).requires_grad_(grad)
+        res = triangulate_views(th, phLs, phRs, mcs, nom, ctx)
+        vals = torch.stack([plane_flatness(x, v) for x, v in res])
+        if grad:
+            vals.sum().backward()
+            return vals.detach().cpu(), th.grad * scale * mask
+        return vals.detach().cpu()
+
+    for _ in range(outer):
+        th_now = ref + z * scale
+        mcs = [rectified_match(th_now[v], nom[v], phLs[v], phRs[v], kLs[v], kRs[v], pha_f,
+                               ctx=ctx) for v in range(V)]
+        start = None
+        for _ in range(inner):
+            f0, g = objs(z, mcs, grad=True)
+            if start is None:
+                start = f0.clone()
+            gn = g.norm(dim=1, keepdim=True)
+            d = torch.where(gn > 0, -g / gn.clamp_min(1e-300), torch.zeros_like(g))
+            a = torch.full((V, 1), step, dtype=torch.float64)
+            pending = gn[:, 0] > 0
+            while bool(pending.any()):  # every view's step halves to 1e-4 at most
+                trial = torch.where(pending[:, None], z + a * d, z)
+                f1 = objs(trial, mcs)
+                better = pending & (f1 < f0)
+                z = torch.where(better[:, None], trial, z)
+                pending = pending & ~better
+                a = torch.where(pending[:, None], a * 0.5, a)
+                pending = pending & (a[:, 0] > 1e-4)
+            if not bool((gn[:, 0] > 0).any()):
+                break
+        end = objs(z, mcs)
+        for v in range(V):
+            hist[v].append((float(start[v]), float(end[v])))
     return ref + z * scale, hist

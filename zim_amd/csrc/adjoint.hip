@@ -19,6 +19,8 @@
 // The adjoint accumulates dL/dtheta (per-workgroup reduction, then f64
 // atomics) and scatters dL/dphi into the raw maps with f64 atomics.
 
+#include <vector>
+
 #include "psgc_internal.h"
 
 namespace psgc {
@@ -196,15 +198,11 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   e->ok = true;
 }
 
-__global__ void adj_forward_kernel(Theta t, Nominal n, const double* __restrict__ phL,
-                                   const double* __restrict__ phR,
-                                   const int32_t* __restrict__ mc, int rows, int cols,
-                                   double* __restrict__ xyz, uint8_t* __restrict__ valid) {
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (size_t)rows * cols) return;
+__device__ inline void adj_forward_px(const Theta& t, const Nominal& n, const double (&R)[3][3],
+                                      const double (&dR)[3][3][3], const double* phL,
+                                      const double* phR, const int32_t* mc, int rows, int cols,
+                                      size_t p, double* xyz, uint8_t* valid) {
   const int i = (int)(p / cols), j = (int)(p % cols);
-  double R[3][3], dR[3][3][3];
-  rot(t, R, dR);
   PointEval e;
   eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
   xyz[p * 3 + 0] = e.ok ? e.xyz[0] : 0.0;
@@ -213,38 +211,38 @@ __global__ void adj_forward_kernel(Theta t, Nominal n, const double* __restrict_
   if (valid) valid[p] = e.ok ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(256) adj_backward_kernel(
-    Theta t, Nominal n, const double* __restrict__ phL, const double* __restrict__ phR,
-    const int32_t* __restrict__ mc, int rows, int cols, const double* __restrict__ gxyz,
-    double* __restrict__ gth, double* __restrict__ gphL, double* __restrict__ gphR) {
-  __shared__ double red[9][256];
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (p < (size_t)rows * cols) {
-    const int i = (int)(p / cols), j = (int)(p % cols);
-    double R[3][3], dR[3][3][3];
-    rot(t, R, dR);
-    PointEval e;
-    eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
-    if (e.ok) {
-      const double g[3] = {gxyz[p * 3], gxyz[p * 3 + 1], gxyz[p * 3 + 2]};
-      const double gd = g[0] * e.dxyz_dd[0] + g[1] * e.dxyz_dd[1] + g[2] * e.dxyz_dd[2];
-      for (int k = 0; k < 9; ++k)
-        acc[k] = g[0] * e.dxyz_dth[0][k] + g[1] * e.dxyz_dth[1][k] + g[2] * e.dxyz_dth[2][k];
-      double gL = 0.0;
-      for (int m = 0; m < 3; ++m) {
-        const double gc = gd * e.dd_dc[m];  // dL/dc_m
-        gL -= gc;
-        // through the right sample
-        for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * e.dur[m][k] + e.br[m].dv * e.dvr[m][k]);
-        if (gphR)
-          for (int q = 0; q < 4; ++q) atomicAdd(&gphR[e.br[m].idx[q]], gc * e.br[m].w[q]);
-      }
-      for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * e.dul[k] + e.bl.dv * e.dvl[k]);
-      if (gphL)
-        for (int q = 0; q < 4; ++q) atomicAdd(&gphL[e.bl.idx[q]], gL * e.bl.w[q]);
-    }
+// One pixel's contribution to dL/dtheta (acc) and its scatter into the raw
+// phase gradients.
+__device__ inline void adj_backward_px(const Theta& t, const Nominal& n,
+                                       const double (&R)[3][3], const double (&dR)[3][3][3],
+                                       const double* phL, const double* phR, const int32_t* mc,
+                                       int rows, int cols, size_t p, const double* gxyz,
+                                       double* gphL, double* gphR, double (&acc)[9]) {
+  const int i = (int)(p / cols), j = (int)(p % cols);
+  PointEval e;
+  eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
+  if (!e.ok) return;
+  const double g[3] = {gxyz[p * 3], gxyz[p * 3 + 1], gxyz[p * 3 + 2]};
+  const double gd = g[0] * e.dxyz_dd[0] + g[1] * e.dxyz_dd[1] + g[2] * e.dxyz_dd[2];
+  for (int k = 0; k < 9; ++k)
+    acc[k] = g[0] * e.dxyz_dth[0][k] + g[1] * e.dxyz_dth[1][k] + g[2] * e.dxyz_dth[2][k];
+  double gL = 0.0;
+  for (int m = 0; m < 3; ++m) {
+    const double gc = gd * e.dd_dc[m];  // dL/dc_m
+    gL -= gc;
+    // through the right sample
+    for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * e.dur[m][k] + e.br[m].dv * e.dvr[m][k]);
+    if (gphR)
+      for (int q = 0; q < 4; ++q) atomicAdd(&gphR[e.br[m].idx[q]], gc * e.br[m].w[q]);
   }
+  for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * e.dul[k] + e.bl.dv * e.dvl[k]);
+  if (gphL)
+    for (int q = 0; q < 4; ++q) atomicAdd(&gphL[e.bl.idx[q]], gL * e.bl.w[q]);
+}
+
+// Workgroup sum of acc, added to gth (f64 atomics).
+__device__ inline void reduce_theta(double (&acc)[9], double* gth) {
+  __shared__ double red[9][256];
   for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = acc[k];
   __syncthreads();
   for (int h = 128; h >= 1; h /= 2) {
@@ -253,6 +251,85 @@ __global__ void __launch_bounds__(256) adj_backward_kernel(
     __syncthreads();
   }
   if (threadIdx.x < 9) atomicAdd(&gth[threadIdx.x], red[threadIdx.x][0]);
+}
+
+__global__ void adj_forward_kernel(Theta t, Nominal n, const double* __restrict__ phL,
+                                   const double* __restrict__ phR,
+                                   const int32_t* __restrict__ mc, int rows, int cols,
+                                   double* __restrict__ xyz, uint8_t* __restrict__ valid) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (size_t)rows * cols) return;
+  double R[3][3], dR[3][3][3];
+  rot(t, R, dR);
+  adj_forward_px(t, n, R, dR, phL, phR, mc, rows, cols, p, xyz, valid);
+}
+
+__global__ void __launch_bounds__(256) adj_backward_kernel(
+    Theta t, Nominal n, const double* __restrict__ phL, const double* __restrict__ phR,
+    const int32_t* __restrict__ mc, int rows, int cols, const double* __restrict__ gxyz,
+    double* __restrict__ gth, double* __restrict__ gphL, double* __restrict__ gphR) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < (size_t)rows * cols) {
+    double R[3][3], dR[3][3][3];
+    rot(t, R, dR);
+    adj_backward_px(t, n, R, dR, phL, phR, mc, rows, cols, p, gxyz, gphL, gphR, acc);
+  }
+  reduce_theta(acc, gth);
+}
+
+// ---- many views in one launch ---------------------------------------------
+// Each view owns a run of whole 256-thread workgroups (block_base = the
+// first), so a workgroup's view, and its rotation, are uniform.
+
+struct AdjView {
+  Theta t;
+  Nominal n;
+  const double* phL;
+  const double* phR;
+  const int32_t* mc;
+  int rows, cols;
+  double* xyz;
+  uint8_t* valid;
+  const double* gxyz;
+  double* gth;
+  double* gphL;
+  double* gphR;
+  int64_t block_base;
+};
+
+__device__ inline int view_of_block(const AdjView* v, int nv, int64_t b) {
+  int lo = 0, hi = nv - 1;  // last view with block_base <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (v[mid].block_base <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) adj_forward_batch_kernel(const AdjView* __restrict__ views,
+                                                                int nv) {
+  const AdjView& V = views[view_of_block(views, nv, blockIdx.x)];
+  const size_t p = (size_t)(blockIdx.x - V.block_base) * 256 + threadIdx.x;
+  if (p >= (size_t)V.rows * V.cols) return;
+  double R[3][3], dR[3][3][3];
+  rot(V.t, R, dR);
+  adj_forward_px(V.t, V.n, R, dR, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.xyz, V.valid);
+}
+
+__global__ void __launch_bounds__(256) adj_backward_batch_kernel(
+    const AdjView* __restrict__ views, int nv) {
+  const AdjView& V = views[view_of_block(views, nv, blockIdx.x)];
+  const size_t p = (size_t)(blockIdx.x - V.block_base) * 256 + threadIdx.x;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < (size_t)V.rows * V.cols) {
+    double R[3][3], dR[3][3][3];
+    rot(V.t, R, dR);
+    adj_backward_px(V.t, V.n, R, dR, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.gxyz, V.gphL,
+                    V.gphR, acc);
+  }
+  reduce_theta(acc, V.gth);
 }
 
 __global__ void zero_f64_kernel(double* p, size_t n) {
@@ -298,6 +375,73 @@ int launch_adj_backward(const double* theta, const double* nominal, const double
                                                                    gxyz, gth, gphL, gphR);
   PSGC_LAUNCHED();
   PSGC_PROF_END("adj_backward", s);
+  return PSGC_OK;
+}
+
+// Batched forward/backward over views (device pointers); zeroes the
+// gradient outputs first (backward).
+int launch_adj_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n, bool backward) {
+  if (n < 1) PSGC_FAIL(PSGC_E_INVALID, "no views");
+  hipStream_t s = ctx->stream;
+  std::vector<AdjView> hv(n);
+  int64_t blocks = 0;
+  for (int v = 0; v < n; ++v) {
+    const psgc_adjoint_view& w = views[v];
+    if (w.rows < 1 || w.cols < 1 || (int64_t)w.rows * w.cols > ((int64_t)1 << 31))
+      PSGC_FAIL(PSGC_E_INVALID, "view %d: size %d x %d", v, w.rows, w.cols);
+    if (!w.phL || !w.phR || !w.match_col || !(w.theta[0] > 0) || !(w.nominal[0] > 0))
+      PSGC_FAIL(PSGC_E_INVALID, "view %d: null map or non-positive focal", v);
+    if (!backward && !w.xyz) PSGC_FAIL(PSGC_E_INVALID, "view %d: null xyz", v);
+    if (backward && (!w.grad_xyz || !w.grad_theta))
+      PSGC_FAIL(PSGC_E_INVALID, "view %d: null grad_xyz or grad_theta", v);
+    AdjView& a = hv[v];
+    a.t = Theta{w.theta[0], w.theta[1], w.theta[2], w.theta[3], w.theta[4],
+                w.theta[5], w.theta[6], w.theta[7], w.theta[8]};
+    a.n = Nominal{w.nominal[0], w.nominal[1], w.nominal[2], w.nominal[3], w.nominal[4]};
+    a.phL = w.phL;
+    a.phR = w.phR;
+    a.mc = w.match_col;
+    a.rows = w.rows;
+    a.cols = w.cols;
+    a.xyz = w.xyz;
+    a.valid = w.valid;
+    a.gxyz = w.grad_xyz;
+    a.gth = w.grad_theta;
+    a.gphL = w.grad_phL;
+    a.gphR = w.grad_phR;
+    a.block_base = blocks;
+    blocks += ((int64_t)w.rows * w.cols + 255) / 256;
+  }
+  void* d;
+  PSGC_TRY(ctx_buf(ctx, S_ADJV, sizeof(AdjView) * n, &d));
+  PSGC_HIP(hipMemcpyAsync(d, hv.data(), sizeof(AdjView) * n, hipMemcpyHostToDevice, s));
+  if (backward) {
+    for (int v = 0; v < n; ++v) {
+      const size_t np = (size_t)hv[v].rows * hv[v].cols;
+      zero_f64_kernel<<<1, 64, 0, s>>>(hv[v].gth, 9);
+      PSGC_LAUNCHED();
+      if (hv[v].gphL) {
+        zero_f64_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(hv[v].gphL, np);
+        PSGC_LAUNCHED();
+      }
+      if (hv[v].gphR) {
+        zero_f64_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(hv[v].gphR, np);
+        PSGC_LAUNCHED();
+      }
+    }
+    PSGC_PROF_BEGIN("adj_backward_batch", s);
+    adj_backward_batch_kernel<<<(unsigned)blocks, 256, 0, s>>>((const AdjView*)d, n);
+    PSGC_LAUNCHED();
+    PSGC_PROF_END("adj_backward_batch", s);
+  } else {
+    PSGC_PROF_BEGIN("adj_forward_batch", s);
+    adj_forward_batch_kernel<<<(unsigned)blocks, 256, 0, s>>>((const AdjView*)d, n);
+    PSGC_LAUNCHED();
+    PSGC_PROF_END("adj_forward_batch", s);
+  }
+  // hv is pageable host memory that ends with this call: wait for the copy
+  // (and the launch) before returning
+  PSGC_HIP(hipStreamSynchronize(s));
   return PSGC_OK;
 }
 

@@ -1428,6 +1428,18 @@ int psgc_adjoint_forward(psgc_ctx* ctx, const double theta[9], const double nomi
   return finish(ctx, mem);
 }
 
+int psgc_adjoint_forward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n) {
+  PSGC_CTX_ENTER(ctx);
+  if (!views) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  return launch_adj_batch(ctx, views, n, false);
+}
+
+int psgc_adjoint_backward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n) {
+  PSGC_CTX_ENTER(ctx);
+  if (!views) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  return launch_adj_batch(ctx, views, n, true);
+}
+
 int psgc_adjoint_backward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
                           const double* phL, const double* phR, const int32_t* match_col,
                           int rows, int cols, const double* grad_xyz, double* grad_theta,

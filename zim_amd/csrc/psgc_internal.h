@@ -56,7 +56,7 @@ enum Slot : int {
   // unorganized-cloud grid: keys and indices (sort in/out), sorted points,
   // occupied cells, sort/scan scratch, small counters, retry list
   S_CL_KEY0, S_CL_KEY1, S_CL_IDX0, S_CL_IDX1, S_CL_PTS, S_CL_UKEY, S_CL_START, S_CL_COUNT,
-  S_CL_TEMP, S_CL_MISC, S_CL_RETRY,
+  S_CL_TEMP, S_CL_MISC, S_CL_RETRY, S_ADJV,
   S_NUM_SLOTS
 };
 
@@ -252,5 +252,8 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
                   float* out, float* tmp);
 int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                        float sigma, float radius, float voxel, float* out);
+
+// Batched adjoint (adjoint.hip): one forward or backward launch over views.
+int launch_adj_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n, bool backward);
 
 }  // namespace psgc
