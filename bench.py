@@ -50,6 +50,10 @@ def parse():
                          "merged into one unorganized cloud; SOR k=50, MLS order 2 at 1 mm, "
                          "Gaussian), timed per filter")
     ap.add_argument("--c5-views", type=int, default=16)
+    ap.add_argument("--ref-timing", action="store_true",
+                    help="instead: the configuration of the reference's own published GPU "
+                         "timing (5120x5120 8-bit, 107/100/94 x 8 steps, MLS off), one pair at "
+                         "a time, device-resident and host-to-host")
     return ap.parse_args()
 
 
@@ -81,6 +85,8 @@ def main():
 
     if args.c5:
         return run_c5(args)
+    if args.ref_timing:
+        return run_ref_timing(args)
 
     dev = torch.device("cuda:0")
     ctx = zim_amd.Context(0)
@@ -266,6 +272,82 @@ def main():
     }
     print(json.dumps(line), flush=True)
     batch.close()
+
+
+def run_ref_timing(args):
+    """The reference's published GPU figure is decode L+R + setData/remap +
+    reconstruct (match, median, reprojection, Gaussian, ROR; MLS off) of one
+    5120x5120 8-bit pair with 24 frames per camera in about 1.5 s on its GPU
+    (JSLAB_Dual_GPU移植说明.md:90-100, 128-133), 17.5 Mpx/s. Here the same
+    configuration (synthetic scene; the remap is the identity for a rectified
+    pair and is skipped, as the reference's GPU path leaves it on the host) is
+    timed one pair at a time: frames resident in HBM with the tiled and with
+    the plain kernels, and host-to-host (frames from host memory, every map
+    back to the host between decode and reconstruct, as the reference's
+    launchers do)."""
+    import numpy as np
+    import torch
+
+    import zim_amd
+    from zim_amd import synth
+
+    log = lambda m: print(f"[bench-ref] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    size = 5120
+    dev = torch.device("cuda:0")
+    ctx = zim_amd.Context(0)
+    p = synth.render_ref_timing(3000, device=dev, size=size)
+    torch.cuda.synchronize()
+    dcfg = zim_amd.DecodeConfig.default(freq=(107, 100, 94), shift_num=(8, 8, 8),
+                                        modulation_threshold=0.3)
+    rcfg = zim_amd.ReconConfig.default_gpu(use_mls=0, radius_search=0.7,
+                                           radius_min_neighbors=10, radius_window_size=10)
+    hostL = [f.cpu().numpy() for f in p.left]
+    hostR = [f.cpu().numpy() for f in p.right]
+
+    def device_pair():
+        L = zim_amd.decode(dcfg, p.left, ctx, want_mod=False, want_valid=False)
+        R = zim_amd.decode(dcfg, p.right, ctx, want_mod=False, want_valid=False)
+        return zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 107, p.Q, p.R1, ctx)
+
+    def host_pair():
+        L = zim_amd.decode(dcfg, hostL, ctx, want_mod=False, want_valid=False)
+        R = zim_amd.decode(dcfg, hostR, ctx, want_mod=False, want_valid=False)
+        return zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 107, p.Q, p.R1, ctx)
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best * 1e3
+
+    res = {}
+    res["device_tiled_ms"] = timed(device_pair)
+    out = device_pair()
+    torch.cuda.synchronize()
+    valid = int((out[1] != 0).sum().item())
+    ctx.set_kernels("plain")
+    res["device_plain_ms"] = timed(device_pair)
+    ctx.set_kernels("tiled")
+    res["host_to_host_ms"] = timed(host_pair)
+    npx = size * size
+    ref_mpx = 26.21 / 1.5
+    line = {"metric": "reference timing configuration: Mpx/s per pair (5120x5120 u8, 107/100/94 "
+                      "x 8 steps, MLS off)",
+            "value": round(npx / (res["device_tiled_ms"] * 1e-3) / 1e6, 3), "unit": "Mpx/s",
+            "n_gpus": 1, "valid_points": valid,
+            **{k: round(v, 3) for k, v in res.items()},
+            "mpx_s": {k.replace("_ms", ""): round(npx / (v * 1e-3) / 1e6, 1) for k, v in res.items()},
+            "reference_gpu_mpx_s": round(ref_mpx, 2),
+            "vs_reference_gpu": {k.replace("_ms", ""): round(npx / (v * 1e-3) / 1e6 / ref_mpx, 1)
+                                 for k, v in res.items()},
+            "dtype": "f64", "data": "synthetic (seeded bump height field, 8-bit frames)"}
+    log(f"times {res}")
+    print(json.dumps(line), flush=True)
 
 
 def run_c5(args):

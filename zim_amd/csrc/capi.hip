@@ -535,10 +535,18 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
     const size_t plane = (size_t)cols * sb * rows;
     void* d;
     PSGC_TRY(ctx_buf(ctx, S_FRAMES, plane * n_frames, &d));
+    const bool dense = row_stride_bytes == (int64_t)cols * sb;
     for (int f = 0; f < n_frames; ++f) {
-      PSGC_HIP(hipMemcpy2DAsync((char*)d + plane * f, (size_t)cols * sb, frames[f],
-                                (size_t)row_stride_bytes, (size_t)cols * sb, rows,
-                                hipMemcpyHostToDevice, ctx->stream));
+      // one flat copy per dense frame: a pitched copy of pageable memory
+      // measured about 50x slower (24 frames of 5120 x 5120: 1.15 s
+      // against 21 ms)
+      if (dense)
+        PSGC_HIP(hipMemcpyAsync((char*)d + plane * f, frames[f], plane, hipMemcpyHostToDevice,
+                                ctx->stream));
+      else
+        PSGC_HIP(hipMemcpy2DAsync((char*)d + plane * f, (size_t)cols * sb, frames[f],
+                                  (size_t)row_stride_bytes, (size_t)cols * sb, rows,
+                                  hipMemcpyHostToDevice, ctx->stream));
       a.frames[f] = (char*)d + plane * f;
     }
     a.stride = (int64_t)cols * sb;

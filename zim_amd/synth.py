@@ -236,6 +236,20 @@ def render_c2(seed: int, device="cpu", rows=2048, cols=2448, f=2400.0,
                    max_val=4095, dtype=torch.uint16)
 
 
+def render_ref_timing(seed: int, device="cpu", size=5120) -> Pair:
+    """The configuration of the reference's published timing
+    (JSLAB_Dual_GPU移植说明.md:128-133): a size x size 8-bit pair, three
+    frequencies 107/100/94 with 8 steps each (24 frames per camera). Scene:
+    the C2 bump height field at the same angular pixel size (f scaled with
+    the width), ambient 25, modulation 45 (I in [25, 115]), N(0, 2 DN)."""
+    f = 2400.0 * size / 2448.0
+    cam = Camera(size, size, f, 150.0)
+    proj = Projector(1280, 1000.0 * f / 2400.0)
+    z = depth_bumps(cam, seed, device)
+    return _render(cam, proj, z, seed, device, (107, 100, 94), (8, 8, 8), ambient=25.0,
+                   modulation=45.0, noise=2.0, poisson=False, max_val=255, dtype=torch.uint8)
+
+
 def render_c1(seed: int, device="cpu", rows=480, cols=640, with_gray=True) -> Pair:
     """CPU-runnable workload: 640x480, f = 800 px, B = 100 mm, plane tilted
     20 deg at 600 mm plus 3 spheres, 4-step fringes of period 16 px on an
