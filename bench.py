@@ -107,8 +107,9 @@ def main():
     batch = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rcfg, pha_f=70,
                                 Q=p.Q, R1=p.R1) for p in pairs], ctx, graph=True, autotune=True)
     torch.cuda.synchronize()
-    grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "median5",
-                                        "repro_gauss", "sor_knn", "mls", "mls_variant")}
+    grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "match_threads",
+                                        "median5", "repro_gauss", "sor_knn", "mls",
+                                        "mls_variant")}
     log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}")
     for w in range(args.warmup):
         batch.run()

@@ -305,8 +305,9 @@ def test_batch_autotune_keeps_results(ctx):
     # the graph and leave the results unchanged
     rows = 6 - b.grid("decode_rows")
     variant = 1 - b.grid("mls_variant")
-    for stage, v in (("decode_rows", rows), ("mls_variant", variant), ("decode", 37),
-                     ("match", 300), ("mls", 257), ("sor_knn", 1)):
+    mt = 128 if b.grid("match_threads") != 128 else 512
+    for stage, v in (("decode_rows", rows), ("mls_variant", variant), ("match_threads", mt),
+                     ("decode", 37), ("match", 300), ("mls", 257), ("sor_knn", 1)):
         b.set_grid(stage, v)
         assert b.grid(stage) == v
     for run in (b.run, b.run_eager):
@@ -316,7 +317,8 @@ def test_batch_autotune_keeps_results(ctx):
         torch.cuda.synchronize()
         assert np.array_equal(host(out["P"]), rP)
         assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
-    for stage, v in (("decode_rows", 3), ("mls_variant", 2), ("decode", 0), ("nope", 64)):
+    for stage, v in (("decode_rows", 3), ("mls_variant", 2), ("match_threads", 64), ("decode", 0),
+                     ("nope", 64)):
         with pytest.raises(zim_amd.PsgcError):
             b.set_grid(stage, v)
     b.close()

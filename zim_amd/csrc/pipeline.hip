@@ -120,6 +120,7 @@ struct Plan {
   int min_sor_k = 1 << 30;
   std::map<std::string, int> grid;  // per stage; "decode_rows": the decode tile height
   int dec_rows = 4;
+  int match_threads = 256;
   int mls_variant = kMlsPlain;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -416,7 +417,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
     PSGC_TRY(launch_post_smooth_valid(h.in, h.valid, h.out, h.rows, h.cols, s));
   if (!pl->mat.host.empty())
     PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
-                                g("match", 4), pl->max_cols, pl->max_knum, s));
+                                g("match", 4), pl->max_cols, pl->max_knum, pl->match_threads, s));
   if (!pl->med.host.empty())
     PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
                                  g("median5", 8), s));
@@ -483,6 +484,7 @@ static int autotune(Plan* pl, hipStream_t s) {
     // second tuning dimension: decode tile rows / MLS kernel variant
     std::vector<int> shapes{0};
     if (n == "decode") shapes = {4, 2};
+    if (n == "match") shapes = {256, 128, 512};
     if (n == "mls") {
       shapes = {kMlsPlain};
       if (pl->max_mls_hw <= kMlsCachedMaxHalf) shapes.push_back(kMlsCached);
@@ -501,7 +503,7 @@ static int autotune(Plan* pl, hipStream_t s) {
                                          grid, shape, s));
           else if (n == "match")
             PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
-                                        grid, pl->max_cols, pl->max_knum, s));
+                                        grid, pl->max_cols, pl->max_knum, shape, s));
           else if (n == "median5")
             PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
                                          grid, s));
@@ -532,6 +534,9 @@ static int autotune(Plan* pl, hipStream_t s) {
     if (n == "decode") {
       PSGC_TRY(relayout_decode(pl, best_shape, s));
       pl->grid["decode_rows"] = best_shape;
+    } else if (n == "match") {
+      pl->match_threads = best_shape;
+      pl->grid["match_threads"] = best_shape;
     } else if (n == "mls") {
       pl->mls_variant = best_shape;
       pl->grid["mls_variant"] = best_shape;
@@ -763,6 +768,10 @@ int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
     *grid = b->plan.mls_variant;
     return PSGC_OK;
   }
+  if (st == "match_threads") {
+    *grid = b->plan.match_threads;
+    return PSGC_OK;
+  }
   auto it = b->plan.grid.find(st);
   *grid = it != b->plan.grid.end() ? it->second : 0;
   return PSGC_OK;
@@ -780,6 +789,11 @@ int psgc_batch_set_grid(psgc_batch* b, const char* stage, int value) {
     if (!b->plan.dec.host.empty()) PSGC_TRY(relayout_decode(&b->plan, value, b->ctx->stream));
     else b->plan.dec_rows = value;
     b->plan.grid["decode_rows"] = value;
+  } else if (st == "match_threads") {
+    if (value != 128 && value != 256 && value != 512)
+      PSGC_FAIL(PSGC_E_INVALID, "match_threads must be 128, 256 or 512 (got %d)", value);
+    b->plan.match_threads = value;
+    b->plan.grid["match_threads"] = value;
   } else if (st == "mls_variant") {
     if (value != kMlsPlain && value != kMlsCached)
       PSGC_FAIL(PSGC_E_INVALID, "mls_variant must be 0 or 1 (got %d)", value);

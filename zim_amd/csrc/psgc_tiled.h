@@ -114,8 +114,9 @@ __device__ inline XcdRange xcd_range(int64_t total) {
 constexpr int kDecodeTileCols = 256;
 int launch_decode_tiled(const DecodeJob* jobs_dev, int njobs, int64_t total_tiles, int grid,
                         int tile_rows, hipStream_t s);
+// threads: the row tile, 128, 256 or 512 threads per row.
 int launch_match_tiled(const MatchJob* jobs_dev, int njobs, int64_t total_rows, int grid,
-                       int max_cols, int max_knum, hipStream_t s);
+                       int max_cols, int max_knum, int threads, hipStream_t s);
 int launch_median_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, hipStream_t s);
 int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid,
                              int max_half, hipStream_t s);

@@ -234,8 +234,6 @@ __global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __
 
 // ------------------------------------------------------------------ match --
 
-constexpr int MT = 256;
-
 __device__ inline double smooth_at(const double* m, int rows, int cols, int r, int x) {
   const int xl = x > 0 ? x - 1 : 0, xr = x < cols - 1 ? x + 1 : cols - 1;
   const int ru = r > 0 ? r - 1 : 0, rd = r < rows - 1 ? r + 1 : rows - 1;
@@ -248,6 +246,10 @@ __device__ inline double smooth_at(const double* m, int rows, int cols, int r, i
   return 0.5 * hm + 0.25 * (hu + hd);
 }
 
+// One rectified row per workgroup of MT threads (the row tile the batch
+// autotuner picks: 128, 256 or 512), the row's right phases and orders and
+// its order spans in LDS.
+template <int MT>
 __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restrict__ jobs,
                                                          int njobs, int64_t total) {
   extern __shared__ double lds[];
@@ -846,13 +848,18 @@ int launch_decode_tiled(const DecodeJob* jobs, int njobs, int64_t total, int gri
 }
 
 int launch_match_tiled(const MatchJob* jobs, int njobs, int64_t total, int grid, int max_cols,
-                       int max_knum, hipStream_t s) {
+                       int max_knum, int threads, hipStream_t s) {
   if (total <= 0) return PSGC_OK;
   const size_t lds = match_tiled_lds(max_cols, max_knum);
   if (lds > (size_t)match_tiled_max_lds())
     PSGC_FAIL(PSGC_E_UNSUPPORTED, "row match needs %zu B of LDS", lds);
+  if (threads != 128 && threads != 256 && threads != 512)
+    PSGC_FAIL(PSGC_E_INVALID, "row match tile of %d threads (128, 256 or 512)", threads);
   PSGC_PROF_BEGIN("match", s);
-  match_tiled_kernel<<<round_grid(total, grid), MT, lds, s>>>(jobs, njobs, total);
+  const int g = round_grid(total, grid);
+  if (threads == 128) match_tiled_kernel<128><<<g, 128, lds, s>>>(jobs, njobs, total);
+  else if (threads == 512) match_tiled_kernel<512><<<g, 512, lds, s>>>(jobs, njobs, total);
+  else match_tiled_kernel<256><<<g, 256, lds, s>>>(jobs, njobs, total);
   PSGC_LAUNCHED();
   PSGC_PROF_END("match", s);
   return PSGC_OK;
