@@ -105,6 +105,7 @@ struct Plan {
   Table<DecodeJob> dec;
   Table<MatchJob> mat;
   Table<CloudJob> med, rg, sor;
+  Table<SorStatJob> sst;             // SOR statistics, one job per pair
   std::vector<Table<CloudJob>> mls;
   std::vector<PairPost> post;
   std::vector<DecodeArgs> gray_fix;       // gray-code jump correction per camera
@@ -165,6 +166,7 @@ int upload_tables(Plan* pl) {
   PSGC_TRY(put(pl->med));
   PSGC_TRY(put(pl->rg));
   PSGC_TRY(put(pl->sor));
+  PSGC_TRY(put(pl->sst));
   for (auto& t : pl->mls) PSGC_TRY(put(t));
   return PSGC_OK;
 }
@@ -180,6 +182,7 @@ size_t table_bytes(const Plan* pl) {
   add(pl->med);
   add(pl->rg);
   add(pl->sor);
+  add(pl->sst);
   for (auto& t : pl->mls) add(t);
   return b;
 }
@@ -352,6 +355,19 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     pp.partial = part;
     pp.stats = st;
     pp.std_mul = rc.sor_std_mul;
+    {
+      SorStatJob sj{};
+      sj.md = md;
+      sj.npix = (size_t)rows * cols;
+      sj.nchunks = sor_chunks(sj.npix);
+      sj.np2 = sor_partial_doubles(sj.npix) / 3;
+      sj.part = part;
+      sj.stats = st;
+      sj.std_mul = rc.sor_std_mul;
+      sj.chunk_base = pl->sst.total;
+      pl->sst.total += (int64_t)sj.nchunks;
+      pl->sst.host.push_back(sj);
+    }
     if (!rc.use_mls) {
       pp.sor_dst = xyz_out;
       pp.sor_Pdst = P_out;
@@ -428,17 +444,18 @@ static int run_plan(Plan* pl, hipStream_t s) {
     PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
                                   g("sor_knn", 8), pl->max_sor_half, pl->min_sor_k,
                                   pl->max_sor_k, s));
+  // every SOR pair's statistics in two launches
+  if (!pl->sst.host.empty())
+    PSGC_TRY(launch_sor_stats_batch(pl->sst.dev, (int)pl->sst.host.size(), pl->sst.total, s));
   for (const PairPost& pp : pl->post) {
     const size_t np = (size_t)pp.rows * pp.cols;
     if (pp.outlier == PSGC_OUTLIER_ROR) {
       PSGC_TRY(launch_ror(pp.ror_src, pp.ror_P, pp.ror_dst, pp.ror_Pdst, pp.rows, pp.cols,
                           pp.radius, pp.min_nb, pp.window, s));
     } else if (pp.outlier == PSGC_OUTLIER_SOR) {
-      if (pp.use_mls)
-        PSGC_TRY(launch_sor_stats(pp.md, np, pp.std_mul, pp.partial, pp.stats, s));
-      else
-        PSGC_TRY(launch_sor_finish(pp.sor_src, pp.md, pp.sor_dst, pp.sor_Pdst, np, pp.std_mul,
-                                   pp.partial, pp.stats, s));
+      // with MLS the statistics are all it needs (it gates on them itself)
+      if (!pp.use_mls)
+        PSGC_TRY(launch_sor_apply(pp.sor_src, pp.md, pp.sor_dst, pp.sor_Pdst, np, pp.stats, s));
     } else if (pp.copy_xyz_dst) {
       PSGC_HIP(hipMemcpyAsync(pp.copy_xyz_dst, pp.copy_xyz_src, np * 12, hipMemcpyDeviceToDevice,
                               s));

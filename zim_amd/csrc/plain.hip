@@ -590,6 +590,78 @@ __global__ void __launch_bounds__(1024) sor_final_kernel(double* __restrict__ pa
   }
 }
 
+// All pairs' statistics in two launches: chunk partials over the
+// concatenated chunks (a workgroup finds its pair), then one workgroup per
+// pair for the final reduction; the same reduction tree as one pair alone.
+__global__ void __launch_bounds__(256) sor_chunk_batch_kernel(const SorStatJob* __restrict__ jobs,
+                                                              int njobs) {
+  int lo = 0, hi = njobs - 1;  // last pair with chunk_base <= blockIdx.x
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].chunk_base <= (int64_t)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const SorStatJob& J = jobs[lo];
+  __shared__ double s[kChunk], q[kChunk], n[kChunk];
+  const size_t c = blockIdx.x - J.chunk_base;
+  for (int t = threadIdx.x; t < kChunk; t += blockDim.x) {
+    const size_t p = c * kChunk + t;
+    const double v = (p < J.npix) ? J.md[p] : -1.0;
+    const bool ok = v >= 0.0;
+    s[t] = ok ? v : 0.0;
+    q[t] = ok ? v * v : 0.0;
+    n[t] = ok ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int h = kChunk / 2; h >= 1; h /= 2) {
+    for (int i = threadIdx.x; i < h; i += blockDim.x) {
+      s[i] += s[i + h];
+      q[i] += q[i + h];
+      n[i] += n[i + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    J.part[c] = s[0];
+    J.part[J.np2 + c] = q[0];
+    J.part[2 * J.np2 + c] = n[0];
+  }
+}
+
+__global__ void __launch_bounds__(1024) sor_final_batch_kernel(const SorStatJob* __restrict__ jobs) {
+  const SorStatJob& J = jobs[blockIdx.x];
+  double* s = J.part;
+  double* q = J.part + J.np2;
+  double* n = J.part + 2 * J.np2;
+  for (size_t i = J.nchunks + threadIdx.x; i < J.np2; i += blockDim.x) {
+    s[i] = 0.0;
+    q[i] = 0.0;
+    n[i] = 0.0;
+  }
+  __syncthreads();
+  for (size_t h = J.np2 / 2; h >= 1; h /= 2) {
+    for (size_t i = threadIdx.x; i < h; i += blockDim.x) {
+      s[i] += s[i + h];
+      q[i] += q[i + h];
+      n[i] += n[i + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double cnt = n[0], sum = s[0], sq = q[0];
+    double mean = 0.0, sd = 0.0;
+    if (cnt > 0) mean = sum / cnt;
+    if (cnt > 1) {
+      const double var = (sq - sum * sum / cnt) / (cnt - 1.0);
+      sd = sqrt(var > 0.0 ? var : 0.0);
+    }
+    J.stats[0] = cnt;
+    J.stats[1] = mean;
+    J.stats[2] = sd;
+    J.stats[3] = mean + (double)J.std_mul * sd;
+  }
+}
+
 __global__ void sor_apply_kernel(const float* __restrict__ src, double* __restrict__ md,
                                  size_t npix, const double* __restrict__ stats,
                                  float* __restrict__ dst, uint8_t* __restrict__ dstP) {
@@ -765,6 +837,28 @@ int launch_sor_stats(const double* md, size_t npix, float std_mul, double* parti
   PSGC_PROF_END("sor_stats", s);
   return PSGC_OK;
 }
+
+int launch_sor_stats_batch(const SorStatJob* jobs_dev, int njobs, int64_t total_chunks,
+                           hipStream_t s) {
+  if (njobs <= 0 || total_chunks <= 0) return PSGC_OK;
+  PSGC_PROF_BEGIN("sor_stats", s);
+  sor_chunk_batch_kernel<<<(unsigned)total_chunks, 256, 0, s>>>(jobs_dev, njobs);
+  PSGC_LAUNCHED();
+  sor_final_batch_kernel<<<njobs, 1024, 0, s>>>(jobs_dev);
+  PSGC_LAUNCHED();
+  PSGC_PROF_END("sor_stats", s);
+  return PSGC_OK;
+}
+
+int launch_sor_apply(const float* xyz, double* md, float* dst, uint8_t* dstP, size_t npix,
+                     const double* stats, hipStream_t s) {
+  if (npix == 0) return PSGC_OK;
+  sor_apply_kernel<<<(unsigned)((npix + 255) / 256), 256, 0, s>>>(xyz, md, npix, stats, dst, dstP);
+  PSGC_LAUNCHED();
+  return PSGC_OK;
+}
+
+size_t sor_chunks(size_t npix) { return (npix + kChunk - 1) / kChunk; }
 
 size_t sor_partial_doubles(size_t npix) {
   const size_t nchunks = (npix + kChunk - 1) / kChunk;

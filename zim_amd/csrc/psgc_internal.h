@@ -152,6 +152,21 @@ int launch_sor_finish(const float* xyz, double* md, float* dst, uint8_t* dstP,
                       size_t npix, float std_mul, double* partial, double* stats,
                       hipStream_t s);
 size_t sor_partial_doubles(size_t npix);
+// SOR statistics of one pair within a batch (chunk_base: its first chunk in
+// the concatenation of all pairs' chunks of 1024 points).
+struct SorStatJob {
+  const double* md;
+  size_t npix, nchunks, np2;
+  double* part;
+  double* stats;
+  float std_mul;
+  int64_t chunk_base;
+};
+int launch_sor_stats_batch(const SorStatJob* jobs_dev, int njobs, int64_t total_chunks,
+                           hipStream_t s);
+int launch_sor_apply(const float* xyz, double* md, float* dst, uint8_t* dstP, size_t npix,
+                     const double* stats, hipStream_t s);
+size_t sor_chunks(size_t npix);
 int launch_sor_stats(const double* md, size_t npix, float std_mul, double* partial,
                      double* stats, hipStream_t s);
 
