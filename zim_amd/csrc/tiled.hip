@@ -181,7 +181,9 @@ __global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __
           double h[DLR];
 #pragma unroll
           for (int rr = 0; rr < DLR; ++rr)
-            h[rr] = c * (double)b[rr][tc] + s * ((double)b[rr][tc - 1] + (double)b[rr][tc + 1]);
+            // integer samples: the outer taps' sum is exact in integers
+            // (and in double), so it is converted once
+            h[rr] = c * (double)b[rr][tc] + s * (double)((int)b[rr][tc - 1] + (int)b[rr][tc + 1]);
           accumulate(f0 + g, h);
         }
         __syncthreads();  // the chunk's readers are done before it is refilled
