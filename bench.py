@@ -396,11 +396,8 @@ def run_c5(args):
             best = min(best, time.perf_counter() - t)
         return best * 1e3
 
-    # voxel autotune for the k-nearest search: one timed SOR per candidate
-    tune = {}
-    for vox in (0.3, 0.4, 0.5, 0.7, 1.0):
-        tune[vox] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, vox, 4, ctx), reps=1)
-    voxel = min(tune, key=tune.get)
+    # the library's voxel autotuner for the k-nearest search
+    voxel, tune = zim_amd.cloud_tune_voxel(cloud, None, 50, (0.3, 0.4, 0.5, 0.7, 1.0), 4, ctx)
     log(f"kNN voxel sweep (ms): {tune} -> {voxel}")
     res = {}
     res["sor_ms"] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, voxel, 4, ctx))

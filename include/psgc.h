@@ -355,6 +355,16 @@ PSGC_API int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t*
 PSGC_API int psgc_cloud_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                             int k, float std_mul, float voxel, int rmax, uint8_t* keep,
                             double* mean_dist, double* stats, int mem);
+/* The autotuner of the k-nearest search: times psgc_cloud_sor (grid build
+ * included) on this cloud at each candidate voxel (HIP events on the
+ * context's stream) and returns the fastest in *best_voxel; times_ms (may be
+ * NULL) receives each candidate's time. Device pointers. Every candidate
+ * gives the same result for points whose k-th nearest neighbour lies within
+ * rmax times the smallest candidate (the eligibility limit scales with the
+ * voxel). */
+PSGC_API int psgc_cloud_tune_voxel(psgc_ctx* ctx, const float* xyz, const uint8_t* valid,
+                                   int64_t n, int k, int rmax, const float* candidates,
+                                   int n_candidates, float* best_voxel, float* times_ms);
 PSGC_API int psgc_cloud_mls(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                             float spatial_sigma, int polynomial_order, float max_radius,
                             int iterations, int semantics, float voxel, float* out, int mem);

@@ -125,3 +125,17 @@ def test_merged_views_filters(ctx):
     got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), None, 0.5, 2, 1.0, 1, 1, 1.0, ctx)
     torch.cuda.synchronize()
     assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL
+
+
+def test_voxel_autotuner_returns_a_candidate_and_same_result(ctx):
+    xyz, valid = _rng_cloud(6000, 15, holes=0.1)
+    d = torch.from_numpy(xyz).cuda()
+    best, times = zim_amd.cloud_tune_voxel(d, None, 20, (0.2, 0.3, 0.6), 4, ctx)
+    assert best in (0.2, 0.3, 0.6)
+    assert len(times) == 3 and all(t > 0 for t in times.values())
+    # the result does not depend on the voxel (exact k nearest)
+    a = zim_amd.cloud_sor(d, None, 20, 1.0, 0.2, 4, ctx)
+    b = zim_amd.cloud_sor(d, None, 20, 1.0, best, 4, ctx)
+    torch.cuda.synchronize()
+    assert torch.equal(a[1], b[1]) and torch.equal(a[0], b[0])
+

@@ -241,6 +241,25 @@ def cloud_sor(xyz, valid=None, k=30, std_mul=1.0, voxel=1.0, rmax=4, ctx: Contex
     return keep, md, st
 
 
+def cloud_tune_voxel(xyz, valid=None, k=30, candidates=(0.3, 0.4, 0.5, 0.7, 1.0), rmax=4,
+                     ctx: Context | None = None):
+    """The k-nearest search's voxel autotuner on a device cloud: returns
+    (fastest voxel, {voxel: ms})."""
+    ctx = (ctx or default_context()).bind_stream()
+    dev, xyz, valid = _cloud_in(xyz, valid)
+    if not dev:
+        raise ValueError("cloud_tune_voxel takes device tensors")
+    cand = np.ascontiguousarray(candidates, np.float32)
+    times = np.zeros(len(cand), np.float32)
+    best = C.c_float()
+    _capi.check(ctx.lib.psgc_cloud_tune_voxel(ctx.h, _capi.ptr(xyz), _capi.ptr(valid),
+                                              xyz.shape[0], int(k), int(rmax), _capi.ptr(cand),
+                                              len(cand), C.byref(best), _capi.ptr(times)))
+    # candidates come back as the values passed (not their float32 roundings)
+    return (float(candidates[int(np.argmin(np.abs(cand - best.value)))]),
+            {float(c): float(t) for c, t in zip(candidates, times)})
+
+
 def cloud_mls(xyz, valid=None, sigma=0.5, order=2, max_radius=1.0, iterations=1,
               semantics=SEM_GPU, voxel=None, ctx: Context | None = None):
     """MLS projection of an unorganized cloud over the neighbours within

@@ -219,6 +219,7 @@ def load(path: str | None = None):
         "psgc_filter_ror": ([vp, vp, vp, vp, vp, i32, i32, flt, i32, i32, i32], i32),
         "psgc_filter_sor": ([vp, vp, vp, vp, vp, i32, i32, i32, flt, i32, vp, vp, i32], i32),
         "psgc_cloud_sor": ([vp, vp, vp, C.c_int64, i32, flt, flt, i32, vp, vp, vp, i32], i32),
+        "psgc_cloud_tune_voxel": ([vp, vp, vp, C.c_int64, i32, i32, vp, i32, vp, vp], i32),
         "psgc_cloud_mls": ([vp, vp, vp, C.c_int64, flt, i32, flt, i32, i32, flt, vp, i32], i32),
         "psgc_cloud_gaussian": ([vp, vp, vp, C.c_int64, flt, flt, flt, vp, i32], i32),
         "psgc_filter_mls": ([vp, vp, vp, vp, i32, i32, i32, flt, i32, flt, i32, i32, i32],

@@ -744,6 +744,15 @@ int psgc_cloud_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_
   return finish(ctx, mem);
 }
 
+int psgc_cloud_tune_voxel(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
+                          int k, int rmax, const float* candidates, int n_candidates,
+                          float* best_voxel, float* times_ms) {
+  PSGC_CTX_ENTER(ctx);
+  if (!xyz || n < 0) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  return cloud_tune_voxel(ctx, xyz, valid, n, k, rmax, candidates, n_candidates, best_voxel,
+                          times_ms);
+}
+
 int psgc_cloud_mls(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                    float spatial_sigma, int polynomial_order, float max_radius, int iterations,
                    int semantics, float voxel, float* out, int mem) {

@@ -627,6 +627,48 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
   return PSGC_OK;
 }
 
+int cloud_tune_voxel(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
+                     int rmax, const float* candidates, int ncand, float* best_voxel,
+                     float* times_ms) {
+  if (ncand < 1 || !candidates || !best_voxel) PSGC_FAIL(PSGC_E_INVALID, "no candidates");
+  hipStream_t s = ctx->stream;
+  void *md, *st;
+  PSGC_TRY(ctx_buf(ctx, S_CL_TUNE_MD, (size_t)std::max<int64_t>(n, 1) * 8, &md));
+  PSGC_TRY(ctx_buf(ctx, S_CL_TUNE_ST, 64, &st));
+  hipEvent_t a, b;
+  PSGC_HIP(hipEventCreate(&a));
+  PSGC_HIP(hipEventCreate(&b));
+  float best = 0.f, best_ms = 1e30f;
+  int rc = PSGC_OK;
+  for (int c = 0; c < ncand && rc == PSGC_OK; ++c) {
+    // one timed SOR per candidate (grid build included), after the first
+    // candidate's untimed warm-up run
+    if (c == 0) rc = cloud_sor_run(ctx, xyz, valid, n, k, 1.0f, candidates[0], rmax, nullptr,
+                                   (double*)md, (double*)st);
+    if (rc != PSGC_OK) break;
+    if (hipEventRecord(a, s) != hipSuccess) { rc = PSGC_E_HIP; break; }
+    rc = cloud_sor_run(ctx, xyz, valid, n, k, 1.0f, candidates[c], rmax, nullptr, (double*)md,
+                       (double*)st);
+    if (rc != PSGC_OK) break;
+    float ms = 0.f;
+    if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+      rc = PSGC_E_HIP;
+      break;
+    }
+    if (times_ms) times_ms[c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = candidates[c];
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc != PSGC_OK) return rc;
+  *best_voxel = best;
+  return PSGC_OK;
+}
+
 int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, float sigma,
                   int order, float max_radius, int iterations, int semantics, float voxel,
                   float* out, float* tmp) {

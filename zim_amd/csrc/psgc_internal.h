@@ -56,7 +56,7 @@ enum Slot : int {
   // unorganized-cloud grid: keys and indices (sort in/out), sorted points,
   // occupied cells, sort/scan scratch, small counters, retry list
   S_CL_KEY0, S_CL_KEY1, S_CL_IDX0, S_CL_IDX1, S_CL_PTS, S_CL_UKEY, S_CL_START, S_CL_COUNT,
-  S_CL_TEMP, S_CL_MISC, S_CL_RETRY, S_ADJV,
+  S_CL_TEMP, S_CL_MISC, S_CL_RETRY, S_CL_TUNE_MD, S_CL_TUNE_ST, S_ADJV,
   S_NUM_SLOTS
 };
 
@@ -261,6 +261,9 @@ int cloud_grid_build(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int6
                      float voxel, CloudGridDev* g);
 int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
                   float std_mul, float voxel, int rmax, uint8_t* keep, double* md, double* stats);
+int cloud_tune_voxel(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, int k,
+                     int rmax, const float* candidates, int ncand, float* best_voxel,
+                     float* times_ms);
 // tmp: n x 3 floats of scratch when iterations > 1
 int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n, float sigma,
                   int order, float max_radius, int iterations, int semantics, float voxel,
