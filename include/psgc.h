@@ -454,7 +454,11 @@ PSGC_API int psgc_batch_set_grid(psgc_batch* batch, const char* stage, int value
  *   f32) and out_gray (n u8, 255 when gray is NULL; may be NULL); *count
  *   (host) receives n. Order = the reference PLY writers' (main.cpp:85-162).
  * psgc_write_ply: binary little-endian PLY, float x y z and, when gray is
- *   given, uchar red green blue (main.cpp:132-162). Host arrays. */
+ *   given, uchar red green blue (main.cpp:132-162). Host arrays.
+ * psgc_write_raw_f64: a rows x cols f64 map (phase, disparity) as raw
+ *   row-major doubles, and, when info_path is not NULL, a text file naming
+ *   rows, cols, type and layout (saveRaw64F, JSLAB_PhaseDisparityExporter.cpp:
+ *   115-141). Host array. */
 PSGC_API int psgc_rectify_maps(psgc_ctx* ctx, const double K_raw[9], const double R[9],
                                const double K_rect[9], int rows, int cols, float* map_x,
                                float* map_y, int mem);
@@ -465,6 +469,8 @@ PSGC_API int psgc_compact(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
                           const uint8_t* gray, int rows, int cols, float* out_xyz,
                           uint8_t* out_gray, int64_t* count, int mem);
 PSGC_API int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
+PSGC_API int psgc_write_raw_f64(const char* raw_path, const char* info_path, const double* data,
+                                int rows, int cols);
 
 /* ---- differentiable triangulation ----------------------------------------
  * Rectification of raw phase maps under a calibration theta, the sub-pixel

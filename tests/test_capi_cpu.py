@@ -147,6 +147,22 @@ def test_write_ply_binary_layout(tmp_path):
     assert np.array_equal(np.frombuffer(body2, "<f4").reshape(2, 3), pts)
 
 
+def test_write_raw_f64_layout(tmp_path):
+    """Raw row-major doubles plus the shape file, as saveRaw64F writes them
+    (JSLAB_PhaseDisparityExporter.cpp:115-141)."""
+    lib = _capi.load()
+    m = np.arange(12, dtype=np.float64).reshape(3, 4) * 0.5 - 1.25
+    raw, info = tmp_path / "phase.raw", tmp_path / "phase.txt"
+    assert lib.psgc_write_raw_f64(str(raw).encode(), str(info).encode(), _capi.ptr(m), 3, 4) == 0
+    assert np.array_equal(np.fromfile(raw, "<f8").reshape(3, 4), m)
+    lines = info.read_text().splitlines()
+    assert lines[:3] == ["rows 3", "cols 4", "type CV_64FC1 (double)"]
+    raw2 = tmp_path / "empty.raw"
+    assert lib.psgc_write_raw_f64(str(raw2).encode(), None, None, 0, 4) == 0
+    assert raw2.stat().st_size == 0
+    assert lib.psgc_write_raw_f64(str(raw2).encode(), None, None, 2, 2) != 0
+
+
 def test_struct_layouts_match_ctypes(tmp_path):
     """The ctypes mirrors in zim_amd/_capi.py have the C structs' sizes and
     field offsets (a drifted field would silently corrupt batch descriptors)."""

@@ -1420,6 +1420,13 @@ int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int6
   return write_ply(path, xyz, gray, n);
 }
 
+int psgc_write_raw_f64(const char* raw_path, const char* info_path, const double* data, int rows,
+                       int cols) {
+  if (!raw_path || rows < 0 || cols < 0 || ((int64_t)rows * cols > 0 && !data))
+    PSGC_FAIL(PSGC_E_INVALID, "bad arguments");
+  return write_raw_f64(raw_path, info_path, data, rows, cols);
+}
+
 int psgc_adjoint_forward(psgc_ctx* ctx, const double theta[9], const double nominal[5],
                          const double* phL, const double* phR, const int32_t* match_col,
                          int rows, int cols, double* xyz, uint8_t* valid, int mem) {

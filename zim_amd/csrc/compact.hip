@@ -228,4 +228,24 @@ int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n
   return PSGC_OK;
 }
 
+// Row-major doubles with no header; the optional text file names the shape
+// (the dump format of JSLAB_PhaseDisparityExporter.cpp:115-141).
+int write_raw_f64(const char* raw_path, const char* info_path, const double* data, int rows,
+                  int cols) {
+  FILE* fp = std::fopen(raw_path, "wb");
+  if (!fp) PSGC_FAIL(PSGC_E_INVALID, "cannot open %s", raw_path);
+  const size_t n = (size_t)rows * cols;
+  bool ok = n == 0 || std::fwrite(data, sizeof(double), n, fp) == n;
+  if (std::fclose(fp) != 0) ok = false;
+  if (!ok) PSGC_FAIL(PSGC_E_INVALID, "write to %s failed", raw_path);
+  if (!info_path) return PSGC_OK;
+  FILE* fi = std::fopen(info_path, "w");
+  if (!fi) PSGC_FAIL(PSGC_E_INVALID, "cannot open %s", info_path);
+  ok = std::fprintf(fi, "rows %d\ncols %d\ntype CV_64FC1 (double)\nlayout row-major, one double per pixel\n",
+                    rows, cols) > 0;
+  if (std::fclose(fi) != 0) ok = false;
+  if (!ok) PSGC_FAIL(PSGC_E_INVALID, "write to %s failed", info_path);
+  return PSGC_OK;
+}
+
 }  // namespace psgc

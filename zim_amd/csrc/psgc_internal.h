@@ -227,6 +227,8 @@ int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size
                    int64_t* scratch, float* out, uint8_t* out_gray, int64_t* total,
                    hipStream_t s);
 int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
+int write_raw_f64(const char* raw_path, const char* info_path, const double* data, int rows,
+                  int cols);
 
 int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols,
                  hipStream_t s);
