@@ -50,6 +50,10 @@ def parse():
                          "merged into one unorganized cloud; SOR k=50, MLS order 2 at 1 mm, "
                          "Gaussian), timed per filter")
     ap.add_argument("--c5-views", type=int, default=16)
+    ap.add_argument("--c3", action="store_true",
+                    help="instead: one batch of 64 uneven pairs (640x480 to 2448x2048; "
+                         "3-, 4- or 8-step three-frequency fringes or a Gray-code schedule), "
+                         "batched against the same pairs one at a time")
     ap.add_argument("--ref-timing", action="store_true",
                     help="instead: the configuration of the reference's own published GPU "
                          "timing (5120x5120 8-bit, 107/100/94 x 8 steps, MLS off), one pair at "
@@ -85,6 +89,8 @@ def main():
 
     if args.c5:
         return run_c5(args)
+    if args.c3:
+        return run_c3(args)
     if args.ref_timing:
         return run_ref_timing(args)
 
@@ -351,6 +357,117 @@ def run_ref_timing(args):
     print(json.dumps(line), flush=True)
 
 
+C3_SIZES = ((480, 640), (1024, 1280), (1200, 1920), (2048, 2448))
+C3_SCHEDULES = ("3-step", "4-step", "8-step", "gray")
+
+
+def run_c3(args):
+    """Uneven batch: `--pairs` pairs (default 64), each with a resolution
+    drawn uniformly from C3_SIZES and a schedule from C3_SCHEDULES (seeded):
+    three-frequency 70/64/59 heterodyne fringes with 3, 4 or 8 steps (the
+    bump height field, f scaled with the width, uint16 12-bit), or the 4-step
+    period-16 fringes plus the 6-bit Gray code with complements (the tilted
+    plane and spheres, uint8). Filters as in the full-size workload (MLS
+    order 2 at 1.5 mm, SOR k=30). value = Mpts/s of the autotuned batch
+    (one graph replay per step, points = left pixels of every pair); the
+    same pairs run one at a time through the plain per-pair path beside it,
+    whose masks and xyz must agree with the batch."""
+    import numpy as np
+    import torch
+
+    import zim_amd
+    from zim_amd import synth
+    from tests import scenes
+
+    log = lambda m: print(f"[bench-c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    dev = torch.device("cuda:0")
+    ctx = zim_amd.Context(0)
+    rng = np.random.default_rng(3)
+    specs, npts = [], 0
+    t0 = time.time()
+    for i in range(args.pairs):
+        rows, cols = C3_SIZES[rng.integers(len(C3_SIZES))]
+        sched = C3_SCHEDULES[rng.integers(len(C3_SCHEDULES))]
+        if sched == "gray":
+            p = synth.render_c1(4000 + i, device=dev, rows=rows, cols=cols)
+            dcfg, pha_f = scenes.c1_gray_config(), 50
+        else:
+            st = int(sched.split("-")[0])
+            p = synth.render_c2(4000 + i, device=dev, rows=rows, cols=cols,
+                                f=2400.0 * cols / 2448.0, steps=(st,) * 3)
+            dcfg, pha_f = scenes.c2_decode_config(shift_num=(st,) * 3), 70
+        specs.append(dict(left=p.left, right=p.right, decode=dcfg, recon=scenes.c2_recon_config(),
+                          pha_f=pha_f, Q=p.Q, R1=p.R1, _key=f"{cols}x{rows} {sched}"))
+        npts += rows * cols
+    torch.cuda.synchronize()
+    mix = {}
+    for sp in specs:
+        mix[sp["_key"]] = mix.get(sp["_key"], 0) + 1
+    log(f"rendered {len(specs)} pairs ({npts / 1e6:.1f} Mpts) in {time.time() - t0:.1f}s: {mix}")
+    pairs = [{k: v for k, v in sp.items() if k != "_key"} for sp in specs]
+
+    t0 = time.time()
+    batch = zim_amd.Batch(pairs, ctx, graph=True, autotune=True)
+    torch.cuda.synchronize()
+    grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "match_threads",
+                                        "median5", "repro_gauss", "sor_knn", "mls",
+                                        "mls_variant")}
+    log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}")
+    for _ in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = npts / (ms_per_step * 1e-3) / 1e6
+    valid = sum(int((o["P"] != 0).sum().item()) for o in batch.outputs)
+    log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} Mpts/s, {valid} valid points")
+
+    # the same pairs one at a time through the plain kernels, a sync per pair
+    ctx.set_kernels("plain")
+
+    def run_plain():
+        outs = []
+        for pr in pairs:
+            L = zim_amd.decode(pr["decode"], pr["left"], ctx)
+            R = zim_amd.decode(pr["decode"], pr["right"], ctx)
+            xyz, P, _, _ = zim_amd.reconstruct(pr["recon"], L[0], R[0], L[1], R[1], pr["pha_f"],
+                                               pr["Q"], pr["R1"], ctx)
+            torch.cuda.synchronize()
+            outs.append((xyz, P))
+        return outs
+
+    run_plain()
+    t0 = time.perf_counter()
+    outs = run_plain()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    ctx.set_kernels("tiled")
+    plain_val = npts / (plain_ms * 1e-3) / 1e6
+    mism, maxerr = 0, 0.0
+    for (xyz, P), bo in zip(outs, batch.outputs):
+        mism += int((P != bo["P"]).sum().item())
+        m = bo["P"] != 0
+        if m.any():
+            maxerr = max(maxerr, float((xyz[m] - bo["xyz"][m]).abs().max().item()))
+    log(f"plain per-pair: {plain_ms:.1f} ms, {plain_val:.1f} Mpts/s; mask mismatch {mism}, "
+        f"xyz max diff {maxerr}")
+    line = {"metric": "Mpts/s of one batch of 64 uneven pairs, batched vs per-pair",
+            "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+            "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic (seeded height fields and plane-sphere scenes rendered on the GPU)",
+            "valid_points": valid,
+            "config": {"workload": "uneven batch", "pairs": len(pairs), "points": npts,
+                       "mix": mix, "path": "batched graph"},
+            "per_pair_plain": {"mpts_per_s": round(plain_val, 3), "ms": round(plain_ms, 3),
+                               "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr},
+            "speedup_vs_per_pair": round(value / plain_val, 2), "autotuned_grids": grids}
+    print(json.dumps(line), flush=True)
+    batch.close()
+
+
 def run_c5(args):
     """Merged-cloud filters: --c5-views views of one 2448x2048 C2 scene (the
     same surface, independent noise) reconstructed on the GPU without
@@ -358,7 +475,9 @@ def run_c5(args):
     3.4 M points per view) resident in HBM; then SOR (k = 50, exact k
     nearest), MLS (order 2, r = 1.0 mm, sigma 0.5 mm, one iteration) and a
     Gaussian (sigma 0.5 mm, r = 1.0 mm) over it, each timed with the grid
-    build included. The kNN voxel comes from a timing sweep (the autotuner)."""
+    build included. The kNN voxel comes from a timing sweep (the autotuner);
+    the radius filters run on cells of the radius and of half of it, and
+    the faster of the two counts in the total."""
     import torch
 
     import zim_amd
@@ -402,9 +521,17 @@ def run_c5(args):
     res = {}
     res["sor_ms"] = timed(lambda: zim_amd.cloud_sor(cloud, None, 50, 1.0, voxel, 4, ctx))
     keep, md, st = zim_amd.cloud_sor(cloud, None, 50, 1.0, voxel, 4, ctx)
-    res["mls_ms"] = timed(lambda: zim_amd.cloud_mls(cloud, keep, 0.5, 2, 1.0, 1, 1, 1.0, ctx),
-                          reps=1)
-    res["gauss_ms"] = timed(lambda: zim_amd.cloud_gaussian(cloud, keep, 0.5, 1.0, 1.0, ctx))
+    # the radius filters on cells of the radius (3x3x3 blocks) and of half
+    # of it (5x5x5 blocks); the faster counts in the total
+    for vx, tag in ((1.0, "r"), (0.5, "half_r")):
+        res[f"mls_ms_voxel_{tag}"] = timed(
+            lambda: zim_amd.cloud_mls(cloud, keep, 0.5, 2, 1.0, 1, 1, vx, ctx), reps=1)
+        res[f"gauss_ms_voxel_{tag}"] = timed(
+            lambda: zim_amd.cloud_gaussian(cloud, keep, 0.5, 1.0, vx, ctx))
+        log(f"voxel {vx}: mls {res[f'mls_ms_voxel_{tag}']:.1f} ms, "
+            f"gauss {res[f'gauss_ms_voxel_{tag}']:.1f} ms")
+    res["mls_ms"] = min(res["mls_ms_voxel_r"], res["mls_ms_voxel_half_r"])
+    res["gauss_ms"] = min(res["gauss_ms_voxel_r"], res["gauss_ms_voxel_half_r"])
     torch.cuda.synchronize()
     total = res["sor_ms"] + res["mls_ms"] + res["gauss_ms"]
     line = {"metric": "merged-cloud filters Mpts/s (SOR k=50 + MLS order 2 r=1 mm + Gaussian)",

@@ -1459,12 +1459,14 @@ int orc_cloud_sor(const float* xyz, const uint8_t* valid, int64_t n, int k, floa
 
 // MLS on an unorganized cloud: the organized filter's fit (orders, fallback,
 // semantics) with the neighbours within `max_radius` (3 sigma if 0) of the
-// point, itself included, found in the 3x3x3 cells around it (voxel >=
-// radius). Each iteration rebuilds the grid from the moved points.
+// point, itself included, found in the (2R+1)^3 cells around it
+// (R = cloud_radius_cells: voxel >= radius, or half-radius cells with R = 2).
+// Each iteration rebuilds the grid from the moved points.
 int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma, int order,
                   float max_radius, int iterations, int semantics, float voxel, float* out) {
   const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
-  if (n < 0 || !(sigma > 0.0f) || voxel < r) return PSGC_E_INVALID;
+  const int R = psgc::cloud_radius_cells(voxel, r);
+  if (n < 0 || !(sigma > 0.0f) || R == 0) return PSGC_E_INVALID;
   const float r2 = r * r;
   const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
   const int ord = std::max(0, std::min(2, order));
@@ -1478,7 +1480,7 @@ int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma
     for (int64_t i = 0; i < n; ++i) {
       if (!cloud_ok(cur.data(), valid, i)) continue;
       auto each = [&](auto&& visit) {
-        grid_block(g, cur.data() + i * 3, 1, [&](int64_t j) { visit(cur.data() + j * 3); });
+        grid_block(g, cur.data() + i * 3, R, [&](int64_t j) { visit(cur.data() + j * 3); });
       };
       mls_point(cur.data() + i * 3, each, inv_2s2, r2, ord, semantics, out + i * 3);
     }
@@ -1488,10 +1490,13 @@ int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma
 }
 
 // Gaussian smoothing of an unorganized cloud: the weighted mean of the points
-// within `radius` (the point itself included), weights exp(-d^2 / 2 sigma^2).
+// within `radius` (the point itself included), weights exp(-d^2 / 2 sigma^2);
+// candidates from the (2R+1)^3 cells as for the MLS.
 int orc_cloud_gaussian(const float* xyz, const uint8_t* valid, int64_t n, float sigma,
                        float radius, float voxel, float* out) {
-  if (n < 0 || !(sigma > 0.0f) || !(radius > 0.0f) || voxel < radius) return PSGC_E_INVALID;
+  if (n < 0 || !(sigma > 0.0f) || !(radius > 0.0f)) return PSGC_E_INVALID;
+  const int R = psgc::cloud_radius_cells(voxel, radius);
+  if (R == 0) return PSGC_E_INVALID;
   CloudGrid g;
   const int rc = build_grid(xyz, valid, n, voxel, &g);
   if (rc != PSGC_OK) return rc;
@@ -1502,7 +1507,7 @@ int orc_cloud_gaussian(const float* xyz, const uint8_t* valid, int64_t n, float 
   for (int64_t i = 0; i < n; ++i) {
     if (!cloud_ok(xyz, valid, i)) continue;
     float sx = 0, sy = 0, sz = 0, sw = 0;
-    grid_block(g, xyz + i * 3, 1, [&](int64_t j) {
+    grid_block(g, xyz + i * 3, R, [&](int64_t j) {
       const float* pk = xyz + j * 3;
       const float d2 = d2_of(xyz + i * 3, pk);
       if (d2 > r2) return;

@@ -43,10 +43,11 @@ def test_cloud_sor_is_exact_knn(k, voxel, rmax):
     assert np.array_equal(keep != 0, eligible & (ref <= st[3]))
 
 
-def test_cloud_gaussian_brute_force():
+@pytest.mark.parametrize("voxel", [0.45, 0.21])   # 3x3x3 cells; 5x5x5 half-radius cells
+def test_cloud_gaussian_brute_force(voxel):
     xyz, valid = _rng_cloud(1200, 5, holes=0.05)
     sigma, radius = 0.15, 0.4
-    got = oracle.cloud_gaussian(xyz, valid, sigma, radius, 0.45)
+    got = oracle.cloud_gaussian(xyz, valid, sigma, radius, voxel)
     x = xyz.astype(np.float64)
     ok = valid != 0
     for i in range(0, len(xyz), 7):
@@ -61,7 +62,8 @@ def test_cloud_gaussian_brute_force():
         assert np.abs(got[i] - ref).max() < 1e-6 * 700
 
 
-def test_cloud_mls_matches_organized_window():
+@pytest.mark.parametrize("voxel", [3.0, 1.5])
+def test_cloud_mls_matches_organized_window(voxel):
     """Points on a regular unit grid in x, y with a gentle height: a 5 x 5
     pixel window with a 2.9 cutoff and a 2.9 radius select the same
     neighbours, so the organized filter (the reference's) and the cloud
@@ -75,13 +77,16 @@ def test_cloud_mls_matches_organized_window():
     org = np.stack([jj, ii, z], -1).astype(np.float32)
     P = np.ones((rows, cols), np.uint8)
     ref = oracle.mls(org, P, 2, 1.2, order, 2.9, 1, 1)
-    got = oracle.cloud_mls(org.reshape(-1, 3), None, 1.2, order, 2.9, 1, 1, 3.0).reshape(rows, cols, 3)
+    got = oracle.cloud_mls(org.reshape(-1, 3), None, 1.2, order, 2.9, 1, 1, voxel).reshape(rows, cols, 3)
     d = np.abs(got - ref).max(-1)
-    assert d.max() < 5e-4 and np.median(d) < 1e-6
+    # the corner points (a quarter disc of neighbours) are the least well
+    # conditioned: there the summation order alone moves the fit by ~2e-3
+    assert d[2:-2, 2:-2].max() < 5e-4 and d.max() < 5e-3 and np.median(d) < 1e-6
     assert np.abs(got - org).max() > 1e-3   # the filter moved points
 
 
-def test_cloud_mls_fixed_points_and_iterations():
+@pytest.mark.parametrize("voxel", [1.2, 0.6])
+def test_cloud_mls_fixed_points_and_iterations(voxel):
     """A plane is a fixed point (order 1); a quadric height field is
     reproduced by order 2 far better than by order 1 (not exactly: in the
     tilted local frame the surface picks up cubic terms); invalid points pass
@@ -89,19 +94,21 @@ def test_cloud_mls_fixed_points_and_iterations():
     g = np.random.default_rng(2)
     xy = g.uniform(-3, 3, (2000, 2))
     plane = np.column_stack([xy, 10 + 0.2 * xy[:, 0] - 0.1 * xy[:, 1]]).astype(np.float32)
-    out = oracle.cloud_mls(plane, None, 0.5, 1, 1.2, 2, 1, 1.2)
+    out = oracle.cloud_mls(plane, None, 0.5, 1, 1.2, 2, 1, voxel)
     assert np.abs(out - plane).max() < 2e-5
     quad = np.column_stack([xy, 0.05 * xy[:, 0] ** 2 - 0.03 * xy[:, 0] * xy[:, 1]
                             + 0.02 * xy[:, 1] ** 2]).astype(np.float32)
     valid = (g.random(2000) > 0.1).astype(np.uint8)
-    out = oracle.cloud_mls(quad, valid, 0.5, 2, 1.2, 1, 1, 1.2)
-    out1 = oracle.cloud_mls(quad, valid, 0.5, 1, 1.2, 1, 1, 1.2)
+    out = oracle.cloud_mls(quad, valid, 0.5, 2, 1.2, 1, 1, voxel)
+    out1 = oracle.cloud_mls(quad, valid, 0.5, 1, 1.2, 1, 1, voxel)
     e2, e1 = np.abs(out - quad).max(), np.abs(out1 - quad).max()
     assert e2 < 5e-4 and e2 < 0.2 * e1, (e2, e1)
     assert np.array_equal(out[valid == 0], quad[valid == 0])
 
 
 def test_cloud_voxel_below_radius_is_rejected():
+    """Radius filters need voxel >= radius / 2 (5x5x5 cells cover the ball)."""
     xyz, _ = _rng_cloud(50, 1)
+    oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.15)
     with pytest.raises(AssertionError):
-        oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.2)
+        oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.14)

@@ -72,7 +72,8 @@ def test_cloud_sor_host_mode_and_edge_cases(ctx):
         zim_amd.cloud_sor(xyz, None, 5, 1.0, 1e-9, 4, ctx)   # 10^27 cells
 
 
-@pytest.mark.parametrize("order,voxel", [(0, 0.4), (1, 0.4), (2, 0.4), (2, 0.6), (2, 1.0)])
+@pytest.mark.parametrize("order,voxel", [(0, 0.4), (1, 0.4), (2, 0.4), (2, 0.6), (2, 1.0),
+                                         (1, 0.18), (2, 0.18), (2, 0.25)])
 @pytest.mark.parametrize("sem", [0, 1])
 def test_cloud_mls_random(ctx, order, voxel, sem):
     xyz, valid = _rng_cloud(5000, 13, holes=0.1)
@@ -87,7 +88,7 @@ def test_cloud_mls_random(ctx, order, voxel, sem):
     assert np.array_equal(got[valid == 0], xyz[valid == 0])
 
 
-@pytest.mark.parametrize("voxel", [0.45, 0.6, 1.0])
+@pytest.mark.parametrize("voxel", [0.45, 0.6, 1.0, 0.2, 0.3])
 def test_cloud_gaussian_random(ctx, voxel):
     xyz, valid = _rng_cloud(5000, 14, holes=0.05)
     ref = oracle.cloud_gaussian(xyz, valid, 0.15, 0.4, voxel)
@@ -121,10 +122,14 @@ def test_merged_views_filters(ctx):
     assert len(xyz) > 6000
     ref, got = _sor_both(ctx, xyz, None, 50, 1.0, 4)
     _assert_sor(ref, got)
-    ref = oracle.cloud_mls(xyz, None, 0.5, 2, 1.0, 1, 1, 1.0)
-    got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), None, 0.5, 2, 1.0, 1, 1, 1.0, ctx)
-    torch.cuda.synchronize()
-    assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL
+    for voxel in (1.0, 0.5):   # 3x3x3 cells of the radius, 5x5x5 of half of it
+        ref = oracle.cloud_mls(xyz, None, 0.5, 2, 1.0, 1, 1, voxel)
+        got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), None, 0.5, 2, 1.0, 1, 1, voxel, ctx)
+        torch.cuda.synchronize()
+        assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL, voxel
+        ref = oracle.cloud_gaussian(xyz, None, 0.5, 1.0, voxel)
+        got = zim_amd.cloud_gaussian(xyz, None, 0.5, 1.0, voxel, ctx)
+        assert np.abs(got - ref).max() <= XYZ_TOL, voxel
 
 
 def test_voxel_autotuner_returns_a_candidate_and_same_result(ctx):

@@ -7,11 +7,12 @@
 // Grid: cells of edge `voxel` (psgc_voxel.h); the points are sorted by cell
 // key with a stable radix sort (ties keep index order), the occupied cells
 // found by run-length encoding. A workgroup takes one occupied cell: the
-// 3x3x3 block of cells around it is looked up once (binary search over the
-// sorted keys), its points streamed through LDS in chunks (512 or 1024),
-// and every thread scans each chunk for one point of the centre cell. Candidates come
-// in a fixed order (cells dz, dy, dx = -1..1, points in grid order), the
-// order of the CPU restatement.
+// block of cells around it (3x3x3; 5x5x5 for radius filters on half-radius
+// cells) is looked up once (binary search over the sorted keys), its points
+// streamed through LDS in chunks (512 or 1024), and every thread scans each
+// chunk for one point of the centre cell. Candidates come in a fixed order
+// (cells dz, dy, dx = -R..R, points in grid order), the order of the CPU
+// restatement.
 
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -109,8 +110,11 @@ __global__ void cloud_gather_kernel(const float* __restrict__ xyz, const int32_t
 
 // ------------------------------------------------------- block staging ----
 
+// The (2R+1)^3 cells around a cell: first point and running counts.
+template <int R>
 struct BlockCells {
-  int32_t st[27], pre[28];
+  static constexpr int W = 2 * R + 1, N = W * W * W;
+  int32_t st[N], pre[N + 1];
 };
 
 // The 27 cells ordered by distance class from the centre: itself, the 6
@@ -119,39 +123,50 @@ struct BlockCells {
 __constant__ int8_t kNearFirst[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11,
                                       15, 17, 19, 21, 23, 25, 0, 2, 6, 8, 18, 20, 24, 26};
 
-// The 27 cells around `key` (absent ones empty) and the running counts; in
-// the canonical order (dz, dy, dx = -1..1), or nearest class first.
-template <bool NEAR_FIRST = false>
+// The cells around `key` (absent ones empty) and the running counts, in the
+// canonical order (dz, dy, dx = -R..R), or (R = 1) nearest class first.
+// TB >= 64 threads; the first wave scans the counts, two cells per lane.
+template <int R, int TB, bool NEAR_FIRST = false>
 __device__ inline void load_block(const CellTab& t, const VoxelFrame& f, uint64_t key,
-                                  BlockCells* b) {
-  __shared__ int32_t cnt[27];
-  if (threadIdx.x < 27) {
-    const int d = NEAR_FIRST ? kNearFirst[threadIdx.x] : (int)threadIdx.x;
-    const int slot = threadIdx.x;
-    const int64_t off = (d % 3 - 1) + f.X * ((d / 3) % 3 - 1 + f.Y * (int64_t)(d / 9 - 1));
-    const int32_t c = cell_find(t, (uint64_t)((int64_t)key + off));
-    b->st[slot] = c >= 0 ? ld_global(t.start, c) : 0;
+                                  BlockCells<R>* b) {
+  using B = BlockCells<R>;
+  static_assert(B::N <= 128 && TB >= 64 && (!NEAR_FIRST || R == 1), "block shape");
+  __shared__ int32_t cnt[128];
+  for (int slot = threadIdx.x; slot < 128; slot += TB) {
+    int32_t c = -1;
+    if (slot < B::N) {
+      const int d = NEAR_FIRST ? kNearFirst[slot] : slot;
+      const int64_t off = (d % B::W - R) +
+                          f.X * ((d / B::W) % B::W - R + f.Y * (int64_t)(d / (B::W * B::W) - R));
+      c = cell_find(t, (uint64_t)((int64_t)key + off));
+      b->st[slot] = c >= 0 ? ld_global(t.start, c) : 0;
+    }
     cnt[slot] = c >= 0 ? ld_global(t.count, c) : 0;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t s = 0;
-    for (int i = 0; i < 27; ++i) {
-      b->pre[i] = s;
-      s += cnt[i];
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    const int32_t a0 = cnt[2 * l], a1 = cnt[2 * l + 1];
+    int32_t inc = a0 + a1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(inc, o, 64);
+      if (l >= o) inc += v;
     }
-    b->pre[27] = s;
+    const int32_t ex = inc - a0 - a1;
+    if (2 * l <= B::N) b->pre[2 * l] = ex;
+    if (2 * l + 1 <= B::N) b->pre[2 * l + 1] = ex + a0;
   }
   __syncthreads();
 }
 
 // Candidates [cb, cb + len) of the block into tile (cooperative, TB threads).
-template <int TB>
-__device__ inline void stage_chunk(const float4* __restrict__ pts, const BlockCells* b, int32_t cb,
-                                   int32_t len, float4* tile) {
+template <int R, int TB>
+__device__ inline void stage_chunk(const float4* __restrict__ pts, const BlockCells<R>* b,
+                                   int32_t cb, int32_t len, float4* tile) {
   for (int32_t j = threadIdx.x; j < len; j += TB) {
     const int32_t g = cb + j;
-    int lo = 0, hi = 26;  // last cell with pre <= g
+    int lo = 0, hi = BlockCells<R>::N - 1;  // last cell with pre <= g
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (b->pre[mid] <= g) lo = mid;
@@ -235,14 +250,14 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
                                                        int32_t* __restrict__ nretry) {
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
-  __shared__ BlockCells bc;
+  __shared__ BlockCells<1> bc;
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
     // nearest cells first: the k-th distance falls early and later
     // candidates rarely enter the list (the selected multiset does not
     // depend on the order)
-    load_block<true>(t, f, ld_global(t.ukey, c), &bc);
-    const int32_t M = bc.pre[27];
+    load_block<1, TB, true>(t, f, ld_global(t.ukey, c), &bc);
+    const int32_t M = bc.pre[BlockCells<1>::N];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
     for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
@@ -254,7 +269,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
       for (int32_t cb = 0; cb < M; cb += CH) {
         const int32_t len = min(CH, M - cb);
         __syncthreads();  // the previous chunk's readers are done
-        stage_chunk<TB>(pts, &bc, cb, len, tile);
+        stage_chunk<1, TB>(pts, &bc, cb, len, tile);
         __syncthreads();
         if (active)
           for (int32_t j = 0; j < len; ++j) {
@@ -332,20 +347,20 @@ __global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
 
 // ---------------------------------------------------------- MLS, Gauss ----
 
-template <int NC, int TB>
+template <int NC, int TB, int R>
 __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, float r2, float inv2s2,
                                                        int order, int semantics,
                                                        float* __restrict__ out) {
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
-  __shared__ BlockCells bc;
+  __shared__ BlockCells<R> bc;
   __shared__ double etab[64];
   if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
-    load_block(t, f, ld_global(t.ukey, c), &bc);  // its barriers also publish etab
-    const int32_t M = bc.pre[27];
+    load_block<R, TB>(t, f, ld_global(t.ukey, c), &bc);  // its barriers also publish etab
+    const int32_t M = bc.pre[BlockCells<R>::N];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
     for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
@@ -357,7 +372,7 @@ __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict_
         for (int32_t cb = 0; cb < M; cb += CH) {
           const int32_t len = min(CH, M - cb);
           __syncthreads();
-          stage_chunk<TB>(pts, &bc, cb, len, tile);
+          stage_chunk<R, TB>(pts, &bc, cb, len, tile);
           __syncthreads();
           if (on)
             for (int32_t j = 0; j < len; ++j) {
@@ -417,19 +432,19 @@ __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict_
   }
 }
 
-template <int TB>
+template <int TB, int R>
 __global__ void __launch_bounds__(TB) cloud_gauss_kernel(const float4* __restrict__ pts, CellTab t,
                                                          VoxelFrame f, float r2, float inv2s2,
                                                          float* __restrict__ out) {
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
-  __shared__ BlockCells bc;
+  __shared__ BlockCells<R> bc;
   __shared__ double etab[64];
   if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
-    load_block(t, f, ld_global(t.ukey, c), &bc);
-    const int32_t M = bc.pre[27];
+    load_block<R, TB>(t, f, ld_global(t.ukey, c), &bc);
+    const int32_t M = bc.pre[BlockCells<R>::N];
     const int32_t c0 = ld_global(t.start, c), cn = ld_global(t.count, c);
     for (int32_t qb = 0; qb < cn; qb += TB) {
       const bool active = qb + (int32_t)threadIdx.x < cn;
@@ -438,7 +453,7 @@ __global__ void __launch_bounds__(TB) cloud_gauss_kernel(const float4* __restric
       for (int32_t cb = 0; cb < M; cb += CH) {
         const int32_t len = min(CH, M - cb);
         __syncthreads();
-        stage_chunk<TB>(pts, &bc, cb, len, tile);
+        stage_chunk<R, TB>(pts, &bc, cb, len, tile);
         __syncthreads();
         if (active)
           for (int32_t j = 0; j < len; ++j) {
@@ -674,7 +689,8 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
                   float* out, float* tmp) {
   const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
   if (!(sigma > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma must be > 0");
-  if (voxel < r) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below the radius %g", (double)voxel, (double)r);
+  const int R = cloud_radius_cells(voxel, r);
+  if (R == 0) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below half the radius %g", (double)voxel, (double)r);
   const float r2 = r * r, inv2s2 = 1.0f / (2.0f * sigma * sigma);
   const int ord = std::max(0, std::min(2, order));
   hipStream_t s = ctx->stream;
@@ -696,8 +712,13 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
       PSGC_PROF_BEGIN("cloud_mls", s);
       with_width(tbv, [&](auto w) {
         constexpr int TB = decltype(w)::value;
-        if (ord >= 2) cloud_mls_kernel<6, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-        else cloud_mls_kernel<3, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+        if (R == 1) {
+          if (ord >= 2) cloud_mls_kernel<6, TB, 1><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+          else cloud_mls_kernel<3, TB, 1><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+        } else {
+          if (ord >= 2) cloud_mls_kernel<6, TB, 2><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+          else cloud_mls_kernel<3, TB, 2><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+        }
       });
       PSGC_LAUNCHED();
       PSGC_PROF_END("cloud_mls", s);
@@ -710,7 +731,9 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
 int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                        float sigma, float radius, float voxel, float* out) {
   if (!(sigma > 0.0f) || !(radius > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma and radius must be > 0");
-  if (voxel < radius) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below the radius %g", (double)voxel, (double)radius);
+  const int R = cloud_radius_cells(voxel, radius);
+  if (R == 0)
+    PSGC_FAIL(PSGC_E_INVALID, "voxel %g below half the radius %g", (double)voxel, (double)radius);
   CloudGridDev g;
   PSGC_TRY(cloud_grid_build(ctx, xyz, valid, n, voxel, &g));
   hipStream_t s = ctx->stream;
@@ -721,8 +744,10 @@ int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, in
     const int tbv = std::max(128, width_for(g));
     with_width(tbv, [&](auto w) {
       constexpr int TB = decltype(w)::value;
-      cloud_gauss_kernel<TB><<<grid_for_cells(ctx, g.ncells, TB), TB, 0, s>>>(
-          g.pts, t, g.f, radius * radius, 1.0f / (2.0f * sigma * sigma), out);
+      const unsigned grid = grid_for_cells(ctx, g.ncells, TB);
+      const float r2 = radius * radius, inv2s2 = 1.0f / (2.0f * sigma * sigma);
+      if (R == 1) cloud_gauss_kernel<TB, 1><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, out);
+      else cloud_gauss_kernel<TB, 2><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, out);
     });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_gauss", s);

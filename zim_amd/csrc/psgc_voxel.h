@@ -14,8 +14,10 @@
 namespace psgc {
 
 // Cells of edge `voxel` from the origin (the bounding box's minimum less two
-// cells, so every occupied cell has a full ring of neighbours inside the
-// frame); key = ix + X (iy + Y iz).
+// cells; two empty cells past the maximum), so every occupied cell has a full
+// ring of neighbours inside the frame, and a block cell two away that falls
+// outside has a linear key (key + offset) of an empty margin cell or of no
+// cell at all; key = ix + X (iy + Y iz).
 struct VoxelFrame {
   float ox, oy, oz;
   float inv;     // 1 / voxel
@@ -72,6 +74,14 @@ PSGC_VX_HD inline bool voxel_point_ok(float x, float y, float z) {
 PSGC_VX_HD inline float cloud_knn_limit(float voxel, int R) {
   const float rs = voxel * (float)R;
   return rs * rs * 0.99999f;
+}
+
+// Cells of edge voxel around a point's own cell, per axis, that hold every
+// neighbour within radius r for the radius filters (MLS, Gaussian): 1 when
+// voxel >= r (a 3x3x3 block), 2 when 2 voxel >= r (5x5x5: half-radius cells
+// cover 15.6 r^3 instead of 27 r^3); 0 = voxel too small.
+PSGC_VX_HD inline int cloud_radius_cells(float voxel, float r) {
+  return voxel >= r ? 1 : (2.0f * voxel >= r ? 2 : 0);
 }
 
 }  // namespace psgc
