@@ -74,6 +74,16 @@ ALG_BYTES_PER_PX = {
 }
 
 
+# VALU lane-instructions per pixel of one launch (SQ_INSTS_VALU x 64 / pixels)
+# on this workload, from the rocprofv3 counter run summarized in
+# profiles/c2_8pair_valu_counters.md; with the live launch time they give the
+# kernel's VALU issue rate against the chip's single-issue peak (256 CUs x 4
+# SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
+VALU_PER_PX_PROFILED = {"mls": 13990, "sor_knn": 1777, "decode": 954, "match": 574,
+                        "repro_gauss": 500, "median5": 271}
+VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -151,6 +161,14 @@ def main():
     # one launch of a batched family covers every pair (MLS: one iteration)
     alg = ALG_BYTES_PER_PX[dom_name] * npix * args.pairs * (2 if dom_name == "decode" else 1)
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    compute = None
+    if dom_name in VALU_PER_PX_PROFILED:
+        px = npix * args.pairs * (2 if dom_name == "decode" else 1)
+        gi = VALU_PER_PX_PROFILED[dom_name] / 64 * px / (avg_ms * 1e-3) / 1e9
+        compute = {"bound": "valu", "wave_insts_per_px": round(VALU_PER_PX_PROFILED[dom_name] / 64, 1),
+                   "achieved_ginst_s": round(gi, 1), "single_issue_peak_ginst_s": VALU_PEAK_GINST_S,
+                   "frac": round(gi / VALU_PEAK_GINST_S, 3),
+                   "insts_from": "profiles/c2_8pair_valu_counters.md"}
     log(f"stage ms/step {stage_ms}")
 
     # ----------------------------------------------------- plain per-pair path
@@ -273,7 +291,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg,
                      "end_to_end": {"alg_bytes_per_pt": 61, "achieved": round(e2e_gbs, 2),
-                                    "frac": round(e2e_gbs / HBM_PEAK_GBS, 5)}},
+                                    "frac": round(e2e_gbs / HBM_PEAK_GBS, 5)},
+                     "compute": compute},
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity,
     }

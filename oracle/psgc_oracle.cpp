@@ -910,11 +910,17 @@ bool chol(const float A[6][6], const float b[6], float x[6], int nc) {
 
 // One point of the MLS filter. each(visit) calls visit(pk) for every
 // candidate neighbour pk (float[3]) in a fixed order; the radius test, the
-// three accumulation passes and the fit follow JSLAB_MLS_Filter.cpp:196-389.
+// three accumulation passes and the fit follow JSLAB_MLS_Filter.cpp:196-389
+// (.cu:214-389). C (GPU semantics): the per-neighbour multiply-adds (d^2,
+// the centroid, covariance and normal-equation sums, the local coordinates
+// u, v, f) are fused, as a CUDA build contracts `s += a * b`; without C each
+// product and sum is rounded (the CPU build). The solve and the evaluation
+// at the point are uncontracted in both.
 // out receives the projected point, or pi where the fit is skipped.
-template <typename Each>
-void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int order,
-               int semantics, float out[3]) {
+template <bool C, typename Each>
+void mls_point_t(const float pi[3], Each&& each, float inv_2s2, float r2cut, int order,
+                 int semantics, float out[3]) {
+  using psgc::madd;
   const int minnb[3] = {3, 5, 8};
   const float pi0 = pi[0], pi1 = pi[1], pi2 = pi[2];
   out[0] = pi0;
@@ -923,13 +929,12 @@ void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int o
   float cx = 0, cy = 0, cz = 0, wsum = 0;
   int nv = 0;
   each([&](const float* pk) {
-    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float d2 = psgc::sq3<C>(pk[0] - pi0, pk[1] - pi1, pk[2] - pi2);
     if (d2 > r2cut) return;
     const float w = psgc::exp_f32(-d2 * inv_2s2);
-    cx += w * pk[0];
-    cy += w * pk[1];
-    cz += w * pk[2];
+    cx = madd<C>(w, pk[0], cx);
+    cy = madd<C>(w, pk[1], cy);
+    cz = madd<C>(w, pk[2], cz);
     wsum += w;
     nv++;
   });
@@ -940,17 +945,16 @@ void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int o
   cz *= iw;
   float cov[3][3] = {};
   each([&](const float* pk) {
-    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float d2 = psgc::sq3<C>(pk[0] - pi0, pk[1] - pi1, pk[2] - pi2);
     if (d2 > r2cut) return;
     const float w = psgc::exp_f32(-d2 * inv_2s2);
     const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
-    cov[0][0] += w * ex * ex;
-    cov[0][1] += w * ex * ey;
-    cov[0][2] += w * ex * ez;
-    cov[1][1] += w * ey * ey;
-    cov[1][2] += w * ey * ez;
-    cov[2][2] += w * ez * ez;
+    cov[0][0] = madd<C>(w * ex, ex, cov[0][0]);
+    cov[0][1] = madd<C>(w * ex, ey, cov[0][1]);
+    cov[0][2] = madd<C>(w * ex, ez, cov[0][2]);
+    cov[1][1] = madd<C>(w * ey, ey, cov[1][1]);
+    cov[1][2] = madd<C>(w * ey, ez, cov[1][2]);
+    cov[2][2] = madd<C>(w * ez, ez, cov[2][2]);
   });
   cov[1][0] = cov[0][1];
   cov[2][0] = cov[0][2];
@@ -978,14 +982,13 @@ void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int o
   const int nc = (cur == 1) ? 3 : 6;
   float AtWA[6][6] = {}, AtWf[6] = {};
   each([&](const float* pk) {
-    const float dx = pk[0] - pi0, dy = pk[1] - pi1, dz = pk[2] - pi2;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float d2 = psgc::sq3<C>(pk[0] - pi0, pk[1] - pi1, pk[2] - pi2);
     if (d2 > r2cut) return;
     const float w = psgc::exp_f32(-d2 * inv_2s2);
     const float ex = pk[0] - cx, ey = pk[1] - cy, ez = pk[2] - cz;
-    const float u = ex * t1x + ey * t1y + ez * t1z;
-    const float v = ex * t2x + ey * t2y + ez * t2z;
-    const float f = ex * nx + ey * ny + ez * nz;
+    const float u = madd<C>(ez, t1z, madd<C>(ey, t1y, ex * t1x));
+    const float v = madd<C>(ez, t2z, madd<C>(ey, t2y, ex * t2x));
+    const float f = madd<C>(ez, nz, madd<C>(ey, ny, ex * nx));
     float basis[6] = {1.0f, u, v, 0, 0, 0};
     if (nc == 6) {
       basis[3] = u * u;
@@ -993,8 +996,8 @@ void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int o
       basis[5] = v * v;
     }
     for (int a = 0; a < nc; ++a) {
-      AtWf[a] += w * basis[a] * f;
-      for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
+      AtWf[a] = madd<C>(w * basis[a], f, AtWf[a]);
+      for (int b = a; b < nc; ++b) AtWA[a][b] = madd<C>(w * basis[a], basis[b], AtWA[a][b]);
     }
   });
   for (int a = 0; a < nc; ++a)
@@ -1025,6 +1028,13 @@ void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int o
   out[0] = cx + u0 * t1x + v0 * t2x + ff * nx;
   out[1] = cy + u0 * t1y + v0 * t2y + ff * ny;
   out[2] = cz + u0 * t1z + v0 * t2z + ff * nz;
+}
+
+template <typename Each>
+void mls_point(const float pi[3], Each&& each, float inv_2s2, float r2cut, int order,
+               int semantics, float out[3]) {
+  if (semantics == PSGC_SEM_GPU) mls_point_t<true>(pi, each, inv_2s2, r2cut, order, semantics, out);
+  else mls_point_t<false>(pi, each, inv_2s2, r2cut, order, semantics, out);
 }
 
 // One MLS iteration over an organized cloud: the window of +-hw pixels,

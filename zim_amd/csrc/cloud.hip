@@ -347,7 +347,8 @@ __global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
 
 // ---------------------------------------------------------- MLS, Gauss ----
 
-template <int NC, int TB, int R>
+// C: the per-neighbour multiply-adds contracted (GPU semantics).
+template <int NC, int TB, int R, bool C>
 __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, float r2, float inv2s2,
                                                        int order, int semantics,
@@ -378,7 +379,7 @@ __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict_
             for (int32_t j = 0; j < len; ++j) {
               const float4 p = tile[j];
               const float pk[3] = {p.x, p.y, p.z};
-              const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+              const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
               if (d2 <= r2) visit(pk, exp_f32(-d2 * inv2s2, etab));
             }
         }
@@ -388,10 +389,7 @@ __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict_
       float wsum = 0.f;
       int nv = 0;
       sweep(active, [&](const float pk[3], float w) {
-        fit.cx += w * pk[0];
-        fit.cy += w * pk[1];
-        fit.cz += w * pk[2];
-        wsum += w;
+        mls_accum_centroid<C>(pk, w, &fit, &wsum);
         nv++;
       });
       const bool go = active && nv >= 3 && !(wsum < 1e-12f);
@@ -403,22 +401,14 @@ __global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict_
         fit.cy *= iw;
         fit.cz *= iw;
       }
-      sweep(go, [&](const float pk[3], float w) {
-        const float ex = pk[0] - fit.cx, ey = pk[1] - fit.cy, ez = pk[2] - fit.cz;
-        cov[0][0] += w * ex * ex;
-        cov[0][1] += w * ex * ey;
-        cov[0][2] += w * ex * ez;
-        cov[1][1] += w * ey * ey;
-        cov[1][2] += w * ey * ez;
-        cov[2][2] += w * ez * ez;
-      });
+      sweep(go, [&](const float pk[3], float w) { mls_accum_cov<C>(fit, pk, w, cov); });
       int cur = 0;
       if (go) {
         mls_frame(cov, &fit);
         cur = mls_order(order, nv, semantics);
       }
       float A[NC][NC] = {}, bv[NC] = {};
-      sweep(go && cur > 0, [&](const float pk[3], float w) { mls_accum_fixed<NC>(fit, pk, w, A, bv); });
+      sweep(go && cur > 0, [&](const float pk[3], float w) { mls_accum_fixed<NC, C>(fit, pk, w, A, bv); });
       if (go) {
         if (cur == 0) mls_plane(fit, pi, o);
         else mls_solve_eval_fixed<NC>(fit, A, bv, cur, pi, o);
@@ -707,19 +697,23 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
       const CellTab t{g.ukey, g.start, g.count, g.ncells};
       // one wave per cell measures fastest for MLS at every density tried
       // (its per-candidate work dwarfs the repeated staging)
-      const int tbv = 64;
-      const int grid = grid_for_cells(ctx, g.ncells, tbv);
+      constexpr int TB = 64;
+      const int grid = grid_for_cells(ctx, g.ncells, TB);
       PSGC_PROF_BEGIN("cloud_mls", s);
-      with_width(tbv, [&](auto w) {
-        constexpr int TB = decltype(w)::value;
-        if (R == 1) {
-          if (ord >= 2) cloud_mls_kernel<6, TB, 1><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-          else cloud_mls_kernel<3, TB, 1><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-        } else {
-          if (ord >= 2) cloud_mls_kernel<6, TB, 2><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-          else cloud_mls_kernel<3, TB, 2><<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
-        }
-      });
+      auto launch = [&](auto nc, auto rc, auto cc) {
+        cloud_mls_kernel<decltype(nc)::value, TB, decltype(rc)::value, decltype(cc)::value>
+            <<<grid, TB, 0, s>>>(g.pts, t, g.f, r2, inv2s2, ord, semantics, dst);
+      };
+      auto by_cells = [&](auto nc, auto cc) {
+        if (R == 1) launch(nc, std::integral_constant<int, 1>{}, cc);
+        else launch(nc, std::integral_constant<int, 2>{}, cc);
+      };
+      auto by_order = [&](auto cc) {
+        if (ord >= 2) by_cells(std::integral_constant<int, 6>{}, cc);
+        else by_cells(std::integral_constant<int, 3>{}, cc);
+      };
+      if (semantics == PSGC_SEM_GPU) by_order(std::true_type{});
+      else by_order(std::false_type{});
       PSGC_LAUNCHED();
       PSGC_PROF_END("cloud_mls", s);
     }

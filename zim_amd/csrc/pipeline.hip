@@ -96,6 +96,13 @@ struct Table {
   int64_t total = 0;
 };
 
+// Semantics bits of a table of MLS jobs.
+int mls_sems(const std::vector<CloudJob>& jobs) {
+  int m = 0;
+  for (const CloudJob& j : jobs) m |= mls_sem_bit(j.mls_sem);
+  return m;
+}
+
 }  // namespace
 
 struct Plan {
@@ -465,7 +472,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
   for (auto& t : pl->mls)
     if (!t.host.empty())
       PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 4), pl->max_mls_hw,
-                                pl->mls_variant, s));
+                                pl->mls_variant, mls_sems(t.host), s));
   return PSGC_OK;
 }
 
@@ -533,7 +540,8 @@ static int autotune(Plan* pl, hipStream_t s) {
                                           pl->max_sor_k, s));
           else
             PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
-                                      pl->mls[0].total, grid, pl->max_mls_hw, shape, s));
+                                      pl->mls[0].total, grid, pl->max_mls_hw, shape,
+                                      mls_sems(pl->mls[0].host), s));
           PSGC_HIP(hipEventRecord(b, s));
           PSGC_HIP(hipEventSynchronize(b));
           float ms = 0;
@@ -954,7 +962,7 @@ int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, flo
     CloudJob* d;
     PSGC_TRY(upload_job(ctx, c, &d, s));
     PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 4), hw,
-                              kMlsPlain, s));
+                              kMlsPlain, mls_sem_bit(semantics), s));
     // the job buffer is reused by the next iteration
     PSGC_HIP(hipStreamSynchronize(s));
     in = out;

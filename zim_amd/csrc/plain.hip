@@ -432,6 +432,8 @@ __global__ void sor_knn_plain_kernel(const float* __restrict__ src, const uint8_
 // One thread per pixel, three passes over the (2hw+1)^2 window read from HBM
 // (structure of mlsKernel, JSLAB_MLS_Filter.cu:165-391; arithmetic of
 // JSLAB_MLS_Filter.cpp:176-389).
+// C: the per-neighbour multiply-adds contracted (GPU semantics).
+template <bool C>
 __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* __restrict__ P,
                                  float* __restrict__ dst, int rows, int cols, int hw,
                                  float inv2s2, int order, float r2cut, int semantics) {
@@ -457,13 +459,10 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
         const size_t q = (size_t)i * cols + j;
         if (P[q] == 0) continue;
         const float* pk = src + q * 3;
-        const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+        const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
         if (d2 > r2cut) continue;
         const float w = exp_f32(-d2 * inv2s2, etab);
-        f.cx += w * pk[0];
-        f.cy += w * pk[1];
-        f.cz += w * pk[2];
-        wsum += w;
+        mls_accum_centroid<C>(pk, w, &f, &wsum);
         nv++;
       }
     if (nv >= 3 && !(wsum < 1e-12f)) {
@@ -477,16 +476,10 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
           const size_t q = (size_t)i * cols + j;
           if (P[q] == 0) continue;
           const float* pk = src + q * 3;
-          const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+          const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
           if (d2 > r2cut) continue;
           const float w = exp_f32(-d2 * inv2s2, etab);
-          const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-          cov[0][0] += w * ex * ex;
-          cov[0][1] += w * ex * ey;
-          cov[0][2] += w * ex * ez;
-          cov[1][1] += w * ey * ey;
-          cov[1][2] += w * ey * ez;
-          cov[2][2] += w * ez * ez;
+          mls_accum_cov<C>(f, pk, w, cov);
         }
       mls_frame(cov, &f);
       const int cur = mls_order(order, nv, semantics);
@@ -504,10 +497,10 @@ __global__ void mls_plain_kernel(const float* __restrict__ src, const uint8_t* _
             const size_t q = (size_t)i * cols + j;
             if (P[q] == 0) continue;
             const float* pk = src + q * 3;
-            const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+            const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
             if (d2 > r2cut) continue;
             const float w = exp_f32(-d2 * inv2s2, etab);
-            mls_accum_poly(f, pk, w, nc, AtWA, AtWf);
+            mls_accum_poly<C>(f, pk, w, nc, AtWA, AtWf);
           }
         mls_solve_eval(f, AtWA, AtWf, cur, pi, out);
       }
@@ -870,8 +863,13 @@ size_t sor_partial_doubles(size_t npix) {
 int launch_mls(const float* src, const uint8_t* P, float* dst, int rows, int cols, int hw,
                float sigma, int order, float r2cut, int semantics, hipStream_t s) {
   PSGC_PROF_BEGIN("mls", s);
-  mls_plain_kernel<<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(
-      src, P, dst, rows, cols, hw, 1.0f / (2.0f * sigma * sigma), order, r2cut, semantics);
+  const float inv2s2 = 1.0f / (2.0f * sigma * sigma);
+  if (semantics == PSGC_SEM_GPU)
+    mls_plain_kernel<true><<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(
+        src, P, dst, rows, cols, hw, inv2s2, order, r2cut, semantics);
+  else
+    mls_plain_kernel<false><<<grid2d(rows, cols), dim3(TB, TB), 0, s>>>(
+        src, P, dst, rows, cols, hw, inv2s2, order, r2cut, semantics);
   PSGC_LAUNCHED();
   PSGC_PROF_END("mls", s);
   return PSGC_OK;

@@ -302,11 +302,9 @@ __device__ inline uint8_t reproject_poly(const float4 cx, const float4 cy, const
 
 // Squared distance in f32, evaluated as dx*dx + dy*dy + dz*dz
 // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:171-174).
+template <bool C = false>
 __device__ inline float dist2(float px, float py, float pz, const float* q) {
-  const float dx = px - q[0];
-  const float dy = py - q[1];
-  const float dz = pz - q[2];
-  return dx * dx + dy * dy + dz * dz;
+  return sq3<C>(px - q[0], py - q[1], pz - q[2]);
 }
 
 // ------------------------------------------------------------------- MLS --
@@ -498,12 +496,13 @@ __device__ inline void mls_solve_eval(const MlsFit& f, float AtWA[6][6], float A
 }
 
 // Accumulates one neighbour pk (weight w) into the normal equations.
+template <bool C>
 __device__ inline void mls_accum_poly(const MlsFit& f, const float* pk, float w, int nc,
                                       float AtWA[6][6], float AtWf[6]) {
   const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-  const float u = ex * f.t1[0] + ey * f.t1[1] + ez * f.t1[2];
-  const float v = ex * f.t2[0] + ey * f.t2[1] + ez * f.t2[2];
-  const float fv = ex * f.n[0] + ey * f.n[1] + ez * f.n[2];
+  const float u = madd<C>(ez, f.t1[2], madd<C>(ey, f.t1[1], ex * f.t1[0]));
+  const float v = madd<C>(ez, f.t2[2], madd<C>(ey, f.t2[1], ex * f.t2[0]));
+  const float fv = madd<C>(ez, f.n[2], madd<C>(ey, f.n[1], ex * f.n[0]));
   float basis[6] = {1.0f, u, v, 0.f, 0.f, 0.f};
   if (nc == 6) {
     basis[3] = u * u;
@@ -511,8 +510,9 @@ __device__ inline void mls_accum_poly(const MlsFit& f, const float* pk, float w,
     basis[5] = v * v;
   }
   for (int a = 0; a < nc; ++a) {
-    AtWf[a] += w * basis[a] * fv;
-    for (int b = a; b < nc; ++b) AtWA[a][b] += w * basis[a] * basis[b];
+    const float wa = w * basis[a];
+    AtWf[a] = madd<C>(wa, fv, AtWf[a]);
+    for (int b = a; b < nc; ++b) AtWA[a][b] = madd<C>(wa, basis[b], AtWA[a][b]);
   }
 }
 
@@ -563,21 +563,44 @@ __device__ inline bool chol_solve_fixed(const float (&A)[NA][NA], const float (&
 // One neighbour into the NC-term normal equations. For NC = 6 the leading
 // 3 x 3 block receives exactly the terms of the NC = 3 system, so a point
 // whose order fell back to 1 solves that block.
-template <int NC>
+template <int NC, bool C>
 __device__ inline void mls_accum_fixed(const MlsFit& f, const float pk[3], float w,
                                        float (&A)[NC][NC], float (&bv)[NC]) {
   const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-  const float u = ex * f.t1[0] + ey * f.t1[1] + ez * f.t1[2];
-  const float v = ex * f.t2[0] + ey * f.t2[1] + ez * f.t2[2];
-  const float fv = ex * f.n[0] + ey * f.n[1] + ez * f.n[2];
+  const float u = madd<C>(ez, f.t1[2], madd<C>(ey, f.t1[1], ex * f.t1[0]));
+  const float v = madd<C>(ez, f.t2[2], madd<C>(ey, f.t2[1], ex * f.t2[0]));
+  const float fv = madd<C>(ez, f.n[2], madd<C>(ey, f.n[1], ex * f.n[0]));
   const float basis[6] = {1.0f, u, v, u * u, u * v, v * v};
 #pragma unroll
   for (int a = 0; a < NC; ++a) {
     const float wa = w * basis[a];  // w * basis[a] * x evaluates as (w * basis[a]) * x
-    bv[a] += wa * fv;
+    bv[a] = madd<C>(wa, fv, bv[a]);
 #pragma unroll
-    for (int b = a; b < NC; ++b) A[a][b] += wa * basis[b];
+    for (int b = a; b < NC; ++b) A[a][b] = madd<C>(wa, basis[b], A[a][b]);
   }
+}
+
+// Centroid and covariance terms of one neighbour (weights w), as the first
+// two passes accumulate them.
+template <bool C>
+__device__ inline void mls_accum_centroid(const float pk[3], float w, MlsFit* f, float* wsum) {
+  f->cx = madd<C>(w, pk[0], f->cx);
+  f->cy = madd<C>(w, pk[1], f->cy);
+  f->cz = madd<C>(w, pk[2], f->cz);
+  *wsum += w;
+}
+
+template <bool C>
+__device__ inline void mls_accum_cov(const MlsFit& f, const float pk[3], float w,
+                                     float (&cov)[3][3]) {
+  const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
+  const float wx = w * ex, wy = w * ey, wz = w * ez;
+  cov[0][0] = madd<C>(wx, ex, cov[0][0]);
+  cov[0][1] = madd<C>(wx, ey, cov[0][1]);
+  cov[0][2] = madd<C>(wx, ez, cov[0][2]);
+  cov[1][1] = madd<C>(wy, ey, cov[1][1]);
+  cov[1][2] = madd<C>(wy, ez, cov[1][2]);
+  cov[2][2] = madd<C>(wz, ez, cov[2][2]);
 }
 
 // mls_solve_eval for an NC-term accumulation (NC = 6 serves orders 1 and 2).

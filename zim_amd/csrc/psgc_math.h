@@ -108,6 +108,21 @@ PSGC_HD inline double fma_rn(double a, double b, double c) {
 #endif
 }
 
+// a * b + c in FP32: with C one fused multiply-add (a single rounding, what a
+// compiler that contracts multiply-adds emits), else a rounded product and a
+// rounded sum.
+template <bool C>
+PSGC_HD inline float madd(float a, float b, float c) {
+  if constexpr (C) return __builtin_fmaf(a, b, c);
+  else return a * b + c;
+}
+
+// dx^2 + dy^2 + dz^2 summed left to right, contracted with C.
+template <bool C>
+PSGC_HD inline float sq3(float dx, float dy, float dz) {
+  return madd<C>(dz, dz, madd<C>(dy, dy, dx * dx));
+}
+
 // 2^(j/64), j = 0..63, each the double nearest to the exact value.
 constexpr double kExp2Tab64[64] = {
     0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,

@@ -128,8 +128,13 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
 // are evaluated again). Results are identical.
 enum { kMlsPlain = 0, kMlsCached = 1 };
 constexpr int kMlsCachedMaxHalf = 6;
+// Semantics present among the MLS jobs of a launch (a bit each): CPU
+// semantics accumulate with rounded products and sums, GPU semantics with
+// fused multiply-adds.
+enum { kSemBitCpu = 1, kSemBitGpu = 2 };
+inline int mls_sem_bit(int semantics) { return semantics == PSGC_SEM_GPU ? kSemBitGpu : kSemBitCpu; }
 int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
-                     int variant, hipStream_t s);
+                     int variant, int sems, hipStream_t s);
 
 // Shared-memory bytes of the row-match kernel for a given width and order count.
 size_t match_tiled_lds(int cols, int knum);

@@ -652,7 +652,7 @@ constexpr int MKC = 24;
 // in order, with their weights: from the cache (entries kc, kc + 1, ... of
 // this thread's column) when CACHED, else the pass-1 expression evaluated
 // again (bit-identical either way).
-template <bool CACHED, int LD, typename F>
+template <bool CACHED, int LD, bool C, typename F>
 __device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0, int rx0,
                                 const float4 (*T)[LD + 1], const float (*Wc)[256], int& kc,
                                 const double* etab, const float pi[3], float inv2s2, F&& visit) {
@@ -665,7 +665,7 @@ __device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0,
       const float pk[3] = {e4.x, e4.y, e4.z};
       float w;
       if (CACHED) w = Wc[kc][threadIdx.x];
-      else w = exp_f32(-dist2(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
+      else w = exp_f32(-dist2<C>(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
       ++kc;
       visit(pk, w);
     }
@@ -673,14 +673,14 @@ __device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0,
 }
 
 // All in-radius entries of the window in order: rows [0, dyc) from the cache.
-template <int LD, typename F>
+template <int LD, bool C, typename F>
 __device__ inline void mls_entries(const uint32_t* bits, int dyc, int Wd, int ry0, int rx0,
                                    const float4 (*T)[LD + 1], const float (*Wc)[256],
                                    const double* etab, const float pi[3], float inv2s2,
                                    F&& visit) {
   int kc = 0;
-  if (dyc > 0) mls_rows<true, LD>(bits, 0, dyc, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
-  mls_rows<false, LD>(bits, dyc, Wd, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
+  if (dyc > 0) mls_rows<true, LD, C>(bits, 0, dyc, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
+  mls_rows<false, LD, C>(bits, dyc, Wd, ry0, rx0, T, Wc, kc, etab, pi, inv2s2, visit);
 }
 
 __device__ inline int wave_min(int v) {
@@ -690,8 +690,10 @@ __device__ inline int wave_min(int v) {
 }
 
 // HM: largest half window served; KC: weight-cache entries per thread (0:
-// every pass evaluates its weights).
-template <int HM, int KC>
+// every pass evaluates its weights); C: the per-neighbour multiply-adds
+// contracted (GPU semantics); tiles of jobs of the other semantics are left
+// to the launch for those.
+template <int HM, int KC, bool C>
 __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restrict__ jobs,
                                                         int njobs, int64_t total) {
   constexpr int LD = CT + 2 * HM;
@@ -707,6 +709,7 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
     const int j = find_job(jobs, njobs, t, jprev);
     jprev = j;
     const CloudJob& cj = jobs[j];
+    if ((cj.mls_sem == PSGC_SEM_GPU) != C) continue;  // the same tile for the whole workgroup
     const int64_t lt = t - cj.tile_base;
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
     const int hw = cj.mls_hw;
@@ -723,12 +726,13 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
     const float pi[3] = {q.x, q.y, q.z};
     float out[3] = {pi[0], pi[1], pi[2]};
     // bits[dy] bit dx: window entry (dy - hw, dx - hw) is a valid point
-    // within the cutoff radius (the same uncontracted d^2 test as the CPU
-    // restatement, so the neighbour set and every order fallback decision
-    // are identical). The three accumulation passes keep the CPU filter's
-    // arithmetic (JSLAB_MLS_Filter.cpp:196-340): weighted centroid,
-    // covariance about it, then the normal equations; an ill-conditioned fit
-    // amplifies any reassociation far beyond the 1e-3 mm output tolerance.
+    // within the cutoff radius (the d^2 test of the CPU restatement, so the
+    // neighbour set and every order fallback decision are identical). The
+    // three accumulation passes keep the filter's arithmetic
+    // (JSLAB_MLS_Filter.cpp:196-340, .cu:214-343): weighted centroid,
+    // covariance about it, then the normal equations, with the same
+    // contraction as the restatement; an ill-conditioned fit amplifies any
+    // reassociation far beyond the 1e-3 mm output tolerance.
     uint32_t bits[MWD];
 #pragma unroll
     for (int w = 0; w < MWD; ++w) bits[w] = 0u;
@@ -755,15 +759,12 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             // is computed and discarded)
             const float4 e4 = T[ry][cx - hw + dx];
             const float pk[3] = {e4.x, e4.y, e4.z};
-            const float d2 = dist2(pi[0], pi[1], pi[2], pk);
+            const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
             if ((e4.w != 0.f) & (d2 <= r2)) {
               row_bits |= 1u << dx;
               const float w = exp_f32(-d2 * inv2s2, etab);
               if (KC > 0 && nv < KC) Wc[nv][threadIdx.x] = w;
-              f.cx += w * pk[0];
-              f.cy += w * pk[1];
-              f.cz += w * pk[2];
-              wsum += w;
+              mls_accum_centroid<C>(pk, w, &f, &wsum);
               nv++;
             }
           }
@@ -783,29 +784,21 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
       f.cy *= iw;
       f.cz *= iw;
       float cov[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-      mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
-                      [&](const float pk[3], float w) {
-                        const float ex = pk[0] - f.cx, ey = pk[1] - f.cy, ez = pk[2] - f.cz;
-                        cov[0][0] += w * ex * ex;
-                        cov[0][1] += w * ex * ey;
-                        cov[0][2] += w * ex * ez;
-                        cov[1][1] += w * ey * ey;
-                        cov[1][2] += w * ey * ez;
-                        cov[2][2] += w * ez * ez;
-                      });
+      mls_entries<LD, C>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                         [&](const float pk[3], float w) { mls_accum_cov<C>(f, pk, w, cov); });
       mls_frame(cov, &f);
       const int cur = mls_order(cj.mls_order, nv, cj.mls_sem);
       if (cur == 0) {
         mls_plane(f, pi, out);
       } else if (cj.mls_order >= 2) {  // uniform per tile
         float A[6][6] = {}, bv[6] = {};
-        mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
-                        [&](const float pk[3], float w) { mls_accum_fixed<6>(f, pk, w, A, bv); });
+        mls_entries<LD, C>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                           [&](const float pk[3], float w) { mls_accum_fixed<6, C>(f, pk, w, A, bv); });
         mls_solve_eval_fixed<6>(f, A, bv, cur, pi, out);
       } else {
         float A[3][3] = {}, bv[3] = {};
-        mls_entries<LD>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
-                        [&](const float pk[3], float w) { mls_accum_fixed<3>(f, pk, w, A, bv); });
+        mls_entries<LD, C>(bits, dyc, Wd, cy - hw, cx - hw, T, Wc, etab, pi, inv2s2,
+                           [&](const float pk[3], float w) { mls_accum_fixed<3, C>(f, pk, w, A, bv); });
         mls_solve_eval_fixed<3>(f, A, bv, cur, pi, out);
       }
     }
@@ -906,7 +899,7 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
 }
 
 int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
-                     int variant, hipStream_t s) {
+                     int variant, int sems, hipStream_t s) {
   if (total <= 0) return PSGC_OK;
   if (max_hw > MH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "MLS half window above %d", MH_MAX);
   if (variant != kMlsPlain && variant != kMlsCached)
@@ -914,11 +907,18 @@ int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, i
   if (variant == kMlsCached && max_hw > MH_CACHED)
     PSGC_FAIL(PSGC_E_UNSUPPORTED, "cached MLS variant: half window above %d", MH_CACHED);
   PSGC_PROF_BEGIN("mls", s);
-  if (variant == kMlsCached)
-    mls_tiled_kernel<MH_CACHED, MKC><<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
-  else
-    mls_tiled_kernel<MH_MAX, 0><<<round_grid(total, grid), 256, 0, s>>>(jobs, njobs, total);
-  PSGC_LAUNCHED();
+  const int g = round_grid(total, grid);
+  // one launch per semantics present (each skips the other's tiles)
+  if (sems & kSemBitCpu) {
+    if (variant == kMlsCached) mls_tiled_kernel<MH_CACHED, MKC, false><<<g, 256, 0, s>>>(jobs, njobs, total);
+    else mls_tiled_kernel<MH_MAX, 0, false><<<g, 256, 0, s>>>(jobs, njobs, total);
+    PSGC_LAUNCHED();
+  }
+  if (sems & kSemBitGpu) {
+    if (variant == kMlsCached) mls_tiled_kernel<MH_CACHED, MKC, true><<<g, 256, 0, s>>>(jobs, njobs, total);
+    else mls_tiled_kernel<MH_MAX, 0, true><<<g, 256, 0, s>>>(jobs, njobs, total);
+    PSGC_LAUNCHED();
+  }
   PSGC_PROF_END("mls", s);
   return PSGC_OK;
 }
