@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -496,7 +497,9 @@ def run_c5(args):
     Gaussian (sigma 0.5 mm, r = 1.0 mm) over it, each timed with the grid
     build included. The kNN voxel comes from a timing sweep (the autotuner);
     the radius filters run on cells of the radius and of half of it, and
-    the faster of the two counts in the total."""
+    the faster of the two counts in the total. Before the filters, one step
+    of the per-view calibration refinement (batched triangulation, plane-fit
+    loss, batched adjoint) over the 16 views is timed on its own."""
     import torch
 
     import zim_amd
@@ -509,19 +512,53 @@ def run_c5(args):
     dcfg = scenes.c2_decode_config()
     rcfg = scenes.c2_recon_config(use_mls=0, outlier_mode=0)
     t0 = time.time()
-    parts = []
+    parts, views = [], []
     for v in range(args.c5_views):
         p = synth.render_c2(2000, device=dev, noise_seed=3000 + v)
         L = zim_amd.decode(dcfg, p.left, ctx)
         R = zim_amd.decode(dcfg, p.right, ctx)
-        xyz, P, _, _ = zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1, ctx)
+        xyz, P, _, mc = zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1, ctx)
         parts.append(xyz.reshape(-1, 3)[P.reshape(-1) != 0].clone())
+        views.append((L[0].contiguous(), R[0].contiguous(), mc.contiguous()))
+        cam = p.cam
         del p, L, R, xyz, P
     cloud = torch.cat(parts).contiguous()
     del parts
     torch.cuda.synchronize()
     n = cloud.shape[0]
     log(f"{args.c5_views} views merged: {n} points in {time.time() - t0:.1f}s")
+
+    # one step of the per-view calibration refinement: the batched FP64
+    # triangulation of every view under its own perturbed calibration
+    # (focal +-1 %, baseline +-0.5 %, principal points +-3 px, rotation
+    # +-0.05 deg; seeded), a plane-fit loss per view, and the batched adjoint
+    from zim_amd import autograd as zag
+    g = torch.Generator().manual_seed(5)
+    V = len(views)
+    th0 = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy, cam.baseline, 0, 0, 0],
+                       dtype=torch.float64)
+    scale = torch.tensor([0.01 * cam.f, 3, 3, 3, 3, 0.005 * cam.baseline] +
+                         [math.radians(0.05)] * 3, dtype=torch.float64)
+    thetas = (th0 + (2 * torch.rand(V, 9, generator=g, dtype=torch.float64) - 1) * scale)
+    nominal = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy], dtype=torch.float64)
+
+    def refine_step():
+        th = thetas.clone().requires_grad_(True)
+        outs = zag.triangulate_views(th, [v[0] for v in views], [v[1] for v in views],
+                                     [v[2] for v in views], nominal, ctx)
+        loss = sum(zag.plane_loss(x, m) for x, m in outs)
+        loss.backward()
+        torch.cuda.synchronize()
+        return th.grad
+
+    refine_step()
+    t = time.perf_counter()
+    for _ in range(3):
+        grad = refine_step()
+    adj_ms = (time.perf_counter() - t) * 1e3 / 3
+    del views
+    log(f"batched adjoint step over {V} views: {adj_ms:.1f} ms "
+        f"(|grad| finite: {bool(torch.isfinite(grad).all())})")
 
     def timed(fn, reps=3):
         fn()
@@ -557,6 +594,8 @@ def run_c5(args):
             "value": round(n / (total * 1e-3) / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1,
             "points": n, "views": args.c5_views, "voxel_knn": voxel, "voxel_sweep_ms": tune,
             "kept_after_sor": int(keep.sum().item()), "sor_stats": [float(x) for x in st.cpu()],
+            "refine_step_ms": round(adj_ms, 3),
+            "refine_step_mpx_per_s": round(V * 2048 * 2448 / (adj_ms * 1e-3) / 1e6, 3),
             **{k: round(v, 3) for k, v in res.items()}, "dtype": "f32",
             "data": "synthetic (16 noisy views of one seeded bump height field)"}
     print(json.dumps(line), flush=True)

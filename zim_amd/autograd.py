@@ -81,22 +81,31 @@ def triangulate_views(thetas, phLs, phRs, match_cols, nominal, ctx: Context | No
     return out
 
 
+def _scatter(xyz, valid):
+    """3 x 3 scatter matrix of the valid points about their mean, from
+    masked reductions over every point (no compaction, so the backward pass
+    is elementwise too; a library GEMM of shape 3 x N x 3 runs ~1000x slower
+    on millions of points)."""
+    m = valid.reshape(-1).bool()
+    p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
+    mu = p.sum(0) / m.sum()
+    x, y, z = torch.where(m[:, None], p - mu, 0.0).unbind(1)
+    sxx, sxy, sxz = (x * x).sum(), (x * y).sum(), (x * z).sum()
+    syy, syz, szz = (y * y).sum(), (y * z).sum(), (z * z).sum()
+    return torch.stack([torch.stack([sxx, sxy, sxz]), torch.stack([sxy, syy, syz]),
+                        torch.stack([sxz, syz, szz])])
+
+
 def plane_loss(xyz, valid):
     """Sum of squared residuals of the valid points to their least-squares
     plane: the smallest eigenvalue of the scatter matrix."""
-    m = valid.bool()
-    pts = xyz[m]
-    e = pts - pts.mean(0)
-    return torch.linalg.eigvalsh(e.T @ e)[0]
+    return torch.linalg.eigvalsh(_scatter(xyz, valid))[0]
 
 
 def plane_flatness(xyz, valid):
     """Scale-invariant flatness: smallest eigenvalue over the trace of the
     scatter matrix of the valid points."""
-    m = valid.bool()
-    pts = xyz[m]
-    e = pts - pts.mean(0)
-    ev = torch.linalg.eigvalsh(e.T @ e)
+    ev = torch.linalg.eigvalsh(_scatter(xyz, valid))
     return ev[0] / ev.sum()
 
 
