@@ -55,6 +55,9 @@ def parse():
                     help="instead: one batch of 64 uneven pairs (640x480 to 2448x2048; "
                          "3-, 4- or 8-step three-frequency fringes or a Gray-code schedule), "
                          "batched against the same pairs one at a time")
+    ap.add_argument("--c4", action="store_true",
+                    help="instead: the differentiation workload at full size (1280x1024 plane "
+                         "target): adjoint timing, central-difference check, refinement")
     ap.add_argument("--ref-timing", action="store_true",
                     help="instead: the configuration of the reference's own published GPU "
                          "timing (5120x5120 8-bit, 107/100/94 x 8 steps, MLS off), one pair at "
@@ -102,6 +105,8 @@ def main():
         return run_c5(args)
     if args.c3:
         return run_c3(args)
+    if args.c4:
+        return run_c4(args)
     if args.ref_timing:
         return run_ref_timing(args)
 
@@ -374,6 +379,98 @@ def run_ref_timing(args):
                                  for k, v in res.items()},
             "dtype": "f64", "data": "synthetic (seeded bump height field, 8-bit frames)"}
     log(f"times {res}")
+    print(json.dumps(line), flush=True)
+
+
+def run_c4(args):
+    """Differentiation at full size: a 1280x1024 pair of a flat plane target
+    at 700 mm rendered with the true calibration (f = 1600 px, B = 120 mm,
+    three-frequency 4-step fringes, uint16), decoded and matched on the GPU.
+    Under a seeded perturbed calibration (focal +-1 %, baseline +-0.5 %,
+    principal points +-3 px, rectification rotation +-0.05 deg) the FP64
+    triangulation, the plane-fit loss and its adjoint (autograd Function
+    over the hand-written kernels) are timed; the gradient is checked
+    against central differences (h = 1e-4, 1e-5, 1e-6 in units of those
+    scales); then a plane refinement runs from the perturbed calibration."""
+    import numpy as np
+    import torch
+
+    import zim_amd
+    from zim_amd import autograd as zag
+    from zim_amd import synth
+    from tests import scenes
+
+    log = lambda m: print(f"[bench-c4] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    dev = torch.device("cuda:0")
+    ctx = zim_amd.Context(0)
+    p = synth.render_c4(4, device=dev)
+    dcfg = scenes.c2_decode_config()
+    L = zim_amd.decode(dcfg, p.left, ctx)
+    R = zim_amd.decode(dcfg, p.right, ctx)
+    rcfg = zim_amd.ReconConfig.default_gpu(use_mls=0, outlier_mode=0)
+    _, _, _, mc = zim_amd.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, p.Q, p.R1, ctx)
+    phL, phR, mc = L[0].contiguous(), R[0].contiguous(), mc.contiguous()
+    cam = p.cam
+    theta0 = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy, cam.baseline, 0, 0, 0],
+                          dtype=torch.float64)
+    nominal = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy], dtype=torch.float64)
+    g = np.random.default_rng(7)
+    d = np.zeros(9)
+    d[0] = cam.f * 0.01 * g.uniform(-1, 1)
+    d[1:5] = 3.0 * g.uniform(-1, 1, 4)
+    d[5] = cam.baseline * 0.005 * g.uniform(-1, 1)
+    d[6:9] = math.radians(0.05) * g.uniform(-1, 1, 3)
+    th_p = theta0 + torch.from_numpy(d)
+    scale = zag.default_scale(theta0)
+
+    def fwd_bwd():
+        th = th_p.clone().requires_grad_(True)
+        xyz, valid = zag.triangulate(th, phL, phR, mc, nominal, ctx)
+        loss = zag.plane_loss(xyz, valid)
+        loss.backward()
+        torch.cuda.synchronize()
+        return th.grad, int(valid.sum().item())
+
+    fwd_bwd()
+    t = time.perf_counter()
+    for _ in range(5):
+        grad, nvalid = fwd_bwd()
+    step_ms = (time.perf_counter() - t) * 1e3 / 5
+    grad = grad * scale
+    # central differences at three steps: where the loss curves on the scale
+    # of the step, the error falls as h^2 (the adjoint is the derivative the
+    # differences converge to)
+    rels = {}
+    for h in (1e-4, 1e-5, 1e-6):
+        fd = torch.zeros(9, dtype=torch.float64)
+        with torch.no_grad():
+            for k in range(9):
+                e = torch.zeros(9, dtype=torch.float64)
+                e[k] = h * scale[k]
+                lp = zag.plane_loss(*zag.triangulate(th_p + e, phL, phR, mc, nominal, ctx))
+                lm = zag.plane_loss(*zag.triangulate(th_p - e, phL, phR, mc, nominal, ctx))
+                fd[k] = (lp - lm) / (2 * h)
+        rels[h] = (grad - fd).abs() / grad.abs().clamp_min(1e-3 * grad.abs().max())
+        log(f"h = {h:g}: FD rel err per parameter {[f'{x:.1e}' for x in rels[h].tolist()]}")
+    rel = rels[1e-4]
+    log(f"forward+backward {step_ms:.2f} ms over {nvalid} valid points")
+    t = time.perf_counter()
+    theta, hist = zag.refine_plane(th_p, theta0, phL, phR, L[1].contiguous(), R[1].contiguous(),
+                                   nominal, 70, outer=4, inner=4, ctx=ctx)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t
+    log(f"refinement: flatness {hist[0][0]:.3e} -> {hist[-1][1]:.3e} in {ref_s:.2f} s")
+    npx = cam.rows * cam.cols
+    line = {"metric": "calibration adjoint: forward + backward ms per 1280x1024 pair, and its "
+                      "agreement with central differences",
+            "value": round(step_ms, 3), "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+            "mpx_per_s": round(npx / (step_ms * 1e-3) / 1e6, 3), "valid_points": nvalid,
+            "fd_max_rel_err_by_h": {f"{h:g}": float(r.max()) for h, r in rels.items()},
+            "fd_rel_err_by_h": {f"{h:g}": [float(x) for x in r] for h, r in rels.items()},
+            "refine": {"outer": 4, "inner": 4, "seconds": round(ref_s, 3),
+                       "flatness_start": float(hist[0][0]), "flatness_end": float(hist[-1][1])},
+            "dtype": "f64",
+            "data": "synthetic (seeded plane target; calibration perturbed with a seeded draw)"}
     print(json.dumps(line), flush=True)
 
 

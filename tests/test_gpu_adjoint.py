@@ -24,6 +24,9 @@ from tests import adjoint_ref, scenes
 pytestmark = pytest.mark.gpu
 
 
+_ORDERS = {}   # fringe orders of each rendered plane pair, by (rows, cols, seed)
+
+
 @functools.lru_cache(maxsize=None)
 def plane_pair(rows=256, cols=320, seed=4):
     f = 1600.0 * cols / 1280.0
@@ -39,8 +42,15 @@ def plane_pair(rows=256, cols=320, seed=4):
     theta0 = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy, cam.baseline, 0, 0, 0],
                           dtype=torch.float64)
     nominal = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy], dtype=torch.float64)
-    plane_pair.orders = (L[1].contiguous(), R[1].contiguous())
+    _ORDERS[(rows, cols, seed)] = (L[1].contiguous(), R[1].contiguous())
     return ctx, L[0].contiguous(), R[0].contiguous(), mc.contiguous(), theta0, nominal
+
+
+def plane_orders(rows=256, cols=320, seed=4):
+    """Fringe orders (left, right) of plane_pair(rows, cols, seed); cached
+    with it, so they belong to the same pair whatever was called between."""
+    plane_pair(rows, cols, seed)
+    return _ORDERS[(rows, cols, seed)]
 
 
 def perturbed(theta0, seed=7):
@@ -70,7 +80,7 @@ def test_forward_at_true_calibration_equals_pipeline_disparity():
     import ctypes as C
     from zim_amd import _capi
     ctx, phL, phR, mc, theta0, nominal = plane_pair()
-    kL, kR = plane_pair.orders
+    kL, kR = plane_orders()
     xyz, valid = zag.triangulate(theta0, phL, phR, mc, nominal, ctx)
     rows, cols = phL.shape
     disp = torch.empty((rows, cols), dtype=torch.float64, device="cuda")
@@ -136,12 +146,44 @@ def test_plane_loss_adjoint_vs_central_differences():
     assert rel.max().item() <= 1e-4, rel
 
 
+def test_full_size_adjoint_is_the_limit_of_central_differences():
+    """At the differentiation workload's full 1280 x 1024 size the plane
+    loss curves on the scale of h = 1e-4 along focal length, the right
+    principal point and the rotation about x (a few points' sub-pixel roots
+    are ill-conditioned), so central differences there are off by a few
+    percent; their error falls as h^2, the signature of a smooth function
+    whose derivative is the adjoint, and at h = 1e-6 every parameter agrees
+    within 1e-4."""
+    ctx, phL, phR, mc, theta0, nominal = plane_pair(rows=1024, cols=1280)
+    scale = zag.default_scale(theta0)
+    thp = perturbed(theta0)
+    th = thp.clone().requires_grad_(True)
+    zag.plane_loss(*zag.triangulate(th, phL, phR, mc, nominal, ctx)).backward()
+    grad = th.grad.clone() * scale
+    rel = {}
+    with torch.no_grad():
+        for h in (1e-4, 1e-5, 1e-6):
+            fd = torch.zeros(9, dtype=torch.float64)
+            for k in range(9):
+                e = torch.zeros(9, dtype=torch.float64)
+                e[k] = h * scale[k]
+                lp = zag.plane_loss(*zag.triangulate(thp + e, phL, phR, mc, nominal, ctx))
+                lm = zag.plane_loss(*zag.triangulate(thp - e, phL, phR, mc, nominal, ctx))
+                fd[k] = (lp - lm) / (2 * h)
+            rel[h] = (grad - fd).abs() / grad.abs().clamp_min(1e-3 * grad.abs().max())
+            print("h", h, "rel", rel[h].tolist())
+    assert rel[1e-6].max().item() <= 1e-4, rel[1e-6]
+    for k in range(9):
+        if rel[1e-4][k] > 1e-3:   # curvature-dominated: ~100x smaller per decade of h
+            assert rel[1e-5][k] < 0.03 * rel[1e-4][k], (k, rel[1e-4][k], rel[1e-5][k])
+
+
 def test_plane_refinement_reduces_flatness():
     """Refinement from the perturbed calibration: each outer step re-rectifies
     and re-matches on the GPU, then descends the flatness objective along
     the adjoint gradient; the objective must decrease."""
     ctx, phL, phR, mc, theta0, nominal = plane_pair()
-    kL, kR = plane_pair.orders
+    kL, kR = plane_orders()
     th = perturbed(theta0)
     theta, hist = zag.refine_plane(th, theta0, phL, phR, kL, kR, nominal, 70, outer=4, inner=4,
                                    ctx=ctx)

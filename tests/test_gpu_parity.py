@@ -346,7 +346,9 @@ def test_batch_mls_variants_match_oracle(ctx):
     """Both MLS kernels, pinned in turn on one captured batch: weights
     evaluated in every pass, and weights kept in LDS between the passes
     (points with more than 24 in-radius neighbours evaluate the rest
-    again); each against the CPU restatement."""
+    again); each against the CPU restatement. The batch holds the pair
+    twice, under GPU semantics (fused sums) and CPU semantics (rounded
+    products and sums), so each MLS launch serves one of them."""
     dcfg = scenes.c2_decode_config()
     # 13 x 13 window; a 2.5 mm radius on this ~1 m deep scene gives about
     # half the points more than 24 neighbours (the full-size 5 MP scene holds
@@ -360,14 +362,20 @@ def test_batch_mls_variants_match_oracle(ctx):
     cnt = _inradius_counts(nx, nP, 6, 2.5)
     print("in-radius neighbours: max", cnt.max(), "points above 24:", int((cnt > 24).sum()))
     assert (cnt > 24).sum() > 100 and ((cnt > 3) & (cnt <= 24)).sum() > 100
-    b = zim_amd.Batch([dict(left=L, right=R, decode=dcfg, recon=rcfg, pha_f=70, Q=p.Q, R1=p.R1)],
-                      ctx, graph=True)
+    ccfg = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0,
+                                  semantics=_capi.SEM_CPU)
+    cx, cP, _, _ = oracle.reconstruct(ccfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
+    m = (rP != 0) & (cP != 0)
+    assert not np.array_equal(rx[m], cx[m])   # the two roundings differ somewhere
+    b = zim_amd.Batch([dict(left=L, right=R, decode=dcfg, recon=c, pha_f=70, Q=p.Q, R1=p.R1)
+                       for c in (rcfg, ccfg)], ctx, graph=True)
     for variant in (1, 0, 1):
         b.set_grid("mls_variant", variant)
-        out = b.run()[0]
+        outs = b.run()
         torch.cuda.synchronize()
-        assert np.array_equal(host(out["P"]), rP)
-        assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL
+        for out, (ex, eP) in zip(outs, ((rx, rP), (cx, cP))):
+            assert np.array_equal(host(out["P"]), eP)
+            assert np.array_equal(host(out["xyz"]), ex), np.abs(host(out["xyz"]) - ex).max()
     b.close()
 
 

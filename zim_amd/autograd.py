@@ -98,14 +98,25 @@ def _scatter(xyz, valid):
 
 def plane_loss(xyz, valid):
     """Sum of squared residuals of the valid points to their least-squares
-    plane: the smallest eigenvalue of the scatter matrix."""
-    return torch.linalg.eigvalsh(_scatter(xyz, valid))[0]
+    plane (the smallest eigenvalue of the scatter matrix), summed from the
+    residuals n . (x - mean) themselves, n the plane normal: the eigenvalue
+    read off the scatter matrix carries an absolute error of ~1e-16 times
+    its largest eigenvalue (the plane's extent squared, times the point
+    count), which swamps finite differences of the loss. The loss is
+    stationary in n, so n needs no gradient: d loss / d x is the residual's."""
+    m = valid.reshape(-1).bool()
+    p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
+    mu = p.sum(0) / m.sum()
+    with torch.no_grad():   # a 3 x 3 eigenproblem: on the host
+        n = torch.linalg.eigh(_scatter(xyz, valid).cpu())[1][:, 0].to(xyz.device)
+    r = torch.where(m, (p - mu) @ n, 0.0)
+    return (r * r).sum()
 
 
 def plane_flatness(xyz, valid):
     """Scale-invariant flatness: smallest eigenvalue over the trace of the
     scatter matrix of the valid points."""
-    ev = torch.linalg.eigvalsh(_scatter(xyz, valid))
+    ev = torch.linalg.eigvalsh(_scatter(xyz, valid).cpu())   # 3 x 3: on the host
     return ev[0] / ev.sum()
 
 
