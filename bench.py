@@ -91,8 +91,8 @@ VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pairs shard "
-              "across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pairs would "
+              "shard across GPUs with no exchange, one process per GPU)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -708,15 +708,19 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
     right = [f[:sample_rows].cpu().numpy() for f in pair.right]
     log(f"cpu baseline: one pair, rows [0,{sample_rows}), {cores} threads")
     oracle.decode(dcfg, [f[:8] for f in left])  # load + thread pool warm-up
-    t0 = time.perf_counter()
-    L = oracle.decode(dcfg, left)
-    R = oracle.decode(dcfg, right)
-    rx, rP, _, _ = oracle.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, pair.Q, pair.R1)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(5):   # median of five runs of the whole sample
+        t0 = time.perf_counter()
+        L = oracle.decode(dcfg, left)
+        R = oracle.decode(dcfg, right)
+        rx, rP, _, _ = oracle.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, pair.Q, pair.R1)
+        runs.append(time.perf_counter() - t0)
+    dt = sorted(runs)[len(runs) // 2]
     pts = sample_rows * COLS
     res = {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
            "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
-                     f"decode L+R + reconstruct, {dt:.1f} s"}
+                     f"decode L+R + reconstruct, median of 5 runs {dt:.2f} s "
+                     f"(range {min(runs):.2f}-{max(runs):.2f} s)"}
     return res, ((rx, rP) if sample_rows == ROWS else None)
 
 
