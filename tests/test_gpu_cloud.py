@@ -117,6 +117,16 @@ def _merged_views(n_views=3, rows=96, cols=128):
     return np.concatenate(pts).astype(np.float32)
 
 
+def test_radius_filters_reject_cells_below_half_the_radius(ctx):
+    xyz, valid = _rng_cloud(500, 16)
+    zim_amd.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.2, ctx)        # 2 x 0.2 >= 0.4: 5x5x5
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.cloud_gaussian(xyz, valid, 0.15, 0.4, 0.19, ctx)
+    d = torch.from_numpy(xyz).cuda()
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.cloud_mls(d, None, 0.2, 2, 0.35, 1, 1, 0.17, ctx)
+
+
 def test_merged_views_filters(ctx):
     xyz = _merged_views()
     assert len(xyz) > 6000
