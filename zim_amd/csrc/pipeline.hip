@@ -491,7 +491,7 @@ static int relayout_decode(Plan* pl, int tile_rows, hipStream_t s) {
 // fastest; the plan's inputs of every stage must already be valid.
 static int autotune(Plan* pl, hipStream_t s) {
   const int cu = pl->ctx->cu_count > 0 ? pl->ctx->cu_count : 256;
-  const int cands[] = {1, 2, 4, 8, 16};
+  const int cands[] = {1, 2, 4, 8, 16, 32, 64, 128, 256};
   hipEvent_t a, b;
   PSGC_HIP(hipEventCreate(&a));
   PSGC_HIP(hipEventCreate(&b));
