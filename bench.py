@@ -643,7 +643,7 @@ def run_c5(args):
         th = thetas.clone().requires_grad_(True)
         outs = zag.triangulate_views(th, [v[0] for v in views], [v[1] for v in views],
                                      [v[2] for v in views], nominal, ctx)
-        loss = sum(zag.plane_loss(x, m) for x, m in outs)
+        loss = zag.plane_losses(outs).sum()
         loss.backward()
         torch.cuda.synchronize()
         return th.grad

@@ -96,28 +96,45 @@ def _scatter(xyz, valid):
                         torch.stack([sxz, syz, szz])])
 
 
+def plane_losses(views):
+    """Sum of squared residuals of the valid points of each (xyz, valid) to
+    their least-squares plane (the smallest eigenvalue of the scatter
+    matrix), summed from the residuals n . (x - mean) themselves, n the plane
+    normal: the eigenvalue read off the scatter matrix carries an absolute
+    error of ~1e-16 times its largest eigenvalue (the plane's extent squared,
+    times the point count), which swamps finite differences of the loss. The
+    loss is stationary in n, so n needs no gradient. The 3 x 3 eigenproblems
+    of all views are solved on the host in one round trip."""
+    S = torch.stack([_scatter(x, v) for x, v in views])
+    with torch.no_grad():
+        ns = torch.linalg.eigh(S.detach().cpu())[1][:, :, 0].to(S.device)
+    out = []
+    for (xyz, valid), n in zip(views, ns):
+        m = valid.reshape(-1).bool()
+        p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
+        mu = p.sum(0) / m.sum()
+        d = p - mu   # elementwise dot with n: a library GEMV of N x 3 is far slower
+        r = torch.where(m, d[:, 0] * n[0] + d[:, 1] * n[1] + d[:, 2] * n[2], 0.0)
+        out.append((r * r).sum())
+    return torch.stack(out)
+
+
 def plane_loss(xyz, valid):
-    """Sum of squared residuals of the valid points to their least-squares
-    plane (the smallest eigenvalue of the scatter matrix), summed from the
-    residuals n . (x - mean) themselves, n the plane normal: the eigenvalue
-    read off the scatter matrix carries an absolute error of ~1e-16 times
-    its largest eigenvalue (the plane's extent squared, times the point
-    count), which swamps finite differences of the loss. The loss is
-    stationary in n, so n needs no gradient: d loss / d x is the residual's."""
-    m = valid.reshape(-1).bool()
-    p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
-    mu = p.sum(0) / m.sum()
-    with torch.no_grad():   # a 3 x 3 eigenproblem: on the host
-        n = torch.linalg.eigh(_scatter(xyz, valid).cpu())[1][:, 0].to(xyz.device)
-    r = torch.where(m, (p - mu) @ n, 0.0)
-    return (r * r).sum()
+    """plane_losses of one view."""
+    return plane_losses([(xyz, valid)])[0]
+
+
+def plane_flatnesses(views):
+    """Scale-invariant flatness of each (xyz, valid): smallest eigenvalue over
+    the trace of the scatter matrix of the valid points (host eigensolve, one
+    round trip for all views). Returns a host tensor."""
+    ev = torch.linalg.eigvalsh(torch.stack([_scatter(x, v) for x, v in views]).cpu())
+    return ev[:, 0] / ev.sum(1)
 
 
 def plane_flatness(xyz, valid):
-    """Scale-invariant flatness: smallest eigenvalue over the trace of the
-    scatter matrix of the valid points."""
-    ev = torch.linalg.eigvalsh(_scatter(xyz, valid).cpu())   # 3 x 3: on the host
-    return ev[0] / ev.sum()
+    """plane_flatnesses of one view."""
+    return plane_flatnesses([(xyz, valid)])[0]
 
 
 def default_scale(theta0):
@@ -243,7 +260,7 @@ def refine_planes(thetas_start, thetas_ref, phLs, phRs, kLs, kRs, nominal, pha_f
     def objs(zz, mcs, grad=False):
         th = (ref + zz * scale).requires_grad_(grad)
         res = triangulate_views(th, phLs, phRs, mcs, nom, ctx)
-        vals = torch.stack([plane_flatness(x, v) for x, v in res])
+        vals = plane_flatnesses(res)
         if grad:
             vals.sum().backward()
             return vals.detach().cpu(), th.grad * scale * mask

@@ -488,10 +488,13 @@ void with_width(int tb, F&& f) {
 
 unsigned blocks_for(int64_t n, int per) { return (unsigned)std::max<int64_t>(1, (n + per - 1) / per); }
 
-// Persistent grid: up to 2048 threads' worth of workgroups per CU.
+// Grid over the occupied cells: up to 128 K threads' worth of workgroups per
+// CU, in practice one workgroup per cell, so the hardware dispatcher balances
+// cells of very different sizes (62.7 M-point merged cloud, against 2 K
+// threads per CU: kNN 183 -> 147 ms, MLS 381 -> 333 ms, Gaussian 118 -> 95 ms).
 int grid_for_cells(psgc_ctx* ctx, int32_t ncells, int tb) {
   const int cu = ctx->cu_count > 0 ? ctx->cu_count : 256;
-  const int64_t g = std::min<int64_t>((int64_t)cu * (2048 / tb), ncells);
+  const int64_t g = std::min<int64_t>((int64_t)cu * (131072 / tb), ncells);
   return (int)std::max<int64_t>(kNumXcd, (g + kNumXcd - 1) / kNumXcd * kNumXcd);
 }
 
