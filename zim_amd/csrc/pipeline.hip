@@ -424,6 +424,9 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
 }
 
 static int run_plan(Plan* pl, hipStream_t s) {
+  // grid per stage: the autotuner's or a pinned choice, else the workgroups
+  // per CU the tuner settles on for full-size pairs (far more than resident:
+  // the dispatcher balances uneven tiles)
   auto g = [&](const char* st, int per_cu) {
     auto it = pl->grid.find(st);
     return it != pl->grid.end() ? it->second : default_grid(pl->ctx, per_cu);
@@ -432,7 +435,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
     PSGC_TRY(launch_decode_plain(d.a, d.phase, d.k, d.mod, d.valid, s));
   if (!pl->dec.host.empty())
     PSGC_TRY(launch_decode_tiled(pl->dec.dev, (int)pl->dec.host.size(), pl->dec.total,
-                                 g("decode", 4), pl->dec_rows, s));
+                                 g("decode", 64), pl->dec_rows, s));
   for (size_t c = 0; c < pl->gray_fix.size(); ++c)
     PSGC_TRY(launch_gray_fix(pl->gray_fix[c], pl->gray_in_ph[c], pl->gray_out_ph[c],
                              pl->gray_in_k[c], pl->gray_out_k[c], pl->gray_valid[c], s));
@@ -440,16 +443,16 @@ static int run_plan(Plan* pl, hipStream_t s) {
     PSGC_TRY(launch_post_smooth_valid(h.in, h.valid, h.out, h.rows, h.cols, s));
   if (!pl->mat.host.empty())
     PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
-                                g("match", 4), pl->max_cols, pl->max_knum, pl->match_threads, s));
+                                g("match", 64), pl->max_cols, pl->max_knum, pl->match_threads, s));
   if (!pl->med.host.empty())
     PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
-                                 g("median5", 8), s));
+                                 g("median5", 64), s));
   if (!pl->rg.host.empty())
     PSGC_TRY(launch_repro_gauss_tiled(pl->rg.dev, (int)pl->rg.host.size(), pl->rg.total,
-                                      g("repro_gauss", 8), pl->max_ghalf, s));
+                                      g("repro_gauss", 128), pl->max_ghalf, s));
   if (!pl->sor.host.empty())
     PSGC_TRY(launch_sor_knn_tiled(pl->sor.dev, (int)pl->sor.host.size(), pl->sor.total,
-                                  g("sor_knn", 8), pl->max_sor_half, pl->min_sor_k,
+                                  g("sor_knn", 256), pl->max_sor_half, pl->min_sor_k,
                                   pl->max_sor_k, s));
   // every SOR pair's statistics in two launches
   if (!pl->sst.host.empty())
@@ -471,7 +474,7 @@ static int run_plan(Plan* pl, hipStream_t s) {
   }
   for (auto& t : pl->mls)
     if (!t.host.empty())
-      PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 4), pl->max_mls_hw,
+      PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 128), pl->max_mls_hw,
                                 pl->mls_variant, mls_sems(t.host), s));
   return PSGC_OK;
 }
@@ -859,7 +862,7 @@ int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t*
   PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(DecodeJob), &d));
   PSGC_HIP(hipMemcpyAsync(d, &j, sizeof(DecodeJob), hipMemcpyHostToDevice, s));
   return launch_decode_tiled((const DecodeJob*)d, 1, (int64_t)j.tiles_x * j.tiles_y,
-                             default_grid(ctx, 4), 4, s);
+                             default_grid(ctx, 64), 4, s);
 }
 
 int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
@@ -911,7 +914,7 @@ int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols
   c.disp_med = dst;
   CloudJob* d;
   PSGC_TRY(upload_job(ctx, c, &d, s));
-  PSGC_TRY(launch_median_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 8), s));
+  PSGC_TRY(launch_median_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 64), s));
   return hipStreamSynchronize(s) == hipSuccess ? PSGC_OK : PSGC_E_HIP;
 }
 
@@ -929,7 +932,7 @@ int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uin
   c.md = md;
   CloudJob* d;
   PSGC_TRY(upload_job(ctx, c, &d, s));
-  PSGC_TRY(launch_sor_knn_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 8), half,
+  PSGC_TRY(launch_sor_knn_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 256), half,
                                 k, k, s));
   PSGC_TRY(launch_sor_finish(xyz, md, dst, dstP, (size_t)rows * cols, std_mul, partial, stats, s));
   return hipStreamSynchronize(s) == hipSuccess ? PSGC_OK : PSGC_E_HIP;
@@ -961,7 +964,7 @@ int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, flo
     c.mls_r2 = r2;
     CloudJob* d;
     PSGC_TRY(upload_job(ctx, c, &d, s));
-    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 4), hw,
+    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 128), hw,
                               kMlsPlain, mls_sem_bit(semantics), s));
     // the job buffer is reused by the next iteration
     PSGC_HIP(hipStreamSynchronize(s));
