@@ -62,8 +62,18 @@ __device__ inline bool level_finish(double S, double C, double G, int n, double 
                                     double* wrap, double* mod) {
   *wrap = psgc::atan2(S, C);
   double b = S * S + C * C;
-  const double mean = G / n;
-  b = 2.0 * sqrt(b) / n;
+  double mean;
+  if ((n & (n - 1)) == 0) {
+    // dividing by a power of two only scales (exactly, for these non-negative
+    // sums): the same bits as the multiplication by 2^-log2(n), without the
+    // FP64 division sequence
+    const double inv = ldexp(1.0, -__builtin_ctz((unsigned)n));
+    mean = G * inv;
+    b = 2.0 * sqrt(b) * inv;
+  } else {
+    mean = G / n;
+    b = 2.0 * sqrt(b) / n;
+  }
   double nm = 0.0;
   if (mean > 1e-3) nm = b / mean;
   *mod = nm;
