@@ -83,8 +83,8 @@ ALG_BYTES_PER_PX = {
 # profiles/c2_8pair_valu_counters.md; with the live launch time they give the
 # kernel's VALU issue rate against the chip's single-issue peak (256 CUs x 4
 # SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
-VALU_PER_PX_PROFILED = {"mls": 11385, "sor_knn": 1790, "decode": 956, "match": 578,
-                        "repro_gauss": 502, "median5": 271}
+VALU_PER_PX_PROFILED = {"mls": 11385, "sor_knn": 1790, "decode": 897, "match": 578,
+                        "repro_gauss": 504, "median5": 272}
 VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
 
 
