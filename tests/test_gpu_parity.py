@@ -441,6 +441,39 @@ def test_batch_mixed_schedules_match_oracle(ctx):
     b.close()
 
 
+def test_batch_at_uneven_full_sizes_matches_oracle(ctx):
+    """One captured, autotuned batch of the uneven configuration's real sizes
+    and schedules: 640x480 3-step, 1280x1024 Gray code, 1920x1200 8-step and
+    2448x2048 4-step; each pair equal to the oracle run on it alone."""
+    from zim_amd import synth
+    specs = []
+    for i, (rows, cols, sched) in enumerate(((480, 640, 3), (1024, 1280, "gray"),
+                                             (1200, 1920, 8), (2048, 2448, 4))):
+        if sched == "gray":
+            p = synth.render_c1(4100 + i, device="cuda", rows=rows, cols=cols)
+            dcfg, pha_f = scenes.c1_gray_config(), 50
+        else:
+            p = synth.render_c2(4100 + i, device="cuda", rows=rows, cols=cols,
+                                f=2400.0 * cols / 2448.0, steps=(sched,) * 3)
+            dcfg, pha_f = scenes.c2_decode_config(shift_num=(sched,) * 3), 70
+        specs.append((p, dcfg, pha_f))
+    rcfg = scenes.c2_recon_config()
+    b = zim_amd.Batch([dict(left=p.left, right=p.right, decode=d, recon=rcfg, pha_f=f, Q=p.Q,
+                            R1=p.R1) for p, d, f in specs], ctx, graph=True, autotune=True)
+    outs = b.run()
+    torch.cuda.synchronize()
+    for i, ((p, dcfg, pha_f), out) in enumerate(zip(specs, outs)):
+        lh = [x.cpu().numpy() for x in p.left]
+        rh = [x.cpu().numpy() for x in p.right]
+        Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
+        rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], pha_f, p.Q, p.R1)
+        print("pair", i, p.left[0].shape, "valid", int(rP.sum()))
+        assert rP.sum() > 0.3 * rP.size, i
+        assert np.array_equal(host(out["P"]), rP), (i, (host(out["P"]) != rP).sum())
+        assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL, i
+    b.close()
+
+
 def test_full_size_pair_matches_oracle(ctx):
     """One full 2448x2048 config-2 pair (rendered on the GPU, decoded and
     reconstructed by the batched pipeline) against the oracle on the same
