@@ -33,6 +33,8 @@ def lib():
         L.orc_ror.argtypes = [vp, vp, vp, vp, i32, i32, flt, i32, i32]
         L.orc_sor.argtypes = [vp, vp, vp, vp, i32, i32, i32, flt, i32, vp, vp]
         L.orc_mls.argtypes = [vp, vp, vp, i32, i32, i32, flt, i32, flt, i32, i32]
+        L.orc_eigen3.argtypes = [vp, vp, vp, i32]
+        L.orc_chol.argtypes = [vp, vp, vp, i32]
         L.orc_reconstruct.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
@@ -216,6 +218,30 @@ def mls(xyz, P, window, sigma, order, max_radius=0.0, iterations=1, semantics=0)
     lib().orc_mls(_p(xyz), _p(P), _p(d), rows, cols, window, sigma, order, max_radius,
                   iterations, semantics)
     return d
+
+
+def eigen3(C):
+    """The MLS filter's symmetric 3x3 eigen solver on a stack of float32
+    matrices (..., 3, 3): eigenvalues (..., 3) in the solver's ascending
+    order and eigenvectors (..., 3, 3), one per row."""
+    C = np.ascontiguousarray(C, np.float32)
+    shp = C.shape[:-2]
+    Cf = C.reshape(-1, 3, 3)
+    ev = np.empty((len(Cf), 3), np.float32)
+    V = np.empty((len(Cf), 3, 3), np.float32)
+    assert lib().orc_eigen3(_p(Cf), _p(ev), _p(V), len(Cf)) == 0
+    return ev.reshape(*shp, 3), V.reshape(*shp, 3, 3)
+
+
+def chol(A, b):
+    """The MLS filter's float Cholesky solve of A x = b (n <= 6): (x, ok)."""
+    A = np.ascontiguousarray(A, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    n = len(b)
+    x = np.zeros(n, np.float32)
+    rc = lib().orc_chol(_p(A), _p(b), _p(x), n)
+    assert rc in (0, 1), rc
+    return x, bool(rc)
 
 
 def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1):

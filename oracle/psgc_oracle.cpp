@@ -1081,6 +1081,36 @@ int orc_mls(const float* src, const uint8_t* P, float* dst, int rows, int cols,
   return PSGC_OK;
 }
 
+// The MLS filter's 3x3 symmetric eigen solver (symmetricEigen3x3,
+// JSLAB_MLS_Filter.cpp:11-106) on n row-major matrices: ev[3] ascending as
+// the solver orders them, V[3][3] one eigenvector per row.
+int orc_eigen3(const float* C, float* ev, float* V, int n) {
+  for (int i = 0; i < n; ++i) {
+    float A[3][3], W[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) A[r][c] = C[i * 9 + r * 3 + c];
+    eigen3(A, ev + i * 3, W);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) V[i * 9 + r * 3 + c] = W[r][c];
+  }
+  return PSGC_OK;
+}
+
+// The MLS filter's Cholesky solve (choleskySolve, JSLAB_MLS_Filter.cpp:113-156)
+// of one nc x nc system (nc <= 6, A row-major nc x nc): 1 if solved, 0 when a
+// pivot is <= 1e-12.
+int orc_chol(const float* A, const float* b, float* x, int nc) {
+  if (nc < 1 || nc > 6) return PSGC_E_INVALID;
+  float M[6][6] = {}, bb[6] = {}, xx[6] = {};
+  for (int r = 0; r < nc; ++r) {
+    bb[r] = b[r];
+    for (int c = 0; c < nc; ++c) M[r][c] = A[r * nc + c];
+  }
+  const bool ok = chol(M, bb, xx, nc);
+  for (int r = 0; r < nc; ++r) x[r] = xx[r];
+  return ok ? 1 : 0;
+}
+
 // JSLAB_PhaseEpipolarReconstructor_V2::startCalculate
 // (JSLAB_PhaseEpipolarReconstructor_V2.cpp:397-480) on rectified inputs.
 static int reconstruct_impl(const psgc_recon_config* cfg, const double* l_phase,
