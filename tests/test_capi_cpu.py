@@ -101,8 +101,7 @@ def test_rectify_ideal_closed_form():
 
 
 def test_header_compiles_as_c_and_links(tmp_path):
-    """include/psgc.h is valid C99 and C++17 and links against the library
-    (the integration header compiles without OpenCV too)."""
+    """include/psgc.h is valid C99 and C++17 and links against the library."""
     import subprocess
     inc = os.path.join(ROOT, "include")
     libdir = os.path.join(ROOT, "zim_amd")
@@ -119,7 +118,7 @@ def test_header_compiles_as_c_and_links(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.split() == ["1", "8"], out
     cpp = tmp_path / "t.cpp"
-    cpp.write_text('#include "psgc_jslab.hpp"\nint main(){ return psgc_abi_version() == 1 ? 0 : 1; }\n')
+    cpp.write_text('#include "psgc.h"\nint main(){ return psgc_abi_version() == 1 ? 0 : 1; }\n')
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{inc}", str(cpp), "-o",
                         str(tmp_path / "tpp"), f"-L{libdir}", "-l:libpsgc.so",
                         f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
