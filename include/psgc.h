@@ -470,6 +470,21 @@ PSGC_API int psgc_remap(psgc_ctx* ctx, int kind, const void* src, int src_rows, 
 PSGC_API int psgc_compact(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
                           const uint8_t* gray, int rows, int cols, float* out_xyz,
                           uint8_t* out_gray, int64_t* count, int mem);
+
+/* psgc_compact_noalloc: psgc_compact without any allocation. It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc). Every pointer is device memory; the call is
+ * asynchronous on the context's stream, and *count (a device int64)
+ * receives n. At most PSGC_COMPACT_NOALLOC_MAX_POINTS pixels per call
+ * (larger calls fail with PSGC_E_INVALID). The block counts live in a static
+ * per-GPU device array of 1 MiB, held until the process ends and shared by
+ * every call on that GPU: calls that overlap on one GPU (other streams or
+ * contexts) race and must be serialized by the caller. */
+#define PSGC_COMPACT_NOALLOC_MAX_POINTS ((size_t)1 << 27)
+PSGC_API int psgc_compact_noalloc(psgc_ctx* ctx, const float* xyz, const uint8_t* P,
+                                  const uint8_t* gray, int rows, int cols, float* out_xyz,
+                                  uint8_t* out_gray, int64_t* count);
 PSGC_API int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n);
 PSGC_API int psgc_write_raw_f64(const char* raw_path, const char* info_path, const double* data,
                                 int rows, int cols);

@@ -230,6 +230,7 @@ def load(path: str | None = None):
         "psgc_rectify_maps": ([vp, vp, vp, vp, i32, i32, vp, vp, i32], i32),
         "psgc_remap": ([vp, i32, vp, i32, i32, vp, vp, i32, i32, vp, i32], i32),
         "psgc_compact": ([vp, vp, vp, vp, i32, i32, vp, vp, C.POINTER(C.c_int64), i32], i32),
+        "psgc_compact_noalloc": ([vp, vp, vp, vp, i32, i32, vp, vp, vp], i32),
         "psgc_write_ply": ([C.c_char_p, vp, vp, C.c_int64], i32),
         "psgc_write_raw_f64": ([C.c_char_p, C.c_char_p, vp, i32, i32], i32),
         "psgc_adjoint_forward": ([vp, vp, vp, vp, vp, vp, i32, i32, vp, vp, i32], i32),

@@ -476,6 +476,10 @@ int psgc_create(psgc_ctx** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
   ctx->bufs.resize(S_NUM_SLOTS);
+  // set-up of the allocation-free compaction on this device (a failure here
+  // is reported by psgc_compact_noalloc, which repeats the lookup)
+  int64_t* scratch_dev;
+  if (compact_noalloc_scratch(device, &scratch_dev) != PSGC_OK) set_error("%s", "");
   *out = ctx;
   return PSGC_OK;
 }
@@ -1413,6 +1417,15 @@ int psgc_compact(psgc_ctx* ctx, const float* xyz, const uint8_t* P, const uint8_
   PSGC_TRY(copy_back(ctx, mem, out_xyz, ox, n * 3));
   PSGC_TRY(copy_back(ctx, mem, out_gray, og, n));
   return finish(ctx, mem);
+}
+
+int psgc_compact_noalloc(psgc_ctx* ctx, const float* xyz, const uint8_t* P, const uint8_t* gray,
+                         int rows, int cols, float* out_xyz, uint8_t* out_gray, int64_t* count) {
+  PSGC_CTX_ENTER(ctx);
+  PSGC_TRY(check_size(rows, cols));
+  if (!xyz || !P || !out_xyz || !count) PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  return launch_compact_noalloc(ctx->device, xyz, P, gray, (size_t)rows * cols, out_xyz, out_gray,
+                                count, ctx->stream);
 }
 
 int psgc_write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n) {

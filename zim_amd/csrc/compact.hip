@@ -16,6 +16,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "psgc_internal.h"
@@ -190,19 +191,70 @@ int launch_remap(int kind, const void* src, void* dst, const float* mx, const fl
 
 size_t compact_scratch_int64(size_t n) { return (n + CPB - 1) / CPB + 1; }
 
-int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size_t n,
+namespace {
+
+int compact_launch(const float* xyz, const uint8_t* P, const uint8_t* gray, size_t n,
                    int64_t* scratch, float* out, uint8_t* out_gray, int64_t* total,
-                   hipStream_t s) {
+                   hipStream_t s, bool profile) {
   const size_t nb = (n + CPB - 1) / CPB;
-  PSGC_PROF_BEGIN("compact", s);
+  if (profile) PSGC_PROF_BEGIN("compact", s);
   compact_count_kernel<<<(unsigned)nb, CB, 0, s>>>(P, n, scratch);
   PSGC_LAUNCHED();
   compact_scan_kernel<<<1, 1024, 0, s>>>(scratch, nb, total);
   PSGC_LAUNCHED();
   compact_scatter_kernel<<<(unsigned)nb, CB, 0, s>>>(xyz, P, gray, n, scratch, out, out_gray);
   PSGC_LAUNCHED();
-  PSGC_PROF_END("compact", s);
+  if (profile) PSGC_PROF_END("compact", s);
   return PSGC_OK;
+}
+
+// Scratch of the allocation-free compaction: one int64 count per block of
+// CPB pixels (overwritten by every call) and the total in the last entry.
+// Each GPU holds its own copy from code-object load to process exit.
+static_assert(PSGC_COMPACT_NOALLOC_MAX_POINTS % CPB == 0, "whole blocks");
+constexpr size_t NOALLOC_SCRATCH_INT64 = PSGC_COMPACT_NOALLOC_MAX_POINTS / CPB + 1;
+__device__ __align__(256) int64_t scratch[NOALLOC_SCRATCH_INT64];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int64_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+}  // namespace
+
+int launch_compact(const float* xyz, const uint8_t* P, const uint8_t* gray, size_t n,
+                   int64_t* scratch, float* out, uint8_t* out_gray, int64_t* total,
+                   hipStream_t s) {
+  return compact_launch(xyz, P, gray, n, scratch, out, out_gray, total, s, true);
+}
+
+// The device address of `scratch` on `device` (which must be current): looked
+// up once per device and kept; host code never passes the array's own name.
+int compact_noalloc_scratch(int device, int64_t** out) {
+  if (device < 0 || device >= MAX_DEVICES) PSGC_FAIL(PSGC_E_NODEVICE, "device %d", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int64_t* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      PSGC_FAIL(PSGC_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return PSGC_OK;
+}
+
+int launch_compact_noalloc(int device, const float* xyz, const uint8_t* P, const uint8_t* gray,
+                           size_t n, float* out, uint8_t* out_gray, int64_t* count,
+                           hipStream_t s) {
+  if (n > PSGC_COMPACT_NOALLOC_MAX_POINTS)
+    PSGC_FAIL(PSGC_E_INVALID, "%zu points exceed the allocation-free limit of %zu", n,
+              (size_t)PSGC_COMPACT_NOALLOC_MAX_POINTS);
+  int64_t* sc;
+  PSGC_TRY(compact_noalloc_scratch(device, &sc));
+  return compact_launch(xyz, P, gray, n, sc, out, out_gray, count, s, false);
 }
 
 int write_ply(const char* path, const float* xyz, const uint8_t* gray, int64_t n) {
