@@ -150,3 +150,23 @@ def poly_coefs_equivalent_to_q(rows, cols, f, cx1, cy, cx2, B, form):
     else:
         raise ValueError(form)
     return cx, cy_, cz
+
+
+@functools.lru_cache(maxsize=None)
+def closeup(seed=41, rows=128, cols=192, noise=4.0):
+    """A close-range pair at ~0.025 mm pixel pitch (a gently curved surface
+    near Z = 60 mm; f = 2400 px, B = 2 mm, 80 px of disparity): the scale of
+    the reference GPU class's defaults (ROR radius 0.05 mm with 5 of an 11 x 11
+    window, MLS at 3 sigma = 1.5 mm in a 51 x 51 window,
+    JSLAB_PhaseEpipolarReconstructor_V2_GPU.h:27-61). 12-bit frames of the
+    config-2 patterns with N(0, noise DN)."""
+    import torch
+    cam = synth.Camera(rows, cols, 2400.0, 2.0)
+    proj = synth.Projector(1280, 1000.0)
+    ii = torch.arange(rows, dtype=torch.float64).view(-1, 1)
+    jj = torch.arange(cols, dtype=torch.float64).view(1, -1)
+    z = 60.0 + 0.004 * (jj - cam.cx) + 0.3 * torch.sin(jj / 23.0) * torch.cos(ii / 17.0)
+    p = synth._render(cam, proj, z, seed, "cpu", (70, 64, 59), (4, 4, 4), ambient=400.0,
+                      modulation=750.0, noise=noise, poisson=False, max_val=4095,
+                      dtype=torch.uint16)
+    return p, [x.numpy() for x in p.left], [x.numpy() for x in p.right]

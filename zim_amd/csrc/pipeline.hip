@@ -54,8 +54,18 @@ struct PairPost {
   double* partial;
   double* stats;
   float std_mul;
-  float* sor_dst;           // when no MLS: final xyz
-  uint8_t* sor_Pdst;        // when no MLS: final mask
+  float* sor_dst;           // when no MLS (or a one-thread-per-pixel MLS): filtered xyz
+  uint8_t* sor_Pdst;        // ... and its mask
+  // SOR window beyond the tiled kernel: the one-thread-per-pixel kNN
+  int sor_plain, sor_k, sor_half;
+  const uint8_t* sor_P;
+  // MLS window beyond the tiled kernel: one-thread-per-pixel iterations
+  int mls_plain, mls_hw, mls_order, mls_sem, mls_iters;
+  float mls_sigma, mls_r2;
+  const float* mls_src;
+  const uint8_t* mls_P;
+  float* mls_tmp;
+  float* mls_dst;
   // copies when neither an outlier stage writes the outputs nor MLS runs
   const float* copy_xyz_src;
   const uint8_t* copy_P_src;
@@ -81,6 +91,23 @@ struct HdrSmooth {
   const uint8_t* valid;
   double* out;
   int rows, cols;
+};
+
+// A pair whose rows exceed the tiled match's LDS budget: post-smooth (when
+// the decode left it to the match) and the one-thread-per-pixel match.
+struct PlainMatch {
+  const double* lp_in;
+  const double* rp_in;
+  double* lp_smooth;  // NULL: lp_in is already smoothed
+  double* rp_smooth;
+  const int32_t* lk;
+  const int32_t* rk;
+  int rows, cols, pha_f;
+  double grad_thr;
+  int32_t* span;
+  double* disp64;
+  float* disp32;
+  int32_t* match_col;
 };
 
 // Host data copied into the arena once the plan is built.
@@ -123,6 +150,7 @@ struct Plan {
   std::vector<const uint8_t*> gray_valid;
   std::vector<PlainDecode> plain_dec;
   std::vector<HdrSmooth> hdr_smooth;
+  std::vector<PlainMatch> plain_match;
   std::vector<Blob> blobs;
   int max_cols = 0, max_knum = 0, max_ghalf = 0, max_sor_half = 0, max_sor_k = 0, max_mls_hw = 0;
   int min_sor_k = 1 << 30;
@@ -226,42 +254,65 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", rc.outlier_mode);
   if (rc.outlier_mode == PSGC_OUTLIER_SOR) {
     if (rc.sor_k < 1 || rc.sor_k > 64) PSGC_FAIL(PSGC_E_INVALID, "sor_k=%d", rc.sor_k);
-    if (rc.sor_window < 1 || rc.sor_window > 8)
-      PSGC_FAIL(PSGC_E_UNSUPPORTED, "sor_window=%d above the tiled limit 8", rc.sor_window);
+    if (rc.sor_window < 1 || rc.sor_window > 512)
+      PSGC_FAIL(PSGC_E_INVALID, "sor_window=%d", rc.sor_window);
   }
   if (rc.use_mls) {
     if (!(rc.mls_spatial_sigma > 0)) PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
-    if (rc.mls_window_size < 0 || rc.mls_window_size > 8)
-      PSGC_FAIL(PSGC_E_UNSUPPORTED, "mls_window_size=%d above the tiled limit 8",
-                rc.mls_window_size);
+    if (rc.mls_window_size < 0 || rc.mls_window_size > 256)
+      PSGC_FAIL(PSGC_E_INVALID, "mls_window_size=%d", rc.mls_window_size);
   }
-  if (match_tiled_lds(cols, pha_f) > (size_t)match_tiled_max_lds())
-    PSGC_FAIL(PSGC_E_UNSUPPORTED, "row of %d px with %d orders exceeds the LDS budget", cols,
-              pha_f);
+  // Beyond what the tiled kernels hold (windows of half size 8, a row and its
+  // order spans in the match's LDS budget) a pair's stage runs on the
+  // one-thread-per-pixel kernels instead, with the same results.
+  const bool sor_plain = rc.outlier_mode == PSGC_OUTLIER_SOR && rc.sor_window > kTiledMaxHalf;
+  const bool mls_plain = rc.use_mls && rc.mls_window_size > kTiledMaxHalf;
+  const bool match_plain = match_tiled_lds(cols, pha_f) > (size_t)match_tiled_max_lds();
   const bool cpu = rc.semantics == PSGC_SEM_CPU;
   const bool use64 = !rc.use_disp_median_filter && cpu;
 
   // ---- match
   float* disp = disp_out && !rc.use_disp_median_filter ? disp_out : (float*)A.take(np * 4);
   double* d64 = d64_out ? d64_out : (use64 ? (double*)A.take(np * 8) : nullptr);
-  MatchJob mj{};
-  mj.lp = in.lph;
-  mj.rp = in.rph;
-  mj.lk = in.lk;
-  mj.rk = in.rk;
-  mj.rows = rows;
-  mj.cols = cols;
-  mj.pha_f = pha_f;
-  mj.smooth = in.smooth;
-  mj.grad_thr = rc.grad_threshold;
-  mj.disp = disp;
-  mj.disp64 = d64;
-  mj.match_col = mc_out;
-  mj.tile_base = pl->mat.total;
-  pl->mat.total += rows;
-  pl->mat.host.push_back(mj);
-  pl->max_cols = std::max(pl->max_cols, cols);
-  pl->max_knum = std::max(pl->max_knum, pha_f);
+  if (match_plain) {
+    PlainMatch pm{};
+    pm.lp_in = in.lph;
+    pm.rp_in = in.rph;
+    if (in.smooth) {
+      pm.lp_smooth = (double*)A.take(np * 8);
+      pm.rp_smooth = (double*)A.take(np * 8);
+    }
+    pm.lk = in.lk;
+    pm.rk = in.rk;
+    pm.rows = rows;
+    pm.cols = cols;
+    pm.pha_f = pha_f;
+    pm.grad_thr = rc.grad_threshold;
+    pm.span = (int32_t*)A.take((size_t)rows * (pha_f + 1) * 2 * sizeof(int32_t));
+    pm.disp64 = d64;
+    pm.disp32 = disp;
+    pm.match_col = mc_out;
+    pl->plain_match.push_back(pm);
+  } else {
+    MatchJob mj{};
+    mj.lp = in.lph;
+    mj.rp = in.rph;
+    mj.lk = in.lk;
+    mj.rk = in.rk;
+    mj.rows = rows;
+    mj.cols = cols;
+    mj.pha_f = pha_f;
+    mj.smooth = in.smooth;
+    mj.grad_thr = rc.grad_threshold;
+    mj.disp = disp;
+    mj.disp64 = d64;
+    mj.match_col = mc_out;
+    mj.tile_base = pl->mat.total;
+    pl->mat.total += rows;
+    pl->mat.host.push_back(mj);
+    pl->max_cols = std::max(pl->max_cols, cols);
+    pl->max_knum = std::max(pl->max_knum, pha_f);
+  }
 
   CloudJob base{};
   base.rows = rows;
@@ -341,6 +392,25 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     mls_src = dst;
     mls_mask = P_out;
     mls_Pout = nullptr;
+  } else if (rc.outlier_mode == PSGC_OUTLIER_SOR && sor_plain) {
+    double* md = (double*)A.take(np * 8);
+    double* part = (double*)A.take(sor_partial_doubles(np) * 8);
+    double* st = (double*)A.take(4 * 8);
+    float* dst = rc.use_mls ? (t0 = (float*)A.take(np * 12)) : xyz_out;
+    pp.sor_plain = 1;
+    pp.sor_src = xyzg;
+    pp.sor_P = Pin;
+    pp.sor_k = rc.sor_k;
+    pp.sor_half = rc.sor_window;
+    pp.md = md;
+    pp.partial = part;
+    pp.stats = st;
+    pp.std_mul = rc.sor_std_mul;
+    pp.sor_dst = dst;
+    pp.sor_Pdst = P_out;
+    mls_src = dst;
+    mls_mask = P_out;
+    mls_Pout = nullptr;
   } else if (rc.outlier_mode == PSGC_OUTLIER_SOR) {
     double* md = (double*)A.take(np * 8);
     double* part = (double*)A.take(sor_partial_doubles(np) * 8);
@@ -378,17 +448,43 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     if (!rc.use_mls) {
       pp.sor_dst = xyz_out;
       pp.sor_Pdst = P_out;
+    } else if (mls_plain) {
+      // the one-thread-per-pixel MLS does not gate on the statistics itself
+      pp.sor_dst = t0 = (float*)A.take(np * 12);
+      pp.sor_Pdst = P_out;
+      mls_src = t0;
+      mls_mask = P_out;
+      mls_Pout = nullptr;
+    } else {
+      mls_md = md;
+      mls_thr = st + 3;
     }
-    mls_md = md;
-    mls_thr = st + 3;
   } else if (!rc.use_mls) {
     pp.copy_xyz_src = xyzg;
     pp.copy_P_src = Pin;
     pp.copy_xyz_dst = xyz_out;
     pp.copy_P_dst = P_out;
+  } else if (mls_plain) {
+    // the mask passes through unchanged
+    pp.copy_P_src = Pin;
+    pp.copy_P_dst = P_out;
+    mls_Pout = nullptr;
   }
 
-  if (rc.use_mls) {
+  if (rc.use_mls && mls_plain) {
+    pp.mls_plain = 1;
+    pp.mls_iters = rc.mls_iterations > 1 ? rc.mls_iterations : 1;
+    pp.mls_order = std::max(0, std::min(2, rc.mls_polynomial_order));
+    pp.mls_r2 = (rc.mls_max_radius > 0.0f) ? rc.mls_max_radius * rc.mls_max_radius
+                                           : 9.0f * rc.mls_spatial_sigma * rc.mls_spatial_sigma;
+    pp.mls_sigma = rc.mls_spatial_sigma;
+    pp.mls_hw = rc.mls_window_size;
+    pp.mls_sem = rc.semantics;
+    pp.mls_src = mls_src;
+    pp.mls_P = mls_mask;
+    pp.mls_tmp = t1;
+    pp.mls_dst = xyz_out;
+  } else if (rc.use_mls) {
     const int n_it = rc.mls_iterations > 1 ? rc.mls_iterations : 1;
     const int order = std::max(0, std::min(2, rc.mls_polynomial_order));
     const float r2 = (rc.mls_max_radius > 0.0f) ? rc.mls_max_radius * rc.mls_max_radius
@@ -444,6 +540,18 @@ static int run_plan(Plan* pl, hipStream_t s) {
   if (!pl->mat.host.empty())
     PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
                                 g("match", 64), pl->max_cols, pl->max_knum, pl->match_threads, s));
+  for (const PlainMatch& m : pl->plain_match) {
+    const double* lp = m.lp_in;
+    const double* rp = m.rp_in;
+    if (m.lp_smooth) {
+      PSGC_TRY(launch_post_smooth(m.lp_in, m.lp_smooth, m.rows, m.cols, s));
+      PSGC_TRY(launch_post_smooth(m.rp_in, m.rp_smooth, m.rows, m.cols, s));
+      lp = m.lp_smooth;
+      rp = m.rp_smooth;
+    }
+    PSGC_TRY(launch_match_plain(lp, rp, m.lk, m.rk, m.rows, m.cols, m.pha_f, m.grad_thr, m.span,
+                                m.disp64, m.disp32, m.match_col, s));
+  }
   if (!pl->med.host.empty())
     PSGC_TRY(launch_median_tiled(pl->med.dev, (int)pl->med.host.size(), pl->med.total,
                                  g("median5", 64), s));
@@ -462,14 +570,28 @@ static int run_plan(Plan* pl, hipStream_t s) {
     if (pp.outlier == PSGC_OUTLIER_ROR) {
       PSGC_TRY(launch_ror(pp.ror_src, pp.ror_P, pp.ror_dst, pp.ror_Pdst, pp.rows, pp.cols,
                           pp.radius, pp.min_nb, pp.window, s));
+    } else if (pp.outlier == PSGC_OUTLIER_SOR && pp.sor_plain) {
+      PSGC_TRY(launch_sor(pp.sor_src, pp.sor_P, pp.sor_dst, pp.sor_Pdst, pp.rows, pp.cols,
+                          pp.sor_k, pp.std_mul, pp.sor_half, pp.md, pp.partial, pp.stats, s));
     } else if (pp.outlier == PSGC_OUTLIER_SOR) {
-      // with MLS the statistics are all it needs (it gates on them itself)
-      if (!pp.use_mls)
+      // the tiled MLS gates on the statistics itself; otherwise apply them
+      if (pp.sor_dst)
         PSGC_TRY(launch_sor_apply(pp.sor_src, pp.md, pp.sor_dst, pp.sor_Pdst, np, pp.stats, s));
-    } else if (pp.copy_xyz_dst) {
-      PSGC_HIP(hipMemcpyAsync(pp.copy_xyz_dst, pp.copy_xyz_src, np * 12, hipMemcpyDeviceToDevice,
-                              s));
-      PSGC_HIP(hipMemcpyAsync(pp.copy_P_dst, pp.copy_P_src, np, hipMemcpyDeviceToDevice, s));
+    } else {
+      if (pp.copy_xyz_dst)
+        PSGC_HIP(hipMemcpyAsync(pp.copy_xyz_dst, pp.copy_xyz_src, np * 12,
+                                hipMemcpyDeviceToDevice, s));
+      if (pp.copy_P_dst)
+        PSGC_HIP(hipMemcpyAsync(pp.copy_P_dst, pp.copy_P_src, np, hipMemcpyDeviceToDevice, s));
+    }
+    if (pp.mls_plain) {
+      const float* in = pp.mls_src;
+      for (int it = 0; it < pp.mls_iters; ++it) {
+        float* out = ((pp.mls_iters - 1 - it) % 2 == 0) ? pp.mls_dst : pp.mls_tmp;
+        PSGC_TRY(launch_mls(in, pp.mls_P, out, pp.rows, pp.cols, pp.mls_hw, pp.mls_sigma,
+                            pp.mls_order, pp.mls_r2, pp.mls_sem, s));
+        in = out;
+      }
     }
   }
   for (auto& t : pl->mls)
@@ -921,7 +1043,7 @@ int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols
 int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
               int rows, int cols, int k, float std_mul, int half, double* md, double* partial,
               double* stats, hipStream_t s) {
-  if (half > 8) return launch_sor(xyz, P, dst, dstP, rows, cols, k, std_mul, half, md, partial,
+  if (half > kTiledMaxHalf) return launch_sor(xyz, P, dst, dstP, rows, cols, k, std_mul, half, md, partial,
                                   stats, s);
   if (k < 1 || k > 64) PSGC_FAIL(PSGC_E_INVALID, "sor k=%d outside [1, 64]", k);
   CloudJob c = base_job(rows, cols);
@@ -941,7 +1063,7 @@ int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uin
 int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, float* tmp, int rows,
               int cols, int hw, float sigma, int order, float r2, int iterations, int semantics,
               hipStream_t s) {
-  if (hw > 8) {
+  if (hw > kTiledMaxHalf) {
     const float* in = src;
     for (int it = 0; it < iterations; ++it) {
       float* out = ((iterations - 1 - it) % 2 == 0) ? dst : tmp;

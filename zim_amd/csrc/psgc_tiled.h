@@ -127,6 +127,8 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
 // first 24 in-radius weights of each point in LDS between the passes (the rest
 // are evaluated again). Results are identical.
 enum { kMlsPlain = 0, kMlsCached = 1 };
+// Largest half window the tiled SOR and MLS kernels stage in LDS.
+constexpr int kTiledMaxHalf = 8;
 constexpr int kMlsCachedMaxHalf = 6;
 // Semantics present among the MLS jobs of a launch (a bit each): CPU
 // semantics accumulate with rounded products and sums, GPU semantics with
