@@ -89,6 +89,69 @@ def _is_dev(x):
     return isinstance(x, torch.Tensor) and x.is_cuda
 
 
+_NP = {torch.float64: np.float64, torch.float32: np.float32, torch.int32: np.int32,
+       torch.uint8: np.uint8}
+
+
+def _map(name, a, dev, device, dtype, shape):
+    """One input map of a call, checked before its address reaches a kernel:
+    device tensors must sit on the call's device with exactly `dtype`, `shape`
+    and a dense layout (a wrong one would be read out of bounds or as the
+    wrong type); host arrays must have `shape` and are converted to `dtype`."""
+    if dev:
+        if not _is_dev(a):
+            raise ValueError(f"{name}: expected a device tensor like the call's other inputs")
+        if a.device != device:
+            raise ValueError(f"{name}: on {a.device}, the call's other inputs on {device}")
+        if a.dtype != dtype:
+            raise ValueError(f"{name}: dtype {a.dtype}, expected {dtype}")
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"{name}: shape {tuple(a.shape)}, expected {tuple(shape)}")
+        if not a.is_contiguous():
+            raise ValueError(f"{name}: must be contiguous")
+        return a
+    if _is_dev(a):
+        raise ValueError(f"{name}: expected a host array like the call's other inputs")
+    arr = np.ascontiguousarray(a.numpy() if isinstance(a, torch.Tensor) else a, _NP[dtype])
+    if arr.shape != tuple(shape):
+        raise ValueError(f"{name}: shape {arr.shape}, expected {tuple(shape)}")
+    return arr
+
+
+def _frames(frames, name="frames"):
+    """A camera's frame stack: one sample type, one shape; on the device one
+    device and one row stride with unit column stride (the ABI takes one
+    stride for all frames). Returns (frames, is_device, sample type, row
+    stride bytes or 0, rows, cols)."""
+    if len(frames) == 0:
+        raise ValueError(f"{name}: no frames")
+    f0 = frames[0]
+    dev = _is_dev(f0)
+    if f0.ndim != 2:
+        raise ValueError(f"{name}: frames must be 2-D")
+    if f0.dtype not in _SAMPLE:
+        raise ValueError(f"{name}: sample type {f0.dtype} is not uint8, uint16 or float64")
+    if dev:
+        for i, f in enumerate(frames):
+            if not _is_dev(f) or f.device != f0.device:
+                raise ValueError(f"{name}[{i}]: not on {f0.device}")
+            if f.dtype != f0.dtype or tuple(f.shape) != tuple(f0.shape):
+                raise ValueError(f"{name}[{i}]: {f.dtype} {tuple(f.shape)}, frame 0 is "
+                                 f"{f0.dtype} {tuple(f0.shape)}")
+            if f.stride(-1) != 1 or f.stride(0) != f0.stride(0) or f.stride(0) < f0.shape[1]:
+                raise ValueError(f"{name}[{i}]: rows must be dense with the stride of frame 0")
+        stride = f0.stride(0) * f0.element_size()
+    else:
+        frames = [np.ascontiguousarray(f) for f in frames]
+        for i, f in enumerate(frames):
+            if f.dtype != frames[0].dtype or f.shape != frames[0].shape:
+                raise ValueError(f"{name}[{i}]: {f.dtype} {f.shape}, frame 0 is "
+                                 f"{frames[0].dtype} {frames[0].shape}")
+        stride = 0
+    rows, cols = frames[0].shape
+    return frames, dev, _SAMPLE[frames[0].dtype], stride, rows, cols
+
+
 def _out(like_dev, shape, dtype, device):
     if like_dev:
         return torch.empty(shape, dtype=dtype, device=device)
@@ -101,20 +164,9 @@ def decode(cfg: DecodeConfig, frames, ctx: Context | None = None, want_mod=True,
     one sample type (u8, u16 or f64). Returns (phase f64, k i32, mod f64,
     valid u8)."""
     ctx = ctx or default_context()
-    dev = _is_dev(frames[0])
-    if dev:
-        frames = [f if f.stride(-1) == 1 else f.contiguous() for f in frames]
-        st = _SAMPLE[frames[0].dtype]
-        stride = frames[0].stride(0) * frames[0].element_size()
-        ptrs = [f.data_ptr() for f in frames]
-        device = frames[0].device
-    else:
-        frames = [np.ascontiguousarray(f) for f in frames]
-        st = _SAMPLE[frames[0].dtype]
-        stride = 0
-        ptrs = [f.ctypes.data for f in frames]
-        device = None
-    rows, cols = frames[0].shape
+    frames, dev, st, stride, rows, cols = _frames(frames)
+    ptrs = [f.data_ptr() if dev else f.ctypes.data for f in frames]
+    device = frames[0].device if dev else None
     arr = (C.c_void_p * len(ptrs))(*ptrs)
     ph = _out(dev, (rows, cols), torch.float64, device)
     k = _out(dev, (rows, cols), torch.int32, device)
@@ -126,22 +178,31 @@ def decode(cfg: DecodeConfig, frames, ctx: Context | None = None, want_mod=True,
     return ph, k, md, vd
 
 
+def _recon_maps(lp, rp, lk, rk):
+    """The four rectified maps of a reconstruction: f64 phases and i32 orders
+    of one shape, all on the host or all on one device."""
+    dev = _is_dev(lp)
+    if getattr(lp, "ndim", 0) != 2:
+        raise ValueError("l_phase must be 2-D")
+    rows, cols = lp.shape
+    device = lp.device if dev else None
+    maps = [_map(n, a, dev, device, t, (rows, cols))
+            for n, a, t in (("l_phase", lp, torch.float64), ("r_phase", rp, torch.float64),
+                            ("l_k", lk, torch.int32), ("r_k", rk, torch.int32))]
+    return dev, device, rows, cols, maps
+
+
 def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1, ctx: Context | None = None):
     """Match -> median -> reproject -> Gaussian -> outlier -> MLS on rectified
     maps. Returns (xyz f32 HxWx3, P u8, disp f32, match_col i32)."""
     ctx = ctx or default_context()
-    dev = _is_dev(lp)
-    rows, cols = lp.shape
-    device = lp.device if dev else None
+    dev, device, rows, cols, (lp, rp, lk, rk) = _recon_maps(lp, rp, lk, rk)
     xyz = _out(dev, (rows, cols, 3), torch.float32, device)
     P = _out(dev, (rows, cols), torch.uint8, device)
     disp = _out(dev, (rows, cols), torch.float32, device)
     mc = _out(dev, (rows, cols), torch.int32, device)
     Qa = np.ascontiguousarray(Q, np.float64).reshape(16)
     Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
-    if not dev:
-        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
-        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
     _capi.check(ctx.lib.psgc_reconstruct(ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp),
                                          _capi.ptr(lk), _capi.ptr(rk), rows, cols, int(pha_f),
                                          _capi.ptr(Qa), _capi.ptr(Ra), _capi.ptr(xyz),
@@ -150,18 +211,13 @@ def reconstruct(cfg: ReconConfig, lp, rp, lk, rk, pha_f, Q, R1, ctx: Context | N
     return xyz, P, disp, mc
 
 
-def _coef_maps(coef, dev, rows, cols):
-    out = []
-    for c in coef:
-        if dev:
-            assert c.is_cuda and c.dtype == torch.float32 and c.is_contiguous()
-            assert tuple(c.shape) == (rows, cols, 4)
-            out.append(c)
-        else:
-            a = np.ascontiguousarray(c, np.float32)
-            assert a.shape == (rows, cols, 4)
-            out.append(a)
-    return out
+def _coef_maps(coef, dev, rows, cols, device=None):
+    if len(coef) != 3:
+        raise ValueError("coef: expected the three maps (cx, cy, cz)")
+    if dev and device is None:
+        device = coef[0].device if _is_dev(coef[0]) else None
+    return [_map(f"coef[{a}]", c, dev, device, torch.float32, (rows, cols, 4))
+            for a, c in enumerate(coef)]
 
 
 def reconstruct_poly(cfg: ReconConfig, lp, rp, lk, rk, pha_f, coef, R1,
@@ -170,18 +226,13 @@ def reconstruct_poly(cfg: ReconConfig, lp, rp, lk, rk, pha_f, coef, R1,
     triangulation. coef = (cx, cy, cz), each rows x cols x 4 float32 (the
     _rx/_ry/_rz calibration maps). Returns (xyz, P, disp, match_col)."""
     ctx = ctx or default_context()
-    dev = _is_dev(lp)
-    rows, cols = lp.shape
-    device = lp.device if dev else None
+    dev, device, rows, cols, (lp, rp, lk, rk) = _recon_maps(lp, rp, lk, rk)
     xyz = _out(dev, (rows, cols, 3), torch.float32, device)
     P = _out(dev, (rows, cols), torch.uint8, device)
     disp = _out(dev, (rows, cols), torch.float32, device)
     mc = _out(dev, (rows, cols), torch.int32, device)
-    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols, device)
     Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
-    if not dev:
-        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
-        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
     _capi.check(ctx.lib.psgc_reconstruct_poly(ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp),
                                               _capi.ptr(lk), _capi.ptr(rk), rows, cols,
                                               int(pha_f), _capi.ptr(cx), _capi.ptr(cy),
@@ -198,11 +249,10 @@ def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1,
     dev = _is_dev(disp)
     rows, cols = disp.shape
     device = disp.device if dev else None
+    disp = _map("disp", disp, dev, device, torch.float64, (rows, cols))
     xyz = _out(dev, (rows, cols, 3), torch.float32, device)
     P = _out(dev, (rows, cols), torch.uint8, device)
-    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
-    if not dev:
-        disp = np.ascontiguousarray(disp, np.float64)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols, device)
     Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
     _capi.check(ctx.lib.psgc_reproject_poly(ctx.h, _capi.ptr(disp), rows, cols, _capi.ptr(cx),
                                             _capi.ptr(cy), _capi.ptr(cz), _capi.ptr(Ra),
@@ -213,15 +263,19 @@ def reproject_poly(disp, coef, R1, min_depth=0.0, max_depth=1000.0, use_rect=1,
 
 
 def _cloud_in(xyz, valid):
+    """An unorganized cloud (N x 3 f32) and its optional mask (N u8)."""
     dev = _is_dev(xyz)
     if dev:
-        xyz = xyz.reshape(-1, 3).contiguous()
-        if valid is not None:
-            valid = valid.reshape(-1).to(torch.uint8).contiguous()
+        if xyz.numel() % 3:
+            raise ValueError("xyz: expected N x 3 coordinates")
+        xyz = _map("xyz", xyz.reshape(-1, 3), True, xyz.device, torch.float32,
+                   (xyz.numel() // 3, 3))
     else:
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
-        if valid is not None:
-            valid = np.ascontiguousarray(valid, np.uint8).reshape(-1)
+    if valid is not None:
+        n = xyz.shape[0]
+        valid = _map("valid", valid.reshape(-1) if dev else np.asarray(valid).reshape(-1), dev,
+                     xyz.device if dev else None, torch.uint8, (n,))
     return dev, xyz, valid
 
 
@@ -296,10 +350,11 @@ def roi_right_window(lk, rk, pha_f, left_rect, mask, ctx: Context | None = None)
     ctx = ctx or default_context()
     dev = _is_dev(lk)
     rows, cols = lk.shape
-    lr = np.asarray(left_rect, np.int32)
-    if not dev:
-        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
-        mask = np.ascontiguousarray(mask, np.uint8)
+    device = lk.device if dev else None
+    lr = np.asarray(left_rect, np.int32).reshape(4)
+    lk = _map("l_k", lk, dev, device, torch.int32, (rows, cols))
+    rk = _map("r_k", rk, dev, device, torch.int32, (rows, cols))
+    mask = _map("mask", mask, dev, device, torch.uint8, (int(lr[3]), int(lr[2])))
     rr = np.zeros(4, np.int32)
     _capi.check(ctx.lib.psgc_roi_right_window(ctx.h, _capi.ptr(lk), _capi.ptr(rk), rows, cols,
                                               int(pha_f), _capi.ptr(lr), _capi.ptr(mask),
@@ -313,19 +368,15 @@ def reconstruct_poly_roi(cfg: ReconConfig, lp, rp, lk, rk, pha_f, left_rect, rig
     = (x, y, w, h), mask of left_rect's size). Returns crop-sized
     (xyz, P, disp)."""
     ctx = ctx or default_context()
-    dev = _is_dev(lp)
-    rows, cols = lp.shape
-    lr, rr = np.asarray(left_rect, np.int32), np.asarray(right_rect, np.int32)
-    device = lp.device if dev else None
+    dev, device, rows, cols, (lp, rp, lk, rk) = _recon_maps(lp, rp, lk, rk)
+    lr = np.asarray(left_rect, np.int32).reshape(4)
+    rr = np.asarray(right_rect, np.int32).reshape(4)
     h, w = int(lr[3]), int(lr[2])
+    mask = _map("mask", mask, dev, device, torch.uint8, (h, w))
     xyz = _out(dev, (h, w, 3), torch.float32, device)
     P = _out(dev, (h, w), torch.uint8, device)
     disp = _out(dev, (h, w), torch.float32, device)
-    cx, cy, cz = _coef_maps(coef, dev, rows, cols)
-    if not dev:
-        lp, rp = np.ascontiguousarray(lp, np.float64), np.ascontiguousarray(rp, np.float64)
-        lk, rk = np.ascontiguousarray(lk, np.int32), np.ascontiguousarray(rk, np.int32)
-        mask = np.ascontiguousarray(mask, np.uint8)
+    cx, cy, cz = _coef_maps(coef, dev, rows, cols, device)
     Ra = np.ascontiguousarray(R1, np.float64).reshape(9)
     _capi.check(ctx.lib.psgc_reconstruct_poly_roi(
         ctx.h, C.byref(cfg), _capi.ptr(lp), _capi.ptr(rp), _capi.ptr(lk), _capi.ptr(rk), rows,
@@ -366,15 +417,24 @@ class Batch:
         self._keep = []
         for i, p in enumerate(pairs):
             L, R = p["left"], p["right"]
-            rows, cols = L[0].shape
+            L, ldev, lst, lstride, rows, cols = _frames(L, f"pairs[{i}].left")
+            R, rdev, rst, rstride, rrows, rcols = _frames(R, f"pairs[{i}].right")
+            if not (ldev and rdev) or R[0].device != L[0].device:
+                raise ValueError(f"pairs[{i}]: frames must be device tensors on one device")
+            if len(L) != len(R) or len(L) > len(descs[i].left):
+                raise ValueError(f"pairs[{i}]: {len(L)} left and {len(R)} right frames "
+                                 f"(equal counts of at most {len(descs[i].left)})")
+            if (rst, rstride, rrows, rcols) != (lst, lstride, rows, cols):
+                raise ValueError(f"pairs[{i}]: right frames differ from the left in type, "
+                                 f"shape or row stride")
             d = descs[i]
             for j, f in enumerate(L):
                 d.left[j] = f.data_ptr()
             for j, f in enumerate(R):
                 d.right[j] = f.data_ptr()
             d.n_frames = len(L)
-            d.sample_type = _SAMPLE[L[0].dtype]
-            d.row_stride_bytes = L[0].stride(0) * L[0].element_size()
+            d.sample_type = lst
+            d.row_stride_bytes = lstride
             d.rows, d.cols = rows, cols
             d.decode = p["decode"]
             d.recon = p["recon"]
@@ -390,7 +450,7 @@ class Batch:
             d.P = out["P"].data_ptr()
             d.disp = out["disp"].data_ptr() if "disp" in out else None
             if p.get("coef") is not None:
-                cm = _coef_maps(p["coef"], True, rows, cols)
+                cm = _coef_maps(p["coef"], True, rows, cols, dev)
                 for a in range(3):
                     d.coef[a] = cm[a].data_ptr()
                 self._keep.append(cm)
@@ -517,10 +577,10 @@ def applyMLSFilterToPointCloud(src_xyz, P, window_size, spatial_sigma, use_gpu=T
     ctx = ctx or default_context()
     dev = _is_dev(src_xyz)
     rows, cols = P.shape
-    dst = _out(dev, (rows, cols, 3), torch.float32, src_xyz.device if dev else None)
-    if not dev:
-        src_xyz = np.ascontiguousarray(src_xyz, np.float32)
-        P = np.ascontiguousarray(P, np.uint8)
+    device = src_xyz.device if dev else None
+    src_xyz = _map("src_xyz", src_xyz, dev, device, torch.float32, (rows, cols, 3))
+    P = _map("P", P, dev, device, torch.uint8, (rows, cols))
+    dst = _out(dev, (rows, cols, 3), torch.float32, device)
     _capi.check(ctx.lib.psgc_filter_mls(ctx.h, _capi.ptr(src_xyz), _capi.ptr(P), _capi.ptr(dst),
                                         rows, cols, int(window_size), float(spatial_sigma),
                                         int(polynomial_order), float(max_radius),
