@@ -5,13 +5,16 @@ full-size single-pair shape), three-frequency heterodyne 4-step fringes
 (70/64/59 across a 1280-px projector), uint16 12-bit, rendered on the GPU
 before timing. One step = decode L + decode R + reconstruct (order match,
 5x5 median, Q reprojection, 5x5 Gaussian sigma 1, SOR k=30, MLS order 2 at
-1.5 mm, 2 iterations) of every pair, frames resident in HBM.
+1.5 mm in a window that holds that radius at the pair's nearest depth,
+2 iterations) of every pair, frames resident in HBM.
 
 value = Mpts/s of the batched pipeline (one hipGraph replay per step; one
-point = one left rectified pixel). The plain per-pair GPU path (one
-launch per stage per pair, one-thread-per-pixel kernels, a sync after
-each pair, as the reference GPU path is structured) is timed on the same
-pairs beside it. Prints one JSON line.
+point = one left rectified pixel). Beside it, on the same pairs: the plain
+per-pair GPU path (one launch per stage per pair, one-thread-per-pixel
+kernels, a sync after each pair, as the reference GPU path is structured)
+and the tiled kernels run one pair at a time (one graph per pair, a sync
+after each), which separates what batching gains from what the kernels
+gain. Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -38,10 +41,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=64)
-    ap.add_argument("--plain-pairs", type=int, default=8,
-                    help="pairs timed through the plain per-pair path (subset)")
+    ap.add_argument("--plain-pairs", type=int, default=None,
+                    help="pairs timed through the per-pair paths (default: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-per-pair-tiled", action="store_true")
     ap.add_argument("--cpu-sample-rows", type=int, default=ROWS)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the measurement, copy 1 GiB device-to-device once (a kernel "
@@ -78,13 +82,15 @@ ALG_BYTES_PER_PX = {
 }
 
 
+MLS_RADIUS = 1.5  # mm
+
+
 # VALU lane-instructions per pixel of one launch (SQ_INSTS_VALU x 64 / pixels)
 # on this workload, from the rocprofv3 counter run summarized in
 # profiles/c2_8pair_valu_counters.md; with the live launch time they give the
 # kernel's VALU issue rate against the chip's single-issue peak (256 CUs x 4
 # SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
-VALU_PER_PX_PROFILED = {"mls": 11385, "sor_knn": 1790, "decode": 897, "match": 578,
-                        "repro_gauss": 504, "median5": 272}
+VALU_PER_PX_PROFILED = {}
 VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
 
 
@@ -114,7 +120,6 @@ def main():
     ctx = zim_amd.Context(0)
     lib = ctx.lib
     dcfg = scenes.c2_decode_config()
-    rcfg = scenes.c2_recon_config()
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     log(f"rendering {args.pairs} pairs of {COLS}x{ROWS} on the GPU")
@@ -123,11 +128,16 @@ def main():
     torch.cuda.synchronize()
     log(f"rendered in {time.time() - t0:.1f}s")
     npix = ROWS * COLS
+    # each pair's MLS window holds the radius at its nearest depth
+    windows = [scenes.c2_window(p, MLS_RADIUS) for p in pairs]
+    rcfgs = [scenes.c2_recon_config(mls_window_size=w) for w in windows]
+    log(f"MLS half windows {min(windows)}-{max(windows)}, mean {sum(windows) / len(windows):.2f}")
 
     # ---------------------------------------------------------- batched path
     t0 = time.time()
-    batch = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rcfg, pha_f=70,
-                                Q=p.Q, R1=p.R1) for p in pairs], ctx, graph=True, autotune=True)
+    batch = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rc, pha_f=70,
+                                Q=p.Q, R1=p.R1) for p, rc in zip(pairs, rcfgs)], ctx, graph=True,
+                          autotune=True)
     torch.cuda.synchronize()
     grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "match_threads",
                                         "median5", "repro_gauss", "sor_knn", "mls",
@@ -174,15 +184,55 @@ def main():
         compute = {"bound": "valu", "wave_insts_per_px": round(VALU_PER_PX_PROFILED[dom_name] / 64, 1),
                    "achieved_ginst_s": round(gi, 1), "single_issue_peak_ginst_s": VALU_PEAK_GINST_S,
                    "frac": round(gi / VALU_PEAK_GINST_S, 3),
-                   "insts_from": "profiles/c2_8pair_valu_counters.md"}
+                   "insts_from": "rocprofv3 SQ_INSTS_VALU on this workload, "
+                                 "profiles/c2_8pair_valu_counters.md"}
     log(f"stage ms/step {stage_ms}")
+
+    n_sub = args.plain_pairs if args.plain_pairs else args.pairs
+
+    def agreement(outs):
+        """Mask pixels and max |xyz| difference against the batched outputs."""
+        mism, maxerr = 0, 0.0
+        for o, bo in zip(outs, batch.outputs):
+            mism += int((o["P"] != bo["P"]).sum().item())
+            m = bo["P"] != 0
+            if m.any():
+                maxerr = max(maxerr, float((o["xyz"][m] - bo["xyz"][m]).abs().max().item()))
+        return mism, maxerr
+
+    # ------------------------------------- tiled kernels, one pair at a time
+    per_pair_tiled = None
+    if not args.no_per_pair_tiled:
+        singles = [zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rc,
+                                       pha_f=70, Q=p.Q, R1=p.R1)], ctx, graph=True)
+                   for p, rc in zip(pairs[:n_sub], rcfgs[:n_sub])]
+
+        def run_singles():
+            for b in singles:
+                b.run()
+                torch.cuda.synchronize()
+
+        run_singles()
+        t0 = time.perf_counter()
+        run_singles()
+        single_ms = (time.perf_counter() - t0) * 1e3
+        mism, maxerr = agreement([b.outputs[0] for b in singles])
+        per_pair_tiled = {"mpts_per_s": round(npix * n_sub / (single_ms * 1e-3) / 1e6, 3),
+                          "ms_per_pair": round(single_ms / n_sub, 3), "pairs": n_sub,
+                          "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr}
+        for b in singles:
+            b.close()
+        del singles
+        torch.cuda.empty_cache()
+        log(f"tiled kernels, one pair at a time: {per_pair_tiled}")
 
     # ----------------------------------------------------- plain per-pair path
     plain = None
     if not args.no_plain:
         ctx.set_kernels("plain")
         ctx.bind_stream()
-        sub = pairs[:args.plain_pairs]
+        sub = pairs[:n_sub]
+        sub_rc = rcfgs[:n_sub]
         bufs = [dict(lph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
                      rph=torch.empty((ROWS, COLS), dtype=torch.float64, device=dev),
                      lk=torch.empty((ROWS, COLS), dtype=torch.int32, device=dev),
@@ -194,7 +244,7 @@ def main():
         Ra = [np.ascontiguousarray(p.R1, np.float64) for p in sub]
 
         def run_plain():
-            for p, o, q, r in zip(sub, bufs, Qa, Ra):
+            for p, o, q, r, rcfg in zip(sub, bufs, Qa, Ra, sub_rc):
                 for frames, ph, k in ((p.left, o["lph"], o["lk"]), (p.right, o["rph"], o["rk"])):
                     arr = (C.c_void_p * len(frames))(*[f.data_ptr() for f in frames])
                     _capi.check(lib.psgc_decode(ctx.h, C.byref(dcfg), arr, len(frames),
@@ -217,13 +267,7 @@ def main():
         plain_ms = (time.perf_counter() - t0) * 1e3
         plain_val = npix * len(sub) / (plain_ms * 1e-3) / 1e6
         # the two paths must agree on the pairs both ran
-        mism = 0
-        maxerr = 0.0
-        for o, bo in zip(bufs, batch.outputs):
-            mism += int((o["P"] != bo["P"]).sum().item())
-            m = bo["P"] != 0
-            if m.any():
-                maxerr = max(maxerr, float((o["xyz"][m] - bo["xyz"][m]).abs().max().item()))
+        mism, maxerr = agreement(bufs)
         plain = {"mpts_per_s": round(plain_val, 3), "ms_per_pair": round(plain_ms / len(sub), 3),
                  "pairs": len(sub), "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr}
         ctx.set_kernels("tiled")
@@ -241,7 +285,7 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
-        cpu, ref = cpu_baseline(pairs[0], dcfg, rcfg, args.cpu_sample_rows, log)
+        cpu, ref = cpu_baseline(pairs[0], dcfg, rcfgs[0], args.cpu_sample_rows, log)
         if ref is not None:
             # the oracle's output for the whole of pair 0 checks the batched one
             rx, rP = ref
@@ -249,7 +293,8 @@ def main():
             gP = out["P"].cpu().numpy()
             gx = out["xyz"].cpu().numpy()
             m = rP != 0
-            parity = {"pair": 0, "points": int(m.sum()),
+            parity = {"pair": 0, "points": int(m.sum()), "mls_half_window": windows[0],
+                      "ring_points": scenes.ring_points(rx, rP, windows[0], MLS_RADIUS),
                       "mask_mismatch_px": int((gP != rP).sum()),
                       "xyz_max_diff_mm": float(np.abs(gx[m] - rx[m]).max()) if m.any() else 0.0,
                       "xyz_p99999_diff_mm": float(np.quantile(np.abs(gx[m] - rx[m]).max(-1),
@@ -263,7 +308,8 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
-        if (tj.get("workload") == {"pairs": args.pairs, "rows": ROWS, "cols": COLS}
+        if (tj.get("workload") == {"pairs": args.pairs, "rows": ROWS, "cols": COLS,
+                                   "mls_windows": "per pair"}
                 and dom_name in tj.get("kernels", {})):
             traffic = int(tj["kernels"][dom_name]["traffic_bytes"])
     # whole pipeline: frames of both cameras read once, xyz + mask written once
@@ -285,16 +331,23 @@ def main():
         "data": "synthetic (seeded bump height fields rendered on the GPU)",
         "config": {"workload": f"{args.pairs} seeded 2448x2048 pairs, 3-freq 70/64/59 4-step "
                                "u16 12-bit; median 5x5, Gaussian 5x5 s=1, SOR k=30 (9x9 window), "
-                               "MLS order 2 r=1.5 mm (13x13 window, 2 iterations)",
-                   "pairs": args.pairs, "rows": ROWS, "cols": COLS, "path": "batched graph"},
+                               "MLS order 2 r=1.5 mm (per pair the half window that holds r at "
+                               "the pair's nearest depth, ceil(r f / z_min + 0.5); 2 iterations)",
+                   "pairs": args.pairs, "rows": ROWS, "cols": COLS, "path": "batched graph",
+                   "mls_half_windows": {str(w): windows.count(w) for w in sorted(set(windows))}},
         "valid_mpts_per_s": round(valid / (ms_per_step * 1e-3) / 1e6, 3),
         "per_pair_plain": plain,
+        "per_pair_tiled": per_pair_tiled,
         "speedup_vs_per_pair": round(value / plain["mpts_per_s"], 2) if plain else None,
+        "batching_gain": (round(value / per_pair_tiled["mpts_per_s"], 2)
+                          if per_pair_tiled else None),
         "stage_ms_per_step": stage_ms,
         "autotuned_grids": grids,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_from": ("rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                      "profiles/c2_64pair_traffic.json") if traffic else None,
                      "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg,
                      "end_to_end": {"alg_bytes_per_pt": 61, "achieved": round(e2e_gbs, 2),
                                     "frac": round(e2e_gbs / HBM_PEAK_GBS, 5)},
@@ -717,7 +770,17 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
         runs.append(time.perf_counter() - t0)
     dt = sorted(runs)[len(runs) // 2]
     pts = sample_rows * COLS
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     res = {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
+           "cpu": model,
            "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
                      f"decode L+R + reconstruct, median of 5 runs {dt:.2f} s "
                      f"(range {min(runs):.2f}-{max(runs):.2f} s)"}

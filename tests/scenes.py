@@ -170,3 +170,33 @@ def closeup(seed=41, rows=128, cols=192, noise=4.0):
                       modulation=750.0, noise=noise, poisson=False, max_val=4095,
                       dtype=torch.uint16)
     return p, [x.numpy() for x in p.left], [x.numpy() for x in p.right]
+
+
+def ring_points(xyz, P, hw, radius):
+    """Valid points with a valid neighbour within `radius` on the outer ring
+    of their (2 hw + 1)^2 window (Chebyshev distance exactly hw): 0 means a
+    window one pixel wider would select no further neighbour, i.e. the
+    window holds the radius for this cloud."""
+    rows, cols = P.shape
+    X = np.asarray(xyz, np.float64)
+    valid = np.asarray(P) > 0
+    hit = np.zeros_like(valid)
+    r2 = float(radius) ** 2
+    for dy in range(-hw, hw + 1):
+        for dx in range(-hw, hw + 1):
+            if max(abs(dy), abs(dx)) != hw:
+                continue
+            ys, ye = max(0, -dy), min(rows, rows - dy)
+            xs, xe = max(0, -dx), min(cols, cols - dx)
+            a = X[ys:ye, xs:xe]
+            b = X[ys + dy:ye + dy, xs + dx:xe + dx]
+            both = valid[ys:ye, xs:xe] & valid[ys + dy:ye + dy, xs + dx:xe + dx]
+            hit[ys:ye, xs:xe] |= both & (((a - b) ** 2).sum(-1) <= r2)
+    return int(hit.sum())
+
+
+def c2_window(pair, radius=1.5):
+    """The MLS half window of a config-2 pair: holds `radius` at the pair's
+    nearest depth."""
+    import zim_amd
+    return zim_amd.mls_half_window(radius, pair.cam.f, float(pair.z_left.min()))

@@ -207,6 +207,31 @@ def test_mls(kctx, order, sem):
     assert err <= XYZ_TOL
 
 
+@pytest.mark.parametrize("sem", [0, 1])
+def test_mls_non_finite_points(kctx, sem):
+    """Valid points with a NaN or infinite coordinate: the reference's radius
+    test (d^2 > r^2 rejects) keeps a NaN distance, so every point that has
+    such a neighbour comes out NaN, and infinite distances are rejected;
+    both kernel sets give the oracle's result (NaN where it has NaN)."""
+    ctx = kctx
+    xyz, P = scenes.rng_points(48, 64, seed=9, holes=0.2)
+    P[10, 10] = P[30, 40] = P[20, 5] = P[40, 60] = 1
+    xyz[10, 10, 0] = np.nan          # one coordinate NaN
+    xyz[30, 40] = np.nan             # every coordinate NaN
+    xyz[20, 5, 2] = np.inf           # infinite: never a neighbour
+    xyz[40, 60, 1] = -np.inf
+    for hw in (3, 9):
+        ref = oracle.mls(xyz, P, hw, 0.5, 2, 1.5, 1, sem)
+        got = np.empty_like(ref)
+        _capi.check(ctx.lib.psgc_filter_mls(ctx.h, _capi.ptr(xyz), _capi.ptr(P), _capi.ptr(got),
+                                            48, 64, hw, 0.5, 2, 1.5, 1, sem, _capi.MEM_HOST))
+        nan_ref = np.isnan(ref).any(-1)
+        assert 10 < nan_ref.sum() < 0.2 * P.size       # NaN spreads to the neighbourhoods
+        assert np.array_equal(np.isnan(got), np.isnan(ref)), (hw, (np.isnan(got) != np.isnan(ref)).sum())
+        fin = ~np.isnan(ref)
+        assert np.array_equal(got[fin], ref[fin]), hw
+
+
 @pytest.mark.parametrize("variant", ["cpu_ref", "gpu_ref", "c2"])
 def test_reconstruct_chain(kctx, variant):
     ctx = kctx
@@ -345,35 +370,39 @@ def _inradius_counts(xyz, P, hw, r):
 def test_batch_mls_variants_match_oracle(ctx):
     """Both MLS kernels, pinned in turn on one captured batch: weights
     evaluated in every pass, and weights kept in LDS between the passes
-    (points with more than 24 in-radius neighbours evaluate the rest
-    again); each against the CPU restatement. The batch holds the pair
-    twice, under GPU semantics (fused sums) and CPU semantics (rounded
-    products and sums), so each MLS launch serves one of them."""
+    (points with more in-radius neighbours than the cache holds evaluate the
+    rest again); each against the CPU restatement. The batch holds the pair
+    four times, with half windows 6, 8 and 10 (one launch per window class,
+    caches of 24, 20 and 16 weights) under GPU semantics (fused sums) and
+    CPU semantics (rounded products and sums)."""
     dcfg = scenes.c2_decode_config()
-    # 13 x 13 window; a 2.5 mm radius on this ~1 m deep scene gives about
-    # half the points more than 24 neighbours (the full-size 5 MP scene holds
-    # 40-60 within 1.5 mm)
-    rcfg = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0)
+    # a 2.5 mm radius on this ~1 m deep scene gives many points more
+    # neighbours than any cache holds
     p, L, R, lh, rh = _pair_inputs(41, 128, 192)
     Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
-    rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
     nx, nP, _, _ = oracle.reconstruct(scenes.c2_recon_config(use_mls=0), Lr[0], Rr[0], Lr[1],
                                       Rr[1], 70, p.Q, p.R1)
-    cnt = _inradius_counts(nx, nP, 6, 2.5)
-    print("in-radius neighbours: max", cnt.max(), "points above 24:", int((cnt > 24).sum()))
-    assert (cnt > 24).sum() > 100 and ((cnt > 3) & (cnt <= 24)).sum() > 100
-    ccfg = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0,
-                                  semantics=_capi.SEM_CPU)
-    cx, cP, _, _ = oracle.reconstruct(ccfg, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)
-    m = (rP != 0) & (cP != 0)
-    assert not np.array_equal(rx[m], cx[m])   # the two roundings differ somewhere
+    specs = [(6, _capi.SEM_GPU, 24), (6, _capi.SEM_CPU, 24), (8, _capi.SEM_GPU, 20),
+             (10, _capi.SEM_CPU, 16)]
+    cfgs, refs = [], []
+    for hw, sem, kc in specs:
+        cnt = _inradius_counts(nx, nP, hw, 2.5)
+        print(f"hw {hw}: in-radius neighbours max {cnt.max()}, points above {kc}: "
+              f"{int((cnt > kc).sum())}", flush=True)
+        assert (cnt > kc).sum() > 100 and ((cnt > 3) & (cnt <= kc)).sum() > 100
+        c = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0,
+                                   mls_window_size=hw, semantics=sem)
+        cfgs.append(c)
+        refs.append(oracle.reconstruct(c, Lr[0], Rr[0], Lr[1], Rr[1], 70, p.Q, p.R1)[:2])
+    m = (refs[0][1] != 0) & (refs[1][1] != 0)
+    assert not np.array_equal(refs[0][0][m], refs[1][0][m])   # the two roundings differ
     b = zim_amd.Batch([dict(left=L, right=R, decode=dcfg, recon=c, pha_f=70, Q=p.Q, R1=p.R1)
-                       for c in (rcfg, ccfg)], ctx, graph=True)
+                       for c in cfgs], ctx, graph=True)
     for variant in (1, 0, 1):
         b.set_grid("mls_variant", variant)
         outs = b.run()
         torch.cuda.synchronize()
-        for out, (ex, eP) in zip(outs, ((rx, rP), (cx, cP))):
+        for out, (ex, eP) in zip(outs, refs):
             assert np.array_equal(host(out["P"]), eP)
             assert np.array_equal(host(out["xyz"]), ex), np.abs(host(out["xyz"]) - ex).max()
     b.close()
@@ -474,14 +503,20 @@ def test_batch_at_uneven_full_sizes_matches_oracle(ctx):
     b.close()
 
 
-def test_full_size_pair_matches_oracle(ctx):
-    """One full 2448x2048 config-2 pair (rendered on the GPU, decoded and
+@pytest.mark.parametrize("seed", [1000, 1031])
+def test_full_size_pair_matches_oracle(ctx, seed):
+    """A full 2448x2048 config-2 pair (rendered on the GPU, decoded and
     reconstructed by the batched pipeline) against the oracle on the same
-    frames: masks bit-exact, XYZ within 1e-3 mm over ~4 M points."""
+    frames: masks bit-exact, XYZ within 1e-3 mm over millions of points. The
+    MLS window is the pair's own (seed 1031 is the nearest pair of the
+    benchmark, half window 10), and no in-radius neighbour of the MLS input
+    lies on its outer ring: the window holds the 1.5 mm radius."""
     from zim_amd import synth
-    p = synth.render_c2(1000, device="cuda")
+    p = synth.render_c2(seed, device="cuda")
     dcfg = scenes.c2_decode_config()
-    rcfg = scenes.c2_recon_config()
+    hw = scenes.c2_window(p)
+    rcfg = scenes.c2_recon_config(mls_window_size=hw)
+    print(f"[full-size] seed {seed}: MLS half window {hw}", flush=True)
     b = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg, recon=rcfg, pha_f=70,
                             Q=p.Q, R1=p.R1)], ctx, graph=False)
     out = b.run()[0]
@@ -494,6 +529,9 @@ def test_full_size_pair_matches_oracle(ctx):
     m = rP != 0
     print("points", int(m.sum()), "max |dxyz|", np.abs(gx[m] - rx[m]).max())
     assert m.sum() > 2_000_000
+    sx, sP, _, _ = oracle.reconstruct(scenes.c2_recon_config(use_mls=0), Lr[0], Rr[0], Lr[1],
+                                      Rr[1], 70, p.Q, p.R1)
+    assert scenes.ring_points(sx, sP, hw, 1.5) == 0
     assert np.array_equal(gP, rP), f"mask mismatch {(gP != rP).sum()}"
     assert np.abs(gx - rx).max() <= XYZ_TOL
     b.close()

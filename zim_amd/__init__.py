@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import math
 
 import numpy as np
 import torch
@@ -83,6 +84,16 @@ def default_context(device: int = 0) -> Context:
     if device not in _default_ctx:
         _default_ctx[device] = Context(device)
     return _default_ctx[device].bind_stream()
+
+
+def mls_half_window(radius, f, z_min):
+    """A half window that holds an MLS radius (mm) around every point of a
+    pair whose nearest surface lies at z_min (mm), for focal length f (px):
+    r f / z_min pixels span the radius on a fronto-parallel surface, and the
+    half-pixel margin covers points off the optical axis and tilted
+    surfaces. The organized window of the reference's filter
+    (JSLAB_MLS_Filter.cpp:191-194) is otherwise a cut through the ball."""
+    return int(math.ceil(radius * f / z_min + 0.5))
 
 
 def _is_dev(x):

@@ -368,6 +368,17 @@ void hdr_table_bind(DecodeArgs* a, const char* dev) {
   }
 }
 
+int check_mls_weights(float sigma, float max_radius) {
+  if (!(sigma > 0.0f) || !std::isfinite(sigma))
+    PSGC_FAIL(PSGC_E_INVALID, "MLS spatial sigma must be finite and > 0");
+  if (!std::isfinite(1.0f / (2.0f * sigma * sigma)))
+    PSGC_FAIL(PSGC_E_INVALID, "MLS spatial sigma %g too small: 1/(2 sigma^2) overflows", sigma);
+  const float r = max_radius > 0.0f ? max_radius : 3.0f * sigma;
+  if (!std::isfinite(r * r))
+    PSGC_FAIL(PSGC_E_INVALID, "MLS radius %g: its square overflows", max_radius);
+  return PSGC_OK;
+}
+
 }  // namespace psgc
 
 using namespace psgc;
@@ -872,7 +883,7 @@ int psgc_filter_mls(psgc_ctx* ctx, const float* src_xyz, const uint8_t* P, float
   PSGC_TRY(check_size(rows, cols));
   if (!src_xyz || !P || !dst_xyz) PSGC_FAIL(PSGC_E_INVALID, "null argument");
   if (window_size < 0 || window_size > 256) PSGC_FAIL(PSGC_E_INVALID, "window=%d", window_size);
-  if (!(spatial_sigma > 0)) PSGC_FAIL(PSGC_E_INVALID, "spatial_sigma must be > 0");
+  PSGC_TRY(check_mls_weights(spatial_sigma, max_radius));
   const size_t npix = (size_t)rows * cols;
   const float r2 = (max_radius > 0.0f) ? max_radius * max_radius
                                        : 9.0f * spatial_sigma * spatial_sigma;
@@ -930,8 +941,7 @@ int reconstruct_impl(psgc_ctx* ctx, const psgc_recon_config* cfg, const double* 
   if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
   if (cfg->outlier_mode < PSGC_OUTLIER_NONE || cfg->outlier_mode > PSGC_OUTLIER_SOR)
     PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", cfg->outlier_mode);
-  if (cfg->use_mls && !(cfg->mls_spatial_sigma > 0))
-    PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+  if (cfg->use_mls) PSGC_TRY(check_mls_weights(cfg->mls_spatial_sigma, cfg->mls_max_radius));
   const size_t npix = (size_t)rows * cols;
   const double *lp, *rp;
   const int32_t *lk, *rk;
@@ -1192,8 +1202,7 @@ int psgc_reconstruct_poly_roi(psgc_ctx* ctx, const psgc_recon_config* cfg, const
   if (gw < 1 || gw > 7 || (gw & 1) == 0) PSGC_FAIL(PSGC_E_INVALID, "gauss_window=%d", gw);
   if (cfg->outlier_mode < PSGC_OUTLIER_NONE || cfg->outlier_mode > PSGC_OUTLIER_SOR)
     PSGC_FAIL(PSGC_E_INVALID, "outlier_mode=%d", cfg->outlier_mode);
-  if (cfg->use_mls && !(cfg->mls_spatial_sigma > 0))
-    PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+  if (cfg->use_mls) PSGC_TRY(check_mls_weights(cfg->mls_spatial_sigma, cfg->mls_max_radius));
   const size_t npix = (size_t)rows * cols;
   const int lw = left_rect[2], h = left_rect[3];
   const size_t n = (size_t)lw * h;

@@ -8,6 +8,7 @@
 // kernels), so a captured hipGraph replays the full workload with one call.
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -140,7 +141,8 @@ struct Plan {
   Table<MatchJob> mat;
   Table<CloudJob> med, rg, sor;
   Table<SorStatJob> sst;             // SOR statistics, one job per pair
-  std::vector<Table<CloudJob>> mls;
+  // MLS jobs by iteration and window class (one launch each)
+  std::vector<std::array<Table<CloudJob>, kMlsClasses>> mls;
   std::vector<PairPost> post;
   std::vector<DecodeArgs> gray_fix;       // gray-code jump correction per camera
   std::vector<const double*> gray_in_ph;
@@ -202,7 +204,8 @@ int upload_tables(Plan* pl) {
   PSGC_TRY(put(pl->rg));
   PSGC_TRY(put(pl->sor));
   PSGC_TRY(put(pl->sst));
-  for (auto& t : pl->mls) PSGC_TRY(put(t));
+  for (auto& it : pl->mls)
+    for (auto& t : it) PSGC_TRY(put(t));
   return PSGC_OK;
 }
 
@@ -218,7 +221,8 @@ size_t table_bytes(const Plan* pl) {
   add(pl->rg);
   add(pl->sor);
   add(pl->sst);
-  for (auto& t : pl->mls) add(t);
+  for (auto& it : pl->mls)
+    for (auto& t : it) add(t);
   return b;
 }
 
@@ -258,15 +262,16 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
       PSGC_FAIL(PSGC_E_INVALID, "sor_window=%d", rc.sor_window);
   }
   if (rc.use_mls) {
-    if (!(rc.mls_spatial_sigma > 0)) PSGC_FAIL(PSGC_E_INVALID, "mls_spatial_sigma must be > 0");
+    PSGC_TRY(check_mls_weights(rc.mls_spatial_sigma, rc.mls_max_radius));
     if (rc.mls_window_size < 0 || rc.mls_window_size > 256)
       PSGC_FAIL(PSGC_E_INVALID, "mls_window_size=%d", rc.mls_window_size);
   }
-  // Beyond what the tiled kernels hold (windows of half size 8, a row and its
-  // order spans in the match's LDS budget) a pair's stage runs on the
-  // one-thread-per-pixel kernels instead, with the same results.
-  const bool sor_plain = rc.outlier_mode == PSGC_OUTLIER_SOR && rc.sor_window > kTiledMaxHalf;
-  const bool mls_plain = rc.use_mls && rc.mls_window_size > kTiledMaxHalf;
+  // Beyond what the tiled kernels hold (SOR and MLS windows of half size 8
+  // and 10, a row and its order spans in the match's LDS budget) a pair's
+  // stage runs on the one-thread-per-pixel kernels instead, with the same
+  // results.
+  const bool sor_plain = rc.outlier_mode == PSGC_OUTLIER_SOR && rc.sor_window > kSorTiledMaxHalf;
+  const bool mls_plain = rc.use_mls && rc.mls_window_size > kMlsTiledMaxHalf;
   const bool match_plain = match_tiled_lds(cols, pha_f) > (size_t)match_tiled_max_lds();
   const bool cpu = rc.semantics == PSGC_SEM_CPU;
   const bool use64 = !rc.use_disp_median_filter && cpu;
@@ -493,7 +498,7 @@ static int plan_reconstruct(Plan* pl, int i, const psgc_recon_config& rc, int ro
     const float* src = mls_src;
     for (int it = 0; it < n_it; ++it) {
       CloudJob c = base;
-      Table<CloudJob>& tab = pl->mls[it];
+      Table<CloudJob>& tab = pl->mls[it][mls_class(rc.mls_window_size)];
       c.tile_base = tab.total;
       tab.total += ntile;
       float* dst = (it == n_it - 1) ? xyz_out : ((it % 2 == 0) ? t1 : t0);
@@ -594,10 +599,11 @@ static int run_plan(Plan* pl, hipStream_t s) {
       }
     }
   }
-  for (auto& t : pl->mls)
-    if (!t.host.empty())
-      PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, g("mls", 128), pl->max_mls_hw,
-                                pl->mls_variant, mls_sems(t.host), s));
+  for (auto& it : pl->mls)
+    for (int c = 0; c < kMlsClasses; ++c)
+      if (!it[c].host.empty())
+        PSGC_TRY(launch_mls_tiled(it[c].dev, (int)it[c].host.size(), it[c].total, g("mls", 128),
+                                  c, pl->mls_variant, mls_sems(it[c].host), s));
   return PSGC_OK;
 }
 
@@ -634,10 +640,7 @@ static int autotune(Plan* pl, hipStream_t s) {
     std::vector<int> shapes{0};
     if (n == "decode") shapes = {4, 2};
     if (n == "match") shapes = {256, 128, 512};
-    if (n == "mls") {
-      shapes = {kMlsPlain};
-      if (pl->max_mls_hw <= kMlsCachedMaxHalf) shapes.push_back(kMlsCached);
-    }
+    if (n == "mls") shapes = {kMlsPlain, kMlsCached};
     float best_ms = 1e30f;
     int best_g = 0, best_shape = shapes[0];
     for (int shape : shapes) {
@@ -664,9 +667,12 @@ static int autotune(Plan* pl, hipStream_t s) {
                                           grid, pl->max_sor_half, pl->min_sor_k,
                                           pl->max_sor_k, s));
           else
-            PSGC_TRY(launch_mls_tiled(pl->mls[0].dev, (int)pl->mls[0].host.size(),
-                                      pl->mls[0].total, grid, pl->max_mls_hw, shape,
-                                      mls_sems(pl->mls[0].host), s));
+            for (int c = 0; c < kMlsClasses; ++c) {
+              const Table<CloudJob>& t = pl->mls[0][c];
+              if (!t.host.empty())
+                PSGC_TRY(launch_mls_tiled(t.dev, (int)t.host.size(), t.total, grid, c, shape,
+                                          mls_sems(t.host), s));
+            }
           PSGC_HIP(hipEventRecord(b, s));
           PSGC_HIP(hipEventSynchronize(b));
           float ms = 0;
@@ -947,9 +953,6 @@ int psgc_batch_set_grid(psgc_batch* b, const char* stage, int value) {
   } else if (st == "mls_variant") {
     if (value != kMlsPlain && value != kMlsCached)
       PSGC_FAIL(PSGC_E_INVALID, "mls_variant must be 0 or 1 (got %d)", value);
-    if (value == kMlsCached && b->plan.max_mls_hw > kMlsCachedMaxHalf)
-      PSGC_FAIL(PSGC_E_UNSUPPORTED, "mls_variant 1 serves MLS half windows up to %d (batch has %d)",
-                kMlsCachedMaxHalf, b->plan.max_mls_hw);
     b->plan.mls_variant = value;
     b->plan.grid["mls_variant"] = value;
   } else {
@@ -1043,7 +1046,7 @@ int tiled_median(psgc_ctx* ctx, const float* src, float* dst, int rows, int cols
 int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uint8_t* dstP,
               int rows, int cols, int k, float std_mul, int half, double* md, double* partial,
               double* stats, hipStream_t s) {
-  if (half > kTiledMaxHalf) return launch_sor(xyz, P, dst, dstP, rows, cols, k, std_mul, half, md, partial,
+  if (half > kSorTiledMaxHalf) return launch_sor(xyz, P, dst, dstP, rows, cols, k, std_mul, half, md, partial,
                                   stats, s);
   if (k < 1 || k > 64) PSGC_FAIL(PSGC_E_INVALID, "sor k=%d outside [1, 64]", k);
   CloudJob c = base_job(rows, cols);
@@ -1063,7 +1066,7 @@ int tiled_sor(psgc_ctx* ctx, const float* xyz, const uint8_t* P, float* dst, uin
 int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, float* tmp, int rows,
               int cols, int hw, float sigma, int order, float r2, int iterations, int semantics,
               hipStream_t s) {
-  if (hw > kTiledMaxHalf) {
+  if (hw > kMlsTiledMaxHalf) {
     const float* in = src;
     for (int it = 0; it < iterations; ++it) {
       float* out = ((iterations - 1 - it) % 2 == 0) ? dst : tmp;
@@ -1086,8 +1089,8 @@ int tiled_mls(psgc_ctx* ctx, const float* src, const uint8_t* P, float* dst, flo
     c.mls_r2 = r2;
     CloudJob* d;
     PSGC_TRY(upload_job(ctx, c, &d, s));
-    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 128), hw,
-                              kMlsPlain, mls_sem_bit(semantics), s));
+    PSGC_TRY(launch_mls_tiled(d, 1, (int64_t)c.tiles_x * c.tiles_y, default_grid(ctx, 128),
+                              mls_class(hw), kMlsPlain, mls_sem_bit(semantics), s));
     // the job buffer is reused by the next iteration
     PSGC_HIP(hipStreamSynchronize(s));
     in = out;

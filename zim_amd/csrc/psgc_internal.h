@@ -222,6 +222,9 @@ int launch_rect_maps(int rows, int cols, const double* M_dev, float* mx, float* 
                      hipStream_t s);
 int launch_remap(int kind, const void* src, void* dst, const float* mx, const float* my, int rows,
                  int cols, int src_rows, int src_cols, hipStream_t s);
+// MLS weights exp(-d^2 / (2 sigma^2)) need a finite positive sigma whose
+// 1/(2 sigma^2) is finite, and a finite squared cutoff radius.
+int check_mls_weights(float sigma, float max_radius);
 int compact_noalloc_scratch(int device, int64_t** out);
 int launch_compact_noalloc(int device, const float* xyz, const uint8_t* P, const uint8_t* gray,
                            size_t n, float* out, uint8_t* out_gray, int64_t* count,

@@ -157,12 +157,9 @@ constexpr double kExp2Tab64[64] = {
 // integer within a few units of 64 x / ln 2 keeps |r| far inside the series'
 // range). The double result is within a few units of 2^-53 of e^x, and over
 // every float its rounding to float equals the correctly rounded e^x.
-// Arguments are clamped to [-104, 89] rather than branched on: e^-104 rounds to
-// float 0 and e^89 to float infinity, the values of the unclamped tails.
 // `tab` holds kExp2Tab64 (on the device a copy in shared memory, so that the
-// per-lane lookup is one LDS read).
-PSGC_HD inline float exp_f32(float xf, const double* tab) {
-  const float xc = __builtin_fminf(__builtin_fmaxf(xf, -104.0f), 89.0f);
+// per-lane lookup is one LDS read). exp_f32_core takes xc in [-104, 89].
+PSGC_HD inline float exp_f32_core(float xc, const double* tab) {
   const float kf = __builtin_rintf(xc * (float)(64.0 * kInvLn2));
   const int ki = (int)kf;
   const double t = tab[ki & 63];
@@ -173,8 +170,21 @@ PSGC_HD inline float exp_f32(float xf, const double* tab) {
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   p = __builtin_fma(p, r, 1.0);
-  const float res = (float)__builtin_ldexp(p * t, ki >> 6);
+  return (float)__builtin_ldexp(p * t, ki >> 6);
+}
+
+// Arguments are clamped to [-104, 89] rather than branched on: e^-104 rounds to
+// float 0 and e^89 to float infinity, the values of the unclamped tails.
+PSGC_HD inline float exp_f32(float xf, const double* tab) {
+  const float res = exp_f32_core(__builtin_fminf(__builtin_fmaxf(xf, -104.0f), 89.0f), tab);
   return __builtin_isnan(xf) ? xf : res;
+}
+
+// exp_f32 of an argument known to be <= 0 and not NaN (a Gaussian weight of
+// a finite squared distance): the same result with the upper clamp and the
+// NaN test left out.
+PSGC_HD inline float exp_f32_nonpos(float xf, const double* tab) {
+  return exp_f32_core(__builtin_fmaxf(xf, -104.0f), tab);
 }
 
 PSGC_HD inline float exp_f32(float xf) { return exp_f32(xf, kExp2Tab64); }

@@ -122,20 +122,26 @@ int launch_repro_gauss_tiled(const CloudJob* jobs, int njobs, int64_t total, int
                              int max_half, hipStream_t s);
 int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_half,
                          int min_k, int max_k, hipStream_t s);
-// MLS kernel variants: kMlsPlain serves half windows up to 8 and evaluates
-// each weight in all three passes; kMlsCached (half windows up to 6) keeps the
-// first 24 in-radius weights of each point in LDS between the passes (the rest
-// are evaluated again). Results are identical.
+// MLS kernel variants: kMlsPlain evaluates each weight in all three passes;
+// kMlsCached keeps the first in-radius weights of each point in LDS between
+// the passes (the rest are evaluated again). Results are identical. Jobs are
+// launched by window class: half windows up to 6, 8 and 10 (the class fixes
+// the LDS tile, so small windows do not pay for the largest halo).
 enum { kMlsPlain = 0, kMlsCached = 1 };
-// Largest half window the tiled SOR and MLS kernels stage in LDS.
-constexpr int kTiledMaxHalf = 8;
-constexpr int kMlsCachedMaxHalf = 6;
+// Largest half windows the tiled SOR and MLS kernels stage in LDS.
+constexpr int kSorTiledMaxHalf = 8;
+constexpr int kMlsTiledMaxHalf = 10;
+constexpr int kMlsClasses = 3;
+constexpr int kMlsClassHalf[kMlsClasses] = {6, 8, 10};
+inline int mls_class(int hw) { return hw <= 6 ? 0 : (hw <= 8 ? 1 : 2); }
 // Semantics present among the MLS jobs of a launch (a bit each): CPU
 // semantics accumulate with rounded products and sums, GPU semantics with
 // fused multiply-adds.
 enum { kSemBitCpu = 1, kSemBitGpu = 2 };
 inline int mls_sem_bit(int semantics) { return semantics == PSGC_SEM_GPU ? kSemBitGpu : kSemBitCpu; }
-int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
+// One launch over jobs of one window class (every job's half window at most
+// kMlsClassHalf[cls]).
+int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int cls,
                      int variant, int sems, hipStream_t s);
 
 // Shared-memory bytes of the row-match kernel for a given width and order count.

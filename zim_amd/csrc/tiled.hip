@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
 
 // --------------------------------------------------------------------- SOR --
 
-constexpr int SH_MAX = 8;
+constexpr int SH_MAX = kSorTiledMaxHalf;
 constexpr int SL = CT + 2 * SH_MAX;
 
 // Loads points and mask of the tile plus a halo of `h` into LDS (mask 0
@@ -640,13 +640,12 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
 
 // --------------------------------------------------------------------- MLS --
 
-constexpr int MH_MAX = 8;
-// Variant with a per-thread weight cache: windows up to 13 x 13, the first
-// MKC in-radius weights of each point kept in LDS between the passes.
-constexpr int MH_CACHED = kMlsCachedMaxHalf;
-// 24 entries keep four workgroups per CU resident (LDS 13 KB tile + 24 KB
-// cache each), the occupancy the plain variant runs at.
-constexpr int MKC = 24;
+static_assert(kMlsClassHalf[kMlsClasses - 1] == kMlsTiledMaxHalf, "classes cover the limit");
+// Weight-cache entries per thread of each window class: as many as keep four
+// workgroups per CU resident beside the class's LDS tile (tile + cache about
+// 37 KB), the occupancy the registers allow.
+template <int HM>
+constexpr int mls_cache_entries() { return HM <= 6 ? 24 : (HM <= 8 ? 20 : 16); }
 
 // Visits the in-radius entries of window rows [dy0, dy1) (bit masks per row)
 // in order, with their weights: from the cache (entries kc, kc + 1, ... of
@@ -665,7 +664,7 @@ __device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0,
       const float pk[3] = {e4.x, e4.y, e4.z};
       float w;
       if (CACHED) w = Wc[kc][threadIdx.x];
-      else w = exp_f32(-dist2<C>(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
+      else w = exp_f32_nonpos(-dist2<C>(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
       ++kc;
       visit(pk, w);
     }
@@ -741,44 +740,59 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
     float wsum = 0.f;
     int nv = 0;
     int rows_fit = 0;  // leading window rows whose weights all sit in the cache
-    bool fit = false;
+    bool fit = false, nan_in = false;
+    bool mine = false;
     if (inimg) {
-      const bool mine = q.w != 0.f;
+      mine = q.w != 0.f;
       if (cj.mls_Pout) st_global(cj.mls_Pout, p, (uint8_t)(mine ? 1 : 0));
       // points removed by SOR enter MLS as zeros (the SOR output), and MLS
       // copies its input for points outside the mask
       if (!mine && cj.mls_md) out[0] = out[1] = out[2] = 0.f;
-      if (mine) {
-        for (int dy = 0; dy < Wd; ++dy) {
-          const int ry = cy - hw + dy;
-          uint32_t row_bits = 0u;
+    }
+    // The window scan runs for the whole wave when any of its points is
+    // valid (the row loop is then uniform and the entries of a row unroll);
+    // an entry is a neighbour when it is valid and not farther than the
+    // cutoff, so a NaN distance counts, as in the reference's test.
+    if (__any(mine)) {
+      for (int dy = 0; dy < Wd; ++dy) {
+        const int ry = cy - hw + dy;
+        uint32_t row_bits = 0u;
 #pragma unroll
-          for (int dx = 0; dx < MWD; ++dx) {
-            if (dx >= Wd) break;
+        for (int dx = 0; dx < MWD; ++dx) {
+          if (dx < Wd) {
             // one 16-byte read: point and mask (an invalid entry's distance
             // is computed and discarded)
             const float4 e4 = T[ry][cx - hw + dx];
             const float pk[3] = {e4.x, e4.y, e4.z};
             const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
-            if ((e4.w != 0.f) & (d2 <= r2)) {
+            if (mine & (e4.w != 0.f) & !(d2 > r2)) {
               row_bits |= 1u << dx;
-              const float w = exp_f32(-d2 * inv2s2, etab);
+              const float w = exp_f32_nonpos(-d2 * inv2s2, etab);
               if (KC > 0 && nv < KC) Wc[nv][threadIdx.x] = w;
               mls_accum_centroid<C>(pk, w, &f, &wsum);
               nv++;
             }
           }
-          bits[dy] = row_bits;
-          if (nv <= KC) rows_fit = dy + 1;
         }
-        fit = nv >= 3 && !(wsum < 1e-12f);
+        bits[dy] = row_bits;
+        if (nv <= KC) rows_fit = dy + 1;
       }
+      // A NaN coordinate among the neighbours (or of the point itself)
+      // reaches the centroid sums; the reference's fit then yields NaN in
+      // every coordinate (its weight, centroid and frame are NaN), which is
+      // written directly. Weights here are never NaN (a NaN distance gives
+      // the weight of the clamped argument), so the sums of such a point
+      // are otherwise meaningless.
+      nan_in = (f.cx != f.cx) | (f.cy != f.cy) | (f.cz != f.cz);
+      fit = mine && nv >= 3 && (nan_in || !(wsum < 1e-12f));
     }
     // rows served from the cache: the same for every lane of the wave (every
     // lane is active here), so the passes below do not diverge on it
     int dyc = 0;
     if (KC > 0) dyc = __builtin_amdgcn_readfirstlane(wave_min(fit ? rows_fit : MWD));
-    if (fit) {
+    if (fit && nan_in) {
+      out[0] = out[1] = out[2] = __builtin_nanf("");
+    } else if (fit) {
       const float iw = 1.0f / wsum;
       f.cx *= iw;
       f.cy *= iw;
@@ -898,27 +912,39 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
   return PSGC_OK;
 }
 
-int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int max_hw,
-                     int variant, int sems, hipStream_t s) {
-  if (total <= 0) return PSGC_OK;
-  if (max_hw > MH_MAX) PSGC_FAIL(PSGC_E_UNSUPPORTED, "MLS half window above %d", MH_MAX);
-  if (variant != kMlsPlain && variant != kMlsCached)
-    PSGC_FAIL(PSGC_E_INVALID, "MLS variant %d", variant);
-  if (variant == kMlsCached && max_hw > MH_CACHED)
-    PSGC_FAIL(PSGC_E_UNSUPPORTED, "cached MLS variant: half window above %d", MH_CACHED);
-  PSGC_PROF_BEGIN("mls", s);
-  const int g = round_grid(total, grid);
+namespace {
+
+template <int HM>
+int launch_mls_class(const CloudJob* jobs, int njobs, int64_t total, int g, int variant, int sems,
+                     hipStream_t s) {
+  constexpr int KC = mls_cache_entries<HM>();
   // one launch per semantics present (each skips the other's tiles)
   if (sems & kSemBitCpu) {
-    if (variant == kMlsCached) mls_tiled_kernel<MH_CACHED, MKC, false><<<g, 256, 0, s>>>(jobs, njobs, total);
-    else mls_tiled_kernel<MH_MAX, 0, false><<<g, 256, 0, s>>>(jobs, njobs, total);
+    if (variant == kMlsCached) mls_tiled_kernel<HM, KC, false><<<g, 256, 0, s>>>(jobs, njobs, total);
+    else mls_tiled_kernel<HM, 0, false><<<g, 256, 0, s>>>(jobs, njobs, total);
     PSGC_LAUNCHED();
   }
   if (sems & kSemBitGpu) {
-    if (variant == kMlsCached) mls_tiled_kernel<MH_CACHED, MKC, true><<<g, 256, 0, s>>>(jobs, njobs, total);
-    else mls_tiled_kernel<MH_MAX, 0, true><<<g, 256, 0, s>>>(jobs, njobs, total);
+    if (variant == kMlsCached) mls_tiled_kernel<HM, KC, true><<<g, 256, 0, s>>>(jobs, njobs, total);
+    else mls_tiled_kernel<HM, 0, true><<<g, 256, 0, s>>>(jobs, njobs, total);
     PSGC_LAUNCHED();
   }
+  return PSGC_OK;
+}
+
+}  // namespace
+
+int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, int cls,
+                     int variant, int sems, hipStream_t s) {
+  if (total <= 0) return PSGC_OK;
+  if (cls < 0 || cls >= kMlsClasses) PSGC_FAIL(PSGC_E_INVALID, "MLS window class %d", cls);
+  if (variant != kMlsPlain && variant != kMlsCached)
+    PSGC_FAIL(PSGC_E_INVALID, "MLS variant %d", variant);
+  PSGC_PROF_BEGIN("mls", s);
+  const int g = round_grid(total, grid);
+  if (cls == 0) PSGC_TRY(launch_mls_class<kMlsClassHalf[0]>(jobs, njobs, total, g, variant, sems, s));
+  else if (cls == 1) PSGC_TRY(launch_mls_class<kMlsClassHalf[1]>(jobs, njobs, total, g, variant, sems, s));
+  else PSGC_TRY(launch_mls_class<kMlsClassHalf[2]>(jobs, njobs, total, g, variant, sems, s));
   PSGC_PROF_END("mls", s);
   return PSGC_OK;
 }
