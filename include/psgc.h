@@ -490,7 +490,8 @@ PSGC_API int psgc_write_raw_f64(const char* raw_path, const char* info_path, con
                                 int rows, int cols);
 
 /* ---- differentiable triangulation ----------------------------------------
- * Rectification of raw phase maps under a calibration theta, the sub-pixel
+ * Rectification of raw phase maps under a calibration theta (Catmull-Rom
+ * bicubic resampling, so the result is C1 in theta), the sub-pixel
  * root (JSLAB_PhaseEpipolarReconstructor_V2.cpp:992-1003) with the integer
  * match held fixed, and the Q triangulation (:430-438), in FP64, with its
  * hand-written adjoint. No counterpart in the reference (it has no
@@ -499,7 +500,15 @@ PSGC_API int psgc_write_raw_f64(const char* raw_path, const char* info_path, con
  *               rectification rotation Rz(wz) Ry(wy) Rx(wx))
  *   nominal[5]= f0, c0Lx, c0Ly, c0Rx, c0Ry: intrinsics of the raw maps
  *   match_col = integer right column per left rectified pixel (-1: none)
- *   xyz (rows*cols*3 f64, rectified left frame), valid (u8, may be NULL)
+ *   xyz (rows*cols*3 f64, rectified left frame), valid (u8, may be NULL):
+ *               1 = triangulated, every phase sample's 4 x 4 Catmull-Rom
+ *               taps within pi of their neighbours (Itoh's condition: the
+ *               sample lies in smoothly unwrapped phase); 2 = triangulated,
+ *               but a sample
+ *               spans a phase jump of pi or more (fringe-order error, surface
+ *               edge, region boundary), where xyz is not a smooth function of
+ *               theta; 0 = no point (no match, taps outside the map, or a
+ *               degenerate root)
  * The backward pass takes dL/dxyz and returns dL/dtheta (9 f64) and, when
  * the pointers are non-NULL, dL/dphi for both raw maps (f64, overwritten).
  * theta and nominal are host arrays; grad_theta lives where `mem` says. */

@@ -3,7 +3,9 @@ triangulation, written with torch tensor ops so that torch's own autograd
 gives the reference gradient for the hand-written HIP adjoint.
 
 theta = (f, cx1, cy1, cx2, cy2, B, wx, wy, wz); nominal = (f0, c0Lx, c0Ly,
-c0Rx, c0Ry). Bilinear taps clamp the lower corner to [0, n-2]."""
+c0Rx, c0Ry). Catmull-Rom bicubic sampling over 4 x 4 taps (the sample
+needs u in [1, cols - 2], v in [1, rows - 2]); a point is conditioned when
+every sample's adjacent taps differ by less than pi (Itoh's condition)."""
 from __future__ import annotations
 
 import torch
@@ -22,24 +24,41 @@ def _rot(wx, wy, wz):
     return Rz @ Ry @ Rx
 
 
-def _bilinear(m, u, v):
+def _catmull_rom(t):
+    t2, t3 = t * t, t * t * t
+    return ((-t3 + 2 * t2 - t) / 2, (3 * t3 - 5 * t2 + 2) / 2, (-3 * t3 + 4 * t2 + t) / 2,
+            (t3 - t2) / 2)
+
+
+def _bicubic(m, u, v):
+    """Catmull-Rom bicubic sample of m at (u, v) over the 4 x 4 taps around
+    it: (value, taps inside the map, adjacent taps pairwise within pi)."""
     rows, cols = m.shape
-    ok = (u >= 0) & (u <= cols - 1) & (v >= 0) & (v <= rows - 1)
-    x0 = torch.clamp(torch.floor(u.detach()), 0, max(cols - 2, 0)).long()
-    y0 = torch.clamp(torch.floor(v.detach()), 0, max(rows - 2, 0)).long()
-    x1 = torch.clamp(x0 + 1, max=cols - 1)
-    y1 = torch.clamp(y0 + 1, max=rows - 1)
-    a = u - x0.double()
-    c = v - y0.double()
+    ok = (u >= 1) & (u <= cols - 2) & (v >= 1) & (v <= rows - 2) & (rows >= 4) & (cols >= 4)
+    x0 = torch.clamp(torch.floor(u.detach()), 1, max(cols - 3, 1)).long()
+    y0 = torch.clamp(torch.floor(v.detach()), 1, max(rows - 3, 1)).long()
+    wx, wy = _catmull_rom(u - x0.double()), _catmull_rom(v - y0.double())
     flat = m.reshape(-1)
     g = lambda yy, xx: flat[(yy * cols + xx).clamp(0, rows * cols - 1)]  # noqa: E731
-    val = (1 - a) * (1 - c) * g(y0, x0) + a * (1 - c) * g(y0, x1) + (1 - a) * c * g(y1, x0) \
-        + a * c * g(y1, x1)
-    return val, ok
+    taps = [[g(y0 + j - 1, x0 + i - 1) for i in range(4)] for j in range(4)]
+    val = 0
+    for j in range(4):
+        for i in range(4):
+            val = val + wy[j] * wx[i] * taps[j][i]
+    pi = 3.14159265358979323846
+    smooth = torch.ones_like(ok)
+    for j in range(4):
+        for i in range(4):
+            if i > 0:
+                smooth = smooth & ((taps[j][i] - taps[j][i - 1]).abs() < pi)
+            if j > 0:
+                smooth = smooth & ((taps[j][i] - taps[j - 1][i]).abs() < pi)
+    return val, ok, smooth.detach()
 
 
 def forward(theta, nominal, phL, phR, match_col):
-    """Returns xyz (rows, cols, 3) and the validity mask."""
+    """Returns xyz (rows, cols, 3), the validity mask and the conditioning
+    mask (every sample's adjacent taps within pi)."""
     f, cx1, cy1, cx2, cy2, B, wx, wy, wz = [theta[k] for k in range(9)]
     f0, cLx, cLy, cRx, cRy = [float(v) for v in nominal]
     rows, cols = phL.shape
@@ -49,7 +68,7 @@ def forward(theta, nominal, phL, phR, match_col):
     ok = (b >= 1) & (b <= cols - 2)
     ul = f0 / f * (jj - cx1) + cLx
     vl = f0 / f * (ii - cy1) + cLy
-    pl, okl = _bilinear(phL, ul, vl)
+    pl, okl, sm = _bicubic(phL, ul, vl)
     ok = ok & okl
     R = _rot(wx, wy, wz)
     cs = []
@@ -62,8 +81,9 @@ def forward(theta, nominal, phL, phR, match_col):
         q2 = R[2, 0] * r0 + R[2, 1] * r1 + R[2, 2]
         u = f0 * q0 / q2 + cRx
         v = f0 * q1 / q2 + cRy
-        pr, okr = _bilinear(phR, u, v)
+        pr, okr, smr = _bicubic(phR, u, v)
         ok = ok & okr
+        sm = sm & smr
         cs.append(pr - pl)
     c0, c1, c2 = cs
     N = c1 * (c0 - c2)
@@ -78,7 +98,7 @@ def forward(theta, nominal, phL, phR, match_col):
     Y = (ii - cy1) * B / Ds
     Z = f * B / Ds
     xyz = torch.stack([X, Y, Z], -1) * ok[..., None]
-    return xyz, ok
+    return xyz, ok, ok & sm
 
 
 def plane_loss(xyz, ok):

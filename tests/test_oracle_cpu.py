@@ -433,17 +433,18 @@ def test_adjoint_reference_model_at_true_calibration_equals_reprojection():
     theta0 = torch.tensor([cam.f, cam.cx, cam.cy, cam.cx, cam.cy, cam.baseline, 0, 0, 0],
                           dtype=torch.float64)
     nominal = [cam.f, cam.cx, cam.cy, cam.cx, cam.cy]
-    xyz, ok = adjoint_ref.forward(theta0, nominal, torch.from_numpy(L[0]), torch.from_numpy(R[0]),
-                                  torch.from_numpy(mc))
+    xyz, ok, cond = adjoint_ref.forward(theta0, nominal, torch.from_numpy(L[0]),
+                                        torch.from_numpy(R[0]), torch.from_numpy(mc))
+    assert cond.sum() > 0.9 * ok.sum()
     rx, rP = oracle.reproject(disp, p.Q, p.R1, 0, 1e9, 1)
     m = ok.numpy() & (rP > 0)
     assert m.sum() > 0.3 * m.size
     assert np.abs(xyz.numpy()[m] - rx[m]).max() < 1e-3
     # and it is differentiable: autograd of the plane loss is finite
     th = theta0.clone().requires_grad_(True)
-    loss = adjoint_ref.plane_loss(*adjoint_ref.forward(th, nominal, torch.from_numpy(L[0]),
-                                                       torch.from_numpy(R[0]),
-                                                       torch.from_numpy(mc)))
+    xyz, _, cond = adjoint_ref.forward(th, nominal, torch.from_numpy(L[0]), torch.from_numpy(R[0]),
+                                       torch.from_numpy(mc))
+    loss = adjoint_ref.plane_loss(xyz, cond)
     loss.backward()
     assert torch.isfinite(th.grad).all()
 

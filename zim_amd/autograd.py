@@ -81,12 +81,23 @@ def triangulate_views(thetas, phLs, phRs, match_cols, nominal, ctx: Context | No
     return out
 
 
+def point_mask(valid):
+    """The points a plane loss sums over: a boolean mask as given (e.g. one
+    held fixed across the evaluations of a finite-difference check), or, for
+    the triangulation's validity codes, the conditioned points (code 1:
+    every phase sample in smoothly unwrapped phase; code 2 marks samples
+    across a fringe-order error or an edge, where the point is not a smooth
+    function of the calibration)."""
+    v = valid.reshape(-1)
+    return v if v.dtype == torch.bool else v == 1
+
+
 def _scatter(xyz, valid):
     """3 x 3 scatter matrix of the valid points about their mean, from
     masked reductions over every point (no compaction, so the backward pass
     is elementwise too; a library GEMM of shape 3 x N x 3 runs ~1000x slower
     on millions of points)."""
-    m = valid.reshape(-1).bool()
+    m = point_mask(valid)
     p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
     mu = p.sum(0) / m.sum()
     x, y, z = torch.where(m[:, None], p - mu, 0.0).unbind(1)
@@ -97,7 +108,8 @@ def _scatter(xyz, valid):
 
 
 def plane_losses(views):
-    """Sum of squared residuals of the valid points of each (xyz, valid) to
+    """Sum of squared residuals of the points of each (xyz, valid) (see
+    point_mask) to
     their least-squares plane (the smallest eigenvalue of the scatter
     matrix), summed from the residuals n . (x - mean) themselves, n the plane
     normal: the eigenvalue read off the scatter matrix carries an absolute
@@ -110,7 +122,7 @@ def plane_losses(views):
         ns = torch.linalg.eigh(S.detach().cpu())[1][:, :, 0].to(S.device)
     out = []
     for (xyz, valid), n in zip(views, ns):
-        m = valid.reshape(-1).bool()
+        m = point_mask(valid)
         p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
         mu = p.sum(0) / m.sum()
         d = p - mu   # elementwise dot with n: a library GEMV of N x 3 is far slower

@@ -11,11 +11,17 @@
 //   right rectified pixel (x, i) samples the raw right map at the
 //   projection of R(w) K_R^-1 (x, i, 1), R(w) = Rz(wz) Ry(wy) Rx(wx),
 //   K_R = [f 0 cx2; 0 f cy2; 0 0 1], with the raw intrinsics (f0, c0R).
-//   Bilinear sampling; taps outside the map invalidate the point.
+//   Catmull-Rom bicubic sampling (C1 in theta); taps outside the map
+//   invalidate the point.
 //   With the integer match b fixed, c_m = phiR(i, b-1+m) - phiL(i, j),
 //   delta = c1 (c0 - c2) / (2 (c0 - c1)(c1 - c2)), d = j - b - delta and
 //   X = (j - cx1) B / D, Y = (i - cy1) B / D, Z = f B / D, D = d - (cx1 - cx2)
 //   (Q with R1 = I). Everything is FP64.
+//   valid = 1 marks a triangulated point whose samples all satisfy
+//   Itoh's condition (adjacent unwrapped phase taps differ by less than pi:
+//   no fringe-order error, surface edge or region boundary inside a
+//   sample), 2 a triangulated point with a sample across such a jump (its
+//   coordinates are not a smooth function of theta there), 0 no point.
 // The adjoint accumulates dL/dtheta (per-workgroup reduction, then f64
 // atomics) and scatters dL/dphi into the raw maps with f64 atomics.
 
@@ -64,35 +70,94 @@ __device__ inline void rot(const Theta& t, double R[3][3], double dR[3][3][3]) {
   mul3(dRz, Ry, Rx, dR[2]);
 }
 
-// Bilinear sample of map m at (u, v); returns false if a tap is outside.
-// w: tap weights, idx: tap indices, du/dv: partial derivatives.
-struct Bil {
-  double val, du, dv;
-  double w[4];
-  size_t idx[4];
+// Catmull-Rom cubic weights of taps -1, 0, 1, 2 at fraction t, and their
+// derivatives: the interpolant is C1 across pixel boundaries, so the
+// triangulated points are continuously differentiable in theta (a bilinear
+// sample has a kink at every pixel boundary, which central differences of
+// the calibration objective straddle).
+__device__ inline void catmull_rom(double t, double w[4], double dw[4]) {
+  const double t2 = t * t, t3 = t2 * t;
+  w[0] = 0.5 * (-t3 + 2.0 * t2 - t);
+  w[1] = 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0);
+  w[2] = 0.5 * (-3.0 * t3 + 4.0 * t2 + t);
+  w[3] = 0.5 * (t3 - t2);
+  dw[0] = 0.5 * (-3.0 * t2 + 4.0 * t - 1.0);
+  dw[1] = 0.5 * (9.0 * t2 - 10.0 * t);
+  dw[2] = 0.5 * (-9.0 * t2 + 8.0 * t + 1.0);
+  dw[3] = 0.5 * (3.0 * t2 - 2.0 * t);
+}
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Bicubic sample of map m at (u, v) over the 4 x 4 taps around it; false if
+// a tap would lie outside the map (u in [1, cols - 2], v in [1, rows - 2]).
+struct Samp {
+  double val, du, dv;  // value and its partial derivatives in u, v
+  double a, c;         // fractions within the tap cell
+  int x0, y0;          // tap (x0 - 1 .. x0 + 2, y0 - 1 .. y0 + 2)
+  bool smooth;         // Itoh's condition: adjacent taps within pi
 };
 
-__device__ inline bool bilinear(const double* m, int rows, int cols, double u, double v, Bil* b) {
-  if (!(u >= 0.0 && u <= (double)(cols - 1) && v >= 0.0 && v <= (double)(rows - 1))) return false;
+__device__ inline bool bicubic(const double* m, int rows, int cols, double u, double v, Samp* s) {
+  if (rows < 4 || cols < 4) return false;
+  if (!(u >= 1.0 && u <= (double)(cols - 2) && v >= 1.0 && v <= (double)(rows - 2))) return false;
   int x0 = (int)floor(u), y0 = (int)floor(v);
-  if (x0 > cols - 2) x0 = cols >= 2 ? cols - 2 : 0;
-  if (y0 > rows - 2) y0 = rows >= 2 ? rows - 2 : 0;
-  const int x1 = min(x0 + 1, cols - 1), y1 = min(y0 + 1, rows - 1);
-  const double fu = (double)x0, fv = (double)y0;
-  const double a = u - fu, c = v - fv;
-  b->idx[0] = (size_t)y0 * cols + x0;
-  b->idx[1] = (size_t)y0 * cols + x1;
-  b->idx[2] = (size_t)y1 * cols + x0;
-  b->idx[3] = (size_t)y1 * cols + x1;
-  const double p00 = m[b->idx[0]], p10 = m[b->idx[1]], p01 = m[b->idx[2]], p11 = m[b->idx[3]];
-  b->w[0] = (1 - a) * (1 - c);
-  b->w[1] = a * (1 - c);
-  b->w[2] = (1 - a) * c;
-  b->w[3] = a * c;
-  b->val = b->w[0] * p00 + b->w[1] * p10 + b->w[2] * p01 + b->w[3] * p11;
-  b->du = (1 - c) * (p10 - p00) + c * (p11 - p01);
-  b->dv = (1 - a) * (p01 - p00) + a * (p11 - p10);
+  if (x0 > cols - 3) x0 = cols - 3;
+  if (y0 > rows - 3) y0 = rows - 3;
+  s->x0 = x0;
+  s->y0 = y0;
+  s->a = u - (double)x0;
+  s->c = v - (double)y0;
+  double wx[4], dwx[4], wy[4], dwy[4];
+  catmull_rom(s->a, wx, dwx);
+  catmull_rom(s->c, wy, dwy);
+  double val = 0.0, du = 0.0, dv = 0.0;
+  // Itoh's condition as one running maximum of the adjacent-tap differences
+  // (with the ROCm 7.2 compiler, a chain of && over the sixteen tap
+  // comparisons folded to a constant for gfx950 at -O2 and above)
+  double jump = 0.0, nan_chk = 0.0;
+  double prev[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < 4; ++j) {
+    const double* row = m + (size_t)(y0 - 1 + j) * cols + (x0 - 1);
+    const double p[4] = {row[0], row[1], row[2], row[3]};
+    double rv = 0.0, rd = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      val += wy[j] * wx[i] * p[i];
+      rv += wx[i] * p[i];
+      rd += dwx[i] * p[i];
+      if (i > 0) {
+        const double d = p[i] - p[i - 1];
+        jump = fmax(jump, fabs(d));
+        nan_chk += d - d;  // NaN unless d is finite
+      }
+      if (j > 0) {
+        const double d = p[i] - prev[i];
+        jump = fmax(jump, fabs(d));
+        nan_chk += d - d;
+      }
+      prev[i] = p[i];
+    }
+    du += wy[j] * rd;
+    dv += dwy[j] * rv;
+  }
+  // fmax keeps the other operand of a NaN, so non-finite differences are
+  // caught by nan_chk (every comparison with a NaN difference fails)
+  const bool smooth = jump < kPi && nan_chk == 0.0;
+  s->val = val;
+  s->du = du;
+  s->dv = dv;
+  s->smooth = smooth;
   return true;
+}
+
+// Adds g times each tap's weight of sample s into the gradient map gm.
+__device__ inline void bicubic_scatter(const Samp& s, int cols, double g, double* gm) {
+  double wx[4], dwx[4], wy[4], dwy[4];
+  catmull_rom(s.a, wx, dwx);
+  catmull_rom(s.c, wy, dwy);
+  for (int j = 0; j < 4; ++j)
+    for (int i = 0; i < 4; ++i)
+      atomicAdd(&gm[(size_t)(s.y0 - 1 + j) * cols + (s.x0 - 1 + i)], g * (wy[j] * wx[i]));
 }
 
 // Right remap of rectified (x, i): raw coordinates and their derivatives
@@ -130,12 +195,13 @@ __device__ inline void right_map(const Theta& t, const Nominal& n, const double 
 
 struct PointEval {
   bool ok;
+  bool smooth;  // every sample's taps satisfy Itoh's condition
   double xyz[3];
   // for the adjoint
   double dd_dc[3];        // d d / d c_m
   double dxyz_dd[3];
   double dxyz_dth[3][9];  // direct dependence through Q
-  Bil bl, br[3];
+  Samp bl, br[3];
   double dul[9], dvl[9], dur[3][9], dvr[3][9];
 };
 
@@ -146,7 +212,7 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   if (b < 1 || b > cols - 2) return;
   const double ul = n.f0 / t.f * (j - t.cx1) + n.cLx;
   const double vl = n.f0 / t.f * (i - t.cy1) + n.cLy;
-  if (!bilinear(phL, rows, cols, ul, vl, &e->bl)) return;
+  if (!bicubic(phL, rows, cols, ul, vl, &e->bl)) return;
   for (int k = 0; k < 9; ++k) e->dul[k] = e->dvl[k] = 0.0;
   e->dul[0] = -n.f0 / (t.f * t.f) * (j - t.cx1);
   e->dvl[0] = -n.f0 / (t.f * t.f) * (i - t.cy1);
@@ -156,7 +222,7 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   for (int m = 0; m < 3; ++m) {
     double u, v;
     right_map(t, n, R, dR, (double)(b - 1 + m), (double)i, &u, &v, e->dur[m], e->dvr[m]);
-    if (!bilinear(phR, rows, cols, u, v, &e->br[m])) return;
+    if (!bicubic(phR, rows, cols, u, v, &e->br[m])) return;
     c[m] = e->br[m].val - e->bl.val;
   }
   const double N = c[1] * (c[0] - c[2]);
@@ -196,6 +262,7 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   e->dxyz_dth[1][5] = (i - t.cy1) * iD;
   e->dxyz_dth[2][5] = t.f * iD;
   e->ok = true;
+  e->smooth = e->bl.smooth && e->br[0].smooth && e->br[1].smooth && e->br[2].smooth;
 }
 
 __device__ inline void adj_forward_px(const Theta& t, const Nominal& n, const double (&R)[3][3],
@@ -208,7 +275,7 @@ __device__ inline void adj_forward_px(const Theta& t, const Nominal& n, const do
   xyz[p * 3 + 0] = e.ok ? e.xyz[0] : 0.0;
   xyz[p * 3 + 1] = e.ok ? e.xyz[1] : 0.0;
   xyz[p * 3 + 2] = e.ok ? e.xyz[2] : 0.0;
-  if (valid) valid[p] = e.ok ? 1 : 0;
+  if (valid) valid[p] = e.ok ? (e.smooth ? 1 : 2) : 0;
 }
 
 // One pixel's contribution to dL/dtheta (acc) and its scatter into the raw
@@ -232,12 +299,10 @@ __device__ inline void adj_backward_px(const Theta& t, const Nominal& n,
     gL -= gc;
     // through the right sample
     for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * e.dur[m][k] + e.br[m].dv * e.dvr[m][k]);
-    if (gphR)
-      for (int q = 0; q < 4; ++q) atomicAdd(&gphR[e.br[m].idx[q]], gc * e.br[m].w[q]);
+    if (gphR) bicubic_scatter(e.br[m], cols, gc, gphR);
   }
   for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * e.dul[k] + e.bl.dv * e.dvl[k]);
-  if (gphL)
-    for (int q = 0; q < 4; ++q) atomicAdd(&gphL[e.bl.idx[q]], gL * e.bl.w[q]);
+  if (gphL) bicubic_scatter(e.bl, cols, gL, gphL);
 }
 
 // Workgroup sum of acc, added to gth (f64 atomics).
