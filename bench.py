@@ -5,8 +5,8 @@ full-size single-pair shape), three-frequency heterodyne 4-step fringes
 (70/64/59 across a 1280-px projector), uint16 12-bit, rendered on the GPU
 before timing. One step = decode L + decode R + reconstruct (order match,
 5x5 median, Q reprojection, 5x5 Gaussian sigma 1, SOR k=30, MLS order 2 at
-1.5 mm in a window that holds that radius at the pair's nearest depth,
-2 iterations) of every pair, frames resident in HBM.
+1.5 mm in the smallest window that holds that radius for the pair, 2
+iterations) of every pair, frames resident in HBM.
 
 value = Mpts/s of the batched pipeline (one hipGraph replay per step; one
 point = one left rectified pixel). Beside it, on the same pairs: the plain
@@ -128,10 +128,11 @@ def main():
     torch.cuda.synchronize()
     log(f"rendered in {time.time() - t0:.1f}s")
     npix = ROWS * COLS
-    # each pair's MLS window holds the radius at its nearest depth
-    windows = [scenes.c2_window(p, MLS_RADIUS) for p in pairs]
+    t0 = time.time()
+    windows, ring_left = mls_windows(pairs, dcfg, ctx)
     rcfgs = [scenes.c2_recon_config(mls_window_size=w) for w in windows]
-    log(f"MLS half windows {min(windows)}-{max(windows)}, mean {sum(windows) / len(windows):.2f}")
+    log(f"MLS half windows {min(windows)}-{max(windows)}, mean {sum(windows) / len(windows):.2f} "
+        f"(chosen in {time.time() - t0:.1f}s; in-radius neighbours left on an outer ring: {ring_left})")
 
     # ---------------------------------------------------------- batched path
     t0 = time.time()
@@ -294,7 +295,6 @@ def main():
             gx = out["xyz"].cpu().numpy()
             m = rP != 0
             parity = {"pair": 0, "points": int(m.sum()), "mls_half_window": windows[0],
-                      "ring_points": scenes.ring_points(rx, rP, windows[0], MLS_RADIUS),
                       "mask_mismatch_px": int((gP != rP).sum()),
                       "xyz_max_diff_mm": float(np.abs(gx[m] - rx[m]).max()) if m.any() else 0.0,
                       "xyz_p99999_diff_mm": float(np.quantile(np.abs(gx[m] - rx[m]).max(-1),
@@ -331,10 +331,11 @@ def main():
         "data": "synthetic (seeded bump height fields rendered on the GPU)",
         "config": {"workload": f"{args.pairs} seeded 2448x2048 pairs, 3-freq 70/64/59 4-step "
                                "u16 12-bit; median 5x5, Gaussian 5x5 s=1, SOR k=30 (9x9 window), "
-                               "MLS order 2 r=1.5 mm (per pair the half window that holds r at "
-                               "the pair's nearest depth, ceil(r f / z_min + 0.5); 2 iterations)",
+                               "MLS order 2 r=1.5 mm (per pair the smallest half window whose "
+                               "outer ring holds no neighbour within r; 2 iterations)",
                    "pairs": args.pairs, "rows": ROWS, "cols": COLS, "path": "batched graph",
-                   "mls_half_windows": {str(w): windows.count(w) for w in sorted(set(windows))}},
+                   "mls_half_windows": {str(w): windows.count(w) for w in sorted(set(windows))},
+                   "mls_ring_points_left": ring_left},
         "valid_mpts_per_s": round(valid / (ms_per_step * 1e-3) / 1e6, 3),
         "per_pair_plain": plain,
         "per_pair_tiled": per_pair_tiled,
@@ -758,6 +759,34 @@ def run_c5(args):
             **{k: round(v, 3) for k, v in res.items()}, "dtype": "f32",
             "data": "synthetic (16 noisy views of one seeded bump height field)"}
     print(json.dumps(line), flush=True)
+
+
+def mls_windows(pairs, dcfg, ctx, radius=MLS_RADIUS, largest=12):
+    """Per pair, the MLS half window that holds the radius: the pipeline runs
+    once without MLS (its output is the MLS input cloud), and from the
+    depth-based estimate upward the first half window whose outer ring has
+    no valid neighbour within the radius (FP64 distances, a 1e-6 relative
+    margin over the filter's FP32 test) is taken."""
+    import torch
+    import zim_amd
+    from tests import scenes
+    probe = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg,
+                                recon=scenes.c2_recon_config(use_mls=0), pha_f=70, Q=p.Q, R1=p.R1)
+                           for p in pairs], ctx, graph=False)
+    outs = probe.run()
+    out, left = [], 0
+    for p, o in zip(pairs, outs):
+        hw = scenes.c2_window(p, radius)
+        ring = scenes.ring_points(o["xyz"], o["P"], hw, radius * (1 + 1e-6))
+        while ring and hw < largest:
+            hw += 1
+            ring = scenes.ring_points(o["xyz"], o["P"], hw, radius * (1 + 1e-6))
+        out.append(hw)
+        left += ring
+    probe.close()
+    del outs
+    torch.cuda.empty_cache()
+    return out, left
 
 
 def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):

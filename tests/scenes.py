@@ -176,11 +176,18 @@ def ring_points(xyz, P, hw, radius):
     """Valid points with a valid neighbour within `radius` on the outer ring
     of their (2 hw + 1)^2 window (Chebyshev distance exactly hw): 0 means a
     window one pixel wider would select no further neighbour, i.e. the
-    window holds the radius for this cloud."""
-    rows, cols = P.shape
-    X = np.asarray(xyz, np.float64)
-    valid = np.asarray(P) > 0
-    hit = np.zeros_like(valid)
+    window holds the radius for this cloud. numpy arrays or torch tensors
+    (on any device); distances in FP64."""
+    import torch
+    if isinstance(xyz, torch.Tensor):
+        X = xyz.double()
+        valid = P != 0
+        hit = torch.zeros_like(valid)
+    else:
+        X = np.asarray(xyz, np.float64)
+        valid = np.asarray(P) > 0
+        hit = np.zeros_like(valid)
+    rows, cols = valid.shape
     r2 = float(radius) ** 2
     for dy in range(-hw, hw + 1):
         for dx in range(-hw, hw + 1):
