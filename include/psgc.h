@@ -375,12 +375,26 @@ PSGC_API int psgc_cloud_gaussian(psgc_ctx* ctx, const float* xyz, const uint8_t*
 
 /* Closed-form rectification for an ideal, already rectified pair (zero
  * distortion, R = I, translation along x): fills Q (4x4) and R1 (3x3) as
- * cv::stereoRectify would with flags 0, plus the full-left canvas shift of
- * computeFullLeftRectification (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
- * 248-289) when full_left != 0 (a zero shift for this case). */
+ * cv::stereoRectify would with flags 0. */
 PSGC_API int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1,
                        double cx2, double baseline, double Q[16],
                        double R1[9]);
+
+/* Full-left rectification canvas: computeFullLeftRectification
+ * (JSLAB_PhaseEpipolarReconstructor_V2.cpp:248-289) for zero distortion, the
+ * branch setCalibrationParams takes when use_full_left_rectification is set
+ * (…V2.cpp:314-329). The four corners of the width x height left image,
+ * (0,0), (w-1,0), (w-1,h-1), (0,h-1) as float, go through
+ * cv::undistortPoints(K1, 0, R1, P1) (normalise by K1, apply P1[:3,:3] R1,
+ * divide by the third coordinate, round to float); with their float bounding
+ * box [min_x, max_x] x [min_y, max_y], shift = -floor(min) is added to the
+ * principal points of P1 and P2 (3x4 row-major, in place) and subtracted
+ * from Q[0][3] and Q[1][3] (in place); *out_width = ceil(max_x - min_x + 1)
+ * and *out_height likewise (float arithmetic), the size of the rectified
+ * maps. Host-only. */
+PSGC_API int psgc_full_left_rectification(const double K1[9], int width, int height,
+                                          const double R1[9], double P1[12], double P2[12],
+                                          double Q[16], int* out_width, int* out_height);
 
 /* ---- kernel selection -----------------------------------------------------
  * PSGC_KERNELS_TILED (default): LDS-tiled kernels. PSGC_KERNELS_PLAIN: one
@@ -450,8 +464,13 @@ PSGC_API int psgc_batch_set_grid(psgc_batch* batch, const char* stage, int value
  *   (JSLAB_PhaseEpipolarReconstructor_V2.cpp:333-338): for each rectified
  *   pixel, raw = K_raw R^T K_rect^-1 (u, v, 1); f32 maps rows x cols.
  * psgc_remap: cv::remap of the reference's setData (…V2.cpp:368-395):
- *   kind 0 = f64 map, INTER_LINEAR (exact bilinear weights); kind 1 = i32
- *   map, INTER_NEAREST; BORDER_CONSTANT 0.
+ *   kind 0 = f64 map, INTER_LINEAR; kind 1 = i32 map, INTER_NEAREST;
+ *   BORDER_CONSTANT 0. The rule is OpenCV 4.7.0's for CV_32FC1 maps: the
+ *   coordinate becomes cvRound(map * 32) (half to even) for INTER_LINEAR,
+ *   i.e. 1/32 px, whose integer part and fraction pick the four taps and
+ *   the float weights ((1-fy)(1-fx), (1-fy)fx, fy(1-fx), fy fx), summed in
+ *   that order in double; INTER_NEAREST takes cvRound(map); a tap outside
+ *   the source reads 0. maps are rows x cols f32 (source pixel coordinates).
  * psgc_compact: valid points (P != 0) packed row-major into out_xyz (n*3
  *   f32) and out_gray (n u8, 255 when gray is NULL; may be NULL); *count
  *   (host) receives n. Order = the reference PLY writers' (main.cpp:85-162).

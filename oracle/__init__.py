@@ -38,6 +38,7 @@ def lib():
         L.orc_reconstruct.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp,
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
+        L.orc_remap.argtypes = [i32, vp, i32, i32, vp, vp, i32, i32, vp]
         L.orc_reproject_poly.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
         L.orc_roi_right_window.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
         L.orc_reconstruct_poly_roi.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp,
@@ -319,6 +320,19 @@ def reconstruct_poly_roi(cfg: ReconConfig, lp, rp, lk, rk, pha_f, left_rect, rig
                                    pha_f, _p(lr), _p(rr), _p(mask), _p(cx), _p(cy), _p(cz),
                                    _p(R1), _p(xyz), _p(P), _p(disp))
     return xyz, P, disp
+
+
+def remap(src, map_x, map_y, nearest=False):
+    """cv::remap (OpenCV 4.7.0) of a f64 map with INTER_LINEAR, or of an i32
+    map with INTER_NEAREST, BORDER_CONSTANT 0."""
+    src = _c(src, np.int32 if nearest else np.float64)
+    mx, my = _c(map_x, np.float32), _c(map_y, np.float32)
+    assert mx.shape == my.shape and src.ndim == 2
+    dst = np.zeros(mx.shape, src.dtype)
+    r = lib().orc_remap(1 if nearest else 0, _p(src), src.shape[0], src.shape[1], _p(mx), _p(my),
+                        mx.shape[0], mx.shape[1], _p(dst))
+    assert r == 0
+    return dst
 
 
 def atan2(y, x):

@@ -1352,6 +1352,53 @@ int orc_reconstruct_poly_roi(const psgc_recon_config* cfg, const double* l_phase
   return PSGC_OK;
 }
 
+// ---- setData remap (JSLAB_PhaseEpipolarReconstructor_V2.cpp:368-377) -------
+// cv::remap of OpenCV 4.7.0 (the reference's pinned version, PhaseShiftGrayCode3D.vcxproj:86;
+// absent here) with CV_32FC1 map pairs and the default BORDER_CONSTANT 0,
+// restated from its published imgwarp.cpp: RemapInvoker converts each map
+// pair to fixed point, X = cvRound(map_x * 32) (round half to even; a value
+// outside int range, NaN or infinity gives INT_MIN), integer tap
+// saturate_cast<short>(X >> 5) and fraction X & 31 (likewise for y);
+// remapBilinear for 64F images then reads the float table of
+// initInterTab2D(INTER_LINEAR): w = {(1-fy)(1-fx), (1-fy)fx, fy(1-fx), fy fx}
+// with fx = (X & 31)/32, exact in float, and sums
+// ((v00 w0 + v01 w1) + v10 w2) + v11 w3 in double; a tap outside the source
+// reads 0. remapNearest takes the tap saturate_cast<short>(cvRound(map)).
+namespace {
+int cv_round(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)std::nearbyint(v) : INT32_MIN;
+}
+int sat_short(int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); }
+}  // namespace
+
+int orc_remap(int kind, const void* src, int src_rows, int src_cols, const float* map_x,
+              const float* map_y, int rows, int cols, void* dst) {
+  if (kind != 0 && kind != 1) return PSGC_E_INVALID;
+  auto inside = [&](int y, int x) { return x >= 0 && x < src_cols && y >= 0 && y < src_rows; };
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      const size_t p = (size_t)r * cols + c;
+      if (kind == 1) {
+        const int x = sat_short(cv_round(map_x[p])), y = sat_short(cv_round(map_y[p]));
+        ((int32_t*)dst)[p] = inside(y, x) ? ((const int32_t*)src)[(size_t)y * src_cols + x] : 0;
+        continue;
+      }
+      const int X = cv_round(map_x[p] * 32.0f), Y = cv_round(map_y[p] * 32.0f);
+      const int sx = sat_short(X >> 5), sy = sat_short(Y >> 5);
+      const float ax = (float)(X & 31) * (1.0f / 32), ay = (float)(Y & 31) * (1.0f / 32);
+      const float w[4] = {(1.0f - ay) * (1.0f - ax), (1.0f - ay) * ax, ay * (1.0f - ax), ay * ax};
+      const double* S = (const double*)src;
+      auto tap = [&](int y, int x) { return inside(y, x) ? S[(size_t)y * src_cols + x] : 0.0; };
+      double v = tap(sy, sx) * (double)w[0];
+      v = v + tap(sy, sx + 1) * (double)w[1];
+      v = v + tap(sy + 1, sx) * (double)w[2];
+      v = v + tap(sy + 1, sx + 1) * (double)w[3];
+      ((double*)dst)[p] = v;
+    }
+  return PSGC_OK;
+}
+
 int orc_atan2_many(const double* y, const double* x, double* out, long n) {
   for (long i = 0; i < n; ++i) out[i] = psgc::atan2(y[i], x[i]);
   return PSGC_OK;

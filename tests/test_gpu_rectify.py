@@ -4,12 +4,19 @@ import math
 import numpy as np
 import pytest
 
+import oracle
+import torch
+import zim_amd
+from tests import scenes
+from tests.test_remap_cpu import special_maps
 from zim_amd import _capi
 
 pytestmark = pytest.mark.gpu
 
 
-def test_rectify_maps_and_remap(ctx):
+def test_rectify_maps(ctx):
+    """initUndistortRectifyMap for zero distortion: the exact homography
+    K R^T K_rect^-1, rounded once to float."""
     rows, cols = 60, 80
     K = np.array([[700.0, 0, 39.5], [0, 700.0, 29.5], [0, 0, 1]])
     th = math.radians(0.5)
@@ -24,30 +31,82 @@ def test_rectify_maps_and_remap(ctx):
     ray = np.linalg.inv(Krect) @ np.stack([uu.ravel(), vv.ravel(), np.ones(uu.size)])
     raw = K @ (R.T @ ray)
     ex, ey = (raw[0] / raw[2]).reshape(rows, cols), (raw[1] / raw[2]).reshape(rows, cols)
-    assert np.abs(mx - ex).max() < 1e-3 and np.abs(my - ey).max() < 1e-3
-    g = np.random.default_rng(0)
-    src = g.normal(0, 1, (rows, cols))
-    dst = np.zeros_like(src)
-    _capi.check(ctx.lib.psgc_remap(ctx.h, 0, _capi.ptr(src), rows, cols, _capi.ptr(mx),
-                                   _capi.ptr(my), rows, cols, _capi.ptr(dst), _capi.MEM_HOST))
-    x0 = np.floor(mx.astype(np.float64)).astype(int)
-    y0 = np.floor(my.astype(np.float64)).astype(int)
-    a = mx.astype(np.float64) - x0
-    b = my.astype(np.float64) - y0
-    pad = np.pad(src, 2)
-    tap = lambda yy, xx: np.where((xx >= 0) & (xx < cols) & (yy >= 0) & (yy < rows),  # noqa: E731
-                                  pad[np.clip(yy, -2, rows + 1) + 2, np.clip(xx, -2, cols + 1) + 2], 0)
-    ref = ((1 - a) * (1 - b) * tap(y0, x0) + a * (1 - b) * tap(y0, x0 + 1)
-           + (1 - a) * b * tap(y0 + 1, x0) + a * b * tap(y0 + 1, x0 + 1))
-    assert np.abs(dst - ref).max() < 1e-12
-    ks = g.integers(0, 70, (rows, cols)).astype(np.int32)
-    kd = np.zeros_like(ks)
-    _capi.check(ctx.lib.psgc_remap(ctx.h, 1, _capi.ptr(ks), rows, cols, _capi.ptr(mx),
-                                   _capi.ptr(my), rows, cols, _capi.ptr(kd), _capi.MEM_HOST))
-    xi, yi = np.rint(mx).astype(int), np.rint(my).astype(int)
-    inside = (xi >= 0) & (xi < cols) & (yi >= 0) & (yi < rows)
-    refk = np.where(inside, ks[np.clip(yi, 0, rows - 1), np.clip(xi, 0, cols - 1)], 0)
-    assert np.array_equal(kd, refk)
+    # within one float rounding (half an ulp, plus the FP64 evaluation)
+    for got, want in ((mx, ex), (my, ey)):
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(got - want) <= ulp * 0.5 + 1e-9)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_remap_matches_oracle(ctx, seed):
+    """cv::remap's rule (OpenCV 4.7.0: 1/32 px INTER_LINEAR, INTER_NEAREST,
+    BORDER_CONSTANT 0) on the device equals the oracle bit for bit, on
+    ties, NaN, infinite and far-out coordinates included."""
+    g = np.random.default_rng(300 + seed)
+    src = g.normal(0, 10, (37, 45))
+    ks = g.integers(0, 70, (37, 45)).astype(np.int32)
+    mx, my = special_maps(41, 53, seed)
+    dst = np.zeros(mx.shape)
+    _capi.check(ctx.lib.psgc_remap(ctx.h, 0, _capi.ptr(src), 37, 45, _capi.ptr(mx),
+                                   _capi.ptr(my), 41, 53, _capi.ptr(dst), _capi.MEM_HOST))
+    assert np.array_equal(dst, oracle.remap(src, mx, my))
+    kd = np.zeros(mx.shape, np.int32)
+    _capi.check(ctx.lib.psgc_remap(ctx.h, 1, _capi.ptr(ks), 37, 45, _capi.ptr(mx),
+                                   _capi.ptr(my), 41, 53, _capi.ptr(kd), _capi.MEM_HOST))
+    assert np.array_equal(kd, oracle.remap(ks, mx, my, nearest=True))
+    # device tensors through the Python API
+    t = zim_amd.remap(torch.from_numpy(src).cuda(), torch.from_numpy(mx).cuda(),
+                      torch.from_numpy(my).cuda(), ctx=ctx)
+    assert np.array_equal(t.cpu().numpy(), dst)
+
+
+def _rot(ax, deg):
+    t = math.radians(deg)
+    c, s = math.cos(t), math.sin(t)
+    if ax == "x":
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+
+
+@pytest.mark.parametrize("full_left", [1, 0])
+def test_reconstructor_with_rectification_matches_oracle(ctx, full_left):
+    """The reference's setCalibrationParams + setData + startCalculate with
+    a rotated zero-distortion rectification: full-left canvas, device remap
+    tables, then the chain; against the oracle's remap and chain on the
+    same tables."""
+    p, left, right = scenes.c2_small(3)
+    dcfg = scenes.c2_decode_config()
+    L, R = oracle.decode(dcfg, left), oracle.decode(dcfg, right)
+    rows, cols = L[0].shape
+    f = 2400.0
+    K1 = np.array([[f, 0, cols / 2 - 0.5], [0, f, rows / 2 - 0.5], [0, 0, 1]])
+    K2 = K1.copy()
+    R1 = _rot("y", 0.2) @ _rot("x", 0.1)
+    R2 = R1.copy()
+    Q = np.asarray(p.Q, np.float64).reshape(4, 4)
+    P1 = np.array([[f, 0, cols / 2 + 3.25, 0], [0, f, rows / 2 - 2.75, 0], [0, 0, 1, 0]])
+    P2 = P1.copy()
+    cfg = scenes.c2_recon_config(use_full_left_rectification=full_left, mls_window_size=4)
+    rec = zim_amd.PhaseEpipolarReconstructor_V2(cfg, ctx)
+    rec.setRectification(K1, K2, R1, R2, P1, P2, Q, (cols, rows))
+    w, h = rec.output_size
+    if full_left:
+        assert (w, h) != (cols, rows)
+    rec.setData(*(torch.from_numpy(np.ascontiguousarray(a)).cuda()
+                  for a in (L[0], R[0], L[1], R[1], L[2])), 70)
+    gx, gP = rec.startCalculate()
+    torch.cuda.synchronize()
+    (lx, ly), (rx, ry) = [(a.cpu().numpy(), b.cpu().numpy()) for a, b in rec._maps]
+    lp, rp = oracle.remap(L[0], lx, ly), oracle.remap(R[0], rx, ry)
+    lk, rk = oracle.remap(L[1], lx, ly, nearest=True), oracle.remap(R[1], rx, ry, nearest=True)
+    assert np.array_equal(rec._in[0].cpu().numpy(), lp)
+    assert np.array_equal(rec._in[3].cpu().numpy(), rk)
+    ox, oP, od, omc = oracle.reconstruct(cfg, lp, rp, lk, rk, 70, rec.Q, rec.R1)
+    gx, gP = gx.cpu().numpy(), gP.cpu().numpy()
+    assert gP.shape == (h, w)
+    assert np.array_equal(gP, oP), f"mask mismatch {(gP != oP).sum()}"
+    assert np.abs(gx - ox).max() <= 1e-3
+    assert oP.mean() > 0.2
 
 
 @pytest.mark.parametrize("shape,density", [((37, 53), 0.6), ((512, 700), 0.3), ((1, 1), 1.0),

@@ -30,7 +30,7 @@ from ._capi import (DecodeConfig, ReconConfig, PsgcError, MEM_DEVICE, MEM_HOST, 
 
 __all__ = ["Batch", "Context", "DecodeConfig", "ReconConfig", "PhaseShiftDecoder", "Version",
            "PhaseEpipolarReconstructor_V2", "applyMLSFilterToPointCloud", "decode",
-           "reconstruct", "PsgcError"]
+           "reconstruct", "PsgcError", "full_left_rectification", "rectify_maps", "remap"]
 
 _SAMPLE = {torch.uint8: _capi.SAMPLE_U8, torch.uint16: _capi.SAMPLE_U16,
            torch.float64: _capi.SAMPLE_F64, np.dtype(np.uint8): _capi.SAMPLE_U8,
@@ -87,11 +87,13 @@ def default_context(device: int = 0) -> Context:
 
 
 def mls_half_window(radius, f, z_min):
-    """A half window that holds an MLS radius (mm) around every point of a
+    """First estimate of the half window that holds an MLS radius (mm) for a
     pair whose nearest surface lies at z_min (mm), for focal length f (px):
-    r f / z_min pixels span the radius on a fronto-parallel surface, and the
-    half-pixel margin covers points off the optical axis and tilted
-    surfaces. The organized window of the reference's filter
+    r f / z_min pixels span the radius on a fronto-parallel surface, plus a
+    half-pixel margin. Points off the optical axis, tilted surfaces and
+    smoothing can need more: a window holds the radius for a cloud when its
+    outer ring has no neighbour within the radius, which the caller checks on
+    the cloud itself. The organized window of the reference's filter
     (JSLAB_MLS_Filter.cpp:191-194) is otherwise a cut through the ball."""
     return int(math.ceil(radius * f / z_min + 0.5))
 
@@ -552,23 +554,113 @@ class PhaseShiftDecoder:
         return self._valid
 
 
+def full_left_rectification(K1, image_size, R1, P1, P2, Q):
+    """computeFullLeftRectification (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+    248-289) for zero distortion: returns (P1, P2, Q, (width, height)) with
+    the principal points shifted so that the whole left image lands on the
+    rectified canvas, and the canvas size (psgc_full_left_rectification)."""
+    lib = _capi.load()
+    K1 = np.ascontiguousarray(K1, np.float64).reshape(9)
+    R1 = np.ascontiguousarray(R1, np.float64).reshape(9)
+    P1 = np.array(P1, np.float64).reshape(12)
+    P2 = np.array(P2, np.float64).reshape(12)
+    Q = np.array(Q, np.float64).reshape(16)
+    w, h = C.c_int(), C.c_int()
+    _capi.check(lib.psgc_full_left_rectification(_capi.ptr(K1), int(image_size[0]),
+                                                 int(image_size[1]), _capi.ptr(R1), _capi.ptr(P1),
+                                                 _capi.ptr(P2), _capi.ptr(Q), C.byref(w),
+                                                 C.byref(h)))
+    return P1.reshape(3, 4), P2.reshape(3, 4), Q.reshape(4, 4), (w.value, h.value)
+
+
+def rectify_maps(K, R, P, size, ctx: Context | None = None, device="cuda"):
+    """cv::initUndistortRectifyMap(K, 0, R, P, size, CV_32FC1) on the device
+    (psgc_rectify_maps): two f32 maps (height, width) of source pixel
+    coordinates."""
+    ctx = ctx or default_context()
+    w, h = int(size[0]), int(size[1])
+    K = np.ascontiguousarray(K, np.float64).reshape(9)
+    R = np.ascontiguousarray(R, np.float64).reshape(9)
+    Kr = np.ascontiguousarray(np.asarray(P, np.float64).reshape(3, -1)[:, :3]).reshape(9)
+    mx = torch.empty((h, w), dtype=torch.float32, device=device)
+    my = torch.empty_like(mx)
+    _capi.check(ctx.lib.psgc_rectify_maps(ctx.h, _capi.ptr(K), _capi.ptr(R), _capi.ptr(Kr), h, w,
+                                          _capi.ptr(mx), _capi.ptr(my), MEM_DEVICE))
+    return mx, my
+
+
+def remap(src, map_x, map_y, nearest=False, ctx: Context | None = None):
+    """cv::remap as the reference's setData calls it (…V2.cpp:372-376): a f64
+    map with INTER_LINEAR, or an i32 map with INTER_NEAREST (nearest=True),
+    BORDER_CONSTANT 0 (psgc_remap). Device tensors in, a device tensor of the
+    maps' shape out."""
+    ctx = ctx or default_context()
+    if not (_is_dev(src) and _is_dev(map_x) and _is_dev(map_y)):
+        raise ValueError("remap: src and maps must be device tensors")
+    rows, cols = map_x.shape
+    dev = src.device
+    dt = torch.int32 if nearest else torch.float64
+    if src.dim() != 2:
+        raise ValueError(f"remap: src must be 2-D, got {tuple(src.shape)}")
+    src = _map("src", src, True, dev, dt, tuple(src.shape))
+    map_x = _map("map_x", map_x, True, dev, torch.float32, (rows, cols))
+    map_y = _map("map_y", map_y, True, dev, torch.float32, (rows, cols))
+    dst = torch.empty((rows, cols), dtype=dt, device=dev)
+    _capi.check(ctx.lib.psgc_remap(ctx.h, 1 if nearest else 0, _capi.ptr(src), src.shape[0],
+                                   src.shape[1], _capi.ptr(map_x), _capi.ptr(map_y), rows, cols,
+                                   _capi.ptr(dst), MEM_DEVICE))
+    return dst
+
+
 class PhaseEpipolarReconstructor_V2:
     """JSLAB_PhaseEpipolarReconstructor_V2 (JSLAB_PhaseEpipolarReconstructor_V2.h:
-    11-171) on the GPU, for rectified inputs (the calibration is given as Q
-    and R1; remap is the identity for a pre-rectified pair)."""
+    11-171) on the GPU. The calibration is given either as Q and R1 for a
+    pre-rectified pair (setCalibrationParams; setData then takes the maps as
+    they are), or as the stereoRectify outputs of a zero-distortion pair
+    (setRectification: the full-left canvas when the config asks for it, and
+    device remap tables that setData applies as the reference does)."""
 
     def __init__(self, config: ReconConfig, ctx: Context | None = None):
         self.config = config
         self.ctx = ctx
         self.Q = self.R1 = None
         self._disp = None
+        self._maps = None
+        self.output_size = None
 
     def setCalibrationParams(self, Q, R1):
         self.Q = np.asarray(Q, np.float64).reshape(16)
         self.R1 = np.asarray(R1, np.float64).reshape(9)
+        self._maps = None
+
+    def setRectification(self, K1, K2, R1, R2, P1, P2, Q, image_size, device="cuda"):
+        """The part of setCalibrationParams after cv::stereoRectify
+        (…V2.cpp:313-338) for zero distortion: with use_full_left_rectification
+        the principal points move and the canvas grows to hold the whole left
+        image (computeFullLeftRectification), then the left and right remap
+        tables are built on the device (initUndistortRectifyMap)."""
+        size = (int(image_size[0]), int(image_size[1]))
+        if self.config.use_full_left_rectification:
+            P1, P2, Q, size = full_left_rectification(K1, image_size, R1, P1, P2, Q)
+        self.Q = np.asarray(Q, np.float64).reshape(16)
+        self.R1 = np.asarray(R1, np.float64).reshape(9)
+        self.output_size = size
+        self._maps = (rectify_maps(K1, R1, P1, size, self.ctx, device),
+                      rectify_maps(K2, R2, P2, size, self.ctx, device))
 
     def setData(self, left, right, left_k, right_k, left_mod, pha_f):
+        """setData (…V2.cpp:368-395): phase and modulation remapped with
+        INTER_LINEAR, fringe orders with INTER_NEAREST, when remap tables are
+        set."""
+        if self._maps is not None:
+            (lx, ly), (rx, ry) = self._maps
+            left, right = remap(left, lx, ly, ctx=self.ctx), remap(right, rx, ry, ctx=self.ctx)
+            left_k = remap(left_k, lx, ly, nearest=True, ctx=self.ctx)
+            right_k = remap(right_k, rx, ry, nearest=True, ctx=self.ctx)
+            if left_mod is not None:
+                left_mod = remap(left_mod, lx, ly, ctx=self.ctx)
         self._in = (left, right, left_k, right_k, int(pha_f))
+        self._mod = left_mod
 
     def startCalculate(self):
         lp, rp, lk, rk, pha_f = self._in

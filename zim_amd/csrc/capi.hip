@@ -2,6 +2,7 @@
 // argument validation, context-owned buffers, host staging and the
 // orchestration of the per-stage kernels.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdarg>
@@ -1556,6 +1557,57 @@ int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1, double cx2,
   std::memcpy(Q, q, sizeof(q));
   const double r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   std::memcpy(R1, r, sizeof(r));
+  return PSGC_OK;
+}
+
+int psgc_full_left_rectification(const double K1[9], int width, int height, const double R1[9],
+                                 double P1[12], double P2[12], double Q[16], int* out_width,
+                                 int* out_height) {
+  if (!K1 || !R1 || !P1 || !P2 || !Q || !out_width || !out_height)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (width < 1 || height < 1) PSGC_FAIL(PSGC_E_INVALID, "image size %dx%d", width, height);
+  if (!(K1[0] != 0) || !(K1[4] != 0)) PSGC_FAIL(PSGC_E_INVALID, "K1 focal lengths");
+  // undistortPoints with zero distortion: its iterations leave the
+  // normalised point unchanged, then RR = P1[:3,:3] R1 maps it
+  double RR[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      RR[i][j] = P1[i * 4 + 0] * R1[0 * 3 + j] + P1[i * 4 + 1] * R1[1 * 3 + j] +
+                 P1[i * 4 + 2] * R1[2 * 3 + j];
+  const double ifx = 1.0 / K1[0], ify = 1.0 / K1[4], cx = K1[2], cy = K1[5];
+  const float cu[4] = {0.0f, (float)width - 1.0f, (float)width - 1.0f, 0.0f};
+  const float cv[4] = {0.0f, 0.0f, (float)height - 1.0f, (float)height - 1.0f};
+  float min_x = 0, max_x = 0, min_y = 0, max_y = 0;
+  for (int c = 0; c < 4; ++c) {
+    const double x = ((double)cu[c] - cx) * ifx, y = ((double)cv[c] - cy) * ify;
+    const double xx = RR[0][0] * x + RR[0][1] * y + RR[0][2];
+    const double yy = RR[1][0] * x + RR[1][1] * y + RR[1][2];
+    const double ww = 1.0 / (RR[2][0] * x + RR[2][1] * y + RR[2][2]);
+    const float px = (float)(xx * ww), py = (float)(yy * ww);
+    if (!std::isfinite(px) || !std::isfinite(py))
+      PSGC_FAIL(PSGC_E_INVALID, "corner %d maps to infinity", c);
+    if (c == 0) {
+      min_x = max_x = px;
+      min_y = max_y = py;
+    } else {
+      min_x = std::min(min_x, px);
+      max_x = std::max(max_x, px);
+      min_y = std::min(min_y, py);
+      max_y = std::max(max_y, py);
+    }
+  }
+  const double shift_x = -std::floor(min_x), shift_y = -std::floor(min_y);
+  const float wf = std::ceil(max_x - min_x + 1.0f), hf = std::ceil(max_y - min_y + 1.0f);
+  if (!(wf <= 65536.0f) || !(hf <= 65536.0f))
+    PSGC_FAIL(PSGC_E_INVALID, "rectified canvas %.0fx%.0f", wf, hf);
+  P1[2] += shift_x;
+  P1[6] += shift_y;
+  P2[2] += shift_x;
+  P2[6] += shift_y;
+  Q[3] -= shift_x;
+  Q[7] -= shift_y;
+  *out_width = (int)wf;
+  *out_height = (int)hf;
   return PSGC_OK;
 }
 

@@ -3,9 +3,9 @@
 //
 // remap:    cv::remap semantics of the reference's setData
 //           (JSLAB_PhaseEpipolarReconstructor_V2.cpp:368-395): phase and
-//           modulation INTER_LINEAR, orders INTER_NEAREST, BORDER_CONSTANT 0.
-//           Bilinear weights are exact FP64 (OpenCV quantises them to 1/32 px;
-//           that quantisation is not reproduced).
+//           modulation INTER_LINEAR, orders INTER_NEAREST, BORDER_CONSTANT 0,
+//           with OpenCV 4.7.0's fixed-point coordinates (1/32 px for
+//           bilinear) and float weight table.
 // maps:     initUndistortRectifyMap for zero distortion
 //           (…V2.cpp:333-338): raw = K_raw R^T K_rect^-1 (u, v, 1).
 // compact:  stable row-major compaction of the points with P != 0 (the
@@ -38,27 +38,35 @@ __global__ void rect_maps_kernel(int rows, int cols, const double* __restrict__ 
   my[p] = (float)(y / w);
 }
 
-template <typename T>
-__device__ inline T sample_linear(const T* src, int rows, int cols, float fx, float fy) {
-  const double x = fx, y = fy;
-  const double x0f = floor(x), y0f = floor(y);
-  const int x0 = (int)x0f, y0 = (int)y0f;
-  const double a = x - x0f, b = y - y0f;
-  auto at = [&](int yy, int xx) -> double {
-    return (xx >= 0 && xx < cols && yy >= 0 && yy < rows) ? (double)src[(size_t)yy * cols + xx]
-                                                           : 0.0;
-  };
-  const double v = (1 - a) * (1 - b) * at(y0, x0) + a * (1 - b) * at(y0, x0 + 1) +
-                   (1 - a) * b * at(y0 + 1, x0) + a * b * at(y0 + 1, x0 + 1);
-  return (T)v;
+// cv::remap's fixed-point coordinates (OpenCV 4.7.0 RemapInvoker):
+// cvRound (half to even; NaN, infinities and values outside int range give
+// INT_MIN) and saturate_cast<short>.
+__device__ inline int cv_round(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)rintf(v) : INT32_MIN;
 }
+__device__ inline int sat_short(int v) { return min(max(v, -32768), 32767); }
 
+// INTER_LINEAR on a f64 map: the coordinate is quantised to 1/32 px, the
+// four float weights of OpenCV's bilinear table (exact products of k/32)
+// multiply the taps in double, summed in the table's order; taps outside the
+// source read 0 (BORDER_CONSTANT).
 __global__ void remap_f64_kernel(const double* __restrict__ src, double* __restrict__ dst,
                                  const float* __restrict__ mx, const float* __restrict__ my,
                                  int rows, int cols, int src_rows, int src_cols) {
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (size_t)rows * cols) return;
-  dst[p] = sample_linear(src, src_rows, src_cols, mx[p], my[p]);
+  const int X = cv_round(mx[p] * 32.0f), Y = cv_round(my[p] * 32.0f);
+  const int sx = sat_short(X >> 5), sy = sat_short(Y >> 5);
+  const float ax = (float)(X & 31) * (1.0f / 32), ay = (float)(Y & 31) * (1.0f / 32);
+  auto tap = [&](int y, int x) -> double {
+    return (x >= 0 && x < src_cols && y >= 0 && y < src_rows) ? src[(size_t)y * src_cols + x]
+                                                               : 0.0;
+  };
+  double v = tap(sy, sx) * (double)((1.0f - ay) * (1.0f - ax));
+  v = v + tap(sy, sx + 1) * (double)((1.0f - ay) * ax);
+  v = v + tap(sy + 1, sx) * (double)(ay * (1.0f - ax));
+  v = v + tap(sy + 1, sx + 1) * (double)(ay * ax);
+  dst[p] = v;
 }
 
 __global__ void remap_i32_nearest_kernel(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
@@ -67,8 +75,7 @@ __global__ void remap_i32_nearest_kernel(const int32_t* __restrict__ src, int32_
                                          int src_rows, int src_cols) {
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (size_t)rows * cols) return;
-  // cv::remap INTER_NEAREST rounds the coordinates to the nearest integer
-  const int x = (int)rintf(mx[p]), y = (int)rintf(my[p]);
+  const int x = sat_short(cv_round(mx[p])), y = sat_short(cv_round(my[p]));
   dst[p] = (x >= 0 && x < src_cols && y >= 0 && y < src_rows) ? src[(size_t)y * src_cols + x] : 0;
 }
 
