@@ -129,7 +129,9 @@ def main():
     log(f"rendered in {time.time() - t0:.1f}s")
     npix = ROWS * COLS
     t0 = time.time()
-    windows, ring_left = mls_windows(pairs, dcfg, ctx)
+    windows, ring_left = mls_windows(
+        [dict(left=p.left, right=p.right, decode=dcfg, pha_f=70, Q=p.Q, R1=p.R1) for p in pairs],
+        [scenes.c2_window(p, MLS_RADIUS) for p in pairs], ctx)
     rcfgs = [scenes.c2_recon_config(mls_window_size=w) for w in windows]
     log(f"MLS half windows {min(windows)}-{max(windows)}, mean {sum(windows) / len(windows):.2f} "
         f"(chosen in {time.time() - t0:.1f}s; in-radius neighbours left on an outer ring: {ring_left})")
@@ -538,20 +540,25 @@ def run_c4(args):
 
 
 C3_SIZES = ((480, 640), (1024, 1280), (1200, 1920), (2048, 2448))
-C3_SCHEDULES = ("3-step", "4-step", "8-step", "gray")
+C3_STEPS = (3, 4, 8)
+C3_SCHEDULES = ("heterodyne", "gray")
 
 
 def run_c3(args):
     """Uneven batch: `--pairs` pairs (default 64), each with a resolution
-    drawn uniformly from C3_SIZES and a schedule from C3_SCHEDULES (seeded):
-    three-frequency 70/64/59 heterodyne fringes with 3, 4 or 8 steps (the
-    bump height field, f scaled with the width, uint16 12-bit), or the 4-step
-    period-16 fringes plus the 6-bit Gray code with complements (the tilted
-    plane and spheres, uint8). Filters as in the full-size workload (MLS
-    order 2 at 1.5 mm, SOR k=30). value = Mpts/s of the autotuned batch
-    (one graph replay per step, points = left pixels of every pair); the
-    same pairs run one at a time through the plain per-pair path beside it,
-    whose masks and xyz must agree with the batch."""
+    drawn uniformly from C3_SIZES, a step count from {3, 4, 8} and a
+    three-frequency (70/64/59 heterodyne) or Gray-code (period 16 px plus
+    6-bit Gray code with complements) schedule, seeded; every pair is the
+    config-1 scene (plane tilted 20 deg at 600 mm plus 3 spheres, f = 800 px
+    per 640 columns, B = 100 mm) in uint16 12-bit. Filters as in the
+    full-size workload (Gaussian 5x5, SOR k=30, MLS order 2 at 1.5 mm in the
+    smallest window that holds it for the pair). value = Mpts/s of the
+    autotuned batch (one graph replay per step, points = left pixels of
+    every pair). The autotuner also tunes the row match per image width
+    (its own launch, grid and threads per row for each width) against one
+    launch for all widths; both are timed here. The same pairs run one at a
+    time through the plain per-pair path beside it, whose masks and xyz must
+    agree with the batch."""
     import numpy as np
     import torch
 
@@ -563,47 +570,80 @@ def run_c3(args):
     dev = torch.device("cuda:0")
     ctx = zim_amd.Context(0)
     rng = np.random.default_rng(3)
-    specs, npts = [], 0
+    specs, npts, starts = [], 0, []
     t0 = time.time()
     for i in range(args.pairs):
         rows, cols = C3_SIZES[rng.integers(len(C3_SIZES))]
-        sched = C3_SCHEDULES[rng.integers(len(C3_SCHEDULES))]
-        if sched == "gray":
-            p = synth.render_c1(4000 + i, device=dev, rows=rows, cols=cols)
-            dcfg, pha_f = scenes.c1_gray_config(), 50
+        steps = C3_STEPS[rng.integers(len(C3_STEPS))]
+        gray = C3_SCHEDULES[rng.integers(len(C3_SCHEDULES))] == "gray"
+        p = synth.render_c3(4000 + i, device=dev, rows=rows, cols=cols, steps=steps, gray=gray)
+        if gray:
+            dcfg, pha_f = scenes.c1_gray_config(shift_num=(steps, 0, 0)), 50
         else:
-            st = int(sched.split("-")[0])
-            p = synth.render_c2(4000 + i, device=dev, rows=rows, cols=cols,
-                                f=2400.0 * cols / 2448.0, steps=(st,) * 3)
-            dcfg, pha_f = scenes.c2_decode_config(shift_num=(st,) * 3), 70
-        specs.append(dict(left=p.left, right=p.right, decode=dcfg, recon=scenes.c2_recon_config(),
-                          pha_f=pha_f, Q=p.Q, R1=p.R1, _key=f"{cols}x{rows} {sched}"))
+            dcfg, pha_f = scenes.c2_decode_config(shift_num=(steps,) * 3), 70
+        specs.append(dict(left=p.left, right=p.right, decode=dcfg, pha_f=pha_f, Q=p.Q, R1=p.R1,
+                          _key=f"{cols}x{rows} {steps}-step {'gray' if gray else 'heterodyne'}"))
+        starts.append(scenes.c2_window(p, MLS_RADIUS))
         npts += rows * cols
     torch.cuda.synchronize()
     mix = {}
     for sp in specs:
         mix[sp["_key"]] = mix.get(sp["_key"], 0) + 1
     log(f"rendered {len(specs)} pairs ({npts / 1e6:.1f} Mpts) in {time.time() - t0:.1f}s: {mix}")
-    pairs = [{k: v for k, v in sp.items() if k != "_key"} for sp in specs]
+    base = [{k: v for k, v in sp.items() if k != "_key"} for sp in specs]
+    t0 = time.time()
+    windows, ring_left = mls_windows(base, starts, ctx)
+    pairs = [dict(sp, recon=scenes.c2_recon_config(mls_window_size=w))
+             for sp, w in zip(base, windows)]
+    log(f"MLS half windows {min(windows)}-{max(windows)} (chosen in {time.time() - t0:.1f}s; "
+        f"in-radius neighbours left on an outer ring: {ring_left})")
 
     t0 = time.time()
     batch = zim_amd.Batch(pairs, ctx, graph=True, autotune=True)
     torch.cuda.synchronize()
     grids = {s: batch.grid(s) for s in ("decode", "decode_rows", "match", "match_threads",
-                                        "median5", "repro_gauss", "sor_knn", "mls",
-                                        "mls_variant")}
-    log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}")
-    for _ in range(args.warmup):
-        batch.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run()
-    torch.cuda.synchronize()
-    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+                                        "match_groups", "median5", "repro_gauss", "sor_knn",
+                                        "mls", "mls_variant")}
+    widths = sorted({c for _, c in C3_SIZES})
+    per_width = {}
+    for w in widths:
+        try:
+            per_width[str(w)] = {"grid": batch.grid(f"match@{w}"),
+                                 "threads": batch.grid(f"match_threads@{w}")}
+        except zim_amd.PsgcError:
+            pass
+    log(f"batch planned, autotuned and captured in {time.time() - t0:.1f}s; grids {grids}; "
+        f"row match per width {per_width}")
+
+    def timed(steps):
+        for _ in range(args.warmup):
+            batch.run()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            batch.run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) * 1e3 / steps
+
+    ms_per_step = timed(args.steps)
     value = npts / (ms_per_step * 1e-3) / 1e6
     valid = sum(int((o["P"] != 0).sum().item()) for o in batch.outputs)
     log(f"batched: {ms_per_step:.2f} ms/step, {value:.1f} Mpts/s, {valid} valid points")
+    # the other row-match mode, for the record (same outputs)
+    match_modes = None
+    if per_width:
+        chosen = 1 if grids["match_groups"] else 0
+        ref = [(o["P"].clone(), o["xyz"].clone()) for o in batch.outputs]
+        batch.set_grid("match_groups", 1 - chosen)
+        other = timed(max(1, args.steps // 2))
+        same = all(torch.equal(P, o["P"]) and torch.equal(x, o["xyz"])
+                   for (P, x), o in zip(ref, batch.outputs))
+        batch.set_grid("match_groups", chosen)
+        match_modes = {"per_width_ms_per_step": round(other if not chosen else ms_per_step, 3),
+                       "single_ms_per_step": round(ms_per_step if not chosen else other, 3),
+                       "chosen": "per width" if chosen else "single", "outputs_identical": same}
+        log(f"row match modes: {match_modes}")
+        del ref
 
     # the same pairs one at a time through the plain kernels, a sync per pair
     ctx.set_kernels("plain")
@@ -637,10 +677,14 @@ def run_c3(args):
             "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
             "higher_is_better": True, "dtype": "f64",
-            "data": "synthetic (seeded height fields and plane-sphere scenes rendered on the GPU)",
+            "data": "synthetic (seeded plane-sphere scenes rendered on the GPU)",
             "valid_points": valid,
-            "config": {"workload": "uneven batch", "pairs": len(pairs), "points": npts,
-                       "mix": mix, "path": "batched graph"},
+            "config": {"workload": "uneven batch of config-1 scenes, u16 12-bit",
+                       "pairs": len(pairs), "points": npts, "mix": mix, "path": "batched graph",
+                       "mls_half_windows": {str(w): windows.count(w)
+                                            for w in sorted(set(windows))},
+                       "mls_ring_points_left": ring_left},
+            "row_match_per_width": per_width, "row_match_modes": match_modes,
             "per_pair_plain": {"mpts_per_s": round(plain_val, 3), "ms": round(plain_ms, 3),
                                "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr},
             "speedup_vs_per_pair": round(value / plain_val, 2), "autotuned_grids": grids}
@@ -761,22 +805,23 @@ def run_c5(args):
     print(json.dumps(line), flush=True)
 
 
-def mls_windows(pairs, dcfg, ctx, radius=MLS_RADIUS, largest=12):
+def mls_windows(specs, starts, ctx, radius=MLS_RADIUS, largest=14):
     """Per pair, the MLS half window that holds the radius: the pipeline runs
     once without MLS (its output is the MLS input cloud), and from the
-    depth-based estimate upward the first half window whose outer ring has
-    no valid neighbour within the radius (FP64 distances, a 1e-6 relative
-    margin over the filter's FP32 test) is taken."""
+    depth-based estimate `starts[i]` upward the first half window whose outer
+    ring has no valid neighbour within the radius (FP64 distances, a 1e-6
+    relative margin over the filter's FP32 test) is taken. specs: Batch pair
+    dicts (their recon configs are replaced by the same chain without MLS).
+    Returns the windows and the in-radius neighbours left on the outer ring
+    of pairs that reached `largest`."""
     import torch
     import zim_amd
     from tests import scenes
-    probe = zim_amd.Batch([dict(left=p.left, right=p.right, decode=dcfg,
-                                recon=scenes.c2_recon_config(use_mls=0), pha_f=70, Q=p.Q, R1=p.R1)
-                           for p in pairs], ctx, graph=False)
+    probe = zim_amd.Batch([dict(sp, recon=scenes.c2_recon_config(use_mls=0)) for sp in specs],
+                          ctx, graph=False)
     outs = probe.run()
     out, left = [], 0
-    for p, o in zip(pairs, outs):
-        hw = scenes.c2_window(p, radius)
+    for hw, o in zip(starts, outs):
         ring = scenes.ring_points(o["xyz"], o["P"], hw, radius * (1 + 1e-6))
         while ring and hw < largest:
             hw += 1

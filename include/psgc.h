@@ -449,12 +449,17 @@ PSGC_API int psgc_batch_run_eager(psgc_batch* batch);
  * "repro_gauss", "sor_knn", "mls"): workgroups of the persistent grid;
  * "decode_rows": rows per decode tile (2 or 4); "match_threads": threads per
  * rectified row in the row match (128, 256 or 512); "mls_variant": 0 = weights
- * evaluated in each MLS pass, 1 = weights kept in LDS between the passes
- * (MLS half windows up to 6). 0 = the stage's default grid. */
+ * evaluated in each MLS pass, 1 = the first weights kept in LDS between the
+ * passes. 0 = the stage's default grid. A batch whose pairs have several
+ * image widths also tunes the row match per width: "match_groups" is the
+ * number of per-width launches that replace the single one (0: the single
+ * launch is faster and runs), "match@<width>" and "match_threads@<width>"
+ * the grid and threads per row of that width's launch. */
 PSGC_API int psgc_batch_grid(psgc_batch* batch, const char* stage, int* grid);
 /* Pins a launch shape (same stage names; grids 1..2^20 workgroups, decode_rows
- * 2 or 4, match_threads 128, 256 or 512, mls_variant 0 or 1), e.g. one saved from an earlier autotuned batch of the same
- * resolutions. A captured graph is re-captured. Results do not depend on the
+ * 2 or 4, match_threads 128, 256 or 512, mls_variant 0 or 1, match_groups 0
+ * (single launch) or 1 (per-width launches)), e.g. one saved from an earlier
+ * autotuned batch of the same resolutions. A captured graph is re-captured. Results do not depend on the
  * launch shape. No counterpart in the reference (its launches have fixed
  * shapes). */
 PSGC_API int psgc_batch_set_grid(psgc_batch* batch, const char* stage, int value);

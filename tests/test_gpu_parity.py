@@ -372,9 +372,9 @@ def test_batch_mls_variants_match_oracle(ctx):
     evaluated in every pass, and weights kept in LDS between the passes
     (points with more in-radius neighbours than the cache holds evaluate the
     rest again); each against the CPU restatement. The batch holds the pair
-    four times, with half windows 6, 8 and 10 (one launch per window class,
-    caches of 24, 20 and 16 weights) under GPU semantics (fused sums) and
-    CPU semantics (rounded products and sums)."""
+    six times, with half windows 6, 8, 10, 12 and 14 (one launch per window
+    class, caches of 24, 20 and 16 weights, none above 10) under GPU
+    semantics (fused sums) and CPU semantics (rounded products and sums)."""
     dcfg = scenes.c2_decode_config()
     # a 2.5 mm radius on this ~1 m deep scene gives many points more
     # neighbours than any cache holds
@@ -383,13 +383,14 @@ def test_batch_mls_variants_match_oracle(ctx):
     nx, nP, _, _ = oracle.reconstruct(scenes.c2_recon_config(use_mls=0), Lr[0], Rr[0], Lr[1],
                                       Rr[1], 70, p.Q, p.R1)
     specs = [(6, _capi.SEM_GPU, 24), (6, _capi.SEM_CPU, 24), (8, _capi.SEM_GPU, 20),
-             (10, _capi.SEM_CPU, 16)]
+             (10, _capi.SEM_CPU, 16), (12, _capi.SEM_GPU, 0), (14, _capi.SEM_CPU, 0)]
     cfgs, refs = [], []
     for hw, sem, kc in specs:
         cnt = _inradius_counts(nx, nP, hw, 2.5)
         print(f"hw {hw}: in-radius neighbours max {cnt.max()}, points above {kc}: "
               f"{int((cnt > kc).sum())}", flush=True)
-        assert (cnt > kc).sum() > 100 and ((cnt > 3) & (cnt <= kc)).sum() > 100
+        if kc:
+            assert (cnt > kc).sum() > 100 and ((cnt > 3) & (cnt <= kc)).sum() > 100
         c = scenes.c2_recon_config(mls_max_radius=2.5, mls_spatial_sigma=1.0,
                                    mls_window_size=hw, semantics=sem)
         cfgs.append(c)
@@ -472,34 +473,48 @@ def test_batch_mixed_schedules_match_oracle(ctx):
 
 def test_batch_at_uneven_full_sizes_matches_oracle(ctx):
     """One captured, autotuned batch of the uneven configuration's real sizes
-    and schedules: 640x480 3-step, 1280x1024 Gray code, 1920x1200 8-step and
-    2448x2048 4-step; each pair equal to the oracle run on it alone."""
+    and schedules, every pair the config-1 scene in uint16 12-bit: 640x480
+    3-step heterodyne, 1280x1024 4-step Gray code, 1920x1200 8-step
+    heterodyne and 2448x2048 3-step Gray code; each pair equal to the oracle
+    run on it alone, with the row match in one launch and per image width
+    (identical outputs)."""
     from zim_amd import synth
     specs = []
-    for i, (rows, cols, sched) in enumerate(((480, 640, 3), (1024, 1280, "gray"),
-                                             (1200, 1920, 8), (2048, 2448, 4))):
-        if sched == "gray":
-            p = synth.render_c1(4100 + i, device="cuda", rows=rows, cols=cols)
-            dcfg, pha_f = scenes.c1_gray_config(), 50
+    for i, (rows, cols, steps, gray) in enumerate(((480, 640, 3, False), (1024, 1280, 4, True),
+                                                   (1200, 1920, 8, False),
+                                                   (2048, 2448, 3, True))):
+        p = synth.render_c3(4100 + i, device="cuda", rows=rows, cols=cols, steps=steps,
+                            gray=gray)
+        if gray:
+            dcfg, pha_f = scenes.c1_gray_config(shift_num=(steps, 0, 0)), 50
         else:
-            p = synth.render_c2(4100 + i, device="cuda", rows=rows, cols=cols,
-                                f=2400.0 * cols / 2448.0, steps=(sched,) * 3)
-            dcfg, pha_f = scenes.c2_decode_config(shift_num=(sched,) * 3), 70
+            dcfg, pha_f = scenes.c2_decode_config(shift_num=(steps,) * 3), 70
         specs.append((p, dcfg, pha_f))
     rcfg = scenes.c2_recon_config()
     b = zim_amd.Batch([dict(left=p.left, right=p.right, decode=d, recon=rcfg, pha_f=f, Q=p.Q,
                             R1=p.R1) for p, d, f in specs], ctx, graph=True, autotune=True)
-    outs = b.run()
+    for w in (640, 1280, 1920, 2448):
+        assert b.grid(f"match_threads@{w}") in (128, 256, 512) and b.grid(f"match@{w}") > 0
+    chosen = b.grid("match_groups")
+    assert chosen in (0, 4)
+    outs = [{k: v.clone() for k, v in o.items()} for o in b.run()]
+    b.set_grid("match_groups", 0 if chosen else 1)
+    assert b.grid("match_groups") == (0 if chosen else 4)
+    again = b.run()
     torch.cuda.synchronize()
+    for o, a in zip(outs, again):
+        assert torch.equal(o["P"], a["P"]) and torch.equal(o["xyz"], a["xyz"])
     for i, ((p, dcfg, pha_f), out) in enumerate(zip(specs, outs)):
         lh = [x.cpu().numpy() for x in p.left]
         rh = [x.cpu().numpy() for x in p.right]
         Lr, Rr = oracle.decode(dcfg, lh), oracle.decode(dcfg, rh)
         rx, rP, _, _ = oracle.reconstruct(rcfg, Lr[0], Rr[0], Lr[1], Rr[1], pha_f, p.Q, p.R1)
-        print("pair", i, p.left[0].shape, "valid", int(rP.sum()))
+        print("pair", i, p.left[0].shape, "valid", int(rP.sum()), flush=True)
         assert rP.sum() > 0.3 * rP.size, i
         assert np.array_equal(host(out["P"]), rP), (i, (host(out["P"]) != rP).sum())
         assert np.abs(host(out["xyz"]) - rx).max() <= XYZ_TOL, i
+    with pytest.raises(zim_amd.PsgcError):
+        b.grid("match@1000")
     b.close()
 
 

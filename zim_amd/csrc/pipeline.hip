@@ -11,6 +11,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -133,12 +134,24 @@ int mls_sems(const std::vector<CloudJob>& jobs) {
 
 }  // namespace
 
+// Row-match jobs of one image width: launched on their own, the LDS row
+// buffer is sized for that width (narrow rows then keep more workgroups per
+// CU), with a grid and threads per row of their own.
+struct MatchGroup {
+  int cols = 0, knum = 0;
+  int grid = 0;        // 0: the batch-wide "match" grid
+  int threads = 256;
+  Table<MatchJob> tab;
+};
+
 struct Plan {
   psgc_ctx* ctx = nullptr;
   int n = 0;
   Arena arena;
   Table<DecodeJob> dec;
   Table<MatchJob> mat;
+  std::vector<MatchGroup> match_groups;  // by width, when the batch has several
+  bool match_grouped = false;            // one launch per width instead of one
   Table<CloudJob> med, rg, sor;
   Table<SorStatJob> sst;             // SOR statistics, one job per pair
   // MLS jobs by iteration and window class (one launch each)
@@ -200,6 +213,7 @@ int upload_tables(Plan* pl) {
     PSGC_HIP(hipMemcpy(b.dev, b.host.data(), b.host.size(), hipMemcpyHostToDevice));
   PSGC_TRY(put(pl->dec));
   PSGC_TRY(put(pl->mat));
+  for (MatchGroup& mg : pl->match_groups) PSGC_TRY(put(mg.tab));
   PSGC_TRY(put(pl->med));
   PSGC_TRY(put(pl->rg));
   PSGC_TRY(put(pl->sor));
@@ -217,6 +231,7 @@ size_t table_bytes(const Plan* pl) {
   };
   add(pl->dec);
   add(pl->mat);
+  for (const MatchGroup& mg : pl->match_groups) add(mg.tab);
   add(pl->med);
   add(pl->rg);
   add(pl->sor);
@@ -224,6 +239,32 @@ size_t table_bytes(const Plan* pl) {
   for (auto& it : pl->mls)
     for (auto& t : it) add(t);
   return b;
+}
+
+// Splits the row-match jobs by image width (each group's rows renumbered
+// from 0); a batch of one width keeps the single table only.
+void group_matches(Plan* pl) {
+  pl->match_groups.clear();
+  std::map<int, size_t> at;
+  for (const MatchJob& j : pl->mat.host)
+    if (!at.count(j.cols)) {
+      at[j.cols] = pl->match_groups.size();
+      MatchGroup mg;
+      mg.cols = j.cols;
+      pl->match_groups.push_back(mg);
+    }
+  if (pl->match_groups.size() < 2) {
+    pl->match_groups.clear();
+    pl->match_grouped = false;
+    return;
+  }
+  for (MatchJob j : pl->mat.host) {
+    MatchGroup& mg = pl->match_groups[at[j.cols]];
+    j.tile_base = mg.tab.total;
+    mg.tab.total += j.rows;
+    mg.knum = std::max(mg.knum, j.pha_f);
+    mg.tab.host.push_back(j);
+  }
 }
 
 int default_grid(psgc_ctx* ctx, int per_cu) {
@@ -542,9 +583,15 @@ static int run_plan(Plan* pl, hipStream_t s) {
                              pl->gray_in_k[c], pl->gray_out_k[c], pl->gray_valid[c], s));
   for (const HdrSmooth& h : pl->hdr_smooth)
     PSGC_TRY(launch_post_smooth_valid(h.in, h.valid, h.out, h.rows, h.cols, s));
-  if (!pl->mat.host.empty())
+  if (pl->match_grouped) {
+    for (const MatchGroup& mg : pl->match_groups)
+      PSGC_TRY(launch_match_tiled(mg.tab.dev, (int)mg.tab.host.size(), mg.tab.total,
+                                  mg.grid ? mg.grid : g("match", 64), mg.cols, mg.knum,
+                                  mg.threads, s));
+  } else if (!pl->mat.host.empty()) {
     PSGC_TRY(launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
                                 g("match", 64), pl->max_cols, pl->max_knum, pl->match_threads, s));
+  }
   for (const PlainMatch& m : pl->plain_match) {
     const double* lp = m.lp_in;
     const double* rp = m.rp_in;
@@ -626,6 +673,19 @@ static int autotune(Plan* pl, hipStream_t s) {
   hipEvent_t a, b;
   PSGC_HIP(hipEventCreate(&a));
   PSGC_HIP(hipEventCreate(&b));
+  // milliseconds of the second of two runs of `launch` (the first warms caches)
+  auto time2 = [&](auto&& launch, float* ms) -> int {
+    float t = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      PSGC_HIP(hipEventRecord(a, s));
+      PSGC_TRY(launch());
+      PSGC_HIP(hipEventRecord(b, s));
+      PSGC_HIP(hipEventSynchronize(b));
+      PSGC_HIP(hipEventElapsedTime(&t, a, b));
+    }
+    *ms = t;
+    return PSGC_OK;
+  };
   struct St {
     const char* name;
     bool on;
@@ -698,6 +758,40 @@ static int autotune(Plan* pl, hipStream_t s) {
       pl->grid["mls_variant"] = best_shape;
     }
   }
+  // per image width: each width's rows on their own, over the same grid and
+  // threads-per-row candidates; the widths' launches replace the single one
+  // when, run back to back, they beat it
+  if (!pl->match_groups.empty()) {
+    for (MatchGroup& mg : pl->match_groups) {
+      float best = 1e30f;
+      for (int threads : {256, 128, 512})
+        for (int pc : cands) {
+          float ms = 0;
+          PSGC_TRY(time2([&]() -> int {
+            return launch_match_tiled(mg.tab.dev, (int)mg.tab.host.size(), mg.tab.total,
+                                      cu * pc, mg.cols, mg.knum, threads, s);
+          }, &ms));
+          if (ms < best) {
+            best = ms;
+            mg.grid = cu * pc;
+            mg.threads = threads;
+          }
+        }
+    }
+    float single = 0, grouped = 0;
+    PSGC_TRY(time2([&]() -> int {
+      return launch_match_tiled(pl->mat.dev, (int)pl->mat.host.size(), pl->mat.total,
+                                pl->grid["match"], pl->max_cols, pl->max_knum,
+                                pl->match_threads, s);
+    }, &single));
+    PSGC_TRY(time2([&]() -> int {
+      for (const MatchGroup& mg : pl->match_groups)
+        PSGC_TRY(launch_match_tiled(mg.tab.dev, (int)mg.tab.host.size(), mg.tab.total, mg.grid,
+                                    mg.cols, mg.knum, mg.threads, s));
+      return PSGC_OK;
+    }, &grouped));
+    pl->match_grouped = grouped < single;
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
   // the MLS tuning pass overwrote iteration 0's output only, which every
@@ -712,6 +806,7 @@ static int build_plan(Plan* pl, Build&& build) {
   dry.ctx = pl->ctx;
   dry.arena.base = nullptr;
   PSGC_TRY(build(&dry));
+  group_matches(&dry);
   const size_t bytes = dry.arena.used + table_bytes(&dry) + 4 * kAlign;
   if (hipMalloc(&pl->arena.base, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -720,6 +815,7 @@ static int build_plan(Plan* pl, Build&& build) {
   }
   pl->arena.size = bytes;
   PSGC_TRY(build(pl));
+  group_matches(pl);
   return upload_tables(pl);
 }
 
@@ -913,6 +1009,17 @@ int psgc_batch_destroy(psgc_batch* b) {
   return PSGC_OK;
 }
 
+// The row-match group named "match@<width>" or "match_threads@<width>".
+static MatchGroup* find_group(Plan* pl, const std::string& st) {
+  const size_t at = st.find('@');
+  char* end = nullptr;
+  const long w = std::strtol(st.c_str() + at + 1, &end, 10);
+  if (end == st.c_str() + at + 1 || *end != 0) return nullptr;
+  for (MatchGroup& mg : pl->match_groups)
+    if (mg.cols == w) return &mg;
+  return nullptr;
+}
+
 int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
   if (!b || !stage || !grid) PSGC_FAIL(PSGC_E_INVALID, "null argument");
   const std::string st = stage;
@@ -926,6 +1033,16 @@ int psgc_batch_grid(psgc_batch* b, const char* stage, int* grid) {
   }
   if (st == "match_threads") {
     *grid = b->plan.match_threads;
+    return PSGC_OK;
+  }
+  if (st == "match_groups") {
+    *grid = b->plan.match_grouped ? (int)b->plan.match_groups.size() : 0;
+    return PSGC_OK;
+  }
+  if (st.rfind("match@", 0) == 0 || st.rfind("match_threads@", 0) == 0) {
+    MatchGroup* mg = find_group(&b->plan, st);
+    if (!mg) PSGC_FAIL(PSGC_E_INVALID, "no row-match group '%s'", stage);
+    *grid = st[5] == '@' ? mg->grid : mg->threads;
     return PSGC_OK;
   }
   auto it = b->plan.grid.find(st);
@@ -950,6 +1067,23 @@ int psgc_batch_set_grid(psgc_batch* b, const char* stage, int value) {
       PSGC_FAIL(PSGC_E_INVALID, "match_threads must be 128, 256 or 512 (got %d)", value);
     b->plan.match_threads = value;
     b->plan.grid["match_threads"] = value;
+  } else if (st == "match_groups") {
+    if (value != 0 && value != 1) PSGC_FAIL(PSGC_E_INVALID, "match_groups must be 0 or 1");
+    if (value && b->plan.match_groups.empty())
+      PSGC_FAIL(PSGC_E_INVALID, "the batch has rows of one width only");
+    b->plan.match_grouped = value != 0;
+  } else if (st.rfind("match@", 0) == 0 || st.rfind("match_threads@", 0) == 0) {
+    MatchGroup* mg = find_group(&b->plan, st);
+    if (!mg) PSGC_FAIL(PSGC_E_INVALID, "no row-match group '%s'", stage);
+    if (st[5] == '@') {
+      if (value < 1 || value > (1 << 20))
+        PSGC_FAIL(PSGC_E_INVALID, "grid of %d workgroups for '%s' (1..2^20)", value, stage);
+      mg->grid = value;
+    } else {
+      if (value != 128 && value != 256 && value != 512)
+        PSGC_FAIL(PSGC_E_INVALID, "%s must be 128, 256 or 512 (got %d)", stage, value);
+      mg->threads = value;
+    }
   } else if (st == "mls_variant") {
     if (value != kMlsPlain && value != kMlsCached)
       PSGC_FAIL(PSGC_E_INVALID, "mls_variant must be 0 or 1 (got %d)", value);

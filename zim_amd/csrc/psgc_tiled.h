@@ -124,16 +124,21 @@ int launch_sor_knn_tiled(const CloudJob* jobs, int njobs, int64_t total, int gri
                          int min_k, int max_k, hipStream_t s);
 // MLS kernel variants: kMlsPlain evaluates each weight in all three passes;
 // kMlsCached keeps the first in-radius weights of each point in LDS between
-// the passes (the rest are evaluated again). Results are identical. Jobs are
-// launched by window class: half windows up to 6, 8 and 10 (the class fixes
-// the LDS tile, so small windows do not pay for the largest halo).
+// the passes (the rest are evaluated again; classes above half window 10
+// have no cache). Results are identical. Jobs are launched by window class:
+// half windows up to 6, 8, 10, 12 and 14 (the class fixes the LDS tile, so
+// small windows do not pay for the largest halo).
 enum { kMlsPlain = 0, kMlsCached = 1 };
 // Largest half windows the tiled SOR and MLS kernels stage in LDS.
 constexpr int kSorTiledMaxHalf = 8;
-constexpr int kMlsTiledMaxHalf = 10;
-constexpr int kMlsClasses = 3;
-constexpr int kMlsClassHalf[kMlsClasses] = {6, 8, 10};
-inline int mls_class(int hw) { return hw <= 6 ? 0 : (hw <= 8 ? 1 : 2); }
+constexpr int kMlsTiledMaxHalf = 14;
+constexpr int kMlsClasses = 5;
+constexpr int kMlsClassHalf[kMlsClasses] = {6, 8, 10, 12, 14};
+inline int mls_class(int hw) {
+  int c = 0;
+  while (c < kMlsClasses - 1 && hw > kMlsClassHalf[c]) ++c;
+  return c;
+}
 // Semantics present among the MLS jobs of a launch (a bit each): CPU
 // semantics accumulate with rounded products and sums, GPU semantics with
 // fused multiply-adds.

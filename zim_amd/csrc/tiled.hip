@@ -643,9 +643,11 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
 static_assert(kMlsClassHalf[kMlsClasses - 1] == kMlsTiledMaxHalf, "classes cover the limit");
 // Weight-cache entries per thread of each window class: as many as keep four
 // workgroups per CU resident beside the class's LDS tile (tile + cache about
-// 37 KB), the occupancy the registers allow.
+// 37 KB), the occupancy the registers allow; none above half window 10.
 template <int HM>
-constexpr int mls_cache_entries() { return HM <= 6 ? 24 : (HM <= 8 ? 20 : 16); }
+constexpr int mls_cache_entries() {
+  return HM <= 6 ? 24 : (HM <= 8 ? 20 : (HM <= 10 ? 16 : 0));
+}
 
 // Visits the in-radius entries of window rows [dy0, dy1) (bit masks per row)
 // in order, with their weights: from the cache (entries kc, kc + 1, ... of
@@ -918,14 +920,15 @@ template <int HM>
 int launch_mls_class(const CloudJob* jobs, int njobs, int64_t total, int g, int variant, int sems,
                      hipStream_t s) {
   constexpr int KC = mls_cache_entries<HM>();
+  const bool cached = KC > 0 && variant == kMlsCached;
   // one launch per semantics present (each skips the other's tiles)
   if (sems & kSemBitCpu) {
-    if (variant == kMlsCached) mls_tiled_kernel<HM, KC, false><<<g, 256, 0, s>>>(jobs, njobs, total);
+    if (cached) mls_tiled_kernel<HM, KC, false><<<g, 256, 0, s>>>(jobs, njobs, total);
     else mls_tiled_kernel<HM, 0, false><<<g, 256, 0, s>>>(jobs, njobs, total);
     PSGC_LAUNCHED();
   }
   if (sems & kSemBitGpu) {
-    if (variant == kMlsCached) mls_tiled_kernel<HM, KC, true><<<g, 256, 0, s>>>(jobs, njobs, total);
+    if (cached) mls_tiled_kernel<HM, KC, true><<<g, 256, 0, s>>>(jobs, njobs, total);
     else mls_tiled_kernel<HM, 0, true><<<g, 256, 0, s>>>(jobs, njobs, total);
     PSGC_LAUNCHED();
   }
@@ -942,9 +945,14 @@ int launch_mls_tiled(const CloudJob* jobs, int njobs, int64_t total, int grid, i
     PSGC_FAIL(PSGC_E_INVALID, "MLS variant %d", variant);
   PSGC_PROF_BEGIN("mls", s);
   const int g = round_grid(total, grid);
-  if (cls == 0) PSGC_TRY(launch_mls_class<kMlsClassHalf[0]>(jobs, njobs, total, g, variant, sems, s));
-  else if (cls == 1) PSGC_TRY(launch_mls_class<kMlsClassHalf[1]>(jobs, njobs, total, g, variant, sems, s));
-  else PSGC_TRY(launch_mls_class<kMlsClassHalf[2]>(jobs, njobs, total, g, variant, sems, s));
+  switch (cls) {
+    case 0: PSGC_TRY(launch_mls_class<kMlsClassHalf[0]>(jobs, njobs, total, g, variant, sems, s)); break;
+    case 1: PSGC_TRY(launch_mls_class<kMlsClassHalf[1]>(jobs, njobs, total, g, variant, sems, s)); break;
+    case 2: PSGC_TRY(launch_mls_class<kMlsClassHalf[2]>(jobs, njobs, total, g, variant, sems, s)); break;
+    case 3: PSGC_TRY(launch_mls_class<kMlsClassHalf[3]>(jobs, njobs, total, g, variant, sems, s)); break;
+    default: PSGC_TRY(launch_mls_class<kMlsClassHalf[4]>(jobs, njobs, total, g, variant, sems, s));
+  }
+  static_assert(kMlsClasses == 5, "one case per class");
   PSGC_PROF_END("mls", s);
   return PSGC_OK;
 }

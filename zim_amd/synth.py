@@ -264,6 +264,28 @@ def render_c1(seed: int, device="cpu", rows=480, cols=640, with_gray=True) -> Pa
     return pair
 
 
+def render_c3(seed: int, device="cpu", rows=480, cols=640, steps=4, gray=False) -> Pair:
+    """Uneven-batch pair: the config-1 scene (plane tilted 20 deg at 600 mm
+    plus 3 spheres, f = 800 px per 640 columns, B = 100 mm) at any
+    resolution, uint16 12-bit (ambient 400, modulation 750, Poisson +
+    N(0, 4 DN), 2 % low-modulation pixels), with `steps`-step fringes:
+    three-frequency 70/64/59 across a 1280-px projector, or (gray) period
+    16 px on an 800-px projector plus the 6-bit Gray code with complements.
+    Both projectors span the same field (focal 700 px per 800 projector
+    columns at 640 camera columns)."""
+    f = 800.0 * cols / 640.0
+    cam = Camera(rows, cols, f, 100.0)
+    z = depth_plane_spheres(cam, seed, device)
+    common = dict(ambient=400.0, modulation=750.0, noise=4.0, poisson=True, max_val=4095,
+                  dtype=torch.uint16, lowmod_frac=0.02)
+    if gray:
+        proj = Projector(800, 700.0 * cols / 640.0)
+        return _render(cam, proj, z, seed, device, (800 / 16.0,), (steps,), gray=(16.0, 6),
+                       **common)
+    proj = Projector(1280, 1120.0 * cols / 640.0)
+    return _render(cam, proj, z, seed, device, (70, 64, 59), (steps,) * 3, **common)
+
+
 def _render(cam, proj, z, seed, device, freqs, steps, ambient, modulation,
             noise, poisson, max_val, dtype, lowmod_frac=0.0, gray=None) -> Pair:
     gen = torch.Generator(device=device).manual_seed(seed + 1000)
