@@ -1,8 +1,9 @@
 """Unorganized-cloud filters on the device (voxel grid, LDS neighbour
 tiles) against the CPU restatement: SOR masks and mean distances
 bit-exact, MLS and Gaussian within 1e-3 mm (bit-identical in practice: the
-same candidate order and arithmetic), on random clouds, on a merged cloud
-of several noisy views of one C2 scene, and on edge cases."""
+same candidate order and arithmetic), on random clouds, on merged clouds
+(noisy views of one C2 scene; ring views of the plane + spheres and boxes
+scene), and on edge cases."""
 import numpy as np
 import pytest
 import torch
@@ -139,6 +140,50 @@ def test_merged_views_filters(ctx):
         assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL, voxel
         ref = oracle.cloud_gaussian(xyz, None, 0.5, 1.0, voxel)
         got = zim_amd.cloud_gaussian(xyz, None, 0.5, 1.0, voxel, ctx)
+        assert np.abs(got - ref).max() <= XYZ_TOL, voxel
+
+
+def _multi_object_views(n_views=4, rows=240, cols=320):
+    """A small instance of the merged-cloud workload: views of the seeded
+    plane + 12 spheres and boxes scene from ring poses (the full workload's
+    geometry, f scaled with the width), reconstructed on the GPU without
+    filters (FP64 triangulation), moved into the world frame and merged.
+    Returns the cloud and, per point, whether the ray hit the plane."""
+    dcfg = scenes.c2_decode_config()
+    rcfg = scenes.c2_recon_config(use_mls=0, outlier_mode=0)
+    c = zim_amd.Context(0)
+    pts, onplane = [], []
+    for v in range(n_views):
+        p, R, C, obj = zim_amd.synth.render_c5_view(7, v * 4, device="cuda", rows=rows, cols=cols,
+                                                    f=2400.0 * cols / 2448.0)
+        L = zim_amd.decode(dcfg, p.left, c)
+        Rr = zim_amd.decode(dcfg, p.right, c)
+        xyz, P, _, _ = zim_amd.reconstruct(rcfg, L[0], Rr[0], L[1], Rr[1], 70, p.Q, p.R1, c)
+        m = P != 0
+        pts.append(zim_amd.synth.to_world(xyz[m], R, C).cpu().numpy())
+        onplane.append((obj[m] == 0).cpu().numpy())
+    return np.concatenate(pts).astype(np.float32), np.concatenate(onplane)
+
+
+def test_multi_object_scene_merged_filters(ctx):
+    """Merged views of the multi-object scene: the poses put the plane's
+    points at world z = 0 (scene truth), and SOR (k = 50), MLS (order 2) and
+    the Gaussian over the merged cloud equal the oracle's. Radii scale with
+    this instance's coarser point spacing (about 2 mm)."""
+    xyz, onplane = _multi_object_views()
+    assert len(xyz) > 100_000 and 0.3 < onplane.mean() < 0.95
+    assert np.median(np.abs(xyz[onplane, 2])) < 1.0
+    ref, got = _sor_both(ctx, xyz, None, 50, 4.0, 4)
+    _assert_sor(ref, got)
+    keep = ref[0]
+    for voxel in (5.0, 2.5):
+        ref = oracle.cloud_mls(xyz, keep, 2.5, 2, 5.0, 1, 1, voxel)
+        got = zim_amd.cloud_mls(torch.from_numpy(xyz).cuda(), torch.from_numpy(keep).cuda(), 2.5,
+                                2, 5.0, 1, 1, voxel, ctx)
+        torch.cuda.synchronize()
+        assert np.abs(got.cpu().numpy() - ref).max() <= XYZ_TOL, voxel
+        ref = oracle.cloud_gaussian(xyz, keep, 2.5, 5.0, voxel)
+        got = zim_amd.cloud_gaussian(xyz, keep, 2.5, 5.0, voxel, ctx)
         assert np.abs(got - ref).max() <= XYZ_TOL, voxel
 
 

@@ -236,6 +236,125 @@ def render_c2(seed: int, device="cpu", rows=2048, cols=2448, f=2400.0,
                    max_val=4095, dtype=torch.uint16)
 
 
+@dataclasses.dataclass
+class Scene:
+    """World frame in mm, z up; the plane z = 0 carries every object."""
+    spheres: list   # (cx, cy, cz, r)
+    boxes: list     # (cx, cy, cz, hx, hy, hz, yaw)
+
+
+def c5_scene(seed: int, n_objects=12, extent=180.0) -> Scene:
+    """One plane plus n_objects resting on it, alternately spheres (radius
+    20-50 mm) and boxes (half extents 15-40 mm, yaw 0-90 deg), centres
+    uniform over +-extent mm in x and y, seeded."""
+    g = torch.Generator().manual_seed(seed)
+    u = lambda: torch.rand(1, generator=g, dtype=torch.float64).item()  # noqa: E731
+    spheres, boxes = [], []
+    for i in range(n_objects):
+        x, y = (2 * u() - 1) * extent, (2 * u() - 1) * extent
+        if i % 2 == 0:
+            r = 20.0 + 30.0 * u()
+            spheres.append((x, y, r, r))
+        else:
+            hx, hy, hz = 15.0 + 25.0 * u(), 15.0 + 25.0 * u(), 15.0 + 25.0 * u()
+            boxes.append((x, y, hz, hx, hy, hz, u() * math.pi / 2))
+    return Scene(spheres, boxes)
+
+
+def c5_pose(seed: int, view: int, n_views=16):
+    """Left-camera pose of one view: on a ring around the scene (azimuth
+    360 view / n_views deg, +-5 deg), elevation 55-75 deg, 650-800 mm from a
+    target within 30 mm of the origin, looking at it (OpenCV axes: x right, y
+    down, z forward). Returns (R, C): R maps world directions to the camera
+    frame, C is the camera centre (FP64 tensors)."""
+    g = torch.Generator().manual_seed(seed * 1000 + view)
+    u = lambda: torch.rand(1, generator=g, dtype=torch.float64).item()  # noqa: E731
+    az = 2 * math.pi * view / n_views + math.radians(10.0 * u() - 5.0)
+    el = math.radians(55.0 + 20.0 * u())
+    dist = 650.0 + 150.0 * u()
+    tgt = torch.tensor([(2 * u() - 1) * 30.0, (2 * u() - 1) * 30.0, 0.0], dtype=torch.float64)
+    C = tgt + dist * torch.tensor([math.cos(el) * math.cos(az), math.cos(el) * math.sin(az),
+                                   math.sin(el)], dtype=torch.float64)
+    fwd = (tgt - C) / torch.linalg.norm(tgt - C)
+    right = torch.linalg.cross(fwd, torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64))
+    right = right / torch.linalg.norm(right)
+    down = torch.linalg.cross(fwd, right)
+    return torch.stack([right, down, fwd]), C
+
+
+def raycast(scene: Scene, R, C, cam: Camera, device="cpu"):
+    """Camera-frame depth (Z) of the nearest surface seen by each left pixel
+    (inf where none) and the index of what it hit (0 the plane, 1.. the
+    spheres then the boxes, -1 nothing)."""
+    ii = torch.arange(cam.rows, dtype=torch.float64, device=device).view(-1, 1)
+    jj = torch.arange(cam.cols, dtype=torch.float64, device=device).view(1, -1)
+    dc = torch.stack(torch.broadcast_tensors((jj - cam.cx) / cam.f, (ii - cam.cy) / cam.f,
+                                             torch.ones_like(ii * jj)), -1)
+    Rd, Cd = R.to(device), C.to(device)
+    D = dc @ Rd       # world direction per pixel (R^T d); its camera Z is 1
+    best = torch.full((cam.rows, cam.cols), math.inf, dtype=torch.float64, device=device)
+    obj = torch.full((cam.rows, cam.cols), -1, dtype=torch.int32, device=device)
+
+    def take(t, idx):
+        nonlocal best, obj
+        hit = torch.isfinite(t) & (t > 0) & (t < best)
+        best = torch.where(hit, t, best)
+        obj = torch.where(hit, torch.full_like(obj, idx), obj)
+
+    take(-Cd[2] / D[..., 2], 0)
+    k = 1
+    for (x, y, z, r) in scene.spheres:
+        o = Cd - torch.tensor([x, y, z], dtype=torch.float64, device=device)
+        a = (D * D).sum(-1)
+        b = 2 * (D * o).sum(-1)
+        c = (o * o).sum() - r * r
+        disc = b * b - 4 * a * c
+        t = torch.where(disc >= 0, (-b - torch.sqrt(disc.clamp(min=0))) / (2 * a),
+                        torch.full_like(a, math.inf))
+        take(t, k)
+        k += 1
+    for (x, y, z, hx, hy, hz, yaw) in scene.boxes:
+        cy_, sy_ = math.cos(yaw), math.sin(yaw)
+        Rb = torch.tensor([[cy_, sy_, 0.0], [-sy_, cy_, 0.0], [0.0, 0.0, 1.0]],
+                          dtype=torch.float64, device=device)   # world -> box
+        o = Rb @ (Cd - torch.tensor([x, y, z], dtype=torch.float64, device=device))
+        d = D @ Rb.T
+        h = torch.tensor([hx, hy, hz], dtype=torch.float64, device=device)
+        inv = 1.0 / d
+        t1, t2 = (-h - o) * inv, (h - o) * inv
+        tmin = torch.minimum(t1, t2).amax(-1)
+        tmax = torch.maximum(t1, t2).amin(-1)
+        t = torch.where((tmax >= tmin) & (tmin > 0), tmin, torch.full_like(tmin, math.inf))
+        take(t, k)
+        k += 1
+    return best, obj
+
+
+def render_c5_view(scene_seed: int, view: int, device="cpu", rows=2048, cols=2448,
+                   f=2400.0, baseline=150.0, n_views=16):
+    """One view of the multi-object scene (c5_scene) from c5_pose: the
+    rectified pair rendered as the full-size workload renders its own (70/64/59
+    4-step fringes on a 1280-px projector between the cameras, uint16 12-bit,
+    ambient 400, modulation 750, Poisson + N(0, 4 DN)). Returns the pair,
+    the pose (R, C) and the object index per left pixel."""
+    cam = Camera(rows, cols, f, baseline)
+    proj = Projector(1280, 1000.0 * f / 2400.0)
+    R, C = c5_pose(scene_seed, view, n_views)
+    z, obj = raycast(c5_scene(scene_seed), R, C, cam, device)
+    pair = _render(cam, proj, z, scene_seed * 100 + view, device, (70, 64, 59), (4, 4, 4),
+                   ambient=400.0, modulation=750.0, noise=4.0, poisson=True, max_val=4095,
+                   dtype=torch.uint16)
+    return pair, R, C, obj
+
+
+def to_world(xyz, R, C):
+    """Camera-frame points (..., 3) of a view into the scene's world frame:
+    R^T x + C (FP64 pose, FP32 points out)."""
+    R = R.to(xyz.device)
+    Cd = C.to(xyz.device)
+    return (xyz.double() @ R + Cd).float()
+
+
 def render_ref_timing(seed: int, device="cpu", size=5120) -> Pair:
     """The configuration of the reference's published timing
     (JSLAB_Dual_GPU移植说明.md:128-133): a size x size 8-bit pair, three
