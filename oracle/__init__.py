@@ -39,6 +39,7 @@ def lib():
                                       vp, vp]
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
         L.orc_remap.argtypes = [i32, vp, i32, i32, vp, vp, i32, i32, vp]
+        L.orc_voxel_cells.argtypes = [vp, i64, flt, flt, vp]
         L.orc_reproject_poly.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
         L.orc_roi_right_window.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
         L.orc_reconstruct_poly_roi.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp,
@@ -320,6 +321,14 @@ def reconstruct_poly_roi(cfg: ReconConfig, lp, rp, lk, rk, pha_f, left_rect, rig
                                    pha_f, _p(lr), _p(rr), _p(mask), _p(cx), _p(cy), _p(cz),
                                    _p(R1), _p(xyz), _p(P), _p(disp))
     return xyz, P, disp
+
+
+def voxel_cells(v, o, voxel):
+    """The voxel frame's cell of each coordinate in v (origin o, edge voxel)."""
+    v = _c(v, np.float32)
+    out = np.zeros(v.shape, np.int64)
+    assert lib().orc_voxel_cells(_p(v), v.size, float(o), float(voxel), _p(out)) == 0
+    return out
 
 
 def remap(src, map_x, map_y, nearest=False):

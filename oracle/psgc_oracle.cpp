@@ -1552,7 +1552,7 @@ int orc_cloud_sor(const float* xyz, const uint8_t* valid, int64_t n, int k, floa
 int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma, int order,
                   float max_radius, int iterations, int semantics, float voxel, float* out) {
   const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
-  const int R = psgc::cloud_radius_cells(voxel, r);
+  const int R = psgc::cloud_radius_cells(&voxel, r);
   if (n < 0 || !(sigma > 0.0f) || R == 0) return PSGC_E_INVALID;
   const float r2 = r * r;
   const float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
@@ -1582,7 +1582,7 @@ int orc_cloud_mls(const float* xyz, const uint8_t* valid, int64_t n, float sigma
 int orc_cloud_gaussian(const float* xyz, const uint8_t* valid, int64_t n, float sigma,
                        float radius, float voxel, float* out) {
   if (n < 0 || !(sigma > 0.0f) || !(radius > 0.0f)) return PSGC_E_INVALID;
-  const int R = psgc::cloud_radius_cells(voxel, radius);
+  const int R = psgc::cloud_radius_cells(&voxel, radius);
   if (R == 0) return PSGC_E_INVALID;
   CloudGrid g;
   const int rc = build_grid(xyz, valid, n, voxel, &g);
@@ -1633,6 +1633,16 @@ int orc_cloud_brute(const float* xyz, const uint8_t* valid, int64_t n, int k, fl
     for (int m = 0; m < k; ++m) knn_d2[i * k + m] = m < (int)d2s.size() ? d2s[m] : INFINITY;
     within[i] = cnt;
   }
+  return PSGC_OK;
+}
+
+// Cell coordinates of the shared voxel frame (psgc_voxel.h) along one axis:
+// out[i] = the cell of coordinate v[i] for origin o and edge voxel, as the
+// grid builds of the device and of this restatement assign it.
+int orc_voxel_cells(const float* v, int64_t n, float o, float voxel, int64_t* out) {
+  if (!(voxel > 0.0f)) return PSGC_E_INVALID;
+  const double inv = 1.0 / (double)voxel;
+  for (int64_t i = 0; i < n; ++i) out[i] = psgc::voxel_coord(v[i], o, inv);
   return PSGC_OK;
 }
 

@@ -112,3 +112,74 @@ def test_cloud_voxel_below_radius_is_rejected():
     oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.15)
     with pytest.raises(AssertionError):
         oracle.cloud_gaussian(xyz, None, 0.1, 0.3, 0.14)
+
+
+def _exact_cells(v, o, voxel):
+    """floor((v - o) / voxel) in exact rational arithmetic, and whether the
+    quotient sits within 1e-9 cells of an integer (a tie the pin skips)."""
+    from fractions import Fraction
+    fo, fv = Fraction(float(np.float32(o))), Fraction(float(np.float32(voxel)))
+    cells, near = [], []
+    for x in np.asarray(v, np.float32):
+        q = (Fraction(float(x)) - fo) / fv
+        c = q.numerator // q.denominator
+        cells.append(c)
+        near.append(min(q - c, c + 1 - q) < Fraction(1, 10 ** 9))
+    return np.array(cells, np.int64), np.array(near)
+
+
+@pytest.mark.parametrize("voxel", [0.3, 0.7, 1.1])
+def test_voxel_cells_are_exact_across_a_wide_grid(voxel):
+    """Cell assignment of the shared voxel frame across about 2000 cells: the
+    exact floor((v - o) / voxel) for random coordinates and for coordinates
+    within a few ulps of cell boundaries. The same coordinates assigned in
+    float, (v - o) * (1 / voxel) rounded at each step, miss by a cell on
+    some of them: the data would catch that."""
+    g = np.random.default_rng(int(voxel * 10))
+    o = np.float32(-2.0) * np.float32(voxel)
+    v = g.uniform(0, 2000 * voxel, 4000).astype(np.float32)
+    bnd = (o.astype(np.float64) + voxel * np.arange(900, 2000, 7)).astype(np.float32)
+    steps = np.arange(-6, 7, dtype=np.float32)
+    v = np.concatenate([v, (bnd[:, None] + steps[None, :] * np.spacing(bnd)[:, None]).ravel()])
+    got = oracle.voxel_cells(v, o, voxel)
+    want, near = _exact_cells(v, o, voxel)
+    assert near.sum() < 5
+    assert np.array_equal(got[~near], want[~near])
+    in_float = np.floor((v - o) * (np.float32(1) / np.float32(voxel))).astype(np.int64)
+    assert (in_float[~near] != want[~near]).sum() > 0
+
+
+def test_filters_exact_on_a_cloud_a_thousand_voxels_wide():
+    """kNN (SOR) and radius (Gaussian) neighbour sets on a cloud spanning
+    about 1000 voxels, by brute force: strips of points across the whole
+    extent, so neighbours meet at cell boundaries far from the origin."""
+    g = np.random.default_rng(11)
+    voxel, radius = 0.3, 0.3
+    xs = g.uniform(0, 1000 * voxel, 3000)
+    ys = g.uniform(0, 1.5, 3000)
+    zs = 500 + g.normal(0, 0.05, 3000)
+    xyz = np.column_stack([xs, ys, zs]).astype(np.float32)
+    k = 8
+    keep, md, st = oracle.cloud_sor(xyz, None, k, 1.0, voxel, 4)
+    d2, within = oracle.cloud_brute(xyz, None, k, radius)
+    lim = np.float32(voxel * 4) ** 2 * np.float32(0.99999)
+    eligible = d2[:, k - 1] <= lim
+    assert eligible.sum() > 1000
+    ref = np.zeros(len(xyz))
+    for i in np.nonzero(eligible)[0]:
+        s = 0.0
+        for v in d2[i]:
+            s += np.sqrt(np.float64(v))
+        ref[i] = s / k
+    assert np.array_equal(md, ref)
+    # Gaussian: the weighted mean over exactly the brute-force radius set
+    got = oracle.cloud_gaussian(xyz, None, 0.15, radius, voxel)
+    x = xyz.astype(np.float64)
+    r2 = np.float32(radius) * np.float32(radius)
+    for i in range(0, len(xyz), 3):
+        dd = ((xyz - xyz[i]) ** 2).astype(np.float32)
+        m = (dd[:, 0] + dd[:, 1] + dd[:, 2]) <= r2
+        assert m.sum() == within[i]
+        w = np.exp(-((x[m] - x[i]) ** 2).sum(1) / (2 * 0.15 * 0.15))
+        ref_i = (w[:, None] * x[m]).sum(0) / w.sum()
+        assert np.abs(got[i] - ref_i).max() < 1e-6 * 500

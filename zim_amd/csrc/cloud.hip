@@ -682,7 +682,7 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
                   float* out, float* tmp) {
   const float r = (max_radius > 0.0f) ? max_radius : 3.0f * sigma;
   if (!(sigma > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma must be > 0");
-  const int R = cloud_radius_cells(voxel, r);
+  const int R = cloud_radius_cells(&voxel, r);
   if (R == 0) PSGC_FAIL(PSGC_E_INVALID, "voxel %g below half the radius %g", (double)voxel, (double)r);
   const float r2 = r * r, inv2s2 = 1.0f / (2.0f * sigma * sigma);
   const int ord = std::max(0, std::min(2, order));
@@ -728,7 +728,7 @@ int cloud_mls_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
 int cloud_gaussian_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t n,
                        float sigma, float radius, float voxel, float* out) {
   if (!(sigma > 0.0f) || !(radius > 0.0f)) PSGC_FAIL(PSGC_E_INVALID, "sigma and radius must be > 0");
-  const int R = cloud_radius_cells(voxel, radius);
+  const int R = cloud_radius_cells(&voxel, radius);
   if (R == 0)
     PSGC_FAIL(PSGC_E_INVALID, "voxel %g below half the radius %g", (double)voxel, (double)radius);
   CloudGridDev g;

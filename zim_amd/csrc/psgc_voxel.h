@@ -18,15 +18,23 @@ namespace psgc {
 // ring of neighbours inside the frame, and a block cell two away that falls
 // outside has a linear key (key + offset) of an empty margin cell or of no
 // cell at all; key = ix + X (iy + Y iz).
+// Cell coordinates are computed in double: the difference of two floats is
+// exact there (for any two coordinates within 2^29 of each other's
+// magnitude), and the one rounding of the product by 1 / voxel moves a
+// coordinate by at most 2^-52 of its value in cells, under 1e-9 cells for any
+// grid of up to 2^40 cells. A point therefore lies within 1e-9 cells of the
+// cell it is assigned to, which the two margins below rely on; in float the
+// error grows with the extent (about 1e-4 cells across 1000 cells) and would
+// exceed them.
 struct VoxelFrame {
   float ox, oy, oz;
-  float inv;     // 1 / voxel
+  double inv;    // 1 / voxel
   float voxel;
   int64_t X, Y, Z;
 };
 
-PSGC_VX_HD inline int64_t voxel_coord(float v, float o, float inv) {
-  return (int64_t)floorf((v - o) * inv);
+PSGC_VX_HD inline int64_t voxel_coord(float v, float o, double inv) {
+  return (int64_t)floor(((double)v - (double)o) * inv);
 }
 
 PSGC_VX_HD inline uint64_t voxel_key(const VoxelFrame& f, int64_t ix, int64_t iy, int64_t iz) {
@@ -43,7 +51,7 @@ PSGC_VX_HD inline uint64_t voxel_key_of(const VoxelFrame& f, float x, float y, f
 inline bool voxel_frame(const float mn[3], const float mx[3], float voxel, VoxelFrame* f) {
   if (!(voxel > 0.0f)) return false;
   f->voxel = voxel;
-  f->inv = 1.0f / voxel;
+  f->inv = 1.0 / (double)voxel;
   f->ox = mn[0] - 2.0f * voxel;
   f->oy = mn[1] - 2.0f * voxel;
   f->oz = mn[2] - 2.0f * voxel;
@@ -68,20 +76,35 @@ PSGC_VX_HD inline bool voxel_point_ok(float x, float y, float z) {
 }
 
 // k-nearest search over the (2R+1)^3 cells around a point: every point
-// outside that block lies at least R voxels away, so a k-th squared distance
-// at or below this limit (R voxels less a margin far above float rounding)
-// makes the k found the exact k nearest.
+// outside that block lies at least R voxels away (less 2e-9 voxels of cell
+// assignment error), so a k-th squared distance at or below this limit (R
+// voxels less a relative margin of 1e-5, far above that error and the float
+// rounding of the squared distances, about 4e-7) makes the k found the exact
+// k nearest.
 PSGC_VX_HD inline float cloud_knn_limit(float voxel, int R) {
   const float rs = voxel * (float)R;
   return rs * rs * 0.99999f;
 }
 
-// Cells of edge voxel around a point's own cell, per axis, that hold every
-// neighbour within radius r for the radius filters (MLS, Gaussian): 1 when
-// voxel >= r (a 3x3x3 block), 2 when 2 voxel >= r (5x5x5: half-radius cells
-// cover 15.6 r^3 instead of 27 r^3); 0 = voxel too small.
-PSGC_VX_HD inline int cloud_radius_cells(float voxel, float r) {
-  return voxel >= r ? 1 : (2.0f * voxel >= r ? 2 : 0);
+// Cells around a point's own cell, per axis, that hold every neighbour
+// within radius r for the radius filters (MLS, Gaussian): 1 when voxel >= r
+// (a 3x3x3 block), 2 when 2 voxel >= r (5x5x5: half-radius cells cover
+// 15.6 r^3 instead of 27 r^3); 0 = voxel too small. A neighbour passes the
+// float test d^2 <= r^2 up to r (1 + 2^-22) away and sits within 1e-9 cells
+// of its cell, so the cell edge used is at least r (1 + 2^-20) (or half of
+// it): *voxel is raised to that when below it (by under a millionth; the
+// neighbour sets do not depend on the cell edge).
+PSGC_VX_HD inline int cloud_radius_cells(float* voxel, float r) {
+  const float need = r + r * 0x1p-20f;
+  if (*voxel >= r) {
+    *voxel = *voxel >= need ? *voxel : need;
+    return 1;
+  }
+  if (2.0f * *voxel >= r) {
+    *voxel = 2.0f * *voxel >= need ? *voxel : 0.5f * need;
+    return 2;
+  }
+  return 0;
 }
 
 }  // namespace psgc
