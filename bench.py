@@ -85,13 +85,16 @@ ALG_BYTES_PER_PX = {
 MLS_RADIUS = 1.5  # mm
 
 
-# VALU lane-instructions per pixel of one launch (SQ_INSTS_VALU x 64 / pixels)
-# on this workload, from the rocprofv3 counter run summarized in
-# profiles/c2_8pair_valu_counters.md; with the live launch time they give the
-# kernel's VALU issue rate against the chip's single-issue peak (256 CUs x 4
-# SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
-VALU_PER_PX_PROFILED = {}
-VALU_PEAK_GINST_S = 256 * 4 * 2.4 / 4
+# VALU lane-instructions per left pixel of one pass over the 64 pairs
+# (SQ_INSTS_VALU x 64 / pixels; MLS: the window-class launches of one
+# iteration, per kernel variant) on this workload, from the rocprofv3 counter
+# run summarized in profiles/c2_64pair_valu_counters.md. With the live pass
+# time they give the wave-instruction rate, set against the rate at which
+# FP32 FMAs reach MI355X's published vector peak: 157.3 TFLOP/s / (2 flops x
+# 64 lanes) = 1229 G wave-instructions/s.
+VALU_PER_PX_PROFILED = {"mls/0": 12373, "mls/1": 12006, "decode": 1789, "sor_knn": 1764,
+                        "match": 573, "repro_gauss": 497, "median5": 271}
+VALU_PEAK_GINST_S = 157.3e3 / 2 / 64
 
 
 def main():
@@ -176,19 +179,24 @@ def main():
     lib.psgc_profile_enable(0)
     stage_ms = {k: round(v[1] / prof_steps, 3) for k, v in prof.items()}
     dom_name, (dom_n, dom_ms) = max(prof.items(), key=lambda kv: kv[1][1])
-    avg_ms = dom_ms / dom_n
-    # one launch of a batched family covers every pair (MLS: one iteration)
+    # one pass of a batched stage covers every pair once: one launch, except
+    # the MLS, whose iterations are a pass each, launched per window class
+    passes = prof_steps * (rcfgs[0].mls_iterations if dom_name == "mls" else 1)
+    avg_ms = dom_ms / passes
+    launches_per_pass = dom_n / passes
     alg = ALG_BYTES_PER_PX[dom_name] * npix * args.pairs * (2 if dom_name == "decode" else 1)
     achieved = alg / (avg_ms * 1e-3) / 1e9
     compute = None
-    if dom_name in VALU_PER_PX_PROFILED:
-        px = npix * args.pairs * (2 if dom_name == "decode" else 1)
-        gi = VALU_PER_PX_PROFILED[dom_name] / 64 * px / (avg_ms * 1e-3) / 1e9
-        compute = {"bound": "valu", "wave_insts_per_px": round(VALU_PER_PX_PROFILED[dom_name] / 64, 1),
-                   "achieved_ginst_s": round(gi, 1), "single_issue_peak_ginst_s": VALU_PEAK_GINST_S,
+    vkey = f"mls/{grids['mls_variant']}" if dom_name == "mls" else dom_name
+    if vkey in VALU_PER_PX_PROFILED and args.pairs == 64:
+        gi = VALU_PER_PX_PROFILED[vkey] / 64 * npix * args.pairs / (avg_ms * 1e-3) / 1e9
+        compute = {"bound": "valu", "wave_insts_per_px": round(VALU_PER_PX_PROFILED[vkey] / 64, 1),
+                   "achieved_ginst_s": round(gi, 1),
+                   "peak_ginst_s": round(VALU_PEAK_GINST_S, 1),
+                   "peak_from": "157.3 TFLOP/s FP32 vector / (2 flops per FMA x 64 lanes)",
                    "frac": round(gi / VALU_PEAK_GINST_S, 3),
-                   "insts_from": "rocprofv3 SQ_INSTS_VALU on this workload, "
-                                 "profiles/c2_8pair_valu_counters.md"}
+                   "insts_from": "rocprofv3 SQ_INSTS_VALU on this workload (profile-derived), "
+                                 "profiles/c2_64pair_valu_counters.md"}
     log(f"stage ms/step {stage_ms}")
 
     n_sub = args.plain_pairs if args.plain_pairs else args.pairs
@@ -303,7 +311,7 @@ def main():
                                                               0.99999)) if m.any() else 0.0}
             log(f"full-pair parity vs oracle: {parity}")
 
-    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # HBM bytes per pass of the dominant kernel, measured with rocprofv3
     # FETCH_SIZE / WRITE_SIZE on this workload (profiles/c2_64pair_traffic.*)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "c2_64pair_traffic.json")
@@ -312,8 +320,8 @@ def main():
             tj = json.load(fh)
         if (tj.get("workload") == {"pairs": args.pairs, "rows": ROWS, "cols": COLS,
                                    "mls_windows": "per pair"}
-                and dom_name in tj.get("kernels", {})):
-            traffic = int(tj["kernels"][dom_name]["traffic_bytes"])
+                and vkey in tj.get("kernels", {})):
+            traffic = int(tj["kernels"][vkey]["traffic_bytes"])
     # whole pipeline: frames of both cameras read once, xyz + mask written once
     e2e_bytes = (24 * 2 + 13) * npix * args.pairs
     e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
@@ -349,9 +357,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "traffic_from": ("rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                                      "profiles/c2_64pair_traffic.json") if traffic else None,
-                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": alg,
+                     "traffic_from": ("profile-derived: rocprofv3 FETCH_SIZE/WRITE_SIZE of "
+                                      "this workload, profiles/c2_64pair_traffic.json")
+                     if traffic else None,
+                     "ms_per_pass": round(avg_ms, 4), "launches_per_pass": launches_per_pass,
+                     "alg_bytes_per_pass": alg,
                      "end_to_end": {"alg_bytes_per_pt": 61, "achieved": round(e2e_gbs, 2),
                                     "frac": round(e2e_gbs / HBM_PEAK_GBS, 5)},
                      "compute": compute},
