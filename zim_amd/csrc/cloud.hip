@@ -237,6 +237,61 @@ struct Knn {
   }
 };
 
+// The NB smallest of a candidate stream (NB a power of two, 32 or 64),
+// ascending, taken in groups of 16 with fixed compare-exchange networks and
+// no data-dependent branches: a group is sorted (bitonic sort), its reverse
+// paired with the list's top 16 (min of each pair: with the list ascending
+// and the group descending the result is a bitonic sequence holding the NB
+// smallest of both), and the list re-sorted by a bitonic merge. The first K
+// <= NB entries are then exactly the K smallest seen, as a one-by-one
+// insertion keeps them (the same multiset, so the same ascending list), at a
+// cost that does not grow with how far each value travels. NaN candidates
+// enter as +inf.
+template <int NB>
+struct KnnNet {
+  static_assert(NB == 32 || NB == 64, "list of 32 or 64");
+  float best[NB];
+  __device__ void init() {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) best[i] = INFINITY;
+  }
+  __device__ static void cx(float& a, float& b) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = lo;
+    b = hi;
+  }
+  __device__ void merge16(float (&g)[16]) {
+#pragma unroll
+    for (int size = 2; size <= 16; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int j = i ^ stride;
+          if (j > i) {
+            if ((i & size) == 0) cx(g[i], g[j]);
+            else cx(g[j], g[i]);
+          }
+        }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) best[NB - 16 + i] = fminf(best[NB - 16 + i], g[15 - i]);
+#pragma unroll
+    for (int stride = NB >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int j = i ^ stride;
+        if (j > i) cx(best[i], best[j]);
+      }
+  }
+  __device__ double mean_dist(int k) const {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+      if (t < k) s += sqrt((double)best[t]);
+    return s / k;
+  }
+};
+
 // First pass: the 3x3x3 block. Points it cannot settle (fewer than k
 // candidates, or a k-th distance beyond one voxel) go to the retry list.
 // TB threads per workgroup (one cell at a time), chunks of chunk_of(TB).
@@ -264,24 +319,58 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
       const int32_t q = c0 + qb + threadIdx.x;
       const float4 qp = active ? pts[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
       const int qi = __float_as_int(qp.w);
-      Knn<K> sel;
-      sel.init();
-      for (int32_t cb = 0; cb < M; cb += CH) {
-        const int32_t len = min(CH, M - cb);
-        __syncthreads();  // the previous chunk's readers are done
-        stage_chunk<1, TB>(pts, &bc, cb, len, tile);
-        __syncthreads();
-        if (active)
-          for (int32_t j = 0; j < len; ++j) {
-            const float4 p = tile[j];
-            if (__float_as_int(p.w) == qi) continue;
-            const float pk[3] = {p.x, p.y, p.z};
-            sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
+      if constexpr (K > 0) {
+        // groups of 16 candidates, merged when one would enter some lane's
+        // K nearest (uniform per wave)
+        KnnNet<(K <= 32 ? 32 : 64)> sel;
+        sel.init();
+        for (int32_t cb = 0; cb < M; cb += CH) {
+          const int32_t len = min(CH, M - cb);
+          __syncthreads();  // the previous chunk's readers are done
+          stage_chunk<1, TB>(pts, &bc, cb, len, tile);
+          __syncthreads();
+          for (int32_t j0 = 0; j0 < len; j0 += 16) {
+            float g[16];
+            float gmin = INFINITY;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+              float d = INFINITY;
+              if (j0 + u < len) {
+                const float4 p = tile[j0 + u];
+                const float pk[3] = {p.x, p.y, p.z};
+                const float e = dist2(qp.x, qp.y, qp.z, pk);
+                if (__float_as_int(p.w) != qi && e == e) d = e;
+              }
+              g[u] = d;
+              gmin = fminf(gmin, d);
+            }
+            if (__any(active && gmin < sel.best[K - 1])) sel.merge16(g);
           }
-      }
-      if (active) {
-        if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
-        else st_global(retry, atomicAdd(nretry, 1), q);
+        }
+        if (active) {
+          if (sel.best[K - 1] <= lim) st_global(md, qi, sel.mean_dist(K));
+          else st_global(retry, atomicAdd(nretry, 1), q);
+        }
+      } else {
+        Knn<K> sel;
+        sel.init();
+        for (int32_t cb = 0; cb < M; cb += CH) {
+          const int32_t len = min(CH, M - cb);
+          __syncthreads();  // the previous chunk's readers are done
+          stage_chunk<1, TB>(pts, &bc, cb, len, tile);
+          __syncthreads();
+          if (active)
+            for (int32_t j = 0; j < len; ++j) {
+              const float4 p = tile[j];
+              if (__float_as_int(p.w) == qi) continue;
+              const float pk[3] = {p.x, p.y, p.z};
+              sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
+            }
+        }
+        if (active) {
+          if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
+          else st_global(retry, atomicAdd(nretry, 1), q);
+        }
       }
     }
     __syncthreads();  // block cells are reloaded for the next cell
@@ -290,7 +379,9 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
 
 // Retry: for each listed point (grid position), the (2R+1)^3 blocks for
 // R = 2..rmax, read straight from the sorted points, until the k-th distance
-// is below R voxels. Unsettled points keep md = -1 (not eligible).
+// is below R voxels: each R adds the shell of cells at Chebyshev distance R
+// to the selection of the smaller blocks (the selected multiset does not
+// depend on the order). Unsettled points keep md = -1 (not eligible).
 template <int K>
 __global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __restrict__ pts,
                                                              CellTab t, VoxelFrame f, int k,
@@ -304,12 +395,14 @@ __global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __res
     const int qi = __float_as_int(qp.w);
     const int64_t ix = voxel_coord(qp.x, f.ox, f.inv), iy = voxel_coord(qp.y, f.oy, f.inv),
                   iz = voxel_coord(qp.z, f.oz, f.inv);
+    Knn<K> sel;
+    sel.init();
     for (int R = 2; R <= rmax; ++R) {
-      Knn<K> sel;
-      sel.init();
       for (int dz = -R; dz <= R; ++dz)
         for (int dy = -R; dy <= R; ++dy)
           for (int dx = -R; dx <= R; ++dx) {
+            // R == 2 takes the whole 5x5x5 block; larger R only their shell
+            if (R > 2 && abs(dz) != R && abs(dy) != R && abs(dx) != R) continue;
             const int64_t cx = ix + dx, cy = iy + dy, cz = iz + dz;
             if (cx < 0 || cy < 0 || cz < 0 || cx >= f.X || cy >= f.Y || cz >= f.Z) continue;
             const int32_t c = cell_find(t, voxel_key(f, cx, cy, cz));
