@@ -380,6 +380,22 @@ PSGC_API int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1,
                        double cx2, double baseline, double Q[16],
                        double R1[9]);
 
+/* Stereo rectification for zero distortion: cv::stereoRectify as
+ * setCalibrationParams calls it (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+ * 302-307: flags 0, alpha -1, new size = image size), restated from OpenCV
+ * 4.7.0's cvStereoRectify: each camera turns by half the relative rotation
+ * (Rodrigues vector of R times -1/2), then both by the rotation that takes the
+ * baseline onto the x axis (or y, when |T_y| > |T_x|); R1 = wR r^T, R2 = wR r.
+ * Focal length of both rectified cameras: the mean of the two cameras' f_y
+ * (f_x for vertical rigs); principal points put the mean of the four
+ * projected image corners at the image centre, with the two cameras' y (x)
+ * averaged. P1, P2 are 3x4, Q 4x4, row-major; P2[0][3] (P2[1][3]) =
+ * t fc. K1, K2, R 3x3 row-major; T 3. Host-only; distortion must be zero
+ * (the caller passes none). */
+PSGC_API int psgc_stereo_rectify(const double K1[9], const double K2[9], const double R[9],
+                                 const double T[3], int width, int height, double R1[9],
+                                 double R2[9], double P1[12], double P2[12], double Q[16]);
+
 /* Full-left rectification canvas: computeFullLeftRectification
  * (JSLAB_PhaseEpipolarReconstructor_V2.cpp:248-289) for zero distortion, the
  * branch setCalibrationParams takes when use_full_left_rectification is set

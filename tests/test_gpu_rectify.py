@@ -127,3 +127,36 @@ def test_compact_stable_order(ctx, shape, density):
     assert cnt.value == n
     assert np.array_equal(out[:n], xyz[P > 0])
     assert np.array_equal(og[:n], gray[P > 0])
+
+
+def test_reconstructor_from_calibration_params_matches_oracle(ctx):
+    """setCalibrationParams from the stereo calibration (K1, K2, R, t, zero
+    distortion): stereoRectify, the full-left canvas and the remap tables on
+    the device; setData and startCalculate against the oracle's remap and
+    chain on the same tables and matrices."""
+    p, left, right = scenes.c2_small(5)
+    dcfg = scenes.c2_decode_config()
+    L, R = oracle.decode(dcfg, left), oracle.decode(dcfg, right)
+    rows, cols = L[0].shape
+    K = np.array([[2400.0, 0, cols / 2 - 0.5], [0, 2400.0, rows / 2 - 0.5], [0, 0, 1]])
+    Rrig = _rot("y", 0.15) @ _rot("x", -0.1)
+    params = zim_amd.StereoCalibParams_V1((K, K), (np.zeros(5), np.zeros(5)), Rrig,
+                                          np.array([-20.0, 0.1, 0.05]), (cols, rows))
+    cfg = scenes.c2_recon_config(use_full_left_rectification=1, mls_window_size=4)
+    rec = zim_amd.PhaseEpipolarReconstructor_V2(cfg, ctx)
+    rec.setCalibrationParams(params)
+    R1, R2, P1, P2, Q = zim_amd.stereo_rectify(K, K, Rrig, params.t, (cols, rows))
+    assert np.allclose(rec.R1, R1.ravel())
+    rec.setData(*(torch.from_numpy(np.ascontiguousarray(a)).cuda()
+                  for a in (L[0], R[0], L[1], R[1], L[2])), 70)
+    gx, gP = rec.startCalculate()
+    torch.cuda.synchronize()
+    (lx, ly), (rx, ry) = [(a.cpu().numpy(), b.cpu().numpy()) for a, b in rec._maps]
+    lp, rp = oracle.remap(L[0], lx, ly), oracle.remap(R[0], rx, ry)
+    lk, rk = oracle.remap(L[1], lx, ly, nearest=True), oracle.remap(R[1], rx, ry, nearest=True)
+    ox, oP, _, _ = oracle.reconstruct(cfg, lp, rp, lk, rk, 70, rec.Q, rec.R1)
+    gx, gP = gx.cpu().numpy(), gP.cpu().numpy()
+    assert gP.shape == lx.shape
+    assert np.array_equal(gP, oP), f"mask mismatch {(gP != oP).sum()}"
+    assert np.abs(gx - ox).max() <= 1e-3
+    assert oP.sum() > 1000

@@ -221,3 +221,99 @@ def test_full_left_rejects_bad_input():
     P1, P2, Q = _stereo(K1, 1.0, 0.0, 0.0, 1.0)
     assert _full_left(np.zeros((3, 3)), (10, 10), np.eye(3), P1, P2, Q)[0] == _capi.PSGC_E_INVALID
     assert _full_left(K1, (0, 10), np.eye(3), P1, P2, Q)[0] == _capi.PSGC_E_INVALID
+
+
+# ------------------------------------------------------ stereo rectification --
+
+def _rodrigues(axis, deg):
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    t = np.radians(deg)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
+
+
+def _rig(seed, vertical=False):
+    g = np.random.default_rng(seed)
+    K1 = np.array([[2400 + g.uniform(-30, 30), 0, 1224 + g.uniform(-20, 20)],
+                   [0, 2400 + g.uniform(-30, 30), 1024 + g.uniform(-20, 20)], [0, 0, 1]])
+    K2 = np.array([[2400 + g.uniform(-30, 30), 0, 1224 + g.uniform(-20, 20)],
+                   [0, 2400 + g.uniform(-30, 30), 1024 + g.uniform(-20, 20)], [0, 0, 1]])
+    R = _rodrigues(g.normal(size=3), g.uniform(1, 6))
+    T = np.array([-150.0, g.uniform(-8, 8), g.uniform(-8, 8)])
+    if vertical:
+        T = T[[1, 0, 2]]
+    return K1, K2, R, T
+
+
+@pytest.mark.parametrize("seed,vertical", [(1, False), (2, False), (3, False), (4, True)])
+def test_stereo_rectify_geometry(seed, vertical):
+    """What stereoRectify must achieve, from the geometry alone: both
+    rectified cameras share one orientation (R2 R R1^T = I), the baseline
+    lies on the x axis (y for a vertical rig), rows (columns) of a scene point
+    agree between the two rectified images, and Q takes a rectified pixel and
+    its disparity back to the point."""
+    import zim_amd
+    K1, K2, R, T = _rig(seed, vertical)
+    R1, R2, P1, P2, Q = zim_amd.stereo_rectify(K1, K2, R, T, (2448, 2048))
+    for Rk in (R1, R2):
+        assert np.abs(Rk @ Rk.T - np.eye(3)).max() < 1e-12 and abs(np.linalg.det(Rk) - 1) < 1e-12
+    assert np.abs(R2 @ R @ R1.T - np.eye(3)).max() < 1e-12
+    t = R2 @ T
+    ax = 1 if vertical else 0
+    assert np.abs(np.delete(t, ax)).max() < 1e-9 * np.linalg.norm(T)
+    assert P1[0, 0] == P1[1, 1] == P2[0, 0] == P2[1, 1]
+    assert P2[ax, 3] == pytest.approx(t[ax] * P1[0, 0], rel=1e-12)
+    assert P1[1 - ax, 2] == P2[1 - ax, 2]
+    g = np.random.default_rng(seed + 50)
+    X = np.column_stack([g.uniform(-200, 200, 40), g.uniform(-150, 150, 40), g.uniform(400, 900, 40)])
+    # raw pixels of scene points (left camera frame), then their rectified pixels
+    ul = (K1 @ X.T).T
+    ur = (K2 @ (R @ X.T + T[:, None])).T
+    rl = (P1[:, :3] @ R1 @ np.linalg.inv(K1) @ ul.T).T
+    rr = (P2[:, :3] @ R2 @ np.linalg.inv(K2) @ ur.T).T
+    rl, rr = rl[:, :2] / rl[:, 2:], rr[:, :2] / rr[:, 2:]
+    assert np.abs(rl[:, 1 - ax] - rr[:, 1 - ax]).max() < 1e-6
+    d = rl[:, ax] - rr[:, ax]
+    h = Q @ np.column_stack([rl, d, np.ones(len(d))]).T
+    Xq = (h[:3] / h[3]).T
+    assert np.abs(Xq - (R1 @ X.T).T).max() < 1e-6 * 900
+
+
+def test_stereo_rectify_of_an_ideal_rig_is_the_identity():
+    """An already rectified rig (R = I, T along x, equal focal lengths)
+    keeps its cameras and principal points, and Q is the closed form."""
+    import zim_amd
+    K = np.array([[2400.0, 0, 1223.5], [0, 2400.0, 1023.5], [0, 0, 1]])
+    R1, R2, P1, P2, Q = zim_amd.stereo_rectify(K, K, np.eye(3), [-150.0, 0, 0], (2448, 2048))
+    assert np.abs(R1 - np.eye(3)).max() < 1e-15 and np.abs(R2 - np.eye(3)).max() < 1e-15
+    assert np.abs(P1[:, :3] - K).max() < 1e-3
+    lib = _capi.load()
+    Qi, R1i = np.zeros(16), np.zeros(9)
+    assert lib.psgc_rectify_ideal(2400.0, 2400.0, P1[0, 2], P1[1, 2], P2[0, 2], 150.0,
+                                  _capi.ptr(Qi), _capi.ptr(R1i)) == 0
+    assert np.abs(Q.ravel() - Qi).max() < 1e-12
+
+
+def test_stereo_rectify_half_turn():
+    """Cameras turned by a half turn about a skew axis: the half rotation
+    each camera takes still aligns them (the rotation vector's axis and sign
+    are recovered at theta = pi)."""
+    import zim_amd
+    K = np.array([[2400.0, 0, 1223.5], [0, 2400.0, 1023.5], [0, 0, 1]])
+    for axis in ([0.2, 1.0, 0.1], [0.0, 0.6, 0.8], [1.0, -0.3, 0.2]):
+        R = _rodrigues(axis, 180.0)
+        R1, R2, P1, P2, Q = zim_amd.stereo_rectify(K, K, R, [-150.0, 3.0, 1.0], (2448, 2048))
+        assert np.isfinite(Q).all()
+        # the angle comes from acos of the trace, whose slope is unbounded at
+        # pi: about sqrt(eps) of angle error (a wrong axis or sign is O(1))
+        assert np.abs(R2 @ R @ R1.T - np.eye(3)).max() < 1e-6, axis
+
+
+def test_calibration_params_object_rejects_distortion():
+    import zim_amd
+    K = np.eye(3)
+    p = zim_amd.StereoCalibParams_V1((K, K), (np.zeros(5), np.array([0.1, 0, 0, 0, 0])),
+                                     np.eye(3), np.array([-1.0, 0, 0]), (64, 48))
+    with pytest.raises(NotImplementedError):
+        zim_amd.PhaseEpipolarReconstructor_V2(_capi.ReconConfig.default()).setCalibrationParams(p)

@@ -17,6 +17,7 @@ built library or a GPU every call raises.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import enum
 import math
 
@@ -30,7 +31,8 @@ from ._capi import (DecodeConfig, ReconConfig, PsgcError, MEM_DEVICE, MEM_HOST, 
 
 __all__ = ["Batch", "Context", "DecodeConfig", "ReconConfig", "PhaseShiftDecoder", "Version",
            "PhaseEpipolarReconstructor_V2", "applyMLSFilterToPointCloud", "decode",
-           "reconstruct", "PsgcError", "full_left_rectification", "rectify_maps", "remap"]
+           "reconstruct", "PsgcError", "full_left_rectification", "rectify_maps", "remap",
+           "stereo_rectify", "StereoCalibParams_V1"]
 
 _SAMPLE = {torch.uint8: _capi.SAMPLE_U8, torch.uint16: _capi.SAMPLE_U16,
            torch.float64: _capi.SAMPLE_F64, np.dtype(np.uint8): _capi.SAMPLE_U8,
@@ -554,6 +556,34 @@ class PhaseShiftDecoder:
         return self._valid
 
 
+def stereo_rectify(K1, K2, R, T, image_size):
+    """cv::stereoRectify(K1, 0, K2, 0, size, R, T, flags 0, alpha -1) as
+    setCalibrationParams calls it (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
+    302-307), zero distortion (psgc_stereo_rectify): returns R1, R2 (3x3),
+    P1, P2 (3x4) and Q (4x4)."""
+    lib = _capi.load()
+    K1, K2, R = (np.ascontiguousarray(a, np.float64).reshape(9) for a in (K1, K2, R))
+    T = np.ascontiguousarray(T, np.float64).reshape(3)
+    R1, R2, Q = np.zeros(9), np.zeros(9), np.zeros(16)
+    P1, P2 = np.zeros(12), np.zeros(12)
+    _capi.check(lib.psgc_stereo_rectify(_capi.ptr(K1), _capi.ptr(K2), _capi.ptr(R), _capi.ptr(T),
+                                        int(image_size[0]), int(image_size[1]), _capi.ptr(R1),
+                                        _capi.ptr(R2), _capi.ptr(P1), _capi.ptr(P2), _capi.ptr(Q)))
+    return R1.reshape(3, 3), R2.reshape(3, 3), P1.reshape(3, 4), P2.reshape(3, 4), Q.reshape(4, 4)
+
+
+@dataclasses.dataclass
+class StereoCalibParams_V1:
+    """JSLAB_StereoCalibParams_V1's fields (camera matrices, distortion,
+    R, t of the right camera, image size (width, height)); only zero
+    distortion is supported."""
+    camera_matrix: tuple
+    dist_coeffs: tuple
+    R: object
+    t: object
+    image_size: tuple
+
+
 def full_left_rectification(K1, image_size, R1, P1, P2, Q):
     """computeFullLeftRectification (JSLAB_PhaseEpipolarReconstructor_V2.cpp:
     248-289) for zero distortion: returns (P1, P2, Q, (width, height)) with
@@ -628,10 +658,23 @@ class PhaseEpipolarReconstructor_V2:
         self._maps = None
         self.output_size = None
 
-    def setCalibrationParams(self, Q, R1):
-        self.Q = np.asarray(Q, np.float64).reshape(16)
-        self.R1 = np.asarray(R1, np.float64).reshape(9)
-        self._maps = None
+    def setCalibrationParams(self, params, R1=None, device="cuda"):
+        """setCalibrationParams (…V2.cpp:292-338) from a StereoCalibParams_V1
+        with zero distortion: stereoRectify, the full-left canvas when the
+        config asks for it, then the remap tables. setCalibrationParams(Q, R1)
+        instead takes a pre-rectified pair's Q and R1 (setData then uses the
+        maps as they are)."""
+        if R1 is not None:
+            self.Q = np.asarray(params, np.float64).reshape(16)
+            self.R1 = np.asarray(R1, np.float64).reshape(9)
+            self._maps = None
+            return
+        p = params
+        if any(np.any(np.asarray(d, np.float64) != 0) for d in p.dist_coeffs):
+            raise NotImplementedError("setCalibrationParams: only zero distortion is supported")
+        K1, K2 = (np.asarray(k, np.float64) for k in p.camera_matrix)
+        R1_, R2_, P1, P2, Q = stereo_rectify(K1, K2, p.R, p.t, p.image_size)
+        self.setRectification(K1, K2, R1_, R2_, P1, P2, Q, p.image_size, device)
 
     def setRectification(self, K1, K2, R1, R2, P1, P2, Q, image_size, device="cuda"):
         """The part of setCalibrationParams after cv::stereoRectify

@@ -226,6 +226,7 @@ def load(path: str | None = None):
                             i32),
         "psgc_rectify_ideal": ([dbl, dbl, dbl, dbl, dbl, dbl, vp, vp], i32),
         "psgc_full_left_rectification": ([vp, i32, i32, vp, vp, vp, vp, vp, vp], i32),
+        "psgc_stereo_rectify": ([vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp], i32),
         "psgc_profile_enable": ([i32], i32),
         "psgc_set_kernels": ([vp, i32], i32),
         "psgc_rectify_maps": ([vp, vp, vp, vp, i32, i32, vp, vp, i32], i32),

@@ -1560,6 +1560,140 @@ int psgc_rectify_ideal(double fx, double fy, double cx1, double cy1, double cx2,
   return PSGC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Rotation vector of a rotation matrix (cvRodrigues2, matrix to vector): the
+// axis from the skew part, the angle from the trace; near 0 and pi the
+// symmetric part gives the axis.
+void rodrigues_vec(const double R[9], double r[3]) {
+  const double rx = R[7] - R[5], ry = R[2] - R[6], rz = R[3] - R[1];
+  const double s = std::sqrt((rx * rx + ry * ry + rz * rz) * 0.25);
+  double c = (R[0] + R[4] + R[8] - 1) * 0.5;
+  c = c > 1. ? 1. : c < -1. ? -1. : c;
+  const double theta = std::acos(c);
+  if (s < 1e-5) {
+    if (c > 0) {
+      r[0] = r[1] = r[2] = 0;
+      return;
+    }
+    // theta ~ pi: R ~ 2 n n^T - I, so |n_i| from the diagonal and the signs
+    // of n_y, n_z relative to n_x >= 0 from the first row
+    const double t0 = std::sqrt(std::max((R[0] + 1) * 0.5, 0.));
+    const double t1 = std::sqrt(std::max((R[4] + 1) * 0.5, 0.)) * (R[1] < 0 ? -1. : 1.);
+    double t2 = std::sqrt(std::max((R[8] + 1) * 0.5, 0.)) * (R[2] < 0 ? -1. : 1.);
+    // with n_x ~ 0 the first row carries no sign; the (1, 2) entry does
+    if (std::fabs(t0) < std::fabs(t1) && std::fabs(t0) < std::fabs(t2) && (R[5] > 0) != (t1 * t2 > 0))
+      t2 = -t2;
+    const double n = std::sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+    r[0] = t0 * theta / n;
+    r[1] = t1 * theta / n;
+    r[2] = t2 * theta / n;
+    return;
+  }
+  const double v = 1. / (2 * s) * theta;
+  r[0] = rx * v;
+  r[1] = ry * v;
+  r[2] = rz * v;
+}
+
+// Rotation matrix of a rotation vector (cvRodrigues2, vector to matrix).
+void rodrigues_mat(const double r[3], double R[9]) {
+  const double theta = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  if (theta < 2.220446049250313e-16) {
+    for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1. : 0.;
+    return;
+  }
+  const double c = std::cos(theta), s = std::sin(theta), c1 = 1. - c;
+  const double x = r[0] / theta, y = r[1] / theta, z = r[2] / theta;
+  const double rrt[9] = {x * x, x * y, x * z, x * y, y * y, y * z, x * z, y * z, z * z};
+  const double rx[9] = {0, -z, y, z, 0, -x, -y, x, 0};
+  for (int i = 0; i < 9; ++i) R[i] = c * ((i % 4 == 0) ? 1. : 0.) + c1 * rrt[i] + s * rx[i];
+}
+
+void matmul3(const double A[9], const double B[9], bool bt, double C[9]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * (bt ? B[j * 3 + k] : B[k * 3 + j]);
+      C[i * 3 + j] = s;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int psgc_stereo_rectify(const double K1[9], const double K2[9], const double R[9],
+                        const double T[3], int width, int height, double R1[9], double R2[9],
+                        double P1[12], double P2[12], double Q[16]) {
+  if (!K1 || !K2 || !R || !T || !R1 || !R2 || !P1 || !P2 || !Q)
+    PSGC_FAIL(PSGC_E_INVALID, "null argument");
+  if (width < 1 || height < 1) PSGC_FAIL(PSGC_E_INVALID, "image size %dx%d", width, height);
+  if (!(K1[0] != 0) || !(K1[4] != 0) || !(K2[0] != 0) || !(K2[4] != 0))
+    PSGC_FAIL(PSGC_E_INVALID, "focal lengths");
+  // half of the relative rotation for each camera
+  double om[3], rr[9], t[3];
+  rodrigues_vec(R, om);
+  for (double& v : om) v *= -0.5;
+  rodrigues_mat(om, rr);
+  for (int i = 0; i < 3; ++i) t[i] = rr[i * 3] * T[0] + rr[i * 3 + 1] * T[1] + rr[i * 3 + 2] * T[2];
+  const int idx = std::fabs(t[0]) > std::fabs(t[1]) ? 0 : 1;
+  const double c = t[idx], nt = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  if (!(nt > 0)) PSGC_FAIL(PSGC_E_INVALID, "zero baseline");
+  double uu[3] = {0, 0, 0};
+  uu[idx] = c > 0 ? 1 : -1;
+  // the rotation taking the baseline onto the axis
+  double ww[3] = {t[1] * uu[2] - t[2] * uu[1], t[2] * uu[0] - t[0] * uu[2],
+                  t[0] * uu[1] - t[1] * uu[0]};
+  const double nw = std::sqrt(ww[0] * ww[0] + ww[1] * ww[1] + ww[2] * ww[2]);
+  if (nw > 0)
+    for (double& v : ww) v *= std::acos(std::fabs(c) / nt) / nw;
+  double wR[9];
+  rodrigues_mat(ww, wR);
+  matmul3(wR, rr, true, R1);
+  matmul3(wR, rr, false, R2);
+  for (int i = 0; i < 3; ++i) t[i] = R2[i * 3] * T[0] + R2[i * 3 + 1] * T[1] + R2[i * 3 + 2] * T[2];
+  // common focal length; principal points from the projected image corners
+  const double nx = width, ny = height;
+  const double fc = (idx == 0 ? K1[4] + K2[4] : K1[0] + K2[0]) * 0.5;
+  double cc[2][2];
+  for (int k = 0; k < 2; ++k) {
+    const double* A = k == 0 ? K1 : K2;
+    const double* Rk = k == 0 ? R1 : R2;
+    double sx = 0, sy = 0;
+    for (int i = 0; i < 4; ++i) {
+      const float u = (float)((i % 2) * (nx - 1)), v = (float)((i < 2 ? 0 : 1) * (ny - 1));
+      // undistortPoints with zero distortion, as float
+      const float xn = (float)(((double)u - A[2]) * (1. / A[0]));
+      const float yn = (float)(((double)v - A[5]) * (1. / A[4]));
+      const double X = Rk[0] * xn + Rk[1] * yn + Rk[2];
+      const double Y = Rk[3] * xn + Rk[4] * yn + Rk[5];
+      const double Z = Rk[6] * xn + Rk[7] * yn + Rk[8];
+      sx += (double)(float)(fc * (X / Z));
+      sy += (double)(float)(fc * (Y / Z));
+    }
+    cc[k][0] = (nx - 1) / 2 - sx / 4;
+    cc[k][1] = (ny - 1) / 2 - sy / 4;
+  }
+  if (idx == 0) cc[0][1] = cc[1][1] = (cc[0][1] + cc[1][1]) * 0.5;
+  else cc[0][0] = cc[1][0] = (cc[0][0] + cc[1][0]) * 0.5;
+  for (int i = 0; i < 12; ++i) P1[i] = P2[i] = 0;
+  P1[0] = P1[5] = P2[0] = P2[5] = fc;
+  P1[2] = cc[0][0];
+  P1[6] = cc[0][1];
+  P2[2] = cc[1][0];
+  P2[6] = cc[1][1];
+  P1[10] = P2[10] = 1;
+  P2[idx * 4 + 3] = t[idx] * fc;
+  const double q[16] = {1, 0, 0, -cc[0][0], 0, 1, 0, -cc[0][1], 0, 0, 0, fc, 0, 0,
+                        -1. / t[idx],
+                        (idx == 0 ? cc[0][0] - cc[1][0] : cc[0][1] - cc[1][1]) / t[idx]};
+  std::memcpy(Q, q, sizeof(q));
+  return PSGC_OK;
+}
+
 int psgc_full_left_rectification(const double K1[9], int width, int height, const double R1[9],
                                  double P1[12], double P2[12], double Q[16], int* out_width,
                                  int* out_height) {
