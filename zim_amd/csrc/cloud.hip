@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
                                                        VoxelFrame f, int k, float lim,
                                                        double* __restrict__ md,
                                                        int32_t* __restrict__ retry,
+                                                       float* __restrict__ retry_kth,
                                                        int32_t* __restrict__ nretry) {
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
@@ -348,8 +349,13 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
           }
         }
         if (active) {
-          if (sel.best[K - 1] <= lim) st_global(md, qi, sel.mean_dist(K));
-          else st_global(retry, atomicAdd(nretry, 1), q);
+          if (sel.best[K - 1] <= lim) {
+            st_global(md, qi, sel.mean_dist(K));
+          } else {
+            const int32_t slot = atomicAdd(nretry, 1);
+            st_global(retry, slot, q);
+            st_global(retry_kth, slot, sel.best[K - 1]);
+          }
         }
       } else {
         Knn<K> sel;
@@ -368,8 +374,13 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
             }
         }
         if (active) {
-          if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
-          else st_global(retry, atomicAdd(nretry, 1), q);
+          if (sel.kth(k) <= lim) {
+            st_global(md, qi, sel.mean_dist(k));
+          } else {
+            const int32_t slot = atomicAdd(nretry, 1);
+            st_global(retry, slot, q);
+            st_global(retry_kth, slot, sel.kth(k));
+          }
         }
       }
     }
@@ -377,49 +388,73 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
   }
 }
 
-// Retry: for each listed point (grid position), the (2R+1)^3 blocks for
-// R = 2..rmax, read straight from the sorted points, until the k-th distance
-// is below R voxels: each R adds the shell of cells at Chebyshev distance R
-// to the selection of the smaller blocks (the selected multiset does not
-// depend on the order). Unsettled points keep md = -1 (not eligible).
+// First cell whose key is not below `key` (cells are sorted by key).
+__device__ inline int32_t cell_lower_bound(const CellTab& t, uint64_t key) {
+  int32_t lo = 0, hi = t.ncells;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (ld_global(t.ukey, mid) < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Cell coordinate of a double position (a search box bound; clamped to the
+// frame).
+__device__ inline int64_t box_cell(double v, float o, double inv, int64_t n) {
+  const double c = floor((v - (double)o) * inv);
+  return c < 0. ? 0 : (c > (double)(n - 1) ? n - 1 : (int64_t)c);
+}
+
+// Retry: for each listed point (grid position), the exact k nearest within
+// the ball that must hold them: the k-th distance the 3x3x3 block found
+// bounds the true one from above (infinite when the block held fewer than k
+// candidates), and a k-th nearest beyond rmax voxels makes the point not
+// eligible anyway, so the radius is the smaller of the two, widened by 1e-5
+// (far above the float rounding of the distances and the cell assignment).
+// Every point within it lies in the box of cells it spans, read row by row
+// (one key search per row of cells, then consecutive keys). A k-th squared
+// distance up to the limit of rmax voxels settles the point, as the block
+// search widening by shells up to rmax would (both find the true k nearest
+// whenever it lies within that limit). Unsettled points keep md = -1 (not
+// eligible).
 template <int K>
 __global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __restrict__ pts,
                                                              CellTab t, VoxelFrame f, int k,
                                                              int rmax,
                                                              const int32_t* __restrict__ retry,
+                                                             const float* __restrict__ retry_kth,
                                                              const int32_t* __restrict__ nretry,
                                                              double* __restrict__ md) {
   const int32_t nr = ld_global(nretry, 0);
+  const float lim = cloud_knn_limit(f.voxel, rmax);
   for (int32_t r = blockIdx.x * CB + threadIdx.x; r < nr; r += gridDim.x * CB) {
     const float4 qp = pts[ld_global(retry, r)];
     const int qi = __float_as_int(qp.w);
-    const int64_t ix = voxel_coord(qp.x, f.ox, f.inv), iy = voxel_coord(qp.y, f.oy, f.inv),
-                  iz = voxel_coord(qp.z, f.oz, f.inv);
+    const float k1 = ld_global(retry_kth, r);
+    double rho = (double)f.voxel * rmax;
+    if (k1 < INFINITY) rho = fmin(rho, sqrt((double)k1));
+    rho *= 1.0 + 1e-5;
+    const int64_t x0 = box_cell(qp.x - rho, f.ox, f.inv, f.X), x1 = box_cell(qp.x + rho, f.ox, f.inv, f.X);
+    const int64_t y0 = box_cell(qp.y - rho, f.oy, f.inv, f.Y), y1 = box_cell(qp.y + rho, f.oy, f.inv, f.Y);
+    const int64_t z0 = box_cell(qp.z - rho, f.oz, f.inv, f.Z), z1 = box_cell(qp.z + rho, f.oz, f.inv, f.Z);
     Knn<K> sel;
     sel.init();
-    for (int R = 2; R <= rmax; ++R) {
-      for (int dz = -R; dz <= R; ++dz)
-        for (int dy = -R; dy <= R; ++dy)
-          for (int dx = -R; dx <= R; ++dx) {
-            // R == 2 takes the whole 5x5x5 block; larger R only their shell
-            if (R > 2 && abs(dz) != R && abs(dy) != R && abs(dx) != R) continue;
-            const int64_t cx = ix + dx, cy = iy + dy, cz = iz + dz;
-            if (cx < 0 || cy < 0 || cz < 0 || cx >= f.X || cy >= f.Y || cz >= f.Z) continue;
-            const int32_t c = cell_find(t, voxel_key(f, cx, cy, cz));
-            if (c < 0) continue;
-            const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
-            for (int32_t j = s0; j < s1; ++j) {
-              const float4 p = pts[j];
-              if (__float_as_int(p.w) == qi) continue;
-              const float pk[3] = {p.x, p.y, p.z};
-              sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
-            }
+    for (int64_t cz = z0; cz <= z1; ++cz)
+      for (int64_t cy = y0; cy <= y1; ++cy) {
+        const uint64_t khi = voxel_key(f, x1, cy, cz);
+        for (int32_t c = cell_lower_bound(t, voxel_key(f, x0, cy, cz));
+             c < t.ncells && ld_global(t.ukey, c) <= khi; ++c) {
+          const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
+          for (int32_t j = s0; j < s1; ++j) {
+            const float4 p = pts[j];
+            if (__float_as_int(p.w) == qi) continue;
+            const float pk[3] = {p.x, p.y, p.z};
+            sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
           }
-      if (sel.kth(k) <= cloud_knn_limit(f.voxel, R)) {
-        st_global(md, qi, sel.mean_dist(k));
-        break;
+        }
       }
-    }
+    if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
   }
 }
 
@@ -685,9 +720,10 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
   }
   if (g.ncells > 0) {
     void* rl;
-    PSGC_TRY(ctx_buf(ctx, S_CL_RETRY, (size_t)g.np * 4 + 64, &rl));
+    PSGC_TRY(ctx_buf(ctx, S_CL_RETRY, (size_t)g.np * 8 + 64, &rl));
     int32_t* nretry = (int32_t*)rl;
     int32_t* retry = nretry + 16;
+    float* retry_kth = (float*)(retry + g.np);
     PSGC_HIP(hipMemsetAsync(nretry, 0, 4, s));
     const CellTab t{g.ukey, g.start, g.count, g.ncells};
     const int tbv = width_for(g);
@@ -696,20 +732,20 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     PSGC_PROF_BEGIN("cloud_knn", s);
     with_width(tbv, [&](auto w) {
       constexpr int TB = decltype(w)::value;
-      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
-      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, nretry);
+      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
+      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
+      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
+      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
     });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_knn", s);
     if (rmax >= 2) {
       const int rg = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
       PSGC_PROF_BEGIN("cloud_knn_retry", s);
-      if (k == 20) cloud_knn_retry_kernel<20><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
-      else if (k == 30) cloud_knn_retry_kernel<30><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
-      else if (k == 50) cloud_knn_retry_kernel<50><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
-      else cloud_knn_retry_kernel<0><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, nretry, md);
+      if (k == 20) cloud_knn_retry_kernel<20><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
+      else if (k == 30) cloud_knn_retry_kernel<30><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
+      else if (k == 50) cloud_knn_retry_kernel<50><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
+      else cloud_knn_retry_kernel<0><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
       PSGC_LAUNCHED();
       PSGC_PROF_END("cloud_knn_retry", s);
     }
