@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
               const float p3[3] = {e4.x, e4.y, e4.z};
               const float d2 = dist2(qx, qy, qz, p3);
               // a NaN distance never enters the list (as with the insertion below)
-              cand[i] = (e4.w != 0.f) & (d2 < __builtin_huge_valf()) ? d2 : __builtin_huge_valf();
+              cand[i] = ((e4.w != 0.f) & (d2 < __builtin_huge_valf())) ? d2 : __builtin_huge_valf();
             }
           }
           constexpr SortNet net = make_sort_net(NP);
