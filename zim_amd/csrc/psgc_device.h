@@ -59,8 +59,8 @@ __device__ inline double blurred_sample(const DecodeArgs& a, int f, int r, int x
 // (JSLAB_PhaseShiftDecoder_V2.cpp:99-117). Returns false when the pixel is
 // rejected (modulation below the threshold).
 __device__ inline bool level_finish(double S, double C, double G, int n, double thr,
-                                    double* wrap, double* mod) {
-  *wrap = psgc::atan2(S, C);
+                                    double* wrap, double* mod, const double* atab = kAtanTab) {
+  *wrap = psgc::atan2(S, C, atab);
   double b = S * S + C * C;
   double mean;
   if ((n & (n - 1)) == 0) {

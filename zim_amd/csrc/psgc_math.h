@@ -48,16 +48,18 @@ PSGC_HD inline double atan_small(double t) {
 // atan(x) for x in [0, 1]: x = c + d with c = i/64 the nearest table point,
 // atan(x) = atan(c) + atan((x - c) / (1 + x c)). The numerator is exact
 // (Sterbenz), the argument of atan_small is at most 1/128.
-PSGC_HD inline double atan01(double x) {
+// tab: the 65 table values (kAtanTab, or a copy of it, e.g. in LDS, where a
+// per-lane lookup does not wait on the memory hierarchy).
+PSGC_HD inline double atan01(double x, const double* tab) {
   const int i = (int)(x * 64.0 + 0.5);
   const double c = (double)i * 0.015625;
   const double t = (x - c) / (1.0 + x * c);
-  return kAtanTab[i] + atan_small(t);
+  return tab[i] + atan_small(t);
 }
 
 // atan2 with C99 semantics for signed zeros and infinities; within 2 ulp of
 // a correctly rounded result (checked against the C library in the tests).
-PSGC_HD inline double atan2(double y, double x) {
+PSGC_HD inline double atan2(double y, double x, const double* tab) {
   if (__builtin_isnan(x) || __builtin_isnan(y)) return x + y;
   const double ax = __builtin_fabs(x);
   const double ay = __builtin_fabs(y);
@@ -70,12 +72,14 @@ PSGC_HD inline double atan2(double y, double x) {
   } else {
     const bool swap = ay > ax;
     const double r = swap ? (ax / ay) : (ay / ax);
-    a = atan01(r);
+    a = atan01(r, tab);
     if (swap) a = (kPio2Hi - a) + kPio2Lo;
     if (xneg) a = (kPiHi - a) + kPiLo;
   }
   return __builtin_signbit(y) ? -a : a;
 }
+
+PSGC_HD inline double atan2(double y, double x) { return atan2(y, x, kAtanTab); }
 
 // C round(): halves away from zero.
 PSGC_HD inline double round_half_away(double v) { return __builtin_round(v); }

@@ -118,6 +118,9 @@ __global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __
                                                              int njobs, int64_t total) {
   constexpr int DLR = DRT + 2;
   __shared__ __align__(16) unsigned char lds[decode_lds_bytes<DRT>()];
+  // the atan2 table (published by the first tile's staging barrier)
+  __shared__ double atab[65];
+  if (threadIdx.x < 65) atab[threadIdx.x] = kAtanTab[threadIdx.x];
   const XcdRange xr = xcd_range(total);
   int jprev = 0;
   for (int64_t t = xr.first; t < xr.end; t += xr.stride) {
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __
         for (int q = 0; q < DRT; ++q) {
           double m;
           const bool lv = level_finish(S[q], Cs[q], G[q], a.shift_num[level], a.mod_thresh,
-                                       &wrap[level][q], &m);
+                                       &wrap[level][q], &m, atab);
           if (level == 0) mod0[q] = m;
           ok[q] = ok[q] && lv;
           S[q] = Cs[q] = G[q] = 0.0;
