@@ -307,6 +307,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
   __shared__ BlockCells<1> bc;
+  __shared__ float queue[K > 0 ? 16 : 1][TB];  // per-lane candidate queues
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
     // nearest cells first: the k-th distance falls early and later
@@ -321,33 +322,44 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
       const float4 qp = active ? pts[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
       const int qi = __float_as_int(qp.w);
       if constexpr (K > 0) {
-        // groups of 16 candidates, merged when one would enter some lane's
-        // K nearest (uniform per wave)
+        // Each lane queues the candidates below its current K-th distance
+        // (which only falls, so a candidate at or above it can never enter)
+        // in 16 LDS slots of its own; the queues are merged into the sorted
+        // lists when one fills (uniform per wave), so the merge networks run
+        // about once per 16 accepted candidates instead of once per 16 read.
         KnnNet<(K <= 32 ? 32 : 64)> sel;
         sel.init();
+        int qn = 0;
+        auto flush = [&]() {
+          float g[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) g[u] = u < qn ? queue[u][threadIdx.x] : INFINITY;
+          sel.merge16(g);
+          qn = 0;
+        };
         for (int32_t cb = 0; cb < M; cb += CH) {
           const int32_t len = min(CH, M - cb);
           __syncthreads();  // the previous chunk's readers are done
           stage_chunk<1, TB>(pts, &bc, cb, len, tile);
           __syncthreads();
-          for (int32_t j0 = 0; j0 < len; j0 += 16) {
-            float g[16];
-            float gmin = INFINITY;
+          // four candidates per step (their reads issue together), after
+          // making room for four in every queue
+          for (int32_t j0 = 0; j0 < len; j0 += 4) {
+            if (__any(qn > 12)) flush();
+            float4 p[4];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-              float d = INFINITY;
-              if (j0 + u < len) {
-                const float4 p = tile[j0 + u];
-                const float pk[3] = {p.x, p.y, p.z};
-                const float e = dist2(qp.x, qp.y, qp.z, pk);
-                if (__float_as_int(p.w) != qi && e == e) d = e;
-              }
-              g[u] = d;
-              gmin = fminf(gmin, d);
+            for (int u = 0; u < 4; ++u) p[u] = tile[min(j0 + u, len - 1)];
+            const float kth = sel.best[K - 1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const float pk[3] = {p[u].x, p[u].y, p[u].z};
+              const float e = dist2(qp.x, qp.y, qp.z, pk);
+              if (active && j0 + u < len && __float_as_int(p[u].w) != qi && e < kth)
+                queue[qn++][threadIdx.x] = e;
             }
-            if (__any(active && gmin < sel.best[K - 1])) sel.merge16(g);
           }
         }
+        if (__any(qn > 0)) flush();
         if (active) {
           if (sel.best[K - 1] <= lim) {
             st_global(md, qi, sel.mean_dist(K));
@@ -438,22 +450,28 @@ __global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __res
     const int64_t x0 = box_cell(qp.x - rho, f.ox, f.inv, f.X), x1 = box_cell(qp.x + rho, f.ox, f.inv, f.X);
     const int64_t y0 = box_cell(qp.y - rho, f.oy, f.inv, f.Y), y1 = box_cell(qp.y + rho, f.oy, f.inv, f.Y);
     const int64_t z0 = box_cell(qp.z - rho, f.oz, f.inv, f.Z), z1 = box_cell(qp.z + rho, f.oz, f.inv, f.Z);
-    Knn<K> sel;
-    sel.init();
-    for (int64_t cz = z0; cz <= z1; ++cz)
-      for (int64_t cy = y0; cy <= y1; ++cy) {
-        const uint64_t khi = voxel_key(f, x1, cy, cz);
-        for (int32_t c = cell_lower_bound(t, voxel_key(f, x0, cy, cz));
-             c < t.ncells && ld_global(t.ukey, c) <= khi; ++c) {
-          const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
-          for (int32_t j = s0; j < s1; ++j) {
-            const float4 p = pts[j];
-            if (__float_as_int(p.w) == qi) continue;
-            const float pk[3] = {p.x, p.y, p.z};
-            sel.add(dist2(qp.x, qp.y, qp.z, pk), k);
+    // visit(d2) for every candidate of the box but the point itself
+    auto scan = [&](auto&& visit) {
+      for (int64_t cz = z0; cz <= z1; ++cz)
+        for (int64_t cy = y0; cy <= y1; ++cy) {
+          const uint64_t khi = voxel_key(f, x1, cy, cz);
+          for (int32_t c = cell_lower_bound(t, voxel_key(f, x0, cy, cz));
+               c < t.ncells && ld_global(t.ukey, c) <= khi; ++c) {
+            const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
+            for (int32_t j = s0; j < s1; ++j) {
+              const float4 p = pts[j];
+              if (__float_as_int(p.w) == qi) continue;
+              const float pk[3] = {p.x, p.y, p.z};
+              visit(dist2(qp.x, qp.y, qp.z, pk));
+            }
           }
         }
-      }
+    };
+    // one-by-one insertion (here it measures faster than the queued merge
+    // networks of the first pass: each thread walks its own box)
+    Knn<K> sel;
+    sel.init();
+    scan([&](float d) { sel.add(d, k); });
     if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
   }
 }
