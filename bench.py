@@ -95,6 +95,10 @@ MLS_RADIUS = 1.5  # mm
 VALU_PER_PX_PROFILED = {"mls/0": 12373, "mls/1": 12006, "decode": 1789, "sor_knn": 1764,
                         "match": 573, "repro_gauss": 497, "median5": 271}
 VALU_PEAK_GINST_S = 157.3e3 / 2 / 64
+# The published FP32 vector peak counts packed FP32 FMAs (two per lane per
+# instruction); packed FP32 is disabled in this build, so single-lane FMAs
+# issue at most half that rate.
+VALU_UNPACKED_PEAK_GINST_S = VALU_PEAK_GINST_S / 2
 
 
 def main():
@@ -198,6 +202,19 @@ def main():
                    "insts_from": "rocprofv3 SQ_INSTS_VALU on this workload (profile-derived), "
                                  "profiles/c2_64pair_valu_counters.md"}
     log(f"stage ms/step {stage_ms}")
+    # the device's measured FMA issue ceilings (FP32 and FP64 chains, no
+    # packed instructions), outside the timed region
+    c32, c64 = C.c_double(), C.c_double()
+    _capi.check(lib.psgc_probe_valu_rate(ctx.h, C.byref(c32), C.byref(c64)))
+    fma_ceiling = {"fp32": round(c32.value, 1), "fp64": round(c64.value, 1)}
+    log(f"measured FMA issue ceilings (G wave-instructions/s): {fma_ceiling}")
+    if compute is not None:
+        compute["peak_unpacked_ginst_s"] = round(VALU_UNPACKED_PEAK_GINST_S, 1)
+        compute["peak_unpacked_from"] = ("the published peak counts packed FP32 FMAs (2 per lane "
+                                         "per instruction); packed FP32 is disabled in this build")
+        compute["fma_ceiling_measured_ginst_s"] = fma_ceiling
+        compute["frac_of_measured_fp32_fma_ceiling"] = round(compute["achieved_ginst_s"]
+                                                             / fma_ceiling["fp32"], 3)
 
     n_sub = args.plain_pairs if args.plain_pairs else args.pairs
 

@@ -592,6 +592,14 @@ PSGC_API int psgc_adjoint_backward_batch(psgc_ctx* ctx, const psgc_adjoint_view*
 PSGC_API int psgc_profile_enable(int on);
 PSGC_API int psgc_profile_read(const char* name, int64_t* launches, double* total_ms);
 
+/* VALU issue ceilings of the context's device (no counterpart in the
+ * reference; the measure the instruction-bound filter kernels are set
+ * against): every wave of a grid that fills the device runs eight independent
+ * fused multiply-add chains per lane, FP32 then FP64, no packed instructions.
+ * Returns wave-instructions (64 lanes each) per second, in units of 1e9, of
+ * one timed launch each after an untimed one. */
+PSGC_API int psgc_probe_valu_rate(psgc_ctx* ctx, double* fp32_ginst_s, double* fp64_ginst_s);
+
 #ifdef __cplusplus
 }
 #endif

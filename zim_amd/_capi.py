@@ -244,6 +244,7 @@ def load(path: str | None = None):
         "psgc_batch_grid": ([vp, C.c_char_p, C.POINTER(C.c_int)], i32),
         "psgc_batch_set_grid": ([vp, C.c_char_p, i32], i32),
         "psgc_profile_read": ([C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)], i32),
+        "psgc_probe_valu_rate": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
