@@ -48,6 +48,7 @@ def lib():
                                            vp, vp, vp, vp]
         L.orc_num_threads.restype = i32
         L.orc_check_f32.argtypes = [i32, flt, flt, i32, vp, vp, vp, vp]
+        L.orc_check_exp_w.argtypes = [flt, flt, i32, vp, vp]
         L.orc_check_sort_net.argtypes = [i32, vp]
         i64 = C.c_int64
         L.orc_cloud_sor.argtypes = [vp, vp, i64, i32, flt, flt, i32, vp, vp, vp]
@@ -68,6 +69,15 @@ def check_f32(which: str, lo: float, hi: float, stride: int = 1):
                              *[C.byref(o) for o in out])
     assert rc == 0
     return tuple(o.value for o in out)
+
+
+def check_exp_w(lo: float, hi: float, stride: int = 1):
+    """Floats in [lo, hi] (every `stride`-th) where psgc::exp_f32_w, the
+    organized MLS kernel's weight exponential, differs from psgc::exp_f32 in
+    any bit; returns (#differences, #floats tested)."""
+    nd, nt = C.c_int64(), C.c_int64()
+    assert lib().orc_check_exp_w(lo, hi, stride, C.byref(nd), C.byref(nt)) == 0
+    return nd.value, nt.value
 
 
 def check_sort_net(k: int) -> int:

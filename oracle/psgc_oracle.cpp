@@ -1722,6 +1722,37 @@ int orc_check_f32(int which, float lo, float hi, int stride, int64_t* max_ulp,
   return PSGC_OK;
 }
 
+// The organized MLS kernel's weight exponential psgc::exp_f32_w (256-entry
+// table, one fused multiply-add for the reduction's integer) against
+// psgc::exp_f32, which the restatement calls: the number of floats in
+// [lo, hi] (every `stride`-th) whose results differ in any bit.
+int orc_check_exp_w(float lo, float hi, int stride, int64_t* n_diff, int64_t* n_total) {
+  if (!n_diff || !n_total || stride < 1) return -1;
+  auto key = [](float f) -> int64_t {
+    int32_t i;
+    std::memcpy(&i, &f, 4);
+    return i < 0 ? -(int64_t)(i & 0x7fffffff) : (int64_t)i;
+  };
+  auto from_key = [](int64_t k) -> float {
+    int32_t i = k < 0 ? (int32_t)((-k) | 0x80000000LL) : (int32_t)k;
+    float f;
+    std::memcpy(&f, &i, 4);
+    return f;
+  };
+  const int64_t k0 = key(lo), k1 = key(hi);
+  int64_t nd = 0, nt = 0;
+#pragma omp parallel for reduction(+ : nd, nt) schedule(static, 1 << 16)
+  for (int64_t k = k0; k <= k1; k += stride) {
+    const float x = from_key(k);
+    const float a = psgc::exp_f32_w(x, psgc::kExp2Tab256), b = psgc::exp_f32(x);
+    nd += std::memcmp(&a, &b, 4) != 0;
+    ++nt;
+  }
+  *n_diff = nd;
+  *n_total = nt;
+  return PSGC_OK;
+}
+
 // Threads the OpenMP loops above run on (the timed CPU baseline's core count).
 int orc_num_threads() { return omp_get_max_threads(); }
 

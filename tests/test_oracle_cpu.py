@@ -58,6 +58,45 @@ def test_atan_table_correctly_rounded():
     assert np.all(err <= half_ulp * 1.001)
 
 
+def test_exp2_table_nearest_doubles():
+    """exp2_table.inc (2^(j/256), the weight exponential's table): each entry
+    is within half an ulp of 2^(j/256) evaluated in 60-digit decimal
+    arithmetic, and every fourth entry equals the 64-entry table of the
+    restatement's exp_f32 (kExp2Tab64 = 2^(i/64))."""
+    import os
+    import re
+    from decimal import Decimal, getcontext
+    here = os.path.join(os.path.dirname(__file__), "..", "zim_amd", "csrc")
+    vals = [float.fromhex(l.strip().rstrip(","))
+            for l in open(os.path.join(here, "exp2_table.inc")) if l.strip().startswith("0x")]
+    assert len(vals) == 256
+    getcontext().prec = 60
+    ln2 = Decimal(2).ln()
+    for j, v in enumerate(vals):
+        exact = (ln2 * j / 256).exp()
+        assert abs(Decimal(v) - exact) <= Decimal(np.spacing(v)) / 2, j
+    src = open(os.path.join(here, "psgc_math.h")).read()
+    body = re.search(r"kExp2Tab64\[64\] = \{(.*?)\};", src, re.S).group(1)
+    tab64 = [float.fromhex(x.strip()) for x in body.split(",") if x.strip()]
+    assert vals[::4] == tab64
+
+
+def test_weight_exp_equals_exp_f32_on_every_nonpositive_float():
+    """The organized MLS kernel computes its Gaussian weights with exp_f32_w
+    (a 256-entry table, a degree-4 series and one fused multiply-add for the
+    reduction's integer) instead of the restatement's exp_f32 (64 entries,
+    degree 5): both return the same float on every one of the 2^31 floats
+    x <= 0 from -0 down to -inf, so the filter's bits do not depend on which
+    one runs."""
+    nd, nt = oracle.check_exp_w(float("-inf"), -0.0)
+    assert nt == 0x7f800001 and nd == 0
+    # far positive arguments lie outside the weights' domain: exp_f32 clamps
+    # them to +inf, exp_w (clamped only from below) does not reduce them, so
+    # the two differ there, which shows that the check compares results
+    nd, _ = oracle.check_exp_w(1e15, 3e38, 9973)
+    assert nd > 0
+
+
 # ----------------------------------------------------------------- decode --
 
 def _phase_frames(phi, n, A=1150.0, B=750.0):

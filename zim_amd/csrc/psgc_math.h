@@ -193,6 +193,47 @@ PSGC_HD inline float exp_f32_nonpos(float xf, const double* tab) {
 
 PSGC_HD inline float exp_f32(float xf) { return exp_f32(xf, kExp2Tab64); }
 
+// 2^(j/256), j = 0..255, each the double nearest to the exact value
+// (generated by gen_exp2_table.py).
+constexpr double kExp2Tab256[256] = {
+#include "exp2_table.inc"
+};
+
+// p * r + q with the scalar coefficient p in a scalar register on the device
+// (one v_fma_f64; the addend q stays where it is).
+PSGC_HD inline double fma_rn_s(double p, double r, double q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "s"(p), "v"(r), "v"(q));
+  return d;
+#else
+  return __builtin_fma(p, r, q);
+#endif
+}
+
+// exp_f32_nonpos with fewer instructions, for the Gaussian weights of the
+// organized MLS kernel: x = (256 e + j) ln2/256 + r with |r| <= ln2/512, where
+// k = 256 e + j, the integer nearest to 256 x / ln 2, comes from one fused
+// multiply-add with 1.5 * 2^52 (the sum is rounded to an integer, which its
+// low word holds); e^r by its Taylor series through r^4 (first omitted term
+// below 4e-17 e^r) in Horner form with fused multiply-adds; `tab` holds
+// kExp2Tab256 (on the device a copy in shared memory). For every float
+// x <= 0 (and NaN, which both clamp to -104) it returns exp_f32_nonpos's
+// float, checked exhaustively over the 2^31 non-positive floats.
+PSGC_HD inline float exp_f32_w(float xf, const double* tab) {
+  const double x = (double)__builtin_fmaxf(xf, -104.0f);
+  constexpr double kRound = 0x1.8p52;
+  const double s = __builtin_fma(x, 256.0 * kInvLn2, kRound);
+  const double k = s - kRound;
+  const int ki = (int)(unsigned)__builtin_bit_cast(unsigned long long, s);
+  const double r = __builtin_fma(-k, kLn2Lo / 256.0, __builtin_fma(-k, kLn2Hi / 256.0, x));
+  double p = fma_rn_s(1.0 / 24.0, r, 1.0 / 6.0);
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r, 1.0);
+  p = __builtin_fma(p, r, 1.0);
+  return (float)__builtin_ldexp(p * tab[ki & 255], ki >> 8);
+}
+
 // acos(x) = atan2(sqrt((1 - x)(1 + x)), x) in double (the product of two
 // float-derived factors is exact, the square root correctly rounded).
 PSGC_HD inline float acos_f32(float xf) {

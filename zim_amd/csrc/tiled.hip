@@ -669,7 +669,7 @@ __device__ inline void mls_rows(const uint32_t* bits, int dy0, int dy1, int ry0,
       const float pk[3] = {e4.x, e4.y, e4.z};
       float w;
       if (CACHED) w = Wc[kc][threadIdx.x];
-      else w = exp_f32_nonpos(-dist2<C>(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
+      else w = exp_f32_w(-dist2<C>(pi[0], pi[1], pi[2], pk) * inv2s2, etab);
       ++kc;
       visit(pk, w);
     }
@@ -704,8 +704,8 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
   constexpr int MWD = 2 * HM + 1;
   __shared__ float4 T[LD][LD + 1];
   __shared__ float Wc[KC > 0 ? KC : 1][256];
-  __shared__ double etab[64];  // 2^(j/64) for exp_f32 (published by the first tile's barrier)
-  if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
+  __shared__ double etab[256];  // 2^(j/256) for exp_f32_w (published by the first tile's barrier)
+  etab[threadIdx.x] = kExp2Tab256[threadIdx.x];  // 256 threads
   const XcdRange xr = xcd_range(total);
   const int lx = threadIdx.x % CT, ly = threadIdx.x / CT;
   int jprev = 0;
@@ -772,7 +772,7 @@ __global__ void __launch_bounds__(256) mls_tiled_kernel(const CloudJob* __restri
             const float d2 = dist2<C>(pi[0], pi[1], pi[2], pk);
             if (mine & (e4.w != 0.f) & !(d2 > r2)) {
               row_bits |= 1u << dx;
-              const float w = exp_f32_nonpos(-d2 * inv2s2, etab);
+              const float w = exp_f32_w(-d2 * inv2s2, etab);
               if (KC > 0 && nv < KC) Wc[nv][threadIdx.x] = w;
               mls_accum_centroid<C>(pk, w, &f, &wsum);
               nv++;
