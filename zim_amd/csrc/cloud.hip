@@ -575,8 +575,11 @@ __global__ void __launch_bounds__(TB) cloud_gauss_kernel(const float4* __restric
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
   __shared__ BlockCells<R> bc;
-  __shared__ double etab[64];
-  if (threadIdx.x < 64) etab[threadIdx.x] = kExp2Tab64[threadIdx.x];
+  // 2^(j/256) for exp_f32_w (fewer instructions per weight than exp_f32;
+  // the cloud MLS keeps exp_f32's 64-entry table, whose smaller LDS
+  // footprint measures faster there)
+  __shared__ double etab[256];
+  for (int i = threadIdx.x; i < 256; i += TB) etab[i] = kExp2Tab256[i];
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
     load_block<R, TB>(t, f, ld_global(t.ukey, c), &bc);
@@ -597,7 +600,7 @@ __global__ void __launch_bounds__(TB) cloud_gauss_kernel(const float4* __restric
             const float pk[3] = {p.x, p.y, p.z};
             const float d2 = dist2(qp.x, qp.y, qp.z, pk);
             if (d2 > r2) continue;
-            const float w = exp_f32(-d2 * inv2s2, etab);
+            const float w = exp_f32_w(-d2 * inv2s2, etab);  // as exp_f32 for d2 in [0, r2]
             sx += w * pk[0];
             sy += w * pk[1];
             sz += w * pk[2];
