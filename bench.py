@@ -584,8 +584,9 @@ def run_c3(args):
     every pair). The autotuner also tunes the row match per image width
     (its own launch, grid and threads per row for each width) against one
     launch for all widths; both are timed here. The same pairs run one at a
-    time through the plain per-pair path beside it, whose masks and xyz must
-    agree with the batch."""
+    time through the same tiled kernels (one graph per pair: what batching
+    alone gains) and through the plain per-pair path beside it; their masks
+    and xyz must agree with the batch."""
     import numpy as np
     import torch
 
@@ -672,6 +673,38 @@ def run_c3(args):
         log(f"row match modes: {match_modes}")
         del ref
 
+    def agreement(outs):
+        mism, maxerr = 0, 0.0
+        for (xyz, P), bo in zip(outs, batch.outputs):
+            mism += int((P != bo["P"]).sum().item())
+            m = bo["P"] != 0
+            if m.any():
+                maxerr = max(maxerr, float((xyz[m] - bo["xyz"][m]).abs().max().item()))
+        return mism, maxerr
+
+    # the same tiled kernels one pair at a time (one captured graph per pair,
+    # default launch shapes, a sync after each): what batching alone gains
+    singles = [zim_amd.Batch([pr], ctx, graph=True) for pr in pairs]
+
+    def run_singles():
+        for b in singles:
+            b.run()
+            torch.cuda.synchronize()
+
+    run_singles()
+    t0 = time.perf_counter()
+    run_singles()
+    single_ms = (time.perf_counter() - t0) * 1e3
+    single_val = npts / (single_ms * 1e-3) / 1e6
+    mism, maxerr = agreement([(b.outputs[0]["xyz"], b.outputs[0]["P"]) for b in singles])
+    per_pair_tiled = {"mpts_per_s": round(single_val, 3), "ms": round(single_ms, 3),
+                      "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr}
+    for b in singles:
+        b.close()
+    del singles
+    torch.cuda.empty_cache()
+    log(f"tiled kernels one pair at a time: {per_pair_tiled}")
+
     # the same pairs one at a time through the plain kernels, a sync per pair
     ctx.set_kernels("plain")
 
@@ -692,12 +725,7 @@ def run_c3(args):
     plain_ms = (time.perf_counter() - t0) * 1e3
     ctx.set_kernels("tiled")
     plain_val = npts / (plain_ms * 1e-3) / 1e6
-    mism, maxerr = 0, 0.0
-    for (xyz, P), bo in zip(outs, batch.outputs):
-        mism += int((P != bo["P"]).sum().item())
-        m = bo["P"] != 0
-        if m.any():
-            maxerr = max(maxerr, float((xyz[m] - bo["xyz"][m]).abs().max().item()))
+    mism, maxerr = agreement(outs)
     log(f"plain per-pair: {plain_ms:.1f} ms, {plain_val:.1f} Mpts/s; mask mismatch {mism}, "
         f"xyz max diff {maxerr}")
     line = {"metric": "Mpts/s of one batch of 64 uneven pairs, batched vs per-pair",
@@ -714,7 +742,9 @@ def run_c3(args):
             "row_match_per_width": per_width, "row_match_modes": match_modes,
             "per_pair_plain": {"mpts_per_s": round(plain_val, 3), "ms": round(plain_ms, 3),
                                "mask_mismatch_px": mism, "xyz_max_diff_mm": maxerr},
-            "speedup_vs_per_pair": round(value / plain_val, 2), "autotuned_grids": grids}
+            "per_pair_tiled": per_pair_tiled,
+            "speedup_vs_per_pair": round(value / plain_val, 2),
+            "batching_gain": round(value / single_val, 2), "autotuned_grids": grids}
     print(json.dumps(line), flush=True)
     batch.close()
 
