@@ -162,7 +162,9 @@ PSGC_API int psgc_device_count(void);
 PSGC_API int psgc_create(psgc_ctx** out, int device);
 PSGC_API int psgc_destroy(psgc_ctx* ctx);
 /* Use the caller's hipStream_t (e.g. torch's current stream) instead of the
- * context's own; NULL restores the context's stream. */
+ * context's own; NULL restores the context's stream. The legacy default
+ * stream is given as hipStreamLegacy ((hipStream_t)1): work on it is ordered
+ * with every blocking stream's, and so with other contexts bound to it. */
 PSGC_API int psgc_set_stream(psgc_ctx* ctx, void* hip_stream);
 PSGC_API int psgc_synchronize(psgc_ctx* ctx);
 
@@ -583,6 +585,36 @@ typedef struct psgc_adjoint_view {
 } psgc_adjoint_view;
 PSGC_API int psgc_adjoint_forward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n);
 PSGC_API int psgc_adjoint_backward_batch(psgc_ctx* ctx, const psgc_adjoint_view* views, int n);
+
+/* ---- plane-fit objectives of the calibration refinement -------------------
+ * No counterpart in the reference (the refinement is this library's). Per
+ * view a set of points: xyz (n x 3 f64, row-major) and a selection byte per
+ * point (a point counts where it is 1: a boolean mask, or the triangulation's
+ * conditioning codes, 1 = conditioned); device pointers, on the context's
+ * stream. The per-view results and parameters are host arrays; every sum is
+ * FP64 in a fixed order (the same bits on every run), and each call returns
+ * when its results are written.
+ *   moments: count[v], mean[3v..], scatter[9v..] (the 3 x 3 matrix of the
+ *     selected points about their mean, row-major);
+ *   residuals: sumsq[v] = sum of ((x - mean) . normal)^2, sum[v] = sum of
+ *     (x - mean) . normal over the selected points;
+ *   grad: grad_xyz = a (x - mean) + b at selected points (a: 3 x 3 row-major,
+ *     b: 3 per view), 0 elsewhere; the gradient of an objective of the
+ *     scatter matrix S with dL/dS = G is a = G + G^T, b = 0 (the mean's
+ *     terms sum to zero over the selected points). */
+typedef struct psgc_plane_view {
+    const double* xyz;
+    const uint8_t* sel;
+    int64_t n;
+    double* grad_xyz;         /* grad: n x 3 output                       */
+} psgc_plane_view;
+PSGC_API int psgc_plane_moments_batch(psgc_ctx* ctx, const psgc_plane_view* views, int n,
+                                      double* count, double* mean, double* scatter);
+PSGC_API int psgc_plane_residuals_batch(psgc_ctx* ctx, const psgc_plane_view* views, int n,
+                                        const double* mean, const double* normal, double* sumsq,
+                                        double* sum);
+PSGC_API int psgc_plane_grad_batch(psgc_ctx* ctx, const psgc_plane_view* views, int n,
+                                   const double* mean, const double* a, const double* b);
 
 /* ---- instrumentation ------------------------------------------------------
  * Per-kernel-family timing with hipEvents recorded on the launch stream.

@@ -109,3 +109,16 @@ def test_mls_and_cloud_inputs(as_device):
         zim_amd.applyMLSFilterToPointCloud(xyz, P.bool(), 3, 0.5, ctx=_Stub())
     with pytest.raises(ValueError, match="valid: shape"):
         zim_amd.cloud_mls(xyz.reshape(-1, 3), torch.zeros(47, dtype=torch.uint8), ctx=_Stub())
+
+
+def test_plane_objectives_reject_host_or_mismatched_points():
+    """The plane objectives take f64 device points with one selection byte per
+    point; anything else is rejected before the library sees an address."""
+    import torch
+
+    from zim_amd import autograd as zag
+    x = torch.zeros((5, 3), dtype=torch.float64)
+    with pytest.raises(ValueError):
+        zag._plane_table([x], [torch.ones(5, dtype=torch.uint8)])
+    with pytest.raises(ValueError):
+        zag._plane_table([x.float()], [torch.ones(5, dtype=torch.uint8)])

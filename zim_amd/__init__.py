@@ -58,10 +58,10 @@ class Context:
         return self
 
     def bind_stream(self):
-        s = torch.cuda.current_stream(self.device)
-        if self._bound_stream != s.cuda_stream:
-            _capi.check(self.lib.psgc_set_stream(self.h, C.c_void_p(s.cuda_stream)))
-            self._bound_stream = s.cuda_stream
+        h = _capi.stream_handle(torch.cuda.current_stream(self.device))
+        if self._bound_stream != h:
+            _capi.check(self.lib.psgc_set_stream(self.h, C.c_void_p(h)))
+            self._bound_stream = h
         return self
 
     def release(self):

@@ -162,6 +162,29 @@ class PairDesc(C.Structure):
     ]
 
 
+class PlaneView(C.Structure):
+    """psgc_plane_view (include/psgc.h)."""
+    _fields_ = [
+        ("xyz", C.c_void_p),
+        ("sel", C.c_void_p),
+        ("n", C.c_int64),
+        ("grad_xyz", C.c_void_p),
+    ]
+
+
+# hipStreamLegacy: the legacy default stream, addressed explicitly
+HIP_STREAM_LEGACY = 1
+
+
+def stream_handle(stream) -> int:
+    """The hipStream_t to give the library for a torch stream. torch's
+    default stream is the legacy null stream (handle 0), which the library
+    would read as "the context's own stream": it gets hipStreamLegacy
+    instead, so that library work is ordered with torch's and with every
+    other context's on that stream."""
+    return stream.cuda_stream or HIP_STREAM_LEGACY
+
+
 def ptr(x):
     """Address of a numpy array, a torch tensor, or None."""
     if x is None:
@@ -245,6 +268,9 @@ def load(path: str | None = None):
         "psgc_batch_set_grid": ([vp, C.c_char_p, i32], i32),
         "psgc_profile_read": ([C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)], i32),
         "psgc_probe_valu_rate": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)], i32),
+        "psgc_plane_moments_batch": ([vp, vp, i32, vp, vp, vp], i32),
+        "psgc_plane_residuals_batch": ([vp, vp, i32, vp, vp, vp, vp], i32),
+        "psgc_plane_grad_batch": ([vp, vp, i32, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -50,7 +50,7 @@ def _load():
 
 def triangulate(theta, phL, phR, match_col, nominal, ctx: Context | None = None):
     ctx = ctx or default_context()
-    stream = torch.cuda.current_stream(phL.device).cuda_stream
+    stream = _capi.stream_handle(torch.cuda.current_stream(phL.device))
     nominal = torch.as_tensor(nominal, dtype=torch.float64)
     xyz, valid = _load().triangulate(theta, phL, phR, match_col, nominal, ctx.h.value, stream)
     return xyz, valid
@@ -63,7 +63,7 @@ def triangulate_views(thetas, phLs, phRs, match_cols, nominal, ctx: Context | No
     (xyz, valid) per view, differentiable like triangulate."""
     ctx = ctx or default_context()
     V = len(phLs)
-    stream = torch.cuda.current_stream(phLs[0].device).cuda_stream
+    stream = _capi.stream_handle(torch.cuda.current_stream(phLs[0].device))
     nom = torch.as_tensor(nominal, dtype=torch.float64)
     if nom.dim() == 1:
         nom = nom.expand(V, 5)
@@ -92,61 +92,144 @@ def point_mask(valid):
     return v if v.dtype == torch.bool else v == 1
 
 
-def _scatter(xyz, valid):
-    """3 x 3 scatter matrix of the valid points about their mean, from
-    masked reductions over every point (no compaction, so the backward pass
-    is elementwise too; a library GEMM of shape 3 x N x 3 runs ~1000x slower
-    on millions of points)."""
-    m = point_mask(valid)
-    p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
-    mu = p.sum(0) / m.sum()
-    x, y, z = torch.where(m[:, None], p - mu, 0.0).unbind(1)
-    sxx, sxy, sxz = (x * x).sum(), (x * y).sum(), (x * z).sum()
-    syy, syz, szz = (y * y).sum(), (y * z).sum(), (z * z).sum()
-    return torch.stack([torch.stack([sxx, sxy, sxz]), torch.stack([sxy, syy, syz]),
-                        torch.stack([sxz, syz, szz])])
+def _sel(valid):
+    """point_mask as the selection bytes of the plane kernels (1 = counts)."""
+    return point_mask(valid).reshape(-1).contiguous().view(torch.uint8)
 
 
-def plane_losses(views):
+def _plane_table(xyzs, sels, grads=None):
+    tab = (_capi.PlaneView * len(xyzs))()
+    for i, (x, m) in enumerate(zip(xyzs, sels)):
+        if x.dtype != torch.float64 or not x.is_cuda or x.numel() != 3 * m.numel():
+            raise ValueError("plane objectives take f64 device points (n x 3) and one "
+                             "selection per point")
+        tab[i].xyz = x.data_ptr()
+        tab[i].sel = m.data_ptr()
+        tab[i].n = m.numel()
+        tab[i].grad_xyz = grads[i].data_ptr() if grads is not None else None
+    return tab
+
+
+def _moments(ctx, xyzs, sels):
+    """(count [V], mean [V, 3], scatter [V, 3, 3]) as host f64 tensors, from
+    psgc_plane_moments_batch (FP64, fixed summation order)."""
+    V = len(xyzs)
+    cnt, mu, S = (C.c_double * V)(), (C.c_double * (3 * V))(), (C.c_double * (9 * V))()
+    tab = _plane_table(xyzs, sels)
+    ctx.bind_stream()
+    _capi.check(ctx.lib.psgc_plane_moments_batch(ctx.h, tab, V, cnt, mu, S))
+    return (torch.tensor(list(cnt), dtype=torch.float64),
+            torch.tensor(list(mu), dtype=torch.float64).view(V, 3),
+            torch.tensor(list(S), dtype=torch.float64).view(V, 3, 3))
+
+
+def _plane_grad(ctx, xs, sels, mu, A, b):
+    """grad = A (x - mu) + b at the selected points, 0 elsewhere, per view."""
+    grads = [torch.empty_like(x) for x in xs]
+    # host arrays held by name for the call
+    hm, ha, hb = (np.ascontiguousarray(t.numpy(), dtype=np.float64) for t in (mu, A, b))
+    ctx.bind_stream()
+    _capi.check(ctx.lib.psgc_plane_grad_batch(ctx.h, _plane_table(xs, sels, grads), len(xs),
+                                              _capi.ptr(hm), _capi.ptr(ha), _capi.ptr(hb)))
+    return grads
+
+
+class _PlaneScatter(torch.autograd.Function):
+    """Scatter matrices [V, 3, 3] (host) of the selected points of each view
+    about their mean, differentiable in the points: for dL/dS = G the
+    gradient is (G + G^T)(x - mean) at the selected points (the mean's own
+    terms sum to zero over them)."""
+
+    @staticmethod
+    def forward(actx, ctx, sels, *xyzs):
+        xs = [x.detach().reshape(-1, 3).contiguous() for x in xyzs]
+        _, mu, S = _moments(ctx, xs, sels)
+        actx.ctx, actx.sels, actx.mu = ctx, sels, mu
+        actx.shapes = [x.shape for x in xyzs]
+        actx.save_for_backward(*xs)
+        return S
+
+    @staticmethod
+    def backward(actx, gS):
+        xs = actx.saved_tensors
+        G = gS.detach().to("cpu", torch.float64)
+        A = G + G.transpose(1, 2)
+        b = torch.zeros(len(xs), 3, dtype=torch.float64)
+        grads = _plane_grad(actx.ctx, xs, actx.sels, actx.mu, A, b)
+        return (None, None, *[g.view(sh) for g, sh in zip(grads, actx.shapes)])
+
+
+class _PlaneLoss(torch.autograd.Function):
+    """Per view the sum of squared residuals r = n . (x - mean) of the
+    selected points to their least-squares plane (n: the scatter matrix's
+    eigenvector of the smallest eigenvalue, held fixed: the loss is
+    stationary in it). Gradient 2 n (r - sum(r) / N) at the selected points,
+    the second term from the mean."""
+
+    @staticmethod
+    def forward(actx, ctx, sels, *xyzs):
+        xs = [x.detach().reshape(-1, 3).contiguous() for x in xyzs]
+        V = len(xs)
+        cnt, mu, S = _moments(ctx, xs, sels)
+        n = torch.linalg.eigh(S)[1][:, :, 0].contiguous()
+        sq, sm = (C.c_double * V)(), (C.c_double * V)()
+        hm, hn = (np.ascontiguousarray(t.numpy(), dtype=np.float64) for t in (mu, n))
+        ctx.bind_stream()
+        _capi.check(ctx.lib.psgc_plane_residuals_batch(ctx.h, _plane_table(xs, sels), V,
+                                                       _capi.ptr(hm), _capi.ptr(hn), sq, sm))
+        actx.ctx, actx.sels, actx.mu, actx.n = ctx, sels, mu, n
+        actx.rbar = torch.tensor(list(sm), dtype=torch.float64) / cnt.clamp_min(1.0)
+        actx.shapes = [x.shape for x in xyzs]
+        actx.save_for_backward(*xs)
+        return torch.tensor(list(sq), dtype=torch.float64)
+
+    @staticmethod
+    def backward(actx, g):
+        xs = actx.saved_tensors
+        g = g.detach().to("cpu", torch.float64)
+        n = actx.n
+        A = 2.0 * g[:, None, None] * n[:, :, None] * n[:, None, :]
+        b = -2.0 * (g * actx.rbar)[:, None] * n
+        grads = _plane_grad(actx.ctx, xs, actx.sels, actx.mu, A, b)
+        return (None, None, *[gr.view(sh) for gr, sh in zip(grads, actx.shapes)])
+
+
+def _scatter_views(views, ctx=None):
+    ctx = ctx or default_context()
+    return _PlaneScatter.apply(ctx, [_sel(v) for _, v in views], *[x for x, _ in views])
+
+
+def plane_losses(views, ctx: Context | None = None):
     """Sum of squared residuals of the points of each (xyz, valid) (see
-    point_mask) to
-    their least-squares plane (the smallest eigenvalue of the scatter
-    matrix), summed from the residuals n . (x - mean) themselves, n the plane
-    normal: the eigenvalue read off the scatter matrix carries an absolute
-    error of ~1e-16 times its largest eigenvalue (the plane's extent squared,
-    times the point count), which swamps finite differences of the loss. The
-    loss is stationary in n, so n needs no gradient. The 3 x 3 eigenproblems
-    of all views are solved on the host in one round trip."""
-    S = torch.stack([_scatter(x, v) for x, v in views])
-    with torch.no_grad():
-        ns = torch.linalg.eigh(S.detach().cpu())[1][:, :, 0].to(S.device)
-    out = []
-    for (xyz, valid), n in zip(views, ns):
-        m = point_mask(valid)
-        p = torch.where(m[:, None], xyz.reshape(-1, 3), 0.0)
-        mu = p.sum(0) / m.sum()
-        d = p - mu   # elementwise dot with n: a library GEMV of N x 3 is far slower
-        r = torch.where(m, d[:, 0] * n[0] + d[:, 1] * n[1] + d[:, 2] * n[2], 0.0)
-        out.append((r * r).sum())
-    return torch.stack(out)
+    point_mask) to their least-squares plane, summed from the residuals
+    n . (x - mean) themselves, n the plane normal: the smallest eigenvalue
+    read off the scatter matrix carries an absolute error of ~1e-16 times its
+    largest eigenvalue (the plane's extent squared, times the point count),
+    which swamps finite differences of the loss. The loss is stationary in n,
+    so n needs no gradient. Moments, residuals and gradient run in the
+    library's fused kernels (psgc_plane_*_batch: one launch per pass for all
+    views, FP64 sums in a fixed order); the 3 x 3 eigenproblems of all views
+    are solved on the host. Returns a host tensor [V]."""
+    ctx = ctx or default_context()
+    return _PlaneLoss.apply(ctx, [_sel(v) for _, v in views], *[x for x, _ in views])
 
 
-def plane_loss(xyz, valid):
+def plane_loss(xyz, valid, ctx: Context | None = None):
     """plane_losses of one view."""
-    return plane_losses([(xyz, valid)])[0]
+    return plane_losses([(xyz, valid)], ctx)[0]
 
 
-def plane_flatnesses(views):
+def plane_flatnesses(views, ctx: Context | None = None):
     """Scale-invariant flatness of each (xyz, valid): smallest eigenvalue over
     the trace of the scatter matrix of the valid points (host eigensolve, one
     round trip for all views). Returns a host tensor."""
-    ev = torch.linalg.eigvalsh(torch.stack([_scatter(x, v) for x, v in views]).cpu())
+    ev = torch.linalg.eigvalsh(_scatter_views(views, ctx))
     return ev[:, 0] / ev.sum(1)
 
 
-def plane_flatness(xyz, valid):
+def plane_flatness(xyz, valid, ctx: Context | None = None):
     """plane_flatnesses of one view."""
-    return plane_flatnesses([(xyz, valid)])[0]
+    return plane_flatnesses([(xyz, valid)], ctx)[0]
 
 
 def default_scale(theta0):
@@ -218,7 +301,7 @@ def refine_plane(theta_start, theta_ref, phL, phR, kL, kR, nominal, pha_f, outer
 
     def obj(zz, mc, grad=False):
         th = (ref + zz * scale).requires_grad_(grad)
-        v = plane_flatness(*triangulate(th, phL, phR, mc, nominal, ctx))
+        v = plane_flatness(*triangulate(th, phL, phR, mc, nominal, ctx), ctx)
         if grad:
             v.backward()
             return v.detach(), th.grad * scale * mask
@@ -272,7 +355,7 @@ def refine_planes(thetas_start, thetas_ref, phLs, phRs, kLs, kRs, nominal, pha_f
     def objs(zz, mcs, grad=False):
         th = (ref + zz * scale).requires_grad_(grad)
         res = triangulate_views(th, phLs, phRs, mcs, nom, ctx)
-        vals = plane_flatnesses(res)
+        vals = plane_flatnesses(res, ctx)
         if grad:
             vals.sum().backward()
             return vals.detach().cpu(), th.grad * scale * mask

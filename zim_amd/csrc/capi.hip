@@ -146,13 +146,6 @@ struct DeviceGuard {
   }
 };
 
-#define PSGC_CTX_ENTER(ctx)                                              \
-  do {                                                                   \
-    if (!(ctx)) PSGC_FAIL(PSGC_E_INVALID, "null context");               \
-    if (hipSetDevice((ctx)->device) != hipSuccess)                       \
-      PSGC_FAIL(PSGC_E_HIP, "hipSetDevice(%d) failed", (ctx)->device);    \
-  } while (0)
-
 // Device pointer for a caller buffer: in host mode, a staging slot (filled
 // from the host when `upload`); in device mode, the pointer itself.
 template <typename T>

@@ -888,9 +888,12 @@ struct psgc_batch {
   bool graph = false;
 };
 
-// Captures the plan's launches into a graph (replacing an earlier one).
+// Captures the plan's launches into a graph (replacing an earlier one). The
+// launches are recorded on the context's own stream (the legacy default
+// stream, which the context may be bound to, cannot be captured); replays go
+// to the bound stream.
 static int capture(psgc_batch* b) {
-  hipStream_t s = b->ctx->stream;
+  hipStream_t s = b->ctx->own_stream;
   Plan& pl = b->plan;
   if (pl.exec) hipGraphExecDestroy(pl.exec);
   if (pl.graph) hipGraphDestroy(pl.graph);

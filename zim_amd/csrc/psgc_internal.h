@@ -57,6 +57,8 @@ enum Slot : int {
   // occupied cells, sort/scan scratch, small counters, retry list
   S_CL_KEY0, S_CL_KEY1, S_CL_IDX0, S_CL_IDX1, S_CL_PTS, S_CL_UKEY, S_CL_START, S_CL_COUNT,
   S_CL_TEMP, S_CL_MISC, S_CL_RETRY, S_CL_TUNE_MD, S_CL_TUNE_ST, S_ADJV,
+  // plane-fit objectives: view table, per-chunk partial sums, per-view sums
+  S_PLANEV, S_PLANEP, S_PLANEO,
   S_NUM_SLOTS
 };
 
@@ -72,6 +74,14 @@ struct psgc_ctx {
 };
 
 namespace psgc {
+
+// Entry of a C-ABI call on a context: a non-null context, its device current.
+#define PSGC_CTX_ENTER(ctx)                                              \
+  do {                                                                   \
+    if (!(ctx)) PSGC_FAIL(PSGC_E_INVALID, "null context");               \
+    if (hipSetDevice((ctx)->device) != hipSuccess)                       \
+      PSGC_FAIL(PSGC_E_HIP, "hipSetDevice(%d) failed", (ctx)->device);    \
+  } while (0)
 
 // Returns a device buffer of at least `bytes` for `slot` (grown with a
 // stream synchronisation so that in-flight kernels never see a freed block).
