@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpsgc.so")
 
 SOURCES = ["capi.hip", "plain.hip", "tiled.hip", "pipeline.hip", "adjoint.hip", "compact.hip",
-           "roi.hip", "cloud.hip", "probe.hip", "plane.hip"]
+           "roi.hip", "cloud.hip", "probe.hip", "plane.hip", "hoststage.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PSGC_OFFLOAD_ARCH", "gfx950")

@@ -80,6 +80,7 @@ int ctx_buf(psgc_ctx* ctx, int slot, size_t bytes, void** out) {
 
 int ctx_release(psgc_ctx* ctx) {
   if (ctx->stream) PSGC_HIP(hipStreamSynchronize(ctx->stream));
+  host_stage_release(ctx);
   for (auto& b : ctx->bufs)
     if (b.ptr) {
       PSGC_HIP(hipFree(b.ptr));
@@ -156,7 +157,7 @@ int stage_in(psgc_ctx* ctx, int mem, int slot, const T* p, size_t n, const T** o
   }
   void* d;
   PSGC_TRY(ctx_buf(ctx, slot, n * sizeof(T), &d));
-  PSGC_HIP(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  PSGC_TRY(host_to_device(ctx, d, p, n * sizeof(T)));
   *out = (const T*)d;
   return PSGC_OK;
 }
@@ -179,8 +180,7 @@ int stage_out(psgc_ctx* ctx, int mem, int slot, T* p, size_t n, T** out) {
 
 template <typename T>
 int copy_back(psgc_ctx* ctx, int mem, T* host, const T* dev, size_t n) {
-  if (mem == PSGC_MEM_HOST && host)
-    PSGC_HIP(hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  if (mem == PSGC_MEM_HOST && host) PSGC_TRY(device_to_host(ctx, host, dev, n * sizeof(T)));
   return PSGC_OK;
 }
 
@@ -544,18 +544,12 @@ int psgc_decode(psgc_ctx* ctx, const psgc_decode_config* cfg, const void* const*
     const size_t plane = (size_t)cols * sb * rows;
     void* d;
     PSGC_TRY(ctx_buf(ctx, S_FRAMES, plane * n_frames, &d));
-    const bool dense = row_stride_bytes == (int64_t)cols * sb;
     for (int f = 0; f < n_frames; ++f) {
-      // one flat copy per dense frame: a pitched copy of pageable memory
-      // measured about 50x slower (24 frames of 5120 x 5120: 1.15 s
-      // against 21 ms)
-      if (dense)
-        PSGC_HIP(hipMemcpyAsync((char*)d + plane * f, frames[f], plane, hipMemcpyHostToDevice,
-                                ctx->stream));
-      else
-        PSGC_HIP(hipMemcpy2DAsync((char*)d + plane * f, (size_t)cols * sb, frames[f],
-                                  (size_t)row_stride_bytes, (size_t)cols * sb, rows,
-                                  hipMemcpyHostToDevice, ctx->stream));
+      // through the pinned chunks; pitched frames are packed row by row on
+      // the host (a pitched DMA from pageable memory measured about 50x
+      // slower than a flat one)
+      PSGC_TRY(host_to_device_2d(ctx, (char*)d + plane * f, frames[f], (size_t)row_stride_bytes,
+                                 (size_t)cols * sb, rows));
       a.frames[f] = (char*)d + plane * f;
     }
     a.stride = (int64_t)cols * sb;
@@ -1290,10 +1284,10 @@ int psgc_reconstruct_poly_roi(psgc_ctx* ctx, const psgc_recon_config* cfg, const
     PSGC_HIP(hipMemcpyAsync(d_xyz, xo, n * 12, hipMemcpyDeviceToDevice, s));
   }
   PSGC_HIP(hipMemcpyAsync(d_P, po, n, hipMemcpyDeviceToDevice, s));
-  if (disp)
-    PSGC_HIP(hipMemcpyAsync(disp, dfinal, n * sizeof(float),
-                            mem == PSGC_MEM_HOST ? hipMemcpyDeviceToHost
-                                                 : hipMemcpyDeviceToDevice, s));
+  if (disp) {
+    if (mem == PSGC_MEM_HOST) PSGC_TRY(device_to_host(ctx, disp, dfinal, n * sizeof(float)));
+    else PSGC_HIP(hipMemcpyAsync(disp, dfinal, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
   PSGC_TRY(copy_back(ctx, mem, xyz, d_xyz, n * 3));
   PSGC_TRY(copy_back(ctx, mem, P, d_P, n));
   return finish(ctx, mem);

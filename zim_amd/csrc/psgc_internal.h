@@ -71,6 +71,11 @@ struct psgc_ctx {
   std::vector<psgc::DevBuf> bufs;
   int cu_count = 0;
   int kernels = PSGC_KERNELS_TILED;
+  // pinned staging chunks of the host-pointer calls, their last DMA's event
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  bool pin_busy[2] = {false, false};
+  bool pin_failed = false;
 };
 
 namespace psgc {
@@ -87,6 +92,16 @@ namespace psgc {
 // stream synchronisation so that in-flight kernels never see a freed block).
 int ctx_buf(psgc_ctx* ctx, int slot, size_t bytes, void** out);
 int ctx_release(psgc_ctx* ctx);
+
+// Host-memory transfers through the context's pinned staging chunks
+// (hoststage.hip). host_to_device(_2d) returns once the caller's memory has
+// been read (the device copy is ordered on the context's stream);
+// device_to_host returns when the data is in the caller's memory.
+int host_to_device(psgc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int host_to_device_2d(psgc_ctx* ctx, void* dst, const void* src, size_t spitch, size_t width,
+                      size_t rows);
+int device_to_host(psgc_ctx* ctx, void* dst, const void* src, size_t bytes);
+void host_stage_release(psgc_ctx* ctx);
 
 // ---- kernel launchers (all asynchronous on `s`, device pointers) --------
 
