@@ -54,3 +54,37 @@ def test_full_size_host_maps_reconstruct_as_device_maps(ctx):
     hL2 = zim_amd.decode(cfg, [f.cpu().numpy() for f in p.left], ctx)
     for a, b in zip(hL2, L):
         assert np.array_equal(a, b.cpu().numpy())
+
+
+def test_concurrent_host_calls_on_two_contexts():
+    """Two host threads, each with its own context, decode host frames at the
+    same time (the calls release the GIL): the host-side copies share one
+    thread pool and each context its own pinned chunks; every result equals
+    the device decode of the same frames."""
+    import threading
+    cfg = scenes.c2_decode_config()
+    g = np.random.default_rng(9)
+    stacks = [[g.integers(300, 2000, (1024, 1280)).astype(np.uint16) for _ in range(12)]
+              for _ in range(2)]
+    ctxs = [zim_amd.Context(0), zim_amd.Context(0)]
+    refs = [[t.cpu().numpy() for t in zim_amd.decode(cfg, [torch.from_numpy(f).cuda()
+                                                          for f in st], ctxs[0])]
+            for st in stacks]
+    errors = []
+
+    def worker(i):
+        try:
+            for _ in range(5):
+                got = zim_amd.decode(cfg, stacks[i], ctxs[i])
+                if not all(np.array_equal(a, b) for a, b in zip(got, refs[i])):
+                    errors.append(i)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th), "a decode thread did not finish"
+    assert not errors, errors
