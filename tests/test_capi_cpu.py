@@ -169,7 +169,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
     import subprocess
     inc = os.path.join(ROOT, "include")
     fields = {"psgc_decode_config": _capi.DecodeConfig, "psgc_recon_config": _capi.ReconConfig,
-              "psgc_pair_desc": _capi.PairDesc}
+              "psgc_pair_desc": _capi.PairDesc, "psgc_plane_view": _capi.PlaneView}
     lines = ['#include "psgc.h"', "#include <stdio.h>", "#include <stddef.h>", "int main(void){"]
     for cname, cls in fields.items():
         lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
