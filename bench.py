@@ -917,14 +917,19 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
     right = [f[:sample_rows].cpu().numpy() for f in pair.right]
     log(f"cpu baseline: one pair, rows [0,{sample_rows}), {cores} threads")
     oracle.decode(dcfg, [f[:8] for f in left])  # load + thread pool warm-up
-    runs = []
+    runs, dec, rec = [], [], []
     for _ in range(5):   # median of five runs of the whole sample
         t0 = time.perf_counter()
         L = oracle.decode(dcfg, left)
         R = oracle.decode(dcfg, right)
+        t1 = time.perf_counter()
         rx, rP, _, _ = oracle.reconstruct(rcfg, L[0], R[0], L[1], R[1], 70, pair.Q, pair.R1)
-        runs.append(time.perf_counter() - t0)
-    dt = sorted(runs)[len(runs) // 2]
+        t2 = time.perf_counter()
+        runs.append(t2 - t0)
+        dec.append(t1 - t0)
+        rec.append(t2 - t1)
+    med = lambda v: sorted(v)[len(v) // 2]  # noqa: E731
+    dt = med(runs)
     pts = sample_rows * COLS
     model = "unknown"
     try:
@@ -937,6 +942,9 @@ def cpu_baseline(pair, dcfg, rcfg, sample_rows, log):
         pass
     res = {"value": round(pts / dt / 1e6, 4), "unit": "Mpts/s", "cores": cores, "kind": "port",
            "cpu": model,
+           # the reference's stage boundaries (decode L+R, reconstruct; the
+           # pairs are rendered rectified, so there is no remap stage)
+           "stage_s": {"decode_lr": round(med(dec), 3), "reconstruct": round(med(rec), 3)},
            "sample": f"rows [0,{sample_rows}) of one 2448x2048 pair ({pts / 1e6:.2f} Mpts): "
                      f"decode L+R + reconstruct, median of 5 runs {dt:.2f} s "
                      f"(range {min(runs):.2f}-{max(runs):.2f} s)"}
