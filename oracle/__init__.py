@@ -40,6 +40,7 @@ def lib():
         L.orc_atan2_many.argtypes = [vp, vp, vp, C.c_long]
         L.orc_remap.argtypes = [i32, vp, i32, i32, vp, vp, i32, i32, vp]
         L.orc_voxel_cells.argtypes = [vp, i64, flt, flt, vp]
+        L.orc_knn_point_limit.argtypes = [vp, i64, vp, flt, vp]
         L.orc_reproject_poly.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, dbl, i32, vp, vp]
         L.orc_roi_right_window.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
         L.orc_reconstruct_poly_roi.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp,
@@ -338,6 +339,16 @@ def voxel_cells(v, o, voxel):
     v = _c(v, np.float32)
     out = np.zeros(v.shape, np.int64)
     assert lib().orc_voxel_cells(_p(v), v.size, float(o), float(voxel), _p(out)) == 0
+    return out
+
+
+def knn_point_limit(xyz, o, voxel):
+    """The device's per-point bound on the squared k-th distance that its
+    3x3x3 k-nearest pass settles (psgc_voxel.h, cloud_knn_point_limit)."""
+    xyz = _c(xyz, np.float32).reshape(-1, 3)
+    o = _c(o, np.float32)
+    out = np.zeros(len(xyz), np.float32)
+    assert lib().orc_knn_point_limit(_p(xyz), len(xyz), _p(o), float(voxel), _p(out)) == 0
     return out
 
 

@@ -1646,6 +1646,22 @@ int orc_voxel_cells(const float* v, int64_t n, float o, float voxel, int64_t* ou
   return PSGC_OK;
 }
 
+// The device's per-point bound of its first k-nearest pass
+// (psgc::cloud_knn_point_limit) for points xyz[n][3] in the frame of origin
+// o[3] and edge voxel.
+int orc_knn_point_limit(const float* xyz, int64_t n, const float* o, float voxel, float* out) {
+  if (!(voxel > 0.0f)) return PSGC_E_INVALID;
+  psgc::VoxelFrame f{};
+  f.ox = o[0];
+  f.oy = o[1];
+  f.oz = o[2];
+  f.voxel = voxel;
+  f.inv = 1.0 / (double)voxel;
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = psgc::cloud_knn_point_limit(f, xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]);
+  return PSGC_OK;
+}
+
 // The fixed sorting network of the device's k-nearest selection
 // (psgc::make_sort_net(k), k <= 26) on every input of zeros and ones (2^k of
 // them, 64 at a time as bit lanes): the number left unsorted. By the 0-1

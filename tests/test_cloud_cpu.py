@@ -149,6 +149,66 @@ def test_voxel_cells_are_exact_across_a_wide_grid(voxel):
     assert (in_float[~near] != want[~near]).sum() > 0
 
 
+def test_knn_point_limit_bounds_every_point_outside_the_block():
+    """The device's first k-nearest pass settles a point whose k-th squared
+    distance within its 3x3x3 block of cells is at most the point's own
+    bound: one voxel plus its distance to the nearest face of its cell, less
+    a margin. Every point outside the block must then be farther. Points
+    placed on and just past the block's faces (kept when the exact cell
+    assignment puts them outside), straight across from the query or offset
+    along another axis, all have float squared distances (as the kernel
+    computes them) above the bound, and the closest come within 1e-4 of it,
+    so the bound is the face distance and no looser; the same bound taken to
+    the farthest face of the cell is exceeded by some of them."""
+    g = np.random.default_rng(5)
+    voxel = np.float32(0.7)
+    o = np.array([-3.1, 12.5, 480.0], np.float32)
+    n = 20000
+    q = (o + g.uniform(2, 900, (n, 3)) * voxel).astype(np.float32)
+    lim = oracle.knn_point_limit(q, o, voxel)
+    assert (lim >= voxel * voxel * np.float32(0.9999)).all()
+    assert (lim <= (np.float32(1.5) * voxel) ** 2).all()
+    # planted: the bound to the farthest face of the point's own cell
+    t = (q.astype(np.float64) - o) / np.float64(voxel)
+    t -= np.floor(t)
+    far = (np.float64(voxel) * (1 + np.minimum(t, 1 - t).max(axis=1))) ** 2 * (1 - 4e-5)
+    qc = np.stack([oracle.voxel_cells(q[:, c], o[c], voxel) for c in range(3)], 1)
+    d2s, idx = [], []
+    for a in range(3):
+        for side in (-1, 1):
+            # the nearest face of the neighbouring block cell, a few ulps on
+            # either side
+            face = (o[a].astype(np.float64) +
+                    (qc[:, a] + (2 if side > 0 else -1)) * np.float64(voxel)).astype(np.float32)
+            for ulps in (-2, -1, 0, 1, 2):
+                for off in (0.0, 0.4):
+                    p = q.copy()
+                    p[:, a] = face + np.float32(ulps) * np.spacing(face)
+                    if off:
+                        b = (a + 1) % 3
+                        p[:, b] = (p[:, b] + g.uniform(-off, off, n) * voxel).astype(np.float32)
+                    pc = np.stack([oracle.voxel_cells(p[:, c], o[c], voxel) for c in range(3)], 1)
+                    out = np.flatnonzero((np.abs(pc - qc) >= 2).any(axis=1))
+                    d = (q[out] - p[out]).astype(np.float32)
+                    d2s.append((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+                    idx.append(out)
+    d2, idx = np.concatenate(d2s), np.concatenate(idx)
+    assert len(d2) > 200000
+    assert (d2 > lim[idx]).all()
+    assert (d2 / lim[idx]).min() < 1 + 1e-4
+    assert (d2 <= far[idx]).any()
+
+
+def test_frame_rejects_more_than_2_30_cells_along_an_axis():
+    """Cell assignment stays within 2^-22 cells of exact only up to 2^30 cells
+    along an axis; a frame longer than that is refused even when its total
+    cell count is allowed."""
+    ok = np.array([[0, 0, 0], [1.0e6, 0, 0]], np.float32)        # 1.0e9 cells along x
+    oracle.cloud_sor(ok, None, 1, 1.0, 1e-3)
+    with pytest.raises(AssertionError):
+        oracle.cloud_sor(ok * np.float32(1.1), None, 1, 1.0, 1e-3)  # 1.1e9
+
+
 def test_filters_exact_on_a_cloud_a_thousand_voxels_wide():
     """kNN (SOR) and radius (Gaussian) neighbour sets on a cloud spanning
     about 1000 voxels, by brute force: strips of points across the whole

@@ -71,6 +71,12 @@ def test_cloud_sor_host_mode_and_edge_cases(ctx):
         zim_amd.cloud_sor(xyz, None, 65, 1.0, 0.2, 4, ctx)
     with pytest.raises(zim_amd.PsgcError):
         zim_amd.cloud_sor(xyz, None, 5, 1.0, 1e-9, 4, ctx)   # 10^27 cells
+    # 1.1e9 cells along x (2.8e10 in all): past the 2^30 cells per axis
+    line = np.array([[0, 0, 0], [1.1e6, 0, 0]], np.float32)
+    with pytest.raises(zim_amd.PsgcError):
+        zim_amd.cloud_sor(line, None, 1, 1.0, 1e-3, 4, ctx)
+    with pytest.raises(AssertionError):
+        oracle.cloud_sor(line, None, 1, 1.0, 1e-3)
 
 
 @pytest.mark.parametrize("order,voxel", [(0, 0.4), (1, 0.4), (2, 0.4), (2, 0.6), (2, 1.0),

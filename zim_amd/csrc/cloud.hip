@@ -293,14 +293,15 @@ struct KnnNet {
 };
 
 // First pass: the 3x3x3 block. Points it cannot settle (fewer than k
-// candidates, or a k-th distance beyond one voxel) go to the retry list.
+// candidates, or a k-th distance beyond the point's bound: one voxel plus
+// its distance to the nearest face of its cell) go to the retry list.
 // TB threads per workgroup (one cell at a time), chunks of chunk_of(TB).
 constexpr int chunk_of(int tb) { return tb >= 256 ? 1024 : 512; }
 
 template <int K, int TB>
 __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, int k, float lim,
-                                                       double* __restrict__ md,
+                                                       float limmax, double* __restrict__ md,
                                                        int32_t* __restrict__ retry,
                                                        float* __restrict__ retry_kth,
                                                        int32_t* __restrict__ nretry) {
@@ -321,6 +322,9 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
       const int32_t q = c0 + qb + threadIdx.x;
       const float4 qp = active ? pts[q] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
       const int qi = __float_as_int(qp.w);
+      // the point's own bound on the block search (at least the block's,
+      // at most the one of the widest search the retry may make)
+      const float limq = fminf(fmaxf(lim, cloud_knn_point_limit(f, qp.x, qp.y, qp.z)), limmax);
       if constexpr (K > 0) {
         // Each lane queues the candidates below its current K-th distance
         // (which only falls, so a candidate at or above it can never enter)
@@ -361,7 +365,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
         }
         if (__any(qn > 0)) flush();
         if (active) {
-          if (sel.best[K - 1] <= lim) {
+          if (sel.best[K - 1] <= limq) {
             st_global(md, qi, sel.mean_dist(K));
           } else {
             const int32_t slot = atomicAdd(nretry, 1);
@@ -386,7 +390,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
             }
         }
         if (active) {
-          if (sel.kth(k) <= lim) {
+          if (sel.kth(k) <= limq) {
             st_global(md, qi, sel.mean_dist(k));
           } else {
             const int32_t slot = atomicAdd(nretry, 1);
@@ -749,14 +753,14 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     const CellTab t{g.ukey, g.start, g.count, g.ncells};
     const int tbv = width_for(g);
     const int grid = grid_for_cells(ctx, g.ncells, tbv);
-    const float lim = cloud_knn_limit(voxel, 1);
+    const float lim = cloud_knn_limit(g.f.voxel, 1), limmax = cloud_knn_limit(g.f.voxel, rmax);
     PSGC_PROF_BEGIN("cloud_knn", s);
     with_width(tbv, [&](auto w) {
       constexpr int TB = decltype(w)::value;
-      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
-      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
-      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
-      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, md, retry, retry_kth, nretry);
+      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
+      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
+      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
+      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
     });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_knn", s);

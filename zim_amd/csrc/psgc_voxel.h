@@ -20,11 +20,12 @@ namespace psgc {
 // cell at all; key = ix + X (iy + Y iz).
 // Cell coordinates are computed in double: the difference of two floats is
 // exact there (for any two coordinates within 2^29 of each other's
-// magnitude), and the one rounding of the product by 1 / voxel moves a
-// coordinate by at most 2^-52 of its value in cells, under 1e-9 cells for any
-// grid of up to 2^40 cells. A point therefore lies within 1e-9 cells of the
-// cell it is assigned to, which the two margins below rely on; in float the
-// error grows with the extent (about 1e-4 cells across 1000 cells) and would
+// magnitude), and the roundings of 1 / voxel and of the product move a
+// coordinate by at most 2^-52 of its value in cells: under 1e-9 cells up to
+// 4.5 M cells along an axis, and under 2^-22 cells for the 2^30 cells per axis
+// that voxel_frame allows. A point therefore lies within 2^-22 cells of the
+// cell it is assigned to, which the margins below exceed; in float the error
+// grows with the extent (about 1e-4 cells across 1000 cells) and would
 // exceed them.
 struct VoxelFrame {
   float ox, oy, oz;
@@ -47,7 +48,8 @@ PSGC_VX_HD inline uint64_t voxel_key_of(const VoxelFrame& f, float x, float y, f
 }
 
 // Frame for a bounding box [mn, mx] (host side). Returns false when the
-// cell count would exceed 2^40 (voxel too small for the extent).
+// cell count would exceed 2^40, or 2^30 along one axis (voxel too small for
+// the extent).
 inline bool voxel_frame(const float mn[3], const float mx[3], float voxel, VoxelFrame* f) {
   if (!(voxel > 0.0f)) return false;
   f->voxel = voxel;
@@ -59,7 +61,8 @@ inline bool voxel_frame(const float mn[3], const float mx[3], float voxel, Voxel
   f->Y = voxel_coord(mx[1], f->oy, f->inv) + 3;
   f->Z = voxel_coord(mx[2], f->oz, f->inv) + 3;
   const double cells = (double)f->X * (double)f->Y * (double)f->Z;
-  return cells < 1099511627776.0;
+  const int64_t axis = (int64_t)1 << 30;
+  return cells < 1099511627776.0 && f->X < axis && f->Y < axis && f->Z < axis;
 }
 
 // Number of key bits a radix sort of the cell keys needs.
@@ -76,9 +79,9 @@ PSGC_VX_HD inline bool voxel_point_ok(float x, float y, float z) {
 }
 
 // k-nearest search over the (2R+1)^3 cells around a point: every point
-// outside that block lies at least R voxels away (less 2e-9 voxels of cell
+// outside that block lies at least R voxels away (less 2^-21 voxels of cell
 // assignment error), so a k-th squared distance at or below this limit (R
-// voxels less a relative margin of 1e-5, far above that error and the float
+// voxels less a relative margin of 1e-5, above that error and the float
 // rounding of the squared distances, about 4e-7) makes the k found the exact
 // k nearest.
 PSGC_VX_HD inline float cloud_knn_limit(float voxel, int R) {
@@ -86,11 +89,29 @@ PSGC_VX_HD inline float cloud_knn_limit(float voxel, int R) {
   return rs * rs * 0.99999f;
 }
 
+// The same bound for one point of the 3x3x3 search: a point outside the
+// block lies at least voxel * (1 + m) away, m being the point's distance in
+// cells to the nearest face of its own cell (0 to 1/2), so a k-th squared
+// distance at or below (voxel (1 + m))^2 less a relative margin of 4e-5
+// (above the error of m, at most 2^-52 of the point's coordinate in cells,
+// and the float rounding of the distances) settles the point as well.
+PSGC_VX_HD inline float cloud_knn_point_limit(const VoxelFrame& f, float x, float y, float z) {
+  const double c[3] = {((double)x - (double)f.ox) * f.inv, ((double)y - (double)f.oy) * f.inv,
+                       ((double)z - (double)f.oz) * f.inv};
+  double m = 0.5;
+  for (int a = 0; a < 3; ++a) {
+    const double t = c[a] - floor(c[a]);
+    m = fmin(m, fmin(t, 1.0 - t));
+  }
+  const double b = (double)f.voxel * (1.0 + m);
+  return (float)(b * b * (1.0 - 4e-5));
+}
+
 // Cells around a point's own cell, per axis, that hold every neighbour
 // within radius r for the radius filters (MLS, Gaussian): 1 when voxel >= r
 // (a 3x3x3 block), 2 when 2 voxel >= r (5x5x5: half-radius cells cover
 // 15.6 r^3 instead of 27 r^3); 0 = voxel too small. A neighbour passes the
-// float test d^2 <= r^2 up to r (1 + 2^-22) away and sits within 1e-9 cells
+// float test d^2 <= r^2 up to r (1 + 2^-22) away and sits within 2^-22 cells
 // of its cell, so the cell edge used is at least r (1 + 2^-20) (or half of
 // it): *voxel is raised to that when below it (by under a millionth; the
 // neighbour sets do not depend on the cell edge).
