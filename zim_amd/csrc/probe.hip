@@ -71,11 +71,11 @@ int time_fma(psgc_ctx* ctx, float* out, int blocks, double* ginst_s) {
 using namespace psgc;
 
 extern "C" int psgc_probe_valu_rate(psgc_ctx* ctx, double* fp32_ginst_s, double* fp64_ginst_s) {
-  if (!ctx || !fp32_ginst_s || !fp64_ginst_s) PSGC_FAIL(PSGC_E_INVALID, "null argument");
-  PSGC_HIP(hipSetDevice(ctx->device));
+  PSGC_CTX_ENTER(ctx);
+  if (!fp32_ginst_s || !fp64_ginst_s) PSGC_FAIL(PSGC_E_INVALID, "null argument");
   // 64 workgroups per CU: many times the resident slots, so every SIMD
   // stays full until the tail
-  const int blocks = ctx->cu_count * 64;
+  const int blocks = (ctx->cu_count > 0 ? ctx->cu_count : 256) * 64;
   void* out = nullptr;
   PSGC_TRY(ctx_buf(ctx, S_TMP0, (size_t)blocks * 256 * sizeof(float), &out));
   PSGC_TRY(time_fma<float>(ctx, (float*)out, blocks, fp32_ginst_s));
