@@ -294,7 +294,8 @@ struct KnnNet {
 
 // First pass: the 3x3x3 block. Points it cannot settle (fewer than k
 // candidates, or a k-th distance beyond the point's bound: one voxel plus
-// its distance to the nearest face of its cell) go to the retry list.
+// its distance to the nearest face of its cell) are flagged in `again` at
+// their grid position, with the block's k-th distance in `kth_of`.
 // TB threads per workgroup (one cell at a time), chunks of chunk_of(TB).
 constexpr int chunk_of(int tb) { return tb >= 256 ? 1024 : 512; }
 
@@ -302,9 +303,8 @@ template <int K, int TB>
 __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, int k, float lim,
                                                        float limmax, double* __restrict__ md,
-                                                       int32_t* __restrict__ retry,
-                                                       float* __restrict__ retry_kth,
-                                                       int32_t* __restrict__ nretry) {
+                                                       uint8_t* __restrict__ again,
+                                                       float* __restrict__ kth_of) {
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
   __shared__ BlockCells<1> bc;
@@ -368,9 +368,8 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
           if (sel.best[K - 1] <= limq) {
             st_global(md, qi, sel.mean_dist(K));
           } else {
-            const int32_t slot = atomicAdd(nretry, 1);
-            st_global(retry, slot, q);
-            st_global(retry_kth, slot, sel.best[K - 1]);
+            st_global(again, q, (uint8_t)1);
+            st_global(kth_of, q, sel.best[K - 1]);
           }
         }
       } else {
@@ -393,9 +392,8 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
           if (sel.kth(k) <= limq) {
             st_global(md, qi, sel.mean_dist(k));
           } else {
-            const int32_t slot = atomicAdd(nretry, 1);
-            st_global(retry, slot, q);
-            st_global(retry_kth, slot, sel.kth(k));
+            st_global(again, q, (uint8_t)1);
+            st_global(kth_of, q, sel.kth(k));
           }
         }
       }
@@ -422,61 +420,93 @@ __device__ inline int64_t box_cell(double v, float o, double inv, int64_t n) {
   return c < 0. ? 0 : (c > (double)(n - 1) ? n - 1 : (int64_t)c);
 }
 
-// Retry: for each listed point (grid position), the exact k nearest within
-// the ball that must hold them: the k-th distance the 3x3x3 block found
-// bounds the true one from above (infinite when the block held fewer than k
-// candidates), and a k-th nearest beyond rmax voxels makes the point not
-// eligible anyway, so the radius is the smaller of the two, widened by 1e-5
-// (far above the float rounding of the distances and the cell assignment).
-// Every point within it lies in the box of cells it spans, read row by row
-// (one key search per row of cells, then consecutive keys). A k-th squared
-// distance up to the limit of rmax voxels settles the point, as the block
-// search widening by shells up to rmax would (both find the true k nearest
-// whenever it lies within that limit). Unsettled points keep md = -1 (not
-// eligible).
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Retry: for each listed point (grid positions in ascending order, so the
+// points of one cell and of its x-neighbours are consecutive), the exact k
+// nearest within the ball that must hold them: the k-th distance the 3x3x3
+// block found bounds the true one from above (infinite when the block held
+// fewer than k candidates), and a k-th nearest beyond rmax voxels makes the
+// point not eligible anyway, so the radius is the smaller of the two,
+// widened by 1e-5 (far above the float rounding of the distances and the
+// cell assignment). Every point within it lies in the box of cells it spans.
+// A wave takes 64 consecutive listed points; where their boxes lie close
+// together it walks their union row by row (one key search per row of
+// cells, then consecutive keys), every lane reading the same candidate at a
+// time: one memory transaction per candidate for the wave, where boxes
+// walked lane by lane take one per lane (as the other waves do). A lane
+// keeps a candidate only up to its own bound (the block's k-th distance,
+// which the k nearest cannot exceed, and the limit of rmax voxels), so the
+// k it keeps are those of its own box either way. A k-th squared distance up to
+// the limit settles the point, as the block search widening by shells up to
+// rmax would (both find the true k nearest whenever it lies within that
+// limit). Unsettled points keep md = -1 (not eligible).
 template <int K>
 __global__ void __launch_bounds__(CB) cloud_knn_retry_kernel(const float4* __restrict__ pts,
                                                              CellTab t, VoxelFrame f, int k,
                                                              int rmax,
                                                              const int32_t* __restrict__ retry,
-                                                             const float* __restrict__ retry_kth,
+                                                             const float* __restrict__ kth_of,
                                                              const int32_t* __restrict__ nretry,
                                                              double* __restrict__ md) {
   const int32_t nr = ld_global(nretry, 0);
   const float lim = cloud_knn_limit(f.voxel, rmax);
-  for (int32_t r = blockIdx.x * CB + threadIdx.x; r < nr; r += gridDim.x * CB) {
-    const float4 qp = pts[ld_global(retry, r)];
+  const int lane = (int)threadIdx.x & 63;
+  for (int64_t base = (int64_t)blockIdx.x * CB + ((int)threadIdx.x - lane); base < nr;
+       base += (int64_t)gridDim.x * CB) {
+    const int64_t r = base + lane;
+    const bool act = r < nr;
+    const int32_t q = act ? ld_global(retry, r) : ld_global(retry, base);
+    const float4 qp = pts[q];
     const int qi = __float_as_int(qp.w);
-    const float k1 = ld_global(retry_kth, r);
+    const float k1 = ld_global(kth_of, q);
     double rho = (double)f.voxel * rmax;
     if (k1 < INFINITY) rho = fmin(rho, sqrt((double)k1));
     rho *= 1.0 + 1e-5;
-    const int64_t x0 = box_cell(qp.x - rho, f.ox, f.inv, f.X), x1 = box_cell(qp.x + rho, f.ox, f.inv, f.X);
-    const int64_t y0 = box_cell(qp.y - rho, f.oy, f.inv, f.Y), y1 = box_cell(qp.y + rho, f.oy, f.inv, f.Y);
-    const int64_t z0 = box_cell(qp.z - rho, f.oz, f.inv, f.Z), z1 = box_cell(qp.z + rho, f.oz, f.inv, f.Z);
-    // visit(d2) for every candidate of the box but the point itself
-    auto scan = [&](auto&& visit) {
-      for (int64_t cz = z0; cz <= z1; ++cz)
-        for (int64_t cy = y0; cy <= y1; ++cy) {
-          const uint64_t khi = voxel_key(f, x1, cy, cz);
-          for (int32_t c = cell_lower_bound(t, voxel_key(f, x0, cy, cz));
-               c < t.ncells && ld_global(t.ukey, c) <= khi; ++c) {
-            const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
-            for (int32_t j = s0; j < s1; ++j) {
-              const float4 p = pts[j];
-              if (__float_as_int(p.w) == qi) continue;
-              const float pk[3] = {p.x, p.y, p.z};
-              visit(dist2(qp.x, qp.y, qp.z, pk));
-            }
-          }
-        }
-    };
-    // one-by-one insertion (here it measures faster than the queued merge
-    // networks of the first pass: each thread walks its own box)
+    // the lane's box of cells (coordinates below 2^30) and the wave's union
+    const int x0 = (int)box_cell(qp.x - rho, f.ox, f.inv, f.X);
+    const int x1 = (int)box_cell(qp.x + rho, f.ox, f.inv, f.X);
+    const int y0 = (int)box_cell(qp.y - rho, f.oy, f.inv, f.Y);
+    const int y1 = (int)box_cell(qp.y + rho, f.oy, f.inv, f.Y);
+    const int z0 = (int)box_cell(qp.z - rho, f.oz, f.inv, f.Z);
+    const int z1 = (int)box_cell(qp.z + rho, f.oz, f.inv, f.Z);
+    const int X0 = wave_min_i(x0), X1 = wave_max_i(x1), Y0 = wave_min_i(y0),
+              Y1 = wave_max_i(y1), Z0 = wave_min_i(z0), Z1 = wave_max_i(z1);
+    const float cut = fminf(k1, lim);
+    // the union when it is compact (its rows of cells at most twice the
+    // largest box's; uniform per wave), else each lane its own box
+    const int rows_own = wave_max_i((y1 - y0 + 1) * (z1 - z0 + 1));
+    const bool together = (int64_t)(Y1 - Y0 + 1) * (Z1 - Z0 + 1) <= 2 * (int64_t)rows_own &&
+                          X1 - X0 <= 2 * wave_max_i(x1 - x0) + 2;
+    const int bx0 = together ? X0 : x0, bx1 = together ? X1 : x1;
+    const int by0 = together ? Y0 : y0, by1 = together ? Y1 : y1;
+    const int bz0 = together ? Z0 : z0, bz1 = together ? Z1 : z1;
     Knn<K> sel;
     sel.init();
-    scan([&](float d) { sel.add(d, k); });
-    if (sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
+    for (int cz = bz0; cz <= bz1; ++cz)
+      for (int cy = by0; cy <= by1; ++cy) {
+        const uint64_t khi = voxel_key(f, bx1, cy, cz);
+        for (int32_t c = cell_lower_bound(t, voxel_key(f, bx0, cy, cz));
+             c < t.ncells && ld_global(t.ukey, c) <= khi; ++c) {
+          const int32_t s0 = ld_global(t.start, c), s1 = s0 + ld_global(t.count, c);
+          for (int32_t j = s0; j < s1; ++j) {
+            const float4 p = pts[j];
+            const float pk[3] = {p.x, p.y, p.z};
+            const float d = dist2(qp.x, qp.y, qp.z, pk);
+            if (act && __float_as_int(p.w) != qi && d <= cut) sel.add(d, k);
+          }
+        }
+      }
+    if (act && sel.kth(k) <= lim) st_global(md, qi, sel.mean_dist(k));
   }
 }
 
@@ -744,12 +774,15 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     PSGC_LAUNCHED();
   }
   if (g.ncells > 0) {
+    // the retry list: count, grid positions, the block's k-th distance per
+    // grid position, the flags of the first pass
     void* rl;
-    PSGC_TRY(ctx_buf(ctx, S_CL_RETRY, (size_t)g.np * 8 + 64, &rl));
+    PSGC_TRY(ctx_buf(ctx, S_CL_RETRY, (size_t)g.np * 9 + 64, &rl));
     int32_t* nretry = (int32_t*)rl;
     int32_t* retry = nretry + 16;
-    float* retry_kth = (float*)(retry + g.np);
-    PSGC_HIP(hipMemsetAsync(nretry, 0, 4, s));
+    float* kth_of = (float*)(retry + g.np);
+    uint8_t* again = (uint8_t*)(kth_of + g.np);
+    PSGC_HIP(hipMemsetAsync(again, 0, (size_t)g.np, s));
     const CellTab t{g.ukey, g.start, g.count, g.ncells};
     const int tbv = width_for(g);
     const int grid = grid_for_cells(ctx, g.ncells, tbv);
@@ -757,20 +790,27 @@ int cloud_sor_run(psgc_ctx* ctx, const float* xyz, const uint8_t* valid, int64_t
     PSGC_PROF_BEGIN("cloud_knn", s);
     with_width(tbv, [&](auto w) {
       constexpr int TB = decltype(w)::value;
-      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
-      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
-      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
-      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, retry, retry_kth, nretry);
+      if (k == 20) cloud_knn_kernel<20, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, again, kth_of);
+      else if (k == 30) cloud_knn_kernel<30, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, again, kth_of);
+      else if (k == 50) cloud_knn_kernel<50, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, again, kth_of);
+      else cloud_knn_kernel<0, TB><<<grid, TB, 0, s>>>(g.pts, t, g.f, k, lim, limmax, md, again, kth_of);
     });
     PSGC_LAUNCHED();
     PSGC_PROF_END("cloud_knn", s);
     if (rmax >= 2) {
+      // flagged grid positions in ascending order
+      size_t tb = 0;
+      const hipcub::CountingInputIterator<int32_t> ids(0);
+      PSGC_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, ids, again, retry, nretry, (int)g.np, s));
+      void* tmp;
+      PSGC_TRY(ctx_buf(ctx, S_CL_TEMP, tb, &tmp));
+      PSGC_HIP(hipcub::DeviceSelect::Flagged(tmp, tb, ids, again, retry, nretry, (int)g.np, s));
       const int rg = 4 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
       PSGC_PROF_BEGIN("cloud_knn_retry", s);
-      if (k == 20) cloud_knn_retry_kernel<20><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
-      else if (k == 30) cloud_knn_retry_kernel<30><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
-      else if (k == 50) cloud_knn_retry_kernel<50><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
-      else cloud_knn_retry_kernel<0><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, retry_kth, nretry, md);
+      if (k == 20) cloud_knn_retry_kernel<20><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, kth_of, nretry, md);
+      else if (k == 30) cloud_knn_retry_kernel<30><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, kth_of, nretry, md);
+      else if (k == 50) cloud_knn_retry_kernel<50><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, kth_of, nretry, md);
+      else cloud_knn_retry_kernel<0><<<rg, CB, 0, s>>>(g.pts, t, g.f, k, rmax, retry, kth_of, nretry, md);
       PSGC_LAUNCHED();
       PSGC_PROF_END("cloud_knn_retry", s);
     }
