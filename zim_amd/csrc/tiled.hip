@@ -153,11 +153,13 @@ __device__ inline void decode_stage_chunk(const DecodeArgs& a, int f0, int nf, i
   }
 }
 
-// Four waves per SIMD (at most 128 VGPRs) measure faster than three without
-// the bound, despite a few spilled scalars (4-row tiles).
+// 4-row tiles: four waves per SIMD (at most 128 VGPRs) measure faster than
+// three without the bound, despite a few spilled scalars. 2-row tiles: five
+// (at most 96 VGPRs, 36 bytes per lane spilled) measure 2-3 % faster than
+// four; six spill more and measure slower.
 template <int DRT>
-__global__ void __launch_bounds__(DC, 4) decode_tiled_kernel(const DecodeJob* __restrict__ jobs,
-                                                             int njobs, int64_t total) {
+__global__ void __launch_bounds__(DC, DRT <= 2 ? 5 : 4)
+    decode_tiled_kernel(const DecodeJob* __restrict__ jobs, int njobs, int64_t total) {
   constexpr int DLR = DRT + 2;
   __shared__ __align__(16) unsigned char lds[decode_lds_bytes<DRT>()];
   // the atan2 table (published by the first tile's staging barrier)
