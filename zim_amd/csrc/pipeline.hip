@@ -1118,13 +1118,14 @@ int tiled_decode(psgc_ctx* ctx, const DecodeArgs& a, double* phase_raw, int32_t*
   j.mod = mod;
   j.valid = valid;
   j.tiles_x = tiles(a.cols, kDecodeTileCols);
-  j.tiles_y = tiles(a.rows, 4);
+  // 2-row tiles: the batch autotuner's choice on full-size pairs
+  j.tiles_y = tiles(a.rows, 2);
   j.tile_base = 0;
   void* d;
   PSGC_TRY(ctx_buf(ctx, S_QR, sizeof(DecodeJob), &d));
   PSGC_HIP(hipMemcpyAsync(d, &j, sizeof(DecodeJob), hipMemcpyHostToDevice, s));
   return launch_decode_tiled((const DecodeJob*)d, 1, (int64_t)j.tiles_x * j.tiles_y,
-                             default_grid(ctx, 64), 4, s);
+                             default_grid(ctx, 64), 2, s);
 }
 
 int tiled_reconstruct(psgc_ctx* ctx, const psgc_recon_config& rc, const double* lp,
