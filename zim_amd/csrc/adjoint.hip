@@ -39,7 +39,8 @@ struct Nominal {
   double f0, cLx, cLy, cRx, cRy;
 };
 
-__device__ inline void rot(const Theta& t, double R[3][3], double dR[3][3][3]) {
+__device__ __attribute__((always_inline)) inline void rot(const Theta& t, double R[3][3],
+                                                         double dR[3][3][3]) {
   const double cx = cos(t.wx), sx = sin(t.wx), cy = cos(t.wy), sy = sin(t.wy), cz = cos(t.wz),
                sz = sin(t.wz);
   const double Rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
@@ -160,16 +161,25 @@ __device__ inline void bicubic_scatter(const Samp& s, int cols, double g, double
       atomicAdd(&gm[(size_t)(s.y0 - 1 + j) * cols + (s.x0 - 1 + i)], g * (wy[j] * wx[i]));
 }
 
-// Right remap of rectified (x, i): raw coordinates and their derivatives
-// with respect to theta (indices 0..8).
-__device__ inline void right_map(const Theta& t, const Nominal& n, const double R[3][3],
-                                 const double dR[3][3][3], double x, double i, double* u,
-                                 double* v, double du[9], double dv[9]) {
-  const double r[3] = {(x - t.cx2) / t.f, (i - t.cy2) / t.f, 1.0};
-  double q[3];
+// Right remap of rectified (x, i): raw coordinates (right_uv) and their
+// derivatives with respect to theta (indices 0..8; right_map, the same
+// coordinates and their derivatives).
+__device__ inline void right_uv(const Theta& t, const Nominal& n, const double (&R)[3][3],
+                                double x, double i, double r[3], double q[3], double* u,
+                                double* v) {
+  r[0] = (x - t.cx2) / t.f;
+  r[1] = (i - t.cy2) / t.f;
+  r[2] = 1.0;
   for (int a = 0; a < 3; ++a) q[a] = R[a][0] * r[0] + R[a][1] * r[1] + R[a][2] * r[2];
   *u = n.f0 * q[0] / q[2] + n.cRx;
   *v = n.f0 * q[1] / q[2] + n.cRy;
+}
+
+__device__ inline void right_map(const Theta& t, const Nominal& n, const double (&R)[3][3],
+                                 const double (&dR)[3][3][3], double x, double i, double* u,
+                                 double* v, double du[9], double dv[9]) {
+  double r[3], q[3];
+  right_uv(t, n, R, x, i, r, q, u, v);
   // d(u, v)/dq
   const double iz = 1.0 / q[2];
   const double duq[3] = {n.f0 * iz, 0.0, -n.f0 * q[0] * iz * iz};
@@ -193,35 +203,33 @@ __device__ inline void right_map(const Theta& t, const Nominal& n, const double 
   }
 }
 
+// A pixel's triangulated point and what its adjoint needs of it (the
+// derivatives of the right samples' coordinates are recomputed where they
+// are consumed, one sample at a time, so they are never all live together).
 struct PointEval {
   bool ok;
   bool smooth;  // every sample's taps satisfy Itoh's condition
   double xyz[3];
-  // for the adjoint
-  double dd_dc[3];        // d d / d c_m
+  // for the adjoint (GRAD)
+  double dd_dc[3];  // d d / d c_m
   double dxyz_dd[3];
-  double dxyz_dth[3][9];  // direct dependence through Q
+  double iD, iD2;   // 1 / D, 1 / D^2
   Samp bl, br[3];
-  double dul[9], dvl[9], dur[3][9], dvr[3][9];
 };
 
-__device__ inline void eval_point(const Theta& t, const Nominal& n, const double R[3][3],
-                                  const double dR[3][3][3], const double* phL, const double* phR,
-                                  int rows, int cols, int i, int j, int b, PointEval* e) {
+template <bool GRAD>
+__device__ inline void eval_point(const Theta& t, const Nominal& n, const double (&R)[3][3],
+                                  const double* phL, const double* phR, int rows, int cols, int i,
+                                  int j, int b, PointEval* e) {
   e->ok = false;
   if (b < 1 || b > cols - 2) return;
   const double ul = n.f0 / t.f * (j - t.cx1) + n.cLx;
   const double vl = n.f0 / t.f * (i - t.cy1) + n.cLy;
   if (!bicubic(phL, rows, cols, ul, vl, &e->bl)) return;
-  for (int k = 0; k < 9; ++k) e->dul[k] = e->dvl[k] = 0.0;
-  e->dul[0] = -n.f0 / (t.f * t.f) * (j - t.cx1);
-  e->dvl[0] = -n.f0 / (t.f * t.f) * (i - t.cy1);
-  e->dul[1] = -n.f0 / t.f;
-  e->dvl[2] = -n.f0 / t.f;
   double c[3];
   for (int m = 0; m < 3; ++m) {
-    double u, v;
-    right_map(t, n, R, dR, (double)(b - 1 + m), (double)i, &u, &v, e->dur[m], e->dvr[m]);
+    double r[3], q[3], u, v;
+    right_uv(t, n, R, (double)(b - 1 + m), (double)i, r, q, &u, &v);
     if (!bicubic(phR, rows, cols, u, v, &e->br[m])) return;
     c[m] = e->br[m].val - e->bl.val;
   }
@@ -229,9 +237,12 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   const double Dn = 2.0 * (c[0] - c[1]) * (c[1] - c[2]);
   if (!(fabs(Dn) > 1e-9)) return;
   const double delta = N / Dn;
-  const double dN[3] = {c[1], c[0] - c[2], -c[1]};
-  const double dD[3] = {2.0 * (c[1] - c[2]), 2.0 * (c[0] - 2.0 * c[1] + c[2]), -2.0 * (c[0] - c[1])};
-  for (int m = 0; m < 3; ++m) e->dd_dc[m] = -(dN[m] * Dn - N * dD[m]) / (Dn * Dn);
+  if (GRAD) {
+    const double dN[3] = {c[1], c[0] - c[2], -c[1]};
+    const double dD[3] = {2.0 * (c[1] - c[2]), 2.0 * (c[0] - 2.0 * c[1] + c[2]),
+                          -2.0 * (c[0] - c[1])};
+    for (int m = 0; m < 3; ++m) e->dd_dc[m] = -(dN[m] * Dn - N * dD[m]) / (Dn * Dn);
+  }
   const double d = (double)j - (double)b - delta;
   const double D = d - (t.cx1 - t.cx2);
   if (!(D > 0.0)) return;
@@ -239,39 +250,24 @@ __device__ inline void eval_point(const Theta& t, const Nominal& n, const double
   e->xyz[0] = X;
   e->xyz[1] = Y;
   e->xyz[2] = Z;
-  const double iD = 1.0 / D, iD2 = iD * iD;
-  e->dxyz_dd[0] = -(j - t.cx1) * t.B * iD2;
-  e->dxyz_dd[1] = -(i - t.cy1) * t.B * iD2;
-  e->dxyz_dd[2] = -t.f * t.B * iD2;
-  for (int a = 0; a < 3; ++a)
-    for (int k = 0; k < 9; ++k) e->dxyz_dth[a][k] = 0.0;
-  // f
-  e->dxyz_dth[2][0] = t.B * iD;
-  // cx1: D depends on -cx1
-  e->dxyz_dth[0][1] = -t.B * iD + (j - t.cx1) * t.B * iD2;
-  e->dxyz_dth[1][1] = (i - t.cy1) * t.B * iD2;
-  e->dxyz_dth[2][1] = t.f * t.B * iD2;
-  // cy1
-  e->dxyz_dth[1][2] = -t.B * iD;
-  // cx2
-  e->dxyz_dth[0][3] = -(j - t.cx1) * t.B * iD2;
-  e->dxyz_dth[1][3] = -(i - t.cy1) * t.B * iD2;
-  e->dxyz_dth[2][3] = -t.f * t.B * iD2;
-  // B
-  e->dxyz_dth[0][5] = (j - t.cx1) * iD;
-  e->dxyz_dth[1][5] = (i - t.cy1) * iD;
-  e->dxyz_dth[2][5] = t.f * iD;
+  if (GRAD) {
+    const double iD = 1.0 / D, iD2 = iD * iD;
+    e->iD = iD;
+    e->iD2 = iD2;
+    e->dxyz_dd[0] = -(j - t.cx1) * t.B * iD2;
+    e->dxyz_dd[1] = -(i - t.cy1) * t.B * iD2;
+    e->dxyz_dd[2] = -t.f * t.B * iD2;
+  }
   e->ok = true;
   e->smooth = e->bl.smooth && e->br[0].smooth && e->br[1].smooth && e->br[2].smooth;
 }
 
 __device__ inline void adj_forward_px(const Theta& t, const Nominal& n, const double (&R)[3][3],
-                                      const double (&dR)[3][3][3], const double* phL,
-                                      const double* phR, const int32_t* mc, int rows, int cols,
-                                      size_t p, double* xyz, uint8_t* valid) {
+                                      const double* phL, const double* phR, const int32_t* mc,
+                                      int rows, int cols, size_t p, double* xyz, uint8_t* valid) {
   const int i = (int)(p / cols), j = (int)(p % cols);
   PointEval e;
-  eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
+  eval_point<false>(t, n, R, phL, phR, rows, cols, i, j, mc[p], &e);
   xyz[p * 3 + 0] = e.ok ? e.xyz[0] : 0.0;
   xyz[p * 3 + 1] = e.ok ? e.xyz[1] : 0.0;
   xyz[p * 3 + 2] = e.ok ? e.xyz[2] : 0.0;
@@ -286,22 +282,49 @@ __device__ inline void adj_backward_px(const Theta& t, const Nominal& n,
                                        int rows, int cols, size_t p, const double* gxyz,
                                        double* gphL, double* gphR, double (&acc)[9]) {
   const int i = (int)(p / cols), j = (int)(p % cols);
+  const int b = mc[p];
   PointEval e;
-  eval_point(t, n, R, dR, phL, phR, rows, cols, i, j, mc[p], &e);
+  eval_point<true>(t, n, R, phL, phR, rows, cols, i, j, b, &e);
   if (!e.ok) return;
   const double g[3] = {gxyz[p * 3], gxyz[p * 3 + 1], gxyz[p * 3 + 2]};
   const double gd = g[0] * e.dxyz_dd[0] + g[1] * e.dxyz_dd[1] + g[2] * e.dxyz_dd[2];
-  for (int k = 0; k < 9; ++k)
-    acc[k] = g[0] * e.dxyz_dth[0][k] + g[1] * e.dxyz_dth[1][k] + g[2] * e.dxyz_dth[2][k];
+  {
+    // direct dependence of the point on theta through Q
+    const double iD = e.iD, iD2 = e.iD2;
+    double dth[3][9];
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 9; ++k) dth[a][k] = 0.0;
+    dth[2][0] = t.B * iD;                                // f
+    dth[0][1] = -t.B * iD + (j - t.cx1) * t.B * iD2;     // cx1: D depends on -cx1
+    dth[1][1] = (i - t.cy1) * t.B * iD2;
+    dth[2][1] = t.f * t.B * iD2;
+    dth[1][2] = -t.B * iD;                               // cy1
+    dth[0][3] = -(j - t.cx1) * t.B * iD2;                // cx2
+    dth[1][3] = -(i - t.cy1) * t.B * iD2;
+    dth[2][3] = -t.f * t.B * iD2;
+    dth[0][5] = (j - t.cx1) * iD;                        // B
+    dth[1][5] = (i - t.cy1) * iD;
+    dth[2][5] = t.f * iD;
+    for (int k = 0; k < 9; ++k) acc[k] = g[0] * dth[0][k] + g[1] * dth[1][k] + g[2] * dth[2][k];
+  }
   double gL = 0.0;
   for (int m = 0; m < 3; ++m) {
     const double gc = gd * e.dd_dc[m];  // dL/dc_m
     gL -= gc;
     // through the right sample
-    for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * e.dur[m][k] + e.br[m].dv * e.dvr[m][k]);
+    double u, v, dur[9], dvr[9];
+    right_map(t, n, R, dR, (double)(b - 1 + m), (double)i, &u, &v, dur, dvr);
+    for (int k = 0; k < 9; ++k) acc[k] += gc * (e.br[m].du * dur[k] + e.br[m].dv * dvr[k]);
     if (gphR) bicubic_scatter(e.br[m], cols, gc, gphR);
   }
-  for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * e.dul[k] + e.bl.dv * e.dvl[k]);
+  // the left sample's coordinates depend on f, cx1, cy1
+  double dul[9], dvl[9];
+  for (int k = 0; k < 9; ++k) dul[k] = dvl[k] = 0.0;
+  dul[0] = -n.f0 / (t.f * t.f) * (j - t.cx1);
+  dvl[0] = -n.f0 / (t.f * t.f) * (i - t.cy1);
+  dul[1] = -n.f0 / t.f;
+  dvl[2] = -n.f0 / t.f;
+  for (int k = 0; k < 9; ++k) acc[k] += gL * (e.bl.du * dul[k] + e.bl.dv * dvl[k]);
   if (gphL) bicubic_scatter(e.bl, cols, gL, gphL);
 }
 
@@ -318,28 +341,41 @@ __device__ inline void reduce_theta(double (&acc)[9], double* gth) {
   if (threadIdx.x < 9) atomicAdd(&gth[threadIdx.x], red[threadIdx.x][0]);
 }
 
-__global__ void adj_forward_kernel(Theta t, Nominal n, const double* __restrict__ phL,
-                                   const double* __restrict__ phR,
-                                   const int32_t* __restrict__ mc, int rows, int cols,
-                                   double* __restrict__ xyz, uint8_t* __restrict__ valid) {
+// The rotation of theta's angles and its derivatives, computed once per
+// workgroup (one thread) into LDS: uniform for every pixel of a view.
+struct Rot {
+  double R[3][3];
+  double dR[3][3][3];
+};
+
+__device__ __attribute__((always_inline)) inline void rot_shared(const Theta& t, Rot* r) {
+  if (threadIdx.x == 0) rot(t, r->R, r->dR);
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) adj_forward_kernel(Theta t, Nominal n,
+                                                          const double* __restrict__ phL,
+                                                          const double* __restrict__ phR,
+                                                          const int32_t* __restrict__ mc, int rows,
+                                                          int cols, double* __restrict__ xyz,
+                                                          uint8_t* __restrict__ valid) {
+  __shared__ Rot rs;
+  rot_shared(t, &rs);
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (size_t)rows * cols) return;
-  double R[3][3], dR[3][3][3];
-  rot(t, R, dR);
-  adj_forward_px(t, n, R, dR, phL, phR, mc, rows, cols, p, xyz, valid);
+  adj_forward_px(t, n, rs.R, phL, phR, mc, rows, cols, p, xyz, valid);
 }
 
 __global__ void __launch_bounds__(256) adj_backward_kernel(
     Theta t, Nominal n, const double* __restrict__ phL, const double* __restrict__ phR,
     const int32_t* __restrict__ mc, int rows, int cols, const double* __restrict__ gxyz,
     double* __restrict__ gth, double* __restrict__ gphL, double* __restrict__ gphR) {
+  __shared__ Rot rs;
+  rot_shared(t, &rs);
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (p < (size_t)rows * cols) {
-    double R[3][3], dR[3][3][3];
-    rot(t, R, dR);
-    adj_backward_px(t, n, R, dR, phL, phR, mc, rows, cols, p, gxyz, gphL, gphR, acc);
-  }
+  if (p < (size_t)rows * cols)
+    adj_backward_px(t, n, rs.R, rs.dR, phL, phR, mc, rows, cols, p, gxyz, gphL, gphR, acc);
   reduce_theta(acc, gth);
 }
 
@@ -375,25 +411,24 @@ __device__ inline int view_of_block(const AdjView* v, int nv, int64_t b) {
 
 __global__ void __launch_bounds__(256) adj_forward_batch_kernel(const AdjView* __restrict__ views,
                                                                 int nv) {
+  __shared__ Rot rs;
   const AdjView& V = views[view_of_block(views, nv, blockIdx.x)];
+  rot_shared(V.t, &rs);
   const size_t p = (size_t)(blockIdx.x - V.block_base) * 256 + threadIdx.x;
   if (p >= (size_t)V.rows * V.cols) return;
-  double R[3][3], dR[3][3][3];
-  rot(V.t, R, dR);
-  adj_forward_px(V.t, V.n, R, dR, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.xyz, V.valid);
+  adj_forward_px(V.t, V.n, rs.R, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.xyz, V.valid);
 }
 
 __global__ void __launch_bounds__(256) adj_backward_batch_kernel(
     const AdjView* __restrict__ views, int nv) {
+  __shared__ Rot rs;
   const AdjView& V = views[view_of_block(views, nv, blockIdx.x)];
+  rot_shared(V.t, &rs);
   const size_t p = (size_t)(blockIdx.x - V.block_base) * 256 + threadIdx.x;
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (p < (size_t)V.rows * V.cols) {
-    double R[3][3], dR[3][3][3];
-    rot(V.t, R, dR);
-    adj_backward_px(V.t, V.n, R, dR, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.gxyz, V.gphL,
+  if (p < (size_t)V.rows * V.cols)
+    adj_backward_px(V.t, V.n, rs.R, rs.dR, V.phL, V.phR, V.mc, V.rows, V.cols, p, V.gxyz, V.gphL,
                     V.gphR, acc);
-  }
   reduce_theta(acc, V.gth);
 }
 
