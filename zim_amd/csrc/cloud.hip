@@ -232,7 +232,7 @@ struct Knn {
     double s = 0.0;
 #pragma unroll
     for (int t = 0; t < N; ++t)
-      if (t < k) s += sqrt((double)best[t]);
+      if (t < k) s += sqrt_of_float(best[t]);
     return s / k;
   }
 };
@@ -287,7 +287,7 @@ struct KnnNet {
     double s = 0.0;
 #pragma unroll
     for (int t = 0; t < NB; ++t)
-      if (t < k) s += sqrt((double)best[t]);
+      if (t < k) s += sqrt_of_float(best[t]);
     return s / k;
   }
 };

@@ -160,6 +160,27 @@ __device__ inline bool gray_order(const DecodeArgs& a, int r, int x, double wrap
   return true;
 }
 
+// sqrt((double)f) for a finite f >= 0: the instruction sequence the
+// compiler emits for a double square root (reciprocal square-root estimate,
+// then fused multiply-add corrections of the root and the half reciprocal),
+// without its scaling of arguments below 2^-767 and its fix-up of zero and
+// infinity, which no non-zero float needs; zero is returned as is. The same
+// bits as sqrt for every such argument, in about two thirds of the
+// instructions (the SOR sums of k square roots per point).
+__device__ inline double sqrt_of_float(float f) {
+  const double x = (double)f;
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  return f == 0.0f ? x : g;
+}
+
 // k-th smallest of v[0..n) (n small), by partial selection.
 __device__ inline double select_kth(double* v, int n, int kth) {
   for (int i = 0; i <= kth; ++i) {

@@ -662,7 +662,7 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
           if (best[KFIX - 1] < __builtin_huge_valf()) {
             double s = 0.0;
 #pragma unroll
-            for (int m = 0; m < KFIX; ++m) s += sqrt((double)best[m]);
+            for (int m = 0; m < KFIX; ++m) s += sqrt_of_float(best[m]);
             out = s / KFIX;
           }
         } else {
@@ -679,7 +679,7 @@ __global__ void __launch_bounds__(256) sor_knn_tiled_kernel(const CloudJob* __re
             double s = 0.0;
 #pragma unroll
             for (int m = 0; m < KMAX; ++m)
-              if (m < k) s += sqrt((double)best[m]);
+              if (m < k) s += sqrt_of_float(best[m]);
             out = s / k;
           }
         }
