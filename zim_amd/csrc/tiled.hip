@@ -432,15 +432,32 @@ __global__ void __launch_bounds__(256) repro_gauss_tiled_kernel(const CloudJob* 
     const int x0 = (int)(lt % cj.tiles_x) * CT, y0 = (int)(lt / cj.tiles_x) * CT;
     const int h = (cj.gwin - 1) / 2;
     const int L = CT + 2 * h;
-    for (int idx = threadIdx.x; idx < L * L; idx += 256) {
+    // the tile's disparities, both of a thread's loads issued before either
+    // is used ((CT + 2 GH_MAX)^2 <= 2 x 256 entries)
+    static_assert(GL * GL <= 2 * 256, "two entries per thread");
+    double dds[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      const int rr = idx / L, cc = idx % L;
+      const int gy = y0 - h + rr, gx = x0 - h + cc;
+      dds[u] = 0.0;
+      if (idx < L * L && gy >= 0 && gy < cj.rows && gx >= 0 && gx < cj.cols) {
+        const size_t p = (size_t)gy * cj.cols + gx;
+        dds[u] = cj.disp_src64 ? ld_global(cj.disp_src64, p) : (double)ld_global(cj.disp_src, p);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int idx = threadIdx.x + u * 256;
+      if (idx >= L * L) break;
       const int rr = idx / L, cc = idx % L;
       const int gy = y0 - h + rr, gx = x0 - h + cc;
       float o[3] = {0.f, 0.f, 0.f};
       uint8_t v = 0;
       if (gy >= 0 && gy < cj.rows && gx >= 0 && gx < cj.cols) {
         const size_t p = (size_t)gy * cj.cols + gx;
-        const double dd =
-            cj.disp_src64 ? ld_global(cj.disp_src64, p) : (double)ld_global(cj.disp_src, p);
+        const double dd = dds[u];
         v = cj.coef[2] ? reproject_poly(ld_global(cj.coef[0], p), ld_global(cj.coef[1], p),
                                         ld_global(cj.coef[2], p), cj.r1, dd,
                                         cj.min_depth, cj.max_depth, cj.use_rect, o)
