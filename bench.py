@@ -92,8 +92,8 @@ MLS_RADIUS = 1.5  # mm
 # time they give the wave-instruction rate, set against the rate at which
 # FP32 FMAs reach MI355X's published vector peak: 157.3 TFLOP/s / (2 flops x
 # 64 lanes) = 1229 G wave-instructions/s.
-VALU_PER_PX_PROFILED = {"mls/0": 11400, "mls/1": 11152, "decode": 1874, "sor_knn": 1764,
-                        "match": 573, "repro_gauss": 497, "median5": 271}
+VALU_PER_PX_PROFILED = {"mls/0": 11400, "mls/1": 11152, "decode": 1889, "sor_knn": 1649,
+                        "match": 573, "repro_gauss": 504, "median5": 271}
 VALU_PEAK_GINST_S = 157.3e3 / 2 / 64
 # The published FP32 vector peak counts packed FP32 FMAs (two per lane per
 # instruction); packed FP32 is disabled in this build, so single-lane FMAs
