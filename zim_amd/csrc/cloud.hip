@@ -9,7 +9,7 @@
 // found by run-length encoding. A workgroup takes one occupied cell: the
 // block of cells around it (3x3x3; 5x5x5 for radius filters on half-radius
 // cells) is looked up once (binary search over the sorted keys), its points
-// streamed through LDS in chunks (512 or 1024), and every thread scans each
+// streamed through LDS in chunks (one point per thread), and every thread scans each
 // chunk for one point of the centre cell. Candidates come in a fixed order
 // (cells dz, dy, dx = -R..R, points in grid order), the order of the CPU
 // restatement.
@@ -238,11 +238,12 @@ struct Knn {
 };
 
 // The NB smallest of a candidate stream (NB a power of two, 32 or 64),
-// ascending, taken in groups of 16 with fixed compare-exchange networks and
-// no data-dependent branches: a group is sorted (bitonic sort), its reverse
-// paired with the list's top 16 (min of each pair: with the list ascending
-// and the group descending the result is a bitonic sequence holding the NB
-// smallest of both), and the list re-sorted by a bitonic merge. The first K
+// ascending, taken in groups of G (16 or 32) with fixed compare-exchange
+// networks and no data-dependent branches: a group is sorted (bitonic sort),
+// its reverse paired with the list's top G (min of each pair: with the list
+// ascending and the group descending the result is a bitonic sequence
+// holding the NB smallest of both), and the list re-sorted by a bitonic
+// merge. The first K
 // <= NB entries are then exactly the K smallest seen, as a one-by-one
 // insertion keeps them (the same multiset, so the same ascending list), at a
 // cost that does not grow with how far each value travels. NaN candidates
@@ -260,13 +261,15 @@ struct KnnNet {
     a = lo;
     b = hi;
   }
-  __device__ void merge16(float (&g)[16]) {
+  template <int G>
+  __device__ __attribute__((always_inline)) void merge(float (&g)[G]) {
+    static_assert(G == 16 || G == 32, "groups of 16 or 32");
 #pragma unroll
-    for (int size = 2; size <= 16; size <<= 1)
+    for (int size = 2; size <= G; size <<= 1)
 #pragma unroll
       for (int stride = size >> 1; stride > 0; stride >>= 1)
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
+        for (int i = 0; i < G; ++i) {
           const int j = i ^ stride;
           if (j > i) {
             if ((i & size) == 0) cx(g[i], g[j]);
@@ -274,7 +277,7 @@ struct KnnNet {
           }
         }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) best[NB - 16 + i] = fminf(best[NB - 16 + i], g[15 - i]);
+    for (int i = 0; i < G; ++i) best[NB - G + i] = fminf(best[NB - G + i], g[G - 1 - i]);
 #pragma unroll
     for (int stride = NB >> 1; stride > 0; stride >>= 1)
 #pragma unroll
@@ -296,8 +299,12 @@ struct KnnNet {
 // candidates, or a k-th distance beyond the point's bound: one voxel plus
 // its distance to the nearest face of its cell) are flagged in `again` at
 // their grid position, with the block's k-th distance in `kth_of`.
-// TB threads per workgroup (one cell at a time), chunks of chunk_of(TB).
-constexpr int chunk_of(int tb) { return tb >= 256 ? 1024 : 512; }
+// TB threads per workgroup (one cell at a time), chunks of chunk_of(TB)
+// candidates (one per thread: the smaller LDS tile keeps more workgroups
+// resident; 2 or 4 per thread measured slower), candidate queues of QL per
+// lane (32: half the merges of 16, whose networks dominate the pass).
+constexpr int chunk_of(int tb) { return tb; }
+constexpr int QL = 32;
 
 template <int K, int TB>
 __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
@@ -308,7 +315,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
   constexpr int CH = chunk_of(TB);
   __shared__ float4 tile[CH];
   __shared__ BlockCells<1> bc;
-  __shared__ float queue[K > 0 ? 16 : 1][TB];  // per-lane candidate queues
+  __shared__ float queue[K > 0 ? QL : 1][TB];  // per-lane candidate queues
   const XcdRange xr = xcd_range(t.ncells);
   for (int64_t c = xr.first; c < xr.end; c += xr.stride) {
     // nearest cells first: the k-th distance falls early and later
@@ -328,17 +335,17 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
       if constexpr (K > 0) {
         // Each lane queues the candidates below its current K-th distance
         // (which only falls, so a candidate at or above it can never enter)
-        // in 16 LDS slots of its own; the queues are merged into the sorted
+        // in QL LDS slots of its own; the queues are merged into the sorted
         // lists when one fills (uniform per wave), so the merge networks run
-        // about once per 16 accepted candidates instead of once per 16 read.
+        // about once per QL accepted candidates instead of once per QL read.
         KnnNet<(K <= 32 ? 32 : 64)> sel;
         sel.init();
         int qn = 0;
-        auto flush = [&]() {
-          float g[16];
+        auto flush = [&]() __attribute__((always_inline)) {
+          float g[QL];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) g[u] = u < qn ? queue[u][threadIdx.x] : INFINITY;
-          sel.merge16(g);
+          for (int u = 0; u < QL; ++u) g[u] = u < qn ? queue[u][threadIdx.x] : INFINITY;
+          sel.merge(g);
           qn = 0;
         };
         for (int32_t cb = 0; cb < M; cb += CH) {
@@ -349,7 +356,7 @@ __global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict_
           // four candidates per step (their reads issue together), after
           // making room for four in every queue
           for (int32_t j0 = 0; j0 < len; j0 += 4) {
-            if (__any(qn > 12)) flush();
+            if (__any(qn > QL - 4)) flush();
             float4 p[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) p[u] = tile[min(j0 + u, len - 1)];
