@@ -53,6 +53,26 @@ def test_cloud_sor_random(ctx, k, voxel, rmax):
     assert ref[0].sum() > 100
 
 
+def test_cloud_sor_retry_waves_near_and_far(ctx):
+    """The second k-nearest pass takes the unsettled points 64 at a time in
+    grid order and walks the union of their search boxes when those lie close
+    together, each point's own box otherwise. A dense patch (its unsettled
+    points neighbours in the grid: compact unions) beside sparse points
+    strung along y and z (consecutive in grid order but rows of cells apart:
+    boxes walked one by one) give both; the k nearest, the eligibility and
+    the statistics equal the oracle's bit for bit."""
+    g = np.random.default_rng(21)
+    patch = np.column_stack([g.uniform(0, 2.0, 3000), g.uniform(0, 2.0, 3000),
+                             500 + g.normal(0, 0.05, 3000)])
+    t = np.sort(g.uniform(0, 40.0, 600))
+    strand = np.column_stack([5.0 + g.normal(0, 0.3, 600), t, 500 + 0.5 * t + g.normal(0, 0.3, 600)])
+    xyz = np.concatenate([patch, strand]).astype(np.float32)
+    for k, voxel, rmax in ((20, 0.12, 4), (30, 0.15, 6), (50, 0.2, 8)):
+        ref, got = _sor_both(ctx, xyz, None, k, voxel, rmax)
+        _assert_sor(ref, got)
+        assert 0 < ref[0].sum() < len(xyz)
+
+
 def test_cloud_sor_host_mode_and_edge_cases(ctx):
     xyz, valid = _rng_cloud(3000, 12)
     xyz[5] = np.nan                      # not finite: removed
