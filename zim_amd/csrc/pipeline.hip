@@ -241,9 +241,15 @@ size_t table_bytes(const Plan* pl) {
   return b;
 }
 
+// Row tile before any tuning: rows wider than 2048 pixels fill most of a
+// workgroup's LDS (12 bytes per pixel), so few workgroups fit a CU and
+// wider ones keep more waves resident.
+int default_match_threads(int cols) { return cols > 2048 ? 512 : 256; }
+
 // Splits the row-match jobs by image width (each group's rows renumbered
 // from 0); a batch of one width keeps the single table only.
 void group_matches(Plan* pl) {
+  pl->match_threads = default_match_threads(pl->max_cols);
   pl->match_groups.clear();
   std::map<int, size_t> at;
   for (const MatchJob& j : pl->mat.host)
@@ -251,6 +257,7 @@ void group_matches(Plan* pl) {
       at[j.cols] = pl->match_groups.size();
       MatchGroup mg;
       mg.cols = j.cols;
+      mg.threads = default_match_threads(j.cols);
       pl->match_groups.push_back(mg);
     }
   if (pl->match_groups.size() < 2) {
