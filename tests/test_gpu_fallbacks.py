@@ -82,11 +82,13 @@ def test_batch_mixes_tiled_and_fallback_pairs(ctx, graph):
     b.close()
 
 
-def test_rows_wider_than_the_match_lds_row(kctx, ctx):
-    """6000-px rows (12 B per pixel beyond the 64 KB row budget) through the
-    single-pair path and the batch (whose decode leaves the post-smooth to
-    the match)."""
-    p, left, right = scenes.c2_small(61, rows=8, cols=6000)
+@pytest.mark.parametrize("cols", [6000, 7000])
+def test_rows_wider_than_the_match_lds_row(kctx, ctx, cols):
+    """Wide rows through the single-pair path and the batch (whose decode
+    leaves the post-smooth to the match): 6000 px fit the 64 KB row budget
+    of the tiled match (10 B per pixel), 7000 px run the one-thread-per-pixel
+    match."""
+    p, left, right = scenes.c2_small(61, rows=8, cols=cols)
     dcfg = scenes.c2_decode_config()
     Lr, Rr = oracle.decode(dcfg, left), oracle.decode(dcfg, right)
     rc = scenes.c2_recon_config(mls_window_size=4)

@@ -7,7 +7,7 @@
 //              (6 x 258 samples) staged once through LDS (double-buffered),
 //              the horizontal blur pass done once per sample;
 //   match      one workgroup per row: the right phase row (post-smoothed on
-//              the fly from the raw map) and its orders in LDS, per-row order
+//              the fly from the raw map) and its 16-bit orders in LDS, per-row order
 //              spans built with LDS atomics, the argmin scan reads LDS only;
 //   median     20x20 tile in LDS, branch-free 202-op selection network;
 //   reproject+Gaussian fused on a 16x16 tile with its halo in LDS;
@@ -316,9 +316,11 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
     const int r = (int)(t - m.tile_base);
     const int cols = m.cols;
     const int knum = m.pha_f > 0 ? m.pha_f : 0;
+    // the right row's orders as 16 bits (orders run to pha_f <= 32767):
+    // 10 bytes per pixel, so three workgroups of a 5120-px row share a CU
     double* sR = lds;
-    int32_t* kR = (int32_t*)(sR + cols);
-    int32_t* lo = kR + cols;
+    int16_t* kR = (int16_t*)(sR + cols);
+    int32_t* lo = (int32_t*)(kR + ((cols + 1) & ~1));
     int32_t* hi = lo + (knum + 1);
     const size_t row = (size_t)r * cols;
     for (int k = threadIdx.x; k <= knum; k += MT) {
@@ -327,7 +329,9 @@ __global__ void __launch_bounds__(MT) match_tiled_kernel(const MatchJob* __restr
     }
     for (int c = threadIdx.x; c < cols; c += MT) {
       sR[c] = m.smooth ? smooth_at(m.rp, m.rows, cols, r, c) : ld_global(m.rp, row + c);
-      kR[c] = ld_global(m.rk, row + c);
+      // an order outside 16 bits is never matched (above pha_f): stored as -1
+      const int32_t kc = ld_global(m.rk, row + c);
+      kR[c] = (int16_t)(kc >= -1 && kc <= 32767 ? kc : -1);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < cols; c += MT) {
@@ -907,7 +911,10 @@ int round_grid(int64_t total, int grid) {
 }  // namespace
 
 size_t match_tiled_lds(int cols, int knum) {
-  return (size_t)cols * (sizeof(double) + sizeof(int32_t)) + (size_t)(knum + 1) * 2 * sizeof(int32_t);
+  // orders beyond 16 bits: no tiled row (the plan runs the plain match)
+  if (knum > 32767) return (size_t)1 << 40;
+  return (size_t)cols * sizeof(double) + (size_t)((cols + 1) & ~1) * sizeof(int16_t) +
+         (size_t)(knum + 1) * 2 * sizeof(int32_t);
 }
 
 int match_tiled_max_lds() { return 64 * 1024; }
