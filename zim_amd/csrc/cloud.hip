@@ -307,7 +307,7 @@ constexpr int chunk_of(int tb) { return tb; }
 constexpr int QL = 32;
 
 template <int K, int TB>
-__global__ void __launch_bounds__(TB) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
+__global__ void __launch_bounds__(TB, (K >= 50 ? 4 : 1)) cloud_knn_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, int k, float lim,
                                                        float limmax, double* __restrict__ md,
                                                        uint8_t* __restrict__ again,
@@ -536,7 +536,7 @@ __global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
 
 // C: the per-neighbour multiply-adds contracted (GPU semantics).
 template <int NC, int TB, int R, bool C>
-__global__ void __launch_bounds__(TB) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
+__global__ void __launch_bounds__(TB, 5) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, float r2, float inv2s2,
                                                        int order, int semantics,
                                                        float* __restrict__ out) {
