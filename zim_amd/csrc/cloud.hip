@@ -536,7 +536,7 @@ __global__ void cloud_sor_apply_kernel(double* __restrict__ md, int64_t n,
 
 // C: the per-neighbour multiply-adds contracted (GPU semantics).
 template <int NC, int TB, int R, bool C>
-__global__ void __launch_bounds__(TB, 6) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
+__global__ void __launch_bounds__(TB, 7) cloud_mls_kernel(const float4* __restrict__ pts, CellTab t,
                                                        VoxelFrame f, float r2, float inv2s2,
                                                        int order, int semantics,
                                                        float* __restrict__ out) {
